@@ -1,0 +1,179 @@
+"""Benchmark of the BUG hot path on one MI355X.
+
+Prints ONE JSON line (rank 0).  Workload: the XXZ chain (delta = 0.5) at L = 128 on a seeded
+synthetic U(1) state with bond dimension up to D = 1024 (Gaussian charge-sector weights, sigma 4).
+
+``--workload heff_sweep``: one step = a left-to-right pass over all L sites doing the one-site
+effective-Hamiltonian apply and the left environment push at every site (the contraction core
+of a half-sweep), on device-resident tensors.
+
+The dominant kernel (grouped complex block GEMM of the one-site apply) is timed separately with
+HIP events on its own stream for the roofline; the CPU baseline times the oracle's one-site apply
+on the same environments (downloaded) with numpy/OpenBLAS on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "BUG steps/s L=128 D=1024 mixed vs FP64 @1e-6 err; p50/p99 request latency"
+PEAK = {"fp64": 78.6, "fp32": 157.3}   # dense MFMA TFLOP/s, MI355X datasheet
+HBM_PEAK = 8000.0                      # GB/s
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--L", type=int, default=128)
+    p.add_argument("--D", type=int, default=1024)
+    p.add_argument("--delta", type=float, default=0.5)
+    p.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    p.add_argument("--workload", choices=["heff_sweep"], default="heff_sweep")
+    p.add_argument("--kernel-reps", type=int, default=20)
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--kernel-only", action="store_true",
+                   help="run only the dominant-kernel timing (for profiling)")
+    return p.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}), flush=True)
+        sys.exit(2)
+    import torch
+
+    from bondupdategalerkin_amd import _lib
+    from bondupdategalerkin_amd import environments as ev
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+    from bondupdategalerkin_amd.mps import random_mps
+
+    torch.cuda.set_device(0)
+    prec = _lib.PREC_FP64 if args.precision == "fp64" else _lib.PREC_FP32
+    L, D = args.L, args.D
+    t0 = time.time()
+    psi = random_mps(L, D, seed=0)
+    mpo = xxz_mpo(L, delta=args.delta)
+    torch.cuda.synchronize()
+    print(f"# state built in {time.time() - t0:.1f}s; bond dims max {max(psi.bond_dims())}",
+          file=sys.stderr, flush=True)
+    mid = L // 2
+    # right environments once (inputs of every apply)
+    renvs = ev.right_env_stack(psi, mpo, downto=mid + 1, precision=prec)
+    lenvs = ev.left_env_stack(psi, mpo, upto=mid, precision=prec)
+    torch.cuda.synchronize()
+
+    ctx = _lib.context()
+
+    # ---- dominant kernel: one-site H_eff apply at the centre site ----
+    Lm, Rm, X = lenvs[mid], renvs[mid + 1], psi[mid]
+    ev.apply_one_site(Lm, mpo[mid], Rm, X, precision=prec)
+    flops_apply = ctx.last_flops()
+    stream = torch.cuda.current_stream()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.kernel_reps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.kernel_reps)]
+    for r in range(args.kernel_reps):
+        starts[r].record(stream)
+        ev.apply_one_site(Lm, mpo[mid], Rm, X, precision=prec)
+        ends[r].record(stream)
+    torch.cuda.synchronize()
+    apply_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    achieved = flops_apply / (apply_ms * 1e-3) / 1e12
+    peak = PEAK[args.precision]
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                "kernel": "grouped_zgemm (one-site H_eff apply, 2 launches)",
+                "flops_per_apply": flops_apply, "apply_ms": round(apply_ms, 4)}
+    if args.kernel_only:
+        print(json.dumps({"kernel_only": True, "roofline": roofline}), flush=True)
+        return
+
+    # ---- the timed step ----
+    def step():
+        lenv = lenvs[0]
+        for i in range(L):
+            R = renvs[i + 1] if i + 1 < len(renvs) and renvs[i + 1] is not None else None
+            if R is not None:
+                ev.apply_one_site(lenv, mpo[i], R, psi[i], precision=prec)
+            lenv = ev.push_left(lenv, psi[i], mpo[i], precision=prec)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t1
+    ms_per_step = 1e3 * elapsed / args.steps
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(Lm, mpo[mid], Rm, X, args.cpu_seconds, apply_ms)
+
+    out = {
+        "metric": METRIC,
+        "value": round(1e3 / ms_per_step, 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128" if args.precision == "fp64" else "c64-mfma/c128-storage",
+        "data": "synthetic",
+        "config": {"workload": "heff_sweep: one-site H_eff apply + left env push at every site "
+                               "(full BUG step not measured by this line)",
+                   "L": L, "D": D, "model": f"xxz delta={args.delta}", "symmetry": "U1",
+                   "precision": args.precision, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(Lm, W, Rm, X, seconds, gpu_apply_ms):
+    """Oracle one-site apply on the host cores on the same tensors (bounded sample)."""
+    from oracle import blocks as ob
+
+    def fam(t):
+        lay = t.layout
+        return ob.Fam(lay.row.as_dict(), lay.col.as_dict(), lay.shifts, t.to_blocks())
+
+    Lf, Rf, Xf = fam(Lm), fam(Rm), fam(X)
+    Wf = ob.MpoSite(W.cl, W.cr, zip(W.a, W.b, W.sout, W.sin, W.coeff))
+    n, t0 = 0, time.perf_counter()
+    while True:
+        ob.apply_one_site(Lf, Wf, Rf, Xf)
+        n += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    per = (time.perf_counter() - t0) / n
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
+    except Exception:
+        threads = os.cpu_count()
+    return {"value": round(1.0 / per, 4), "unit": "one-site applies/s", "cores": threads,
+            "kind": "port", "sample": f"{n} oracle one-site H_eff applies at the centre site "
+                                     f"(L=128, D=1024), {per:.3f}s each vs GPU "
+                                     f"{gpu_apply_ms:.3f} ms"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,208 @@
+"""ctypes binding of ``libbug.so``, the gfx950 HIP library behind the BUG hot path.
+
+Every numerical entry point of this package goes through here.  The library is loaded from the
+package directory (built in-tree by ``bondupdategalerkin_amd.build``); if it is missing or fails
+to load, :func:`lib` raises -- there is no CPU fallback anywhere in the package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbug.so")
+
+BUG_OK = 0
+BUG_EINVAL = 1
+BUG_ENONFINITE = 2
+BUG_EINVARIANT = 3
+BUG_EDEVICE = 4
+BUG_ENOMEM = 5
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+
+OP_N, OP_T, OP_C, OP_R = 0, 1, 2, 3
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class BugLeg(ctypes.Structure):
+    _fields_ = [("nsec", ctypes.c_int32), ("charge", _i32p), ("dim", _i32p)]
+
+
+class BugBsm(ctypes.Structure):
+    _fields_ = [
+        ("row", BugLeg),
+        ("col", BugLeg),
+        ("nslice", ctypes.c_int32),
+        ("shift", _i32p),
+        ("offset", _i64p),
+        ("data", ctypes.c_void_p),
+        ("nelem", ctypes.c_int64),
+    ]
+
+
+class BugMpoSite(ctypes.Structure):
+    _fields_ = [
+        ("wl", ctypes.c_int32),
+        ("wr", ctypes.c_int32),
+        ("cl", _i32p),
+        ("cr", _i32p),
+        ("nent", ctypes.c_int32),
+        ("a", _i32p),
+        ("b", _i32p),
+        ("sout", _i32p),
+        ("sin", _i32p),
+        ("coeff", _f64p),
+    ]
+
+
+class BugTerm(ctypes.Structure):
+    _fields_ = [
+        ("o", ctypes.c_int32),
+        ("p", ctypes.c_int32),
+        ("q", ctypes.c_int32),
+        ("op_a", ctypes.c_int32),
+        ("op_b", ctypes.c_int32),
+        ("alpha_re", ctypes.c_double),
+        ("alpha_im", ctypes.c_double),
+    ]
+
+
+# name -> (restype, argtypes); the list is the ABI of include/bug.h
+_P = ctypes.c_void_p
+_BSMP = ctypes.POINTER(BugBsm)
+_MPOP = ctypes.POINTER(BugMpoSite)
+SIGNATURES = {
+    "bug_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
+    "bug_ctx_destroy": (ctypes.c_int, [_P]),
+    "bug_last_error": (ctypes.c_char_p, [_P]),
+    "bug_set_tile_config": (ctypes.c_int, [_P, ctypes.c_int]),
+    "bug_tile_config_count": (ctypes.c_int, []),
+    "bug_ctx_stats": (ctypes.c_int, [_P, _i64p]),
+    "bug_last_op_flops": (ctypes.c_int, [_P, _f64p]),
+    "bug_bsm_product": (
+        ctypes.c_int,
+        [_P, _BSMP, _BSMP, _BSMP, ctypes.c_int32, ctypes.POINTER(BugTerm), ctypes.c_int32,
+         ctypes.c_int32, _P],
+    ),
+    "bug_env_push_left": (ctypes.c_int, [_P, _BSMP, _BSMP, _BSMP, _MPOP, _BSMP, ctypes.c_int32, _P]),
+    "bug_env_push_right": (ctypes.c_int, [_P, _BSMP, _BSMP, _BSMP, _MPOP, _BSMP, ctypes.c_int32, _P]),
+    "bug_apply_one_site": (
+        ctypes.c_int,
+        [_P, _BSMP, _MPOP, _BSMP, _BSMP, _BSMP, ctypes.c_double, ctypes.c_int32, _P],
+    ),
+    "bug_apply_zero_site": (
+        ctypes.c_int,
+        [_P, _BSMP, _BSMP, _BSMP, _BSMP, ctypes.c_double, ctypes.c_int32, _P],
+    ),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load_library(path: str = LIB_PATH):
+    """Load ``libbug.so`` and attach the C signatures.  Raises if it is missing."""
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"libbug.so not found at {path}: build it with bondupdategalerkin_amd.build.build() "
+            "(the package has no CPU fallback)"
+        )
+    handle = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(handle, name)
+        fn.restype = res
+        fn.argtypes = args
+    return handle
+
+
+def lib():
+    global _lib
+    with _lock:
+        if _lib is None:
+            _lib = load_library()
+        return _lib
+
+
+class BugError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"libbug error {code}: {message}")
+        self.code = code
+
+
+class Context:
+    """One library context (plan cache + per-stream scratch) on one device."""
+
+    def __init__(self, device: int = 0):
+        self._lib = lib()
+        handle = ctypes.c_void_p()
+        rc = self._lib.bug_ctx_create(device, ctypes.byref(handle))
+        if rc != BUG_OK:
+            raise BugError(rc, f"cannot create a context on device {device}")
+        self.handle = handle
+        self.device = device
+
+    def check(self, rc: int):
+        if rc != BUG_OK:
+            msg = self._lib.bug_last_error(self.handle)
+            raise BugError(rc, msg.decode() if msg else "")
+
+    def last_flops(self) -> float:
+        v = ctypes.c_double()
+        self.check(self._lib.bug_last_op_flops(self.handle, ctypes.byref(v)))
+        return v.value
+
+    def stats(self):
+        out = (ctypes.c_int64 * 4)()
+        self.check(self._lib.bug_ctx_stats(self.handle, out))
+        return {"bytes": out[0], "plans": out[1], "hits": out[2], "misses": out[3]}
+
+    def set_tile_config(self, config: int):
+        self.check(self._lib.bug_set_tile_config(self.handle, config))
+
+    def close(self):
+        if self.handle:
+            self._lib.bug_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_contexts = {}
+
+
+def context(device: int = 0) -> Context:
+    with _lock:
+        ctx = _contexts.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        with _lock:
+            _contexts[device] = ctx
+    return ctx
+
+
+def i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def ptr32(a: np.ndarray):
+    return a.ctypes.data_as(_i32p)
+
+
+def ptr64(a: np.ndarray):
+    return a.ctypes.data_as(_i64p)
+
+
+def ptrf(a: np.ndarray):
+    return a.ctypes.data_as(_f64p)

@@ -1,0 +1,677 @@
+// C ABI of the block-contraction engine: context, block-product planning, plan cache, the
+// environment pushes and the one-/zero-site effective-Hamiltonian applies.
+//
+// A contraction of block-sparse U(1) tensors is planned on the host once per layout signature
+// (charges, dims, offsets, terms) and cached: the plan lists every output block with the block
+// products that feed it.  Execution is one grouped-GEMM launch per contraction stage.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "internal.h"
+
+namespace bug {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(expr)                                                                  \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess)                                                            \
+            throw Error(BUG_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// ---- hashing of layouts -----------------------------------------------------------------
+
+struct Hasher {
+    uint64_t h = 1469598103934665603ull;
+    void bytes(const void* p, size_t n) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; ++i) {
+            h ^= c[i];
+            h *= 1099511628211ull;
+        }
+    }
+    template <typename T>
+    void pod(const T& v) { bytes(&v, sizeof(T)); }
+    void leg(const bug_leg& l) {
+        pod(l.nsec);
+        if (l.nsec > 0) {
+            bytes(l.charge, sizeof(int32_t) * l.nsec);
+            bytes(l.dim, sizeof(int32_t) * l.nsec);
+        }
+    }
+    void bsm(const bug_bsm& m) {
+        leg(m.row);
+        leg(m.col);
+        pod(m.nslice);
+        if (m.nslice > 0) {
+            bytes(m.shift, sizeof(int32_t) * m.nslice);
+            bytes(m.offset, sizeof(int64_t) * (size_t)m.nslice * m.row.nsec);
+        }
+    }
+};
+
+// ---- host view of a block-sparse matrix family ---------------------------------------------
+
+struct Leg {
+    std::vector<int32_t> q, d;
+    std::unordered_map<int32_t, int32_t> idx;
+    Leg() = default;
+    explicit Leg(const bug_leg& l) {
+        if (l.nsec < 0 || (l.nsec > 0 && (!l.charge || !l.dim)))
+            throw Error(BUG_EINVAL, "leg with negative sector count or null tables");
+        q.assign(l.charge, l.charge + l.nsec);
+        d.assign(l.dim, l.dim + l.nsec);
+        for (int i = 0; i < l.nsec; ++i) {
+            if (d[i] < 0) throw Error(BUG_EINVAL, "negative sector dimension");
+            if (!idx.emplace(q[i], i).second)
+                throw Error(BUG_EINVARIANT, "repeated charge on one leg");
+        }
+    }
+    int find(int32_t charge) const {
+        auto it = idx.find(charge);
+        return it == idx.end() ? -1 : it->second;
+    }
+    bug_leg view() const {
+        return bug_leg{(int32_t)q.size(), q.data(), d.data()};
+    }
+};
+
+struct Block {
+    int64_t off;
+    int rows, cols;
+    int32_t col_charge;
+};
+
+struct Bsm {
+    Leg row, col;
+    std::vector<int32_t> shift;
+    std::vector<int64_t> offset;
+    const void* data = nullptr;
+    int64_t nelem = 0;
+
+    Bsm() = default;
+    explicit Bsm(const bug_bsm* m, const char* what) {
+        if (!m) throw Error(BUG_EINVAL, std::string(what) + ": null descriptor");
+        row = Leg(m->row);
+        col = Leg(m->col);
+        if (m->nslice < 0 || (m->nslice > 0 && (!m->shift || (!m->offset && m->row.nsec > 0))))
+            throw Error(BUG_EINVAL, std::string(what) + ": bad slice tables");
+        shift.assign(m->shift, m->shift + m->nslice);
+        offset.assign(m->offset, m->offset + (size_t)m->nslice * m->row.nsec);
+        data = m->data;
+        nelem = m->nelem;
+        validate(what);
+    }
+    int nslice() const { return (int)shift.size(); }
+    void validate(const char* what) const {
+        for (int k = 0; k < nslice(); ++k)
+            for (size_t i = 0; i < row.q.size(); ++i) {
+                const int64_t off = offset[k * row.q.size() + i];
+                if (off < 0) continue;
+                const int j = col.find(row.q[i] + shift[k]);
+                if (j < 0)
+                    throw Error(BUG_EINVARIANT, std::string(what) +
+                                                    ": block present for a charge the column "
+                                                    "leg does not hold");
+                const int64_t end = off + (int64_t)row.d[i] * col.d[j];
+                if (end > nelem)
+                    throw Error(BUG_EINVARIANT,
+                                std::string(what) + ": block extends past the buffer");
+            }
+        if (nelem > 0 && !data) throw Error(BUG_EINVAL, std::string(what) + ": null data");
+    }
+    // block of slice k at row sector i (rows x cols), false if absent
+    bool block(int k, int i, Block& b) const {
+        if (k < 0 || k >= nslice() || i < 0) return false;
+        const int64_t off = offset[(size_t)k * row.q.size() + i];
+        if (off < 0) return false;
+        const int j = col.find(row.q[i] + shift[k]);
+        if (j < 0) return false;
+        b.off = off;
+        b.rows = row.d[i];
+        b.cols = col.d[j];
+        b.col_charge = col.q[j];
+        return true;
+    }
+    bug_bsm view(void* dev) const {
+        bug_bsm v;
+        v.row = row.view();
+        v.col = col.view();
+        v.nslice = nslice();
+        v.shift = shift.data();
+        v.offset = offset.data();
+        v.data = dev;
+        v.nelem = nelem;
+        return v;
+    }
+};
+
+// A dense layout for an intermediate family: slices with the given shifts over (row, col) legs,
+// every block that the charges allow.
+Bsm make_intermediate(const Leg& row, const Leg& col, const std::vector<int32_t>& shifts) {
+    Bsm m;
+    m.row = row;
+    m.col = col;
+    m.shift = shifts;
+    m.offset.assign(shifts.size() * row.q.size(), -1);
+    int64_t off = 0;
+    for (size_t k = 0; k < shifts.size(); ++k)
+        for (size_t i = 0; i < row.q.size(); ++i) {
+            const int j = col.find(row.q[i] + shifts[k]);
+            if (j < 0 || row.d[i] == 0 || col.d[j] == 0) continue;
+            m.offset[k * row.q.size() + i] = off;
+            off += (int64_t)row.d[i] * col.d[j];
+        }
+    m.nelem = off;
+    return m;
+}
+
+bool op_plain(int op) { return op == BUG_OP_N || op == BUG_OP_R; }
+
+// Plan out[o] (+)= sum alpha op_a(P[p]) op_b(Q[q]) over every block of out.
+void build_product_plan(Plan& plan, const Bsm& out, const Bsm& P, const Bsm& Q,
+                        const std::vector<bug_term>& terms, bool accumulate) {
+    plan.tasks.clear();
+    plan.terms.clear();
+    plan.flops = 0.0;
+    std::vector<std::vector<int>> by_out(out.nslice());
+    for (size_t t = 0; t < terms.size(); ++t) {
+        const bug_term& tm = terms[t];
+        if (tm.o < 0 || tm.o >= out.nslice() || tm.p < 0 || tm.p >= P.nslice() || tm.q < 0 ||
+            tm.q >= Q.nslice() || tm.op_a < 0 || tm.op_a > 3 || tm.op_b < 0 || tm.op_b > 3)
+            throw Error(BUG_EINVAL, "product term out of range");
+        by_out[tm.o].push_back((int)t);
+    }
+    for (int o = 0; o < out.nslice(); ++o)
+        for (size_t i = 0; i < out.row.q.size(); ++i) {
+            Block cb;
+            if (!out.block(o, (int)i, cb)) continue;
+            if (cb.rows == 0 || cb.cols == 0) continue;
+            GemmTask task;
+            task.c_off = cb.off;
+            task.ldc = cb.rows;
+            task.m = cb.rows;
+            task.n = cb.cols;
+            task.term_begin = (int32_t)plan.terms.size();
+            task.beta = accumulate ? 1 : 0;
+            const int32_t qi = out.row.q[i];
+            const int32_t qj = cb.col_charge;
+            for (int t : by_out[o]) {
+                const bug_term& tm = terms[t];
+                // op_a(P[p]) : rows of charge qi, inner charge qk
+                Block pb;
+                int32_t qk;
+                int kdim;
+                if (op_plain(tm.op_a)) {
+                    if (!P.block(tm.p, P.row.find(qi), pb)) continue;
+                    if (pb.rows != cb.rows) throw Error(BUG_EINVARIANT, "row dims disagree (P)");
+                    qk = pb.col_charge;
+                    kdim = pb.cols;
+                } else {
+                    qk = qi - P.shift[tm.p];
+                    if (!P.block(tm.p, P.row.find(qk), pb)) continue;
+                    if (pb.col_charge != qi) continue;
+                    if (pb.cols != cb.rows) throw Error(BUG_EINVARIANT, "row dims disagree (P^T)");
+                    kdim = pb.rows;
+                }
+                // op_b(Q[q]) : rows of charge qk, columns of charge qj
+                Block qb;
+                if (op_plain(tm.op_b)) {
+                    if (!Q.block(tm.q, Q.row.find(qk), qb)) continue;
+                    if (qb.col_charge != qj) continue;
+                    if (qb.rows != kdim || qb.cols != cb.cols)
+                        throw Error(BUG_EINVARIANT, "inner/column dims disagree (Q)");
+                } else {
+                    if (!Q.block(tm.q, Q.row.find(qj), qb)) continue;
+                    if (qb.col_charge != qk) continue;
+                    if (qb.cols != kdim || qb.rows != cb.cols)
+                        throw Error(BUG_EINVARIANT, "inner/column dims disagree (Q^T)");
+                }
+                if (kdim == 0) continue;
+                GemmTerm g;
+                g.a_off = pb.off;
+                g.b_off = qb.off;
+                g.lda = pb.rows;
+                g.ldb = qb.rows;
+                g.k = kdim;
+                g.ops = tm.op_a | (tm.op_b << 2);
+                g.alpha_re = tm.alpha_re;
+                g.alpha_im = tm.alpha_im;
+                plan.terms.push_back(g);
+                plan.flops += 8.0 * cb.rows * cb.cols * (double)kdim;
+            }
+            task.term_end = (int32_t)plan.terms.size();
+            // An overwrite with no terms still has to write zeros; an accumulate with none is a
+            // no-op and is dropped.
+            if (task.term_end == task.term_begin && accumulate) continue;
+            plan.tasks.push_back(task);
+        }
+}
+
+// Choose a tile configuration: the largest tile that still puts enough workgroups on the chip.
+int choose_config(const Plan& plan) {
+    double work = 0.0, small = 0.0;
+    for (const GemmTask& t : plan.tasks) {
+        const double w = (double)t.m * t.n;
+        work += w;
+        if (t.m <= 32 || t.n <= 32) small += w;
+    }
+    if (work == 0.0) return 0;
+    auto tiles_for = [&](int c) {
+        const TileConfig& tc = kTileConfigs[c];
+        int64_t n = 0;
+        for (const GemmTask& t : plan.tasks)
+            n += (int64_t)((t.m + tc.tm() - 1) / tc.tm()) * ((t.n + tc.tn() - 1) / tc.tn());
+        return n;
+    };
+    if (small / work > 0.5) return tiles_for(4) > 4 * 256 ? 1 : 4;
+    if (tiles_for(0) >= 2 * 256) return 0;
+    return 2;
+}
+
+void upload_plan(Plan& plan, int config) {
+    if (plan.config == config && plan.dev) return;
+    const TileConfig& tc = kTileConfigs[config];
+    std::vector<GemmTile> tiles;
+    for (size_t t = 0; t < plan.tasks.size(); ++t) {
+        const GemmTask& task = plan.tasks[t];
+        const int mt = (task.m + tc.tm() - 1) / tc.tm();
+        const int nt = (task.n + tc.tn() - 1) / tc.tn();
+        for (int a = 0; a < mt; ++a)
+            for (int b = 0; b < nt; ++b) tiles.push_back(GemmTile{(int32_t)t, a, b, 0});
+    }
+    const size_t bt = tiles.size() * sizeof(GemmTile);
+    const size_t bk = plan.tasks.size() * sizeof(GemmTask);
+    const size_t bm = plan.terms.size() * sizeof(GemmTerm);
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t total = align(bt) + align(bk) + align(bm) + 256;
+    if (plan.dev && plan.dev_bytes < total) {
+        HIP_CHECK(hipFree(plan.dev));
+        plan.dev = nullptr;
+    }
+    if (!plan.dev) {
+        HIP_CHECK(hipMalloc(&plan.dev, total));
+        plan.dev_bytes = total;
+    }
+    std::vector<char> host(total, 0);
+    std::memcpy(host.data(), tiles.data(), bt);
+    std::memcpy(host.data() + align(bt), plan.tasks.data(), bk);
+    std::memcpy(host.data() + align(bt) + align(bk), plan.terms.data(), bm);
+    HIP_CHECK(hipMemcpy(plan.dev, host.data(), total, hipMemcpyHostToDevice));
+    char* base = static_cast<char*>(plan.dev);
+    plan.d_tiles = reinterpret_cast<const GemmTile*>(base);
+    plan.d_tasks = reinterpret_cast<const GemmTask*>(base + align(bt));
+    plan.d_terms = reinterpret_cast<const GemmTerm*>(base + align(bt) + align(bk));
+    plan.ntiles = (int32_t)tiles.size();
+    plan.config = config;
+}
+
+// Look up (or build) the plan of one product stage and launch it.
+void run_product(bug_ctx* ctx, const Bsm& out, void* out_dev, const Bsm& P, const void* p_dev,
+                 const Bsm& Q, const void* q_dev, const std::vector<bug_term>& terms,
+                 bool accumulate, int precision, hipStream_t stream, uint64_t tag) {
+    Hasher h;
+    h.pod(tag);
+    {
+        bug_bsm vo = out.view(nullptr), vp = P.view(nullptr), vq = Q.view(nullptr);
+        vo.nelem = vp.nelem = vq.nelem = 0;
+        h.bsm(vo);
+        h.bsm(vp);
+        h.bsm(vq);
+    }
+    for (const bug_term& t : terms) h.pod(t);
+    h.pod(accumulate);
+    const uint64_t key = h.h;
+    Plan* plan = nullptr;
+    auto it = ctx->plan_index.find(key);
+    if (it != ctx->plan_index.end()) {
+        ctx->plan_lru.splice(ctx->plan_lru.begin(), ctx->plan_lru, it->second);
+        plan = it->second->second;
+        ++ctx->plan_hits;
+    } else {
+        ++ctx->plan_misses;
+        plan = new Plan();
+        try {
+            build_product_plan(*plan, out, P, Q, terms, accumulate);
+        } catch (...) {
+            delete plan;
+            throw;
+        }
+        ctx->plan_lru.emplace_front(key, plan);
+        ctx->plan_index[key] = ctx->plan_lru.begin();
+        while (ctx->plan_lru.size() > ctx->plan_capacity) {
+            Plan* old = ctx->plan_lru.back().second;
+            ctx->plan_index.erase(ctx->plan_lru.back().first);
+            ctx->plan_lru.pop_back();
+            if (old->dev) (void)hipFree(old->dev);
+            delete old;
+        }
+    }
+    const int config = ctx->forced_config >= 0 ? ctx->forced_config : choose_config(*plan);
+    upload_plan(*plan, config);
+    ctx->last_flops += plan->flops;
+    HIP_CHECK(launch_grouped_gemm(*plan, config, precision, p_dev, q_dev, out_dev, stream));
+}
+
+void* scratch(bug_ctx* ctx, hipStream_t stream, size_t bytes) {
+    Workspace& w = ctx->scratch[(void*)stream];
+    if (w.bytes < bytes) {
+        if (w.ptr) {
+            HIP_CHECK(hipStreamSynchronize(stream));
+            HIP_CHECK(hipFree(w.ptr));
+            w.ptr = nullptr;
+            w.bytes = 0;
+        }
+        const size_t want = std::max(bytes, (size_t)(1.5 * (double)bytes));
+        if (hipMalloc(&w.ptr, want) != hipSuccess) {
+            (void)hipGetLastError();
+            w.ptr = nullptr;
+            throw Error(BUG_ENOMEM, "device scratch allocation failed");
+        }
+        w.bytes = want;
+    }
+    return w.ptr;
+}
+
+__global__ void axpy_kernel(double2* __restrict__ y, const double2* __restrict__ x, double a,
+                            int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        y[i].x += a * x[i].x;
+        y[i].y += a * x[i].y;
+    }
+}
+
+void same_layout(const Bsm& a, const Bsm& b, const char* what) {
+    if (a.row.q != b.row.q || a.row.d != b.row.d || a.col.q != b.col.q || a.col.d != b.col.d ||
+        a.shift != b.shift || a.offset != b.offset)
+        throw Error(BUG_EINVARIANT, std::string(what) + ": layouts differ");
+}
+
+struct MpoSite {
+    int wl, wr;
+    std::vector<int32_t> cl, cr, a, b, so, si;
+    std::vector<double> coeff;
+    explicit MpoSite(const bug_mpo_site* w, int d) {
+        if (!w) throw Error(BUG_EINVAL, "null MPO site");
+        wl = w->wl;
+        wr = w->wr;
+        if (wl <= 0 || wr <= 0 || w->nent < 0) throw Error(BUG_EINVAL, "bad MPO site dims");
+        cl.assign(w->cl, w->cl + wl);
+        cr.assign(w->cr, w->cr + wr);
+        a.assign(w->a, w->a + w->nent);
+        b.assign(w->b, w->b + w->nent);
+        so.assign(w->sout, w->sout + w->nent);
+        si.assign(w->sin, w->sin + w->nent);
+        coeff.assign(w->coeff, w->coeff + 2 * w->nent);
+        for (int e = 0; e < w->nent; ++e)
+            if (a[e] < 0 || a[e] >= wl || b[e] < 0 || b[e] >= wr || so[e] < 0 || so[e] >= d ||
+                si[e] < 0 || si[e] >= d)
+                throw Error(BUG_EINVAL, "MPO entry out of range");
+    }
+    int nent() const { return (int)a.size(); }
+};
+
+}  // namespace bug
+
+using namespace bug;
+
+namespace {
+
+template <typename F>
+int guarded(bug_ctx* ctx, F&& f) {
+    if (!ctx) return BUG_EINVAL;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    try {
+        HIP_CHECK(hipSetDevice(ctx->device));
+        ctx->last_flops = 0.0;
+        f();
+        return BUG_OK;
+    } catch (const Error& e) {
+        ctx->last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        ctx->last_error = e.what();
+        return BUG_EINVAL;
+    }
+}
+
+hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+}  // namespace
+
+extern "C" {
+
+int bug_ctx_create(int device, bug_ctx** out) {
+    if (!out) return BUG_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+        (void)hipGetLastError();
+        return BUG_EDEVICE;
+    }
+    bug_ctx* ctx = new bug_ctx();
+    ctx->device = device;
+    *out = ctx;
+    return BUG_OK;
+}
+
+int bug_ctx_destroy(bug_ctx* ctx) {
+    if (!ctx) return BUG_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    for (auto& e : ctx->plan_lru) {
+        if (e.second->dev) (void)hipFree(e.second->dev);
+        delete e.second;
+    }
+    for (auto& s : ctx->scratch)
+        if (s.second.ptr) (void)hipFree(s.second.ptr);
+    delete ctx;
+    return BUG_OK;
+}
+
+const char* bug_last_error(const bug_ctx* ctx) {
+    return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+int bug_set_tile_config(bug_ctx* ctx, int config) {
+    if (!ctx || config < -1 || config >= kNumTileConfigs) return BUG_EINVAL;
+    ctx->forced_config = config;
+    return BUG_OK;
+}
+
+int bug_tile_config_count(void) { return kNumTileConfigs; }
+
+int bug_ctx_stats(const bug_ctx* ctx, int64_t* out4) {
+    if (!ctx || !out4) return BUG_EINVAL;
+    int64_t bytes = 0;
+    for (auto& s : ctx->scratch) bytes += (int64_t)s.second.bytes;
+    for (auto& e : ctx->plan_lru) bytes += (int64_t)e.second->dev_bytes;
+    out4[0] = bytes;
+    out4[1] = (int64_t)ctx->plan_lru.size();
+    out4[2] = ctx->plan_hits;
+    out4[3] = ctx->plan_misses;
+    return BUG_OK;
+}
+
+int bug_last_op_flops(const bug_ctx* ctx, double* flops) {
+    if (!ctx || !flops) return BUG_EINVAL;
+    *flops = ctx->last_flops;
+    return BUG_OK;
+}
+
+int bug_bsm_product(bug_ctx* ctx, const bug_bsm* out, const bug_bsm* P, const bug_bsm* Q,
+                    int32_t nterm, const bug_term* terms, int32_t accumulate, int32_t precision,
+                    void* stream) {
+    return guarded(ctx, [&] {
+        if (nterm < 0 || (nterm > 0 && !terms)) throw Error(BUG_EINVAL, "bad term list");
+        if (precision != BUG_PREC_FP64 && precision != BUG_PREC_FP32)
+            throw Error(BUG_EINVAL, "unknown precision");
+        Bsm o(out, "out"), p(P, "P"), q(Q, "Q");
+        std::vector<bug_term> tl(terms, terms + nterm);
+        run_product(ctx, o, out->data, p, P->data, q, Q->data, tl, accumulate != 0, precision,
+                    as_stream(stream), 0x01);
+    });
+}
+
+int bug_env_push_left(bug_ctx* ctx, const bug_bsm* L, const bug_bsm* A, const bug_bsm* bra,
+                      const bug_mpo_site* W, const bug_bsm* Lout, int32_t precision,
+                      void* stream) {
+    return guarded(ctx, [&] {
+        if (precision != BUG_PREC_FP64 && precision != BUG_PREC_FP32)
+            throw Error(BUG_EINVAL, "unknown precision");
+        Bsm l(L, "L"), a(A, "A"), lo(Lout, "Lout");
+        Bsm br = bra ? Bsm(bra, "bra") : a;
+        const void* bra_dev = bra ? bra->data : A->data;
+        MpoSite w(W, a.nslice());
+        if (l.nslice() != w.wl || lo.nslice() != w.wr)
+            throw Error(BUG_EINVAL, "environment channels do not match the MPO site");
+        if (br.nslice() != a.nslice()) throw Error(BUG_EINVAL, "bra/ket local dims differ");
+        // stage 1: T[(a, sin)] = L_a A_sin
+        std::vector<int> slot(w.wl * a.nslice(), -1);
+        std::vector<int32_t> shifts;
+        std::vector<bug_term> t1;
+        for (int e = 0; e < w.nent(); ++e) {
+            int& s = slot[w.a[e] * a.nslice() + w.si[e]];
+            if (s >= 0) continue;
+            s = (int)shifts.size();
+            shifts.push_back(l.shift[w.a[e]] + a.shift[w.si[e]]);
+            t1.push_back(bug_term{s, w.a[e], w.si[e], BUG_OP_N, BUG_OP_N, 1.0, 0.0});
+        }
+        Bsm T = make_intermediate(l.row, a.col, shifts);
+        hipStream_t st = as_stream(stream);
+        void* tdev = scratch(ctx, st, sizeof(double2) * std::max<int64_t>(T.nelem, 1));
+        run_product(ctx, T, tdev, l, L->data, a, A->data, t1, false, precision, st, 0x11);
+        // stage 2: Lout_b = sum coeff bra_sout^H T[(a, sin)]
+        std::vector<bug_term> t2;
+        for (int e = 0; e < w.nent(); ++e)
+            t2.push_back(bug_term{w.b[e], w.so[e], slot[w.a[e] * a.nslice() + w.si[e]],
+                                  BUG_OP_C, BUG_OP_N, w.coeff[2 * e], w.coeff[2 * e + 1]});
+        run_product(ctx, lo, Lout->data, br, bra_dev, T, tdev, t2, false, precision, st, 0x12);
+    });
+}
+
+int bug_env_push_right(bug_ctx* ctx, const bug_bsm* R, const bug_bsm* B, const bug_bsm* bra,
+                       const bug_mpo_site* W, const bug_bsm* Rout, int32_t precision,
+                       void* stream) {
+    return guarded(ctx, [&] {
+        if (precision != BUG_PREC_FP64 && precision != BUG_PREC_FP32)
+            throw Error(BUG_EINVAL, "unknown precision");
+        Bsm r(R, "R"), b(B, "B"), ro(Rout, "Rout");
+        Bsm br = bra ? Bsm(bra, "bra") : b;
+        const void* bra_dev = bra ? bra->data : B->data;
+        MpoSite w(W, b.nslice());
+        if (r.nslice() != w.wr || ro.nslice() != w.wl)
+            throw Error(BUG_EINVAL, "environment channels do not match the MPO site");
+        if (br.nslice() != b.nslice()) throw Error(BUG_EINVAL, "bra/ket local dims differ");
+        // stage 1: U[(b, sin)] = R_b B_sin^T
+        std::vector<int> slot(w.wr * b.nslice(), -1);
+        std::vector<int32_t> shifts;
+        std::vector<bug_term> t1;
+        for (int e = 0; e < w.nent(); ++e) {
+            int& s = slot[w.b[e] * b.nslice() + w.si[e]];
+            if (s >= 0) continue;
+            s = (int)shifts.size();
+            shifts.push_back(r.shift[w.b[e]] - b.shift[w.si[e]]);
+            t1.push_back(bug_term{s, w.b[e], w.si[e], BUG_OP_N, BUG_OP_T, 1.0, 0.0});
+        }
+        Bsm U = make_intermediate(r.row, b.row, shifts);
+        hipStream_t st = as_stream(stream);
+        void* udev = scratch(ctx, st, sizeof(double2) * std::max<int64_t>(U.nelem, 1));
+        run_product(ctx, U, udev, r, R->data, b, B->data, t1, false, precision, st, 0x21);
+        // stage 2: Rout_a = sum coeff conj(bra_sout) U[(b, sin)]
+        std::vector<bug_term> t2;
+        for (int e = 0; e < w.nent(); ++e)
+            t2.push_back(bug_term{w.a[e], w.so[e], slot[w.b[e] * b.nslice() + w.si[e]],
+                                  BUG_OP_R, BUG_OP_N, w.coeff[2 * e], w.coeff[2 * e + 1]});
+        run_product(ctx, ro, Rout->data, br, bra_dev, U, udev, t2, false, precision, st, 0x22);
+    });
+}
+
+int bug_apply_one_site(bug_ctx* ctx, const bug_bsm* L, const bug_mpo_site* W, const bug_bsm* R,
+                       const bug_bsm* X, const bug_bsm* Y, double shift, int32_t precision,
+                       void* stream) {
+    return guarded(ctx, [&] {
+        if (precision != BUG_PREC_FP64 && precision != BUG_PREC_FP32)
+            throw Error(BUG_EINVAL, "unknown precision");
+        Bsm l(L, "L"), r(R, "R"), x(X, "X"), y(Y, "Y");
+        MpoSite w(W, x.nslice());
+        if (l.nslice() != w.wl || r.nslice() != w.wr)
+            throw Error(BUG_EINVAL, "environment channels do not match the MPO site");
+        if (y.nslice() != x.nslice()) throw Error(BUG_EINVAL, "X/Y local dims differ");
+        // stage 1: T[(a, sin)] = L_a X_sin
+        std::vector<int> slot(w.wl * x.nslice(), -1);
+        std::vector<int32_t> shifts;
+        std::vector<bug_term> t1;
+        for (int e = 0; e < w.nent(); ++e) {
+            int& s = slot[w.a[e] * x.nslice() + w.si[e]];
+            if (s >= 0) continue;
+            s = (int)shifts.size();
+            shifts.push_back(l.shift[w.a[e]] + x.shift[w.si[e]]);
+            t1.push_back(bug_term{s, w.a[e], w.si[e], BUG_OP_N, BUG_OP_N, 1.0, 0.0});
+        }
+        Bsm T = make_intermediate(l.row, x.col, shifts);
+        hipStream_t st = as_stream(stream);
+        void* tdev = scratch(ctx, st, sizeof(double2) * std::max<int64_t>(T.nelem, 1));
+        run_product(ctx, T, tdev, l, L->data, x, X->data, t1, false, precision, st, 0x31);
+        // stage 2: Y_sout = sum coeff T[(a, sin)] R_b^T
+        std::vector<bug_term> t2;
+        for (int e = 0; e < w.nent(); ++e)
+            t2.push_back(bug_term{w.so[e], slot[w.a[e] * x.nslice() + w.si[e]], w.b[e],
+                                  BUG_OP_N, BUG_OP_T, w.coeff[2 * e], w.coeff[2 * e + 1]});
+        run_product(ctx, y, Y->data, T, tdev, r, R->data, t2, false, precision, st, 0x32);
+        if (shift != 0.0) {
+            same_layout(x, y, "apply_one_site shift");
+            const int64_t n = x.nelem;
+            if (n > 0)
+                hipLaunchKernelGGL(axpy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                                   st, static_cast<double2*>(Y->data),
+                                   static_cast<const double2*>(X->data), -shift, n);
+            HIP_CHECK(hipGetLastError());
+        }
+    });
+}
+
+int bug_apply_zero_site(bug_ctx* ctx, const bug_bsm* L, const bug_bsm* R, const bug_bsm* C,
+                        const bug_bsm* Y, double shift, int32_t precision, void* stream) {
+    return guarded(ctx, [&] {
+        if (precision != BUG_PREC_FP64 && precision != BUG_PREC_FP32)
+            throw Error(BUG_EINVAL, "unknown precision");
+        Bsm l(L, "L"), r(R, "R"), c(C, "C"), y(Y, "Y");
+        if (l.nslice() != r.nslice())
+            throw Error(BUG_EINVAL, "root environments have different channel counts");
+        if (c.nslice() != 1 || y.nslice() != 1)
+            throw Error(BUG_EINVAL, "a zero-site core has one slice");
+        // stage 1: T_w = L_w C
+        std::vector<int32_t> shifts;
+        std::vector<bug_term> t1, t2;
+        for (int k = 0; k < l.nslice(); ++k) {
+            shifts.push_back(l.shift[k] + c.shift[0]);
+            t1.push_back(bug_term{k, k, 0, BUG_OP_N, BUG_OP_N, 1.0, 0.0});
+            t2.push_back(bug_term{0, k, k, BUG_OP_N, BUG_OP_T, 1.0, 0.0});
+        }
+        Bsm T = make_intermediate(l.row, c.col, shifts);
+        hipStream_t st = as_stream(stream);
+        void* tdev = scratch(ctx, st, sizeof(double2) * std::max<int64_t>(T.nelem, 1));
+        run_product(ctx, T, tdev, l, L->data, c, C->data, t1, false, precision, st, 0x41);
+        // stage 2: Y = sum_w T_w R_w^T
+        run_product(ctx, y, Y->data, T, tdev, r, R->data, t2, false, precision, st, 0x42);
+        if (shift != 0.0) {
+            same_layout(c, y, "apply_zero_site shift");
+            const int64_t n = c.nelem;
+            if (n > 0)
+                hipLaunchKernelGGL(axpy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                                   st, static_cast<double2*>(Y->data),
+                                   static_cast<const double2*>(C->data), -shift, n);
+            HIP_CHECK(hipGetLastError());
+        }
+    });
+}
+
+}  // extern "C"

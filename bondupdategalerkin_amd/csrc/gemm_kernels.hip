@@ -1,0 +1,254 @@
+// Grouped complex block GEMM for gfx950.
+//
+// One launch executes every output block of a block-sparse contraction: each output block is a
+// "task" C = beta C + sum_t alpha_t op_a(A_t) op_b(B_t) over its own list of terms, and each
+// workgroup owns one output tile of one task, so terms accumulate in registers with no atomics
+// and the result is deterministic.  This replaces the per-sector-pair GEMM loop of Telum's
+// abelian contraction (deps/Telum/src/contract.jl:1022-1036) with one launch per contraction
+// stage.
+//
+// Complex arithmetic is split into four real MFMA products:
+//   re += Br Ar - Bi Ai,  im += Br Ai + Bi Ar
+// FP64 path: v_mfma_f64_16x16x4f64, FP64 accumulate (the drop-in).
+// FP32 path: operands rounded to FP32 on load into LDS, v_mfma_f32_16x16x4f32, FP32 accumulate
+// (the mixed-precision path); output written back as FP64.
+//
+// MFMA 16x16x4 operand layout (one wave, measured on gfx950): operand "a" lane l holds
+// X[l % 16][l / 16], operand "b" lane l holds Y[l / 16][l % 16]; result register r of lane l is
+// D[i][l % 16] with i = 4 (l / 16) + r for f32 and i = (l / 16) + 4 r for f64.
+// Feeding a = B^T and b = A^T produces D = (A B)^T, so lane l % 16 walks the ROW index of the
+// column-major output and 16 lanes store 16 consecutive complex elements.
+#include "internal.h"
+
+namespace bug {
+
+const TileConfig kTileConfigs[] = {
+    {2, 2, 2, 2},  // 64 x 64, 256 threads
+    {1, 1, 2, 2},  // 32 x 32, 64 threads
+    {2, 2, 1, 1},  // 32 x 32, 256 threads
+    {2, 2, 4, 2},  // 128 x 64, 256 threads
+    {1, 1, 1, 1},  // 16 x 16, 64 threads
+};
+const int kNumTileConfigs = sizeof(kTileConfigs) / sizeof(kTileConfigs[0]);
+
+namespace {
+
+constexpr int kTK = 16;  // complex k per LDS stage
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+    typedef d4 acc_t;
+    // row of D held by register r of lane group g = lane / 16
+    static __device__ __forceinline__ int drow(int g, int r) { return g + 4 * r; }
+    static __device__ __forceinline__ acc_t step(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct Mfma<float> {
+    typedef f4 acc_t;
+    static __device__ __forceinline__ int drow(int g, int r) { return 4 * g + r; }
+    static __device__ __forceinline__ acc_t step(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs; map them so that consecutive logical
+// tiles (tiles of one output block, which share operand rows and columns) land on one XCD and
+// share its L2.
+__device__ __forceinline__ int xcd_remap(int bid, int grid) {
+    const int per = grid / 8, rem = grid % 8;
+    const int xcd = bid % 8, local = bid / 8;
+    return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+template <typename T, int WM, int WN, int FM, int FN>
+__global__ __launch_bounds__(64 * WM * WN) void grouped_zgemm(
+    const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks,
+    const GemmTerm* __restrict__ terms, const double2* __restrict__ A,
+    const double2* __restrict__ B, double2* __restrict__ C) {
+    constexpr int NT = 64 * WM * WN;
+    constexpr int TM = 16 * FM * WM;
+    constexpr int TN = 16 * FN * WN;
+    typedef typename Mfma<T>::acc_t acc_t;
+    __shared__ T As_re[kTK][TM];
+    __shared__ T As_im[kTK][TM];
+    __shared__ T Bs_re[kTK][TN];
+    __shared__ T Bs_im[kTK][TN];
+
+    const int logical = xcd_remap(blockIdx.x, gridDim.x);
+    const GemmTile tile = tiles[logical];
+    const GemmTask task = tasks[tile.task];
+    const int m0 = tile.tm * TM;
+    const int n0 = tile.tn * TN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave % WM;
+    const int wn = wave / WM;
+    const int lrow = lane & 15;
+    const int lk = lane >> 4;
+
+    acc_t acc_re[FM][FN];
+    acc_t acc_im[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            acc_re[i][j] = acc_t{0, 0, 0, 0};
+            acc_im[i][j] = acc_t{0, 0, 0, 0};
+        }
+
+    for (int t = task.term_begin; t < task.term_end; ++t) {
+        const GemmTerm term = terms[t];
+        const int opa = term.ops & 3;
+        const int opb = (term.ops >> 2) & 3;
+        const bool a_plain = (opa == BUG_OP_N || opa == BUG_OP_R);
+        const bool b_plain = (opb == BUG_OP_N || opb == BUG_OP_R);
+        const double a_conj = (opa == BUG_OP_C || opa == BUG_OP_R) ? -1.0 : 1.0;
+        const double b_conj = (opb == BUG_OP_C || opb == BUG_OP_R) ? -1.0 : 1.0;
+        const double2* __restrict__ Ap = A + term.a_off;
+        const double2* __restrict__ Bp = B + term.b_off;
+        const int64_t lda = term.lda;
+        const int64_t ldb = term.ldb;
+        for (int k0 = 0; k0 < term.k; k0 += kTK) {
+            // op_a(A)[m0 + i][k0 + kk] scaled by alpha -> LDS (k-major)
+#pragma unroll
+            for (int e = tid; e < TM * kTK; e += NT) {
+                int i, kk;
+                if (a_plain) {
+                    i = e % TM;
+                    kk = e / TM;
+                } else {
+                    kk = e % kTK;
+                    i = e / kTK;
+                }
+                const int gi = m0 + i;
+                const int gk = k0 + kk;
+                double re = 0.0, im = 0.0;
+                if (gi < task.m && gk < term.k) {
+                    const double2 v = a_plain ? Ap[gi + gk * lda] : Ap[gk + gi * lda];
+                    const double vi = a_conj * v.y;
+                    re = term.alpha_re * v.x - term.alpha_im * vi;
+                    im = term.alpha_re * vi + term.alpha_im * v.x;
+                }
+                As_re[kk][i] = static_cast<T>(re);
+                As_im[kk][i] = static_cast<T>(im);
+            }
+            // op_b(B)[k0 + kk][n0 + j] -> LDS (k-major)
+#pragma unroll
+            for (int e = tid; e < kTK * TN; e += NT) {
+                int kk, j;
+                if (b_plain) {
+                    kk = e % kTK;
+                    j = e / kTK;
+                } else {
+                    j = e % TN;
+                    kk = e / TN;
+                }
+                const int gj = n0 + j;
+                const int gk = k0 + kk;
+                double re = 0.0, im = 0.0;
+                if (gj < task.n && gk < term.k) {
+                    const double2 v = b_plain ? Bp[gk + gj * ldb] : Bp[gj + gk * ldb];
+                    re = v.x;
+                    im = b_conj * v.y;
+                }
+                Bs_re[kk][j] = static_cast<T>(re);
+                Bs_im[kk][j] = static_cast<T>(im);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < kTK; kk += 4) {
+                T a_re[FM], a_im[FM], b_re[FN], b_im[FN];
+#pragma unroll
+                for (int i = 0; i < FM; ++i) {
+                    const int r = wm * 16 * FM + i * 16 + lrow;
+                    a_re[i] = As_re[kk + lk][r];
+                    a_im[i] = As_im[kk + lk][r];
+                }
+#pragma unroll
+                for (int j = 0; j < FN; ++j) {
+                    const int c = wn * 16 * FN + j * 16 + lrow;
+                    b_re[j] = Bs_re[kk + lk][c];
+                    b_im[j] = Bs_im[kk + lk][c];
+                }
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j) {
+                        acc_re[i][j] = Mfma<T>::step(b_re[j], a_re[i], acc_re[i][j]);
+                        acc_re[i][j] = Mfma<T>::step(-b_im[j], a_im[i], acc_re[i][j]);
+                        acc_im[i][j] = Mfma<T>::step(b_re[j], a_im[i], acc_im[i][j]);
+                        acc_im[i][j] = Mfma<T>::step(b_im[j], a_re[i], acc_im[i][j]);
+                    }
+            }
+            __syncthreads();
+        }
+    }
+
+    double2* __restrict__ Cp = C + task.c_off;
+    const int64_t ldc = task.ldc;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gm = m0 + wm * 16 * FM + i * 16 + lrow;
+                const int gn = n0 + wn * 16 * FN + j * 16 + Mfma<T>::drow(lk, r);
+                if (gm < task.m && gn < task.n) {
+                    double2 out;
+                    out.x = static_cast<double>(acc_re[i][j][r]);
+                    out.y = static_cast<double>(acc_im[i][j][r]);
+                    double2* dst = Cp + gm + gn * ldc;
+                    if (task.beta) {
+                        const double2 old = *dst;
+                        out.x += old.x;
+                        out.y += old.y;
+                    }
+                    *dst = out;
+                }
+            }
+}
+
+template <typename T, int WM, int WN, int FM, int FN>
+hipError_t launch_one(const Plan& plan, const void* A, const void* B, void* C,
+                      hipStream_t stream) {
+    hipLaunchKernelGGL((grouped_zgemm<T, WM, WN, FM, FN>), dim3(plan.ntiles),
+                       dim3(64 * WM * WN), 0, stream, plan.d_tiles, plan.d_tasks, plan.d_terms,
+                       static_cast<const double2*>(A), static_cast<const double2*>(B),
+                       static_cast<double2*>(C));
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_typed(const Plan& plan, int config, const void* A, const void* B, void* C,
+                        hipStream_t stream) {
+    switch (config) {
+        case 0: return launch_one<T, 2, 2, 2, 2>(plan, A, B, C, stream);
+        case 1: return launch_one<T, 1, 1, 2, 2>(plan, A, B, C, stream);
+        case 2: return launch_one<T, 2, 2, 1, 1>(plan, A, B, C, stream);
+        case 3: return launch_one<T, 2, 2, 4, 2>(plan, A, B, C, stream);
+        case 4: return launch_one<T, 1, 1, 1, 1>(plan, A, B, C, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_grouped_gemm(const Plan& plan, int config, int precision, const void* A,
+                               const void* B, void* C, hipStream_t stream) {
+    if (plan.ntiles == 0) return hipSuccess;
+    if (precision == BUG_PREC_FP32) return launch_typed<float>(plan, config, A, B, C, stream);
+    return launch_typed<double>(plan, config, A, B, C, stream);
+}
+
+}  // namespace bug

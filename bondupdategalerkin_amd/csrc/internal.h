@@ -1,0 +1,99 @@
+// Shared declarations of the gfx950 backend: device plan records, the context, error helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <list>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/bug.h"
+
+namespace bug {
+
+// One product term of an output block: alpha * op_a(A) * op_b(B), A at a_off of the A operand,
+// B at b_off of the B operand, inner dimension k.  ops = op_a | (op_b << 2).
+struct GemmTerm {
+    int64_t a_off;
+    int64_t b_off;
+    int32_t lda;
+    int32_t ldb;
+    int32_t k;
+    int32_t ops;
+    double alpha_re;
+    double alpha_im;
+};
+
+// One output block: m x n at c_off (leading dimension ldc); its terms are
+// terms[term_begin, term_end).  beta = 1 accumulates onto the block, 0 overwrites it.
+struct GemmTask {
+    int64_t c_off;
+    int32_t ldc;
+    int32_t m;
+    int32_t n;
+    int32_t term_begin;
+    int32_t term_end;
+    int32_t beta;
+};
+
+// One workgroup's output tile.
+struct GemmTile {
+    int32_t task;
+    int32_t tm;
+    int32_t tn;
+    int32_t pad;
+};
+
+// A product plan: tasks and terms in host memory, plus their device copy.
+struct Plan {
+    std::vector<GemmTask> tasks;
+    std::vector<GemmTerm> terms;
+    double flops = 0.0;          // algorithmic 8 m n k summed over terms
+    // device image: [tiles | tasks | terms], per tile configuration
+    int config = -1;
+    int32_t ntiles = 0;
+    void* dev = nullptr;
+    size_t dev_bytes = 0;
+    const GemmTile* d_tiles = nullptr;
+    const GemmTask* d_tasks = nullptr;
+    const GemmTerm* d_terms = nullptr;
+};
+
+// Tile configurations of the grouped GEMM (waves in m, waves in n, 16x16 MFMA tiles per wave).
+struct TileConfig {
+    int wm, wn, fm, fn;
+    int tm() const { return 16 * fm * wm; }
+    int tn() const { return 16 * fn * wn; }
+    int threads() const { return 64 * wm * wn; }
+};
+extern const TileConfig kTileConfigs[];
+extern const int kNumTileConfigs;
+
+// Launch the grouped GEMM of `plan` (device image already uploaded for `config`).
+hipError_t launch_grouped_gemm(const Plan& plan, int config, int precision, const void* A,
+                               const void* B, void* C, hipStream_t stream);
+
+struct Workspace {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace bug
+
+struct bug_ctx {
+    int device = 0;
+    std::string last_error;
+    int forced_config = -1;
+    std::mutex mu;
+    // plan cache: signature hash -> plan (LRU)
+    std::unordered_map<uint64_t, std::list<std::pair<uint64_t, bug::Plan*>>::iterator> plan_index;
+    std::list<std::pair<uint64_t, bug::Plan*>> plan_lru;
+    size_t plan_capacity = 512;
+    int64_t plan_hits = 0, plan_misses = 0;
+    double last_flops = 0.0;     // algorithmic flops of the last entry-point call
+    // per-stream scratch for intermediates of the two-stage contractions
+    std::unordered_map<void*, bug::Workspace> scratch;
+};

@@ -1,0 +1,143 @@
+"""Matrix product states on the device: container, product states, synthetic states.
+
+Reference container: ``SymMPS`` (src/BUG/Core/TensorNetworks/symmetric_mps.jl:44-54), tensors
+``(l, s, r)`` with a centre; product states ``product_state``/``domain_wall_state``/``neel_state``
+(src/QMB/Tooling/initial_states.jl:15-80,137-146).  Sites are 0-based here (site ``i`` of the
+reference is ``i-1``); ``center`` follows the same shift.
+"""
+from __future__ import annotations
+
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from .blocks import BlockTensor, Layout, Leg
+from .mpo import DOWN, SPIN_CHARGES_NONE, SPIN_CHARGES_U1, UP
+
+
+class SymMPS:
+    """L site tensors ``A_s`` (row = left bond, col = right bond, shift = local charge)."""
+
+    def __init__(self, tensors: List[BlockTensor], charges: Sequence[int], center: int = 0):
+        self.tensors = list(tensors)
+        self.charges = tuple(int(c) for c in charges)
+        self.center = int(center)
+        for i, t in enumerate(self.tensors):
+            if t.layout.shifts != self.charges:
+                raise ValueError(f"site {i}: slice shifts {t.layout.shifts} != local charges")
+            if i > 0 and t.layout.row != self.tensors[i - 1].layout.col:
+                raise ValueError(f"bond {i}: left leg of site {i} != right leg of site {i - 1}")
+
+    def __len__(self):
+        return len(self.tensors)
+
+    def __getitem__(self, i: int) -> BlockTensor:
+        return self.tensors[i]
+
+    def __setitem__(self, i: int, t: BlockTensor):
+        self.tensors[i] = t
+
+    @property
+    def d(self) -> int:
+        return len(self.charges)
+
+    def bond_leg(self, b: int) -> Leg:
+        """Leg of link b (b = 0 .. L): link b sits left of site b."""
+        return self.tensors[b].layout.row if b < len(self) else self.tensors[-1].layout.col
+
+    def bond_dims(self) -> List[int]:
+        """Dimensions of the L-1 internal bonds (reference ``bond_dims``, symmetric_mps.jl:64)."""
+        return [self.tensors[i].layout.col.total for i in range(len(self) - 1)]
+
+    def copy(self) -> "SymMPS":
+        """Shares the site tensors, as ``copy(psi)`` does (operations replace, never mutate)."""
+        return SymMPS(list(self.tensors), self.charges, self.center)
+
+
+def site_layout(left: Leg, right: Leg, charges: Sequence[int]) -> Layout:
+    return Layout(left, right, charges)
+
+
+def product_state(spins: Sequence[str], symmetry: str = "U1") -> SymMPS:
+    """Product state of ``:up``/``:down`` spins, bond dimension 1 (initial_states.jl:15-80)."""
+    charges = SPIN_CHARGES_U1 if symmetry == "U1" else SPIN_CHARGES_NONE
+    tensors = []
+    q = 0
+    for i, s in enumerate(spins):
+        if s not in ("up", "down"):
+            raise ValueError(f"spin {i} must be 'up' or 'down', got {s!r}")
+        st = UP if s == "up" else DOWN
+        left = Leg((q,), (1,))
+        q2 = q + charges[st]
+        right = Leg((q2,), (1,))
+        lay = Layout(left, right, charges)
+        blocks = {(st, q): np.ones((1, 1), dtype=np.complex128)}
+        tensors.append(BlockTensor.from_blocks(lay, blocks))
+        q = q2
+    return SymMPS(tensors, charges, center=0)
+
+
+def domain_wall_state(L: int, symmetry: str = "U1") -> SymMPS:
+    """up...up down...down with the wall in the middle (initial_states.jl:137-140)."""
+    return product_state(["up" if i < L // 2 else "down" for i in range(L)], symmetry)
+
+
+def neel_state(L: int, symmetry: str = "U1") -> SymMPS:
+    """up down up down ... (initial_states.jl:142-146)."""
+    return product_state(["up" if i % 2 == 0 else "down" for i in range(L)], symmetry)
+
+
+def synthetic_bond_legs(L: int, dmax: int, sigma: float = 4.0) -> List[Leg]:
+    """Bond legs of the synthetic U(1) state used for steady-state timing.
+
+    Link b (b = 0..L) holds the charges reachable after b spins that can still return to total
+    charge 0, dimension min(2^min(b, L-b), dmax) shared over the charges with weights
+    exp(-q^2 / (2 sigma^2)), each sector at least 1 and at most its number of basis states.
+    """
+    from math import comb
+
+    legs = []
+    for b in range(L + 1):
+        n = min(b, L - b)
+        total = min(2 ** n if n < 62 else dmax, dmax)
+        qs = [q for q in range(-b, b + 1, 2) if abs(q) <= L - b]
+        # basis-state counts on the smaller side bound each sector
+        def count(q):
+            left = comb(b, (b + q) // 2) if (b + q) % 2 == 0 else 0
+            right = comb(L - b, (L - b - q) // 2) if (L - b - q) % 2 == 0 else 0
+            return min(left, right)
+        w = np.array([np.exp(-q * q / (2 * sigma * sigma)) for q in qs])
+        w = w / w.sum()
+        dims = [max(1, min(count(q), int(round(total * wi)))) for q, wi in zip(qs, w)]
+        # fix the total on the largest sectors
+        order = np.argsort(-w)
+        k = 0
+        while sum(dims) != total and k < 100000:
+            j = order[k % len(order)]
+            if sum(dims) > total and dims[j] > 1:
+                dims[j] -= 1
+            elif sum(dims) < total and dims[j] < count(qs[j]):
+                dims[j] += 1
+            k += 1
+        legs.append(Leg.make(zip(qs, dims)))
+    return legs
+
+
+def random_mps(L: int, dmax: int, seed: int = 0, sigma: float = 4.0,
+               symmetry: str = "U1") -> SymMPS:
+    """Seeded complex-Gaussian U(1) MPS on the synthetic bond legs (not canonicalised)."""
+    rng = np.random.default_rng(seed)
+    charges = SPIN_CHARGES_U1 if symmetry == "U1" else SPIN_CHARGES_NONE
+    if symmetry == "U1":
+        legs = synthetic_bond_legs(L, dmax, sigma)
+    else:
+        legs = [Leg.dense(min(2 ** min(b, L - b), dmax)) for b in range(L + 1)]
+    tensors = []
+    for i in range(L):
+        lay = Layout(legs[i], legs[i + 1], charges)
+        blocks = {}
+        for k, q, qc, d, dc, off in lay.blocks():
+            blocks[(k, q)] = (rng.standard_normal((d, dc)) + 1j * rng.standard_normal((d, dc))) \
+                / np.sqrt(2.0 * max(1, legs[i].total))
+        tensors.append(BlockTensor.from_blocks(lay, blocks))
+    return SymMPS(tensors, charges, center=0)

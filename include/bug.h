@@ -1,0 +1,155 @@
+/*
+ * bug.h -- C ABI of the MI355X (gfx950) backend for the BUG step hot path.
+ *
+ * Every entry point takes plain pointers and sizes; device buffers are raw HIP device pointers
+ * (complex FP64 interleaved as (re, im) pairs, the memory layout of Julia's ComplexF64 and
+ * numpy's complex128) and streams are hipStream_t passed as void*.  Nothing here allocates
+ * caller-visible memory: outputs are caller-allocated, their block layout is described by the
+ * caller, and the library checks every block against the buffer size before it launches.
+ *
+ * Block-sparse U(1) tensors are held as a family of "slices", each a block-diagonal matrix with
+ * a charge shift (struct bug_bsm).  An MPS site tensor A(l, s, r) is the family A_s (s = local
+ * state, shift = charge of s); an MPO environment E(bra, w, ket) is the family E_w (shift =
+ * minus the charge of MPO channel w); a zero-site core is one slice with shift 0; the dense
+ * (no-symmetry) case is the same format with one charge-0 sector per leg.
+ *
+ * Status codes: 0 ok, otherwise one of BUG_E*; bug_last_error() returns the message.
+ * Numerical shortfall of the adaptive step is never an error code: it is reported as a verdict.
+ */
+#ifndef BUG_H
+#define BUG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BUG_OK 0
+#define BUG_EINVAL 1        /* invalid argument; no output was touched */
+#define BUG_ENONFINITE 2    /* non-finite input to a small dense solver */
+#define BUG_EINVARIANT 3    /* block structure inconsistent (charges, dims, offsets) */
+#define BUG_EDEVICE 4       /* a HIP runtime call failed */
+#define BUG_ENOMEM 5        /* device workspace could not be allocated */
+
+/* Arithmetic of the dense block contractions. */
+#define BUG_PREC_FP64 0     /* complex FP64 on FP64 MFMA, FP64 accumulate */
+#define BUG_PREC_FP32 1     /* complex split into real FP32 MFMA products, FP32 accumulate */
+
+/* Operand transforms of a block product term. */
+#define BUG_OP_N 0          /* as stored */
+#define BUG_OP_T 1          /* transpose */
+#define BUG_OP_C 2          /* conjugate transpose */
+#define BUG_OP_R 3          /* complex conjugate, no transpose */
+
+typedef struct bug_ctx bug_ctx;
+
+/* One leg of a block-sparse tensor: sectors sorted by charge, each with its dimension. */
+typedef struct bug_leg {
+    int32_t nsec;
+    const int32_t* charge;   /* [nsec] U(1) charge (2 S^z for spin-1/2); 0 for dense legs */
+    const int32_t* dim;      /* [nsec] sector dimension */
+} bug_leg;
+
+/*
+ * A family of block-diagonal matrices with charge shifts ("block-sparse matrix", bsm).
+ * Slice k has a block at row sector i iff the column leg holds charge row.charge[i] + shift[k];
+ * the block is row.dim[i] x col.dim[j], column-major with leading dimension row.dim[i], at
+ * element offset offset[k * row.nsec + i] of data (-1 = block absent).
+ */
+typedef struct bug_bsm {
+    bug_leg row;
+    bug_leg col;
+    int32_t nslice;
+    const int32_t* shift;    /* [nslice] */
+    const int64_t* offset;   /* [nslice * row.nsec] element offsets, -1 = absent */
+    void* data;              /* device pointer, complex FP64 */
+    int64_t nelem;           /* complex elements addressable from data */
+} bug_bsm;
+
+/*
+ * One site of an MPO as a sparse operator-valued matrix: W[a, b] = sum of coeff |sout><sin|
+ * over the entries with that (a, b).  cl/cr are the channel charges (cumulative charge change),
+ * so every entry satisfies cr[b] = cl[a] + q(sout) - q(sin).
+ */
+typedef struct bug_mpo_site {
+    int32_t wl, wr;
+    const int32_t* cl;       /* [wl] */
+    const int32_t* cr;       /* [wr] */
+    int32_t nent;
+    const int32_t* a;        /* [nent] left channel */
+    const int32_t* b;        /* [nent] right channel */
+    const int32_t* sout;     /* [nent] local output state */
+    const int32_t* sin;      /* [nent] local input state */
+    const double* coeff;     /* [2 * nent] complex coefficients */
+} bug_mpo_site;
+
+/* One term of a generic block product: out[o] += alpha * op_a(P[p]) * op_b(Q[q]). */
+typedef struct bug_term {
+    int32_t o, p, q;
+    int32_t op_a, op_b;
+    double alpha_re, alpha_im;
+} bug_term;
+
+/* ---- context ---------------------------------------------------------------------------- */
+int bug_ctx_create(int device, bug_ctx** out);
+int bug_ctx_destroy(bug_ctx* ctx);
+const char* bug_last_error(const bug_ctx* ctx);
+/* Force a GEMM tile configuration (autotuner override); -1 restores automatic choice. */
+int bug_set_tile_config(bug_ctx* ctx, int config);
+int bug_tile_config_count(void);
+/* Device workspace and plan-cache statistics: [bytes held, plans cached, plan hits, misses]. */
+int bug_ctx_stats(const bug_ctx* ctx, int64_t* out4);
+/* Algorithmic flops (sum of 8 m n k over the block products actually multiplied) of the last
+ * contraction entry-point call on this context. */
+int bug_last_op_flops(const bug_ctx* ctx, double* flops);
+
+/* ---- block contractions ------------------------------------------------------------------- */
+
+/*
+ * Generic grouped block product: for every block of every slice of `out`,
+ *   out = (accumulate ? out : 0) + sum over matching terms of alpha op_a(P) op_b(Q).
+ * One launch covers every block and every term (replaces the sector-pair loop of Telum's
+ * abelian contract, deps/Telum/src/contract.jl:1022-1036).
+ */
+int bug_bsm_product(bug_ctx* ctx, const bug_bsm* out, const bug_bsm* P, const bug_bsm* Q,
+                    int32_t nterm, const bug_term* terms, int32_t accumulate, int32_t precision,
+                    void* stream);
+
+/*
+ * Left environment push  Lout_b = sum_{a,sout,sin} W[a,b](sout,sin) A_sout^H L_a A_sin
+ * (replaces _mpo_left_step / push_left_channels, src/BUG/Core/Environments/mpo_environments.jl:65-90,113).
+ * `bra` may be NULL (bra = A); otherwise it gives the bra tensor (matrix elements between two states).
+ */
+int bug_env_push_left(bug_ctx* ctx, const bug_bsm* L, const bug_bsm* A, const bug_bsm* bra,
+                      const bug_mpo_site* W, const bug_bsm* Lout, int32_t precision, void* stream);
+
+/*
+ * Right environment push  Rout_a = sum_{b,sout,sin} W[a,b](sout,sin) conj(B_sout) R_b B_sin^T
+ * (replaces _mpo_right_step / push_right_channels, mpo_environments.jl:92-108,117).
+ */
+int bug_env_push_right(bug_ctx* ctx, const bug_bsm* R, const bug_bsm* B, const bug_bsm* bra,
+                       const bug_mpo_site* W, const bug_bsm* Rout, int32_t precision, void* stream);
+
+/*
+ * One-site effective Hamiltonian  Y_sout = sum W[a,b](sout,sin) L_a X_sin R_b^T  - shift * X
+ * (replaces apply_one_site + shift_h, src/BUG/Core/Environments/effective_operators.jl:39-64,118-129).
+ */
+int bug_apply_one_site(bug_ctx* ctx, const bug_bsm* L, const bug_mpo_site* W, const bug_bsm* R,
+                       const bug_bsm* X, const bug_bsm* Y, double shift, int32_t precision,
+                       void* stream);
+
+/*
+ * Zero-site (root) generator  Y = sum_w L_w C R_w^T - shift * C, with L, R the environments
+ * already pushed through the root frames
+ * (replaces apply_zero_site, effective_operators.jl:103-107, with zero_site_h :91-94 done by the
+ * two env pushes above).
+ */
+int bug_apply_zero_site(bug_ctx* ctx, const bug_bsm* L, const bug_bsm* R, const bug_bsm* C,
+                        const bug_bsm* Y, double shift, int32_t precision, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BUG_H */
