@@ -1,0 +1,991 @@
+"""ORACLE (test infrastructure): the CBE-BUG step ``cbe_bug_step!`` restated on ``Fam`` tensors.
+
+Follows, stage by stage (0-based sites; bond b joins sites b and b+1; link j sits left of site j):
+
+* options and budget       bug_step_stages.jl:15-142, basis_budget.jl:75-128,454-494,568
+* the two half-sweeps       src/BUG/Core/Sweeps/half_sweeps.jl:218-410
+* bond frames               effective_operators.jl:201-220 (gauge by unique QR, see linalg.py)
+* DD-QR CBE ``:rank_tail``  src/BUG/Randomised/dd_cbe_expand.jl:19-87,252-301,383-419,
+                            dd_cbe.jl:38-89,132-161, cbe.jl:188-236, sketch.jl:141-180,
+                            the streamed sketch mpo_environments.jl:409-481
+* local Krylov, Tests C/D   krylov_enrichment.jl:58-292, krylov_recurrence.jl, krylov_lanczos.jl,
+                            krylov_arnoldi.jl
+* carries, root start       half_sweeps.jl:300-308,400-408, krylov_closure_frames.jl:207-233
+* projection miss           krylov_closure_frames.jl:242-280
+* root generator and expv   effective_operators.jl:91-107, krylov.jl:128-263, root_methods.jl
+* assembly + compression    diagnostics.jl:269-337, truncation.jl:49-158
+
+The probe stream is this package's counter-based generator, not Julia's MersenneTwister, and the
+probe can be injected; the order-p frontier closure (``order > 1``) is not restated here.
+
+Pinned by tests/test_oracle_bug.py against the exact free-fermion magnetisation of the XX chain
+(test/common/free_fermion.jl:25-47), against 2^L exact diagonalisation of the XXZ chain, and
+against the reference's frozen runs (test/fixtures/b155550/xx_u1_bug_order1.json) at the
+tolerances of test_fixtures_b155550.jl:61-64.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import blocks as ob
+from . import krylov as ok
+from . import linalg as la
+from .blocks import Fam
+from .linalg import CBE_EPS
+
+TOL_DEFAULTS = dict(basis=1e-5, trunc=1e-7, expv=1e-10)
+ADAPTIVE_GROWTH = 1.2
+DEFAULT_BOND_CEILING_FACTOR = 1.2
+
+
+# ---- the state --------------------------------------------------------------------------
+
+class MPS:
+    def __init__(self, sites: List[Fam], center: int = 0):
+        self.sites = list(sites)
+        self.center = center
+
+    def __len__(self):
+        return len(self.sites)
+
+    def copy(self):
+        return MPS(list(self.sites), self.center)
+
+    def bond_dims(self):
+        return [sum(self.sites[b].col.values()) for b in range(len(self.sites) - 1)]
+
+
+def canonical(psi: MPS, site: int):
+    """canonical! (canonical_form.jl:43-54) with QR moves."""
+    while psi.center < site:
+        i = psi.center
+        Q, R = la.qr_left(psi.sites[i])
+        psi.sites[i] = Q
+        psi.sites[i + 1] = la.core_times_site(R, psi.sites[i + 1])
+        psi.center += 1
+    while psi.center > site:
+        i = psi.center
+        Lc, Q = la.qr_right(psi.sites[i])
+        psi.sites[i] = Q
+        psi.sites[i - 1] = la.site_times_core(psi.sites[i - 1], Lc)
+        psi.center -= 1
+    return psi
+
+
+def norm_center(psi: MPS) -> float:
+    return psi.sites[psi.center].norm()
+
+
+def sz_profile(psi: MPS, charges=(1, -1)) -> np.ndarray:
+    """<S^z_j>/<psi|psi> for spin-1/2 sites (local_observables.jl:11-31), any gauge.
+
+    Local state 0 is up (S^z = +1/2), state 1 down."""
+    L = len(psi)
+    sz = [0.5, -0.5]
+    q0 = list(psi.sites[0].row)[0]
+    left = [Fam({q0: 1}, {q0: 1}, [0], {(0, q0): np.ones((1, 1), dtype=complex)})]
+    for i in range(L - 1):
+        left.append(_transfer(left[i], psi.sites[i], None))
+    qL = list(psi.sites[-1].col)[0]
+    right = [None] * (L + 1)
+    right[L] = Fam({qL: 1}, {qL: 1}, [0], {(0, qL): np.ones((1, 1), dtype=complex)})
+    for i in range(L - 1, 0, -1):
+        right[i] = _transfer_right(right[i + 1], psi.sites[i])
+    out = np.zeros(L)
+    nrm = None
+    for i in range(L):
+        E = _transfer(left[i], psi.sites[i], sz)
+        N = _transfer(left[i], psi.sites[i], None)
+        val = sum(np.trace(E.blocks[k] @ right[i + 1].blocks[k]) for k in E.blocks
+                  if k in right[i + 1].blocks)
+        n = sum(np.trace(N.blocks[k] @ right[i + 1].blocks[k]) for k in N.blocks
+                if k in right[i + 1].blocks)
+        out[i] = val.real
+        nrm = n.real if nrm is None else nrm
+    return out / nrm
+
+
+def _transfer(E: Fam, A: Fam, weights) -> Fam:
+    """sum_s w_s A_s^dag E A_s (bra = ket = A), E on the left link of A."""
+    out = Fam.zeros(A.col, A.col, [0])
+    for (s, ql), blk in A.blocks.items():
+        e = E.blocks.get((0, ql))
+        if e is None:
+            continue
+        w = 1.0 if weights is None else weights[s]
+        if w == 0:
+            continue
+        qr = ql + A.shifts[s]
+        out.blocks[(0, qr)] += w * (blk.conj().T @ e @ blk)
+    return out
+
+
+def _transfer_right(E: Fam, A: Fam) -> Fam:
+    """sum_s A_s E A_s^dag, E on the right link of A; pairs with the left one by a trace."""
+    out = Fam.zeros(A.row, A.row, [0])
+    for (s, ql), blk in A.blocks.items():
+        qr = ql + A.shifts[s]
+        e = E.blocks.get((0, qr))
+        if e is None:
+            continue
+        out.blocks[(0, ql)] += blk @ e @ blk.conj().T
+    return out
+
+
+# ---- options ------------------------------------------------------------------------------
+
+@dataclass
+class StepOptions:
+    tol_basis: float = TOL_DEFAULTS["basis"]
+    tol_trunc: float = TOL_DEFAULTS["trunc"]
+    tol_expv: float = TOL_DEFAULTS["expv"]
+    tol_krylov: Optional[float] = None
+    order: int = 1
+    maxdim: int = 128
+    hermitian: bool = True
+    work_ceiling_rule: str = "bond"
+    bond_ceiling_factor: Optional[float] = None
+    root: Optional[int] = None          # 0-based bond of the root (reference root - 1)
+    reorth: bool = True
+    truncate: bool = True
+    mode: str = "rank_tail"
+    fixed_width: int = 8
+    tail_factor: float = 0.10
+    krylov_mode: str = "auto"
+    seed: int = 0x5EED
+
+
+@dataclass
+class Alloc:
+    per_bond: float
+    sample: float
+    tail: float
+    krylov: float
+    scale: float
+
+
+@dataclass
+class Ledger:
+    max_sample: float = 0.0
+    max_tail: float = 0.0
+    max_krylov: float = 0.0
+    max_krylov_c: float = 0.0
+    max_krylov_d: float = 0.0
+    discarded: float = 0.0
+    expv_err: float = 0.0
+    expv_iters: int = 0
+    n_probes: int = 0
+    n_selected: int = 0
+    n_krylov: int = 0
+    n_h1: int = 0
+    workcap_reached: bool = False
+    tolerance_met: bool = True
+    maxdim_reached: bool = False
+    proj_err: float = 0.0
+    r_old: int = 0
+    r_aug: int = 0
+    r_final: int = 0
+    unmet: List[str] = field(default_factory=list)
+
+    def unmet_(self, what):
+        self.tolerance_met = False
+        if what not in self.unmet:
+            self.unmet.append(what)
+
+    @property
+    def status(self):
+        return "met" if self.tolerance_met else "budget_unmet"
+
+
+def basis_allocation(tol_basis, nbonds, tau, nrm, nfront, tol_krylov=None) -> Alloc:
+    """:linear rule of basis_allocation (basis_budget.jl:75-104)."""
+    nb, nf = max(nbonds, 1), max(nfront, 1)
+    kr = 1.0 if tol_krylov is None else tol_krylov / tol_basis
+    at = abs(tau)
+    scale = 1.0 if at == 0 else at / max(nrm, np.finfo(float).eps)
+    per_test = tol_basis / (2 * (nb + 1) * nf)
+    return Alloc(2 * per_test, per_test, per_test, kr * per_test, scale)
+
+
+def bond_work_ceilings(ranks, d, maxdim, factor):
+    def ceil_of(r):
+        return math.ceil(round(factor * d * r, 9)) if factor > 0 else 2 * d * r
+    return [max(min(ceil_of(r), maxdim), 1) for r in ranks]
+
+
+# ---- environments -------------------------------------------------------------------------
+
+def left_boundary(mpo, psi: MPS) -> Fam:
+    q0 = list(psi.sites[0].row)[0]
+    E = ob.env_fam({q0: 1}, {q0: 1}, mpo[0].cl)
+    E.blocks[(0, q0)] = np.ones((1, 1), dtype=complex)
+    return E
+
+
+def right_boundary(mpo, psi: MPS) -> Fam:
+    qL = list(psi.sites[-1].col)[0]
+    E = ob.env_fam({qL: 1}, {qL: 1}, mpo[-1].cr)
+    E.blocks[(0, qL)] = np.ones((1, 1), dtype=complex)
+    return E
+
+
+# ---- probes and sketches ------------------------------------------------------------------
+
+def fused_dims(frame: Fam, side: str) -> Dict[int, int]:
+    """Dimensions of the fused local space of a frame per charge (its (link, site) legs)."""
+    out: Dict[int, int] = {}
+    if side == "left":        # (l, s): charge of l + q(s)
+        for ql, d in frame.row.items():
+            for sh in frame.shifts:
+                out[ql + sh] = out.get(ql + sh, 0) + d
+    else:                     # (s, r): charge of r - q(s)
+        for qr, d in frame.col.items():
+            for sh in frame.shifts:
+                out[qr - sh] = out.get(qr - sh, 0) + d
+    return dict(sorted(out.items()))
+
+
+def frame_widths(frame: Fam, side: str) -> Dict[int, int]:
+    return dict(frame.col) if side == "left" else dict(frame.row)
+
+
+def alloc_columns(dims: Dict[int, int], budget: int) -> Dict[int, int]:
+    """_alloc_columns, cbe.jl:188-195."""
+    tot = sum(dims.values())
+    if tot == 0 or budget <= 0:
+        return {q: 0 for q in dims}
+    return {q: min(d, math.ceil(d / tot * budget)) for q, d in dims.items()}
+
+
+def dd_probe(frame: Fam, side: str, width: int, rng: la.ProbeRNG):
+    """_dd_probe (dd_cbe.jl:132-161): sector-graded Gaussian columns in the discarded space.
+
+    ``side`` is the side of ``frame`` (the opposite of the expanding frame).  Stream order: per
+    fused charge ascending, per slice ascending, a (link dim x dfull) block drawn column-major of
+    which the first ``cols[q]`` columns are kept.
+    """
+    dfull = fused_dims(frame, side)
+    dT = frame_widths(frame, side)
+    dC = {q: max(d - dT.get(q, 0), 0) for q, d in dfull.items()}
+    if sum(dC.values()) == 0:
+        return None
+    cols = alloc_columns(dC, width)
+    wl = {q: c for q, c in cols.items() if c > 0}
+    if side == "left":
+        Om = Fam(frame.row, wl, frame.shifts)
+    else:
+        Om = Fam(wl, frame.col, frame.shifts)
+    for q, df in dfull.items():
+        c = cols.get(q, 0)
+        for s, sh in enumerate(frame.shifts):
+            lq = q - sh if side == "left" else q + sh
+            dl = (frame.row if side == "left" else frame.col).get(lq, 0)
+            if dl == 0:
+                continue
+            n_keep = dl * c
+            vals = rng.complex_normal(n_keep)
+            rng.skip(dl * df - n_keep)
+            if c == 0:
+                continue
+            blk = vals.reshape(c, dl).T          # column-major (dl x c)
+            if side == "left":
+                Om.blocks[(s, lq)] = blk
+            else:
+                Om.blocks[(s, q)] = blk.T.copy()  # (c x dl)
+    proj = la.perp_left(frame, Om) if side == "left" else la.perp_right(frame, Om)
+    return proj if proj.norm() > 0 else None
+
+
+def _width_of(T: Fam, side: str) -> int:
+    return sum(T.col.values()) if side == "left" else sum(T.row.values())
+
+
+def stream_left(L: Fam, W1, W2, R: Fam, A: Fam, V: Fam, Om: Fam) -> Fam:
+    """Y = (H Theta) Om^dag for Theta = A V, Om in V's layout (g, s, r) -> (l, s, g)
+    (_stream_left, mpo_environments.jl:409-444)."""
+    d = len(A.shifts)
+    # F[(c, s2)](r, g) = sum_{b, s2'} W2[c,b](s2', s2) R_b^T conj(Om_{s2'})^T
+    slots, terms = {}, []
+    for c, b, so, si, coef in W2.entries:
+        k = slots.setdefault((c, si), len(slots))
+        terms.append((k, b, so, la.OP_T, la.OP_C, coef))
+    sh = {}
+    for c, b, so, si, coef in W2.entries:
+        k = slots[(c, si)]
+        sh[k] = -R.shifts[b] - Om.shifts[so]
+    F = la.product(R.col, Om.row, [sh[k] for k in range(len(slots))], R, Om, terms)
+    # B_c(m, g) = sum_s2 V_{s2}(m, r) F[(c, s2)](r, g)
+    cs = sorted({c for (c, _) in slots})
+    cidx = {c: i for i, c in enumerate(cs)}
+    tB, shB = [], {}
+    for (c, si), k in slots.items():
+        tB.append((cidx[c], si, k, la.OP_N, la.OP_N, 1.0))
+        shB[cidx[c]] = V.shifts[si] + sh[k]
+    B = la.product(V.row, Om.row, [shB[i] for i in range(len(cs))], V, F, tB)
+    # C[(c, s1)](l, g) = A_{s1}(l, m) B_c(m, g)
+    cslots, tC, shC = {}, [], {}
+    for a, c, so, si, coef in W1.entries:
+        if c not in cidx:
+            continue
+        k = cslots.setdefault((c, si), len(cslots))
+        if k == len(tC):
+            tC.append((k, si, cidx[c], la.OP_N, la.OP_N, 1.0))
+            shC[k] = A.shifts[si] + shB[cidx[c]]
+    C = la.product(A.row, Om.row, [shC[k] for k in range(len(cslots))], A, B, tC)
+    # Y_{s1'}(l', g) = sum W1[a,c](s1', s1) L_a(l', l) C[(c, s1)](l, g)
+    tY = []
+    for a, c, so, si, coef in W1.entries:
+        if (c, si) in cslots:
+            tY.append((so, a, cslots[(c, si)], la.OP_N, la.OP_N, coef))
+    return la.product(L.row, Om.row, A.shifts, L, C, tY)
+
+
+def stream_right(L: Fam, W1, W2, R: Fam, U: Fam, Bv: Fam, Om: Fam) -> Fam:
+    """Y = Om^dag (H Theta) for Theta = U Bv, Om in U's layout (l, s, g) -> (g, s, r)
+    (_stream_right, mpo_environments.jl:446-481)."""
+    # G/F[(c, s1)](l, g) = sum_{a, s1'} W1[a,c](s1', s1) L_a^T conj(Om_{s1'})
+    slots, terms, sh = {}, [], {}
+    for a, c, so, si, coef in W1.entries:
+        k = slots.setdefault((c, si), len(slots))
+        terms.append((k, a, so, la.OP_T, la.OP_R, coef))
+        sh[k] = -L.shifts[a] + Om.shifts[so]
+    F = la.product(L.col, Om.col, [sh[k] for k in range(len(slots))], L, Om, terms)
+    # B_c(m, g) = sum_s1 U_{s1}^T(m, l) F[(c, s1)](l, g)
+    cs = sorted({c for (c, _) in slots})
+    cidx = {c: i for i, c in enumerate(cs)}
+    tB, shB = [], {}
+    for (c, si), k in slots.items():
+        tB.append((cidx[c], si, k, la.OP_T, la.OP_N, 1.0))
+        shB[cidx[c]] = -U.shifts[si] + sh[k]
+    B = la.product(U.col, Om.col, [shB[i] for i in range(len(cs))], U, F, tB)
+    # C[(c, s2)](g, r) = B_c^T(g, m) Bv_{s2}(m, r)
+    cslots, tC, shC = {}, [], {}
+    for c, b, so, si, coef in W2.entries:
+        if c not in cidx:
+            continue
+        k = cslots.setdefault((c, si), len(cslots))
+        if k == len(tC):
+            tC.append((k, cidx[c], si, la.OP_T, la.OP_N, 1.0))
+            shC[k] = -shB[cidx[c]] + Bv.shifts[si]
+    C = la.product(Om.col, Bv.col, [shC[k] for k in range(len(cslots))], B, Bv, tC)
+    # Y_{s2'}(g, r') = sum W2[c,b](s2', s2) C[(c, s2)](g, r) R_b^T(r, r')
+    tY = []
+    for c, b, so, si, coef in W2.entries:
+        if (c, si) in cslots:
+            tY.append((so, cslots[(c, si)], b, la.OP_N, la.OP_T, coef))
+    return la.product(Om.col, R.row, Bv.shifts, C, R, tY)
+
+
+# ---- frames -------------------------------------------------------------------------------
+
+@dataclass
+class BondFrame:
+    U0: Fam
+    S0: Fam
+    V0: Fam
+    old_rank: int
+
+
+def frame_from(left: Fam, right: Fam) -> BondFrame:
+    """_frame_from (effective_operators.jl:201-220)."""
+    U0, Rl = la.qr_left(left)
+    Lr, V0 = la.qr_right(right)
+    S0 = la.product(Rl.row, Lr.col, [0], Rl, Lr, [(0, 0, 0, la.OP_N, la.OP_N, 1.0)])
+    return BondFrame(U0, S0, V0, sum(U0.col.values()))
+
+
+def reortho(frame: Fam, block: Optional[Fam], side: str) -> Optional[Fam]:
+    """_reortho_left/_reortho_right (truncation.jl:374-394)."""
+    if block is None:
+        return None
+    perp = la.perp_left if side == "left" else la.perp_right
+    outside = perp(frame, block)
+    if outside.norm() <= CBE_EPS * max(block.norm(), 1.0):
+        return None
+    if side == "left":
+        r = la.svd_left(outside, cutoff=1e-14)
+        return r.U if r.U.blocks else None
+    r = la.svd_right(outside, cutoff=1e-14)
+    return r.Vh if r.Vh.blocks else None
+
+
+def cap_to_local_room(frame: Fam, cand: Optional[Fam], side: str) -> Optional[Fam]:
+    """_cap_to_local_room (truncation.jl:332-351)."""
+    if cand is None:
+        return None
+    dfull = fused_dims(frame, side)
+    dT = frame_widths(frame, side)
+    dU = frame_widths(cand, side)
+    counts, over = {}, False
+    for q, du in dU.items():
+        if du == 0:
+            continue
+        room = dfull.get(q, 0) - dT.get(q, 0)
+        counts[q] = min(du, room)
+        over |= counts[q] < du
+    if not over:
+        return cand
+    return trim_per_sector(cand, side, counts)
+
+
+def trim_per_sector(T: Fam, side: str, counts: Dict[int, int]) -> Optional[Fam]:
+    if not any(c > 0 for c in counts.values()):
+        return None
+    if side == "left":
+        col = {q: c for q, c in counts.items() if c > 0}
+        out = Fam(T.row, col, T.shifts)
+        for (s, ql), b in T.blocks.items():
+            qc = ql + T.shifts[s]
+            if col.get(qc, 0) > 0:
+                out.blocks[(s, ql)] = b[:, :col[qc]]
+        return out
+    row = {q: c for q, c in counts.items() if c > 0}
+    out = Fam(row, T.col, T.shifts)
+    for (s, ql), b in T.blocks.items():
+        if row.get(ql, 0) > 0:
+            out.blocks[(s, ql)] = b[:row[ql], :]
+    return out
+
+
+def trim_total(T: Fam, side: str, n: int) -> Optional[Fam]:
+    """_trim_total (cbe.jl:219-236): the first sectors in (ascending charge) order."""
+    if n <= 0:
+        return None
+    widths = frame_widths(T, side)
+    if sum(widths.values()) <= n:
+        return T
+    left = n
+    keep = {}
+    for q in sorted(widths):
+        keep[q] = min(widths[q], left)
+        left -= keep[q]
+    return trim_per_sector(T, side, keep)
+
+
+def oplus(frame: Fam, new: Fam, side: str) -> Fam:
+    return la.oplus_left(frame, new) if side == "left" else la.oplus_right(frame, new)
+
+
+def sample_residual(basis, project, Y: Fam, Om: Fam, side: str) -> float:
+    """_sample_residual (sketch.jl:141-180) with the sketch already in hand."""
+    cols = dict(Om.row) if side == "left" else dict(Om.col)
+    cols = {q: n for q, n in cols.items() if n > 0}
+    if not cols:
+        return 0.0
+    residual = project(Y)
+    if residual.norm() <= CBE_EPS:
+        return 0.0
+    if basis is not None:
+        off = la.perp_left(basis, residual) if side == "left" else la.perp_right(basis, residual)
+        residual = project(off)
+    per = {}
+    for (s, q), b in residual.blocks.items():
+        w = q + residual.shifts[s] if side == "left" else q
+        per[w] = per.get(w, 0.0) + float(np.vdot(b, b).real)
+    tot = 0.0
+    for w, v in per.items():
+        n = cols.get(w, 0)
+        if n <= 0:
+            raise RuntimeError("Test B: residual sector without holdout column")
+        tot += v / n
+    return math.sqrt(tot)
+
+
+# ---- one side of a bond expansion (dd_cbe_expand, :rank_tail) ------------------------------
+
+@dataclass
+class Expansion:
+    U_ex: Fam
+    V_ex: Fam
+    n_new: int
+
+
+def dd_cbe_expand(fr: BondFrame, mpo, bond: int, lenv: Fam, renv: Fam, side: str, alloc: Alloc,
+                  ledger: Ledger, workcap: int, opts: StepOptions, rng, probe_hook=None) -> Expansion:
+    left = side == "left"
+    frame, opp = (fr.U0, fr.V0) if left else (fr.V0, fr.U0)
+    oside = "right" if left else "left"
+    old_rank = fr.old_rank
+    none = Expansion(fr.U0, fr.V0, 0)
+    full_self = sum(fused_dims(frame, side).values())
+    full_opp = sum(fused_dims(opp, oside).values())
+    cap = full_self - old_rank
+    wbind = False
+    if workcap > 0:
+        wk = max(workcap - old_rank, 0)
+        wbind = wk < cap
+        cap = min(cap, wk)
+    if cap <= 0:
+        return none
+    W1, W2 = mpo[bond], mpo[bond + 1]
+    width = min(opts.fixed_width, full_opp)
+    Om = probe_hook(opp, oside, width) if probe_hook is not None else dd_probe(opp, oside, width, rng)
+    if Om is None:
+        return none
+    ledger.n_probes += sum(Om.row.values()) if left else sum(Om.col.values())
+    if left:
+        A = la.site_times_core(fr.U0, fr.S0)
+        Ysk = stream_left(lenv, W1, W2, renv, A, fr.V0, Om)
+        project = lambda T: la.perp_left(fr.U0, T)
+    else:
+        Bv = la.core_times_site(fr.S0, fr.V0)
+        Ysk = stream_right(lenv, W1, W2, renv, fr.U0, Bv, Om)
+        project = lambda T: la.perp_right(fr.V0, T)
+    Y = project(Ysk)
+    ny = Y.norm()
+    if ny <= CBE_EPS:
+        return none
+    width_om = sum(Om.row.values()) if left else sum(Om.col.values())
+    rmax = min(cap, width_om)
+    cache = {}
+
+    def candidate(k):
+        if k <= 0:
+            return None
+        if k in cache:
+            return cache[k]
+        if left:
+            r = la.svd_left(Y, cutoff=CBE_EPS, nkeep=k)
+            B = None if not r.kept else reortho(frame, r.U, side)
+        else:
+            r = la.svd_right(Y, cutoff=CBE_EPS, nkeep=k)
+            B = None if not r.kept else reortho(frame, r.Vh, side)
+        cache[k] = B
+        return B
+
+    limit = opts.tail_factor * alloc.sample
+
+    def metric(k):
+        return alloc.scale * sample_residual(candidate(k), project, Y, Om, side)
+
+    if metric(0) <= limit:
+        return none
+    hi = rmax
+    if metric(hi) > limit:
+        directions = candidate(hi)
+        count = hi
+    else:
+        lo = 0
+        while hi - lo > 1:
+            k = (hi + lo) // 2
+            if metric(k) <= limit:
+                hi = k
+            else:
+                lo = k
+        directions = candidate(hi)
+        count = 0 if directions is None else _width_of(directions, side)
+    if directions is None:
+        return none
+    if count > cap:
+        directions = trim_total(directions, side, cap)
+        if directions is None:
+            return none
+    if wbind and count >= cap:
+        ledger.workcap_reached = True
+    directions = cap_to_local_room(frame, directions, side)
+    if directions is None:
+        return none
+    nq = _width_of(directions, side)
+    ledger.n_selected += nq
+    grown = oplus(frame, directions, side)
+    return Expansion(grown, fr.V0, nq) if left else Expansion(fr.U0, grown, nq)
+
+
+# ---- local Krylov enrichment (Tests C/D) -------------------------------------------------
+
+def extend_frame(frame: Fam, residual: Fam, direction: Fam, side: str, weight: float,
+                 allowance: float, room: int):
+    """_extend_frame (krylov_enrichment.jl:135-171) -> (frame, n_added, fits)."""
+    rn = residual.norm()
+    cut = CBE_EPS * max(direction.norm(), rn) / rn
+    svd = la.svd_left if side == "left" else la.svd_right
+    res = svd(residual, cutoff=cut)
+    nk = len(res.kept)
+    keep, tail = ok.tail_rank(res.kept, weight, allowance, min(nk, room), base=la.sq_weight(res.trunc))
+    fits = tail <= allowance
+    if keep == 0:
+        return frame, 0, fits
+    if keep < nk:
+        res = svd(residual, cutoff=cut, nkeep=keep)
+    newb = reortho(frame, res.U if side == "left" else res.Vh, side)
+    newb = cap_to_local_room(frame, newb, side)
+    if newb is None:
+        return frame, 0, fits
+    return oplus(frame, newb, side), _width_of(newb, side), fits
+
+
+@dataclass
+class KrylovStats:
+    eta_c: float = 0.0
+    eta_d: float = 0.0
+    added: int = 0
+    n_h1: int = 0
+    capped: bool = False
+    met: bool = True
+
+
+def krylov_enrich(seed: Fam, applyH, start_frame: Fam, side: str, tau: complex, alloc: Alloc,
+                  workcap: int, recurrence: str = "lanczos", maxdepth: int = ok.KRY_MAXDEPTH):
+    """_krylov_enrich + _krylov_basis + the chains (krylov_enrichment.jl:213-292, ...)."""
+    perp = la.perp_left if side == "left" else la.perp_right
+    allow = alloc.krylov
+    st = KrylovStats()
+    state = {"frame": start_frame}
+
+    def score(direction, weight, first):
+        residual = perp(state["frame"], direction)
+        eta = weight * residual.norm()
+        if first:
+            st.eta_c = eta
+        else:
+            st.eta_d = max(st.eta_d, eta)
+        if not eta > allow:
+            return True
+        room = workcap - _width_of(state["frame"], side) if workcap > 0 else 1 << 60
+        if room <= 0:
+            return False
+        state["frame"], nadd, fits = extend_frame(state["frame"], residual, direction, side,
+                                                  weight, allow, room)
+        st.added += nadd
+        return fits
+
+    snorm = seed.norm()
+    if snorm == 0:
+        return state["frame"], st
+    teff = tau if tau != 0 else 1.0 + 0j
+    v0 = seed.scale(1.0 / snorm)
+    closed, capped = False, False
+    Hm = np.zeros((maxdepth + 1, maxdepth + 1), dtype=complex)
+    vs = [v0]
+    if recurrence == "lanczos":
+        prev_beta = 0.0
+        for k in range(1, maxdepth + 1):
+            img = applyH(vs[k - 1])
+            st.n_h1 += 1
+            if k > 1:
+                img = img.axpy(-prev_beta, vs[k - 2])
+            alpha = ob.inner(vs[k - 1], img)
+            if not abs(alpha.imag) <= 1e-9 * max(abs(alpha), 1.0):
+                raise ValueError("Lanczos needs a Hermitian one-site generator")
+            Hm[k - 1, k - 1] = alpha.real
+            img = img.axpy(-alpha.real, vs[k - 1])
+            if k >= 20 and k % 2 == 0:
+                coeffs = [ob.inner(v, img) for v in vs]
+                for c, v in zip(coeffs, vs):
+                    img = img.axpy(-c, v)
+            beta = img.norm()
+            if beta <= ok.KRY_BREAKDOWN:
+                closed = True
+                break
+            Hm[k, k - 1] = beta
+            Hm[k - 1, k] = beta
+            nv = img.scale(1.0 / beta)
+            vs.append(nv)
+            w = ok.kry_weight(Hm, k, teff)
+            if not score(nv, w, k == 1):
+                capped = True
+                break
+            if w <= allow:
+                closed = True
+                break
+            prev_beta = beta
+    else:
+        for k in range(1, maxdepth + 1):
+            img = applyH(vs[-1])
+            st.n_h1 += 1
+            for _p in range(2):
+                coeffs = [ob.inner(v, img) for v in vs]
+                for row, c in enumerate(coeffs):
+                    Hm[row, k - 1] += c
+                for c, v in zip(coeffs, vs):
+                    img = img.axpy(-c, v)
+            rn = img.norm()
+            if rn <= ok.KRY_BREAKDOWN:
+                closed = True
+                break
+            Hm[k, k - 1] = rn
+            nv = img.scale(1.0 / rn)
+            vs.append(nv)
+            w = ok.kry_weight(Hm, k, teff)
+            if not score(nv, w, k == 1):
+                capped = True
+                break
+            if w <= allow:
+                closed = True
+                break
+    if not closed:
+        capped = True
+    st.capped = capped
+    st.met = closed and not capped
+    return state["frame"], st
+
+
+def record_krylov(ledger: Ledger, st: KrylovStats, alloc: Alloc):
+    ledger.max_krylov_c = max(ledger.max_krylov_c, st.eta_c)
+    ledger.max_krylov_d = max(ledger.max_krylov_d, st.eta_d)
+    ledger.max_krylov = max(ledger.max_krylov, st.eta_c, st.eta_d)
+    ledger.n_krylov += st.added
+    ledger.n_h1 += st.n_h1
+    if st.capped:
+        ledger.workcap_reached = True
+    if not st.met:
+        ledger.unmet_("krylov")
+
+
+# ---- the step -----------------------------------------------------------------------------
+
+@dataclass
+class StepInfo:
+    ledger: Ledger
+    expanded: List[int]
+    root_rank: int
+    discarded: float
+    n_root: int
+
+
+def _cap_at(wcap, b):
+    return wcap if isinstance(wcap, int) else wcap[b]
+
+
+def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None,
+                 probe_hook=None) -> StepInfo:
+    """One order-1 CBE-BUG step in place (bug_step.jl:17-510)."""
+    opts = opts or StepOptions()
+    L = len(psi)
+    if len(mpo) != L:
+        raise ValueError("MPO and state lengths differ")
+    if L < 2:
+        raise ValueError("cbe_bug_step needs at least two sites")
+    if opts.order != 1:
+        raise NotImplementedError("the order-p frontier closure is not restated in the oracle")
+    rl = (max(1, min(L - 1, L // 2)) - 1) if opts.root is None else opts.root
+    d = len(psi.sites[0].shifts)
+    tau = complex(tau)
+    rng = la.ProbeRNG(opts.seed)
+    factor = opts.bond_ceiling_factor
+    if factor is None:
+        factor = DEFAULT_BOND_CEILING_FACTOR if opts.work_ceiling_rule == "bond" else 0.0
+    bd0 = psi.bond_dims()
+    if opts.work_ceiling_rule == "bond":
+        wcap = bond_work_ceilings(bd0, d, opts.maxdim, factor)
+    elif opts.work_ceiling_rule == "maxdim":
+        wcap = max(d * opts.maxdim, d * max(bd0 + [0]), 1)
+    else:
+        raise ValueError(f"unknown work_ceiling_rule {opts.work_ceiling_rule}")
+    nrm = psi.sites[psi.center].norm() if 0 <= psi.center < L else 1.0
+    alloc = basis_allocation(opts.tol_basis, L - 1, tau, nrm, opts.order, opts.tol_krylov)
+    ledger = Ledger(r_old=max(bd0 + [0]))
+    recurrence = "lanczos" if (opts.krylov_mode == "auto" and opts.hermitian) else (
+        "arnoldi" if opts.krylov_mode == "auto" else opts.krylov_mode)
+    expanded = [0] * (L - 1)
+    left_frames: List[Optional[Fam]] = [None] * (rl + 1)
+    right_frames: List[Optional[Fam]] = [None] * (L - 1)
+
+    # ---- left half-sweep (half_sweeps.jl:218-310) ----
+    psiL = psi.copy()
+    canonical(psiL, 0)
+    renvs = [None] * (L + 1)
+    renvs[L] = right_boundary(mpo, psiL)
+    for i in range(L - 1, 1, -1):
+        renvs[i] = ob.push_right(renvs[i + 1], psiL.sites[i], mpo[i])
+    lch = left_boundary(mpo, psiL)
+    lroot = lch
+    carry = None
+    for b in range(rl + 1):
+        carried = psiL.sites[b] if carry is None else la.core_times_site(carry, psiL.sites[b])
+        fr = frame_from(carried, psiL.sites[b + 1])
+        ex = dd_cbe_expand(fr, mpo, b, lch, renvs[b + 2], "left", alloc, ledger,
+                           _cap_at(wcap, b), opts, rng, probe_hook)
+        right_projected = la.overlap_right(psiL.sites[b + 1], ex.V_ex)
+        Kex = la.site_times_core(carried, right_projected)
+        rch1 = ob.push_right(renvs[b + 2], ex.V_ex, mpo[b + 1])
+        Hl, Hw, Hr = lch, mpo[b], rch1
+        Wk, kst = krylov_enrich(Kex, lambda v: ob.apply_one_site(Hl, Hw, Hr, v), ex.U_ex, "left",
+                                tau, alloc, _cap_at(wcap, b), recurrence)
+        record_krylov(ledger, kst, alloc)
+        left_frames[b] = Wk
+        expanded[b] = sum(Wk.col.values())
+        carry = la.overlap_left(Wk, carried)
+        if b == rl:
+            lroot = lch
+        else:
+            lch = ob.push_left(lch, Wk, mpo[b])
+
+    # ---- right half-sweep (half_sweeps.jl:323-410) ----
+    psiR = psi
+    canonical(psiR, L - 1)
+    lenvs = [left_boundary(mpo, psiR)]
+    for i in range(L - 2):
+        lenvs.append(ob.push_left(lenvs[-1], psiR.sites[i], mpo[i]))
+    rch = right_boundary(mpo, psiR)
+    rroot = rch
+    carry = None
+    for b in range(L - 2, rl - 1, -1):
+        carried = psiR.sites[b + 1] if carry is None else la.site_times_core(psiR.sites[b + 1], carry)
+        fr = frame_from(psiR.sites[b], carried)
+        ex = dd_cbe_expand(fr, mpo, b, lenvs[b], rch, "right", alloc, ledger,
+                           _cap_at(wcap, b), opts, rng, probe_hook)
+        left_projected = la.overlap_left(ex.U_ex, psiR.sites[b])
+        Bex = la.core_times_site(left_projected, carried)
+        lch1 = ob.push_left(lenvs[b], ex.U_ex, mpo[b])
+        Hl, Hw, Hr = lch1, mpo[b + 1], rch
+        Zk, kst = krylov_enrich(Bex, lambda v: ob.apply_one_site(Hl, Hw, Hr, v), ex.V_ex, "right",
+                                tau, alloc, _cap_at(wcap, b), recurrence)
+        record_krylov(ledger, kst, alloc)
+        right_frames[b] = Zk
+        if b != rl:
+            expanded[b] = sum(Zk.row.values())
+        carry = la.overlap_right(carried, Zk)
+        if b == rl:
+            rroot = rch
+        else:
+            rch = ob.push_right(rch, Zk, mpo[b + 1])
+    AR = carry
+
+    # ---- root start, projection miss (bug_step.jl:278-327) ----
+    AL = None
+    for i in range(rl + 1):
+        c = psi.sites[i] if AL is None else la.core_times_site(AL, psi.sites[i])
+        AL = la.overlap_left(left_frames[i], c)
+    root_start = la.product(AL.row, AR.col, [0], AL, AR, [(0, 0, 0, la.OP_N, la.OP_N, 1.0)])
+    npsi = psi.sites[psi.center].norm()
+    ledger.r_aug = max(max(expanded + [0]), sum(right_frames[rl].row.values()))
+    ledger.proj_err = projection_miss(psi, left_frames, right_frames, rl) / npsi if npsi > 0 else 0.0
+    if ledger.proj_err > opts.tol_basis:
+        ledger.unmet_("projection")
+
+    # ---- root generator and solve (bug_step.jl:366-382) ----
+    Lr = ob.push_left(lroot, left_frames[rl], mpo[rl])
+    Rr = ob.push_right(rroot, right_frames[rl], mpo[rl + 1])
+    napply = [0]
+
+    def apply_root(C):
+        napply[0] += 1
+        return ob.apply_zero_site(Lr, Rr, C)
+
+    result, rep = ok.lanczos_expv(
+        apply_root, tau, root_start, inner=ob.inner, norm=lambda x: x.norm(),
+        axpy=lambda a, x, y: y.axpy(a, x), scale=lambda a, x: x.scale(a),
+        tol=1e-15, maxiter=ok.ROOT_MAXITER, reorth=opts.reorth, conv_tol=opts.tol_expv,
+        substeps=0)
+    ledger.expv_err = rep.err
+    ledger.expv_iters = rep.iterations
+    if rep.err > opts.tol_expv * (1 + 1e-8):
+        ledger.unmet_("expv")
+
+    # ---- assembly and compression (diagnostics.jl:269-337) ----
+    res = la.svd_core(result, cutoff=0.0)
+    expanded[rl] = sum(left_frames[rl].col.values())
+    discarded = la.trunc_weight(res)
+    for i in range(rl):
+        psi.sites[i] = left_frames[i]
+    psi.sites[rl] = la.site_times_core(left_frames[rl], res.U)
+    SV = la.diag_times(res.S, res.Vh, "left")
+    psi.sites[rl + 1] = la.core_times_site(SV, right_frames[rl])
+    for b in range(rl + 1, L - 1):
+        psi.sites[b + 1] = right_frames[b]
+    psi.center = rl + 1
+    root_rank = sum(psi.sites[rl].col.values())
+    if opts.truncate:
+        wtr, cumw, capped = truncate_recursive(psi, rl + 1, opts.maxdim, opts.tol_trunc)
+        discarded = max(discarded, wtr)
+        ledger.discarded = cumw
+        ledger.maxdim_reached = capped
+        if cumw > opts.tol_trunc * (1 + 1e-8):
+            ledger.unmet_("trunc")
+    ledger.r_final = max(psi.bond_dims() + [0])
+    return StepInfo(ledger, expanded, root_rank, discarded, napply[0])
+
+
+def projection_miss(psi: MPS, left_frames, right_frames, rl: int) -> float:
+    """_projection_miss (krylov_closure_frames.jl:242-280); square-root factors by QR."""
+    L = len(psi)
+    rf = [None] * (L + 1)
+    for i in range(L - 1, 0, -1):
+        carried = psi.sites[L - 1] if i == L - 1 else la.site_times_core(psi.sites[i], rf[i + 1])
+        Lc, _ = la.qr_right(carried)
+        rf[i] = Lc
+    miss2 = 0.0
+    AL = None
+    for i in range(rl + 1):
+        carried = psi.sites[i] if AL is None else la.core_times_site(AL, psi.sites[i])
+        outside = la.perp_left(left_frames[i], carried)
+        miss2 += la.site_times_core(outside, rf[i + 1]).norm() ** 2
+        AL = la.overlap_left(left_frames[i], carried)
+    lf = [None] * (L - 1)
+    _, R = qr_core(AL)
+    lf[rl] = R
+    for b in range(rl + 1, L - 1):
+        _, R = la.qr_left(la.core_times_site(lf[b - 1], psi.sites[b]))
+        lf[b] = R
+    AR = None
+    for b in range(L - 2, rl - 1, -1):
+        carried = psi.sites[b + 1] if AR is None else la.site_times_core(psi.sites[b + 1], AR)
+        outside = la.perp_right(right_frames[b], carried)
+        miss2 += la.core_times_site(lf[b], outside).norm() ** 2
+        AR = la.overlap_right(carried, right_frames[b])
+    return math.sqrt(miss2)
+
+
+def qr_core(C: Fam):
+    """C = Q R per sector of a one-slice core (unique QR)."""
+    qb, rb, bond = {}, {}, {}
+    for (k, q), M in C.blocks.items():
+        if not np.any(M):
+            continue
+        Q, R = la._unique_qr(M)
+        bond[q] = Q.shape[1]
+        qb[(0, q)] = Q
+        rb[(0, q)] = R
+    return Fam(C.row, bond, [0], qb), Fam(bond, C.col, [0], rb)
+
+
+def truncate_recursive(psi: MPS, root: int, maxdim: int, weight_tol: float):
+    """truncate_recursive! with the budgeted rule (truncation.jl:59-158); root = 0-based site."""
+    L = len(psi)
+    sdiv = math.sqrt(root) + math.sqrt(L - 1 - root)
+    state = {"nrm2": -1.0, "worst": 0.0, "capped": False}
+    cum = [0.0, 0.0]
+
+    def cut(T, left_split):
+        svd = la.svd_left if left_split else la.svd_right
+        res = svd(T, cutoff=0.0, nkeep=maxdim)
+        tot = la.sq_weight(res.kept) + la.sq_weight(res.trunc)
+        if state["nrm2"] < 0:
+            state["nrm2"] = tot
+        share = weight_tol * math.sqrt(max(state["nrm2"], 0.0)) / sdiv if sdiv > 0 else 0.0
+        nk = len(res.kept)
+        keep, tail = ok.tail_rank(res.kept, 1.0, share, nk, base=la.sq_weight(res.trunc))
+        if tail > share and res.trunc:
+            state["capped"] = True
+        keep = max(keep, 1)
+        if keep < nk:
+            res = svd(T, cutoff=0.0, nkeep=keep)
+        return res
+
+    def account(res, side):
+        state["worst"] = max(state["worst"], la.trunc_weight(res))
+        cum[side] += la.sq_weight(res.trunc)
+
+    for site in range(root, 0, -1):
+        res = cut(psi.sites[site], False)
+        account(res, 0)
+        psi.sites[site] = res.Vh
+        US = la.diag_times(res.S, res.U, "right")
+        psi.sites[site - 1] = la.site_times_core(psi.sites[site - 1], US)
+        psi.center = site - 1
+    canonical(psi, root)
+    for site in range(root, L - 1):
+        res = cut(psi.sites[site], True)
+        account(res, 1)
+        psi.sites[site] = res.U
+        SV = la.diag_times(res.S, res.Vh, "left")
+        psi.sites[site + 1] = la.core_times_site(SV, psi.sites[site + 1])
+        psi.center = site + 1
+    nrm2 = state["nrm2"]
+    cumulative = (math.sqrt(cum[0]) + math.sqrt(cum[1])) / math.sqrt(nrm2) if nrm2 > 0 else 0.0
+    return state["worst"], cumulative, state["capped"]

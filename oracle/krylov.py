@@ -1,0 +1,231 @@
+"""ORACLE (test infrastructure): the small dense pieces of the Krylov solvers (host FP64).
+
+* ``residual_integral``/``kry_weight``  -- krylov_enrichment.jl:58-120
+* ``tail_rank``                          -- truncation.jl:262-278
+* ``tridiag_eigen``, ``coeffs_from_eigen``, ``shift`` -- src/BUG/Core/LocalSolvers/krylov.jl:44-117
+* ``tail_sup``, ``pick_tau``             -- src/BUG/Core/LocalSolvers/root_methods.jl:86-130
+* ``lanczos_expv``                       -- krylov.jl:128-263 (``:exp`` root, ``reorth``, adaptive
+                                            substeps), on any vector type through callbacks
+
+Pinned by tests/test_oracle_krylov.py: expv against ``scipy.linalg.expm`` on random Hermitian
+matrices, weights against a dense exponential, the 2x2 cancellation case of
+test/BUG/Core/LocalSolvers/test_krylov_weights.jl:42-59, the tail-rank rule on hand spectra.
+"""
+from __future__ import annotations
+
+import math
+from typing import Callable, List
+
+import numpy as np
+import scipy.linalg as sla
+
+KRY_BREAKDOWN = 1e-13          # _KRY_BREAKDOWN, krylov_enrichment.jl:13
+KRY_MAXDEPTH = 64              # _KRY_MAXDEPTH, :24
+ROOT_MAXITER = 30              # _ROOT_MAXITER, :28
+KRY_SKEW_TOL = 1e-12           # _KRY_SKEW_TOL, :33
+RESIDUAL_MAX_PANELS = 4096     # :36
+TAIL_NODES = 8                 # _TAIL_NODES, root_methods.jl:83
+EXPV_SHIFT_TRIGGER = 30.0      # krylov.jl:26
+EXPV_MAX_NORM = 1e150
+
+
+def residual_integral(defect_norm: Callable, M: np.ndarray, tau: complex,
+                      state_seed: bool = True) -> float:
+    nM = np.linalg.norm(M)
+    unitary = np.linalg.norm(M + M.conj().T) <= KRY_SKEW_TOL * nM
+    sigma = 0.0 if unitary else float(np.max(np.linalg.eigvals(M).real))
+    shifted = M - sigma * np.eye(M.shape[0])
+    intervals = 2 * min(max(math.ceil(2 * np.abs(shifted).sum(axis=0).max()), 8),
+                        RESIDUAL_MAX_PANELS)
+    h = 1.0 / intervals
+    prop = sla.expm(h * shifted)
+    col = np.zeros(M.shape[0], dtype=complex)
+    col[0] = 1.0
+    logs = np.empty(intervals + 1)
+    for node in range(intervals + 1):
+        if node > 0:
+            col = prop @ col
+        w = 1.0 if node in (0, intervals) else (4.0 if node % 2 == 1 else 2.0)
+        mag = defect_norm(col)
+        logs[node] = -np.inf if mag == 0 else math.log(w * h / 3 * mag) + node * h * sigma
+    peak = logs.max()
+    if peak == -np.inf:
+        return 0.0
+    lw = math.log(abs(tau)) + peak + math.log(np.exp(logs - peak).sum())
+    if unitary or not state_seed:
+        return math.exp(lw)
+    return math.exp(lw - sigma - math.log(max(np.linalg.norm(col), np.finfo(float).tiny)))
+
+
+def kry_weight(Hm: np.ndarray, k: int, tau: complex, state_seed: bool = True) -> float:
+    """Weight of v_{k+1} (k = 1-based step), Hm[k, k-1] (0-based) = beta_k."""
+    beta = abs(Hm[k, k - 1])
+    if beta == 0:
+        return 0.0
+    return beta * residual_integral(lambda c: abs(c[k - 1]), tau * Hm[:k, :k], tau, state_seed)
+
+
+def tail_rank(kept, scale: float, allowance: float, kmax: int, base: float = 0.0):
+    """_tail_rank: smallest keep <= kmax whose scaled tail fits; (keep, tail)."""
+    n = len(kept)
+    kmax = min(kmax, n)
+    w = [e[0] ** 2 for e in kept]
+    if kmax <= 0:
+        return 0, scale * math.sqrt(base + sum(w))
+    tail = base + sum(w[kmax:])
+    keep = kmax
+    while keep > 0:
+        trial = tail + w[keep - 1]
+        if not scale * math.sqrt(trial) <= allowance:
+            break
+        tail = trial
+        keep -= 1
+    return keep, scale * math.sqrt(tail)
+
+
+def tridiag_eigen(alpha: List[float], beta: List[float]):
+    a = np.asarray(alpha, dtype=float)
+    b = np.asarray(beta, dtype=float)
+    if not (np.all(np.isfinite(a)) and np.all(np.isfinite(b))):
+        raise ValueError("Lanczos tridiagonal has non-finite entries")
+    T = np.diag(a)
+    if len(b):
+        T += np.diag(b, 1) + np.diag(b, -1)
+    return np.linalg.eigh(T)
+
+
+def shift(vals, tau: complex) -> float:
+    if len(vals) == 0:
+        return 0.0
+    ex = np.real(tau * np.asarray(vals, dtype=complex))
+    i = int(np.argmax(ex))
+    return float(vals[i]) if ex[i] > EXPV_SHIFT_TRIGGER else 0.0
+
+
+def coeffs_from_eigen(vals, vecs, tau: complex):
+    sigma = shift(vals, tau)
+    w = np.exp(tau * (vals - sigma)) * vecs[0, :]
+    return vecs.astype(complex) @ w, tau * sigma
+
+
+def _tail(coeff, betak):
+    return 0.0 if len(coeff) == 0 else betak * abs(coeff[-1])
+
+
+def tail_sup(evalcoeff, step: complex, betak: float, above: float = math.inf) -> float:
+    tail = _tail(evalcoeff(step)[0], betak)
+    if step.real != 0:
+        return tail
+    for node in range(1, TAIL_NODES):
+        if tail >= above:
+            return tail
+        tail = max(tail, _tail(evalcoeff(step * (node / TAIL_NODES))[0], betak))
+    return tail
+
+
+def pick_tau(evalcoeff, tau_rem: complex, betak: float, conv_tol: float) -> complex:
+    def accepted(t):
+        tl = tail_sup(evalcoeff, t, betak, above=conv_tol if conv_tol > 0 else math.inf)
+        return math.isfinite(tl) and (conv_tol <= 0 or tl <= conv_tol)
+
+    if accepted(tau_rem):
+        return tau_rem
+    lo, hi = 0.0, 1.0
+    for _ in range(40):
+        mid = 0.5 * (lo + hi)
+        if accepted(mid * tau_rem):
+            lo = mid
+        else:
+            hi = mid
+    return max(lo, 1e-6) * tau_rem
+
+
+class ExpvReport:
+    __slots__ = ("iterations", "substeps", "err")
+
+    def __init__(self, it, ns, err):
+        self.iterations, self.substeps, self.err = it, ns, err
+
+
+def lanczos_expv(apply, tau: complex, vector, *, inner, norm, axpy, scale,
+                 tol: float = 1e-13, maxiter: int = 60, reorth: bool = False,
+                 conv_tol: float = 0.0, substeps: int = 1):
+    """exp(tau A) vector for Hermitian ``apply``; vector algebra through the callbacks.
+
+    ``inner(a, b)`` = <a|b>, ``norm(a)``, ``axpy(alpha, x, y)`` = y + alpha x (new object),
+    ``scale(alpha, x)`` = alpha x.  Returns (result, ExpvReport).
+    """
+    beta0 = norm(vector)
+    if beta0 == 0:
+        return vector, ExpvReport(0, 0, 0.0)
+    cur = vector
+    tau_rem = complex(tau)
+    logscale = 0j
+    total = nsub = 0
+    err = 0.0
+    while True:
+        nsub += 1
+        cur_norm = norm(cur)
+        if cur_norm == 0:
+            break
+        v = scale(1.0 / cur_norm, cur)
+        basis = [v]
+        alpha: List[float] = []
+        betas: List[float] = []
+        res = apply(v)
+        total += 1
+        a = inner(v, res).real
+        alpha.append(a)
+        res = axpy(-a, v, res)
+        for _ in range(2, maxiter + 1):
+            b = norm(res)
+            if b < tol:
+                break
+            if conv_tol > 0:
+                vals, vecs = tridiag_eigen(alpha, betas)
+                tt = tail_sup(lambda t: coeffs_from_eigen(vals, vecs, t), tau_rem, b, above=conv_tol)
+                if tt < conv_tol:
+                    break
+            betas.append(b)
+            v = scale(1.0 / b, res)
+            basis.append(v)
+            res = apply(v)
+            total += 1
+            a = inner(v, res).real
+            alpha.append(a)
+            res = axpy(-b, basis[-2], axpy(-a, v, res))
+            if reorth:
+                for _p in range(2):
+                    coeffs = [inner(p, res) for p in basis]
+                    for c, p in zip(coeffs, basis):
+                        res = axpy(-c, p, res)
+        vals, vecs = tridiag_eigen(alpha, betas)
+
+        def evalcoeff(t, vals=vals, vecs=vecs):
+            return coeffs_from_eigen(vals, vecs, complex(t))
+
+        betak = norm(res)
+        if substeps == 1:
+            tau_s = tau_rem
+        elif substeps > 1:
+            tau_s = tau_rem / (substeps - nsub + 1)
+        else:
+            tau_s = pick_tau(evalcoeff, tau_rem, betak, conv_tol)
+        coeff, ls = evalcoeff(tau_s)
+        err += tail_sup(evalcoeff, tau_s, betak) * cur_norm / beta0
+        out = scale(coeff[0] * cur_norm, basis[0])
+        for c, p in zip(coeff[1:], basis[1:]):
+            out = axpy(c * cur_norm, p, out)
+        cur = out
+        logscale += ls
+        tau_rem -= tau_s
+        if substeps == 1 or abs(tau_rem) <= 1e-14 * max(abs(tau), 1.0):
+            break
+        if nsub >= 10000:
+            raise RuntimeError("expv: substep search failed to advance")
+    sc = np.exp(logscale)
+    if np.isfinite(abs(sc)) and sc != 0:
+        cn = norm(cur)
+        if not (np.isfinite(cn) and cn > 0 and abs(sc) * cn > EXPV_MAX_NORM):
+            cur = scale(sc, cur)
+    return cur, ExpvReport(total, nsub, err)

@@ -1,0 +1,80 @@
+"""Pin the oracle's whole CBE-BUG step against closed forms, ED and the reference's frozen runs."""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import bug as obug
+from oracle import models as om
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "b155550_xx_u1.json")
+
+
+def free_fermion_sz(L, t, occupied, J=1.0):
+    """xx_free_fermion_sz (test/common/free_fermion.jl:37-40)."""
+    h = np.diag(np.full(L - 1, J / 2), 1) + np.diag(np.full(L - 1, J / 2), -1)
+    U = sla.expm(-1j * t * h)
+    return np.array([sum(abs(U[j, k]) ** 2 for k in occupied) - 0.5 for j in range(L)])
+
+
+def test_replays_frozen_order1_run():
+    """xx_u1_bug_order1 (L=10 Neel, dt 0.05): S^z to 1e-5 of the frozen run, error vs free
+    fermions <= 2 err + 1e-6, every status as frozen (test_fixtures_b155550.jl:40-66; the frozen
+    run used the :maxdim ceiling and the Arnoldi recurrence)."""
+    fx = json.load(open(GOLDEN))["xx_u1_bug_order1"]
+    L, dt = fx["L"], fx["dt"]
+    psi = obug.MPS(om.neel_state(L))
+    mpo = om.xxz_mpo(L, delta=0.0)
+    opts = obug.StepOptions(work_ceiling_rule="maxdim", krylov_mode="arnoldi")
+    occ = list(range(0, L, 2))
+    for n, row in enumerate(fx["steps"], start=1):
+        info = obug.cbe_bug_step(psi, mpo, -1j * dt, opts)
+        sz = obug.sz_profile(psi)
+        assert np.abs(sz - np.array(row["sz"])).max() <= 1e-5, n
+        assert np.abs(sz - free_fermion_sz(L, n * dt, occ)).max() <= 2 * row["err_exact"] + 1e-6, n
+        assert info.ledger.status == row["status"], n
+
+
+def test_default_step_tracks_free_fermions_domain_wall():
+    """The default step (rank_tail, Lanczos, per-bond 1.2 d D ceiling) on the L = 20 domain wall."""
+    L, dt = 20, 0.05
+    psi = obug.MPS(om.domain_wall_state(L))
+    mpo = om.xxz_mpo(L, delta=0.0)
+    for n in range(1, 6):
+        info = obug.cbe_bug_step(psi, mpo, -1j * dt)
+        err = np.abs(obug.sz_profile(psi) - free_fermion_sz(L, n * dt, range(L // 2))).max()
+        assert err < 1e-4, (n, err)
+        assert abs(psi.sites[psi.center].norm() - 1.0) < 1e-8
+
+
+def test_xxz_step_local_error_is_second_order_against_ed():
+    """One order-1 step from the Neel state against exp(-i dt H) on 2^L states: the step is exact
+    in the energy and its local error scales as dt^2 (the order-1 BUG integrator)."""
+    L, delta = 8, 0.5
+    H = om.dense_hamiltonian(L, delta=delta)
+    errs = []
+    for dt in (0.05, 0.01):
+        psi = obug.MPS(om.neel_state(L))
+        v0 = om.mps_dense(psi.sites)
+        info = obug.cbe_bug_step(psi, om.xxz_mpo(L, delta=delta), -1j * dt,
+                                 obug.StepOptions(work_ceiling_rule="maxdim"))
+        got = om.mps_dense(psi.sites)
+        exact = sla.expm(-1j * dt * H) @ v0
+        errs.append(np.linalg.norm(got - exact))
+        assert abs(np.vdot(got, H @ got) - np.vdot(v0, H @ v0)) < 1e-10
+        assert info.ledger.status == "met"
+    assert errs[0] < 5e-3
+    assert 20 < errs[0] / errs[1] < 30            # (0.05 / 0.01)^2 = 25
+
+
+def test_invalid_options_are_refused_before_touching_the_state():
+    psi = obug.MPS(om.neel_state(6))
+    before = [s.copy() for s in psi.sites]
+    with pytest.raises(ValueError):
+        obug.cbe_bug_step(psi, om.xxz_mpo(6), -0.1j, obug.StepOptions(work_ceiling_rule="nope"))
+    with pytest.raises(ValueError):
+        obug.cbe_bug_step(psi, om.xxz_mpo(5), -0.1j)
+    for a, b in zip(before, psi.sites):
+        assert a.blocks.keys() == b.blocks.keys()
