@@ -75,6 +75,27 @@ class BugTerm(ctypes.Structure):
     ]
 
 
+class BugCopy(ctypes.Structure):
+    _fields_ = [("src_off", ctypes.c_int64), ("dst_off", ctypes.c_int64),
+                ("src_ld", ctypes.c_int32), ("dst_ld", ctypes.c_int32),
+                ("rows", ctypes.c_int32), ("cols", ctypes.c_int32),
+                ("mode", ctypes.c_int32), ("scale_mode", ctypes.c_int32),
+                ("scale_off", ctypes.c_int64)]
+
+
+class BugQrJob(ctypes.Structure):
+    _fields_ = [("a_off", ctypes.c_int64), ("q_off", ctypes.c_int64), ("r_off", ctypes.c_int64),
+                ("m", ctypes.c_int32), ("n", ctypes.c_int32), ("lda", ctypes.c_int32),
+                ("ldq", ctypes.c_int32), ("ldr", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class BugSvdJob(ctypes.Structure):
+    _fields_ = [("g_off", ctypes.c_int64), ("u_off", ctypes.c_int64), ("s_off", ctypes.c_int64),
+                ("vh_off", ctypes.c_int64), ("m", ctypes.c_int32), ("n", ctypes.c_int32),
+                ("ldg", ctypes.c_int32), ("ldu", ctypes.c_int32), ("ldvh", ctypes.c_int32),
+                ("phase_on_v", ctypes.c_int32)]
+
+
 # name -> (restype, argtypes); the list is the ABI of include/bug.h
 _P = ctypes.c_void_p
 _BSMP = ctypes.POINTER(BugBsm)
@@ -101,6 +122,26 @@ SIGNATURES = {
     "bug_apply_zero_site": (
         ctypes.c_int,
         [_P, _BSMP, _BSMP, _BSMP, _BSMP, ctypes.c_double, ctypes.c_int32, _P],
+    ),
+    "bug_axpy": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _P, _P, _P]),
+    "bug_scal": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _P, _P]),
+    "bug_dotc": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_P), _P, _f64p, _P]),
+    "bug_copy_blocks": (
+        ctypes.c_int,
+        [_P, ctypes.c_int32, ctypes.POINTER(BugCopy), _P, ctypes.c_int64, _P, ctypes.c_int64,
+         _f64p, ctypes.c_int32, _P],
+    ),
+    "bug_block_sqnorms": (
+        ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(BugCopy), _P, ctypes.c_int64, _f64p, _P]),
+    "bug_block_qr": (
+        ctypes.c_int,
+        [_P, ctypes.c_int32, ctypes.POINTER(BugQrJob), _P, ctypes.c_int64, _P, ctypes.c_int64, _P,
+         ctypes.c_int64, _P],
+    ),
+    "bug_block_svd": (
+        ctypes.c_int,
+        [_P, ctypes.c_int32, ctypes.POINTER(BugSvdJob), _P, ctypes.c_int64, _P, ctypes.c_int64,
+         _f64p, ctypes.c_int64, _P, ctypes.c_int64, ctypes.c_int32, _P],
     ),
 }
 

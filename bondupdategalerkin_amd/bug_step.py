@@ -1,0 +1,855 @@
+"""``cbe_bug_step``: one rank-adaptive CBE-BUG step on a device-resident U(1) MPS.
+
+The drop-in for ``cbe_bug_step!(psi, mpo, tau; kwargs...)`` (src/BUG/TimeEvolution/bug_step.jl:
+17-513): same stages, options, tolerance shares and ledger verdict.  Every tensor stays on the
+device; the host makes the data-dependent decisions (ranks, Krylov stops, budget checks) from
+downloaded scalars and spectra.
+
+Stages (0-based sites; bond b joins sites b and b+1; link j sits left of site j):
+
+* options / budget         bug_step_stages.jl:15-142; basis_budget.jl:75-128,454-494,568
+* half-sweeps              src/BUG/Core/Sweeps/half_sweeps.jl:218-410
+* bond frames              effective_operators.jl:201-220 (unique-QR gauge)
+* DD-QR CBE ``:rank_tail`` src/BUG/Randomised/dd_cbe_expand.jl:19-87,252-301,383-419; dd_cbe.jl:
+                           38-89,132-161; cbe.jl:188-236; sketch.jl:141-180; streamed sketch
+                           mpo_environments.jl:409-481
+* local Krylov (C/D)       krylov_enrichment.jl:135-292; krylov_lanczos.jl; krylov_arnoldi.jl
+* projection miss          krylov_closure_frames.jl:242-280
+* root solve               effective_operators.jl:91-107; krylov.jl:128-263
+* assembly + compression   diagnostics.jl:269-337; truncation.jl:49-158
+
+``order > 1`` (the frontier closure) is not implemented and is refused.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+
+from . import _lib
+from . import environments as ev
+from . import krylov as K
+from . import tensor_ops as T
+from .blocks import BlockTensor, Layout, Leg
+from .mpo import MPO, MPOSite
+from .mps import SymMPS
+from .probes import ProbeStream
+
+CBE_EPS = T.CBE_EPS
+TOL_DEFAULTS = dict(basis=1e-5, trunc=1e-7, expv=1e-10)
+DEFAULT_BOND_CEILING_FACTOR = 1.2
+
+
+@dataclass
+class DDQRExpansion:
+    """The DD-QR CBE strategy (``DDQRExpansion``, src/BUG/Randomised/cbe.jl:254-290)."""
+
+    mode: str = "rank_tail"
+    fixed_width: int = 8
+    tail_factor: float = 0.10
+    krylov_mode: str = "auto"
+
+    def __post_init__(self):
+        if self.mode != "rank_tail":
+            raise ValueError(f"DD mode {self.mode!r} is not implemented (rank_tail only)")
+        if self.fixed_width < 1:
+            raise ValueError("rank_tail requires positive fixed_width")
+        if not 0 < self.tail_factor <= 1:
+            raise ValueError("tail_factor must lie in (0, 1]")
+        if self.krylov_mode not in ("auto", "lanczos", "arnoldi"):
+            raise ValueError("krylov_mode must be auto, lanczos or arnoldi")
+
+
+@dataclass
+class AdaptiveLedger:
+    """The step's verdict (``AdaptiveLedger``, basis_budget.jl:144-299)."""
+
+    max_sample: float = 0.0
+    max_tail: float = 0.0
+    max_krylov: float = 0.0
+    max_krylov_c: float = 0.0
+    max_krylov_d: float = 0.0
+    discarded: float = 0.0
+    expv_err: float = 0.0
+    expv_iters: int = 0
+    n_probes: int = 0
+    n_selected: int = 0
+    n_krylov: int = 0
+    n_h1: int = 0
+    workcap_reached: bool = False
+    tolerance_met: bool = True
+    maxdim_reached: bool = False
+    proj_err: float = 0.0
+    r_old: int = 0
+    r_aug: int = 0
+    r_final: int = 0
+    unmet: List[str] = field(default_factory=list)
+
+    def fail(self, what: str):
+        self.tolerance_met = False
+        if what not in self.unmet:
+            self.unmet.append(what)
+
+
+def budget_status(ledger: AdaptiveLedger) -> str:
+    return "met" if ledger.tolerance_met else "budget_unmet"
+
+
+@dataclass
+class CBEBugSweepInfo:
+    """What one step reports (``CBEBugSweepInfo``, diagnostics.jl:209-255)."""
+
+    ledger: AdaptiveLedger
+    expanded: List[int]
+    root_link: int
+    root_rank: int
+    discarded: float
+    root_applications: int
+    seconds: Dict[str, float]
+
+
+@dataclass
+class _Alloc:
+    per_bond: float
+    sample: float
+    tail: float
+    krylov: float
+    scale: float
+
+
+def basis_allocation(tol_basis, nbonds, tau, nrm, nfront, tol_krylov=None) -> _Alloc:
+    """The ``:linear`` rule the step uses (basis_budget.jl:75-104)."""
+    nb, nf = max(nbonds, 1), max(nfront, 1)
+    kr = 1.0 if tol_krylov is None else tol_krylov / tol_basis
+    if kr <= 0:
+        raise ValueError("tol_krylov must be positive")
+    at = abs(tau)
+    scale = 1.0 if at == 0 else at / max(nrm, np.finfo(float).eps)
+    per_test = tol_basis / (2 * (nb + 1) * nf)
+    return _Alloc(2 * per_test, per_test, per_test, kr * per_test, scale)
+
+
+def bond_work_ceilings(ranks, d, maxdim, factor):
+    """Per-bond working-rank caps (bond_work_ceilings, basis_budget.jl:484-494)."""
+    def one(r):
+        return math.ceil(round(factor * d * r, 9)) if factor > 0 else 2 * d * r
+    return [max(min(one(r), maxdim), 1) for r in ranks]
+
+
+# ---- gauge --------------------------------------------------------------------------------
+
+def canonical(psi: SymMPS, site: int) -> SymMPS:
+    """canonical! (canonical_form.jl:43-54) with unique-QR moves."""
+    if not 0 <= site < len(psi):
+        raise IndexError(f"site {site} outside the chain")
+    while psi.center < site:
+        i = psi.center
+        Q, R = T.qr_left(psi[i])
+        psi[i] = Q
+        psi[i + 1] = T.core_times_site(R, psi[i + 1])
+        psi.center += 1
+    while psi.center > site:
+        i = psi.center
+        Lc, Q = T.qr_right(psi[i])
+        psi[i] = Q
+        psi[i - 1] = T.site_times_core(psi[i - 1], Lc)
+        psi.center -= 1
+    return psi
+
+
+# ---- probes and sketches ------------------------------------------------------------------
+
+def _fused_dims(lay: Layout, side: str) -> Dict[int, int]:
+    out: Dict[int, int] = {}
+    if side == "left":
+        for q, d in zip(lay.row.charges, lay.row.dims):
+            for sh in lay.shifts:
+                out[q + sh] = out.get(q + sh, 0) + d
+    else:
+        for q, d in zip(lay.col.charges, lay.col.dims):
+            for sh in lay.shifts:
+                out[q - sh] = out.get(q - sh, 0) + d
+    return dict(sorted(out.items()))
+
+
+def _widths(lay: Layout, side: str) -> Dict[int, int]:
+    return lay.col.as_dict() if side == "left" else lay.row.as_dict()
+
+
+def _width(t: BlockTensor, side: str) -> int:
+    return t.layout.col.total if side == "left" else t.layout.row.total
+
+
+def dd_probe(frame: BlockTensor, side: str, width: int, stream: ProbeStream) -> Optional[BlockTensor]:
+    """Sector-graded Gaussian probe in the discarded space of ``frame`` (_dd_probe, dd_cbe.jl:132-161).
+
+    Columns per fused charge by its discarded dimension (``_alloc_columns``, cbe.jl:188-195);
+    stream order per fused charge ascending, per slice ascending, one (link x dfull) block
+    column-major of which the first ``cols`` columns are kept."""
+    lay = frame.layout
+    dfull = _fused_dims(lay, side)
+    dT = _widths(lay, side)
+    dC = {q: max(d - dT.get(q, 0), 0) for q, d in dfull.items()}
+    tot = sum(dC.values())
+    if tot == 0:
+        return None
+    cols = {q: (min(d, math.ceil(d / tot * width)) if width > 0 else 0) for q, d in dC.items()}
+    wleg = Leg.make((q, c) for q, c in cols.items() if c > 0)
+    link = lay.row if side == "left" else lay.col
+    play = Layout(link, wleg, lay.shifts) if side == "left" else Layout(wleg, link, lay.shifts)
+    blocks = {}
+    for q, df in dfull.items():
+        c = cols.get(q, 0)
+        for s, sh in enumerate(lay.shifts):
+            lq = q - sh if side == "left" else q + sh
+            dl = link.dim_of(lq)
+            if dl == 0:
+                continue
+            vals = stream.draw(dl * c)
+            stream.skip(dl * df - dl * c)
+            if c == 0:
+                continue
+            blk = vals.reshape(c, dl).T
+            if side == "left":
+                blocks[(s, lq)] = blk
+            else:
+                blocks[(s, q)] = blk.T.copy()
+    Om = BlockTensor.from_blocks(play, blocks)
+    proj = T.perp_left(frame, Om) if side == "left" else T.perp_right(frame, Om)
+    return proj if T.norm(proj) > 0 else None
+
+
+def _slots(entries):
+    slots = {}
+    for e in entries:
+        slots.setdefault(e, len(slots))
+    return slots
+
+
+def stream_left(L, W1: MPOSite, W2: MPOSite, R, A, V, Om) -> BlockTensor:
+    """(H Theta) Om^dag for Theta = A V with Om in V's layout: (link_l, s, g)
+    (_stream_left, mpo_environments.jl:409-444), four grouped launches."""
+    N, Tt, C, Rr = T.OP_N, T.OP_T, T.OP_C, T.OP_R
+    Ls, Rs, As, Vs, Os = L.layout.shifts, R.layout.shifts, A.layout.shifts, V.layout.shifts, Om.layout.shifts
+    e2 = list(zip(W2.a, W2.b, W2.sout, W2.sin, W2.coeff))
+    e1 = list(zip(W1.a, W1.b, W1.sout, W1.sin, W1.coeff))
+    fs = _slots([(c, si) for c, b, so, si, _ in e2])
+    fsh = {}
+    tf = []
+    for c, b, so, si, co in e2:
+        k = fs[(c, si)]
+        fsh[k] = -Rs[b] - Os[so]
+        tf.append((k, b, so, Tt, C, co))
+    F = T.product(R.layout.col, Om.layout.row, [fsh[k] for k in range(len(fs))], R, Om, tf)
+    cs = sorted({c for c, _ in fs})
+    ci = {c: i for i, c in enumerate(cs)}
+    bsh, tb = {}, []
+    for (c, si), k in fs.items():
+        tb.append((ci[c], si, k, N, N, 1.0))
+        bsh[ci[c]] = Vs[si] + fsh[k]
+    B = T.product(V.layout.row, Om.layout.row, [bsh[i] for i in range(len(cs))], V, F, tb)
+    cslots, csh, tc = {}, {}, []
+    for a, c, so, si, co in e1:
+        if c in ci and (c, si) not in cslots:
+            k = cslots[(c, si)] = len(cslots)
+            tc.append((k, si, ci[c], N, N, 1.0))
+            csh[k] = As[si] + bsh[ci[c]]
+    Cm = T.product(A.layout.row, Om.layout.row, [csh[k] for k in range(len(cslots))], A, B, tc)
+    ty = [(so, a, cslots[(c, si)], N, N, co) for a, c, so, si, co in e1 if (c, si) in cslots]
+    return T.product(L.layout.row, Om.layout.row, As, L, Cm, ty)
+
+
+def stream_right(L, W1: MPOSite, W2: MPOSite, R, U, Bv, Om) -> BlockTensor:
+    """Om^dag (H Theta) for Theta = U Bv with Om in U's layout: (g, s, link_r)
+    (_stream_right, mpo_environments.jl:446-481), four grouped launches."""
+    N, Tt, C, Rr = T.OP_N, T.OP_T, T.OP_C, T.OP_R
+    Ls, Rs, Us, Bs, Os = L.layout.shifts, R.layout.shifts, U.layout.shifts, Bv.layout.shifts, Om.layout.shifts
+    e1 = list(zip(W1.a, W1.b, W1.sout, W1.sin, W1.coeff))
+    e2 = list(zip(W2.a, W2.b, W2.sout, W2.sin, W2.coeff))
+    fs = _slots([(c, si) for a, c, so, si, _ in e1])
+    fsh, tf = {}, []
+    for a, c, so, si, co in e1:
+        k = fs[(c, si)]
+        fsh[k] = -Ls[a] + Os[so]
+        tf.append((k, a, so, Tt, Rr, co))
+    F = T.product(L.layout.col, Om.layout.col, [fsh[k] for k in range(len(fs))], L, Om, tf)
+    cs = sorted({c for c, _ in fs})
+    ci = {c: i for i, c in enumerate(cs)}
+    bsh, tb = {}, []
+    for (c, si), k in fs.items():
+        tb.append((ci[c], si, k, Tt, N, 1.0))
+        bsh[ci[c]] = -Us[si] + fsh[k]
+    B = T.product(U.layout.col, Om.layout.col, [bsh[i] for i in range(len(cs))], U, F, tb)
+    cslots, csh, tc = {}, {}, []
+    for c, b, so, si, co in e2:
+        if c in ci and (c, si) not in cslots:
+            k = cslots[(c, si)] = len(cslots)
+            tc.append((k, ci[c], si, Tt, N, 1.0))
+            csh[k] = -bsh[ci[c]] + Bs[si]
+    Cm = T.product(Om.layout.col, Bv.layout.col, [csh[k] for k in range(len(cslots))], B, Bv, tc)
+    ty = [(so, cslots[(c, si)], b, N, Tt, co) for c, b, so, si, co in e2 if (c, si) in cslots]
+    return T.product(Om.layout.col, R.layout.row, Bs, Cm, R, ty)
+
+
+# ---- frames -------------------------------------------------------------------------------
+
+@dataclass
+class BondFrame:
+    """The canonical snapshot at a bond (``BondFrame``, src/BUG/Core/Sweeps/frames.jl:17-30)."""
+
+    U0: BlockTensor
+    S0: BlockTensor
+    V0: BlockTensor
+    old_rank: int
+
+
+def frame_from(left: BlockTensor, right: BlockTensor) -> BondFrame:
+    """``_frame_from`` (effective_operators.jl:201-220)."""
+    U0, Rl = T.qr_left(left)
+    Lr, V0 = T.qr_right(right)
+    return BondFrame(U0, T.core_times_core(Rl, Lr), V0, U0.layout.col.total)
+
+
+def _perp(side):
+    return T.perp_left if side == "left" else T.perp_right
+
+
+def reortho(frame: BlockTensor, block: Optional[BlockTensor], side: str) -> Optional[BlockTensor]:
+    """``_reortho_left/right`` (truncation.jl:374-394)."""
+    if block is None:
+        return None
+    outside = _perp(side)(frame, block)
+    if T.norm(outside) <= CBE_EPS * max(T.norm(block), 1.0):
+        return None
+    if side == "left":
+        r = T.svd_left(outside, cutoff=1e-14)
+        return r.U if r.kept else None
+    r = T.svd_right(outside, cutoff=1e-14)
+    return r.Vh if r.kept else None
+
+
+def cap_to_local_room(frame: BlockTensor, cand: Optional[BlockTensor], side: str):
+    """``_cap_to_local_room`` (truncation.jl:332-351)."""
+    if cand is None:
+        return None
+    dfull = _fused_dims(frame.layout, side)
+    dT = _widths(frame.layout, side)
+    counts, over = {}, False
+    for q, du in _widths(cand.layout, side).items():
+        if du == 0:
+            continue
+        counts[q] = min(du, dfull.get(q, 0) - dT.get(q, 0))
+        over |= counts[q] < du
+    return cand if not over else T.trim(cand, side, counts)
+
+
+def trim_total(t: BlockTensor, side: str, n: int) -> Optional[BlockTensor]:
+    """``_trim_total`` (cbe.jl:219-236): the first sectors in ascending charge order."""
+    if n <= 0:
+        return None
+    w = _widths(t.layout, side)
+    if sum(w.values()) <= n:
+        return t
+    left, keep = n, {}
+    for q in sorted(w):
+        keep[q] = min(w[q], left)
+        left -= keep[q]
+    return T.trim(t, side, keep)
+
+
+def _oplus(frame, new, side):
+    return T.oplus_left(frame, new) if side == "left" else T.oplus_right(frame, new)
+
+
+def sample_residual(basis, project, Y: BlockTensor, Om: BlockTensor, side: str) -> float:
+    """Test B's per-width-sector normalised residual (``_sample_residual``, sketch.jl:141-180)."""
+    cols = Om.layout.row.as_dict() if side == "left" else Om.layout.col.as_dict()
+    cols = {q: n for q, n in cols.items() if n > 0}
+    if not cols:
+        return 0.0
+    res = project(Y)
+    if T.norm(res) <= CBE_EPS:
+        return 0.0
+    if basis is not None:
+        res = project(_perp(side)(basis, res))
+    blks, keys = [], []
+    for k, q, qc, d, dc, off in res.layout.blocks():
+        blks.append((off, d, d, dc))
+        keys.append(qc if side == "left" else q)
+    sq = T.block_sqnorms(blks, res.data)
+    per: Dict[int, float] = {}
+    for w, v in zip(keys, sq):
+        per[w] = per.get(w, 0.0) + float(v)
+    tot = 0.0
+    for w, v in per.items():
+        if cols.get(w, 0) <= 0:
+            raise RuntimeError("Test B: residual sector without a holdout column")
+        tot += v / cols[w]
+    return math.sqrt(tot)
+
+
+@dataclass
+class _Expansion:
+    U_ex: BlockTensor
+    V_ex: BlockTensor
+    n_new: int
+
+
+def dd_cbe_expand(fr: BondFrame, mpo: MPO, bond: int, lenv, renv, side: str, alloc: _Alloc,
+                  ledger: AdaptiveLedger, workcap: int, ex: DDQRExpansion, stream: ProbeStream,
+                  probe_hook=None) -> _Expansion:
+    """One side of one bond, ``:rank_tail`` (dd_cbe_expand.jl:19-87,252-301,383-419)."""
+    left = side == "left"
+    frame, opp = (fr.U0, fr.V0) if left else (fr.V0, fr.U0)
+    oside = "right" if left else "left"
+    none = _Expansion(fr.U0, fr.V0, 0)
+    full_self = sum(_fused_dims(frame.layout, side).values())
+    full_opp = sum(_fused_dims(opp.layout, oside).values())
+    cap = full_self - fr.old_rank
+    wbind = False
+    if workcap > 0:
+        wk = max(workcap - fr.old_rank, 0)
+        wbind = wk < cap
+        cap = min(cap, wk)
+    if cap <= 0:
+        return none
+    W1, W2 = mpo[bond], mpo[bond + 1]
+    width = min(ex.fixed_width, full_opp)
+    Om = probe_hook(opp, oside, width) if probe_hook else dd_probe(opp, oside, width, stream)
+    if Om is None:
+        return none
+    ledger.n_probes += _width(Om, oside)
+    if left:
+        Ysk = stream_left(lenv, W1, W2, renv, T.site_times_core(fr.U0, fr.S0), fr.V0, Om)
+        project = lambda t: T.perp_left(fr.U0, t)
+    else:
+        Ysk = stream_right(lenv, W1, W2, renv, fr.U0, T.core_times_site(fr.S0, fr.V0), Om)
+        project = lambda t: T.perp_right(fr.V0, t)
+    Y = project(Ysk)
+    if T.norm(Y) <= CBE_EPS:
+        return none
+    rmax = min(cap, _width(Om, oside))
+    cache = {}
+
+    def candidate(k):
+        if k <= 0:
+            return None
+        if k not in cache:
+            if left:
+                r = T.svd_left(Y, cutoff=CBE_EPS, nkeep=k)
+                cache[k] = reortho(frame, r.U, side) if r.kept else None
+            else:
+                r = T.svd_right(Y, cutoff=CBE_EPS, nkeep=k)
+                cache[k] = reortho(frame, r.Vh, side) if r.kept else None
+        return cache[k]
+
+    limit = ex.tail_factor * alloc.sample
+
+    def metric(k):
+        return alloc.scale * sample_residual(candidate(k), project, Y, Om, side)
+
+    if metric(0) <= limit:
+        return none
+    hi = rmax
+    if metric(hi) > limit:
+        directions, count = candidate(hi), hi
+    else:
+        lo = 0
+        while hi - lo > 1:
+            k = (hi + lo) // 2
+            if metric(k) <= limit:
+                hi = k
+            else:
+                lo = k
+        directions = candidate(hi)
+        count = 0 if directions is None else _width(directions, side)
+    if directions is None:
+        return none
+    if count > cap:
+        directions = trim_total(directions, side, cap)
+        if directions is None:
+            return none
+    if wbind and count >= cap:
+        ledger.workcap_reached = True
+    directions = cap_to_local_room(frame, directions, side)
+    if directions is None:
+        return none
+    nq = _width(directions, side)
+    ledger.n_selected += nq
+    grown = _oplus(frame, directions, side)
+    return _Expansion(grown, fr.V0, nq) if left else _Expansion(fr.U0, grown, nq)
+
+
+# ---- local Krylov enrichment ----------------------------------------------------------------
+
+def extend_frame(frame, residual, direction, side, weight, allowance, room):
+    """``_extend_frame`` (krylov_enrichment.jl:135-171) -> (frame, added, fits)."""
+    rn = T.norm(residual)
+    cut = CBE_EPS * max(T.norm(direction), rn) / rn
+    svd = T.svd_left if side == "left" else T.svd_right
+    res = svd(residual, cutoff=cut)
+    nk = len(res.kept)
+    keep, tail = K.tail_rank(res.kept, weight, allowance, min(nk, room), base=T.sq_weight(res.trunc))
+    fits = tail <= allowance
+    if keep == 0:
+        return frame, 0, fits
+    if keep < nk:
+        res = svd(residual, cutoff=cut, nkeep=keep)
+    new = reortho(frame, res.U if side == "left" else res.Vh, side)
+    new = cap_to_local_room(frame, new, side)
+    if new is None:
+        return frame, 0, fits
+    return _oplus(frame, new, side), _width(new, side), fits
+
+
+@dataclass
+class _KrylovStats:
+    eta_c: float = 0.0
+    eta_d: float = 0.0
+    added: int = 0
+    n_h1: int = 0
+    capped: bool = False
+    met: bool = True
+
+
+def krylov_enrich(seed, applyH, start_frame, side, tau, alloc: _Alloc, workcap: int,
+                  recurrence: str = "lanczos", maxdepth: int = K.MAXDEPTH):
+    """Tests C/D: grow the frame by the local Krylov directions the step needs
+    (``_krylov_enrich`` + ``_lanczos_chain``/``_arnoldi_chain``)."""
+    perp = _perp(side)
+    allow = alloc.krylov
+    st = _KrylovStats()
+    frame = [start_frame]
+
+    def score(direction, weight, first):
+        residual = perp(frame[0], direction)
+        eta = weight * T.norm(residual)
+        if first:
+            st.eta_c = eta
+        else:
+            st.eta_d = max(st.eta_d, eta)
+        if not eta > allow:
+            return True
+        room = workcap - _width(frame[0], side) if workcap > 0 else 1 << 60
+        if room <= 0:
+            return False
+        frame[0], nadd, fits = extend_frame(frame[0], residual, direction, side, weight, allow, room)
+        st.added += nadd
+        return fits
+
+    sn = T.norm(seed)
+    if sn == 0:
+        return frame[0], st
+    teff = tau if tau != 0 else 1.0 + 0j
+    vs = [T.scale(1.0 / sn, seed)]
+    H = np.zeros((maxdepth + 1, maxdepth + 1), dtype=complex)
+    closed = capped = False
+    prev_beta = 0.0
+    for k in range(1, maxdepth + 1):
+        img = applyH(vs[k - 1] if recurrence == "lanczos" else vs[-1])
+        st.n_h1 += 1
+        if recurrence == "lanczos":
+            if k > 1:
+                T.axpy_(-prev_beta, vs[k - 2], img)
+            a = T.inner(vs[k - 1], img)
+            if not abs(a.imag) <= 1e-9 * max(abs(a), 1.0):
+                raise ValueError("Lanczos needs a Hermitian one-site generator")
+            H[k - 1, k - 1] = a.real
+            T.axpy_(-a.real, vs[k - 1], img)
+            if k >= 20 and k % 2 == 0:
+                cs = T.inner_many(vs, img)
+                for c, v in zip(cs, vs):
+                    T.axpy_(-c, v, img)
+        else:
+            for _p in range(2):
+                cs = T.inner_many(vs, img)
+                for row, c in enumerate(cs):
+                    H[row, k - 1] += c
+                for c, v in zip(cs, vs):
+                    T.axpy_(-c, v, img)
+        beta = T.norm(img)
+        if beta <= K.BREAKDOWN:
+            closed = True
+            break
+        H[k, k - 1] = beta
+        if recurrence == "lanczos":
+            H[k - 1, k] = beta
+        vs.append(T.scale(1.0 / beta, img))
+        w = K.direction_weight(H, k, teff)
+        if not score(vs[-1], w, k == 1):
+            capped = True
+            break
+        if w <= allow:
+            closed = True
+            break
+        prev_beta = beta
+    if not closed:
+        capped = True
+    st.capped = capped
+    st.met = closed and not capped
+    return frame[0], st
+
+
+def _record_krylov(ledger: AdaptiveLedger, st: _KrylovStats):
+    ledger.max_krylov_c = max(ledger.max_krylov_c, st.eta_c)
+    ledger.max_krylov_d = max(ledger.max_krylov_d, st.eta_d)
+    ledger.max_krylov = max(ledger.max_krylov, st.eta_c, st.eta_d)
+    ledger.n_krylov += st.added
+    ledger.n_h1 += st.n_h1
+    if st.capped:
+        ledger.workcap_reached = True
+    if not st.met:
+        ledger.fail("krylov")
+
+
+# ---- the step -----------------------------------------------------------------------------
+
+def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[float] = None,
+                 tol_trunc: Optional[float] = None, tol_expv: Optional[float] = None,
+                 tol_root: Optional[float] = None, tol_krylov: Optional[float] = None,
+                 order: int = 1, maxdim: Optional[int] = None, hermitian: bool = True,
+                 work_ceiling_rule: str = "bond", bond_ceiling_factor: Optional[float] = None,
+                 root: Optional[int] = None, reorth: bool = True, truncate: bool = True,
+                 expansion: Optional[DDQRExpansion] = None, rng: Optional[int] = None,
+                 probe_hook: Optional[Callable] = None) -> CBEBugSweepInfo:
+    """One step in place; ``root`` is the 0-based root bond (the reference's ``root`` minus 1),
+    ``rng`` the probe seed (default 0x5EED, fresh per call as the reference's default RNG)."""
+    L = len(psi)
+    # -- option checks, before anything touches the state (bug_step_stages.jl:41-111) --
+    if len(mpo) != L:
+        raise ValueError(f"MPO has {len(mpo)} sites, state has {L}")
+    if L < 2:
+        raise ValueError("cbe_bug_step needs at least two sites")
+    rl = (max(1, min(L - 1, L // 2)) - 1) if root is None else root
+    if not 0 <= rl <= L - 2:
+        raise ValueError(f"root must be a bond in 0:{L - 2}, got {rl}")
+    if tol_root is not None and tol_expv is not None:
+        raise ValueError("tol_root and tol_expv name the same tolerance; pass one")
+    tol_expv = tol_root if tol_root is not None else tol_expv
+    tb = TOL_DEFAULTS["basis"] if tol_basis is None else tol_basis
+    tt = TOL_DEFAULTS["trunc"] if tol_trunc is None else tol_trunc
+    te = TOL_DEFAULTS["expv"] if tol_expv is None else tol_expv
+    if not tb > 0 or not tt >= 0 or not te > 0:
+        raise ValueError("tolerances must be positive (tol_trunc non-negative)")
+    if order < 1:
+        raise ValueError(f"order must be >= 1, got {order}")
+    if order > 1:
+        raise NotImplementedError("the order-p frontier closure (order > 1) is not implemented")
+    if work_ceiling_rule not in ("bond", "maxdim"):
+        raise ValueError(f"work_ceiling_rule must be 'bond' or 'maxdim', got {work_ceiling_rule!r}")
+    factor = bond_ceiling_factor
+    if factor is None:
+        factor = DEFAULT_BOND_CEILING_FACTOR if work_ceiling_rule == "bond" else 0.0
+    if not (factor == 0 or (factor > 1 and work_ceiling_rule == "bond")):
+        raise ValueError("bond_ceiling_factor must be 0, or > 1 with work_ceiling_rule='bond'")
+    if not hermitian:
+        raise NotImplementedError("hermitian=False (Arnoldi root) is not implemented")
+    ex = expansion or DDQRExpansion()
+    maxdim = 128 if maxdim is None else maxdim
+    tau = complex(tau)
+    d = psi.d
+    t0 = time.perf_counter()
+    stream = ProbeStream(0x5EED if rng is None else rng)
+    bd0 = psi.bond_dims()
+    if work_ceiling_rule == "bond":
+        wcap = bond_work_ceilings(bd0, d, maxdim, factor)
+    else:
+        wcap = [max(d * maxdim, d * max(bd0 + [0]), 1)] * (L - 1)
+    nrm = T.norm(psi[psi.center]) if 0 <= psi.center < L else 1.0
+    alloc = basis_allocation(tb, L - 1, tau, nrm, order, tol_krylov)
+    ledger = AdaptiveLedger(r_old=max(bd0 + [0]))
+    recurrence = ex.krylov_mode if ex.krylov_mode != "auto" else ("lanczos" if hermitian else "arnoldi")
+    expanded = [0] * (L - 1)
+    left_frames: List[Optional[BlockTensor]] = [None] * (rl + 1)
+    right_frames: List[Optional[BlockTensor]] = [None] * (L - 1)
+    secs = {}
+
+    # ---- left half-sweep ----
+    ts = time.perf_counter()
+    psiL = psi.copy()
+    canonical(psiL, 0)
+    renvs: List[Optional[BlockTensor]] = [None] * (L + 1)
+    renvs[L] = ev.boundary_right(mpo, psiL.bond_leg(L))
+    for i in range(L - 1, 1, -1):
+        renvs[i] = ev.push_right(renvs[i + 1], psiL[i], mpo[i])
+    lch = ev.boundary_left(mpo, psiL.bond_leg(0))
+    lroot = lch
+    carry = None
+    for b in range(rl + 1):
+        carried = psiL[b] if carry is None else T.core_times_site(carry, psiL[b])
+        fr = frame_from(carried, psiL[b + 1])
+        xp = dd_cbe_expand(fr, mpo, b, lch, renvs[b + 2], "left", alloc, ledger, wcap[b], ex,
+                           stream, probe_hook)
+        Kex = T.site_times_core(carried, T.overlap_right(psiL[b + 1], xp.V_ex))
+        rch1 = ev.push_right(renvs[b + 2], xp.V_ex, mpo[b + 1])
+        Hl, Hw, Hr = lch, mpo[b], rch1
+        Wk, kst = krylov_enrich(Kex, lambda v: ev.apply_one_site(Hl, Hw, Hr, v), xp.U_ex, "left",
+                                tau, alloc, wcap[b], recurrence)
+        _record_krylov(ledger, kst)
+        left_frames[b] = Wk
+        expanded[b] = Wk.layout.col.total
+        carry = T.overlap_left(Wk, carried)
+        if b == rl:
+            lroot = lch
+        else:
+            lch = ev.push_left(lch, Wk, mpo[b])
+    secs["left_sweep"] = time.perf_counter() - ts
+
+    # ---- right half-sweep ----
+    ts = time.perf_counter()
+    psiR = psi
+    canonical(psiR, L - 1)
+    lenvs = [ev.boundary_left(mpo, psiR.bond_leg(0))]
+    for i in range(L - 2):
+        lenvs.append(ev.push_left(lenvs[-1], psiR[i], mpo[i]))
+    rch = ev.boundary_right(mpo, psiR.bond_leg(L))
+    rroot = rch
+    carry = None
+    for b in range(L - 2, rl - 1, -1):
+        carried = psiR[b + 1] if carry is None else T.site_times_core(psiR[b + 1], carry)
+        fr = frame_from(psiR[b], carried)
+        xp = dd_cbe_expand(fr, mpo, b, lenvs[b], rch, "right", alloc, ledger, wcap[b], ex,
+                           stream, probe_hook)
+        Bex = T.core_times_site(T.overlap_left(xp.U_ex, psiR[b]), carried)
+        lch1 = ev.push_left(lenvs[b], xp.U_ex, mpo[b])
+        Hl, Hw, Hr = lch1, mpo[b + 1], rch
+        Zk, kst = krylov_enrich(Bex, lambda v: ev.apply_one_site(Hl, Hw, Hr, v), xp.V_ex,
+                                "right", tau, alloc, wcap[b], recurrence)
+        _record_krylov(ledger, kst)
+        right_frames[b] = Zk
+        if b != rl:
+            expanded[b] = Zk.layout.row.total
+        carry = T.overlap_right(carried, Zk)
+        if b == rl:
+            rroot = rch
+        else:
+            rch = ev.push_right(rch, Zk, mpo[b + 1])
+    AR = carry
+    secs["right_sweep"] = time.perf_counter() - ts
+
+    # ---- root start and projection miss ----
+    AL = None
+    for i in range(rl + 1):
+        c = psi[i] if AL is None else T.core_times_site(AL, psi[i])
+        AL = T.overlap_left(left_frames[i], c)
+    root_start = T.core_times_core(AL, AR)
+    npsi = T.norm(psi[psi.center])
+    ledger.r_aug = max(max(expanded + [0]), right_frames[rl].layout.row.total)
+    ledger.proj_err = projection_miss(psi, left_frames, right_frames, rl) / npsi if npsi > 0 else 0.0
+    if ledger.proj_err > tb:
+        ledger.fail("projection")
+
+    # ---- root solve ----
+    ts = time.perf_counter()
+    Lr = ev.push_left(lroot, left_frames[rl], mpo[rl])
+    Rr = ev.push_right(rroot, right_frames[rl], mpo[rl + 1])
+    napply = [0]
+
+    def apply_root(C):
+        napply[0] += 1
+        return ev.apply_zero_site(Lr, Rr, C)
+
+    result, rep = K.lanczos_expv(apply_root, tau, root_start, tol=1e-15, maxiter=K.ROOT_MAXITER,
+                                 reorth=reorth, conv_tol=te, substeps=0)
+    ledger.expv_err, ledger.expv_iters = rep.err, rep.iterations
+    if rep.err > te * (1 + 1e-8):
+        ledger.fail("expv")
+    secs["root"] = time.perf_counter() - ts
+
+    # ---- assembly and compression ----
+    ts = time.perf_counter()
+    res = T.svd_core(result, cutoff=0.0)
+    expanded[rl] = left_frames[rl].layout.col.total
+    discarded = T.trunc_weight(res)
+    for i in range(rl):
+        psi[i] = left_frames[i]
+    psi[rl] = T.site_times_core(left_frames[rl], res.U)
+    psi[rl + 1] = T.core_times_site(T.diag_scale(res.Vh, res.S, "rows"), right_frames[rl])
+    for b in range(rl + 1, L - 1):
+        psi[b + 1] = right_frames[b]
+    psi.center = rl + 1
+    root_rank = psi[rl].layout.col.total
+    if truncate:
+        wtr, cumw, capped = truncate_recursive(psi, rl + 1, maxdim, tt)
+        discarded = max(discarded, wtr)
+        ledger.discarded, ledger.maxdim_reached = cumw, capped
+        if cumw > tt * (1 + 1e-8):
+            ledger.fail("trunc")
+    ledger.r_final = max(psi.bond_dims() + [0])
+    secs["truncation"] = time.perf_counter() - ts
+    secs["step"] = time.perf_counter() - t0
+    return CBEBugSweepInfo(ledger, expanded, rl, root_rank, discarded, napply[0], secs)
+
+
+def projection_miss(psi: SymMPS, left_frames, right_frames, rl: int) -> float:
+    """``_projection_miss`` (krylov_closure_frames.jl:242-280), square-root factors by QR."""
+    L = len(psi)
+    rf: List[Optional[BlockTensor]] = [None] * (L + 1)
+    for i in range(L - 1, 0, -1):
+        carried = psi[L - 1] if i == L - 1 else T.site_times_core(psi[i], rf[i + 1])
+        rf[i], _ = T.qr_right(carried)
+    miss2 = 0.0
+    AL = None
+    for i in range(rl + 1):
+        carried = psi[i] if AL is None else T.core_times_site(AL, psi[i])
+        miss2 += T.norm(T.site_times_core(T.perp_left(left_frames[i], carried), rf[i + 1])) ** 2
+        AL = T.overlap_left(left_frames[i], carried)
+    lf: List[Optional[BlockTensor]] = [None] * (L - 1)
+    _, lf[rl] = T.qr_core(AL)
+    for b in range(rl + 1, L - 1):
+        _, lf[b] = T.qr_left(T.core_times_site(lf[b - 1], psi[b]))
+    AR = None
+    for b in range(L - 2, rl - 1, -1):
+        carried = psi[b + 1] if AR is None else T.site_times_core(psi[b + 1], AR)
+        miss2 += T.norm(T.core_times_site(lf[b], T.perp_right(right_frames[b], carried))) ** 2
+        AR = T.overlap_right(carried, right_frames[b])
+    return math.sqrt(miss2)
+
+
+def truncate_recursive(psi: SymMPS, root: int, maxdim: int, weight_tol: float):
+    """Budgeted root-to-leaves compression (``truncate_recursive!``, truncation.jl:59-158);
+    ``root`` is the 0-based root site.  Returns (worst, cumulative, capped)."""
+    L = len(psi)
+    sdiv = math.sqrt(root) + math.sqrt(L - 1 - root)
+    st = {"nrm2": -1.0, "worst": 0.0, "capped": False}
+    cum = [0.0, 0.0]
+
+    def cut(t, left_split):
+        svd = T.svd_left if left_split else T.svd_right
+        res = svd(t, cutoff=0.0, nkeep=maxdim)
+        tot = T.sq_weight(res.kept) + T.sq_weight(res.trunc)
+        if st["nrm2"] < 0:
+            st["nrm2"] = tot
+        share = weight_tol * math.sqrt(max(st["nrm2"], 0.0)) / sdiv if sdiv > 0 else 0.0
+        nk = len(res.kept)
+        keep, tail = K.tail_rank(res.kept, 1.0, share, nk, base=T.sq_weight(res.trunc))
+        if tail > share and res.trunc:
+            st["capped"] = True
+        keep = max(keep, 1)
+        if keep < nk:
+            res = svd(t, cutoff=0.0, nkeep=keep)
+        return res
+
+    def account(res, side):
+        st["worst"] = max(st["worst"], T.trunc_weight(res))
+        cum[side] += T.sq_weight(res.trunc)
+
+    for site in range(root, 0, -1):
+        res = cut(psi[site], False)
+        account(res, 0)
+        psi[site] = res.Vh
+        psi[site - 1] = T.site_times_core(psi[site - 1], T.diag_scale(res.U, res.S, "cols"))
+        psi.center = site - 1
+    canonical(psi, root)
+    for site in range(root, L - 1):
+        res = cut(psi[site], True)
+        account(res, 1)
+        psi[site] = res.U
+        psi[site + 1] = T.core_times_site(T.diag_scale(res.Vh, res.S, "rows"), psi[site + 1])
+        psi.center = site + 1
+    n2 = st["nrm2"]
+    cumulative = (math.sqrt(cum[0]) + math.sqrt(cum[1])) / math.sqrt(n2) if n2 > 0 else 0.0
+    return st["worst"], cumulative, st["capped"]

@@ -13,18 +13,6 @@
 
 namespace bug {
 
-struct Error : std::runtime_error {
-    int code;
-    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIP_CHECK(expr)                                                                  \
-    do {                                                                                 \
-        hipError_t _e = (expr);                                                          \
-        if (_e != hipSuccess)                                                            \
-            throw Error(BUG_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
-    } while (0)
-
 // ---- hashing of layouts -----------------------------------------------------------------
 
 struct Hasher {
@@ -359,8 +347,8 @@ void run_product(bug_ctx* ctx, const Bsm& out, void* out_dev, const Bsm& P, cons
     HIP_CHECK(launch_grouped_gemm(*plan, config, precision, p_dev, q_dev, out_dev, stream));
 }
 
-void* scratch(bug_ctx* ctx, hipStream_t stream, size_t bytes) {
-    Workspace& w = ctx->scratch[(void*)stream];
+void* scratch(bug_ctx* ctx, hipStream_t stream, size_t bytes, int slot) {
+    Workspace& w = ctx->scratch[slot][(void*)stream];
     if (w.bytes < bytes) {
         if (w.ptr) {
             HIP_CHECK(hipStreamSynchronize(stream));
@@ -424,24 +412,6 @@ using namespace bug;
 
 namespace {
 
-template <typename F>
-int guarded(bug_ctx* ctx, F&& f) {
-    if (!ctx) return BUG_EINVAL;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    try {
-        HIP_CHECK(hipSetDevice(ctx->device));
-        ctx->last_flops = 0.0;
-        f();
-        return BUG_OK;
-    } catch (const Error& e) {
-        ctx->last_error = e.what();
-        return e.code;
-    } catch (const std::exception& e) {
-        ctx->last_error = e.what();
-        return BUG_EINVAL;
-    }
-}
-
 hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
 }  // namespace
@@ -470,8 +440,10 @@ int bug_ctx_destroy(bug_ctx* ctx) {
         if (e.second->dev) (void)hipFree(e.second->dev);
         delete e.second;
     }
-    for (auto& s : ctx->scratch)
-        if (s.second.ptr) (void)hipFree(s.second.ptr);
+    for (auto& slot : ctx->scratch)
+        for (auto& s : slot)
+            if (s.second.ptr) (void)hipFree(s.second.ptr);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     delete ctx;
     return BUG_OK;
 }
@@ -491,7 +463,8 @@ int bug_tile_config_count(void) { return kNumTileConfigs; }
 int bug_ctx_stats(const bug_ctx* ctx, int64_t* out4) {
     if (!ctx || !out4) return BUG_EINVAL;
     int64_t bytes = 0;
-    for (auto& s : ctx->scratch) bytes += (int64_t)s.second.bytes;
+    for (auto& slot : ctx->scratch)
+        for (auto& s : slot) bytes += (int64_t)s.second.bytes;
     for (auto& e : ctx->plan_lru) bytes += (int64_t)e.second->dev_bytes;
     out4[0] = bytes;
     out4[1] = (int64_t)ctx->plan_lru.size();

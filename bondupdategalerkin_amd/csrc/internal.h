@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <list>
 #include <mutex>
+#include <stdexcept>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -13,6 +14,18 @@
 #include "../../include/bug.h"
 
 namespace bug {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(expr)                                                                  \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess)                                                            \
+            throw ::bug::Error(BUG_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
 
 // One product term of an output block: alpha * op_a(A) * op_b(B), A at a_off of the A operand,
 // B at b_off of the B operand, inner dimension k.  ops = op_a | (op_b << 2).
@@ -45,6 +58,31 @@ struct GemmTile {
     int32_t tm;
     int32_t tn;
     int32_t pad;
+};
+
+// One block copy (see bug_copy in bug.h).
+struct CopyOp {
+    int64_t src_off;
+    int64_t dst_off;
+    int32_t src_ld;
+    int32_t dst_ld;
+    int32_t rows;
+    int32_t cols;
+    int32_t mode;
+    int32_t scale_mode;
+    int64_t scale_off;
+};
+
+// One thin QR: A (m x n) -> Q (m x k), R (k x n); w_off = tau workspace offset.
+struct MatJob {
+    int64_t a_off, q_off, r_off, w_off;
+    int32_t m, n, lda, ldq, ldr;
+};
+
+// One SVD by one-sided Jacobi (m >= n): G overwritten, V accumulated at v_off.
+struct SvdJob {
+    int64_t g_off, v_off, u_off, s_off, vh_off;
+    int32_t m, n, ldg, ldv, ldu, ldvh, phase_on_v;
 };
 
 // A product plan: tasks and terms in host memory, plus their device copy.
@@ -81,6 +119,9 @@ struct Workspace {
     size_t bytes = 0;
 };
 
+// Grow-only per-stream device scratch of the context (slot selects one of several buffers).
+void* scratch(bug_ctx* ctx, hipStream_t stream, size_t bytes, int slot = 0);
+
 }  // namespace bug
 
 struct bug_ctx {
@@ -94,6 +135,31 @@ struct bug_ctx {
     size_t plan_capacity = 512;
     int64_t plan_hits = 0, plan_misses = 0;
     double last_flops = 0.0;     // algorithmic flops of the last entry-point call
-    // per-stream scratch for intermediates of the two-stage contractions
-    std::unordered_map<void*, bug::Workspace> scratch;
+    // per-stream scratch for intermediates of the two-stage contractions and the solvers,
+    // keyed by (stream, slot)
+    std::unordered_map<void*, bug::Workspace> scratch[4];
+    // pinned host staging for small reductions
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
 };
+
+namespace bug {
+// Run f under the context lock, mapping exceptions to status codes and messages.
+template <typename F>
+int guarded(bug_ctx* ctx, F&& f) {
+    if (!ctx) return BUG_EINVAL;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    try {
+        HIP_CHECK(hipSetDevice(ctx->device));
+        ctx->last_flops = 0.0;
+        f();
+        return BUG_OK;
+    } catch (const Error& e) {
+        ctx->last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        ctx->last_error = e.what();
+        return BUG_EINVAL;
+    }
+}
+}  // namespace bug

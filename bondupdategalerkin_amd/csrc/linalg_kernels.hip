@@ -1,0 +1,698 @@
+// Vector, copy and factorization kernels of the BUG hot path for gfx950.
+//
+// * K10 vector kernels: complex axpy / scale, and a fused multi-dot <x_j | y> (j < 64) that reads
+//   y once per block (Krylov inner products, CGS2 passes, norms).  HBM-bound.
+// * Block copies: one launch moves any list of (sub)blocks between two buffers, optionally
+//   transposed / conjugated, with an optional real per-row or per-column scale (matricisations of
+//   block-sparse tensors, direct sums, trims, S Vh products).
+// * Batched thin QR (Householder, one workgroup per matrix) with the unique positive-diagonal
+//   gauge: the gauge moves of the step (canonical_form.jl:10-54, effective_operators.jl:201-220).
+// * Batched SVD by one-sided (Hestenes) Jacobi with Brent-Luk round-robin pair ordering, one
+//   workgroup per matrix, singular values sorted descending and a deterministic phase per
+//   singular pair: the per-sector SVDs of Telum's svd (deps/Telum/src/svd.jl:2013-2211).
+//
+// Waves are 64 lanes; every reduction is a wave butterfly (__shfl_xor) so the sum lands in every
+// lane.  No atomics; results are deterministic.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "internal.h"
+
+namespace bug {
+namespace {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// ---- K10: axpy / scale / multi-dot --------------------------------------------------------
+
+__global__ void zaxpy_kernel(int64_t n, double ar, double ai, const double2* __restrict__ x,
+                             double2* __restrict__ y) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 v = x[i];
+        double2 w = y[i];
+        w.x += ar * v.x - ai * v.y;
+        w.y += ar * v.y + ai * v.x;
+        y[i] = w;
+    }
+}
+
+__global__ void zscal_kernel(int64_t n, double ar, double ai, double2* __restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 v = x[i];
+        x[i] = double2{ar * v.x - ai * v.y, ar * v.y + ai * v.x};
+    }
+}
+
+constexpr int kMaxDots = 64;
+constexpr int kDotThreads = 256;
+constexpr int kDotRows = 4;  // rows per thread per block
+
+struct DotPtrs {
+    const double2* p[kMaxDots];
+};
+
+// partial[j * nblk + b] = sum over block b's rows of conj(x_j) y  (re, im interleaved)
+__global__ __launch_bounds__(kDotThreads) void multidot_partial(int64_t n, int k, DotPtrs xs,
+                                                                 const double2* __restrict__ y,
+                                                                 double2* __restrict__ partial) {
+    __shared__ double red_re[kDotThreads / 64];
+    __shared__ double red_im[kDotThreads / 64];
+    const int64_t base = (int64_t)blockIdx.x * kDotThreads * kDotRows;
+    double2 yv[kDotRows];
+    int64_t idx[kDotRows];
+#pragma unroll
+    for (int r = 0; r < kDotRows; ++r) {
+        idx[r] = base + threadIdx.x + (int64_t)r * kDotThreads;
+        yv[r] = idx[r] < n ? y[idx[r]] : double2{0.0, 0.0};
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j = 0; j < k; ++j) {
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int r = 0; r < kDotRows; ++r) {
+            if (idx[r] < n) {
+                const double2 xv = xs.p[j][idx[r]];
+                re += xv.x * yv[r].x + xv.y * yv[r].y;
+                im += xv.x * yv[r].y - xv.y * yv[r].x;
+            }
+        }
+        re = wave_sum(re);
+        im = wave_sum(im);
+        if (lane == 0) {
+            red_re[wave] = re;
+            red_im[wave] = im;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double sr = 0.0, si = 0.0;
+            for (int w = 0; w < kDotThreads / 64; ++w) {
+                sr += red_re[w];
+                si += red_im[w];
+            }
+            partial[(int64_t)j * gridDim.x + blockIdx.x] = double2{sr, si};
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void multidot_final(int nblk, const double2* __restrict__ partial,
+                               double2* __restrict__ out) {
+    const int j = blockIdx.x;
+    double re = 0.0, im = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 64) {
+        const double2 v = partial[(int64_t)j * nblk + b];
+        re += v.x;
+        im += v.y;
+    }
+    re = wave_sum(re);
+    im = wave_sum(im);
+    if (threadIdx.x == 0) out[j] = double2{re, im};
+}
+
+// ---- block copies -------------------------------------------------------------------------
+
+// dst(r, c) = scale * op(src)(r, c) for r < rows, c < cols; op by mode:
+// 0 copy, 1 transpose, 2 conj-transpose, 3 conj.  Optional real scale from `scale_base`:
+// scale_mode 1 multiplies row r by s[scale_off + r], 2 column c by s[scale_off + c].
+__global__ void copy_blocks_kernel(const CopyOp* __restrict__ ops, const double2* __restrict__ src,
+                                   double2* __restrict__ dst, const double* __restrict__ scale_base,
+                                   int accumulate) {
+    const CopyOp op = ops[blockIdx.x];
+    const int64_t total = (int64_t)op.rows * op.cols;
+    for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.y * blockDim.x) {
+        const int r = (int)(e % op.rows);
+        const int c = (int)(e / op.rows);
+        double2 v;
+        if (op.mode == 0 || op.mode == 3)
+            v = src[op.src_off + r + (int64_t)c * op.src_ld];
+        else
+            v = src[op.src_off + c + (int64_t)r * op.src_ld];
+        if (op.mode == 2 || op.mode == 3) v.y = -v.y;
+        if (op.scale_mode == 1) {
+            const double s = scale_base[op.scale_off + r];
+            v.x *= s;
+            v.y *= s;
+        } else if (op.scale_mode == 2) {
+            const double s = scale_base[op.scale_off + c];
+            v.x *= s;
+            v.y *= s;
+        }
+        double2* d = dst + op.dst_off + r + (int64_t)c * op.dst_ld;
+        if (accumulate) {
+            const double2 o = *d;
+            v.x += o.x;
+            v.y += o.y;
+        }
+        *d = v;
+    }
+}
+
+// squared Frobenius norm of each listed block (CopyOp fields: src_off, src_ld, rows, cols)
+__global__ void block_sqnorm_kernel(const CopyOp* __restrict__ ops, const double2* __restrict__ src,
+                                    double* __restrict__ out) {
+    __shared__ double red[4];
+    const CopyOp op = ops[blockIdx.x];
+    const int64_t total = (int64_t)op.rows * op.cols;
+    double acc = 0.0;
+    for (int64_t e = threadIdx.x; e < total; e += blockDim.x) {
+        const int r = (int)(e % op.rows);
+        const int c = (int)(e / op.rows);
+        const double2 v = src[op.src_off + r + (int64_t)c * op.src_ld];
+        acc += v.x * v.x + v.y * v.y;
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// ---- batched thin QR (Householder, unique gauge) ------------------------------------------
+
+constexpr int kQrThreads = 256;
+
+// A (m x n, lda) is overwritten; Q (m x k, ldq) and R (k x n, ldr), k = min(m, n).
+// R has a real non-negative diagonal (the unique thin QR of a full-rank A).
+__global__ __launch_bounds__(kQrThreads) void qr_kernel(const MatJob* __restrict__ jobs,
+                                                          double2* __restrict__ A,
+                                                          double2* __restrict__ Q,
+                                                          double2* __restrict__ R,
+                                                          double2* __restrict__ tau_ws) {
+    const MatJob job = jobs[blockIdx.x];
+    const int m = job.m, n = job.n, k = min(m, n);
+    double2* a = A + job.a_off;
+    double2* q = Q + job.q_off;
+    double2* rr = R + job.r_off;
+    double2* tau = tau_ws + job.w_off;
+    const int64_t lda = job.lda, ldq = job.ldq, ldr = job.ldr;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int nw = kQrThreads / 64;
+    __shared__ double2 s_tau;
+
+    for (int j = 0; j < k; ++j) {
+        double2* col = a + (int64_t)j * lda;
+        if (wave == 0) {
+            double x2 = 0.0;
+            for (int i = j + 1 + lane; i < m; i += 64) x2 += col[i].x * col[i].x + col[i].y * col[i].y;
+            x2 = wave_sum(x2);
+            const double2 alpha = col[j];
+            double2 t{0.0, 0.0};
+            if (!(x2 == 0.0 && alpha.y == 0.0)) {
+                const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + x2);
+                const double beta = alpha.x >= 0.0 ? -nrm : nrm;
+                t = double2{(beta - alpha.x) / beta, -alpha.y / beta};
+                // v = x / (alpha - beta)
+                const double dr = alpha.x - beta, di = alpha.y;
+                const double den = dr * dr + di * di;
+                const double sr = dr / den, si = -di / den;
+                for (int i = j + 1 + lane; i < m; i += 64) {
+                    const double2 v = col[i];
+                    col[i] = double2{v.x * sr - v.y * si, v.x * si + v.y * sr};
+                }
+                if (lane == 0) col[j] = double2{beta, 0.0};
+            }
+            if (lane == 0) {
+                s_tau = t;
+                tau[j] = t;
+            }
+        }
+        __syncthreads();
+        const double2 t = s_tau;
+        // A <- (I - conj(tau) v v^H) A on columns j+1 .. n-1
+        if (t.x != 0.0 || t.y != 0.0) {
+            for (int c = j + 1 + wave; c < n; c += nw) {
+                double2* ac = a + (int64_t)c * lda;
+                double wr = 0.0, wi = 0.0;
+                for (int i = j + lane; i < m; i += 64) {
+                    const double2 v = (i == j) ? double2{1.0, 0.0} : col[i];
+                    const double2 x = ac[i];
+                    wr += v.x * x.x + v.y * x.y;
+                    wi += v.x * x.y - v.y * x.x;
+                }
+                wr = wave_sum(wr);
+                wi = wave_sum(wi);
+                // f = conj(tau) * w
+                const double fr = t.x * wr + t.y * wi;
+                const double fi = t.x * wi - t.y * wr;
+                for (int i = j + lane; i < m; i += 64) {
+                    const double2 v = (i == j) ? double2{1.0, 0.0} : col[i];
+                    double2 x = ac[i];
+                    x.x -= fr * v.x - fi * v.y;
+                    x.y -= fr * v.y + fi * v.x;
+                    ac[i] = x;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // R = upper triangle (k x n)
+    for (int64_t e = threadIdx.x; e < (int64_t)k * n; e += kQrThreads) {
+        const int r = (int)(e % k), c = (int)(e / k);
+        rr[r + c * ldr] = (r <= c) ? a[r + (int64_t)c * lda] : double2{0.0, 0.0};
+    }
+    // Q = H_0 ... H_{k-1} [I_k; 0]
+    for (int64_t e = threadIdx.x; e < (int64_t)m * k; e += kQrThreads) {
+        const int r = (int)(e % m), c = (int)(e / m);
+        q[r + c * ldq] = double2{r == c ? 1.0 : 0.0, 0.0};
+    }
+    __syncthreads();
+    for (int j = k - 1; j >= 0; --j) {
+        const double2 t = tau[j];
+        if (t.x == 0.0 && t.y == 0.0) continue;
+        const double2* col = a + (int64_t)j * lda;
+        for (int c = j + wave; c < k; c += nw) {
+            double2* qc = q + (int64_t)c * ldq;
+            double wr = 0.0, wi = 0.0;
+            for (int i = j + lane; i < m; i += 64) {
+                const double2 v = (i == j) ? double2{1.0, 0.0} : col[i];
+                const double2 x = qc[i];
+                wr += v.x * x.x + v.y * x.y;
+                wi += v.x * x.y - v.y * x.x;
+            }
+            wr = wave_sum(wr);
+            wi = wave_sum(wi);
+            // f = tau * w
+            const double fr = t.x * wr - t.y * wi;
+            const double fi = t.x * wi + t.y * wr;
+            for (int i = j + lane; i < m; i += 64) {
+                const double2 v = (i == j) ? double2{1.0, 0.0} : col[i];
+                double2 x = qc[i];
+                x.x -= fr * v.x - fi * v.y;
+                x.y -= fr * v.y + fi * v.x;
+                qc[i] = x;
+            }
+        }
+        __syncthreads();
+    }
+    // unique gauge: make every R diagonal entry non-negative
+    for (int j = wave; j < k; j += nw) {
+        if (rr[j + (int64_t)j * ldr].x < 0.0) {
+            for (int c = j + lane; c < n; c += 64) {
+                double2 v = rr[j + (int64_t)c * ldr];
+                rr[j + (int64_t)c * ldr] = double2{-v.x, -v.y};
+            }
+            for (int i = lane; i < m; i += 64) {
+                double2 v = q[i + (int64_t)j * ldq];
+                q[i + (int64_t)j * ldq] = double2{-v.x, -v.y};
+            }
+        }
+    }
+}
+
+// ---- batched SVD: one-sided Jacobi ---------------------------------------------------------
+
+constexpr int kSvdThreads = 512;
+
+// G (m x n, m >= n, ldg) is the working copy (overwritten); V (n x n, ldv) accumulates the
+// rotations.  Outputs: U (m x n, ldu), S (n), Vh (n x n, ldvh), sorted by descending singular
+// value.  phase_on_v selects which factor carries the phase convention (first entry with
+// |x_i| >= 0.5 max|x| real positive): the columns of U (0) or the columns of V (1).
+__global__ __launch_bounds__(kSvdThreads) void svd_jacobi_kernel(const SvdJob* __restrict__ jobs,
+                                                                   double2* __restrict__ G,
+                                                                   double2* __restrict__ V,
+                                                                   double2* __restrict__ U,
+                                                                   double* __restrict__ S,
+                                                                   double2* __restrict__ Vh,
+                                                                   int max_sweeps) {
+    extern __shared__ double smem[];
+    const SvdJob job = jobs[blockIdx.x];
+    const int m = job.m, n = job.n;
+    double2* g = G + job.g_off;
+    double2* v = V + job.v_off;
+    const int64_t ldg = job.ldg, ldv = job.ldv;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int nw = kSvdThreads / 64;
+    double* sig = smem;                                    // n
+    int* perm = reinterpret_cast<int*>(smem + n);          // n
+    __shared__ int s_rotated;
+
+    for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += kSvdThreads) {
+        const int r = (int)(e % n), c = (int)(e / n);
+        v[r + c * ldv] = double2{r == c ? 1.0 : 0.0, 0.0};
+    }
+    const double tol = 2.220446049250313e-16 * sqrt((double)m);
+    const int N = n + (n & 1);
+    __syncthreads();
+    for (int sweep = 0; sweep < max_sweeps && n > 1; ++sweep) {
+        if (threadIdx.x == 0) s_rotated = 0;
+        __syncthreads();
+        for (int round = 0; round < N - 1; ++round) {
+            for (int pi = wave; pi < N / 2; pi += nw) {
+                int p, qq;
+                if (pi == 0) {
+                    p = round % (N - 1);
+                    qq = N - 1;
+                } else {
+                    p = (round + pi) % (N - 1);
+                    qq = (round - pi + (N - 1)) % (N - 1);
+                }
+                if (p >= n || qq >= n) continue;
+                if (p > qq) {
+                    const int t = p;
+                    p = qq;
+                    qq = t;
+                }
+                double2* gp = g + (int64_t)p * ldg;
+                double2* gq = g + (int64_t)qq * ldg;
+                double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+                for (int i = lane; i < m; i += 64) {
+                    const double2 a = gp[i], b = gq[i];
+                    al += a.x * a.x + a.y * a.y;
+                    be += b.x * b.x + b.y * b.y;
+                    gr += a.x * b.x + a.y * b.y;
+                    gi += a.x * b.y - a.y * b.x;
+                }
+                al = wave_sum(al);
+                be = wave_sum(be);
+                gr = wave_sum(gr);
+                gi = wave_sum(gi);
+                const double ag = sqrt(gr * gr + gi * gi);
+                if (!(ag > tol * sqrt(al * be)) || ag == 0.0) continue;
+                const double zeta = (be - al) / (2.0 * ag);
+                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / sqrt(1.0 + t * t);
+                const double s = c * t;
+                const double er = gr / ag, ei = gi / ag;  // e = gamma / |gamma|
+                // x_p' = c x_p - s conj(e) x_q ; x_q' = s e x_p + c x_q
+                for (int i = lane; i < m; i += 64) {
+                    const double2 a = gp[i], b = gq[i];
+                    gp[i] = double2{c * a.x - s * (er * b.x + ei * b.y),
+                                    c * a.y - s * (er * b.y - ei * b.x)};
+                    gq[i] = double2{s * (er * a.x - ei * a.y) + c * b.x,
+                                    s * (er * a.y + ei * a.x) + c * b.y};
+                }
+                double2* vp = v + (int64_t)p * ldv;
+                double2* vq = v + (int64_t)qq * ldv;
+                for (int i = lane; i < n; i += 64) {
+                    const double2 a = vp[i], b = vq[i];
+                    vp[i] = double2{c * a.x - s * (er * b.x + ei * b.y),
+                                    c * a.y - s * (er * b.y - ei * b.x)};
+                    vq[i] = double2{s * (er * a.x - ei * a.y) + c * b.x,
+                                    s * (er * a.y + ei * a.x) + c * b.y};
+                }
+                if (lane == 0) s_rotated = 1;
+            }
+            __syncthreads();
+        }
+        if (!s_rotated) break;
+        __syncthreads();
+    }
+    // singular values and descending order (ties by index)
+    for (int j = wave; j < n; j += nw) {
+        const double2* gj = g + (int64_t)j * ldg;
+        double s2 = 0.0;
+        for (int i = lane; i < m; i += 64) s2 += gj[i].x * gj[i].x + gj[i].y * gj[i].y;
+        s2 = wave_sum(s2);
+        if (lane == 0) sig[j] = sqrt(s2);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < n; j += kSvdThreads) {
+        int rank = 0;
+        const double sj = sig[j];
+        for (int i = 0; i < n; ++i) rank += (sig[i] > sj) || (sig[i] == sj && i < j);
+        perm[rank] = j;
+    }
+    __syncthreads();
+    double2* u = U + job.u_off;
+    double2* vh = Vh + job.vh_off;
+    double* sv = S + job.s_off;
+    const int64_t ldu = job.ldu, ldvh = job.ldvh;
+    for (int r = wave; r < n; r += nw) {
+        const int j = perm[r];
+        const double sj = sig[j];
+        const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
+        const double2* gj = g + (int64_t)j * ldg;
+        const double2* vj = v + (int64_t)j * ldv;
+        // phase of the pair from the chosen factor's column
+        const double2* ref = job.phase_on_v ? vj : gj;
+        const int len = job.phase_on_v ? n : m;
+        double mx = 0.0;
+        for (int i = lane; i < len; i += 64) mx = fmax(mx, sqrt(ref[i].x * ref[i].x + ref[i].y * ref[i].y));
+        mx = wave_max(mx);
+        int first = len;
+        for (int base = 0; base < len && first == len; base += 64) {
+            const int i = base + lane;
+            bool hit = false;
+            if (i < len && mx > 0.0) {
+                const double ai = sqrt(ref[i].x * ref[i].x + ref[i].y * ref[i].y);
+                hit = ai >= 0.5 * mx;
+            }
+            const unsigned long long bal = __ballot(hit);
+            if (bal) first = base + __ffsll((long long)bal) - 1;
+        }
+        double phr = 1.0, phi = 0.0;  // ph = ref[first] / |ref[first]|
+        if (first < len && (sj > 0.0 || job.phase_on_v)) {
+            const double2 x = ref[first];
+            const double ax = sqrt(x.x * x.x + x.y * x.y);
+            if (ax > 0.0) {
+                phr = x.x / ax;
+                phi = x.y / ax;
+            }
+        }
+        // U col *= conj(ph) (and / sigma), Vh row = conj(V col) * ph
+        for (int i = lane; i < m; i += 64) {
+            const double2 x = gj[i];
+            u[i + (int64_t)r * ldu] = double2{(x.x * phr + x.y * phi) * inv,
+                                              (x.y * phr - x.x * phi) * inv};
+        }
+        for (int i = lane; i < n; i += 64) {
+            const double2 x = vj[i];  // conj(x) * ph
+            vh[r + (int64_t)i * ldvh] = double2{x.x * phr + x.y * phi, x.x * phi - x.y * phr};
+        }
+        if (lane == 0) sv[r] = sj;
+    }
+}
+
+}  // namespace
+}  // namespace bug
+
+using namespace bug;
+
+namespace {
+hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+unsigned grid_for(int64_t n, int threads) {
+    const int64_t g = (n + threads - 1) / threads;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 65535 * 8));
+}
+
+double* pinned(bug_ctx* ctx, size_t bytes) {
+    if (ctx->pinned_bytes < bytes) {
+        if (ctx->pinned) HIP_CHECK(hipHostFree(ctx->pinned));
+        ctx->pinned = nullptr;
+        ctx->pinned_bytes = 0;
+        HIP_CHECK(hipHostMalloc(&ctx->pinned, std::max<size_t>(bytes, 4096)));
+        ctx->pinned_bytes = std::max<size_t>(bytes, 4096);
+    }
+    return static_cast<double*>(ctx->pinned);
+}
+
+template <typename T>
+T* upload_list(bug_ctx* ctx, hipStream_t st, const T* host, size_t count, int slot) {
+    void* dev = scratch(ctx, st, sizeof(T) * std::max<size_t>(count, 1), slot);
+    HIP_CHECK(hipMemcpyAsync(dev, host, sizeof(T) * count, hipMemcpyHostToDevice, st));
+    return static_cast<T*>(dev);
+}
+}  // namespace
+
+extern "C" {
+
+int bug_axpy(bug_ctx* ctx, int64_t n, double ar, double ai, const void* x, void* y, void* stream) {
+    return guarded(ctx, [&] {
+        if (n < 0 || (n > 0 && (!x || !y))) throw Error(BUG_EINVAL, "bad axpy arguments");
+        if (n == 0) return;
+        hipLaunchKernelGGL(zaxpy_kernel, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), n,
+                           ar, ai, static_cast<const double2*>(x), static_cast<double2*>(y));
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int bug_scal(bug_ctx* ctx, int64_t n, double ar, double ai, void* x, void* stream) {
+    return guarded(ctx, [&] {
+        if (n < 0 || (n > 0 && !x)) throw Error(BUG_EINVAL, "bad scal arguments");
+        if (n == 0) return;
+        hipLaunchKernelGGL(zscal_kernel, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), n,
+                           ar, ai, static_cast<double2*>(x));
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int bug_dotc(bug_ctx* ctx, int64_t n, int32_t k, const void* const* xs, const void* y,
+             double* out, void* stream) {
+    return guarded(ctx, [&] {
+        if (n < 0 || k < 0 || k > kMaxDots || (k > 0 && (!xs || !out)) || (n > 0 && !y))
+            throw Error(BUG_EINVAL, "bad dotc arguments (k must be <= 64)");
+        if (k == 0) return;
+        hipStream_t st = as_stream(stream);
+        DotPtrs p;
+        for (int j = 0; j < k; ++j) {
+            if (n > 0 && !xs[j]) throw Error(BUG_EINVAL, "null vector in dotc");
+            p.p[j] = static_cast<const double2*>(xs[j]);
+        }
+        const int64_t per_block = (int64_t)kDotThreads * kDotRows;
+        const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>((n + per_block - 1) / per_block, 4096));
+        double2* part = static_cast<double2*>(
+            scratch(ctx, st, sizeof(double2) * ((size_t)nblk * k + kMaxDots), 3));
+        double2* res = part + (size_t)nblk * k;
+        if (n > 0 && nblk * per_block < n) {
+            // more rows than one pass of 4096 blocks covers: fold in row chunks
+            throw Error(BUG_EINVAL, "dotc vector longer than 4096 * 1024 elements");
+        }
+        hipLaunchKernelGGL(multidot_partial, dim3(nblk), dim3(kDotThreads), 0, st, n, k, p,
+                           static_cast<const double2*>(y), part);
+        hipLaunchKernelGGL(multidot_final, dim3(k), dim3(64), 0, st, nblk, part, res);
+        HIP_CHECK(hipGetLastError());
+        double* h = pinned(ctx, sizeof(double2) * k);
+        HIP_CHECK(hipMemcpyAsync(h, res, sizeof(double2) * k, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        std::memcpy(out, h, sizeof(double2) * k);
+    });
+}
+
+int bug_copy_blocks(bug_ctx* ctx, int32_t nop, const bug_copy* ops, const void* src,
+                    int64_t src_nelem, void* dst, int64_t dst_nelem, const double* scale,
+                    int32_t accumulate, void* stream) {
+    return guarded(ctx, [&] {
+        if (nop < 0 || (nop > 0 && (!ops || !src || !dst))) throw Error(BUG_EINVAL, "bad copy list");
+        if (nop == 0) return;
+        std::vector<CopyOp> list(nop);
+        int64_t maxe = 0;
+        for (int i = 0; i < nop; ++i) {
+            const bug_copy& o = ops[i];
+            if (o.rows < 0 || o.cols < 0 || o.mode < 0 || o.mode > 3 || o.scale_mode < 0 ||
+                o.scale_mode > 2 || (o.scale_mode && !scale))
+                throw Error(BUG_EINVAL, "bad copy op");
+            if (o.rows == 0 || o.cols == 0) {
+                list[i] = CopyOp{0, 0, 1, 1, 0, 0, 0, 0, 0};
+                continue;
+            }
+            const bool plain = (o.mode == 0 || o.mode == 3);
+            const int64_t sr = plain ? o.rows : o.cols, sc = plain ? o.cols : o.rows;
+            if (o.src_off < 0 || o.dst_off < 0 || o.src_ld < sr || o.dst_ld < o.rows ||
+                o.src_off + (sc - 1) * (int64_t)o.src_ld + sr > src_nelem ||
+                o.dst_off + (int64_t)(o.cols - 1) * o.dst_ld + o.rows > dst_nelem)
+                throw Error(BUG_EINVARIANT, "copy block outside its buffer");
+            list[i] = CopyOp{o.src_off, o.dst_off, o.src_ld, o.dst_ld, o.rows, o.cols, o.mode,
+                             o.scale_mode, o.scale_off};
+            maxe = std::max<int64_t>(maxe, (int64_t)o.rows * o.cols);
+        }
+        hipStream_t st = as_stream(stream);
+        CopyOp* d = upload_list(ctx, st, list.data(), list.size(), 2);
+        const unsigned gy = (unsigned)std::max<int64_t>(1, std::min<int64_t>((maxe + 255) / 256, 64));
+        hipLaunchKernelGGL(copy_blocks_kernel, dim3(nop, gy), dim3(256), 0, st, d,
+                           static_cast<const double2*>(src), static_cast<double2*>(dst), scale,
+                           accumulate);
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int bug_block_sqnorms(bug_ctx* ctx, int32_t nblk, const bug_copy* blocks, const void* src,
+                      int64_t src_nelem, double* out, void* stream) {
+    return guarded(ctx, [&] {
+        if (nblk < 0 || (nblk > 0 && (!blocks || !src || !out))) throw Error(BUG_EINVAL, "bad block list");
+        if (nblk == 0) return;
+        std::vector<CopyOp> list(nblk);
+        for (int i = 0; i < nblk; ++i) {
+            const bug_copy& o = blocks[i];
+            if (o.rows < 0 || o.cols < 0) throw Error(BUG_EINVAL, "bad block dims");
+            if (o.rows > 0 && o.cols > 0 &&
+                (o.src_off < 0 || o.src_ld < o.rows ||
+                 o.src_off + (int64_t)(o.cols - 1) * o.src_ld + o.rows > src_nelem))
+                throw Error(BUG_EINVARIANT, "block outside its buffer");
+            list[i] = CopyOp{o.src_off, 0, o.src_ld, 1, o.rows, o.cols, 0, 0, 0};
+        }
+        hipStream_t st = as_stream(stream);
+        CopyOp* d = upload_list(ctx, st, list.data(), list.size(), 2);
+        double* res = static_cast<double*>(scratch(ctx, st, sizeof(double) * nblk, 3));
+        hipLaunchKernelGGL(block_sqnorm_kernel, dim3(nblk), dim3(256), 0, st, d,
+                           static_cast<const double2*>(src), res);
+        HIP_CHECK(hipGetLastError());
+        double* h = pinned(ctx, sizeof(double) * nblk);
+        HIP_CHECK(hipMemcpyAsync(h, res, sizeof(double) * nblk, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        std::memcpy(out, h, sizeof(double) * nblk);
+    });
+}
+
+int bug_block_qr(bug_ctx* ctx, int32_t nmat, const bug_qr_job* jobs, void* a, int64_t a_nelem,
+                 void* q, int64_t q_nelem, void* r, int64_t r_nelem, void* stream) {
+    return guarded(ctx, [&] {
+        if (nmat < 0 || (nmat > 0 && (!jobs || !a || !q || !r))) throw Error(BUG_EINVAL, "bad qr jobs");
+        if (nmat == 0) return;
+        std::vector<MatJob> list(nmat);
+        int64_t wtot = 0;
+        for (int i = 0; i < nmat; ++i) {
+            const bug_qr_job& j = jobs[i];
+            const int k = std::min(j.m, j.n);
+            if (j.m < 0 || j.n < 0 || j.lda < std::max(1, j.m) || j.ldq < std::max(1, j.m) ||
+                j.ldr < std::max(1, k))
+                throw Error(BUG_EINVAL, "bad qr job dims");
+            if (j.m > 0 && j.n > 0 &&
+                (j.a_off + (int64_t)(j.n - 1) * j.lda + j.m > a_nelem ||
+                 j.q_off + (int64_t)(k - 1) * j.ldq + j.m > q_nelem ||
+                 j.r_off + (int64_t)(j.n - 1) * j.ldr + k > r_nelem))
+                throw Error(BUG_EINVARIANT, "qr job outside its buffers");
+            list[i] = MatJob{j.a_off, j.q_off, j.r_off, wtot, j.m, j.n, j.lda, j.ldq, j.ldr};
+            wtot += k;
+        }
+        hipStream_t st = as_stream(stream);
+        MatJob* d = upload_list(ctx, st, list.data(), list.size(), 2);
+        double2* tau = static_cast<double2*>(scratch(ctx, st, sizeof(double2) * std::max<int64_t>(wtot, 1), 1));
+        hipLaunchKernelGGL(qr_kernel, dim3(nmat), dim3(kQrThreads), 0, st, d,
+                           static_cast<double2*>(a), static_cast<double2*>(q),
+                           static_cast<double2*>(r), tau);
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int bug_block_svd(bug_ctx* ctx, int32_t nmat, const bug_svd_job* jobs, void* g, int64_t g_nelem,
+                  void* u, int64_t u_nelem, double* s, int64_t s_nelem, void* vh, int64_t vh_nelem,
+                  int32_t max_sweeps, void* stream) {
+    return guarded(ctx, [&] {
+        if (nmat < 0 || (nmat > 0 && (!jobs || !g || !u || !s || !vh)))
+            throw Error(BUG_EINVAL, "bad svd jobs");
+        if (nmat == 0) return;
+        std::vector<SvdJob> list(nmat);
+        int64_t vtot = 0;
+        int nmax = 1;
+        for (int i = 0; i < nmat; ++i) {
+            const bug_svd_job& j = jobs[i];
+            if (j.m < j.n || j.n < 0 || j.ldg < std::max(1, j.m) || j.ldu < std::max(1, j.m) ||
+                j.ldvh < std::max(1, j.n))
+                throw Error(BUG_EINVAL, "svd job needs m >= n and valid leading dimensions");
+            if (j.n > 4096) throw Error(BUG_EINVAL, "svd block wider than 4096");
+            if (j.n > 0 &&
+                (j.g_off + (int64_t)(j.n - 1) * j.ldg + j.m > g_nelem ||
+                 j.u_off + (int64_t)(j.n - 1) * j.ldu + j.m > u_nelem ||
+                 j.s_off + j.n > s_nelem || j.vh_off + (int64_t)(j.n - 1) * j.ldvh + j.n > vh_nelem))
+                throw Error(BUG_EINVARIANT, "svd job outside its buffers");
+            list[i] = SvdJob{j.g_off, vtot, j.u_off, j.s_off, j.vh_off, j.m, j.n, j.ldg, j.n,
+                             j.ldu, j.ldvh, j.phase_on_v};
+            vtot += (int64_t)j.n * j.n;
+            nmax = std::max(nmax, j.n);
+        }
+        hipStream_t st = as_stream(stream);
+        SvdJob* d = upload_list(ctx, st, list.data(), list.size(), 2);
+        double2* v = static_cast<double2*>(scratch(ctx, st, sizeof(double2) * std::max<int64_t>(vtot, 1), 1));
+        const size_t lds = (size_t)nmax * (sizeof(double) + sizeof(int));
+        hipLaunchKernelGGL(svd_jacobi_kernel, dim3(nmat), dim3(kSvdThreads), lds, st, d,
+                           static_cast<double2*>(g), v, static_cast<double2*>(u), s,
+                           static_cast<double2*>(vh), max_sweeps > 0 ? max_sweeps : 60);
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+}  // extern "C"

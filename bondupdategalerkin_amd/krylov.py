@@ -1,0 +1,199 @@
+"""Host FP64 pieces of the Krylov solvers (the k x k side; k <= 65) and the device Krylov loops.
+
+Small dense work stays on the host by design, as in the reference: the tridiagonal
+eigendecomposition and coefficient evaluation of ``lanczos_expv`` (src/BUG/Core/LocalSolvers/
+krylov.jl:44-117,128-263), the substep search (root_methods.jl:86-130) and the residual integral
+that weighs each local Krylov direction (src/BUG/Core/RankAdaptation/krylov_enrichment.jl:58-120).
+Every vector operation (applies, inner products, axpys, recombination) runs on the device through
+``tensor_ops``.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Callable, List
+
+import numpy as np
+import scipy.linalg as sla
+
+from . import tensor_ops as T
+
+BREAKDOWN = 1e-13            # invariant-subspace threshold of the local chains
+MAXDEPTH = 64                # local Krylov depth bound
+ROOT_MAXITER = 30            # root basis per substep
+SKEW_TOL = 1e-12             # |tau H + (tau H)^dag| below which the propagator counts as unitary
+MAX_PANELS = 4096            # Simpson panels of one residual integral
+TAIL_NODES = 8               # nodes of the real-time tail guard
+SHIFT_TRIGGER = 30.0         # exponent past which the spectral shift switches on
+MAX_NORM = 1e150
+
+
+def residual_integral(defect: Callable[[np.ndarray], float], M: np.ndarray, tau: complex,
+                      relative: bool) -> float:
+    """|tau| int_0^1 defect(exp(sM) e1) ds by composite Simpson in log space."""
+    unitary = np.linalg.norm(M + M.conj().T) <= SKEW_TOL * np.linalg.norm(M)
+    sigma = 0.0 if unitary else float(np.max(np.linalg.eigvals(M).real))
+    A = M - sigma * np.eye(M.shape[0])
+    panels = 2 * min(max(math.ceil(2 * np.abs(A).sum(axis=0).max()), 8), MAX_PANELS)
+    h = 1.0 / panels
+    step = sla.expm(h * A)
+    c = np.zeros(M.shape[0], dtype=complex)
+    c[0] = 1.0
+    logw = np.empty(panels + 1)
+    for j in range(panels + 1):
+        if j:
+            c = step @ c
+        w = 1.0 if j in (0, panels) else (4.0 if j & 1 else 2.0)
+        d = defect(c)
+        logw[j] = -np.inf if d == 0 else math.log(w * h / 3 * d) + j * h * sigma
+    top = logw.max()
+    if top == -np.inf:
+        return 0.0
+    lw = math.log(abs(tau)) + top + math.log(np.exp(logw - top).sum())
+    if unitary or not relative:
+        return math.exp(lw)
+    return math.exp(lw - sigma - math.log(max(np.linalg.norm(c), np.finfo(float).tiny)))
+
+
+def direction_weight(H: np.ndarray, k: int, tau: complex) -> float:
+    """Weight of the (k+1)-th Krylov vector in exp(tau H) v1 (``_kry_weight``)."""
+    beta = abs(H[k, k - 1])
+    if beta == 0:
+        return 0.0
+    return beta * residual_integral(lambda c: abs(c[k - 1]), tau * H[:k, :k], tau, True)
+
+
+def tail_rank(kept, scale: float, allowance: float, kmax: int, base: float = 0.0):
+    """Smallest rank <= kmax whose scaled discarded tail fits (Test A, truncation.jl:262-278)."""
+    kmax = min(kmax, len(kept))
+    w = [e[0] * e[0] for e in kept]
+    if kmax <= 0:
+        return 0, scale * math.sqrt(base + sum(w))
+    tail = base + sum(w[kmax:])
+    r = kmax
+    while r > 0 and scale * math.sqrt(tail + w[r - 1]) <= allowance:
+        tail += w[r - 1]
+        r -= 1
+    return r, scale * math.sqrt(tail)
+
+
+class Tridiagonal:
+    """Eigendecomposition of the Lanczos tridiagonal, evaluated for any trial step."""
+
+    def __init__(self, alpha: List[float], beta: List[float]):
+        a, b = np.asarray(alpha, float), np.asarray(beta, float)
+        if not (np.isfinite(a).all() and np.isfinite(b).all()):
+            raise ValueError("Lanczos tridiagonal has non-finite entries: the input vector or "
+                             "the operator is not finite")
+        if len(a) == 1:
+            self.vals, self.vecs = a.copy(), np.ones((1, 1))
+        else:
+            self.vals, self.vecs = sla.eigh_tridiagonal(a, b)
+
+    def coeffs(self, tau: complex):
+        ex = np.real(tau * self.vals)
+        i = int(np.argmax(ex))
+        sigma = float(self.vals[i]) if ex[i] > SHIFT_TRIGGER else 0.0
+        w = np.exp(tau * (self.vals - sigma)) * self.vecs[0, :]
+        return self.vecs.astype(complex) @ w, tau * sigma
+
+
+def tail_sup(tri: Tridiagonal, step: complex, betak: float, above: float = math.inf) -> float:
+    tail = betak * abs(tri.coeffs(step)[0][-1])
+    if step.real != 0:
+        return tail
+    for j in range(1, TAIL_NODES):
+        if tail >= above:
+            return tail
+        tail = max(tail, betak * abs(tri.coeffs(step * (j / TAIL_NODES))[0][-1]))
+    return tail
+
+
+def pick_tau(tri: Tridiagonal, tau_rem: complex, betak: float, conv_tol: float) -> complex:
+    def ok(t):
+        v = tail_sup(tri, t, betak, conv_tol if conv_tol > 0 else math.inf)
+        return math.isfinite(v) and (conv_tol <= 0 or v <= conv_tol)
+
+    if ok(tau_rem):
+        return tau_rem
+    lo, hi = 0.0, 1.0
+    for _ in range(40):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if ok(mid * tau_rem) else (lo, mid)
+    return max(lo, 1e-6) * tau_rem
+
+
+@dataclass
+class ExpvReport:
+    iterations: int
+    substeps: int
+    err: float
+
+
+def lanczos_expv(apply, tau: complex, v0, *, tol: float = 1e-15, maxiter: int = ROOT_MAXITER,
+                 reorth: bool = True, conv_tol: float = 0.0, substeps: int = 0):
+    """exp(tau A) v0 for a Hermitian device action ``apply`` (``lanczos_expv``)."""
+    beta0 = T.norm(v0)
+    if beta0 == 0:
+        return v0, ExpvReport(0, 0, 0.0)
+    cur, tau_rem, logscale = v0, complex(tau), 0j
+    total = nsub = 0
+    err = 0.0
+    while True:
+        nsub += 1
+        cn = T.norm(cur)
+        if cn == 0:
+            break
+        basis = [T.scale(1.0 / cn, cur)]
+        alpha: List[float] = []
+        beta: List[float] = []
+        r = apply(basis[0])
+        total += 1
+        a = T.inner(basis[0], r).real
+        alpha.append(a)
+        T.axpy_(-a, basis[0], r)
+        for _ in range(2, maxiter + 1):
+            b = T.norm(r)
+            if b < tol:
+                break
+            if conv_tol > 0 and tail_sup(Tridiagonal(alpha, beta), tau_rem, b, conv_tol) < conv_tol:
+                break
+            beta.append(b)
+            basis.append(T.scale(1.0 / b, r))
+            r = apply(basis[-1])
+            total += 1
+            a = T.inner(basis[-1], r).real
+            alpha.append(a)
+            T.axpy_(-a, basis[-1], r)
+            T.axpy_(-b, basis[-2], r)
+            if reorth:
+                for _p in range(2):
+                    cs = T.inner_many(basis, r)
+                    for c, p in zip(cs, basis):
+                        T.axpy_(-c, p, r)
+        tri = Tridiagonal(alpha, beta)
+        betak = T.norm(r)
+        if substeps == 1:
+            tau_s = tau_rem
+        elif substeps > 1:
+            tau_s = tau_rem / (substeps - nsub + 1)
+        else:
+            tau_s = pick_tau(tri, tau_rem, betak, conv_tol)
+        coeff, ls = tri.coeffs(tau_s)
+        err += tail_sup(tri, tau_s, betak) * cn / beta0
+        out = T.scale(coeff[0] * cn, basis[0])
+        for c, p in zip(coeff[1:], basis[1:]):
+            T.axpy_(c * cn, p, out)
+        cur = out
+        logscale += ls
+        tau_rem -= tau_s
+        if substeps == 1 or abs(tau_rem) <= 1e-14 * max(abs(tau), 1.0):
+            break
+        if nsub >= 10000:
+            raise RuntimeError("expv: substep search failed to advance")
+    sc = np.exp(logscale)
+    if np.isfinite(abs(sc)) and sc != 0:
+        n = T.norm(cur)
+        if not (np.isfinite(n) and n > 0 and abs(sc) * n > MAX_NORM):
+            cur = T.scale(sc, cur)
+    return cur, ExpvReport(total, nsub, err)

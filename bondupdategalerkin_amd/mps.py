@@ -141,3 +141,34 @@ def random_mps(L: int, dmax: int, seed: int = 0, sigma: float = 4.0,
                 / np.sqrt(2.0 * max(1, legs[i].total))
         tensors.append(BlockTensor.from_blocks(lay, blocks))
     return SymMPS(tensors, charges, center=0)
+
+
+def magnetisation(psi: SymMPS, sz=(0.5, -0.5)):
+    """<S^z_j> / <psi|psi> for every site (``magnetisation``/``sz_expectation``,
+    src/QMB/Observables/local_observables.jl:11-31), any gauge.
+
+    The centre is walked across a copy of the state with unique-QR moves (the copy shares the
+    tensors, so ``psi`` itself is untouched); at each site the expectation is read from the
+    per-slice weights of the centre tensor."""
+    import numpy as np
+
+    from . import tensor_ops as T
+    from .bug_step import canonical
+
+    work = psi.copy()
+    canonical(work, 0)
+    out = np.zeros(len(work))
+    for j in range(len(work)):
+        if j > 0:
+            canonical(work, j)
+        A = work[j]
+        blks, slices = [], []
+        for k, q, qc, d, dc, off in A.layout.blocks():
+            blks.append((off, d, d, dc))
+            slices.append(k)
+        w = T.block_sqnorms(blks, A.data)
+        per = np.zeros(A.layout.nslice)
+        for k, v in zip(slices, w):
+            per[k] += v
+        out[j] = float(np.dot(per, sz[:len(per)]) / per.sum()) if per.sum() > 0 else 0.0
+    return out

@@ -1,0 +1,48 @@
+"""Host-side probe generation of the DD-projected sketches (drawn on the host, uploaded).
+
+The probe stream is counter based: number ``i`` of the stream is a complex Gaussian built by the
+Box-Muller transform from two 53-bit uniforms, splitmix64(key + 2i) and splitmix64(key + 2i + 1),
+scaled to E|z|^2 = 1 as Julia's ``randn(ComplexF64)``.  Being counter based, a probe block can be
+"drawn at full width and trimmed" (``_randomize_like`` + ``_trim_per_sector``,
+src/BUG/Randomised/cbe.jl:156-178 and truncation.jl:283-288) by skipping the unused numbers
+instead of generating them.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_MUL1 = np.uint64(0xBF58476D1CE4E5B9)
+_MUL2 = np.uint64(0x94D049BB133111EB)
+_GOLD = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _mix(x: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        z = x + _GOLD
+        z = (z ^ (z >> np.uint64(30))) * _MUL1
+        z = (z ^ (z >> np.uint64(27))) * _MUL2
+        return z ^ (z >> np.uint64(31))
+
+
+class ProbeStream:
+    """Seeded, skippable stream of complex Gaussian numbers."""
+
+    def __init__(self, seed: int):
+        self._key = _mix(np.array([seed], dtype=np.uint64))[0]
+        self.position = 0
+
+    def draw(self, n: int) -> np.ndarray:
+        i = np.arange(self.position, self.position + n, dtype=np.uint64)
+        with np.errstate(over="ignore"):
+            two_i = np.uint64(2) * i + self._key
+            a = _mix(two_i)
+            b = _mix(two_i + np.uint64(1))
+        u1 = ((a >> np.uint64(11)).astype(np.float64) + 0.5) / 9007199254740992.0
+        u2 = (b >> np.uint64(11)).astype(np.float64) / 9007199254740992.0
+        self.position += n
+        rad = np.sqrt(-np.log(u1))
+        ang = 2.0 * np.pi * u2
+        return rad * np.cos(ang) + 1j * rad * np.sin(ang)
+
+    def skip(self, n: int):
+        self.position += n

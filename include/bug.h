@@ -91,6 +91,29 @@ typedef struct bug_term {
     double alpha_re, alpha_im;
 } bug_term;
 
+/*
+ * One block copy dst(r, c) = op(src)(r, c) for r < rows, c < cols (dst is rows x cols).
+ * mode: 0 copy, 1 transpose, 2 conjugate transpose, 3 conjugate.  scale_mode 1 multiplies row r
+ * by scale[scale_off + r], 2 multiplies column c by scale[scale_off + c] (real scales).
+ */
+typedef struct bug_copy {
+    int64_t src_off, dst_off;
+    int32_t src_ld, dst_ld, rows, cols, mode, scale_mode;
+    int64_t scale_off;
+} bug_copy;
+
+/* One thin QR job: A (m x n, column-major, lda) -> Q (m x k) and R (k x n), k = min(m, n). */
+typedef struct bug_qr_job {
+    int64_t a_off, q_off, r_off;
+    int32_t m, n, lda, ldq, ldr, pad;
+} bug_qr_job;
+
+/* One SVD job, m >= n: G (m x n) -> U (m x n), S (n), Vh (n x n), descending. */
+typedef struct bug_svd_job {
+    int64_t g_off, u_off, s_off, vh_off;
+    int32_t m, n, ldg, ldu, ldvh, phase_on_v;
+} bug_svd_job;
+
 /* ---- context ---------------------------------------------------------------------------- */
 int bug_ctx_create(int device, bug_ctx** out);
 int bug_ctx_destroy(bug_ctx* ctx);
@@ -147,6 +170,53 @@ int bug_apply_one_site(bug_ctx* ctx, const bug_bsm* L, const bug_mpo_site* W, co
  */
 int bug_apply_zero_site(bug_ctx* ctx, const bug_bsm* L, const bug_bsm* R, const bug_bsm* C,
                         const bug_bsm* Y, double shift, int32_t precision, void* stream);
+
+/* ---- vectors (Krylov recurrences: tensor_inner, _cgs2!, _combine; krylov.jl:296-332) ------- */
+
+/* y += (ar + i ai) x over n complex elements. */
+int bug_axpy(bug_ctx* ctx, int64_t n, double ar, double ai, const void* x, void* y, void* stream);
+/* x *= (ar + i ai). */
+int bug_scal(bug_ctx* ctx, int64_t n, double ar, double ai, void* x, void* stream);
+/* out[2j], out[2j+1] = re, im of <xs[j] | y> for j < k <= 64 (host output; synchronizes the
+ * stream).  Replaces tensor_inner (src/BUG/Core/TensorNetworks/symmetric_mps.jl:83-93). */
+int bug_dotc(bug_ctx* ctx, int64_t n, int32_t k, const void* const* xs, const void* y,
+             double* out, void* stream);
+
+/* ---- block movement ------------------------------------------------------------------------ */
+
+/* Execute a list of block copies in one launch; accumulate != 0 adds onto dst.  Bounds of every
+ * block are checked against src_nelem / dst_nelem first (replaces Telum's permutedims, oplus and
+ * getsub on block-sparse tensors). */
+int bug_copy_blocks(bug_ctx* ctx, int32_t nop, const bug_copy* ops, const void* src,
+                    int64_t src_nelem, void* dst, int64_t dst_nelem, const double* scale,
+                    int32_t accumulate, void* stream);
+
+/* Squared Frobenius norms of listed blocks (src_off, src_ld, rows, cols of each bug_copy) into
+ * host out[nblk]; synchronizes the stream. */
+int bug_block_sqnorms(bug_ctx* ctx, int32_t nblk, const bug_copy* blocks, const void* src,
+                      int64_t src_nelem, double* out, void* stream);
+
+/* ---- factorizations -------------------------------------------------------------------------- */
+
+/*
+ * Batched thin QR with the unique gauge (real non-negative R diagonal), one job per charge
+ * sector; A is overwritten.  Stands in for the cutoff-0 SVDs of the exact gauge moves
+ * (move_right!/move_left!/canonical!, src/BUG/Core/TensorNetworks/canonical_form.jl:10-54;
+ * _frame_from, src/BUG/Core/Environments/effective_operators.jl:201-220).
+ */
+int bug_block_qr(bug_ctx* ctx, int32_t nmat, const bug_qr_job* jobs, void* a, int64_t a_nelem,
+                 void* q, int64_t q_nelem, void* r, int64_t r_nelem, void* stream);
+
+/*
+ * Batched SVD (one-sided Jacobi), one job per charge sector; G is overwritten.  Singular values
+ * descending; each singular pair's phase makes the first entry of the chosen factor's vector
+ * with |x_i| >= 0.5 max|x| real positive (phase_on_v selects V instead of U).  The per-sector
+ * factorizations of Telum's svd (deps/Telum/src/svd.jl:2013-2211); the global selection over
+ * sectors (svd.jl:1813-1921) is made by the caller on the returned spectra.
+ */
+int bug_block_svd(bug_ctx* ctx, int32_t nmat, const bug_svd_job* jobs, void* g, int64_t g_nelem,
+                  void* u, int64_t u_nelem, double* s, int64_t s_nelem, void* vh, int64_t vh_nelem,
+                  int32_t max_sweeps, void* stream);
 
 #ifdef __cplusplus
 }

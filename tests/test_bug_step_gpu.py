@@ -1,0 +1,88 @@
+"""GPU parity of the whole CBE-BUG step with the oracle on identical inputs and probe seeds.
+
+The bar: relative state difference <= 1e-10 per step (dense amplitude vectors, L <= 12), local
+magnetisation within 1e-9, identical budget verdicts and bond dimensions."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import bug as obug
+from oracle import models as om
+from tests.helpers import fam_from_device
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "b155550_xx_u1.json")
+
+
+def _device_state(name, L, sym="U1"):
+    from bondupdategalerkin_amd import mps
+
+    return {"neel": mps.neel_state, "dw": mps.domain_wall_state}[name](L, sym)
+
+
+def _oracle_state(name, L, sym="U1"):
+    return obug.MPS({"neel": om.neel_state, "dw": om.domain_wall_state}[name](L, sym))
+
+
+def _dense(psi_dev):
+    return om.mps_dense([fam_from_device(t) for t in psi_dev.tensors])
+
+
+@pytest.mark.parametrize("start,L,delta,rule,steps", [("neel", 10, 0.0, "maxdim", 3),
+                                                       ("dw", 10, 0.0, "bond", 3),
+                                                       ("neel", 8, 0.5, "bond", 3)])
+def test_step_matches_oracle(gpu, start, L, delta, rule, steps):
+    from bondupdategalerkin_amd import bug_step as B
+    from bondupdategalerkin_amd import mps
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    psi_d = _device_state(start, L)
+    psi_o = _oracle_state(start, L)
+    mpo_d, mpo_o = xxz_mpo(L, delta=delta), om.xxz_mpo(L, delta=delta)
+    for n in range(steps):
+        info_d = B.cbe_bug_step(psi_d, mpo_d, -0.05j, work_ceiling_rule=rule)
+        info_o = obug.cbe_bug_step(psi_o, mpo_o, -0.05j, obug.StepOptions(work_ceiling_rule=rule))
+        assert psi_d.bond_dims() == psi_o.bond_dims(), n
+        assert B.budget_status(info_d.ledger) == info_o.ledger.status, n
+        vd, vo = _dense(psi_d), om.mps_dense(psi_o.sites)
+        assert np.linalg.norm(vd - vo) / np.linalg.norm(vo) <= 1e-10, n
+        assert np.abs(mps.magnetisation(psi_d) - obug.sz_profile(psi_o)).max() <= 1e-9, n
+
+
+def test_frozen_order1_run_on_device(gpu):
+    """The device step replays the reference's frozen run (xx_u1_bug_order1) at its tolerances."""
+    from bondupdategalerkin_amd import bug_step as B
+    from bondupdategalerkin_amd import mps
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    fx = json.load(open(GOLDEN))["xx_u1_bug_order1"]
+    L, dt = fx["L"], fx["dt"]
+    psi = mps.neel_state(L)
+    mpo = xxz_mpo(L, delta=0.0)
+    ex = B.DDQRExpansion(krylov_mode="arnoldi")
+    for n, row in enumerate(fx["steps"][:5], start=1):
+        info = B.cbe_bug_step(psi, mpo, -1j * dt, work_ceiling_rule="maxdim", expansion=ex)
+        sz = mps.magnetisation(psi)
+        assert np.abs(sz - np.array(row["sz"])).max() <= 1e-5
+        assert B.budget_status(info.ledger) == row["status"]
+
+
+def test_invalid_options_refused_before_touching_state(gpu):
+    from bondupdategalerkin_amd import bug_step as B
+    from bondupdategalerkin_amd import mps
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    psi = mps.neel_state(6)
+    before = [t.data.clone() for t in psi.tensors]
+    with pytest.raises(ValueError):
+        B.cbe_bug_step(psi, xxz_mpo(6), -0.1j, work_ceiling_rule="nope")
+    with pytest.raises(ValueError):
+        B.cbe_bug_step(psi, xxz_mpo(5), -0.1j)
+    with pytest.raises(ValueError):
+        B.cbe_bug_step(psi, xxz_mpo(6), -0.1j, tol_root=1e-8, tol_expv=1e-8)
+    with pytest.raises(NotImplementedError):
+        B.cbe_bug_step(psi, xxz_mpo(6), -0.1j, order=2)
+    for a, t in zip(before, psi.tensors):
+        assert bool((a == t.data).all())
