@@ -1,11 +1,13 @@
 """Benchmark of the BUG hot path on one MI355X.
 
 Prints ONE JSON line (rank 0).  Workload: the XXZ chain (delta = 0.5) at L = 128 on a seeded
-synthetic U(1) state with bond dimension up to D = 1024 (Gaussian charge-sector weights, sigma 4).
+synthetic U(1) state with bond dimension up to D = 1024 (Gaussian charge-sector weights, sigma 4),
+canonicalised and normalised on the device.
 
-``--workload heff_sweep``: one step = a left-to-right pass over all L sites doing the one-site
-effective-Hamiltonian apply and the left environment push at every site (the contraction core
-of a half-sweep), on device-resident tensors.
+``--workload bug_step`` (default): one step = one ``cbe_bug_step`` (order 1, tau = -0.02 i,
+maxdim = D, work_ceiling_rule = maxdim, default tolerances) on the device-resident state.
+``--workload heff_sweep``: one step = a left-to-right pass doing the one-site effective-Hamiltonian
+apply and the left environment push at every site (the contraction core of a half-sweep).
 
 The dominant kernel (grouped complex block GEMM of the one-site apply) is timed separately with
 HIP events on its own stream for the roofline; the CPU baseline times the oracle's one-site apply
@@ -33,13 +35,13 @@ HBM_PEAK = 8000.0                      # GB/s
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--L", type=int, default=128)
     p.add_argument("--D", type=int, default=1024)
     p.add_argument("--delta", type=float, default=0.5)
     p.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    p.add_argument("--workload", choices=["heff_sweep"], default="heff_sweep")
+    p.add_argument("--workload", choices=["bug_step", "heff_sweep"], default="bug_step")
     p.add_argument("--kernel-reps", type=int, default=20)
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
@@ -102,17 +104,35 @@ def main():
         return
 
     # ---- the timed step ----
-    def step():
-        lenv = lenvs[0]
-        for i in range(L):
-            R = renvs[i + 1] if i + 1 < len(renvs) and renvs[i + 1] is not None else None
-            if R is not None:
-                ev.apply_one_site(lenv, mpo[i], R, psi[i], precision=prec)
-            lenv = ev.push_left(lenv, psi[i], mpo[i], precision=prec)
+    statuses = []
+    if args.workload == "bug_step":
+        from bondupdategalerkin_amd import bug_step as B
+        from bondupdategalerkin_amd import tensor_ops as T
+
+        B.canonical(psi, L - 1)
+        T.scale_(1.0 / T.norm(psi[L - 1]), psi[L - 1])
+        B.canonical(psi, 0)
+        torch.cuda.synchronize()
+
+        def step():
+            info = B.cbe_bug_step(psi, mpo, -0.02j, maxdim=D, work_ceiling_rule="maxdim")
+            statuses.append(B.budget_status(info.ledger))
+            print(f"# step: {info.seconds['step']:.2f}s {B.budget_status(info.ledger)} "
+                  f"{info.ledger.unmet} max bond {max(psi.bond_dims())}", file=sys.stderr,
+                  flush=True)
+    else:
+        def step():
+            lenv = lenvs[0]
+            for i in range(L):
+                R = renvs[i + 1] if i + 1 < len(renvs) and renvs[i + 1] is not None else None
+                if R is not None:
+                    ev.apply_one_site(lenv, mpo[i], R, psi[i], precision=prec)
+                lenv = ev.push_left(lenv, psi[i], mpo[i], precision=prec)
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    statuses.clear()
     t1 = time.perf_counter()
     for _ in range(args.steps):
         step()
@@ -137,10 +157,14 @@ def main():
         "vs_baseline": None,
         "dtype": "c128" if args.precision == "fp64" else "c64-mfma/c128-storage",
         "data": "synthetic",
-        "config": {"workload": "heff_sweep: one-site H_eff apply + left env push at every site "
-                               "(full BUG step not measured by this line)",
+        "config": {"workload": ("cbe_bug_step order 1 (the order-2 closure is not built), "
+                                "tau=-0.02i, maxdim=D, work_ceiling_rule=maxdim, synthetic "
+                                "canonical U(1) state")
+                   if args.workload == "bug_step" else
+                   "heff_sweep: one-site H_eff apply + left env push at every site",
                    "L": L, "D": D, "model": f"xxz delta={args.delta}", "symmetry": "U1",
-                   "precision": args.precision, "parallelism": "single GPU"},
+                   "precision": args.precision, "parallelism": "single GPU",
+                   "budget_status": statuses},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -23,6 +23,8 @@ Stages (0-based sites; bond b joins sites b and b+1; link j sits left of site j)
 from __future__ import annotations
 
 import math
+import os
+import sys
 import time
 from dataclasses import dataclass, field
 from typing import Callable, Dict, List, Optional
@@ -40,6 +42,13 @@ from .probes import ProbeStream
 
 CBE_EPS = T.CBE_EPS
 TOL_DEFAULTS = dict(basis=1e-5, trunc=1e-7, expv=1e-10)
+# BUG_PROGRESS=1 prints one line per 16 bonds of each half-sweep to stderr (long steps).
+_PROGRESS = os.environ.get("BUG_PROGRESS", "") == "1"
+
+
+def _progress(msg: str):
+    if _PROGRESS:
+        print(f"# {time.strftime('%H:%M:%S')} {msg}", file=sys.stderr, flush=True)
 DEFAULT_BOND_CEILING_FACTOR = 1.2
 
 
@@ -433,17 +442,14 @@ def dd_cbe_expand(fr: BondFrame, mpo: MPO, bond: int, lenv, renv, side: str, all
         return none
     rmax = min(cap, _width(Om, oside))
     cache = {}
+    fac = T.factor_left(Y) if left else T.factor_right(Y)
 
     def candidate(k):
         if k <= 0:
             return None
         if k not in cache:
-            if left:
-                r = T.svd_left(Y, cutoff=CBE_EPS, nkeep=k)
-                cache[k] = reortho(frame, r.U, side) if r.kept else None
-            else:
-                r = T.svd_right(Y, cutoff=CBE_EPS, nkeep=k)
-                cache[k] = reortho(frame, r.Vh, side) if r.kept else None
+            r = fac.select(CBE_EPS, k)
+            cache[k] = (reortho(frame, r.U if left else r.Vh, side) if r.kept else None)
         return cache[k]
 
     limit = ex.tail_factor * alloc.sample
@@ -489,15 +495,15 @@ def extend_frame(frame, residual, direction, side, weight, allowance, room):
     """``_extend_frame`` (krylov_enrichment.jl:135-171) -> (frame, added, fits)."""
     rn = T.norm(residual)
     cut = CBE_EPS * max(T.norm(direction), rn) / rn
-    svd = T.svd_left if side == "left" else T.svd_right
-    res = svd(residual, cutoff=cut)
+    fac = T.factor_left(residual) if side == "left" else T.factor_right(residual)
+    res = fac.select(cut)
     nk = len(res.kept)
     keep, tail = K.tail_rank(res.kept, weight, allowance, min(nk, room), base=T.sq_weight(res.trunc))
     fits = tail <= allowance
     if keep == 0:
         return frame, 0, fits
     if keep < nk:
-        res = svd(residual, cutoff=cut, nkeep=keep)
+        res = fac.select(cut, keep)
     new = reortho(frame, res.U if side == "left" else res.Vh, side)
     new = cap_to_local_room(frame, new, side)
     if new is None:
@@ -679,6 +685,8 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     lroot = lch
     carry = None
     for b in range(rl + 1):
+        if b % 16 == 0:
+            _progress(f"left half-sweep bond {b}/{rl}")
         carried = psiL[b] if carry is None else T.core_times_site(carry, psiL[b])
         fr = frame_from(carried, psiL[b + 1])
         xp = dd_cbe_expand(fr, mpo, b, lch, renvs[b + 2], "left", alloc, ledger, wcap[b], ex,
@@ -709,6 +717,8 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     rroot = rch
     carry = None
     for b in range(L - 2, rl - 1, -1):
+        if b % 16 == 0:
+            _progress(f"right half-sweep bond {b}/{rl}")
         carried = psiR[b + 1] if carry is None else T.site_times_core(psiR[b + 1], carry)
         fr = frame_from(psiR[b], carried)
         xp = dd_cbe_expand(fr, mpo, b, lenvs[b], rch, "right", alloc, ledger, wcap[b], ex,
@@ -730,6 +740,7 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     AR = carry
     secs["right_sweep"] = time.perf_counter() - ts
 
+    _progress("root start and projection miss")
     # ---- root start and projection miss ----
     AL = None
     for i in range(rl + 1):
@@ -759,6 +770,7 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         ledger.fail("expv")
     secs["root"] = time.perf_counter() - ts
 
+    _progress("assembly and compression")
     # ---- assembly and compression ----
     ts = time.perf_counter()
     res = T.svd_core(result, cutoff=0.0)
@@ -818,8 +830,8 @@ def truncate_recursive(psi: SymMPS, root: int, maxdim: int, weight_tol: float):
     cum = [0.0, 0.0]
 
     def cut(t, left_split):
-        svd = T.svd_left if left_split else T.svd_right
-        res = svd(t, cutoff=0.0, nkeep=maxdim)
+        fac = T.factor_left(t) if left_split else T.factor_right(t)
+        res = fac.select(0.0, maxdim)
         tot = T.sq_weight(res.kept) + T.sq_weight(res.trunc)
         if st["nrm2"] < 0:
             st["nrm2"] = tot
@@ -830,7 +842,7 @@ def truncate_recursive(psi: SymMPS, root: int, maxdim: int, weight_tol: float):
             st["capped"] = True
         keep = max(keep, 1)
         if keep < nk:
-            res = svd(t, cutoff=0.0, nkeep=keep)
+            res = fac.select(0.0, keep)
         return res
 
     def account(res, side):

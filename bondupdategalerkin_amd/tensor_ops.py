@@ -569,8 +569,71 @@ def _gather_plain(cl: CopyList, batch: _SvdBatch, i: int, off: int, ld: int, r0:
         cl.add(off, ld, g + c0 + r0 * n, n, cols, rows, mode=2)
 
 
-def svd_left(A: BlockTensor, cutoff: float = 0.0, nkeep: Optional[int] = None) -> SVDResult:
-    """A = U S Vh with U over (link, site): U (A.row, bond), S {q: sv}, Vh core (bond, A.col)."""
+class BlockSVD:
+    """Per-sector SVD factors of one matricisation, kept on the device.
+
+    Any number of selections (``select``) can be read from one factorization: a truncated SVD is
+    a prefix of the full one, so the reference's repeated ``svd(...; Nkeep = k)`` calls on the
+    same tensor (rank bisections, tail-rank re-cuts) cost one factorization here."""
+
+    def __init__(self, kind: str, layout: Layout, qlist, secs, mats, batch: _SvdBatch):
+        self.kind, self.layout, self.qlist, self.secs = kind, layout, qlist, secs
+        self.mats, self.batch = mats, batch
+        self.spectra = {q: batch.spectra[i] for i, q in enumerate(qlist)}
+
+    def select(self, cutoff: float = 0.0, nkeep: Optional[int] = None) -> SVDResult:
+        keep, kept, trunc = select(self.spectra, cutoff, nkeep)
+        lay, batch = self.layout, self.batch
+        bond = Leg.make((q, keep[q]) for q in self.qlist if keep[q] > 0)
+        if self.kind == "left":
+            U = zeros(Layout(lay.row, bond, lay.shifts))
+            Vh = zeros(Layout(bond, lay.col, [0]))
+        elif self.kind == "right":
+            U = zeros(Layout(lay.row, bond, [0]))
+            Vh = zeros(Layout(bond, lay.col, lay.shifts))
+        else:
+            U = zeros(Layout(lay.row, bond, [0]))
+            Vh = zeros(Layout(bond, lay.col, [0]))
+        ut, vt = _block_table(U.layout), _block_table(Vh.layout)
+        cu_u, cu_v, cv_u, cv_v = CopyList(), CopyList(), CopyList(), CopyList()
+        S = {}
+        for i, q in enumerate(self.qlist):
+            k = keep[q]
+            if k == 0:
+                continue
+            S[q] = batch.spectra[i][:k]
+            tall = batch.tall[i]
+            m, n = self.mats[i]
+            if self.kind == "left":
+                r0 = 0
+                for s, ql, rows, off in self.secs[q]:
+                    o = ut[(s, ql)]
+                    batch.add_u_rows(cu_u if tall else cu_v, i, r0, rows, k, o[0], o[1])
+                    r0 += rows
+                o = vt[(0, q)]
+                batch.add_vh_cols(cv_v if tall else cv_u, i, 0, n, k, o[0], o[1])
+            elif self.kind == "right":
+                o = ut[(0, q)]
+                batch.add_u_rows(cu_u if tall else cu_v, i, 0, m, k, o[0], o[1])
+                c0 = 0
+                for s, qr, cols, off in self.secs[q]:
+                    o = vt[(s, q)]
+                    batch.add_vh_cols(cv_v if tall else cv_u, i, c0, cols, k, o[0], o[1])
+                    c0 += cols
+            else:
+                o = ut[(0, q)]
+                batch.add_u_rows(cu_u if tall else cu_v, i, 0, m, k, o[0], o[1])
+                o = vt[(0, q)]
+                batch.add_vh_cols(cv_v if tall else cv_u, i, 0, n, k, o[0], o[1])
+        cu_u.run(batch.U, U.data)
+        cu_v.run(batch.Vh, U.data)
+        cv_v.run(batch.Vh, Vh.data)
+        cv_u.run(batch.U, Vh.data)
+        return SVDResult(U, S, Vh, kept, trunc)
+
+
+def factor_left(A: BlockTensor) -> BlockSVD:
+    """Per-sector SVDs of the left matricisation (link, site) | bond."""
     lay = A.layout
     secs = left_sectors(lay)
     cold = lay.col.as_dict()
@@ -585,35 +648,11 @@ def svd_left(A: BlockTensor, cutoff: float = 0.0, nkeep: Optional[int] = None) -
             r0 += rows
     g.run(A.data, batch.ws)
     batch.run()
-    spectra = {q: batch.spectra[i] for i, q in enumerate(qlist)}
-    keep, kept, trunc = select(spectra, cutoff, nkeep)
-    bond = Leg.make((q, keep[q]) for q in qlist if keep[q] > 0)
-    U = zeros(Layout(lay.row, bond, lay.shifts))
-    Vh = zeros(Layout(bond, lay.col, [0]))
-    ut, vt = _block_table(U.layout), _block_table(Vh.layout)
-    cu_u, cu_v, cv_u, cv_v = CopyList(), CopyList(), CopyList(), CopyList()
-    S = {}
-    for i, q in enumerate(qlist):
-        k = keep[q]
-        if k == 0:
-            continue
-        S[q] = batch.spectra[i][:k]
-        r0 = 0
-        for s, ql, rows, off in secs[q]:
-            o = ut[(s, ql)]
-            batch.add_u_rows(cu_u if batch.tall[i] else cu_v, i, r0, rows, k, o[0], o[1])
-            r0 += rows
-        o = vt[(0, q)]
-        batch.add_vh_cols(cv_v if batch.tall[i] else cv_u, i, 0, mats[i][1], k, o[0], o[1])
-    cu_u.run(batch.U, U.data)
-    cu_v.run(batch.Vh, U.data)
-    cv_v.run(batch.Vh, Vh.data)
-    cv_u.run(batch.U, Vh.data)
-    return SVDResult(U, S, Vh, kept, trunc)
+    return BlockSVD("left", lay, qlist, secs, mats, batch)
 
 
-def svd_right(A: BlockTensor, cutoff: float = 0.0, nkeep: Optional[int] = None) -> SVDResult:
-    """A = U S Vh with Vh over (site, link): U core (A.row, bond), Vh (bond, s, r)."""
+def factor_right(A: BlockTensor) -> BlockSVD:
+    """Per-sector SVDs of the right matricisation bond | (site, link)."""
     lay = A.layout
     secs = right_sectors(lay)
     rowd = lay.row.as_dict()
@@ -628,38 +667,14 @@ def svd_right(A: BlockTensor, cutoff: float = 0.0, nkeep: Optional[int] = None) 
             c0 += cols
     g.run(A.data, batch.ws)
     batch.run()
-    spectra = {q: batch.spectra[i] for i, q in enumerate(qlist)}
-    keep, kept, trunc = select(spectra, cutoff, nkeep)
-    bond = Leg.make((q, keep[q]) for q in qlist if keep[q] > 0)
-    U = zeros(Layout(lay.row, bond, [0]))
-    Vh = zeros(Layout(bond, lay.col, lay.shifts))
-    ut, vt = _block_table(U.layout), _block_table(Vh.layout)
-    S = {}
-    cu_u, cu_v, cv_u, cv_v = CopyList(), CopyList(), CopyList(), CopyList()
-    for i, q in enumerate(qlist):
-        k = keep[q]
-        if k == 0:
-            continue
-        S[q] = batch.spectra[i][:k]
-        o = ut[(0, q)]
-        batch.add_u_rows(cu_u if batch.tall[i] else cu_v, i, 0, mats[i][0], k, o[0], o[1])
-        c0 = 0
-        for s, qr, cols, off in secs[q]:
-            o = vt[(s, q)]
-            batch.add_vh_cols(cv_v if batch.tall[i] else cv_u, i, c0, cols, k, o[0], o[1])
-            c0 += cols
-    cu_u.run(batch.U, U.data)
-    cu_v.run(batch.Vh, U.data)
-    cv_v.run(batch.Vh, Vh.data)
-    cv_u.run(batch.U, Vh.data)
-    return SVDResult(U, S, Vh, kept, trunc)
+    return BlockSVD("right", lay, qlist, secs, mats, batch)
 
 
-def svd_core(C: BlockTensor, cutoff: float = 0.0, nkeep: Optional[int] = None) -> SVDResult:
-    """C = U S Vh per sector of a one-slice core."""
+def factor_core(C: BlockTensor) -> BlockSVD:
+    """Per-sector SVDs of a one-slice core."""
     lay = C.layout
     tab = _block_table(lay)
-    keys = [k for k in tab]
+    keys = list(tab)
     mats = [(tab[k][1], tab[k][2]) for k in keys]
     batch = _SvdBatch(mats)
     g = CopyList()
@@ -668,29 +683,22 @@ def svd_core(C: BlockTensor, cutoff: float = 0.0, nkeep: Optional[int] = None) -
         _gather_plain(g, batch, i, off, d, 0, 0, d, dc)
     g.run(C.data, batch.ws)
     batch.run()
-    qlist = [k[1] for k in keys]
-    spectra = {q: batch.spectra[i] for i, q in enumerate(qlist)}
-    keep, kept, trunc = select(spectra, cutoff, nkeep)
-    bond = Leg.make((q, keep[q]) for q in qlist if keep[q] > 0)
-    U = zeros(Layout(lay.row, bond, [0]))
-    Vh = zeros(Layout(bond, lay.col, [0]))
-    ut, vt = _block_table(U.layout), _block_table(Vh.layout)
-    S = {}
-    cu_u, cu_v, cv_u, cv_v = CopyList(), CopyList(), CopyList(), CopyList()
-    for i, q in enumerate(qlist):
-        k = keep[q]
-        if k == 0:
-            continue
-        S[q] = batch.spectra[i][:k]
-        o = ut[(0, q)]
-        batch.add_u_rows(cu_u if batch.tall[i] else cu_v, i, 0, mats[i][0], k, o[0], o[1])
-        o = vt[(0, q)]
-        batch.add_vh_cols(cv_v if batch.tall[i] else cv_u, i, 0, mats[i][1], k, o[0], o[1])
-    cu_u.run(batch.U, U.data)
-    cu_v.run(batch.Vh, U.data)
-    cv_v.run(batch.Vh, Vh.data)
-    cv_u.run(batch.U, Vh.data)
-    return SVDResult(U, S, Vh, kept, trunc)
+    return BlockSVD("core", lay, [k[1] for k in keys], None, mats, batch)
+
+
+def svd_left(A: BlockTensor, cutoff: float = 0.0, nkeep: Optional[int] = None) -> SVDResult:
+    """A = U S Vh with U over (link, site): U (A.row, bond), S {q: sv}, Vh core (bond, A.col)."""
+    return factor_left(A).select(cutoff, nkeep)
+
+
+def svd_right(A: BlockTensor, cutoff: float = 0.0, nkeep: Optional[int] = None) -> SVDResult:
+    """A = U S Vh with Vh over (site, link): U core (A.row, bond), Vh (bond, s, r)."""
+    return factor_right(A).select(cutoff, nkeep)
+
+
+def svd_core(C: BlockTensor, cutoff: float = 0.0, nkeep: Optional[int] = None) -> SVDResult:
+    """C = U S Vh per sector of a one-slice core."""
+    return factor_core(C).select(cutoff, nkeep)
 
 
 def sq_weight(lst) -> float:
