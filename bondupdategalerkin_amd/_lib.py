@@ -138,6 +138,8 @@ SIGNATURES = {
         [_P, ctypes.c_int32, ctypes.POINTER(BugQrJob), _P, ctypes.c_int64, _P, ctypes.c_int64, _P,
          ctypes.c_int64, _P],
     ),
+    "bug_set_svd_variant": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
+    "bug_svd_sweeps": (ctypes.c_int, [_P, ctypes.c_int32, _i32p]),
     "bug_block_svd": (
         ctypes.c_int,
         [_P, ctypes.c_int32, ctypes.POINTER(BugSvdJob), _P, ctypes.c_int64, _P, ctypes.c_int64,

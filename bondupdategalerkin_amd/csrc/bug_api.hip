@@ -15,17 +15,7 @@ namespace bug {
 
 // ---- hashing of layouts -----------------------------------------------------------------
 
-struct Hasher {
-    uint64_t h = 1469598103934665603ull;
-    void bytes(const void* p, size_t n) {
-        const unsigned char* c = static_cast<const unsigned char*>(p);
-        for (size_t i = 0; i < n; ++i) {
-            h ^= c[i];
-            h *= 1099511628211ull;
-        }
-    }
-    template <typename T>
-    void pod(const T& v) { bytes(&v, sizeof(T)); }
+struct BsmHasher : Hasher {
     void leg(const bug_leg& l) {
         pod(l.nsec);
         if (l.nsec > 0) {
@@ -304,7 +294,7 @@ void upload_plan(Plan& plan, int config) {
 void run_product(bug_ctx* ctx, const Bsm& out, void* out_dev, const Bsm& P, const void* p_dev,
                  const Bsm& Q, const void* q_dev, const std::vector<bug_term>& terms,
                  bool accumulate, int precision, hipStream_t stream, uint64_t tag) {
-    Hasher h;
+    BsmHasher h;
     h.pod(tag);
     {
         bug_bsm vo = out.view(nullptr), vp = P.view(nullptr), vq = Q.view(nullptr);
@@ -315,7 +305,12 @@ void run_product(bug_ctx* ctx, const Bsm& out, void* out_dev, const Bsm& P, cons
     }
     for (const bug_term& t : terms) h.pod(t);
     h.pod(accumulate);
-    const uint64_t key = h.h;
+    launch_cached_plan(ctx, h.h, [&](Plan& p) { build_product_plan(p, out, P, Q, terms, accumulate); },
+                       precision, p_dev, q_dev, out_dev, stream);
+}
+
+void launch_cached_plan(bug_ctx* ctx, uint64_t key, const std::function<void(Plan&)>& build,
+                        int precision, const void* A, const void* B, void* C, hipStream_t stream) {
     Plan* plan = nullptr;
     auto it = ctx->plan_index.find(key);
     if (it != ctx->plan_index.end()) {
@@ -326,7 +321,7 @@ void run_product(bug_ctx* ctx, const Bsm& out, void* out_dev, const Bsm& P, cons
         ++ctx->plan_misses;
         plan = new Plan();
         try {
-            build_product_plan(*plan, out, P, Q, terms, accumulate);
+            build(*plan);
         } catch (...) {
             delete plan;
             throw;
@@ -344,7 +339,7 @@ void run_product(bug_ctx* ctx, const Bsm& out, void* out_dev, const Bsm& P, cons
     const int config = ctx->forced_config >= 0 ? ctx->forced_config : choose_config(*plan);
     upload_plan(*plan, config);
     ctx->last_flops += plan->flops;
-    HIP_CHECK(launch_grouped_gemm(*plan, config, precision, p_dev, q_dev, out_dev, stream));
+    HIP_CHECK(launch_grouped_gemm(*plan, config, precision, A, B, C, stream));
 }
 
 void* scratch(bug_ctx* ctx, hipStream_t stream, size_t bytes, int slot) {

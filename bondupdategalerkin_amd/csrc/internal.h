@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <list>
 #include <mutex>
 #include <stdexcept>
@@ -110,6 +111,24 @@ struct TileConfig {
 extern const TileConfig kTileConfigs[];
 extern const int kNumTileConfigs;
 
+// Launch a grouped-GEMM plan cached under `key` (built by `build` on a miss) on A, B -> C.
+void launch_cached_plan(bug_ctx* ctx, uint64_t key, const std::function<void(Plan&)>& build,
+                        int precision, const void* A, const void* B, void* C, hipStream_t stream);
+
+// FNV-1a hash of plan signatures.
+struct Hasher {
+    uint64_t h = 1469598103934665603ull;
+    void bytes(const void* p, size_t n) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; ++i) {
+            h ^= c[i];
+            h *= 1099511628211ull;
+        }
+    }
+    template <typename T>
+    void pod(const T& v) { bytes(&v, sizeof(T)); }
+};
+
 // Launch the grouped GEMM of `plan` (device image already uploaded for `config`).
 hipError_t launch_grouped_gemm(const Plan& plan, int config, int precision, const void* A,
                                const void* B, void* C, hipStream_t stream);
@@ -137,7 +156,11 @@ struct bug_ctx {
     double last_flops = 0.0;     // algorithmic flops of the last entry-point call
     // per-stream scratch for intermediates of the two-stage contractions and the solvers,
     // keyed by (stream, slot)
-    std::unordered_map<void*, bug::Workspace> scratch[4];
+    std::unordered_map<void*, bug::Workspace> scratch[5];
+    // SVD kernel choice (0 automatic, 1 plain one-sided Jacobi, 2 QR-preconditioned) and an optional
+    // device array receiving the sweeps used per matrix of the last batch (diagnostics)
+    int svd_variant = 0;
+    int* svd_sweeps_dev = nullptr;
     // pinned host staging for small reductions
     void* pinned = nullptr;
     size_t pinned_bytes = 0;

@@ -314,7 +314,13 @@ __global__ __launch_bounds__(kQrThreads) void qr_kernel(const MatJob* __restrict
 
 // ---- batched SVD: one-sided Jacobi ---------------------------------------------------------
 
-constexpr int kSvdThreads = 512;
+constexpr int kSvdThreads = 1024;
+// batches with a block at least this wide are QR-preconditioned
+constexpr int kPrecondMinCols = 24;
+
+__device__ void svd_finish(const SvdJob& job, const double2* __restrict__ g,
+                           const double2* __restrict__ v, double2* __restrict__ U,
+                           double* __restrict__ S, double2* __restrict__ Vh, double* sig, int* perm);
 
 // G (m x n, m >= n, ldg) is the working copy (overwritten); V (n x n, ldv) accumulates the
 // rotations.  Outputs: U (m x n, ldu), S (n), Vh (n x n, ldvh), sorted by descending singular
@@ -326,7 +332,8 @@ __global__ __launch_bounds__(kSvdThreads) void svd_jacobi_kernel(const SvdJob* _
                                                                    double2* __restrict__ U,
                                                                    double* __restrict__ S,
                                                                    double2* __restrict__ Vh,
-                                                                   int max_sweeps) {
+                                                                   int max_sweeps,
+                                                                   int* __restrict__ sweeps_out) {
     extern __shared__ double smem[];
     const SvdJob job = jobs[blockIdx.x];
     const int m = job.m, n = job.n;
@@ -346,7 +353,9 @@ __global__ __launch_bounds__(kSvdThreads) void svd_jacobi_kernel(const SvdJob* _
     const double tol = 2.220446049250313e-16 * sqrt((double)m);
     const int N = n + (n & 1);
     __syncthreads();
+    int used = 0;
     for (int sweep = 0; sweep < max_sweeps && n > 1; ++sweep) {
+        ++used;
         if (threadIdx.x == 0) s_rotated = 0;
         __syncthreads();
         for (int round = 0; round < N - 1; ++round) {
@@ -410,7 +419,20 @@ __global__ __launch_bounds__(kSvdThreads) void svd_jacobi_kernel(const SvdJob* _
         if (!s_rotated) break;
         __syncthreads();
     }
-    // singular values and descending order (ties by index)
+    if (sweeps_out && threadIdx.x == 0) sweeps_out[blockIdx.x] = used;
+    svd_finish(job, g, v, U, S, Vh, sig, perm);
+}
+
+// End of the SVD kernel: singular values, descending order (ties by index), phase
+// convention and the U / S / Vh outputs.  Called by every thread of the workgroup.
+__device__ void svd_finish(const SvdJob& job, const double2* __restrict__ g,
+                           const double2* __restrict__ v, double2* __restrict__ U,
+                           double* __restrict__ S, double2* __restrict__ Vh, double* sig, int* perm) {
+    const int m = job.m, n = job.n;
+    const int64_t ldg = job.ldg, ldv = job.ldv;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = blockDim.x / 64;
+    __syncthreads();
     for (int j = wave; j < n; j += nw) {
         const double2* gj = g + (int64_t)j * ldg;
         double s2 = 0.0;
@@ -419,7 +441,7 @@ __global__ __launch_bounds__(kSvdThreads) void svd_jacobi_kernel(const SvdJob* _
         if (lane == 0) sig[j] = sqrt(s2);
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < n; j += kSvdThreads) {
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
         int rank = 0;
         const double sj = sig[j];
         for (int i = 0; i < n; ++i) rank += (sig[i] > sj) || (sig[i] == sj && i < j);
@@ -473,6 +495,55 @@ __global__ __launch_bounds__(kSvdThreads) void svd_jacobi_kernel(const SvdJob* _
             vh[r + (int64_t)i * ldvh] = double2{x.x * phr + x.y * phi, x.x * phi - x.y * phr};
         }
         if (lane == 0) sv[r] = sj;
+    }
+}
+
+// Phase convention on finished factors (QR-preconditioned path): for column r of U (m rows) and
+// row r of Vh (n columns), with ref = the U column (or the conjugated Vh row when phase_on_v),
+// make the first entry with |x_i| >= 0.5 max|x| real positive.  One wave per column.
+__global__ void svd_phase_kernel(const SvdJob* __restrict__ jobs, double2* __restrict__ U,
+                                 double2* __restrict__ Vh) {
+    const SvdJob job = jobs[blockIdx.x];
+    const int m = job.m, n = job.n;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = blockDim.x / 64;
+    double2* u = U + job.u_off;
+    double2* vh = Vh + job.vh_off;
+    const int64_t ldu = job.ldu, ldvh = job.ldvh;
+    for (int r = wave; r < n; r += nw) {
+        const bool onv = job.phase_on_v != 0;
+        const int len = onv ? n : m;
+        double mx = 0.0;
+        for (int i = lane; i < len; i += 64) {
+            const double2 x = onv ? vh[r + (int64_t)i * ldvh] : u[i + (int64_t)r * ldu];
+            mx = fmax(mx, sqrt(x.x * x.x + x.y * x.y));
+        }
+        mx = wave_max(mx);
+        int first = len;
+        for (int base = 0; base < len && first == len; base += 64) {
+            const int i = base + lane;
+            bool hit = false;
+            if (i < len && mx > 0.0) {
+                const double2 x = onv ? vh[r + (int64_t)i * ldvh] : u[i + (int64_t)r * ldu];
+                hit = sqrt(x.x * x.x + x.y * x.y) >= 0.5 * mx;
+            }
+            const unsigned long long bal = __ballot(hit);
+            if (bal) first = base + __ffsll((long long)bal) - 1;
+        }
+        if (first >= len) continue;
+        double2 x = onv ? vh[r + (int64_t)first * ldvh] : u[first + (int64_t)r * ldu];
+        if (onv) x.y = -x.y;   // the V column is the conjugated Vh row
+        const double ax = sqrt(x.x * x.x + x.y * x.y);
+        if (!(ax > 0.0)) continue;
+        const double phr = x.x / ax, phi = x.y / ax;
+        for (int i = lane; i < m; i += 64) {   // U col *= conj(ph)
+            const double2 y = u[i + (int64_t)r * ldu];
+            u[i + (int64_t)r * ldu] = double2{y.x * phr + y.y * phi, y.y * phr - y.x * phi};
+        }
+        for (int i = lane; i < n; i += 64) {   // Vh row *= ph
+            const double2 y = vh[r + (int64_t)i * ldvh];
+            vh[r + (int64_t)i * ldvh] = double2{y.x * phr - y.y * phi, y.x * phi + y.y * phr};
+        }
     }
 }
 
@@ -687,11 +758,124 @@ int bug_block_svd(bug_ctx* ctx, int32_t nmat, const bug_svd_job* jobs, void* g, 
         hipStream_t st = as_stream(stream);
         SvdJob* d = upload_list(ctx, st, list.data(), list.size(), 2);
         double2* v = static_cast<double2*>(scratch(ctx, st, sizeof(double2) * std::max<int64_t>(vtot, 1), 1));
-        const size_t lds = (size_t)nmax * (sizeof(double) + sizeof(int));
-        hipLaunchKernelGGL(svd_jacobi_kernel, dim3(nmat), dim3(kSvdThreads), lds, st, d,
-                           static_cast<double2*>(g), v, static_cast<double2*>(u), s,
-                           static_cast<double2*>(vh), max_sweeps > 0 ? max_sweeps : 60);
+        const int sweeps = max_sweeps > 0 ? max_sweeps : 60;
+        int* sw = ctx->svd_sweeps_dev;
+        bool precond = ctx->svd_variant != 1;
+        if (ctx->svd_variant == 0) {
+            precond = false;
+            for (const SvdJob& j : list) precond |= j.n >= kPrecondMinCols;
+        }
+        if (!precond) {
+            const size_t lds = (size_t)nmax * (sizeof(double) + sizeof(int));
+            hipLaunchKernelGGL(svd_jacobi_kernel, dim3(nmat), dim3(kSvdThreads), lds, st, d,
+                               static_cast<double2*>(g), v, static_cast<double2*>(u), s,
+                               static_cast<double2*>(vh), sweeps, sw);
+        } else {
+            // QR-preconditioned one-sided Jacobi: G = Q R, Jacobi on X = R^H (n x n):
+            // X V_x = U_x S  =>  G = (Q V_x) S U_x^H, so U = Q V_x and Vh = U_x^H.
+            int64_t qtot = 0, ntot = 0;
+            for (const SvdJob& j : list) {
+                qtot += (int64_t)j.m * j.n;
+                ntot += (int64_t)j.n * j.n;
+            }
+            double2* ws = static_cast<double2*>(
+                scratch(ctx, st, sizeof(double2) * std::max<int64_t>(qtot + 4 * ntot + ntot, 1), 4));
+            double2* Qw = ws;
+            double2* Rw = Qw + qtot;
+            double2* Xw = Rw + ntot;
+            double2* Uxw = Xw + ntot;
+            double2* Vxw = Uxw + ntot;
+            double2* tau = Vxw + ntot;
+            std::vector<MatJob> qj(nmat);
+            std::vector<SvdJob> jj(nmat);
+            std::vector<CopyOp> xc(nmat), vc(nmat);
+            int64_t qo = 0, no = 0, to = 0;
+            for (int i = 0; i < nmat; ++i) {
+                const SvdJob& j = list[i];
+                qj[i] = MatJob{j.g_off, qo, no, to, j.m, j.n, j.ldg, std::max(1, j.m), std::max(1, j.n)};
+                xc[i] = CopyOp{no, no, std::max(1, j.n), std::max(1, j.n), j.n, j.n, 2, 0, 0};
+                jj[i] = SvdJob{no, j.v_off, no, j.s_off, no, j.n, j.n, std::max(1, j.n), j.n,
+                               std::max(1, j.n), std::max(1, j.n), 0};
+                vc[i] = CopyOp{no, j.vh_off, std::max(1, j.n), j.ldvh, j.n, j.n, 2, 0, 0};
+                qo += (int64_t)j.m * j.n;
+                no += (int64_t)j.n * j.n;
+                to += j.n;
+            }
+            MatJob* dq = upload_list(ctx, st, qj.data(), qj.size(), 2);
+            hipLaunchKernelGGL(qr_kernel, dim3(nmat), dim3(kQrThreads), 0, st, dq,
+                               static_cast<double2*>(g), Qw, Rw, tau);
+            HIP_CHECK(hipGetLastError());
+            CopyOp* dx = upload_list(ctx, st, xc.data(), xc.size(), 2);
+            hipLaunchKernelGGL(copy_blocks_kernel, dim3(nmat, 8), dim3(256), 0, st, dx, Rw, Xw,
+                               nullptr, 0);
+            HIP_CHECK(hipGetLastError());
+            SvdJob* dj = upload_list(ctx, st, jj.data(), jj.size(), 2);
+            const size_t lds = (size_t)nmax * (sizeof(double) + sizeof(int));
+            hipLaunchKernelGGL(svd_jacobi_kernel, dim3(nmat), dim3(kSvdThreads), lds, st, dj, Xw,
+                               v, Uxw, s, Vxw, sweeps, sw);
+            HIP_CHECK(hipGetLastError());
+            // U = Q Vx  with Vx = (Vh_x)^H, one grouped-GEMM launch over the batch
+            Hasher h;
+            h.pod((uint64_t)0x5F0D);
+            for (const SvdJob& j : list) {
+                h.pod(j.m);
+                h.pod(j.n);
+                h.pod(j.u_off);
+                h.pod(j.ldu);
+            }
+            launch_cached_plan(ctx, h.h, [&](Plan& plan) {
+                int64_t qo2 = 0, no2 = 0;
+                for (int i = 0; i < nmat; ++i) {
+                    const SvdJob& j = list[i];
+                    if (j.m > 0 && j.n > 0) {
+                        plan.tasks.push_back(GemmTask{j.u_off, j.ldu, j.m, j.n,
+                                                      (int32_t)plan.terms.size(),
+                                                      (int32_t)plan.terms.size() + 1, 0});
+                        plan.terms.push_back(GemmTerm{qo2, no2, std::max(1, j.m), std::max(1, j.n),
+                                                      j.n, BUG_OP_N | (BUG_OP_C << 2), 1.0, 0.0});
+                        plan.flops += 8.0 * j.m * j.n * (double)j.n;
+                    }
+                    qo2 += (int64_t)j.m * j.n;
+                    no2 += (int64_t)j.n * j.n;
+                }
+            }, BUG_PREC_FP64, Qw, Vxw, u, st);
+            CopyOp* dv = upload_list(ctx, st, vc.data(), vc.size(), 2);
+            hipLaunchKernelGGL(copy_blocks_kernel, dim3(nmat, 8), dim3(256), 0, st, dv, Uxw,
+                               static_cast<double2*>(vh), nullptr, 0);
+            HIP_CHECK(hipGetLastError());
+            SvdJob* dp = upload_list(ctx, st, list.data(), list.size(), 2);
+            hipLaunchKernelGGL(svd_phase_kernel, dim3(nmat), dim3(512), 0, st, dp,
+                               static_cast<double2*>(u), static_cast<double2*>(vh));
+        }
         HIP_CHECK(hipGetLastError());
+    });
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int bug_set_svd_variant(bug_ctx* ctx, int32_t variant, int32_t record_sweeps) {
+    return guarded(ctx, [&] {
+        if (variant < 0 || variant > 2) throw Error(BUG_EINVAL, "svd variant must be 0, 1 or 2");
+        ctx->svd_variant = variant;
+        if (record_sweeps && !ctx->svd_sweeps_dev) {
+            HIP_CHECK(hipMalloc(&ctx->svd_sweeps_dev, sizeof(int) * 65536));
+            HIP_CHECK(hipMemset(ctx->svd_sweeps_dev, 0, sizeof(int) * 65536));
+        } else if (!record_sweeps && ctx->svd_sweeps_dev) {
+            HIP_CHECK(hipDeviceSynchronize());
+            HIP_CHECK(hipFree(ctx->svd_sweeps_dev));
+            ctx->svd_sweeps_dev = nullptr;
+        }
+    });
+}
+
+int bug_svd_sweeps(bug_ctx* ctx, int32_t n, int32_t* out) {
+    return guarded(ctx, [&] {
+        if (n < 0 || n > 65536 || (n > 0 && !out)) throw Error(BUG_EINVAL, "bad sweep buffer");
+        if (!ctx->svd_sweeps_dev) throw Error(BUG_EINVAL, "sweep recording is off");
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(out, ctx->svd_sweeps_dev, sizeof(int) * n, hipMemcpyDeviceToHost));
     });
 }
 

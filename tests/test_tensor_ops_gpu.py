@@ -84,3 +84,35 @@ def test_projections_direct_sums_and_vectors(gpu):
     assert rel_err(fam_from_device(z), A.scale(0.5 - 1j).axpy(0.25j, A)) < TOL
     many = T.inner_many([x] * 70, y)
     assert len(many) == 70 and all(abs(v - many[0]) < 1e-12 * abs(many[0]) for v in many)
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("m,n", [(2500, 6), (700, 40), (300, 17), (5, 1), (64, 64)])
+def test_svd_kernels_on_tall_blocks(gpu, m, n, variant):
+    """Both SVD paths (plain one-sided Jacobi; QR-preconditioned Jacobi on R^H) must match
+    LAPACK's spectrum, reconstruct the block and return orthonormal singular vectors."""
+    from bondupdategalerkin_amd import _lib
+    from bondupdategalerkin_amd import tensor_ops as T
+
+    ctx = _lib.context()
+    ctx.check(ctx._lib.bug_set_svd_variant(ctx.handle, variant, 0))
+
+    rng = np.random.default_rng(m + n)
+    C = ob.Fam({0: m}, {0: n}, [0])
+    M = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+    M[:, -1] = M[:, 0] * 1e-9 + M[:, -1] * 1e-12 if n > 1 else M[:, -1]   # a tiny singular value
+    C.blocks[(0, 0)] = M
+    r = T.svd_core(device_from_fam(C))
+    o = la.svd_core(C)
+    assert np.allclose(r.S[0], o.S[0], rtol=1e-12, atol=1e-14 * o.S[0][0])
+    U = fam_from_device(r.U).dense()[0]
+    Vh = fam_from_device(r.Vh).dense()[0]
+    assert np.abs(U * r.S[0][None, :] @ Vh - M).max() < 1e-12 * np.abs(M).max()
+    assert np.abs(U.conj().T @ U - np.eye(U.shape[1])).max() < 1e-12
+    # the phase convention: first entry with |u| >= 0.5 max|u| real positive
+    for j in range(U.shape[1]):
+        a = np.abs(U[:, j])
+        i = int(np.argmax(a >= 0.5 * a.max()))
+        assert abs(U[i, j].imag) < 1e-12 and U[i, j].real > 0
+    assert np.allclose(U, la.svd_core(C).U.dense()[0], atol=1e-9)
+    ctx.check(ctx._lib.bug_set_svd_variant(ctx.handle, 0, 0))
