@@ -5,7 +5,11 @@ synthetic U(1) state with bond dimension up to D = 1024 (Gaussian charge-sector 
 canonicalised and normalised on the device.
 
 ``--workload bug_step`` (default): one step = one ``cbe_bug_step`` (order 1, tau = -0.02 i,
-maxdim = D, work_ceiling_rule = maxdim, default tolerances) on the device-resident state.
+maxdim = D, work_ceiling_rule = maxdim, default tolerances) on the device-resident state.  The
+headline form is ``--precision mixed`` (FP32-MFMA block contractions, FP64 Krylov / QR / SVD);
+the FP64 drop-in is stepped from the same state beside it, timed on its own, and the largest
+|<S^z_j>| difference between the two runs after the timed steps is reported as the error of the
+mixed form.  ``--precision fp64`` times the drop-in alone.
 ``--workload heff_sweep``: one step = a left-to-right pass doing the one-site effective-Hamiltonian
 apply and the left environment push at every site (the contraction core of a half-sweep).
 
@@ -40,7 +44,9 @@ def parse():
     p.add_argument("--L", type=int, default=128)
     p.add_argument("--D", type=int, default=1024)
     p.add_argument("--delta", type=float, default=0.5)
-    p.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    p.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    p.add_argument("--no-fp64", action="store_true",
+                   help="with --precision mixed: skip the FP64 comparison run")
     p.add_argument("--workload", choices=["bug_step", "heff_sweep"], default="bug_step")
     p.add_argument("--kernel-reps", type=int, default=20)
     p.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -65,6 +71,7 @@ def main():
 
     torch.cuda.set_device(0)
     prec = _lib.PREC_FP64 if args.precision == "fp64" else _lib.PREC_FP32
+    ckind = "fp64" if args.precision == "fp64" else "fp32"
     L, D = args.L, args.D
     t0 = time.time()
     psi = random_mps(L, D, seed=0)
@@ -94,32 +101,54 @@ def main():
     torch.cuda.synchronize()
     apply_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
     achieved = flops_apply / (apply_ms * 1e-3) / 1e12
-    peak = PEAK[args.precision]
+    peak = PEAK[ckind]
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
                 "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
-                "kernel": "grouped_zgemm (one-site H_eff apply, 2 launches)",
+                "kernel": f"grouped_zgemm {ckind} (one-site H_eff apply, 2 launches)",
                 "flops_per_apply": flops_apply, "apply_ms": round(apply_ms, 4)}
     if args.kernel_only:
         print(json.dumps({"kernel_only": True, "roofline": roofline}), flush=True)
         return
 
     # ---- the timed step ----
-    statuses = []
+    statuses, extra = [], {}
     if args.workload == "bug_step":
         from bondupdategalerkin_amd import bug_step as B
+        from bondupdategalerkin_amd import mps as M
         from bondupdategalerkin_amd import tensor_ops as T
 
         B.canonical(psi, L - 1)
         T.scale_(1.0 / T.norm(psi[L - 1]), psi[L - 1])
         B.canonical(psi, 0)
         torch.cuda.synchronize()
+        twin = psi.copy() if (args.precision == "mixed" and not args.no_fp64) else None
 
-        def step():
-            info = B.cbe_bug_step(psi, mpo, -0.02j, maxdim=D, work_ceiling_rule="maxdim")
-            statuses.append(B.budget_status(info.ledger))
-            print(f"# step: {info.seconds['step']:.2f}s {B.budget_status(info.ledger)} "
-                  f"{info.ledger.unmet} max bond {max(psi.bond_dims())}", file=sys.stderr,
-                  flush=True)
+        def make_step(state, form, record):
+            def step():
+                info = B.cbe_bug_step(state, mpo, -0.02j, maxdim=D, work_ceiling_rule="maxdim",
+                                      precision=form)
+                if record is not None:
+                    record.append(B.budget_status(info.ledger))
+                print(f"# {form} step: {info.seconds['step']:.2f}s {B.budget_status(info.ledger)} "
+                      f"{info.ledger.unmet} max bond {max(state.bond_dims())}", file=sys.stderr,
+                      flush=True)
+            return step
+
+        step = make_step(psi, args.precision, statuses)
+        if twin is not None:
+            fp64_statuses = []
+            step64 = make_step(twin, "fp64", fp64_statuses)
+            for _ in range(args.warmup):
+                step64()
+            torch.cuda.synchronize()
+            fp64_statuses.clear()
+            t1 = time.perf_counter()
+            for _ in range(args.steps):
+                step64()
+            torch.cuda.synchronize()
+            ms64 = 1e3 * (time.perf_counter() - t1) / args.steps
+            extra["fp64"] = {"steps_per_s": round(1e3 / ms64, 4), "ms_per_step": round(ms64, 3),
+                             "budget_status": fp64_statuses}
     else:
         def step():
             lenv = lenvs[0]
@@ -139,6 +168,10 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t1
     ms_per_step = 1e3 * elapsed / args.steps
+    if args.workload == "bug_step" and twin is not None:
+        dsz = float(np.abs(M.magnetisation(psi) - M.magnetisation(twin)).max())
+        extra["fp64"]["max_abs_dSz_mixed_vs_fp64"] = dsz
+        extra["fp64"]["speedup_mixed_vs_fp64"] = round(extra["fp64"]["ms_per_step"] / ms_per_step, 3)
 
     cpu = None
     if not args.no_cpu:
@@ -155,7 +188,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128" if args.precision == "fp64" else "c64-mfma/c128-storage",
+        "dtype": "c128" if args.precision == "fp64" else "c64 contractions / c128 Krylov, QR, SVD",
         "data": "synthetic",
         "config": {"workload": ("cbe_bug_step order 1 (the order-2 closure is not built), "
                                 "tau=-0.02i, maxdim=D, work_ceiling_rule=maxdim, synthetic "
@@ -165,6 +198,7 @@ def main():
                    "L": L, "D": D, "model": f"xxz delta={args.delta}", "symmetry": "U1",
                    "precision": args.precision, "parallelism": "single GPU",
                    "budget_status": statuses},
+        **extra,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -238,7 +238,7 @@ def _slots(entries):
     return slots
 
 
-def stream_left(L, W1: MPOSite, W2: MPOSite, R, A, V, Om) -> BlockTensor:
+def stream_left(L, W1: MPOSite, W2: MPOSite, R, A, V, Om, prec: int = _lib.PREC_FP64) -> BlockTensor:
     """(H Theta) Om^dag for Theta = A V with Om in V's layout: (link_l, s, g)
     (_stream_left, mpo_environments.jl:409-444), four grouped launches."""
     N, Tt, C, Rr = T.OP_N, T.OP_T, T.OP_C, T.OP_R
@@ -252,26 +252,29 @@ def stream_left(L, W1: MPOSite, W2: MPOSite, R, A, V, Om) -> BlockTensor:
         k = fs[(c, si)]
         fsh[k] = -Rs[b] - Os[so]
         tf.append((k, b, so, Tt, C, co))
-    F = T.product(R.layout.col, Om.layout.row, [fsh[k] for k in range(len(fs))], R, Om, tf)
+    F = T.product(R.layout.col, Om.layout.row, [fsh[k] for k in range(len(fs))], R, Om, tf,
+                  precision=prec)
     cs = sorted({c for c, _ in fs})
     ci = {c: i for i, c in enumerate(cs)}
     bsh, tb = {}, []
     for (c, si), k in fs.items():
         tb.append((ci[c], si, k, N, N, 1.0))
         bsh[ci[c]] = Vs[si] + fsh[k]
-    B = T.product(V.layout.row, Om.layout.row, [bsh[i] for i in range(len(cs))], V, F, tb)
+    B = T.product(V.layout.row, Om.layout.row, [bsh[i] for i in range(len(cs))], V, F, tb,
+                  precision=prec)
     cslots, csh, tc = {}, {}, []
     for a, c, so, si, co in e1:
         if c in ci and (c, si) not in cslots:
             k = cslots[(c, si)] = len(cslots)
             tc.append((k, si, ci[c], N, N, 1.0))
             csh[k] = As[si] + bsh[ci[c]]
-    Cm = T.product(A.layout.row, Om.layout.row, [csh[k] for k in range(len(cslots))], A, B, tc)
+    Cm = T.product(A.layout.row, Om.layout.row, [csh[k] for k in range(len(cslots))], A, B, tc,
+                   precision=prec)
     ty = [(so, a, cslots[(c, si)], N, N, co) for a, c, so, si, co in e1 if (c, si) in cslots]
-    return T.product(L.layout.row, Om.layout.row, As, L, Cm, ty)
+    return T.product(L.layout.row, Om.layout.row, As, L, Cm, ty, precision=prec)
 
 
-def stream_right(L, W1: MPOSite, W2: MPOSite, R, U, Bv, Om) -> BlockTensor:
+def stream_right(L, W1: MPOSite, W2: MPOSite, R, U, Bv, Om, prec: int = _lib.PREC_FP64) -> BlockTensor:
     """Om^dag (H Theta) for Theta = U Bv with Om in U's layout: (g, s, link_r)
     (_stream_right, mpo_environments.jl:446-481), four grouped launches."""
     N, Tt, C, Rr = T.OP_N, T.OP_T, T.OP_C, T.OP_R
@@ -284,23 +287,26 @@ def stream_right(L, W1: MPOSite, W2: MPOSite, R, U, Bv, Om) -> BlockTensor:
         k = fs[(c, si)]
         fsh[k] = -Ls[a] + Os[so]
         tf.append((k, a, so, Tt, Rr, co))
-    F = T.product(L.layout.col, Om.layout.col, [fsh[k] for k in range(len(fs))], L, Om, tf)
+    F = T.product(L.layout.col, Om.layout.col, [fsh[k] for k in range(len(fs))], L, Om, tf,
+                  precision=prec)
     cs = sorted({c for c, _ in fs})
     ci = {c: i for i, c in enumerate(cs)}
     bsh, tb = {}, []
     for (c, si), k in fs.items():
         tb.append((ci[c], si, k, Tt, N, 1.0))
         bsh[ci[c]] = -Us[si] + fsh[k]
-    B = T.product(U.layout.col, Om.layout.col, [bsh[i] for i in range(len(cs))], U, F, tb)
+    B = T.product(U.layout.col, Om.layout.col, [bsh[i] for i in range(len(cs))], U, F, tb,
+                  precision=prec)
     cslots, csh, tc = {}, {}, []
     for c, b, so, si, co in e2:
         if c in ci and (c, si) not in cslots:
             k = cslots[(c, si)] = len(cslots)
             tc.append((k, ci[c], si, Tt, N, 1.0))
             csh[k] = -bsh[ci[c]] + Bs[si]
-    Cm = T.product(Om.layout.col, Bv.layout.col, [csh[k] for k in range(len(cslots))], B, Bv, tc)
+    Cm = T.product(Om.layout.col, Bv.layout.col, [csh[k] for k in range(len(cslots))], B, Bv, tc,
+                   precision=prec)
     ty = [(so, cslots[(c, si)], b, N, Tt, co) for c, b, so, si, co in e2 if (c, si) in cslots]
-    return T.product(Om.layout.col, R.layout.row, Bs, Cm, R, ty)
+    return T.product(Om.layout.col, R.layout.row, Bs, Cm, R, ty, precision=prec)
 
 
 # ---- frames -------------------------------------------------------------------------------
@@ -409,7 +415,7 @@ class _Expansion:
 
 def dd_cbe_expand(fr: BondFrame, mpo: MPO, bond: int, lenv, renv, side: str, alloc: _Alloc,
                   ledger: AdaptiveLedger, workcap: int, ex: DDQRExpansion, stream: ProbeStream,
-                  probe_hook=None) -> _Expansion:
+                  probe_hook=None, prec: int = _lib.PREC_FP64) -> _Expansion:
     """One side of one bond, ``:rank_tail`` (dd_cbe_expand.jl:19-87,252-301,383-419)."""
     left = side == "left"
     frame, opp = (fr.U0, fr.V0) if left else (fr.V0, fr.U0)
@@ -432,10 +438,10 @@ def dd_cbe_expand(fr: BondFrame, mpo: MPO, bond: int, lenv, renv, side: str, all
         return none
     ledger.n_probes += _width(Om, oside)
     if left:
-        Ysk = stream_left(lenv, W1, W2, renv, T.site_times_core(fr.U0, fr.S0), fr.V0, Om)
+        Ysk = stream_left(lenv, W1, W2, renv, T.site_times_core(fr.U0, fr.S0), fr.V0, Om, prec)
         project = lambda t: T.perp_left(fr.U0, t)
     else:
-        Ysk = stream_right(lenv, W1, W2, renv, fr.U0, T.core_times_site(fr.S0, fr.V0), Om)
+        Ysk = stream_right(lenv, W1, W2, renv, fr.U0, T.core_times_site(fr.S0, fr.V0), Om, prec)
         project = lambda t: T.perp_right(fr.V0, t)
     Y = project(Ysk)
     if T.norm(Y) <= CBE_EPS:
@@ -522,7 +528,8 @@ class _KrylovStats:
 
 
 def krylov_enrich(seed, applyH, start_frame, side, tau, alloc: _Alloc, workcap: int,
-                  recurrence: str = "lanczos", maxdepth: int = K.MAXDEPTH):
+                  recurrence: str = "lanczos", maxdepth: int = K.MAXDEPTH,
+                  hermitian_tol: float = 1e-9):
     """Tests C/D: grow the frame by the local Krylov directions the step needs
     (``_krylov_enrich`` + ``_lanczos_chain``/``_arnoldi_chain``)."""
     perp = _perp(side)
@@ -561,7 +568,7 @@ def krylov_enrich(seed, applyH, start_frame, side, tau, alloc: _Alloc, workcap: 
             if k > 1:
                 T.axpy_(-prev_beta, vs[k - 2], img)
             a = T.inner(vs[k - 1], img)
-            if not abs(a.imag) <= 1e-9 * max(abs(a), 1.0):
+            if not abs(a.imag) <= hermitian_tol * max(abs(a), 1.0):
                 raise ValueError("Lanczos needs a Hermitian one-site generator")
             H[k - 1, k - 1] = a.real
             T.axpy_(-a.real, vs[k - 1], img)
@@ -620,9 +627,14 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
                  work_ceiling_rule: str = "bond", bond_ceiling_factor: Optional[float] = None,
                  root: Optional[int] = None, reorth: bool = True, truncate: bool = True,
                  expansion: Optional[DDQRExpansion] = None, rng: Optional[int] = None,
-                 probe_hook: Optional[Callable] = None) -> CBEBugSweepInfo:
+                 probe_hook: Optional[Callable] = None, precision: str = "fp64") -> CBEBugSweepInfo:
     """One step in place; ``root`` is the 0-based root bond (the reference's ``root`` minus 1),
-    ``rng`` the probe seed (default 0x5EED, fresh per call as the reference's default RNG)."""
+    ``rng`` the probe seed (default 0x5EED, fresh per call as the reference's default RNG).
+
+    ``precision = "mixed"`` runs the dense block contractions (environment pushes, one- and
+    zero-site applies, streamed sketches) on FP32 MFMA with the complex products split into real
+    ones and FP32 accumulation; Krylov orthogonalisation, the small exponentials, QR and SVD stay
+    FP64.  The Hermitian check of the local Lanczos chain is then held to FP32 rounding (1e-5)."""
     L = len(psi)
     # -- option checks, before anything touches the state (bug_step_stages.jl:41-111) --
     if len(mpo) != L:
@@ -653,6 +665,10 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         raise ValueError("bond_ceiling_factor must be 0, or > 1 with work_ceiling_rule='bond'")
     if not hermitian:
         raise NotImplementedError("hermitian=False (Arnoldi root) is not implemented")
+    if precision not in ("fp64", "mixed"):
+        raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
+    cp = _lib.PREC_FP32 if precision == "mixed" else _lib.PREC_FP64
+    htol = 1e-5 if precision == "mixed" else 1e-9
     ex = expansion or DDQRExpansion()
     maxdim = 128 if maxdim is None else maxdim
     tau = complex(tau)
@@ -680,7 +696,7 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     renvs: List[Optional[BlockTensor]] = [None] * (L + 1)
     renvs[L] = ev.boundary_right(mpo, psiL.bond_leg(L))
     for i in range(L - 1, 1, -1):
-        renvs[i] = ev.push_right(renvs[i + 1], psiL[i], mpo[i])
+        renvs[i] = ev.push_right(renvs[i + 1], psiL[i], mpo[i], precision=cp)
     lch = ev.boundary_left(mpo, psiL.bond_leg(0))
     lroot = lch
     carry = None
@@ -690,12 +706,12 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         carried = psiL[b] if carry is None else T.core_times_site(carry, psiL[b])
         fr = frame_from(carried, psiL[b + 1])
         xp = dd_cbe_expand(fr, mpo, b, lch, renvs[b + 2], "left", alloc, ledger, wcap[b], ex,
-                           stream, probe_hook)
+                           stream, probe_hook, cp)
         Kex = T.site_times_core(carried, T.overlap_right(psiL[b + 1], xp.V_ex))
-        rch1 = ev.push_right(renvs[b + 2], xp.V_ex, mpo[b + 1])
+        rch1 = ev.push_right(renvs[b + 2], xp.V_ex, mpo[b + 1], precision=cp)
         Hl, Hw, Hr = lch, mpo[b], rch1
-        Wk, kst = krylov_enrich(Kex, lambda v: ev.apply_one_site(Hl, Hw, Hr, v), xp.U_ex, "left",
-                                tau, alloc, wcap[b], recurrence)
+        Wk, kst = krylov_enrich(Kex, lambda v: ev.apply_one_site(Hl, Hw, Hr, v, precision=cp),
+                                xp.U_ex, "left", tau, alloc, wcap[b], recurrence, hermitian_tol=htol)
         _record_krylov(ledger, kst)
         left_frames[b] = Wk
         expanded[b] = Wk.layout.col.total
@@ -703,7 +719,7 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         if b == rl:
             lroot = lch
         else:
-            lch = ev.push_left(lch, Wk, mpo[b])
+            lch = ev.push_left(lch, Wk, mpo[b], precision=cp)
     secs["left_sweep"] = time.perf_counter() - ts
 
     # ---- right half-sweep ----
@@ -712,7 +728,7 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     canonical(psiR, L - 1)
     lenvs = [ev.boundary_left(mpo, psiR.bond_leg(0))]
     for i in range(L - 2):
-        lenvs.append(ev.push_left(lenvs[-1], psiR[i], mpo[i]))
+        lenvs.append(ev.push_left(lenvs[-1], psiR[i], mpo[i], precision=cp))
     rch = ev.boundary_right(mpo, psiR.bond_leg(L))
     rroot = rch
     carry = None
@@ -722,12 +738,13 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         carried = psiR[b + 1] if carry is None else T.site_times_core(psiR[b + 1], carry)
         fr = frame_from(psiR[b], carried)
         xp = dd_cbe_expand(fr, mpo, b, lenvs[b], rch, "right", alloc, ledger, wcap[b], ex,
-                           stream, probe_hook)
+                           stream, probe_hook, cp)
         Bex = T.core_times_site(T.overlap_left(xp.U_ex, psiR[b]), carried)
-        lch1 = ev.push_left(lenvs[b], xp.U_ex, mpo[b])
+        lch1 = ev.push_left(lenvs[b], xp.U_ex, mpo[b], precision=cp)
         Hl, Hw, Hr = lch1, mpo[b + 1], rch
-        Zk, kst = krylov_enrich(Bex, lambda v: ev.apply_one_site(Hl, Hw, Hr, v), xp.V_ex,
-                                "right", tau, alloc, wcap[b], recurrence)
+        Zk, kst = krylov_enrich(Bex, lambda v: ev.apply_one_site(Hl, Hw, Hr, v, precision=cp),
+                                xp.V_ex, "right", tau, alloc, wcap[b], recurrence,
+                                hermitian_tol=htol)
         _record_krylov(ledger, kst)
         right_frames[b] = Zk
         if b != rl:
@@ -736,7 +753,7 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         if b == rl:
             rroot = rch
         else:
-            rch = ev.push_right(rch, Zk, mpo[b + 1])
+            rch = ev.push_right(rch, Zk, mpo[b + 1], precision=cp)
     AR = carry
     secs["right_sweep"] = time.perf_counter() - ts
 
@@ -755,13 +772,13 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
 
     # ---- root solve ----
     ts = time.perf_counter()
-    Lr = ev.push_left(lroot, left_frames[rl], mpo[rl])
-    Rr = ev.push_right(rroot, right_frames[rl], mpo[rl + 1])
+    Lr = ev.push_left(lroot, left_frames[rl], mpo[rl], precision=cp)
+    Rr = ev.push_right(rroot, right_frames[rl], mpo[rl + 1], precision=cp)
     napply = [0]
 
     def apply_root(C):
         napply[0] += 1
-        return ev.apply_zero_site(Lr, Rr, C)
+        return ev.apply_zero_site(Lr, Rr, C, precision=cp)
 
     result, rep = K.lanczos_expv(apply_root, tau, root_start, tol=1e-15, maxiter=K.ROOT_MAXITER,
                                  reorth=reorth, conv_tol=te, substeps=0)

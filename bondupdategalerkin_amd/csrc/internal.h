@@ -157,10 +157,13 @@ struct bug_ctx {
     // per-stream scratch for intermediates of the two-stage contractions and the solvers,
     // keyed by (stream, slot)
     std::unordered_map<void*, bug::Workspace> scratch[5];
-    // SVD kernel choice (0 automatic, 1 plain one-sided Jacobi, 2 QR-preconditioned) and an optional
+    // SVD kernel choice (0 automatic, 1 plain one-sided Jacobi, 2 QR-preconditioned with one workgroup
+    // per matrix, 3 QR-preconditioned multi-workgroup block Jacobi) and an optional
     // device array receiving the sweeps used per matrix of the last batch (diagnostics)
     int svd_variant = 0;
     int* svd_sweeps_dev = nullptr;
+    // dynamic LDS bytes a workgroup may use on this device (queried on first use)
+    int lds_limit = 0;
     // pinned host staging for small reductions
     void* pinned = nullptr;
     size_t pinned_bytes = 0;

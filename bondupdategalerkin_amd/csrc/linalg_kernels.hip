@@ -182,7 +182,7 @@ __global__ void block_sqnorm_kernel(const CopyOp* __restrict__ ops, const double
 
 // ---- batched thin QR (Householder, unique gauge) ------------------------------------------
 
-constexpr int kQrThreads = 256;
+constexpr int kQrThreads = 1024;
 
 // A (m x n, lda) is overwritten; Q (m x k, ldq) and R (k x n, ldr), k = min(m, n).
 // R has a real non-negative diagonal (the unique thin QR of a full-rank A).
@@ -547,6 +547,201 @@ __global__ void svd_phase_kernel(const SvdJob* __restrict__ jobs, double2* __res
     }
 }
 
+// ---- multi-workgroup block one-sided Jacobi ------------------------------------------------
+//
+// The n columns of X (n x n) are cut into blocks of w columns.  A sweep is the Brent-Luk
+// round-robin over the blocks: in every round the blocks pair up and each workgroup takes one
+// pair, stages its c <= 2w columns in LDS, runs one one-sided Jacobi sweep over all column pairs
+// of the stage (same rotation and threshold as the one-workgroup kernel), writes the columns
+// back and applies the accumulated c x c rotation Z to the same columns of V.  One launch per
+// round; a per-matrix flag records whether a sweep rotated, and a matrix whose previous sweep did
+// not rotate is skipped.
+
+constexpr int kBjThreads = 1024;
+constexpr int kBjMaxCols = 32;            // columns staged per workgroup
+constexpr size_t kBjStageBytes = 131072;  // LDS for the staged columns
+
+struct BjMat {
+    int64_t x_off, v_off;
+    int32_t n, w, nb, N;      // rows = columns = n; block width; blocks; blocks padded to even
+};
+
+__device__ __forceinline__ void round_robin(int N, int round, int pi, int& p, int& q) {
+    if (pi == 0) {
+        p = round % (N - 1);
+        q = N - 1;
+    } else {
+        p = (round + pi) % (N - 1);
+        q = (round - pi + (N - 1)) % (N - 1);
+    }
+    if (p > q) {
+        const int t = p;
+        p = q;
+        q = t;
+    }
+}
+
+__global__ __launch_bounds__(kBjThreads) void bj_round_kernel(const BjMat* __restrict__ mats,
+                                                              const int2* __restrict__ tasks,
+                                                              double2* __restrict__ X,
+                                                              double2* __restrict__ V, int round,
+                                                              const int* __restrict__ flags_prev,
+                                                              int* __restrict__ flags_cur,
+                                                              double eps) {
+    extern __shared__ double2 xs[];           // n * c staged columns, then c * c Z
+    __shared__ double red[kBjThreads / 64][4];
+    __shared__ int s_rot;
+    const int2 tk = tasks[blockIdx.x];
+    const BjMat mt = mats[tk.x];
+    if (round >= mt.N - 1) return;
+    if (flags_prev[tk.x] == 0) return;        // converged in the previous sweep
+    int bp, bq;
+    round_robin(mt.N, round, tk.y, bp, bq);
+    const int n = mt.n, w = mt.w;
+    const int cp = bp < mt.nb ? min(w, n - bp * w) : 0;
+    const int cq = bq < mt.nb ? min(w, n - bq * w) : 0;
+    const int c = cp + cq;
+    if (c < 2) return;
+    double2* x = X + mt.x_off;
+    double2* v = V + mt.v_off;
+    double2* Z = xs + (int64_t)n * c;
+    auto gcol = [&](int j) { return j < cp ? bp * w + j : bq * w + (j - cp); };
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int nw = kBjThreads / 64;
+    for (int e = tid; e < n * c; e += kBjThreads) {
+        const int i = e % n, j = e / n;
+        xs[e] = x[i + (int64_t)gcol(j) * n];
+    }
+    for (int e = tid; e < c * c; e += kBjThreads) Z[e] = double2{(e % c) == (e / c) ? 1.0 : 0.0, 0.0};
+    if (tid == 0) s_rot = 0;
+    __syncthreads();
+    const double tol = eps * sqrt((double)n);
+    const int C = c + (c & 1), np = C / 2;
+    const int wpp = nw / np;                  // waves per pair (np <= 16)
+    const int grp = wave / wpp, sub = wave % wpp;
+    const int gthreads = wpp * 64, gtid = sub * 64 + lane;
+    for (int ir = 0; ir < C - 1; ++ir) {
+        int a = 0, b = 0;
+        const bool active = grp < np;
+        if (active) round_robin(C, ir, grp, a, b);
+        const bool valid = active && b < c;
+        double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+        if (valid) {
+            const double2* xa = xs + (int64_t)a * n;
+            const double2* xb = xs + (int64_t)b * n;
+            for (int i = gtid; i < n; i += gthreads) {
+                const double2 p = xa[i], q = xb[i];
+                al += p.x * p.x + p.y * p.y;
+                be += q.x * q.x + q.y * q.y;
+                gr += p.x * q.x + p.y * q.y;
+                gi += p.x * q.y - p.y * q.x;
+            }
+        }
+        al = wave_sum(al);
+        be = wave_sum(be);
+        gr = wave_sum(gr);
+        gi = wave_sum(gi);
+        if (lane == 0) {
+            red[wave][0] = al;
+            red[wave][1] = be;
+            red[wave][2] = gr;
+            red[wave][3] = gi;
+        }
+        __syncthreads();
+        if (valid) {
+            al = be = gr = gi = 0.0;
+            for (int k = 0; k < wpp; ++k) {
+                al += red[grp * wpp + k][0];
+                be += red[grp * wpp + k][1];
+                gr += red[grp * wpp + k][2];
+                gi += red[grp * wpp + k][3];
+            }
+            const double ag = sqrt(gr * gr + gi * gi);
+            if (ag > tol * sqrt(al * be) && ag != 0.0) {
+                const double zeta = (be - al) / (2.0 * ag);
+                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double cs = 1.0 / sqrt(1.0 + t * t);
+                const double sn = cs * t;
+                const double er = gr / ag, ei = gi / ag;
+                // x_a' = c x_a - s conj(e) x_b ; x_b' = s e x_a + c x_b
+                double2* xa = xs + (int64_t)a * n;
+                double2* xb = xs + (int64_t)b * n;
+                for (int i = gtid; i < n; i += gthreads) {
+                    const double2 p = xa[i], q = xb[i];
+                    xa[i] = double2{cs * p.x - sn * (er * q.x + ei * q.y),
+                                    cs * p.y - sn * (er * q.y - ei * q.x)};
+                    xb[i] = double2{sn * (er * p.x - ei * p.y) + cs * q.x,
+                                    sn * (er * p.y + ei * p.x) + cs * q.y};
+                }
+                double2* za = Z + (int64_t)a * c;
+                double2* zb = Z + (int64_t)b * c;
+                for (int i = gtid; i < c; i += gthreads) {
+                    const double2 p = za[i], q = zb[i];
+                    za[i] = double2{cs * p.x - sn * (er * q.x + ei * q.y),
+                                    cs * p.y - sn * (er * q.y - ei * q.x)};
+                    zb[i] = double2{sn * (er * p.x - ei * p.y) + cs * q.x,
+                                    sn * (er * p.y + ei * p.x) + cs * q.y};
+                }
+                if (gtid == 0) s_rot = 1;
+            }
+        }
+        __syncthreads();
+    }
+    if (!s_rot) return;                       // uniform: read after the last barrier
+    for (int e = tid; e < n * c; e += kBjThreads) {
+        const int i = e % n, j = e / n;
+        x[i + (int64_t)gcol(j) * n] = xs[e];
+    }
+    __syncthreads();
+    for (int e = tid; e < n * c; e += kBjThreads) {
+        const int i = e % n, j = e / n;
+        xs[e] = v[i + (int64_t)gcol(j) * n];
+    }
+    __syncthreads();
+    // V(:, cols) <- V(:, cols) Z
+    for (int e = tid; e < n * c; e += kBjThreads) {
+        const int i = e % n, j = e / n;
+        double sr = 0.0, si = 0.0;
+        for (int k = 0; k < c; ++k) {
+            const double2 p = xs[i + (int64_t)k * n], z = Z[k + (int64_t)j * c];
+            sr += p.x * z.x - p.y * z.y;
+            si += p.x * z.y + p.y * z.x;
+        }
+        v[i + (int64_t)gcol(j) * n] = double2{sr, si};
+    }
+    if (tid == 0) flags_cur[tk.x] = 1;
+}
+
+// V = I per matrix; flags row 0 = 1 (every matrix runs its first sweep), later rows = 0.
+__global__ void bj_setup_kernel(const BjMat* __restrict__ mats, int nmat, double2* __restrict__ V,
+                                int* __restrict__ flags, int nrows) {
+    const int mi = blockIdx.x;
+    if (mi < nmat) {
+        const BjMat mt = mats[mi];
+        double2* v = V + mt.v_off;
+        const int64_t nn = (int64_t)mt.n * mt.n;
+        for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < nn;
+             e += (int64_t)gridDim.y * blockDim.x)
+            v[e] = double2{(e % mt.n) == (e / mt.n) ? 1.0 : 0.0, 0.0};
+    }
+    if (blockIdx.x == 0 && blockIdx.y == 0)
+        for (int e = threadIdx.x; e < nmat * nrows; e += blockDim.x) flags[e] = e < nmat ? 1 : 0;
+}
+
+// Singular values, order, U / S / Vh of converged Jacobi factors (one workgroup per matrix).
+__global__ __launch_bounds__(kSvdThreads) void svd_finish_kernel(const SvdJob* __restrict__ jobs,
+                                                                   const double2* __restrict__ G,
+                                                                   const double2* __restrict__ V,
+                                                                   double2* __restrict__ U,
+                                                                   double* __restrict__ S,
+                                                                   double2* __restrict__ Vh) {
+    extern __shared__ double smem[];
+    const SvdJob job = jobs[blockIdx.x];
+    double* sig = smem;
+    int* perm = reinterpret_cast<int*>(smem + job.n);
+    svd_finish(job, G + job.g_off, V + job.v_off, U, S, Vh, sig, perm);
+}
+
 }  // namespace
 }  // namespace bug
 
@@ -576,6 +771,100 @@ T* upload_list(bug_ctx* ctx, hipStream_t st, const T* host, size_t count, int sl
     void* dev = scratch(ctx, st, sizeof(T) * std::max<size_t>(count, 1), slot);
     HIP_CHECK(hipMemcpyAsync(dev, host, sizeof(T) * count, hipMemcpyHostToDevice, st));
     return static_cast<T*>(dev);
+}
+
+// LDS bytes the block-Jacobi stage may use: the device's per-workgroup limit less the Z matrix
+// and the kernel's static arrays, at most kBjStageBytes.
+size_t bj_stage_bytes(bug_ctx* ctx) {
+    if (ctx->lds_limit == 0) {
+        int v = 0;
+        HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+        ctx->lds_limit = v > 0 ? v : 65536;
+    }
+    const size_t reserve = sizeof(double2) * kBjMaxCols * kBjMaxCols + 1024;
+    const size_t lim = (size_t)ctx->lds_limit;
+    return lim > reserve ? std::min(kBjStageBytes, lim - reserve) : 0;
+}
+
+// Columns staged per workgroup for an n-column matrix (even, 2 .. kBjMaxCols), 0 if 2 do not fit.
+int bj_cols(size_t stage_bytes, int n) {
+    const int64_t c = std::min<int64_t>(kBjMaxCols, (int64_t)(stage_bytes / sizeof(double2)) / std::max(n, 1));
+    return c >= 2 ? (int)(c & ~1) : 0;
+}
+
+bool block_jacobi_fits(bug_ctx* ctx, int nmax) { return bj_cols(bj_stage_bytes(ctx), nmax) >= 2; }
+
+// Block one-sided Jacobi of the n x n matrices X_i (at the running offsets sum_{k<i} n_k^2 of X,
+// leading dimension n) with V_i at list[i].v_off of V (set to I here): sweeps of round launches
+// until a sweep rotates nothing in any matrix (the host reads the per-sweep flags) or max_sweeps.
+void block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list, double2* X, double2* V, int max_sweeps, int* sweeps_dev) {
+    const int nmat = (int)list.size();
+    const size_t stage = bj_stage_bytes(ctx);
+    std::vector<BjMat> mats(nmat);
+    std::vector<int2> tasks;
+    int64_t xo = 0;
+    int rounds = 0;
+    size_t lds = 0;
+    for (int i = 0; i < nmat; ++i) {
+        const int n = list[i].n;
+        const int c = bj_cols(stage, n);
+        if (c < 2) throw Error(BUG_EINVAL, "block Jacobi: matrix too wide for LDS");
+        const int w = c / 2;
+        const int nb = (n + w - 1) / w;
+        const int N = nb + (nb & 1);
+        mats[i] = BjMat{xo, list[i].v_off, n, w, nb, N};
+        xo += (int64_t)n * n;
+        if (n >= 2) {
+            for (int p = 0; p < N / 2; ++p) tasks.push_back(make_int2(i, p));
+            rounds = std::max(rounds, N - 1);
+            lds = std::max(lds, sizeof(double2) * ((size_t)n * std::min(c, n) + (size_t)c * c));
+        }
+    }
+    const int nrows = max_sweeps + 1;
+    int* flags = static_cast<int*>(scratch(ctx, st, sizeof(int) * (size_t)nmat * nrows, 3));
+    std::vector<unsigned char> blob(sizeof(BjMat) * nmat + sizeof(int2) * tasks.size());
+    std::memcpy(blob.data(), mats.data(), sizeof(BjMat) * nmat);
+    if (!tasks.empty()) std::memcpy(blob.data() + sizeof(BjMat) * nmat, tasks.data(), sizeof(int2) * tasks.size());
+    unsigned char* dblob = upload_list(ctx, st, blob.data(), blob.size(), 2);
+    const BjMat* dm = reinterpret_cast<const BjMat*>(dblob);
+    const int2* dt = reinterpret_cast<const int2*>(dblob + sizeof(BjMat) * nmat);
+    hipLaunchKernelGGL(bj_setup_kernel, dim3(nmat, 16), dim3(256), 0, st, dm, nmat, V, flags, nrows);
+    HIP_CHECK(hipGetLastError());
+    int used = 0;
+    std::vector<int> host(nmat);
+    if (!tasks.empty()) {
+        if (lds > 65536)
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(bj_round_kernel),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        int* hp = reinterpret_cast<int*>(pinned(ctx, sizeof(int) * nmat));
+        for (int s = 0; s < max_sweeps; ++s) {
+            for (int r = 0; r < rounds; ++r)
+                hipLaunchKernelGGL(bj_round_kernel, dim3((unsigned)tasks.size()), dim3(kBjThreads), lds,
+                                   st, dm, dt, X, V, r, flags + (size_t)s * nmat,
+                                   flags + (size_t)(s + 1) * nmat, 2.220446049250313e-16);
+            HIP_CHECK(hipGetLastError());
+            ++used;
+            HIP_CHECK(hipMemcpyAsync(hp, flags + (size_t)(s + 1) * nmat, sizeof(int) * nmat,
+                                     hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipStreamSynchronize(st));
+            bool any = false;
+            for (int i = 0; i < nmat; ++i) any |= hp[i] != 0;
+            if (!any) break;
+        }
+    }
+    if (sweeps_dev) {
+        // sweeps per matrix: the sweeps that rotated plus the one that found it converged
+        std::vector<int> all((size_t)nmat * nrows);
+        HIP_CHECK(hipMemcpyAsync(all.data(), flags, sizeof(int) * all.size(), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        for (int i = 0; i < nmat; ++i) {
+            int k = 0;
+            for (int s = 1; s <= used; ++s) k += all[(size_t)s * nmat + i] != 0;
+            host[i] = std::min(k + 1, used);
+        }
+        HIP_CHECK(hipMemcpyAsync(sweeps_dev, host.data(), sizeof(int) * nmat, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+    }
 }
 }  // namespace
 
@@ -809,10 +1098,17 @@ int bug_block_svd(bug_ctx* ctx, int32_t nmat, const bug_svd_job* jobs, void* g, 
             hipLaunchKernelGGL(copy_blocks_kernel, dim3(nmat, 8), dim3(256), 0, st, dx, Rw, Xw,
                                nullptr, 0);
             HIP_CHECK(hipGetLastError());
-            SvdJob* dj = upload_list(ctx, st, jj.data(), jj.size(), 2);
             const size_t lds = (size_t)nmax * (sizeof(double) + sizeof(int));
-            hipLaunchKernelGGL(svd_jacobi_kernel, dim3(nmat), dim3(kSvdThreads), lds, st, dj, Xw,
-                               v, Uxw, s, Vxw, sweeps, sw);
+            if (ctx->svd_variant == 2 || !block_jacobi_fits(ctx, nmax)) {
+                SvdJob* dj = upload_list(ctx, st, jj.data(), jj.size(), 2);
+                hipLaunchKernelGGL(svd_jacobi_kernel, dim3(nmat), dim3(kSvdThreads), lds, st, dj,
+                                   Xw, v, Uxw, s, Vxw, sweeps, sw);
+            } else {
+                block_jacobi(ctx, st, list, Xw, v, sweeps, sw);
+                SvdJob* dj = upload_list(ctx, st, jj.data(), jj.size(), 2);
+                hipLaunchKernelGGL(svd_finish_kernel, dim3(nmat), dim3(kSvdThreads), lds, st, dj,
+                                   Xw, v, Uxw, s, Vxw);
+            }
             HIP_CHECK(hipGetLastError());
             // U = Q Vx  with Vx = (Vh_x)^H, one grouped-GEMM launch over the batch
             Hasher h;
@@ -857,7 +1153,7 @@ extern "C" {
 
 int bug_set_svd_variant(bug_ctx* ctx, int32_t variant, int32_t record_sweeps) {
     return guarded(ctx, [&] {
-        if (variant < 0 || variant > 2) throw Error(BUG_EINVAL, "svd variant must be 0, 1 or 2");
+        if (variant < 0 || variant > 3) throw Error(BUG_EINVAL, "svd variant must be 0 .. 3");
         ctx->svd_variant = variant;
         if (record_sweeps && !ctx->svd_sweeps_dev) {
             HIP_CHECK(hipMalloc(&ctx->svd_sweeps_dev, sizeof(int) * 65536));

@@ -55,10 +55,10 @@ def empty_buffer(n: int) -> torch.Tensor:
 # ---- products ------------------------------------------------------------------------------
 
 def product(row: Leg, col: Leg, shifts, P: BlockTensor, Q: BlockTensor, terms,
-            out: Optional[BlockTensor] = None) -> BlockTensor:
+            out: Optional[BlockTensor] = None, precision: int = _lib.PREC_FP64) -> BlockTensor:
     """out[o] (+)= sum alpha op_a(P[p]) op_b(Q[q]); a fresh tensor unless ``out`` is given."""
     lay = Layout(row, col, shifts) if out is None else out.layout
-    return bsm_product(lay, P, Q, terms, accumulate_into=out)
+    return bsm_product(lay, P, Q, terms, accumulate_into=out, precision=precision)
 
 
 def core_times_site(C: BlockTensor, A: BlockTensor) -> BlockTensor:

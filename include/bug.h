@@ -218,7 +218,8 @@ int bug_block_svd(bug_ctx* ctx, int32_t nmat, const bug_svd_job* jobs, void* g, 
                   void* u, int64_t u_nelem, double* s, int64_t s_nelem, void* vh, int64_t vh_nelem,
                   int32_t max_sweeps, void* stream);
 
-/* Diagnostics: force the SVD kernel (0 automatic, 1 streaming one-sided Jacobi, 2 LDS block
+/* Diagnostics: force the SVD kernel (0 automatic, 1 one-sided Jacobi with one workgroup per
+ * matrix, 2 QR-preconditioned one-workgroup Jacobi, 3 QR-preconditioned multi-workgroup block
  * Jacobi) and, with record_sweeps != 0, record the Jacobi sweeps each matrix of a batch used;
  * bug_svd_sweeps copies the counts of the last batch (synchronizes the device). */
 int bug_set_svd_variant(bug_ctx* ctx, int32_t variant, int32_t record_sweeps);

@@ -86,3 +86,39 @@ def test_invalid_options_refused_before_touching_state(gpu):
         B.cbe_bug_step(psi, xxz_mpo(6), -0.1j, order=2)
     for a, t in zip(before, psi.tensors):
         assert bool((a == t.data).all())
+
+
+@pytest.mark.parametrize("start,L,delta,steps", [("neel", 12, 0.5, 4), ("dw", 12, 0.0, 4),
+                                                 ("neel", 16, 0.5, 8)])
+def test_mixed_precision_step_error(gpu, start, L, delta, steps):
+    """FP32-MFMA contractions (FP64 Krylov/QR/SVD): magnetisation within 1e-6 of the FP64 oracle
+    after every step, budget verdict unchanged; the measured errors are printed."""
+    from bondupdategalerkin_amd import bug_step as B
+    from bondupdategalerkin_amd import mps
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    psi_m = _device_state(start, L)
+    psi_o = _oracle_state(start, L)
+    mpo_d, mpo_o = xxz_mpo(L, delta=delta), om.xxz_mpo(L, delta=delta)
+    for n in range(steps):
+        info_m = B.cbe_bug_step(psi_m, mpo_d, -0.05j, precision="mixed",
+                                work_ceiling_rule="maxdim")
+        info_o = obug.cbe_bug_step(psi_o, mpo_o, -0.05j,
+                                   obug.StepOptions(work_ceiling_rule="maxdim"))
+        dsz = np.abs(mps.magnetisation(psi_m) - obug.sz_profile(psi_o)).max()
+        vm, vo = _dense(psi_m), om.mps_dense(psi_o.sites)
+        fid = abs(np.vdot(vm, vo)) / (np.linalg.norm(vm) * np.linalg.norm(vo))
+        print(f"mixed step {n}: max |dSz| {dsz:.2e}, 1-fidelity {1 - fid:.2e}, "
+              f"bonds {psi_m.bond_dims() == psi_o.bond_dims()}")
+        assert B.budget_status(info_m.ledger) == info_o.ledger.status == "met", n
+        assert dsz <= 1e-6, n
+        assert 1 - fid <= 1e-10, n
+
+
+def test_mixed_precision_refuses_unknown_form(gpu):
+    from bondupdategalerkin_amd import bug_step as B
+    from bondupdategalerkin_amd import mps
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    with pytest.raises(ValueError):
+        B.cbe_bug_step(mps.neel_state(6), xxz_mpo(6), -0.1j, precision="bf16")
