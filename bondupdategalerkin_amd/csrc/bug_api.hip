@@ -290,6 +290,54 @@ void upload_plan(Plan& plan, int config) {
     plan.config = config;
 }
 
+void stage_plans(bug_ctx* ctx, hipStream_t stream, std::vector<Plan>& plans, int slot) {
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    std::vector<std::vector<GemmTile>> tiles(plans.size());
+    size_t total = 0;
+    for (size_t p = 0; p < plans.size(); ++p) {
+        Plan& plan = plans[p];
+        plan.config = ctx->forced_config >= 0 ? ctx->forced_config : choose_config(plan);
+        const TileConfig& tc = kTileConfigs[plan.config];
+        for (size_t t = 0; t < plan.tasks.size(); ++t) {
+            const GemmTask& task = plan.tasks[t];
+            const int mt = (task.m + tc.tm() - 1) / tc.tm();
+            const int nt = (task.n + tc.tn() - 1) / tc.tn();
+            for (int a = 0; a < mt; ++a)
+                for (int b = 0; b < nt; ++b) tiles[p].push_back(GemmTile{(int32_t)t, a, b, 0});
+        }
+        plan.ntiles = (int32_t)tiles[p].size();
+        total += align(tiles[p].size() * sizeof(GemmTile)) + align(plan.tasks.size() * sizeof(GemmTask)) +
+                 align(plan.terms.size() * sizeof(GemmTerm));
+    }
+    if (total == 0) return;
+    std::vector<char> host(total, 0);
+    char* dev = static_cast<char*>(scratch(ctx, stream, total, slot));
+    size_t off = 0;
+    for (size_t p = 0; p < plans.size(); ++p) {
+        Plan& plan = plans[p];
+        const size_t bt = tiles[p].size() * sizeof(GemmTile);
+        const size_t bk = plan.tasks.size() * sizeof(GemmTask);
+        const size_t bm = plan.terms.size() * sizeof(GemmTerm);
+        if (bt) std::memcpy(host.data() + off, tiles[p].data(), bt);
+        plan.d_tiles = reinterpret_cast<const GemmTile*>(dev + off);
+        off += align(bt);
+        if (bk) std::memcpy(host.data() + off, plan.tasks.data(), bk);
+        plan.d_tasks = reinterpret_cast<const GemmTask*>(dev + off);
+        off += align(bk);
+        if (bm) std::memcpy(host.data() + off, plan.terms.data(), bm);
+        plan.d_terms = reinterpret_cast<const GemmTerm*>(dev + off);
+        off += align(bm);
+    }
+    HIP_CHECK(hipMemcpyAsync(dev, host.data(), total, hipMemcpyHostToDevice, stream));
+}
+
+void launch_staged_plan(bug_ctx* ctx, const Plan& plan, int precision, const void* A, const void* B,
+                        void* C, hipStream_t stream) {
+    if (plan.ntiles == 0) return;
+    ctx->last_flops += plan.flops;
+    HIP_CHECK(launch_grouped_gemm(plan, plan.config, precision, A, B, C, stream));
+}
+
 // Look up (or build) the plan of one product stage and launch it.
 void run_product(bug_ctx* ctx, const Bsm& out, void* out_dev, const Bsm& P, const void* p_dev,
                  const Bsm& Q, const void* q_dev, const std::vector<bug_term>& terms,
@@ -439,6 +487,7 @@ int bug_ctx_destroy(bug_ctx* ctx) {
         for (auto& s : slot)
             if (s.second.ptr) (void)hipFree(s.second.ptr);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->svd_sweeps_dev) (void)hipFree(ctx->svd_sweeps_dev);
     delete ctx;
     return BUG_OK;
 }

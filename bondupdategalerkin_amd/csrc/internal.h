@@ -133,6 +133,13 @@ struct Hasher {
 hipError_t launch_grouped_gemm(const Plan& plan, int config, int precision, const void* A,
                                const void* B, void* C, hipStream_t stream);
 
+// Build the tiles of several transient plans (each under its chosen configuration), copy them all to
+// one device blob in scratch `slot` with one asynchronous copy, and point the plans into it.
+void stage_plans(bug_ctx* ctx, hipStream_t stream, std::vector<Plan>& plans, int slot);
+// Launch a plan prepared by stage_plans (no-op for a plan without tiles).
+void launch_staged_plan(bug_ctx* ctx, const Plan& plan, int precision, const void* A, const void* B,
+                        void* C, hipStream_t stream);
+
 struct Workspace {
     void* ptr = nullptr;
     size_t bytes = 0;
@@ -156,12 +163,14 @@ struct bug_ctx {
     double last_flops = 0.0;     // algorithmic flops of the last entry-point call
     // per-stream scratch for intermediates of the two-stage contractions and the solvers,
     // keyed by (stream, slot)
-    std::unordered_map<void*, bug::Workspace> scratch[5];
+    std::unordered_map<void*, bug::Workspace> scratch[8];
     // SVD kernel choice (0 automatic, 1 plain one-sided Jacobi, 2 QR-preconditioned with one workgroup
     // per matrix, 3 QR-preconditioned multi-workgroup block Jacobi) and an optional
     // device array receiving the sweeps used per matrix of the last batch (diagnostics)
     int svd_variant = 0;
     int* svd_sweeps_dev = nullptr;
+    // QR kernel choice (0 automatic, 1 one workgroup per matrix, 2 blocked panels where rows fit)
+    int qr_variant = 0;
     // dynamic LDS bytes a workgroup may use on this device (queried on first use)
     int lds_limit = 0;
     // pinned host staging for small reductions

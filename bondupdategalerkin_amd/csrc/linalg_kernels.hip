@@ -312,6 +312,233 @@ __global__ __launch_bounds__(kQrThreads) void qr_kernel(const MatJob* __restrict
     }
 }
 
+// ---- blocked Householder QR (panels + grouped-GEMM updates) -------------------------------
+//
+// Panels of PW columns (16, or 8 for blocks of more than 512 rows so that a lane's RPT rows stay in
+// registers).  The panel kernel (one workgroup per matrix, one wave per panel column, the
+// column's rows held in registers, RPT per lane) factors the panel with the same
+// reflectors as qr_kernel, then builds the compact-WY factor T (H_p0 ... H_p0+jb-1 = I - V T V^H)
+// and Y = V T.  The trailing columns are updated with two grouped GEMMs, W = Y^H A_trail and
+// A_trail -= V W; Q = H_0 ... H_{k-1} [I; 0] is formed backwards per panel with W = V^H Q_sub,
+// Q_sub -= Y W.  qr_finish_kernel extracts R and applies the positive-diagonal gauge.
+
+constexpr int kQrPanel = 16;   // widest panel
+
+struct QrPanelJob {
+    int64_t a_off, v_off, y_off;   // A (lda); V and Y: m x k column-major, leading dimension m
+    int32_t m, n, lda, pad;
+};
+
+template <int RPT, int PW>
+__global__ __launch_bounds__(64 * PW) void qr_panel_kernel(const QrPanelJob* __restrict__ jobs, int p0,
+                                                           double2* __restrict__ A,
+                                                           double2* __restrict__ Vb,
+                                                           double2* __restrict__ Yb) {
+    __shared__ double2 vbuf[2][64 * RPT];
+    __shared__ double2 s_tau[PW];
+    __shared__ double2 Gs[PW][PW];
+    __shared__ double2 Ts[PW][PW];
+    const QrPanelJob job = jobs[blockIdx.x];
+    const int m = job.m, n = job.n, k = min(m, n);
+    if (p0 >= k) return;
+    const int jb = min(PW, k - p0);
+    const int mp = m - p0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t lda = job.lda;
+    const bool own = wave < jb;
+    double2* col = A + job.a_off + p0 + (int64_t)(p0 + (own ? wave : 0)) * lda;
+    double2 x[RPT];
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+        const int r = lane + 64 * i;
+        x[i] = (own && r < mp) ? col[r] : double2{0.0, 0.0};
+    }
+    for (int j = 0; j < jb; ++j) {
+        double2* buf = vbuf[j & 1];
+        if (wave == j) {
+            double x2 = 0.0;
+#pragma unroll
+            for (int i = 0; i < RPT; ++i)
+                if (lane + 64 * i > j) x2 += x[i].x * x[i].x + x[i].y * x[i].y;
+            x2 = wave_sum(x2);
+            const double ar = __shfl(x[0].x, j, 64), ai = __shfl(x[0].y, j, 64);  // j < 64
+            const bool refl = !(x2 == 0.0 && ai == 0.0);
+            double2 t{0.0, 0.0};
+            double beta = 0.0, sr = 0.0, si = 0.0;
+            if (refl) {
+                const double nrm = sqrt(ar * ar + ai * ai + x2);
+                beta = ar >= 0.0 ? -nrm : nrm;
+                t = double2{(beta - ar) / beta, -ai / beta};
+                const double dr = ar - beta, di = ai;
+                const double den = dr * dr + di * di;
+                sr = dr / den;
+                si = -di / den;
+            }
+#pragma unroll
+            for (int i = 0; i < RPT; ++i) {
+                const int r = lane + 64 * i;
+                double2 vv{0.0, 0.0};
+                if (r == j) {
+                    vv = double2{1.0, 0.0};
+                    if (refl) x[i] = double2{beta, 0.0};
+                } else if (r > j) {
+                    if (refl) {
+                        const double2 u = x[i];
+                        x[i] = double2{u.x * sr - u.y * si, u.x * si + u.y * sr};
+                        vv = x[i];
+                    } else {
+                        x[i] = double2{0.0, 0.0};
+                    }
+                }
+                buf[r] = vv;
+            }
+            if (lane == 0) s_tau[j] = t;
+        }
+        __syncthreads();
+        const double2 t = s_tau[j];
+        if (wave > j && own && (t.x != 0.0 || t.y != 0.0)) {
+            // x <- (I - conj(tau) v v^H) x
+            double wr = 0.0, wi = 0.0;
+#pragma unroll
+            for (int i = 0; i < RPT; ++i) {
+                const double2 v = buf[lane + 64 * i];
+                wr += v.x * x[i].x + v.y * x[i].y;
+                wi += v.x * x[i].y - v.y * x[i].x;
+            }
+            wr = wave_sum(wr);
+            wi = wave_sum(wi);
+            const double fr = t.x * wr + t.y * wi;
+            const double fi = t.x * wi - t.y * wr;
+#pragma unroll
+            for (int i = 0; i < RPT; ++i) {
+                const double2 v = buf[lane + 64 * i];
+                x[i].x -= fr * v.x - fi * v.y;
+                x[i].y -= fr * v.y + fi * v.x;
+            }
+        } else if (wave < j) {
+            // G[wave][j] = v_wave^H v_j, v_wave = (0 above, 1 at wave, x below)
+            double gr = 0.0, gi = 0.0;
+#pragma unroll
+            for (int i = 0; i < RPT; ++i) {
+                const int r = lane + 64 * i;
+                if (r < wave) continue;
+                const double2 v = r == wave ? double2{1.0, 0.0} : x[i];
+                const double2 b = buf[r];
+                gr += v.x * b.x + v.y * b.y;
+                gi += v.x * b.y - v.y * b.x;
+            }
+            gr = wave_sum(gr);
+            gi = wave_sum(gi);
+            if (lane == 0) Gs[wave][j] = double2{gr, gi};
+        }
+    }
+    // the panel column back into A (R above the diagonal, v below) and explicit V
+    if (own) {
+        double2* vcol = Vb + job.v_off + p0 + (int64_t)(p0 + wave) * m;
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+            const int r = lane + 64 * i;
+            if (r < mp) col[r] = x[i];
+            x[i] = r < wave ? double2{0.0, 0.0} : (r == wave ? double2{1.0, 0.0} : x[i]);
+            if (r < mp) vcol[r] = x[i];
+        }
+    }
+    __syncthreads();
+    // T: T[c][c] = tau_c, T[0:c, c] = -tau_c T[0:c, 0:c] G[0:c, c]
+    for (int c = 0; c < jb; ++c) {
+        const int a = threadIdx.x;
+        if (a < c) {
+            double sr = 0.0, si = 0.0;
+            for (int b = a; b < c; ++b) {
+                const double2 tv = Ts[a][b], g = Gs[b][c];
+                sr += tv.x * g.x - tv.y * g.y;
+                si += tv.x * g.y + tv.y * g.x;
+            }
+            const double2 tc = s_tau[c];
+            Ts[a][c] = double2{-(tc.x * sr - tc.y * si), -(tc.x * si + tc.y * sr)};
+        } else if (a == c) {
+            Ts[c][c] = s_tau[c];
+        } else if (a < PW) {
+            Ts[a][c] = double2{0.0, 0.0};
+        }
+        __syncthreads();
+    }
+    // Y = V T, column c = sum_{a <= c} v_a T[a][c]: broadcast v_a for a = jb-1 .. 0; wave a turns
+    // its own v into y_a = v_a T[a][a] right after broadcasting it.
+    for (int a = jb - 1; a >= 0; --a) {
+        double2* buf = vbuf[a & 1];
+        if (wave == a) {
+            const double2 ta = Ts[a][a];
+#pragma unroll
+            for (int i = 0; i < RPT; ++i) {
+                buf[lane + 64 * i] = x[i];
+                const double2 v = x[i];
+                x[i] = double2{v.x * ta.x - v.y * ta.y, v.x * ta.y + v.y * ta.x};
+            }
+        }
+        __syncthreads();
+        if (wave > a && own) {
+            const double2 ta = Ts[a][wave];
+#pragma unroll
+            for (int i = 0; i < RPT; ++i) {
+                const double2 v = buf[lane + 64 * i];
+                x[i].x += v.x * ta.x - v.y * ta.y;
+                x[i].y += v.x * ta.y + v.y * ta.x;
+            }
+        }
+    }
+    if (own) {
+        double2* ycol = Yb + job.y_off + p0 + (int64_t)(p0 + wave) * m;
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+            const int r = lane + 64 * i;
+            if (r < mp) ycol[r] = x[i];
+        }
+    }
+}
+
+// Q = [I_k; 0] (m x k, ldq) per matrix.
+__global__ void qr_qinit_kernel(const MatJob* __restrict__ jobs, double2* __restrict__ Q) {
+    const MatJob job = jobs[blockIdx.x];
+    const int m = job.m, k = min(job.m, job.n);
+    double2* q = Q + job.q_off;
+    const int64_t total = (int64_t)m * k;
+    for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.y * blockDim.x) {
+        const int r = (int)(e % m), c = (int)(e / m);
+        q[r + c * (int64_t)job.ldq] = double2{r == c ? 1.0 : 0.0, 0.0};
+    }
+}
+
+// R = upper triangle of the factored A (k x n); rows with a negative diagonal are negated together
+// with the matching columns of Q (the unique gauge).
+__global__ __launch_bounds__(1024) void qr_finish_kernel(const MatJob* __restrict__ jobs,
+                                                         const double2* __restrict__ A,
+                                                         double2* __restrict__ Q,
+                                                         double2* __restrict__ R) {
+    const MatJob job = jobs[blockIdx.x];
+    const int m = job.m, n = job.n, k = min(m, n);
+    const double2* a = A + job.a_off;
+    double2* q = Q + job.q_off;
+    double2* rr = R + job.r_off;
+    const int64_t lda = job.lda, ldq = job.ldq, ldr = job.ldr;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = blockDim.x / 64;
+    for (int j = wave; j < k; j += nw) {
+        const bool neg = a[j + (int64_t)j * lda].x < 0.0;
+        for (int c = lane; c < n; c += 64) {
+            double2 v = c >= j ? a[j + (int64_t)c * lda] : double2{0.0, 0.0};
+            if (neg) v = double2{-v.x, -v.y};
+            rr[j + (int64_t)c * ldr] = v;
+        }
+        if (neg)
+            for (int i = lane; i < m; i += 64) {
+                const double2 v = q[i + (int64_t)j * ldq];
+                q[i + (int64_t)j * ldq] = double2{-v.x, -v.y};
+            }
+    }
+}
+
 // ---- batched SVD: one-sided Jacobi ---------------------------------------------------------
 
 constexpr int kSvdThreads = 1024;
@@ -773,6 +1000,121 @@ T* upload_list(bug_ctx* ctx, hipStream_t st, const T* host, size_t count, int sl
     return static_cast<T*>(dev);
 }
 
+// The blocked QR covers batches whose blocks have at most 64 * 16 rows (a panel column per wave in
+// registers) and at least one block wide enough for panels to pay off.
+bool blocked_qr_fits(const std::vector<MatJob>& jobs) {
+    int mmax = 0, kmax = 0;
+    for (const MatJob& j : jobs) {
+        mmax = std::max(mmax, j.m);
+        kmax = std::max(kmax, std::min(j.m, j.n));
+    }
+    return mmax <= 64 * 16 && kmax >= 3 * kQrPanel;
+}
+
+bool use_blocked_qr(bug_ctx* ctx, const std::vector<MatJob>& jobs) {
+    if (ctx->qr_variant == 1) return false;
+    int mmax = 0;
+    for (const MatJob& j : jobs) mmax = std::max(mmax, j.m);
+    if (ctx->qr_variant == 2) return mmax <= 64 * 16;
+    return blocked_qr_fits(jobs);
+}
+
+// Thin QR of every job (A overwritten, Q m x k, R k x n) by panels and grouped-GEMM updates.
+void blocked_qr(bug_ctx* ctx, hipStream_t st, const std::vector<MatJob>& jobs, double2* A,
+                double2* Q, double2* R) {
+    const int nmat = (int)jobs.size();
+    std::vector<QrPanelJob> pj(nmat);
+    std::vector<int64_t> woff(nmat);
+    int64_t vt = 0, wt = 0;
+    int kmax = 0, mmax = 0;
+    for (const MatJob& j : jobs) mmax = std::max(mmax, j.m);
+    const int pw = mmax <= 512 ? 16 : 8;
+    for (int i = 0; i < nmat; ++i) {
+        const MatJob& j = jobs[i];
+        const int k = std::min(j.m, j.n);
+        pj[i] = QrPanelJob{j.a_off, vt, vt, j.m, j.n, j.lda, 0};
+        woff[i] = wt;
+        vt += (int64_t)j.m * k;
+        wt += (int64_t)pw * std::max(j.n, k);
+        kmax = std::max(kmax, k);
+    }
+    double2* ws = static_cast<double2*>(
+        scratch(ctx, st, sizeof(double2) * std::max<int64_t>(2 * vt + wt, 1), 5));
+    double2* Vb = ws;
+    double2* Yb = ws + vt;
+    double2* Wb = ws + 2 * vt;
+    const int npan = (kmax + pw - 1) / pw;
+    // plans: [2p] trailing W = Y^H A_t, [2p+1] A_t -= V W, [2 npan + 2p] W = V^H Q_s,
+    // [2 npan + 2p + 1] Q_s -= Y W
+    std::vector<Plan> plans(4 * (size_t)npan);
+    for (int p = 0; p < npan; ++p) {
+        const int p0 = p * pw;
+        Plan& t1 = plans[2 * p];
+        Plan& t2 = plans[2 * p + 1];
+        Plan& q1 = plans[2 * npan + 2 * p];
+        Plan& q2 = plans[2 * npan + 2 * p + 1];
+        for (int i = 0; i < nmat; ++i) {
+            const MatJob& j = jobs[i];
+            const int k = std::min(j.m, j.n);
+            if (k <= p0) continue;
+            const int jb = std::min(pw, k - p0);
+            const int mp = j.m - p0;
+            const int64_t vo = pj[i].v_off + p0 + (int64_t)p0 * j.m;
+            const int nt = j.n - p0 - jb;
+            if (nt > 0) {
+                const int64_t at = j.a_off + p0 + (int64_t)(p0 + jb) * j.lda;
+                t1.tasks.push_back(GemmTask{woff[i], pw, jb, nt, (int32_t)t1.terms.size(),
+                                            (int32_t)t1.terms.size() + 1, 0});
+                t1.terms.push_back(GemmTerm{vo, at, j.m, j.lda, mp, BUG_OP_C | (BUG_OP_N << 2), 1.0, 0.0});
+                t1.flops += 8.0 * jb * nt * (double)mp;
+                t2.tasks.push_back(GemmTask{at, j.lda, mp, nt, (int32_t)t2.terms.size(),
+                                            (int32_t)t2.terms.size() + 1, 1});
+                t2.terms.push_back(GemmTerm{vo, woff[i], j.m, pw, jb, BUG_OP_N | (BUG_OP_N << 2), -1.0, 0.0});
+                t2.flops += 8.0 * mp * nt * (double)jb;
+            }
+            const int nc = k - p0;
+            const int64_t qs = j.q_off + p0 + (int64_t)p0 * j.ldq;
+            q1.tasks.push_back(GemmTask{woff[i], pw, jb, nc, (int32_t)q1.terms.size(),
+                                        (int32_t)q1.terms.size() + 1, 0});
+            q1.terms.push_back(GemmTerm{vo, qs, j.m, j.ldq, mp, BUG_OP_C | (BUG_OP_N << 2), 1.0, 0.0});
+            q1.flops += 8.0 * jb * nc * (double)mp;
+            q2.tasks.push_back(GemmTask{qs, j.ldq, mp, nc, (int32_t)q2.terms.size(),
+                                        (int32_t)q2.terms.size() + 1, 1});
+            q2.terms.push_back(GemmTerm{vo, woff[i], j.m, pw, jb, BUG_OP_N | (BUG_OP_N << 2), -1.0, 0.0});
+            q2.flops += 8.0 * mp * nc * (double)jb;
+        }
+    }
+    stage_plans(ctx, st, plans, 6);
+    std::vector<unsigned char> blob(sizeof(QrPanelJob) * nmat + sizeof(MatJob) * nmat);
+    std::memcpy(blob.data(), pj.data(), sizeof(QrPanelJob) * nmat);
+    std::memcpy(blob.data() + sizeof(QrPanelJob) * nmat, jobs.data(), sizeof(MatJob) * nmat);
+    unsigned char* dblob = upload_list(ctx, st, blob.data(), blob.size(), 7);
+    const QrPanelJob* dp = reinterpret_cast<const QrPanelJob*>(dblob);
+    const MatJob* dj = reinterpret_cast<const MatJob*>(dblob + sizeof(QrPanelJob) * nmat);
+    hipLaunchKernelGGL(qr_qinit_kernel, dim3(nmat, 8), dim3(256), 0, st, dj, Q);
+    for (int p = 0; p < npan; ++p) {
+        const int p0 = p * pw;
+        int mp = 0;
+        for (const MatJob& j : jobs)
+            if (std::min(j.m, j.n) > p0) mp = std::max(mp, j.m - p0);
+        if (pw == 8)
+            hipLaunchKernelGGL((qr_panel_kernel<16, 8>), dim3(nmat), dim3(512), 0, st, dp, p0, A, Vb, Yb);
+        else if (mp <= 256)
+            hipLaunchKernelGGL((qr_panel_kernel<4, 16>), dim3(nmat), dim3(1024), 0, st, dp, p0, A, Vb, Yb);
+        else
+            hipLaunchKernelGGL((qr_panel_kernel<8, 16>), dim3(nmat), dim3(1024), 0, st, dp, p0, A, Vb, Yb);
+        HIP_CHECK(hipGetLastError());
+        launch_staged_plan(ctx, plans[2 * p], BUG_PREC_FP64, Yb, A, Wb, st);
+        launch_staged_plan(ctx, plans[2 * p + 1], BUG_PREC_FP64, Vb, Wb, A, st);
+    }
+    for (int p = npan - 1; p >= 0; --p) {
+        launch_staged_plan(ctx, plans[2 * npan + 2 * p], BUG_PREC_FP64, Vb, Q, Wb, st);
+        launch_staged_plan(ctx, plans[2 * npan + 2 * p + 1], BUG_PREC_FP64, Yb, Wb, Q, st);
+    }
+    hipLaunchKernelGGL(qr_finish_kernel, dim3(nmat), dim3(1024), 0, st, dj, A, Q, R);
+    HIP_CHECK(hipGetLastError());
+}
+
 // LDS bytes the block-Jacobi stage may use: the device's per-workgroup limit less the Z matrix
 // and the kernel's static arrays, at most kBjStageBytes.
 size_t bj_stage_bytes(bug_ctx* ctx) {
@@ -1009,12 +1351,24 @@ int bug_block_qr(bug_ctx* ctx, int32_t nmat, const bug_qr_job* jobs, void* a, in
             wtot += k;
         }
         hipStream_t st = as_stream(stream);
+        if (use_blocked_qr(ctx, list)) {
+            blocked_qr(ctx, st, list, static_cast<double2*>(a), static_cast<double2*>(q),
+                       static_cast<double2*>(r));
+            return;
+        }
         MatJob* d = upload_list(ctx, st, list.data(), list.size(), 2);
         double2* tau = static_cast<double2*>(scratch(ctx, st, sizeof(double2) * std::max<int64_t>(wtot, 1), 1));
         hipLaunchKernelGGL(qr_kernel, dim3(nmat), dim3(kQrThreads), 0, st, d,
                            static_cast<double2*>(a), static_cast<double2*>(q),
                            static_cast<double2*>(r), tau);
         HIP_CHECK(hipGetLastError());
+    });
+}
+
+int bug_set_qr_variant(bug_ctx* ctx, int32_t variant) {
+    return guarded(ctx, [&] {
+        if (variant < 0 || variant > 2) throw Error(BUG_EINVAL, "qr variant must be 0, 1 or 2");
+        ctx->qr_variant = variant;
     });
 }
 
@@ -1090,9 +1444,13 @@ int bug_block_svd(bug_ctx* ctx, int32_t nmat, const bug_svd_job* jobs, void* g, 
                 no += (int64_t)j.n * j.n;
                 to += j.n;
             }
-            MatJob* dq = upload_list(ctx, st, qj.data(), qj.size(), 2);
-            hipLaunchKernelGGL(qr_kernel, dim3(nmat), dim3(kQrThreads), 0, st, dq,
-                               static_cast<double2*>(g), Qw, Rw, tau);
+            if (use_blocked_qr(ctx, qj)) {
+                blocked_qr(ctx, st, qj, static_cast<double2*>(g), Qw, Rw);
+            } else {
+                MatJob* dq = upload_list(ctx, st, qj.data(), qj.size(), 2);
+                hipLaunchKernelGGL(qr_kernel, dim3(nmat), dim3(kQrThreads), 0, st, dq,
+                                   static_cast<double2*>(g), Qw, Rw, tau);
+            }
             HIP_CHECK(hipGetLastError());
             CopyOp* dx = upload_list(ctx, st, xc.data(), xc.size(), 2);
             hipLaunchKernelGGL(copy_blocks_kernel, dim3(nmat, 8), dim3(256), 0, st, dx, Rw, Xw,

@@ -223,6 +223,11 @@ int bug_block_svd(bug_ctx* ctx, int32_t nmat, const bug_svd_job* jobs, void* g, 
  * Jacobi) and, with record_sweeps != 0, record the Jacobi sweeps each matrix of a batch used;
  * bug_svd_sweeps copies the counts of the last batch (synchronizes the device). */
 int bug_set_svd_variant(bug_ctx* ctx, int32_t variant, int32_t record_sweeps);
+
+/* Diagnostics: force the QR kernel (0 automatic, 1 Householder with one workgroup per matrix,
+ * 2 blocked Householder: panels factored in registers, compact-WY updates on the grouped GEMM,
+ * for blocks of at most 1024 rows). */
+int bug_set_qr_variant(bug_ctx* ctx, int32_t variant);
 int bug_svd_sweeps(bug_ctx* ctx, int32_t n, int32_t* out);
 
 #ifdef __cplusplus

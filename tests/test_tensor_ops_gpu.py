@@ -150,3 +150,37 @@ def test_svd_graded_spectrum_batch(gpu, variant):
         assert np.abs(U.conj().T @ U - np.eye(n)).max() < 1e-11, q
         assert np.abs(Vh @ Vh.conj().T - np.eye(n)).max() < 1e-11, q
     ctx.check(ctx._lib.bug_set_svd_variant(ctx.handle, 0, 0))
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_qr_paths_on_wide_blocks(gpu, variant):
+    """Both QR paths (one workgroup per block; blocked panels with compact-WY GEMM updates, 16- and
+    8-column panels) give the unique thin QR (real non-negative R diagonal) of blocks up to
+    1000 x 480, tall and wide, and a rank-deficient block still gives Q R = A with Q^H Q = I."""
+    from bondupdategalerkin_amd import _lib
+    from bondupdategalerkin_amd import tensor_ops as T
+
+    ctx = _lib.context()
+    ctx.check(ctx._lib.bug_set_qr_variant(ctx.handle, variant))
+    rng = np.random.default_rng(11)
+    for dims in ({-2: (300, 140), 0: (500, 480), 2: (20, 50), 4: (61, 61)},
+                 {0: (1000, 470), 2: (700, 333), -4: (1, 3)}):
+        C = ob.Fam({q: d[0] for q, d in dims.items()}, {q: d[1] for q, d in dims.items()}, [0])
+        for q, (m, n) in dims.items():
+            C.blocks[(0, q)] = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+        C.blocks[(0, 0)][:, 7] = 0.0                       # rank-deficient: one zero column
+        Q, R = T.qr_core(device_from_fam(C))
+        Qf, Rf = fam_from_device(Q), fam_from_device(R)
+        for q, (m, n) in dims.items():
+            M = C.blocks[(0, q)]
+            Qd, Rd = Qf.blocks[(0, q)], Rf.blocks[(0, q)]
+            k = min(m, n)
+            assert Qd.shape == (m, k) and Rd.shape == (k, n)
+            assert np.abs(Qd @ Rd - M).max() < 1e-12 * np.abs(M).max(), q
+            assert np.abs(Qd.conj().T @ Qd - np.eye(k)).max() < 1e-12, q
+            assert np.all(np.abs(np.tril(Rd, -1)) == 0) and np.all(np.diag(Rd).real >= 0), q
+            assert np.abs(np.diag(Rd).imag).max() == 0, q
+            if q != 0:                                       # full rank: the unique factors
+                Qo, Ro = la._unique_qr(M)
+                assert np.abs(Qd - Qo).max() < 1e-11 and np.abs(Rd - Ro).max() < 1e-11 * np.abs(M).max(), q
+    ctx.check(ctx._lib.bug_set_qr_variant(ctx.handle, 0))
