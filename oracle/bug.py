@@ -261,19 +261,13 @@ def alloc_columns(dims: Dict[int, int], budget: int) -> Dict[int, int]:
     return {q: min(d, math.ceil(d / tot * budget)) for q, d in dims.items()}
 
 
-def dd_probe(frame: Fam, side: str, width: int, rng: la.ProbeRNG):
-    """_dd_probe (dd_cbe.jl:132-161): sector-graded Gaussian columns in the discarded space.
-
-    ``side`` is the side of ``frame`` (the opposite of the expanding frame).  Stream order: per
-    fused charge ascending, per slice ascending, a (link dim x dfull) block drawn column-major of
-    which the first ``cols[q]`` columns are kept.
-    """
-    dfull = fused_dims(frame, side)
-    dT = frame_widths(frame, side)
-    dC = {q: max(d - dT.get(q, 0), 0) for q, d in dfull.items()}
-    if sum(dC.values()) == 0:
-        return None
-    cols = alloc_columns(dC, width)
+def _graded_draw(frame: Fam, side: str, dfull: Dict[int, int], cols: Dict[int, int],
+                 rng: la.ProbeRNG) -> Fam:
+    """Gaussian columns on the fused local space of ``frame`` (``_randomize_like`` then
+    ``_trim_per_sector``, cbe.jl:156-178, truncation.jl:283-288).  Stream order: per fused charge
+    ascending, per slice ascending, a (link dim x dfull) block drawn column-major of which the
+    first ``cols[q]`` columns are kept; the layout is the frame's, the width leg in place of its
+    bond leg."""
     wl = {q: c for q, c in cols.items() if c > 0}
     if side == "left":
         Om = Fam(frame.row, wl, frame.shifts)
@@ -296,8 +290,34 @@ def dd_probe(frame: Fam, side: str, width: int, rng: la.ProbeRNG):
                 Om.blocks[(s, lq)] = blk
             else:
                 Om.blocks[(s, q)] = blk.T.copy()  # (c x dl)
+    return Om
+
+
+def dd_probe(frame: Fam, side: str, width: int, rng: la.ProbeRNG):
+    """_dd_probe (dd_cbe.jl:132-161): sector-graded Gaussian columns in the discarded space.
+
+    ``side`` is the side of ``frame`` (the opposite of the expanding frame); columns are shared
+    out by the complement dimension of each sector and the frame is projected out afterwards.
+    """
+    dfull = fused_dims(frame, side)
+    dT = frame_widths(frame, side)
+    dC = {q: max(d - dT.get(q, 0), 0) for q, d in dfull.items()}
+    if sum(dC.values()) == 0:
+        return None
+    Om = _graded_draw(frame, side, dfull, alloc_columns(dC, width), rng)
     proj = la.perp_left(frame, Om) if side == "left" else la.perp_right(frame, Om)
     return proj if proj.norm() > 0 else None
+
+
+def graded_probe(frame: Fam, side: str, width: int, rng: la.ProbeRNG):
+    """sector_graded_sketch with comp_ratio = nothing (sketch.jl:45-54): ``width`` columns shared
+    out by the FULL fused dimension of each sector, no projector in the probe."""
+    dfull = fused_dims(frame, side)
+    cols = alloc_columns(dfull, width)
+    if not any(c > 0 for c in cols.values()):
+        return None
+    Om = _graded_draw(frame, side, dfull, cols, rng)
+    return Om if Om.norm() > 0 else None
 
 
 def _width_of(T: Fam, side: str) -> int:
@@ -629,8 +649,12 @@ class KrylovStats:
 
 
 def krylov_enrich(seed: Fam, applyH, start_frame: Fam, side: str, tau: complex, alloc: Alloc,
-                  workcap: int, recurrence: str = "lanczos", maxdepth: int = ok.KRY_MAXDEPTH):
-    """_krylov_enrich + _krylov_basis + the chains (krylov_enrichment.jl:213-292, ...)."""
+                  workcap: int, recurrence: str = "lanczos", maxdepth: int = ok.KRY_MAXDEPTH,
+                  amp: float = 1.0, state_seed: bool = True):
+    """_krylov_enrich + _krylov_basis + the chains (krylov_enrichment.jl:213-292,
+    krylov_recurrence.jl:27-67, krylov_lanczos.jl:18-71, krylov_arnoldi.jl:13-58).  ``amp``
+    scales every direction's weight (a frontier seed's norm relative to the state) and
+    ``state_seed = False`` weighs a non-unitary step absolutely (_residual_integral)."""
     perp = la.perp_left if side == "left" else la.perp_right
     allow = alloc.krylov
     st = KrylovStats()
@@ -685,7 +709,7 @@ def krylov_enrich(seed: Fam, applyH, start_frame: Fam, side: str, tau: complex, 
             Hm[k - 1, k] = beta
             nv = img.scale(1.0 / beta)
             vs.append(nv)
-            w = ok.kry_weight(Hm, k, teff)
+            w = amp * ok.kry_weight(Hm, k, teff, state_seed)
             if not score(nv, w, k == 1):
                 capped = True
                 break
@@ -710,7 +734,7 @@ def krylov_enrich(seed: Fam, applyH, start_frame: Fam, side: str, tau: complex, 
             Hm[k, k - 1] = rn
             nv = img.scale(1.0 / rn)
             vs.append(nv)
-            w = ok.kry_weight(Hm, k, teff)
+            w = amp * ok.kry_weight(Hm, k, teff, state_seed)
             if not score(nv, w, k == 1):
                 capped = True
                 break
