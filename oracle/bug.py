@@ -189,6 +189,8 @@ class Ledger:
     r_old: int = 0
     r_aug: int = 0
     r_final: int = 0
+    leak: List[float] = field(default_factory=list)
+    defect: float = 0.0
     unmet: List[str] = field(default_factory=list)
 
     def unmet_(self, what):
@@ -777,15 +779,16 @@ def _cap_at(wcap, b):
 
 def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None,
                  probe_hook=None) -> StepInfo:
-    """One order-1 CBE-BUG step in place (bug_step.jl:17-510)."""
+    """One order-p CBE-BUG step in place (bug_step.jl:17-510); order > 1 closes the root
+    frontiers by one enriching sweep each (oracle/closure.py)."""
     opts = opts or StepOptions()
     L = len(psi)
     if len(mpo) != L:
         raise ValueError("MPO and state lengths differ")
     if L < 2:
         raise ValueError("cbe_bug_step needs at least two sites")
-    if opts.order != 1:
-        raise NotImplementedError("the order-p frontier closure is not restated in the oracle")
+    if opts.order < 1:
+        raise ValueError("order must be >= 1")
     rl = (max(1, min(L - 1, L // 2)) - 1) if opts.root is None else opts.root
     d = len(psi.sites[0].shifts)
     tau = complex(tau)
@@ -876,6 +879,18 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
         c = psi.sites[i] if AL is None else la.core_times_site(AL, psi.sites[i])
         AL = la.overlap_left(left_frames[i], c)
     root_start = la.product(AL.row, AR.col, [0], AL, AR, [(0, 0, 0, la.OP_N, la.OP_N, 1.0)])
+
+    # ---- order p > 1: close the root frontiers j = 1..p-1 (bug_step_stages.jl:172-240) ----
+    if opts.order > 1:
+        from .closure import KLFrames, single_pass_closure
+        fr = KLFrames(left_frames, right_frames, rl, lroot, rroot)
+        root_start, _ = single_pass_closure(fr, psi, mpo, root_start, tau, alloc, opts, wcap, nrm,
+                                            rng, recurrence, ledger, opts.order)
+        lroot, rroot = fr.lenv, fr.renv
+        for b in range(rl):
+            expanded[b] = sum(left_frames[b].col.values())
+        for b in range(rl + 1, L - 1):
+            expanded[b] = sum(right_frames[b].row.values())
     npsi = psi.sites[psi.center].norm()
     ledger.r_aug = max(max(expanded + [0]), sum(right_frames[rl].row.values()))
     ledger.proj_err = projection_miss(psi, left_frames, right_frames, rl) / npsi if npsi > 0 else 0.0

@@ -69,6 +69,65 @@ def test_xxz_step_local_error_is_second_order_against_ed():
     assert 20 < errs[0] / errs[1] < 30            # (0.05 / 0.01)^2 = 25
 
 
+def test_replays_frozen_order2_run():
+    """xx_u1_bug_order2 (L=10 Neel, dt 0.05, order 2): S^z to 1e-5 of the frozen run, error vs
+    free fermions <= 2 err + 1e-6, every status as frozen (test_fixtures_b155550.jl:40-66)."""
+    fx = json.load(open(GOLDEN))["xx_u1_bug_order2"]
+    L, dt = fx["L"], fx["dt"]
+    psi = obug.MPS(om.neel_state(L))
+    mpo = om.xxz_mpo(L, delta=0.0)
+    opts = obug.StepOptions(order=2, work_ceiling_rule="maxdim", krylov_mode="arnoldi")
+    occ = list(range(0, L, 2))
+    for n, row in enumerate(fx["steps"], start=1):
+        info = obug.cbe_bug_step(psi, mpo, -1j * dt, opts)
+        sz = obug.sz_profile(psi)
+        assert np.abs(sz - np.array(row["sz"])).max() <= 1e-5, n
+        assert np.abs(sz - free_fermion_sz(L, n * dt, occ)).max() <= 2 * row["err_exact"] + 1e-6, n
+        assert info.ledger.status == row["status"], n
+        assert len(info.ledger.leak) == 2 and info.ledger.leak[1] >= 0.0
+
+
+def test_replays_frozen_order2_run_without_symmetry():
+    """xx_none_bug_order2 (L=8 Neel, no symmetry, 10 steps): every norm within 1e-8 of frozen,
+    the final 256 amplitudes within 1e-5 of the frozen state and the error against the dense
+    propagator within 2 err_dense + 1e-6 (test_fixtures_b155550.jl:68-98)."""
+    fx = json.load(open(GOLDEN))["xx_none_bug_order2"]
+    L, dt = fx["L"], fx["dt"]
+    psi = obug.MPS(om.neel_state(L, "none"))
+    v0 = om.mps_dense(psi.sites)
+    mpo = om.xxz_mpo(L, delta=0.0, symmetry="none")
+    opts = obug.StepOptions(order=2, work_ceiling_rule="maxdim", krylov_mode="arnoldi")
+    for n, row in enumerate(fx["steps"], start=1):
+        info = obug.cbe_bug_step(psi, mpo, -1j * dt, opts)
+        assert abs(np.linalg.norm(om.mps_dense(psi.sites)) - row["norm"]) <= 1e-8, n
+        assert info.ledger.status == row["status"], n
+    v = om.mps_dense(psi.sites)
+    frozen = np.array(fx["final_state"]["re"]) + 1j * np.array(fx["final_state"]["im"])
+    assert np.linalg.norm(v - frozen) <= 1e-5
+    exact = sla.expm(-1j * len(fx["steps"]) * dt * om.dense_hamiltonian(L, delta=0.0)) @ v0
+    assert np.linalg.norm(v - exact) / np.linalg.norm(exact) <= 2 * fx["err_dense"] + 1e-6
+
+
+def test_order2_local_error_is_third_order_against_ed():
+    """One order-2 step (one enriching frontier sweep of F_1 = tau B S_0) from the Neel state of the
+    XXZ chain against exp(-i dt H): the local error falls 8x per halving of dt (O(dt^3)), where
+    the order-1 step falls 4x, and the energy is conserved."""
+    L, delta = 8, 0.5
+    H = om.dense_hamiltonian(L, delta=delta)
+    errs = []
+    for dt in (0.1, 0.05, 0.025):
+        psi = obug.MPS(om.neel_state(L))
+        v0 = om.mps_dense(psi.sites)
+        info = obug.cbe_bug_step(psi, om.xxz_mpo(L, delta=delta), -1j * dt,
+                                 obug.StepOptions(order=2, work_ceiling_rule="maxdim"))
+        got = om.mps_dense(psi.sites)
+        errs.append(np.linalg.norm(got - sla.expm(-1j * dt * H) @ v0))
+        assert abs(np.vdot(got, H @ got) - np.vdot(v0, H @ v0)) < 1e-10
+        assert info.ledger.status == "met"
+    assert errs[0] < 2e-4
+    assert 7.5 < errs[0] / errs[1] < 8.5 and 7.5 < errs[1] / errs[2] < 8.5
+
+
 def test_invalid_options_are_refused_before_touching_the_state():
     psi = obug.MPS(om.neel_state(6))
     before = [s.copy() for s in psi.sites]
