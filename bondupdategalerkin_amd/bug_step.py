@@ -95,12 +95,37 @@ class AdaptiveLedger:
     r_old: int = 0
     r_aug: int = 0
     r_final: int = 0
+    # order-p closure: time-weighted frontier leaks (index = degree; NaN = not measured), their
+    # defect sum_j leak_j / (j + 1), and the one-site applies the frontier sweeps spent
+    leak: List[float] = field(default_factory=list)
+    defect: float = 0.0
+    n_h1_closure: int = 0
+    n_closure_sweeps: int = 0
     unmet: List[str] = field(default_factory=list)
 
     def fail(self, what: str):
         self.tolerance_met = False
         if what not in self.unmet:
             self.unmet.append(what)
+
+    def merge(self, other: "AdaptiveLedger"):
+        """``merge_ledger!`` (basis_budget.jl:343-383) on the fields kept here."""
+        self.max_sample = max(self.max_sample, other.max_sample)
+        self.max_tail = max(self.max_tail, other.max_tail)
+        self.max_krylov = max(self.max_krylov, other.max_krylov)
+        self.max_krylov_c = max(self.max_krylov_c, other.max_krylov_c)
+        self.max_krylov_d = max(self.max_krylov_d, other.max_krylov_d)
+        self.n_probes += other.n_probes
+        self.n_selected += other.n_selected
+        self.n_krylov += other.n_krylov
+        self.n_h1 += other.n_h1
+        self.n_h1_closure += other.n_h1_closure
+        self.n_closure_sweeps += other.n_closure_sweeps
+        self.workcap_reached |= other.workcap_reached
+        self.tolerance_met &= other.tolerance_met
+        for u in other.unmet:
+            if u not in self.unmet:
+                self.unmet.append(u)
 
 
 def budget_status(ledger: AdaptiveLedger) -> str:
@@ -192,20 +217,13 @@ def _width(t: BlockTensor, side: str) -> int:
     return t.layout.col.total if side == "left" else t.layout.row.total
 
 
-def dd_probe(frame: BlockTensor, side: str, width: int, stream: ProbeStream) -> Optional[BlockTensor]:
-    """Sector-graded Gaussian probe in the discarded space of ``frame`` (_dd_probe, dd_cbe.jl:132-161).
-
-    Columns per fused charge by its discarded dimension (``_alloc_columns``, cbe.jl:188-195);
-    stream order per fused charge ascending, per slice ascending, one (link x dfull) block
-    column-major of which the first ``cols`` columns are kept."""
+def _graded_draw(frame: BlockTensor, side: str, dfull: Dict[int, int], cols: Dict[int, int],
+                 stream: ProbeStream) -> BlockTensor:
+    """Gaussian columns on the fused local space of ``frame``, drawn on the host from the step's
+    stream and uploaded (``_randomize_like`` + ``_trim_per_sector``, cbe.jl:156-178): per fused
+    charge ascending, per slice ascending, one (link x dfull) block column-major of which the
+    first ``cols`` columns are kept."""
     lay = frame.layout
-    dfull = _fused_dims(lay, side)
-    dT = _widths(lay, side)
-    dC = {q: max(d - dT.get(q, 0), 0) for q, d in dfull.items()}
-    tot = sum(dC.values())
-    if tot == 0:
-        return None
-    cols = {q: (min(d, math.ceil(d / tot * width)) if width > 0 else 0) for q, d in dC.items()}
     wleg = Leg.make((q, c) for q, c in cols.items() if c > 0)
     link = lay.row if side == "left" else lay.col
     play = Layout(link, wleg, lay.shifts) if side == "left" else Layout(wleg, link, lay.shifts)
@@ -226,9 +244,40 @@ def dd_probe(frame: BlockTensor, side: str, width: int, stream: ProbeStream) -> 
                 blocks[(s, lq)] = blk
             else:
                 blocks[(s, q)] = blk.T.copy()
-    Om = BlockTensor.from_blocks(play, blocks)
+    return BlockTensor.from_blocks(play, blocks)
+
+
+def _alloc_columns(dims: Dict[int, int], width: int) -> Dict[int, int]:
+    """``_alloc_columns`` (cbe.jl:188-195)."""
+    tot = sum(dims.values())
+    if tot == 0 or width <= 0:
+        return {q: 0 for q in dims}
+    return {q: min(d, math.ceil(d / tot * width)) for q, d in dims.items()}
+
+
+def dd_probe(frame: BlockTensor, side: str, width: int, stream: ProbeStream) -> Optional[BlockTensor]:
+    """Sector-graded Gaussian probe in the discarded space of ``frame`` (_dd_probe, dd_cbe.jl:132-161):
+    columns per fused charge by its discarded dimension, the frame projected out afterwards."""
+    lay = frame.layout
+    dfull = _fused_dims(lay, side)
+    dT = _widths(lay, side)
+    dC = {q: max(d - dT.get(q, 0), 0) for q, d in dfull.items()}
+    if sum(dC.values()) == 0:
+        return None
+    Om = _graded_draw(frame, side, dfull, _alloc_columns(dC, width), stream)
     proj = T.perp_left(frame, Om) if side == "left" else T.perp_right(frame, Om)
     return proj if T.norm(proj) > 0 else None
+
+
+def graded_probe(frame: BlockTensor, side: str, width: int, stream: ProbeStream) -> Optional[BlockTensor]:
+    """``sector_graded_sketch`` with comp_ratio = nothing (sketch.jl:45-54): columns shared out by
+    the FULL fused dimension of each sector, no projector in the probe."""
+    dfull = _fused_dims(frame.layout, side)
+    cols = _alloc_columns(dfull, width)
+    if not any(c > 0 for c in cols.values()):
+        return None
+    Om = _graded_draw(frame, side, dfull, cols, stream)
+    return Om if T.norm(Om) > 0 else None
 
 
 def _slots(entries):
@@ -529,9 +578,10 @@ class _KrylovStats:
 
 def krylov_enrich(seed, applyH, start_frame, side, tau, alloc: _Alloc, workcap: int,
                   recurrence: str = "lanczos", maxdepth: int = K.MAXDEPTH,
-                  hermitian_tol: float = 1e-9):
+                  hermitian_tol: float = 1e-9, amp: float = 1.0, state_seed: bool = True):
     """Tests C/D: grow the frame by the local Krylov directions the step needs
-    (``_krylov_enrich`` + ``_lanczos_chain``/``_arnoldi_chain``)."""
+    (``_krylov_enrich`` + ``_lanczos_chain``/``_arnoldi_chain``, krylov_enrichment.jl:213-292).
+    ``amp`` scales every weight (a frontier seed's norm relative to the state)."""
     perp = _perp(side)
     allow = alloc.krylov
     st = _KrylovStats()
@@ -591,7 +641,7 @@ def krylov_enrich(seed, applyH, start_frame, side, tau, alloc: _Alloc, workcap: 
         if recurrence == "lanczos":
             H[k - 1, k] = beta
         vs.append(T.scale(1.0 / beta, img))
-        w = K.direction_weight(H, k, teff)
+        w = amp * K.direction_weight(H, k, teff, state_seed)
         if not score(vs[-1], w, k == 1):
             capped = True
             break
@@ -628,7 +678,9 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
                  root: Optional[int] = None, reorth: bool = True, truncate: bool = True,
                  expansion: Optional[DDQRExpansion] = None, rng: Optional[int] = None,
                  probe_hook: Optional[Callable] = None, precision: str = "fp64") -> CBEBugSweepInfo:
-    """One step in place; ``root`` is the 0-based root bond (the reference's ``root`` minus 1),
+    """One order-p step in place (``order > 1`` closes the root frontiers j = 1..p-1 by one
+    enriching frontier sweep each, ``closure.py``); ``root`` is the 0-based root bond (the
+    reference's ``root`` minus 1),
     ``rng`` the probe seed (default 0x5EED, fresh per call as the reference's default RNG).
 
     ``precision = "mixed"`` runs the dense block contractions (environment pushes, one- and
@@ -654,8 +706,6 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         raise ValueError("tolerances must be positive (tol_trunc non-negative)")
     if order < 1:
         raise ValueError(f"order must be >= 1, got {order}")
-    if order > 1:
-        raise NotImplementedError("the order-p frontier closure (order > 1) is not implemented")
     if work_ceiling_rule not in ("bond", "maxdim"):
         raise ValueError(f"work_ceiling_rule must be 'bond' or 'maxdim', got {work_ceiling_rule!r}")
     factor = bond_ceiling_factor
@@ -764,6 +814,21 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         c = psi[i] if AL is None else T.core_times_site(AL, psi[i])
         AL = T.overlap_left(left_frames[i], c)
     root_start = T.core_times_core(AL, AR)
+
+    # ---- order p > 1: close the root frontiers j = 1..p-1 (bug_step_stages.jl:172-240) ----
+    if order > 1:
+        from .closure import KLFrames, _Ctx, single_pass_closure
+        ts = time.perf_counter()
+        _progress("frontier closure")
+        fr = KLFrames(left_frames, right_frames, rl, lroot, rroot)
+        cx = _Ctx(mpo, tau, alloc, wcap, ex, stream, recurrence, cp, htol)
+        root_start, _ = single_pass_closure(fr, psi, root_start, cx, nrm, ledger, order)
+        lroot, rroot = fr.lenv, fr.renv
+        for b in range(rl):
+            expanded[b] = left_frames[b].layout.col.total
+        for b in range(rl + 1, L - 1):
+            expanded[b] = right_frames[b].layout.row.total
+        secs["closure"] = time.perf_counter() - ts
     npsi = T.norm(psi[psi.center])
     ledger.r_aug = max(max(expanded + [0]), right_frames[rl].layout.row.total)
     ledger.proj_err = projection_miss(psi, left_frames, right_frames, rl) / npsi if npsi > 0 else 0.0

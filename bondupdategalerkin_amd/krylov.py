@@ -55,12 +55,13 @@ def residual_integral(defect: Callable[[np.ndarray], float], M: np.ndarray, tau:
     return math.exp(lw - sigma - math.log(max(np.linalg.norm(c), np.finfo(float).tiny)))
 
 
-def direction_weight(H: np.ndarray, k: int, tau: complex) -> float:
-    """Weight of the (k+1)-th Krylov vector in exp(tau H) v1 (``_kry_weight``)."""
+def direction_weight(H: np.ndarray, k: int, tau: complex, state_seed: bool = True) -> float:
+    """Weight of the (k+1)-th Krylov vector in exp(tau H) v1 (``_kry_weight``); a state seed is
+    weighed relative to its own growth on a non-unitary step, a frontier seed absolutely."""
     beta = abs(H[k, k - 1])
     if beta == 0:
         return 0.0
-    return beta * residual_integral(lambda c: abs(c[k - 1]), tau * H[:k, :k], tau, True)
+    return beta * residual_integral(lambda c: abs(c[k - 1]), tau * H[:k, :k], tau, state_seed)
 
 
 def tail_rank(kept, scale: float, allowance: float, kmax: int, base: float = 0.0):

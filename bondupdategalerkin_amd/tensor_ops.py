@@ -30,6 +30,10 @@ from .blocks import BlockTensor, Layout, Leg, bsm_product, device
 
 OP_N, OP_T, OP_C, OP_R = _lib.OP_N, _lib.OP_T, _lib.OP_C, _lib.OP_R
 CBE_EPS = 100 * np.finfo(float).eps
+# A sector block (or singular value) below this fraction of the tensor's norm (or of the largest
+# singular value) is roundoff: gauge moves drop such blocks and selections such values, so the
+# block structure of a result never depends on how the arithmetic rounded.
+ZERO_REL = 1e-14
 
 
 def _ctx():
@@ -324,6 +328,12 @@ def diag_scale(T: BlockTensor, S: Dict[int, np.ndarray], side: str) -> BlockTens
 
 # ---- QR -----------------------------------------------------------------------------------
 
+def _nonzero(sqnorms) -> List[int]:
+    """Indices of the sector blocks above the roundoff floor (squared norms in)."""
+    floor = ZERO_REL * ZERO_REL * float(sum(sqnorms))
+    return [i for i, v in enumerate(sqnorms) if v > floor]
+
+
 def _run_qr(mats: List[Tuple[int, int]], ws: torch.Tensor, offs: List[int]):
     """QR of the listed (m, n) matrices stored at offs in ws -> (Qbuf, Rbuf, qoffs, roffs, ks)."""
     qoffs, roffs, ks = [], [], []
@@ -368,7 +378,7 @@ def qr_left(A: BlockTensor) -> Tuple[BlockTensor, BlockTensor]:
     ws = empty_buffer(pos)
     gather.run(A.data, ws)
     nz = block_sqnorms([(offs[i], mats[i][0], mats[i][0], mats[i][1]) for i in range(len(mats))], ws)
-    keep = [i for i in range(len(mats)) if nz[i] > 0]
+    keep = _nonzero(nz)
     Qb, Rb, qo, ro, ks = _run_qr([mats[i] for i in keep], ws, [offs[i] for i in keep])
     bond = Leg.make((qlist[i], ks[j]) for j, i in enumerate(keep))
     Q = zeros(Layout(lay.row, bond, lay.shifts))
@@ -413,7 +423,7 @@ def qr_right(A: BlockTensor) -> Tuple[BlockTensor, BlockTensor]:
     ws = empty_buffer(pos)
     gather.run(A.data, ws)
     nz = block_sqnorms([(offs[i], mats[i][0], mats[i][0], mats[i][1]) for i in range(len(mats))], ws)
-    keep = [i for i in range(len(mats)) if nz[i] > 0]
+    keep = _nonzero(nz)
     Qb, Rb, qo, ro, ks = _run_qr([mats[i] for i in keep], ws, [offs[i] for i in keep])
     bond = Leg.make((qlist[i], ks[j]) for j, i in enumerate(keep))
     Lc = zeros(Layout(lay.row, bond, [0]))
@@ -442,7 +452,7 @@ def qr_core(C: BlockTensor) -> Tuple[BlockTensor, BlockTensor]:
     tab = _block_table(lay)
     keys = list(tab.keys())
     nz = block_sqnorms([(tab[k][0], tab[k][1], tab[k][1], tab[k][2]) for k in keys], C.data)
-    keep = [i for i in range(len(keys)) if nz[i] > 0]
+    keep = _nonzero(nz)
     mats = [(tab[keys[i]][1], tab[keys[i]][2]) for i in keep]
     ws = C.data.clone()
     Qb, Rb, qo, ro, ks = _run_qr(mats, ws, [tab[keys[i]][0] for i in keep])
@@ -482,7 +492,7 @@ def select(spectra: Dict[int, np.ndarray], cutoff: float, nkeep: Optional[int]):
             smax = max(smax, float(s))
             entries.append((float(s), q, j))
     full = sorted(entries, key=lambda e: -e[0])
-    thresh = cutoff * smax
+    thresh = max(cutoff, ZERO_REL) * smax
     sel = [e for e in full if e[0] > thresh]
     if nkeep is not None:
         sel = sel[:max(nkeep, 0)]

@@ -973,8 +973,9 @@ def projection_miss(psi: MPS, left_frames, right_frames, rl: int) -> float:
 def qr_core(C: Fam):
     """C = Q R per sector of a one-slice core (unique QR)."""
     qb, rb, bond = {}, {}, {}
+    floor = la.ZERO_REL * C.norm()
     for (k, q), M in C.blocks.items():
-        if not np.any(M):
+        if not np.linalg.norm(M) > floor:
             continue
         Q, R = la._unique_qr(M)
         bond[q] = Q.shape[1]

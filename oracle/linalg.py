@@ -34,6 +34,11 @@ from .blocks import Fam
 
 OP_N, OP_T, OP_C, OP_R = 0, 1, 2, 3
 CBE_EPS = 100 * np.finfo(float).eps          # _CBE_EPS, truncation.jl:173
+# A sector block (or singular value) below this fraction of the tensor's norm (or of the largest
+# singular value) is roundoff: gauge moves drop such blocks and selections such values, so that the
+# block structure of a result never depends on how the arithmetic rounded (a sector that cancels
+# to exactly zero in one evaluation order and to 1e-17 in another).
+ZERO_REL = 1e-14
 
 
 def _op(m, o):
@@ -221,9 +226,11 @@ def qr_left(A: Fam) -> Tuple[Fam, Fam]:
     """A = Q R over (l, s) | r with Q a left isometry (new bond leg per sector), R (bond, r)."""
     secs = left_sectors(A)
     qblocks, rblocks, bond = {}, {}, {}
+    mats = {qr: left_matrix(A, qr, parts) for qr, parts in secs.items()}
+    floor = ZERO_REL * math.sqrt(sum(np.vdot(M, M).real for M in mats.values()))
     for qr, parts in secs.items():
-        M = left_matrix(A, qr, parts)
-        if not np.any(M):
+        M = mats[qr]
+        if not np.linalg.norm(M) > floor:
             continue
         Q, R = _unique_qr(M)
         k = Q.shape[1]
@@ -242,9 +249,11 @@ def qr_right(A: Fam) -> Tuple[Fam, Fam]:
     """A = L Q over l | (s, r) with Q a right isometry: L (l, bond), Q (bond, s, r)."""
     secs = right_sectors(A)
     qblocks, lblocks, bond = {}, {}, {}
+    mats = {ql: right_matrix(A, ql, parts) for ql, parts in secs.items()}
+    floor = ZERO_REL * math.sqrt(sum(np.vdot(M, M).real for M in mats.values()))
     for ql, parts in secs.items():
-        M = right_matrix(A, ql, parts)
-        if not np.any(M):
+        M = mats[ql]
+        if not np.linalg.norm(M) > floor:
             continue
         Q, R = _unique_qr(M.conj().T)            # M^H = Q R  ->  M = R^H Q^H
         k = Q.shape[1]
@@ -283,7 +292,7 @@ def _select(spectra: Dict[int, np.ndarray], cutoff: float, nkeep: Optional[int])
             smax = max(smax, s)
             entries.append((float(s), q, j))
     full = sorted(entries, key=lambda e: -e[0])      # Python sort is stable
-    thresh = cutoff * smax
+    thresh = max(cutoff, ZERO_REL) * smax
     sel = [e for e in full if e[0] > thresh]
     if nkeep is not None:
         sel = sel[:max(nkeep, 0)]
