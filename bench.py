@@ -4,7 +4,7 @@ Prints ONE JSON line (rank 0).  Workload: the XXZ chain (delta = 0.5) at L = 128
 synthetic U(1) state with bond dimension up to D = 1024 (Gaussian charge-sector weights, sigma 4),
 canonicalised and normalised on the device.
 
-``--workload bug_step`` (default): one step = one ``cbe_bug_step`` (order 1, tau = -0.02 i,
+``--workload bug_step`` (default): one step = one ``cbe_bug_step`` (order 2, tau = -0.02 i,
 maxdim = D, work_ceiling_rule = maxdim, default tolerances) on the device-resident state.  The
 headline form is ``--precision mixed`` (FP32-MFMA block contractions, FP64 Krylov / QR / SVD);
 the FP64 drop-in is stepped from the same state beside it, timed on its own, and the largest
@@ -44,6 +44,7 @@ def parse():
     p.add_argument("--L", type=int, default=128)
     p.add_argument("--D", type=int, default=1024)
     p.add_argument("--delta", type=float, default=0.5)
+    p.add_argument("--order", type=int, default=2)
     p.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     p.add_argument("--no-fp64", action="store_true",
                    help="with --precision mixed: skip the FP64 comparison run")
@@ -126,7 +127,7 @@ def main():
         def make_step(state, form, record):
             def step():
                 info = B.cbe_bug_step(state, mpo, -0.02j, maxdim=D, work_ceiling_rule="maxdim",
-                                      precision=form)
+                                      precision=form, order=args.order)
                 if record is not None:
                     record.append(B.budget_status(info.ledger))
                 print(f"# {form} step: {info.seconds['step']:.2f}s {B.budget_status(info.ledger)} "
@@ -190,9 +191,8 @@ def main():
         "vs_baseline": None,
         "dtype": "c128" if args.precision == "fp64" else "c64 contractions / c128 Krylov, QR, SVD",
         "data": "synthetic",
-        "config": {"workload": ("cbe_bug_step order 1 (the order-2 closure is not built), "
-                                "tau=-0.02i, maxdim=D, work_ceiling_rule=maxdim, synthetic "
-                                "canonical U(1) state")
+        "config": {"workload": (f"cbe_bug_step order {args.order}, tau=-0.02i, maxdim=D, "
+                                "work_ceiling_rule=maxdim, synthetic canonical U(1) state")
                    if args.workload == "bug_step" else
                    "heff_sweep: one-site H_eff apply + left env push at every site",
                    "L": L, "D": D, "model": f"xxz delta={args.delta}", "symmetry": "U1",

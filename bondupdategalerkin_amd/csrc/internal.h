@@ -171,6 +171,10 @@ struct bug_ctx {
     int* svd_sweeps_dev = nullptr;
     // QR kernel choice (0 automatic, 1 one workgroup per matrix, 2 blocked panels where rows fit)
     int qr_variant = 0;
+    // block-Jacobi tuning: columns staged per workgroup (0 = as many as the LDS holds, at most 32)
+    // and one-sided sweeps over the staged columns per task
+    int bj_cols = 0;
+    int bj_inner = 1;
     // dynamic LDS bytes a workgroup may use on this device (queried on first use)
     int lds_limit = 0;
     // pinned host staging for small reductions

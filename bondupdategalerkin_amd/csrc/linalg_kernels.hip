@@ -22,16 +22,36 @@
 namespace bug {
 namespace {
 
+// Move a double across lanes by one DPP control (two 32-bit halves); every lane of the wave must
+// be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Sum over the 64 lanes, returned in every lane: within each 16-lane row by DPP (quad swaps, then
+// the half-row and row mirrors), across the four rows by lane reads.  Every lane must be active.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_mov<0xb1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_mov<0x4e>(v);    // quad_perm [2,3,0,1]
+    v += dpp_mov<0x141>(v);   // row_half_mirror
+    v += dpp_mov<0x140>(v);   // row_mirror
+    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
+    v = fmax(v, dpp_mov<0xb1>(v));
+    v = fmax(v, dpp_mov<0x4e>(v));
+    v = fmax(v, dpp_mov<0x141>(v));
+    v = fmax(v, dpp_mov<0x140>(v));
+    return fmax(fmax(read_lane(v, 0), read_lane(v, 16)), fmax(read_lane(v, 32), read_lane(v, 48)));
 }
 
 // ---- K10: axpy / scale / multi-dot --------------------------------------------------------
@@ -497,6 +517,95 @@ __global__ __launch_bounds__(64 * PW) void qr_panel_kernel(const QrPanelJob* __r
     }
 }
 
+// ---- fused block-reflector application: C <- C - P (R^H C) --------------------------------
+//
+// One workgroup per 16-column slab of C (mp rows).  W = R^H C_slab (jb x 16, jb <= 16) is
+// accumulated by v_mfma_f64_16x16x4f64 with the row range split over the four waves and the
+// partials summed through LDS; then C_slab -= P W, again by MFMA, 16 rows at a time.  The trailing
+// update of the blocked QR is (P, R) = (V, Y) on A; the backward formation of Q is (P, R) = (Y, V).
+// MFMA f64 16x16x4 operands: lane l of "a" holds X[l % 16][l / 16], lane l of "b" holds
+// Y[l / 16][l % 16]; register r of lane l of D = X Y holds D[l / 16 + 4 r][l % 16].
+
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+
+struct ReflTask {
+    int64_t c_off, p_off, r_off;   // C slab origin (ldc); P and R origins (leading dimension ldp)
+    int32_t mp, ncols, jb, ldc, ldp, pad;
+};
+
+constexpr int kReflThreads = 256;
+
+__global__ __launch_bounds__(kReflThreads) void apply_reflector_kernel(
+    const ReflTask* __restrict__ tasks, double2* __restrict__ C, const double2* __restrict__ P,
+    const double2* __restrict__ R) {
+    __shared__ double2 Wp[4][16][16];
+    const ReflTask tk = tasks[blockIdx.x];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int mp = tk.mp, nc = tk.ncols, jb = tk.jb;
+    double2* c = C + tk.c_off;
+    const double2* p = P + tk.p_off;
+    const double2* r = R + tk.r_off;
+    const int64_t ldc = tk.ldc, ldp = tk.ldp;
+    // W = R^H C: X = R^H (a: conj(R[k][lr])), Y = C (b: C[k][lr])
+    dvec4 wr = {0.0, 0.0, 0.0, 0.0}, wi = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 4 * wave; k0 < mp; k0 += 16) {
+        const int k = k0 + lk;
+        double2 x{0.0, 0.0}, y{0.0, 0.0};
+        if (k < mp) {
+            if (lr < jb) x = r[k + lr * ldp];
+            if (lr < nc) y = c[k + lr * ldc];
+        }
+        // X = conj(R)^T: Xr = x.x, Xi = -x.y
+        wr = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, wr, 0, 0, 0);
+        wr = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, y.y, wr, 0, 0, 0);   // -Xi Yi = +x.y y.y
+        wi = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.y, wi, 0, 0, 0);
+        wi = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.x, wi, 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Wp[wave][lk + 4 * q][lr] = double2{wr[q], wi[q]};
+    __syncthreads();
+    // every wave needs W as the b operand of P W: lane l, chunk kk: W[kk + lk][lr]
+    double2 wb[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        const int i = 4 * kk + lk;
+        double2 s{0.0, 0.0};
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            s.x += Wp[w][i][lr].x;
+            s.y += Wp[w][i][lr].y;
+        }
+        wb[kk] = s;
+    }
+    // C -= P W, 16 rows per tile: X = P (a: P[row0 + lr][kk + lk]), Y = W
+    for (int row0 = 16 * wave; row0 < mp; row0 += 64) {
+        dvec4 dr = {0.0, 0.0, 0.0, 0.0}, di = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int col = 4 * kk + lk, row = row0 + lr;
+            double2 x{0.0, 0.0};
+            if (row < mp && col < jb) x = p[row + col * ldp];
+            const double2 y = wb[kk];
+            dr = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, dr, 0, 0, 0);
+            dr = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.y, dr, 0, 0, 0);
+            di = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.y, di, 0, 0, 0);
+            di = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, y.x, di, 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = row0 + lk + 4 * q;
+            if (row < mp && lr < nc) {
+                double2* dst = c + row + lr * ldc;
+                double2 v = *dst;
+                v.x -= dr[q];
+                v.y -= di[q];
+                *dst = v;
+            }
+        }
+    }
+}
+
 // Q = [I_k; 0] (m x k, ldq) per matrix.
 __global__ void qr_qinit_kernel(const MatJob* __restrict__ jobs, double2* __restrict__ Q) {
     const MatJob job = jobs[blockIdx.x];
@@ -814,7 +923,7 @@ __global__ __launch_bounds__(kBjThreads) void bj_round_kernel(const BjMat* __res
                                                               double2* __restrict__ V, int round,
                                                               const int* __restrict__ flags_prev,
                                                               int* __restrict__ flags_cur,
-                                                              double eps) {
+                                                              double eps, int inner) {
     extern __shared__ double2 xs[];           // n * c staged columns, then c * c Z
     __shared__ double red[kBjThreads / 64][4];
     __shared__ int s_rot;
@@ -829,112 +938,147 @@ __global__ __launch_bounds__(kBjThreads) void bj_round_kernel(const BjMat* __res
     const int cq = bq < mt.nb ? min(w, n - bq * w) : 0;
     const int c = cp + cq;
     if (c < 2) return;
+    // round 0 of a sweep rotates every pair of the staged columns (so every pair of the matrix,
+    // intra-block ones included, is visited once per sweep); later rounds only the cross pairs
+    const bool cross_only = round > 0 && cp > 0 && cq > 0;
     double2* x = X + mt.x_off;
     double2* v = V + mt.v_off;
     double2* Z = xs + (int64_t)n * c;
     auto gcol = [&](int j) { return j < cp ? bp * w + j : bq * w + (j - cp); };
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr int nw = kBjThreads / 64;
-    for (int e = tid; e < n * c; e += kBjThreads) {
-        const int i = e % n, j = e / n;
-        xs[e] = x[i + (int64_t)gcol(j) * n];
+    for (int j = wave; j < c; j += nw) {
+        const double2* src = x + (int64_t)gcol(j) * n;
+        double2* dst = xs + (int64_t)j * n;
+        for (int i = lane; i < n; i += 64) dst[i] = src[i];
     }
     for (int e = tid; e < c * c; e += kBjThreads) Z[e] = double2{(e % c) == (e / c) ? 1.0 : 0.0, 0.0};
     if (tid == 0) s_rot = 0;
     __syncthreads();
-    const double tol = eps * sqrt((double)n);
-    const int C = c + (c & 1), np = C / 2;
-    const int wpp = nw / np;                  // waves per pair (np <= 16)
+    const double tol2 = eps * eps * (double)n;
+    // pairs per inner round: round-robin over all c columns, or the cyclic shifts of the smaller
+    // block against the larger
+    const int big = cp >= cq ? cp : cq, small = cp >= cq ? cq : cp;
+    const int big0 = cp >= cq ? 0 : cp, small0 = cp >= cq ? cp : 0;
+    const int C = c + (c & 1);
+    const int np = cross_only ? small : C / 2;
+    const int nrounds = cross_only ? big : C - 1;
+    const int wpp = max(1, nw / np);          // waves per pair
     const int grp = wave / wpp, sub = wave % wpp;
     const int gthreads = wpp * 64, gtid = sub * 64 + lane;
-    for (int ir = 0; ir < C - 1; ++ir) {
-        int a = 0, b = 0;
-        const bool active = grp < np;
-        if (active) round_robin(C, ir, grp, a, b);
-        const bool valid = active && b < c;
-        double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
-        if (valid) {
-            const double2* xa = xs + (int64_t)a * n;
-            const double2* xb = xs + (int64_t)b * n;
-            for (int i = gtid; i < n; i += gthreads) {
-                const double2 p = xa[i], q = xb[i];
-                al += p.x * p.x + p.y * p.y;
-                be += q.x * q.x + q.y * q.y;
-                gr += p.x * q.x + p.y * q.y;
-                gi += p.x * q.y - p.y * q.x;
-            }
-        }
-        al = wave_sum(al);
-        be = wave_sum(be);
-        gr = wave_sum(gr);
-        gi = wave_sum(gi);
-        if (lane == 0) {
-            red[wave][0] = al;
-            red[wave][1] = be;
-            red[wave][2] = gr;
-            red[wave][3] = gi;
-        }
-        __syncthreads();
-        if (valid) {
-            al = be = gr = gi = 0.0;
-            for (int k = 0; k < wpp; ++k) {
-                al += red[grp * wpp + k][0];
-                be += red[grp * wpp + k][1];
-                gr += red[grp * wpp + k][2];
-                gi += red[grp * wpp + k][3];
-            }
-            const double ag = sqrt(gr * gr + gi * gi);
-            if (ag > tol * sqrt(al * be) && ag != 0.0) {
-                const double zeta = (be - al) / (2.0 * ag);
-                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                const double cs = 1.0 / sqrt(1.0 + t * t);
-                const double sn = cs * t;
-                const double er = gr / ag, ei = gi / ag;
-                // x_a' = c x_a - s conj(e) x_b ; x_b' = s e x_a + c x_b
-                double2* xa = xs + (int64_t)a * n;
-                double2* xb = xs + (int64_t)b * n;
-                for (int i = gtid; i < n; i += gthreads) {
-                    const double2 p = xa[i], q = xb[i];
-                    xa[i] = double2{cs * p.x - sn * (er * q.x + ei * q.y),
-                                    cs * p.y - sn * (er * q.y - ei * q.x)};
-                    xb[i] = double2{sn * (er * p.x - ei * p.y) + cs * q.x,
-                                    sn * (er * p.y + ei * p.x) + cs * q.y};
+    for (int pass = 0; pass < inner; ++pass) {
+        for (int r0 = 0; r0 < nrounds; ++r0) {
+            // pair slots beyond 16 waves are taken in turns
+            for (int g0 = 0; g0 < np; g0 += nw / wpp) {
+                const int g = g0 + grp;
+                int a = 0, b = 0;
+                bool valid = grp < nw / wpp && g < np;
+                if (valid) {
+                    if (cross_only) {
+                        const int i1 = small0 + g, i2 = big0 + (g + r0) % big;
+                        a = min(i1, i2);
+                        b = max(i1, i2);
+                    } else {
+                        round_robin(C, r0, g, a, b);
+                        valid = b < c;
+                    }
                 }
-                double2* za = Z + (int64_t)a * c;
-                double2* zb = Z + (int64_t)b * c;
-                for (int i = gtid; i < c; i += gthreads) {
-                    const double2 p = za[i], q = zb[i];
-                    za[i] = double2{cs * p.x - sn * (er * q.x + ei * q.y),
-                                    cs * p.y - sn * (er * q.y - ei * q.x)};
-                    zb[i] = double2{sn * (er * p.x - ei * p.y) + cs * q.x,
-                                    sn * (er * p.y + ei * p.x) + cs * q.y};
+                double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+                if (valid) {
+                    const double2* xa = xs + (int64_t)a * n;
+                    const double2* xb = xs + (int64_t)b * n;
+                    for (int i = gtid; i < n; i += gthreads) {
+                        const double2 p = xa[i], q = xb[i];
+                        al += p.x * p.x + p.y * p.y;
+                        be += q.x * q.x + q.y * q.y;
+                        gr += p.x * q.x + p.y * q.y;
+                        gi += p.x * q.y - p.y * q.x;
+                    }
                 }
-                if (gtid == 0) s_rot = 1;
+                al = wave_sum(al);
+                be = wave_sum(be);
+                gr = wave_sum(gr);
+                gi = wave_sum(gi);
+                if (wpp > 1) {
+                    // several waves per pair: combine their partial sums through LDS
+                    if (lane == 0) {
+                        red[wave][0] = al;
+                        red[wave][1] = be;
+                        red[wave][2] = gr;
+                        red[wave][3] = gi;
+                    }
+                    __syncthreads();
+                    if (valid) {
+                        al = be = gr = gi = 0.0;
+                        for (int k = 0; k < wpp; ++k) {
+                            al += red[grp * wpp + k][0];
+                            be += red[grp * wpp + k][1];
+                            gr += red[grp * wpp + k][2];
+                            gi += red[grp * wpp + k][3];
+                        }
+                    }
+                }
+                const double ag2 = gr * gr + gi * gi;
+                // rotate when |gamma| > tol sqrt(al be), tol = eps sqrt(n) (squared: no roots)
+                if (valid && ag2 > tol2 * (al * be) && ag2 != 0.0) {
+                    const double ag = sqrt(ag2);
+                    const double inv = 1.0 / ag;
+                    const double zeta = 0.5 * (be - al) * inv;
+                    const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    const double cs = 1.0 / sqrt(1.0 + t * t);
+                    const double sn = cs * t;
+                    const double er = gr * inv, ei = gi * inv;
+                    // x_a' = c x_a - s conj(e) x_b ; x_b' = s e x_a + c x_b
+                    double2* xa = xs + (int64_t)a * n;
+                    double2* xb = xs + (int64_t)b * n;
+                    for (int i = gtid; i < n; i += gthreads) {
+                        const double2 p = xa[i], q = xb[i];
+                        xa[i] = double2{cs * p.x - sn * (er * q.x + ei * q.y),
+                                        cs * p.y - sn * (er * q.y - ei * q.x)};
+                        xb[i] = double2{sn * (er * p.x - ei * p.y) + cs * q.x,
+                                        sn * (er * p.y + ei * p.x) + cs * q.y};
+                    }
+                    double2* za = Z + (int64_t)a * c;
+                    double2* zb = Z + (int64_t)b * c;
+                    for (int i = gtid; i < c; i += gthreads) {
+                        const double2 p = za[i], q = zb[i];
+                        za[i] = double2{cs * p.x - sn * (er * q.x + ei * q.y),
+                                        cs * p.y - sn * (er * q.y - ei * q.x)};
+                        zb[i] = double2{sn * (er * p.x - ei * p.y) + cs * q.x,
+                                        sn * (er * p.y + ei * p.x) + cs * q.y};
+                    }
+                    if (gtid == 0) s_rot = 1;
+                }
+                __syncthreads();
             }
         }
-        __syncthreads();
     }
     if (!s_rot) return;                       // uniform: read after the last barrier
-    for (int e = tid; e < n * c; e += kBjThreads) {
-        const int i = e % n, j = e / n;
-        x[i + (int64_t)gcol(j) * n] = xs[e];
+    for (int j = wave; j < c; j += nw) {
+        double2* dst = x + (int64_t)gcol(j) * n;
+        const double2* src = xs + (int64_t)j * n;
+        for (int i = lane; i < n; i += 64) dst[i] = src[i];
     }
     __syncthreads();
-    for (int e = tid; e < n * c; e += kBjThreads) {
-        const int i = e % n, j = e / n;
-        xs[e] = v[i + (int64_t)gcol(j) * n];
+    for (int j = wave; j < c; j += nw) {
+        const double2* src = v + (int64_t)gcol(j) * n;
+        double2* dst = xs + (int64_t)j * n;
+        for (int i = lane; i < n; i += 64) dst[i] = src[i];
     }
     __syncthreads();
     // V(:, cols) <- V(:, cols) Z
-    for (int e = tid; e < n * c; e += kBjThreads) {
-        const int i = e % n, j = e / n;
-        double sr = 0.0, si = 0.0;
-        for (int k = 0; k < c; ++k) {
-            const double2 p = xs[i + (int64_t)k * n], z = Z[k + (int64_t)j * c];
-            sr += p.x * z.x - p.y * z.y;
-            si += p.x * z.y + p.y * z.x;
+    for (int j = wave; j < c; j += nw) {
+        double2* dst = v + (int64_t)gcol(j) * n;
+        const double2* zj = Z + (int64_t)j * c;
+        for (int i = lane; i < n; i += 64) {
+            double sr = 0.0, si = 0.0;
+            for (int k = 0; k < c; ++k) {
+                const double2 p = xs[i + (int64_t)k * n], z = zj[k];
+                sr += p.x * z.x - p.y * z.y;
+                si += p.x * z.y + p.y * z.x;
+            }
+            dst[i] = double2{sr, si};
         }
-        v[i + (int64_t)gcol(j) * n] = double2{sr, si};
     }
     if (tid == 0) flags_cur[tk.x] = 1;
 }
@@ -1019,13 +1163,13 @@ bool use_blocked_qr(bug_ctx* ctx, const std::vector<MatJob>& jobs) {
     return blocked_qr_fits(jobs);
 }
 
-// Thin QR of every job (A overwritten, Q m x k, R k x n) by panels and grouped-GEMM updates.
+// Thin QR of every job (A overwritten, Q m x k, R k x n) by panels and fused block-reflector
+// updates.
 void blocked_qr(bug_ctx* ctx, hipStream_t st, const std::vector<MatJob>& jobs, double2* A,
                 double2* Q, double2* R) {
     const int nmat = (int)jobs.size();
     std::vector<QrPanelJob> pj(nmat);
-    std::vector<int64_t> woff(nmat);
-    int64_t vt = 0, wt = 0;
+    int64_t vt = 0;
     int kmax = 0, mmax = 0;
     for (const MatJob& j : jobs) mmax = std::max(mmax, j.m);
     const int pw = mmax <= 512 ? 16 : 8;
@@ -1033,64 +1177,53 @@ void blocked_qr(bug_ctx* ctx, hipStream_t st, const std::vector<MatJob>& jobs, d
         const MatJob& j = jobs[i];
         const int k = std::min(j.m, j.n);
         pj[i] = QrPanelJob{j.a_off, vt, vt, j.m, j.n, j.lda, 0};
-        woff[i] = wt;
         vt += (int64_t)j.m * k;
-        wt += (int64_t)pw * std::max(j.n, k);
         kmax = std::max(kmax, k);
     }
-    double2* ws = static_cast<double2*>(
-        scratch(ctx, st, sizeof(double2) * std::max<int64_t>(2 * vt + wt, 1), 5));
+    double2* ws = static_cast<double2*>(scratch(ctx, st, sizeof(double2) * std::max<int64_t>(2 * vt, 1), 5));
     double2* Vb = ws;
     double2* Yb = ws + vt;
-    double2* Wb = ws + 2 * vt;
     const int npan = (kmax + pw - 1) / pw;
-    // plans: [2p] trailing W = Y^H A_t, [2p+1] A_t -= V W, [2 npan + 2p] W = V^H Q_s,
-    // [2 npan + 2p + 1] Q_s -= Y W
-    std::vector<Plan> plans(4 * (size_t)npan);
-    for (int p = 0; p < npan; ++p) {
-        const int p0 = p * pw;
-        Plan& t1 = plans[2 * p];
-        Plan& t2 = plans[2 * p + 1];
-        Plan& q1 = plans[2 * npan + 2 * p];
-        Plan& q2 = plans[2 * npan + 2 * p + 1];
-        for (int i = 0; i < nmat; ++i) {
-            const MatJob& j = jobs[i];
-            const int k = std::min(j.m, j.n);
-            if (k <= p0) continue;
-            const int jb = std::min(pw, k - p0);
-            const int mp = j.m - p0;
-            const int64_t vo = pj[i].v_off + p0 + (int64_t)p0 * j.m;
-            const int nt = j.n - p0 - jb;
-            if (nt > 0) {
-                const int64_t at = j.a_off + p0 + (int64_t)(p0 + jb) * j.lda;
-                t1.tasks.push_back(GemmTask{woff[i], pw, jb, nt, (int32_t)t1.terms.size(),
-                                            (int32_t)t1.terms.size() + 1, 0});
-                t1.terms.push_back(GemmTerm{vo, at, j.m, j.lda, mp, BUG_OP_C | (BUG_OP_N << 2), 1.0, 0.0});
-                t1.flops += 8.0 * jb * nt * (double)mp;
-                t2.tasks.push_back(GemmTask{at, j.lda, mp, nt, (int32_t)t2.terms.size(),
-                                            (int32_t)t2.terms.size() + 1, 1});
-                t2.terms.push_back(GemmTerm{vo, woff[i], j.m, pw, jb, BUG_OP_N | (BUG_OP_N << 2), -1.0, 0.0});
-                t2.flops += 8.0 * mp * nt * (double)jb;
+    // slab tasks: trailing updates of panel p at [tstart[p], tstart[p+1]), Q formation of panel p
+    // at [qstart[p], qstart[p+1])
+    std::vector<ReflTask> tasks;
+    std::vector<size_t> tstart(npan + 1), qstart(npan + 1);
+    double flops = 0.0;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int p = 0; p < npan; ++p) {
+            (pass == 0 ? tstart : qstart)[p] = tasks.size();
+            const int p0 = p * pw;
+            for (int i = 0; i < nmat; ++i) {
+                const MatJob& j = jobs[i];
+                const int k = std::min(j.m, j.n);
+                if (k <= p0) continue;
+                const int jb = std::min(pw, k - p0);
+                const int mp = j.m - p0;
+                const int64_t vo = pj[i].v_off + p0 + (int64_t)p0 * j.m;
+                const int c0 = pass == 0 ? p0 + jb : p0;
+                const int nc = pass == 0 ? j.n - p0 - jb : k - p0;
+                const int64_t base = pass == 0 ? j.a_off + p0 : j.q_off + p0;
+                const int64_t ld = pass == 0 ? j.lda : j.ldq;
+                for (int s0 = 0; s0 < nc; s0 += 16)
+                    tasks.push_back(ReflTask{base + (int64_t)(c0 + s0) * ld, vo, vo, mp,
+                                             std::min(16, nc - s0), jb, (int32_t)ld, j.m, 0});
+                if (nc > 0) flops += 16.0 * mp * nc * (double)jb;
             }
-            const int nc = k - p0;
-            const int64_t qs = j.q_off + p0 + (int64_t)p0 * j.ldq;
-            q1.tasks.push_back(GemmTask{woff[i], pw, jb, nc, (int32_t)q1.terms.size(),
-                                        (int32_t)q1.terms.size() + 1, 0});
-            q1.terms.push_back(GemmTerm{vo, qs, j.m, j.ldq, mp, BUG_OP_C | (BUG_OP_N << 2), 1.0, 0.0});
-            q1.flops += 8.0 * jb * nc * (double)mp;
-            q2.tasks.push_back(GemmTask{qs, j.ldq, mp, nc, (int32_t)q2.terms.size(),
-                                        (int32_t)q2.terms.size() + 1, 1});
-            q2.terms.push_back(GemmTerm{vo, woff[i], j.m, pw, jb, BUG_OP_N | (BUG_OP_N << 2), -1.0, 0.0});
-            q2.flops += 8.0 * mp * nc * (double)jb;
         }
+        (pass == 0 ? tstart : qstart)[npan] = tasks.size();
     }
-    stage_plans(ctx, st, plans, 6);
-    std::vector<unsigned char> blob(sizeof(QrPanelJob) * nmat + sizeof(MatJob) * nmat);
+    ctx->last_flops += flops;
+    std::vector<unsigned char> blob(sizeof(QrPanelJob) * nmat + sizeof(MatJob) * nmat +
+                                    sizeof(ReflTask) * tasks.size());
     std::memcpy(blob.data(), pj.data(), sizeof(QrPanelJob) * nmat);
     std::memcpy(blob.data() + sizeof(QrPanelJob) * nmat, jobs.data(), sizeof(MatJob) * nmat);
+    if (!tasks.empty())
+        std::memcpy(blob.data() + (sizeof(QrPanelJob) + sizeof(MatJob)) * nmat, tasks.data(),
+                    sizeof(ReflTask) * tasks.size());
     unsigned char* dblob = upload_list(ctx, st, blob.data(), blob.size(), 7);
     const QrPanelJob* dp = reinterpret_cast<const QrPanelJob*>(dblob);
     const MatJob* dj = reinterpret_cast<const MatJob*>(dblob + sizeof(QrPanelJob) * nmat);
+    const ReflTask* dt = reinterpret_cast<const ReflTask*>(dblob + (sizeof(QrPanelJob) + sizeof(MatJob)) * nmat);
     hipLaunchKernelGGL(qr_qinit_kernel, dim3(nmat, 8), dim3(256), 0, st, dj, Q);
     for (int p = 0; p < npan; ++p) {
         const int p0 = p * pw;
@@ -1104,12 +1237,16 @@ void blocked_qr(bug_ctx* ctx, hipStream_t st, const std::vector<MatJob>& jobs, d
         else
             hipLaunchKernelGGL((qr_panel_kernel<8, 16>), dim3(nmat), dim3(1024), 0, st, dp, p0, A, Vb, Yb);
         HIP_CHECK(hipGetLastError());
-        launch_staged_plan(ctx, plans[2 * p], BUG_PREC_FP64, Yb, A, Wb, st);
-        launch_staged_plan(ctx, plans[2 * p + 1], BUG_PREC_FP64, Vb, Wb, A, st);
+        const size_t nt = tstart[p + 1] - tstart[p];
+        if (nt > 0)
+            hipLaunchKernelGGL(apply_reflector_kernel, dim3((unsigned)nt), dim3(kReflThreads), 0, st,
+                               dt + tstart[p], A, Vb, Yb);
     }
     for (int p = npan - 1; p >= 0; --p) {
-        launch_staged_plan(ctx, plans[2 * npan + 2 * p], BUG_PREC_FP64, Vb, Q, Wb, st);
-        launch_staged_plan(ctx, plans[2 * npan + 2 * p + 1], BUG_PREC_FP64, Yb, Wb, Q, st);
+        const size_t nq = qstart[p + 1] - qstart[p];
+        if (nq > 0)
+            hipLaunchKernelGGL(apply_reflector_kernel, dim3((unsigned)nq), dim3(kReflThreads), 0, st,
+                               dt + qstart[p], Q, Yb, Vb);
     }
     hipLaunchKernelGGL(qr_finish_kernel, dim3(nmat), dim3(1024), 0, st, dj, A, Q, R);
     HIP_CHECK(hipGetLastError());
@@ -1149,7 +1286,8 @@ void block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list,
     size_t lds = 0;
     for (int i = 0; i < nmat; ++i) {
         const int n = list[i].n;
-        const int c = bj_cols(stage, n);
+        int c = bj_cols(stage, n);
+        if (ctx->bj_cols >= 2) c = std::min(c, ctx->bj_cols & ~1);
         if (c < 2) throw Error(BUG_EINVAL, "block Jacobi: matrix too wide for LDS");
         const int w = c / 2;
         const int nb = (n + w - 1) / w;
@@ -1183,7 +1321,8 @@ void block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list,
             for (int r = 0; r < rounds; ++r)
                 hipLaunchKernelGGL(bj_round_kernel, dim3((unsigned)tasks.size()), dim3(kBjThreads), lds,
                                    st, dm, dt, X, V, r, flags + (size_t)s * nmat,
-                                   flags + (size_t)(s + 1) * nmat, 2.220446049250313e-16);
+                                   flags + (size_t)(s + 1) * nmat, 2.220446049250313e-16,
+                                   std::max(1, ctx->bj_inner));
             HIP_CHECK(hipGetLastError());
             ++used;
             HIP_CHECK(hipMemcpyAsync(hp, flags + (size_t)(s + 1) * nmat, sizeof(int) * nmat,
@@ -1362,6 +1501,15 @@ int bug_block_qr(bug_ctx* ctx, int32_t nmat, const bug_qr_job* jobs, void* a, in
                            static_cast<double2*>(a), static_cast<double2*>(q),
                            static_cast<double2*>(r), tau);
         HIP_CHECK(hipGetLastError());
+    });
+}
+
+int bug_set_jacobi_params(bug_ctx* ctx, int32_t cols, int32_t inner_sweeps) {
+    return guarded(ctx, [&] {
+        if (cols < 0 || cols > kBjMaxCols || inner_sweeps < 1 || inner_sweeps > 8)
+            throw Error(BUG_EINVAL, "jacobi params: cols in 0..32, inner sweeps in 1..8");
+        ctx->bj_cols = cols;
+        ctx->bj_inner = inner_sweeps;
     });
 }
 

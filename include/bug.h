@@ -228,6 +228,11 @@ int bug_set_svd_variant(bug_ctx* ctx, int32_t variant, int32_t record_sweeps);
  * 2 blocked Householder: panels factored in registers, compact-WY updates on the grouped GEMM,
  * for blocks of at most 1024 rows). */
 int bug_set_qr_variant(bug_ctx* ctx, int32_t variant);
+
+/* Tuning of the multi-workgroup block Jacobi: columns staged per workgroup (0 = as many as the LDS
+ * holds, at most 32; an explicit value only lowers it) and one-sided sweeps over the staged
+ * columns per task (1..8). */
+int bug_set_jacobi_params(bug_ctx* ctx, int32_t cols, int32_t inner_sweeps);
 int bug_svd_sweeps(bug_ctx* ctx, int32_t n, int32_t* out);
 
 #ifdef __cplusplus
