@@ -253,89 +253,59 @@ int choose_config(const Plan& plan) {
     return 2;
 }
 
-void upload_plan(Plan& plan, int config) {
-    if (plan.config == config && plan.dev) return;
-    const TileConfig& tc = kTileConfigs[config];
-    std::vector<GemmTile> tiles;
-    for (size_t t = 0; t < plan.tasks.size(); ++t) {
-        const GemmTask& task = plan.tasks[t];
-        const int mt = (task.m + tc.tm() - 1) / tc.tm();
-        const int nt = (task.n + tc.tn() - 1) / tc.tn();
-        for (int a = 0; a < mt; ++a)
-            for (int b = 0; b < nt; ++b) tiles.push_back(GemmTile{(int32_t)t, a, b, 0});
-    }
-    const size_t bt = tiles.size() * sizeof(GemmTile);
-    const size_t bk = plan.tasks.size() * sizeof(GemmTask);
-    const size_t bm = plan.terms.size() * sizeof(GemmTerm);
+// Make the device image of `plan` for tile configuration `config` resident in the plan arena:
+// build it on the host once per configuration, copy it in stream order (no host wait).
+void upload_plan(bug_ctx* ctx, Plan& plan, int config, hipStream_t stream) {
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
-    const size_t total = align(bt) + align(bk) + align(bm) + 256;
-    if (plan.dev && plan.dev_bytes < total) {
-        HIP_CHECK(hipFree(plan.dev));
-        plan.dev = nullptr;
-    }
-    if (!plan.dev) {
-        HIP_CHECK(hipMalloc(&plan.dev, total));
-        plan.dev_bytes = total;
-    }
-    std::vector<char> host(total, 0);
-    std::memcpy(host.data(), tiles.data(), bt);
-    std::memcpy(host.data() + align(bt), plan.tasks.data(), bk);
-    std::memcpy(host.data() + align(bt) + align(bk), plan.terms.data(), bm);
-    HIP_CHECK(hipMemcpy(plan.dev, host.data(), total, hipMemcpyHostToDevice));
-    char* base = static_cast<char*>(plan.dev);
-    plan.d_tiles = reinterpret_cast<const GemmTile*>(base);
-    plan.d_tasks = reinterpret_cast<const GemmTask*>(base + align(bt));
-    plan.d_terms = reinterpret_cast<const GemmTerm*>(base + align(bt) + align(bk));
-    plan.ntiles = (int32_t)tiles.size();
-    plan.config = config;
-}
-
-void stage_plans(bug_ctx* ctx, hipStream_t stream, std::vector<Plan>& plans, int slot) {
-    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
-    std::vector<std::vector<GemmTile>> tiles(plans.size());
-    size_t total = 0;
-    for (size_t p = 0; p < plans.size(); ++p) {
-        Plan& plan = plans[p];
-        plan.config = ctx->forced_config >= 0 ? ctx->forced_config : choose_config(plan);
-        const TileConfig& tc = kTileConfigs[plan.config];
+    if (plan.config != config || plan.image.empty()) {
+        const TileConfig& tc = kTileConfigs[config];
+        std::vector<GemmTile> tiles;
         for (size_t t = 0; t < plan.tasks.size(); ++t) {
             const GemmTask& task = plan.tasks[t];
             const int mt = (task.m + tc.tm() - 1) / tc.tm();
             const int nt = (task.n + tc.tn() - 1) / tc.tn();
             for (int a = 0; a < mt; ++a)
-                for (int b = 0; b < nt; ++b) tiles[p].push_back(GemmTile{(int32_t)t, a, b, 0});
+                for (int b = 0; b < nt; ++b) tiles.push_back(GemmTile{(int32_t)t, a, b, 0});
         }
-        plan.ntiles = (int32_t)tiles[p].size();
-        total += align(tiles[p].size() * sizeof(GemmTile)) + align(plan.tasks.size() * sizeof(GemmTask)) +
-                 align(plan.terms.size() * sizeof(GemmTerm));
-    }
-    if (total == 0) return;
-    std::vector<char> host(total, 0);
-    char* dev = static_cast<char*>(scratch(ctx, stream, total, slot));
-    size_t off = 0;
-    for (size_t p = 0; p < plans.size(); ++p) {
-        Plan& plan = plans[p];
-        const size_t bt = tiles[p].size() * sizeof(GemmTile);
+        const size_t bt = tiles.size() * sizeof(GemmTile);
         const size_t bk = plan.tasks.size() * sizeof(GemmTask);
         const size_t bm = plan.terms.size() * sizeof(GemmTerm);
-        if (bt) std::memcpy(host.data() + off, tiles[p].data(), bt);
-        plan.d_tiles = reinterpret_cast<const GemmTile*>(dev + off);
-        off += align(bt);
-        if (bk) std::memcpy(host.data() + off, plan.tasks.data(), bk);
-        plan.d_tasks = reinterpret_cast<const GemmTask*>(dev + off);
-        off += align(bk);
-        if (bm) std::memcpy(host.data() + off, plan.terms.data(), bm);
-        plan.d_terms = reinterpret_cast<const GemmTerm*>(dev + off);
-        off += align(bm);
+        plan.image.assign(align(bt) + align(bk) + align(bm) + 256, 0);
+        if (bt) std::memcpy(plan.image.data(), tiles.data(), bt);
+        if (bk) std::memcpy(plan.image.data() + align(bt), plan.tasks.data(), bk);
+        if (bm) std::memcpy(plan.image.data() + align(bt) + align(bk), plan.terms.data(), bm);
+        plan.ntiles = (int32_t)tiles.size();
+        plan.config = config;
+        plan.gen = 0;
+        plan.off_tasks = align(bt);
+        plan.off_terms = align(bt) + align(bk);
     }
-    HIP_CHECK(hipMemcpyAsync(dev, host.data(), total, hipMemcpyHostToDevice, stream));
-}
-
-void launch_staged_plan(bug_ctx* ctx, const Plan& plan, int precision, const void* A, const void* B,
-                        void* C, hipStream_t stream) {
-    if (plan.ntiles == 0) return;
-    ctx->last_flops += plan.flops;
-    HIP_CHECK(launch_grouped_gemm(plan, plan.config, precision, A, B, C, stream));
+    if (plan.gen == ctx->plan_arena_gen) return;
+    const size_t total = plan.image.size();
+    if (ctx->plan_arena_used + total > ctx->plan_arena_bytes) {
+        HIP_CHECK(hipDeviceSynchronize());
+        if (total > ctx->plan_arena_bytes) {
+            if (ctx->plan_arena) HIP_CHECK(hipFree(ctx->plan_arena));
+            ctx->plan_arena = nullptr;
+            const size_t want = std::max<size_t>(std::max<size_t>(2 * ctx->plan_arena_bytes, 4 * total),
+                                                 (size_t)64 << 20);
+            if (hipMalloc(&ctx->plan_arena, want) != hipSuccess) {
+                (void)hipGetLastError();
+                ctx->plan_arena_bytes = 0;
+                throw Error(BUG_ENOMEM, "plan arena allocation failed");
+            }
+            ctx->plan_arena_bytes = want;
+        }
+        ctx->plan_arena_used = 0;
+        ++ctx->plan_arena_gen;
+    }
+    char* dev = static_cast<char*>(ctx->plan_arena) + ctx->plan_arena_used;
+    ctx->plan_arena_used += align(total);
+    HIP_CHECK(hipMemcpyAsync(dev, plan.image.data(), total, hipMemcpyHostToDevice, stream));
+    plan.d_tiles = reinterpret_cast<const GemmTile*>(dev);
+    plan.d_tasks = reinterpret_cast<const GemmTask*>(dev + plan.off_tasks);
+    plan.d_terms = reinterpret_cast<const GemmTerm*>(dev + plan.off_terms);
+    plan.gen = ctx->plan_arena_gen;
 }
 
 // Look up (or build) the plan of one product stage and launch it.
@@ -380,12 +350,11 @@ void launch_cached_plan(bug_ctx* ctx, uint64_t key, const std::function<void(Pla
             Plan* old = ctx->plan_lru.back().second;
             ctx->plan_index.erase(ctx->plan_lru.back().first);
             ctx->plan_lru.pop_back();
-            if (old->dev) (void)hipFree(old->dev);
             delete old;
         }
     }
     const int config = ctx->forced_config >= 0 ? ctx->forced_config : choose_config(*plan);
-    upload_plan(*plan, config);
+    upload_plan(ctx, *plan, config, stream);
     ctx->last_flops += plan->flops;
     HIP_CHECK(launch_grouped_gemm(*plan, config, precision, A, B, C, stream));
 }
@@ -479,10 +448,8 @@ int bug_ctx_destroy(bug_ctx* ctx) {
     if (!ctx) return BUG_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
-    for (auto& e : ctx->plan_lru) {
-        if (e.second->dev) (void)hipFree(e.second->dev);
-        delete e.second;
-    }
+    for (auto& e : ctx->plan_lru) delete e.second;
+    if (ctx->plan_arena) (void)hipFree(ctx->plan_arena);
     for (auto& slot : ctx->scratch)
         for (auto& s : slot)
             if (s.second.ptr) (void)hipFree(s.second.ptr);
@@ -509,7 +476,7 @@ int bug_ctx_stats(const bug_ctx* ctx, int64_t* out4) {
     int64_t bytes = 0;
     for (auto& slot : ctx->scratch)
         for (auto& s : slot) bytes += (int64_t)s.second.bytes;
-    for (auto& e : ctx->plan_lru) bytes += (int64_t)e.second->dev_bytes;
+    bytes += (int64_t)ctx->plan_arena_bytes;
     out4[0] = bytes;
     out4[1] = (int64_t)ctx->plan_lru.size();
     out4[2] = ctx->plan_hits;

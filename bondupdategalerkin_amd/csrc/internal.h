@@ -91,11 +91,13 @@ struct Plan {
     std::vector<GemmTask> tasks;
     std::vector<GemmTerm> terms;
     double flops = 0.0;          // algorithmic 8 m n k summed over terms
-    // device image: [tiles | tasks | terms], per tile configuration
+    // device image: [tiles | tasks | terms] for tile configuration `config`, held in the
+    // context's plan arena while `gen` equals the arena generation
     int config = -1;
     int32_t ntiles = 0;
-    void* dev = nullptr;
-    size_t dev_bytes = 0;
+    uint64_t gen = 0;
+    size_t off_tasks = 0, off_terms = 0;
+    std::vector<char> image;
     const GemmTile* d_tiles = nullptr;
     const GemmTask* d_tasks = nullptr;
     const GemmTerm* d_terms = nullptr;
@@ -133,13 +135,6 @@ struct Hasher {
 hipError_t launch_grouped_gemm(const Plan& plan, int config, int precision, const void* A,
                                const void* B, void* C, hipStream_t stream);
 
-// Build the tiles of several transient plans (each under its chosen configuration), copy them all to
-// one device blob in scratch `slot` with one asynchronous copy, and point the plans into it.
-void stage_plans(bug_ctx* ctx, hipStream_t stream, std::vector<Plan>& plans, int slot);
-// Launch a plan prepared by stage_plans (no-op for a plan without tiles).
-void launch_staged_plan(bug_ctx* ctx, const Plan& plan, int precision, const void* A, const void* B,
-                        void* C, hipStream_t stream);
-
 struct Workspace {
     void* ptr = nullptr;
     size_t bytes = 0;
@@ -160,6 +155,11 @@ struct bug_ctx {
     std::list<std::pair<uint64_t, bug::Plan*>> plan_lru;
     size_t plan_capacity = 512;
     int64_t plan_hits = 0, plan_misses = 0;
+    // device arena of plan images: bump-allocated, copied in stream order without blocking the
+    // host; when full the device is synchronized, the arena reset and its generation advanced
+    void* plan_arena = nullptr;
+    size_t plan_arena_bytes = 0, plan_arena_used = 0;
+    uint64_t plan_arena_gen = 1;
     double last_flops = 0.0;     // algorithmic flops of the last entry-point call
     // per-stream scratch for intermediates of the two-stage contractions and the solvers,
     // keyed by (stream, slot)

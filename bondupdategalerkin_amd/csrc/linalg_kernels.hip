@@ -549,18 +549,28 @@ __global__ __launch_bounds__(kReflThreads) void apply_reflector_kernel(
     const int64_t ldc = tk.ldc, ldp = tk.ldp;
     // W = R^H C: X = R^H (a: conj(R[k][lr])), Y = C (b: C[k][lr])
     dvec4 wr = {0.0, 0.0, 0.0, 0.0}, wi = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 4 * wave; k0 < mp; k0 += 16) {
-        const int k = k0 + lk;
-        double2 x{0.0, 0.0}, y{0.0, 0.0};
-        if (k < mp) {
-            if (lr < jb) x = r[k + lr * ldp];
-            if (lr < nc) y = c[k + lr * ldc];
+    // four k-steps per iteration, loads issued before the products so their latencies overlap
+    for (int k0 = 4 * wave; k0 < mp; k0 += 64) {
+        double2 xv[4], yv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + 16 * u + lk;
+            xv[u] = double2{0.0, 0.0};
+            yv[u] = double2{0.0, 0.0};
+            if (k < mp) {
+                if (lr < jb) xv[u] = r[k + lr * ldp];
+                if (lr < nc) yv[u] = c[k + lr * ldc];
+            }
         }
-        // X = conj(R)^T: Xr = x.x, Xi = -x.y
-        wr = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, wr, 0, 0, 0);
-        wr = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, y.y, wr, 0, 0, 0);   // -Xi Yi = +x.y y.y
-        wi = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.y, wi, 0, 0, 0);
-        wi = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.x, wi, 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double2 x = xv[u], y = yv[u];
+            // X = conj(R)^T: Xr = x.x, Xi = -x.y
+            wr = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, wr, 0, 0, 0);
+            wr = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, y.y, wr, 0, 0, 0);   // -Xi Yi = +x.y y.y
+            wi = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.y, wi, 0, 0, 0);
+            wi = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.x, wi, 0, 0, 0);
+        }
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) Wp[wave][lk + 4 * q][lr] = double2{wr[q], wi[q]};
@@ -581,11 +591,20 @@ __global__ __launch_bounds__(kReflThreads) void apply_reflector_kernel(
     // C -= P W, 16 rows per tile: X = P (a: P[row0 + lr][kk + lk]), Y = W
     for (int row0 = 16 * wave; row0 < mp; row0 += 64) {
         dvec4 dr = {0.0, 0.0, 0.0, 0.0}, di = {0.0, 0.0, 0.0, 0.0};
+        double2 xv[4], cv[4];
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             const int col = 4 * kk + lk, row = row0 + lr;
-            double2 x{0.0, 0.0};
-            if (row < mp && col < jb) x = p[row + col * ldp];
+            xv[kk] = (row < mp && col < jb) ? p[row + col * ldp] : double2{0.0, 0.0};
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = row0 + lk + 4 * q;
+            cv[q] = (row < mp && lr < nc) ? c[row + lr * ldc] : double2{0.0, 0.0};
+        }
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const double2 x = xv[kk];
             const double2 y = wb[kk];
             dr = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, dr, 0, 0, 0);
             dr = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.y, dr, 0, 0, 0);
@@ -595,13 +614,8 @@ __global__ __launch_bounds__(kReflThreads) void apply_reflector_kernel(
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int row = row0 + lk + 4 * q;
-            if (row < mp && lr < nc) {
-                double2* dst = c + row + lr * ldc;
-                double2 v = *dst;
-                v.x -= dr[q];
-                v.y -= di[q];
-                *dst = v;
-            }
+            if (row < mp && lr < nc)
+                c[row + lr * ldc] = double2{cv[q].x - dr[q], cv[q].y - di[q]};
         }
     }
 }
