@@ -227,7 +227,8 @@ def _graded_draw(frame: BlockTensor, side: str, dfull: Dict[int, int], cols: Dic
     wleg = Leg.make((q, c) for q, c in cols.items() if c > 0)
     link = lay.row if side == "left" else lay.col
     play = Layout(link, wleg, lay.shifts) if side == "left" else Layout(wleg, link, lay.shifts)
-    blocks = {}
+    # the stream positions of every kept number, generated in one vectorised pass
+    spans, pos = [], stream.position
     for q, df in dfull.items():
         c = cols.get(q, 0)
         for s, sh in enumerate(lay.shifts):
@@ -235,11 +236,19 @@ def _graded_draw(frame: BlockTensor, side: str, dfull: Dict[int, int], cols: Dic
             dl = link.dim_of(lq)
             if dl == 0:
                 continue
-            vals = stream.draw(dl * c)
-            stream.skip(dl * df - dl * c)
-            if c == 0:
-                continue
-            blk = vals.reshape(c, dl).T
+            if c > 0:
+                spans.append((s, q, lq, dl, c, pos))
+            pos += dl * df
+    stream.skip(pos - stream.position)
+    blocks = {}
+    if spans:
+        idx = np.concatenate([np.arange(p0, p0 + dl * c, dtype=np.uint64)
+                              for (_, _, _, dl, c, p0) in spans])
+        vals = stream.at(idx)
+        o = 0
+        for s, q, lq, dl, c, _ in spans:
+            blk = vals[o: o + dl * c].reshape(c, dl).T
+            o += dl * c
             if side == "left":
                 blocks[(s, lq)] = blk
             else:
@@ -713,8 +722,6 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         factor = DEFAULT_BOND_CEILING_FACTOR if work_ceiling_rule == "bond" else 0.0
     if not (factor == 0 or (factor > 1 and work_ceiling_rule == "bond")):
         raise ValueError("bond_ceiling_factor must be 0, or > 1 with work_ceiling_rule='bond'")
-    if not hermitian:
-        raise NotImplementedError("hermitian=False (Arnoldi root) is not implemented")
     if precision not in ("fp64", "mixed"):
         raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
     cp = _lib.PREC_FP32 if precision == "mixed" else _lib.PREC_FP64
@@ -845,8 +852,12 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         napply[0] += 1
         return ev.apply_zero_site(Lr, Rr, C, precision=cp)
 
-    result, rep = K.lanczos_expv(apply_root, tau, root_start, tol=1e-15, maxiter=K.ROOT_MAXITER,
-                                 reorth=reorth, conv_tol=te, substeps=0)
+    if hermitian:
+        result, rep = K.lanczos_expv(apply_root, tau, root_start, tol=1e-15,
+                                     maxiter=K.ROOT_MAXITER, reorth=reorth, conv_tol=te, substeps=0)
+    else:
+        result, rep = K.arnoldi_expv(apply_root, tau, root_start, tol=1e-15,
+                                     maxiter=K.ROOT_MAXITER, conv_tol=te, substeps=0)
     ledger.expv_err, ledger.expv_iters = rep.err, rep.iterations
     if rep.err > te * (1 + 1e-8):
         ledger.fail("expv")

@@ -99,7 +99,29 @@ class Tridiagonal:
         return self.vecs.astype(complex) @ w, tau * sigma
 
 
-def tail_sup(tri: Tridiagonal, step: complex, betak: float, above: float = math.inf) -> float:
+class Hessenberg:
+    """The leading Arnoldi block, evaluated for any trial step (``_hess_coeffs``,
+    root_methods.jl:136-141): exp(tau (Hm - sigma I)) e1 with the spectral shift sigma taken from
+    the real part of the eigenvalue that grows fastest under tau."""
+
+    def __init__(self, Hm: np.ndarray):
+        if not np.isfinite(Hm).all():
+            raise ValueError("Arnoldi Hessenberg matrix has non-finite entries: the input vector "
+                             "or the operator is not finite")
+        self.Hm = Hm
+        self.vals = np.linalg.eigvals(Hm) if Hm.size else np.zeros(0, complex)
+
+    def coeffs(self, tau: complex):
+        if self.Hm.size == 0:
+            return np.zeros(0, dtype=complex), 0j
+        ex = np.real(tau * self.vals)
+        i = int(np.argmax(ex))
+        sigma = float(self.vals[i].real) if ex[i] > SHIFT_TRIGGER else 0.0
+        prop = sla.expm(tau * (self.Hm - sigma * np.eye(self.Hm.shape[0])))
+        return prop[:, 0], tau * sigma
+
+
+def tail_sup(tri, step: complex, betak: float, above: float = math.inf) -> float:
     tail = betak * abs(tri.coeffs(step)[0][-1])
     if step.real != 0:
         return tail
@@ -110,7 +132,7 @@ def tail_sup(tri: Tridiagonal, step: complex, betak: float, above: float = math.
     return tail
 
 
-def pick_tau(tri: Tridiagonal, tau_rem: complex, betak: float, conv_tol: float) -> complex:
+def pick_tau(tri, tau_rem: complex, betak: float, conv_tol: float) -> complex:
     def ok(t):
         v = tail_sup(tri, t, betak, conv_tol if conv_tol > 0 else math.inf)
         return math.isfinite(v) and (conv_tol <= 0 or v <= conv_tol)
@@ -184,6 +206,73 @@ def lanczos_expv(apply, tau: complex, v0, *, tol: float = 1e-15, maxiter: int = 
         err += tail_sup(tri, tau_s, betak) * cn / beta0
         out = T.scale(coeff[0] * cn, basis[0])
         for c, p in zip(coeff[1:], basis[1:]):
+            T.axpy_(c * cn, p, out)
+        cur = out
+        logscale += ls
+        tau_rem -= tau_s
+        if substeps == 1 or abs(tau_rem) <= 1e-14 * max(abs(tau), 1.0):
+            break
+        if nsub >= 10000:
+            raise RuntimeError("expv: substep search failed to advance")
+    sc = np.exp(logscale)
+    if np.isfinite(abs(sc)) and sc != 0:
+        n = T.norm(cur)
+        if not (np.isfinite(n) and n > 0 and abs(sc) * n > MAX_NORM):
+            cur = T.scale(sc, cur)
+    return cur, ExpvReport(total, nsub, err)
+
+
+def arnoldi_expv(apply, tau: complex, v0, *, tol: float = 1e-13, maxiter: int = ROOT_MAXITER,
+                 conv_tol: float = 0.0, substeps: int = 0):
+    """exp(tau A) v0 for a non-Hermitian device action ``apply`` (``arnoldi_expv``,
+    src/BUG/Core/LocalSolvers/krylov.jl:343-442): upper-Hessenberg Arnoldi with classical
+    Gram-Schmidt applied twice per vector (each pass one batched inner-product launch against the
+    whole basis), the Lanczos path's tail estimate, substep search and scale bookkeeping."""
+    beta0 = T.norm(v0)
+    if beta0 == 0:
+        return v0, ExpvReport(0, 0, 0.0)
+    cur, tau_rem, logscale = v0, complex(tau), 0j
+    total = nsub = 0
+    err = 0.0
+    while True:
+        nsub += 1
+        cn = T.norm(cur)
+        if cn == 0:
+            break
+        basis = [T.scale(1.0 / cn, cur)]
+        H = np.zeros((maxiter + 1, maxiter), dtype=complex)
+        dim = 0
+        betak = 0.0
+        for k in range(1, maxiter + 1):
+            r = apply(basis[k - 1])
+            total += 1
+            for _p in range(2):
+                cs = T.inner_many(basis, r)
+                H[:len(basis), k - 1] += cs
+                for c, p in zip(cs, basis):
+                    T.axpy_(-c, p, r)
+            dim = k
+            betak = T.norm(r)
+            if betak < tol:
+                break
+            if conv_tol > 0 and tail_sup(Hessenberg(H[:dim, :dim].copy()), tau_rem, betak,
+                                         conv_tol) < conv_tol:
+                break
+            H[k, k - 1] = betak
+            if k == maxiter:
+                break
+            basis.append(T.scale(1.0 / betak, r))
+        hess = Hessenberg(H[:dim, :dim].copy())
+        if substeps == 1:
+            tau_s = tau_rem
+        elif substeps > 1:
+            tau_s = tau_rem / (substeps - nsub + 1)
+        else:
+            tau_s = pick_tau(hess, tau_rem, betak, conv_tol)
+        coeff, ls = hess.coeffs(tau_s)
+        err += tail_sup(hess, tau_s, betak) * cn / beta0
+        out = T.scale(coeff[0] * cn, basis[0])
+        for c, p in zip(coeff[1:], basis[1:dim]):
             T.axpy_(c * cn, p, out)
         cur = out
         logscale += ls

@@ -31,18 +31,22 @@ class ProbeStream:
         self._key = _mix(np.array([seed], dtype=np.uint64))[0]
         self.position = 0
 
-    def draw(self, n: int) -> np.ndarray:
-        i = np.arange(self.position, self.position + n, dtype=np.uint64)
+    def at(self, i: np.ndarray) -> np.ndarray:
+        """The numbers at stream positions ``i`` (uint64), without moving the stream."""
         with np.errstate(over="ignore"):
             two_i = np.uint64(2) * i + self._key
             a = _mix(two_i)
             b = _mix(two_i + np.uint64(1))
         u1 = ((a >> np.uint64(11)).astype(np.float64) + 0.5) / 9007199254740992.0
         u2 = (b >> np.uint64(11)).astype(np.float64) / 9007199254740992.0
-        self.position += n
         rad = np.sqrt(-np.log(u1))
         ang = 2.0 * np.pi * u2
         return rad * np.cos(ang) + 1j * rad * np.sin(ang)
+
+    def draw(self, n: int) -> np.ndarray:
+        vals = self.at(np.arange(self.position, self.position + n, dtype=np.uint64))
+        self.position += n
+        return vals
 
     def skip(self, n: int):
         self.position += n

@@ -906,11 +906,16 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
         napply[0] += 1
         return ob.apply_zero_site(Lr, Rr, C)
 
-    result, rep = ok.lanczos_expv(
-        apply_root, tau, root_start, inner=ob.inner, norm=lambda x: x.norm(),
-        axpy=lambda a, x, y: y.axpy(a, x), scale=lambda a, x: x.scale(a),
-        tol=1e-15, maxiter=ok.ROOT_MAXITER, reorth=opts.reorth, conv_tol=opts.tol_expv,
-        substeps=0)
+    vec_ops = dict(inner=ob.inner, norm=lambda x: x.norm(), axpy=lambda a, x, y: y.axpy(a, x),
+                   scale=lambda a, x: x.scale(a))
+    if opts.hermitian:
+        result, rep = ok.lanczos_expv(apply_root, tau, root_start, tol=1e-15,
+                                      maxiter=ok.ROOT_MAXITER, reorth=opts.reorth,
+                                      conv_tol=opts.tol_expv, substeps=0, **vec_ops)
+    else:
+        result, rep = ok.arnoldi_expv(apply_root, tau, root_start, tol=1e-15,
+                                      maxiter=ok.ROOT_MAXITER, conv_tol=opts.tol_expv,
+                                      substeps=0, **vec_ops)
     ledger.expv_err = rep.err
     ledger.expv_iters = rep.iterations
     if rep.err > opts.tol_expv * (1 + 1e-8):

@@ -99,7 +99,7 @@ def shift(vals, tau: complex) -> float:
         return 0.0
     ex = np.real(tau * np.asarray(vals, dtype=complex))
     i = int(np.argmax(ex))
-    return float(vals[i]) if ex[i] > EXPV_SHIFT_TRIGGER else 0.0
+    return float(np.real(vals[i])) if ex[i] > EXPV_SHIFT_TRIGGER else 0.0
 
 
 def coeffs_from_eigen(vals, vecs, tau: complex):
@@ -215,6 +215,93 @@ def lanczos_expv(apply, tau: complex, vector, *, inner, norm, axpy, scale,
         err += tail_sup(evalcoeff, tau_s, betak) * cur_norm / beta0
         out = scale(coeff[0] * cur_norm, basis[0])
         for c, p in zip(coeff[1:], basis[1:]):
+            out = axpy(c * cur_norm, p, out)
+        cur = out
+        logscale += ls
+        tau_rem -= tau_s
+        if substeps == 1 or abs(tau_rem) <= 1e-14 * max(abs(tau), 1.0):
+            break
+        if nsub >= 10000:
+            raise RuntimeError("expv: substep search failed to advance")
+    sc = np.exp(logscale)
+    if np.isfinite(abs(sc)) and sc != 0:
+        cn = norm(cur)
+        if not (np.isfinite(cn) and cn > 0 and abs(sc) * cn > EXPV_MAX_NORM):
+            cur = scale(sc, cur)
+    return cur, ExpvReport(total, nsub, err)
+
+
+def hess_coeffs(Hm: np.ndarray, tau: complex):
+    """_hess_coeffs (root_methods.jl:136-141): exp(tau Hm) e_1 for an upper-Hessenberg Hm, with the
+    spectral shift of the Hermitian path; returns (coefficients, log of the factored scale)."""
+    if Hm.size == 0:
+        return np.zeros(0, dtype=complex), 0j
+    sigma = shift(np.linalg.eigvals(Hm), tau)
+    prop = sla.expm(tau * (Hm - sigma * np.eye(Hm.shape[0])))
+    return prop[:, 0], tau * sigma
+
+
+def arnoldi_expv(apply, tau: complex, vector, *, inner, norm, axpy, scale,
+                 tol: float = 1e-13, maxiter: int = 60, conv_tol: float = 0.0, substeps: int = 1):
+    """exp(tau A) vector for a non-Hermitian ``apply`` (arnoldi_expv, krylov.jl:343-442): the
+    upper-Hessenberg Arnoldi factorisation with every vector orthogonalised twice (CGS2,
+    krylov.jl:321-332), the same tail estimate, substep search and scale bookkeeping as
+    ``lanczos_expv``.  Returns (result, ExpvReport)."""
+    beta0 = norm(vector)
+    if beta0 == 0:
+        return vector, ExpvReport(0, 0, 0.0)
+    cur = vector
+    tau_rem = complex(tau)
+    logscale = 0j
+    total = nsub = 0
+    err = 0.0
+    while True:
+        nsub += 1
+        cur_norm = norm(cur)
+        if cur_norm == 0:
+            break
+        basis = [scale(1.0 / cur_norm, cur)]
+        H = np.zeros((maxiter + 1, maxiter), dtype=complex)
+        dim = 0
+        betak = 0.0
+        for k in range(1, maxiter + 1):
+            res = apply(basis[k - 1])
+            total += 1
+            for _p in range(2):
+                coeffs = [inner(p, res) for p in basis]
+                for row, c in enumerate(coeffs):
+                    H[row, k - 1] += c
+                for c, p in zip(coeffs, basis):
+                    res = axpy(-c, p, res)
+            dim = k
+            rn = norm(res)
+            betak = rn
+            if rn < tol:
+                break
+            if conv_tol > 0:
+                Hk = H[:dim, :dim].copy()
+                tt = tail_sup(lambda t: hess_coeffs(Hk, t), tau_rem, rn, above=conv_tol)
+                if tt < conv_tol:
+                    break
+            H[k, k - 1] = rn
+            if k == maxiter:
+                break
+            basis.append(scale(1.0 / rn, res))
+        Hm = H[:dim, :dim].copy()
+
+        def evalcoeff(t, Hm=Hm):
+            return hess_coeffs(Hm, complex(t))
+
+        if substeps == 1:
+            tau_s = tau_rem
+        elif substeps > 1:
+            tau_s = tau_rem / (substeps - nsub + 1)
+        else:
+            tau_s = pick_tau(evalcoeff, tau_rem, betak, conv_tol)
+        coeff, ls = evalcoeff(tau_s)
+        err += tail_sup(evalcoeff, tau_s, betak) * cur_norm / beta0
+        out = scale(coeff[0] * cur_norm, basis[0])
+        for c, p in zip(coeff[1:], basis[1:dim]):
             out = axpy(c * cur_norm, p, out)
         cur = out
         logscale += ls

@@ -98,3 +98,70 @@ def test_lanczos_expv_matches_expm(tau):
 def test_tridiag_eigen_refuses_nonfinite():
     with pytest.raises(ValueError):
         ok.tridiag_eigen([0.0, float("nan")], [1.0])
+
+
+_VEC = dict(inner=np.vdot, norm=np.linalg.norm, axpy=lambda a, x, y: y + a * x,
+            scale=lambda a, x: a * x)
+
+
+def _sector_fixture():
+    """A vector over four 'sectors' of unequal size and the sector label of each entry: a
+    generator that multiplies each sector by its own number spans exactly four Krylov vectors."""
+    rng = np.random.default_rng(7)
+    labels = np.repeat(np.array([-2, -1, 1, 2]), [3, 5, 2, 4])
+    x = rng.standard_normal(labels.size) + 1j * rng.standard_normal(labels.size)
+    return x, labels
+
+
+@pytest.mark.parametrize("tau", [-0.4 + 0j, -0.7j])
+def test_arnoldi_matches_sector_closed_form(tau):
+    # test/BUG/Core/LocalSolvers/test_krylov.jl:81-87 (atol 1e-11)
+    x, lab = _sector_fixture()
+    s = 0.5 * lab + 1.0
+    y, rep = ok.arnoldi_expv(lambda v: s * v, tau, x, **_VEC)
+    assert np.linalg.norm(y - np.exp(tau * s) * x) <= 1e-11
+    assert rep.iterations == 4          # the true Krylov dimension (test_krylov.jl:166-173)
+
+
+def test_arnoldi_terminates_on_non_hermitian_generator():
+    # test/BUG/Core/LocalSolvers/test_krylov.jl:135-164: complex sector scales
+    x, lab = _sector_fixture()
+    s = (0.5 * lab + 1.0) + 0.3j * lab
+    y, rep = ok.arnoldi_expv(lambda v: s * v, -0.35, x, maxiter=60, **_VEC)
+    assert np.linalg.norm(y - np.exp(-0.35 * s) * x) <= 1e-12
+    assert rep.iterations == 4
+
+
+def test_arnoldi_matches_lanczos_on_hermitian_generator():
+    # test/BUG/Core/LocalSolvers/test_krylov.jl:53-57, on a dense Hermitian generator
+    rng = np.random.default_rng(3)
+    raw = rng.standard_normal((30, 30)) + 1j * rng.standard_normal((30, 30))
+    A = (raw + raw.conj().T) / 4
+    v = rng.standard_normal(30) + 1j * rng.standard_normal(30)
+    kw = dict(tol=1e-15, maxiter=30, conv_tol=1e-12, substeps=0)
+    ya, _ = ok.arnoldi_expv(lambda z: A @ z, -0.2j, v, **kw, **_VEC)
+    yl, _ = ok.lanczos_expv(lambda z: A @ z, -0.2j, v, reorth=True, **kw, **_VEC)
+    assert np.linalg.norm(ya - yl) <= 1e-11
+
+
+@pytest.mark.parametrize("tau", [0.3, 0.05 + 0.02j, -1.0j, 4.0])
+def test_arnoldi_expv_matches_expm_with_substeps(tau):
+    # a non-normal generator; tau = 4 forces the substep search and the spectral shift
+    rng = np.random.default_rng(9)
+    A = (rng.standard_normal((60, 60)) + 1j * rng.standard_normal((60, 60))) / np.sqrt(60)
+    A += 10 * np.eye(60) if tau == 4.0 else 0
+    v = rng.standard_normal(60) + 1j * rng.standard_normal(60)
+    y, rep = ok.arnoldi_expv(lambda z: A @ z, tau, v, tol=1e-15, maxiter=30, conv_tol=1e-10,
+                             substeps=0, **_VEC)
+    ref = sla.expm(tau * A) @ v
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-9
+    assert rep.err <= 1e-10 * (1 + 1e-8) * rep.substeps
+
+
+def test_arnoldi_zero_input_and_shift_uses_real_part():
+    z = np.zeros(5, dtype=complex)
+    y, rep = ok.arnoldi_expv(lambda v: 2 * v, -0.1j, z, **_VEC)
+    assert np.linalg.norm(y) == 0 and rep.iterations == 0
+    # eigenvalues 40 +- 3i under tau = 1: the shift is the real part of the top one
+    assert ok.shift(np.array([40 + 3j, 40 - 3j, 1.0]), 1.0) == 40.0
+    assert ok.shift(np.array([20 + 3j]), 1.0) == 0.0
