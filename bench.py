@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--delta", type=float, default=0.5)
     p.add_argument("--order", type=int, default=2)
     p.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    p.add_argument("--half-sweeps", choices=["serial", "parallel"], default="serial",
+                   help="run the two half-sweeps one after the other or concurrently")
     p.add_argument("--no-fp64", action="store_true",
                    help="with --precision mixed: skip the FP64 comparison run")
     p.add_argument("--workload", choices=["bug_step", "heff_sweep"], default="bug_step")
@@ -128,12 +130,14 @@ def main():
         def make_step(state, form, record):
             def step():
                 info = B.cbe_bug_step(state, mpo, -0.02j, maxdim=D, work_ceiling_rule="maxdim",
-                                      precision=form, order=args.order)
+                                      precision=form, order=args.order,
+                                      parallel=args.half_sweeps == "parallel")
                 if record is not None:
                     record.append(B.budget_status(info.ledger))
+                phases = " ".join(f"{k} {v:.2f}" for k, v in info.seconds.items() if k != "step")
                 print(f"# {form} step: {info.seconds['step']:.2f}s {B.budget_status(info.ledger)} "
-                      f"{info.ledger.unmet} max bond {max(state.bond_dims())}", file=sys.stderr,
-                      flush=True)
+                      f"{info.ledger.unmet} max bond {max(state.bond_dims())} ({phases})",
+                      file=sys.stderr, flush=True)
             return step
 
         step = make_step(psi, args.precision, statuses)
@@ -197,7 +201,8 @@ def main():
                    if args.workload == "bug_step" else
                    "heff_sweep: one-site H_eff apply + left env push at every site",
                    "L": L, "D": D, "model": f"xxz delta={args.delta}", "symmetry": "U1",
-                   "precision": args.precision, "parallelism": "single GPU",
+                   "precision": args.precision, "half_sweeps": args.half_sweeps,
+                   "parallelism": "single GPU",
                    "budget_status": statuses},
         **extra,
         "roofline": roofline,

@@ -224,16 +224,19 @@ class Context:
             pass
 
 
-_contexts = {}
+_local = threading.local()
 
 
 def context(device: int = 0) -> Context:
-    with _lock:
-        ctx = _contexts.get(device)
+    """This thread's context on ``device``: contexts hold plan caches, arenas and pinned staging
+    buffers that are not shared between host threads, so every thread gets its own (created on
+    first use, destroyed with the thread)."""
+    ctxs = getattr(_local, "contexts", None)
+    if ctxs is None:
+        ctxs = _local.contexts = {}
+    ctx = ctxs.get(device)
     if ctx is None:
-        ctx = Context(device)
-        with _lock:
-            _contexts[device] = ctx
+        ctx = ctxs[device] = Context(device)
     return ctx
 
 

@@ -25,11 +25,13 @@ from __future__ import annotations
 import math
 import os
 import sys
+import threading
 import time
 from dataclasses import dataclass, field
 from typing import Callable, Dict, List, Optional
 
 import numpy as np
+import torch
 
 from . import _lib
 from . import environments as ev
@@ -126,6 +128,52 @@ class AdaptiveLedger:
         for u in other.unmet:
             if u not in self.unmet:
                 self.unmet.append(u)
+
+
+_SIDE = {}
+
+
+def _side_executor():
+    """The persistent side thread of parallel half-sweeps (one per process, created on first use:
+    its library context and stream then live as long as the thread)."""
+    ex = _SIDE.get("pool")
+    if ex is None:
+        import concurrent.futures
+
+        ex = _SIDE["pool"] = concurrent.futures.ThreadPoolExecutor(
+            max_workers=1, thread_name_prefix="bug-half-sweep")
+    return ex
+
+
+def _run_beside(main_fn: Callable, side_fn: Callable):
+    """Run ``side_fn`` on the side thread's HIP stream and ``main_fn`` on the caller's, ordered at
+    both ends: the side stream first waits for everything queued so far on the caller's stream,
+    and the caller's stream waits for the side stream's work before the results are used."""
+    from .blocks import device
+
+    main_stream = torch.cuda.current_stream()
+    fork = torch.cuda.Event()
+    fork.record(main_stream)
+
+    def side():
+        local = _SIDE.setdefault(threading.get_ident(), {})
+        st = local.get("stream")
+        if st is None:
+            st = local["stream"] = torch.cuda.Stream(device=device())
+        st.wait_event(fork)
+        with torch.cuda.stream(st):
+            out = side_fn()
+            done = torch.cuda.Event()
+            done.record(st)
+        return out, done
+
+    fut = _side_executor().submit(side)
+    try:
+        a = main_fn()
+    finally:
+        b, done = fut.result()
+    main_stream.wait_event(done)
+    return a, b
 
 
 def budget_status(ledger: AdaptiveLedger) -> str:
@@ -686,7 +734,8 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
                  work_ceiling_rule: str = "bond", bond_ceiling_factor: Optional[float] = None,
                  root: Optional[int] = None, reorth: bool = True, truncate: bool = True,
                  expansion: Optional[DDQRExpansion] = None, rng: Optional[int] = None,
-                 probe_hook: Optional[Callable] = None, precision: str = "fp64") -> CBEBugSweepInfo:
+                 probe_hook: Optional[Callable] = None, precision: str = "fp64",
+                 parallel: bool = False) -> CBEBugSweepInfo:
     """One order-p step in place (``order > 1`` closes the root frontiers j = 1..p-1 by one
     enriching frontier sweep each, ``closure.py``); ``root`` is the 0-based root bond (the
     reference's ``root`` minus 1),
@@ -695,7 +744,12 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     ``precision = "mixed"`` runs the dense block contractions (environment pushes, one- and
     zero-site applies, streamed sketches) on FP32 MFMA with the complex products split into real
     ones and FP32 accumulation; Krylov orthogonalisation, the small exponentials, QR and SVD stay
-    FP64.  The Hermitian check of the local Lanczos chain is then held to FP32 rounding (1e-5)."""
+    FP64.  The Hermitian check of the local Lanczos chain is then held to FP32 rounding (1e-5).
+
+    ``parallel = True`` runs the two half-sweeps concurrently (bug_step.jl:130-134,222-268): the
+    right one on a side host thread with its own library context and HIP stream, drawing from a
+    probe stream split off the step's (so its probes differ from the serial step's, as the
+    reference's do)."""
     L = len(psi)
     # -- option checks, before anything touches the state (bug_step_stages.jl:41-111) --
     if len(mpo) != L:
@@ -722,6 +776,8 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         factor = DEFAULT_BOND_CEILING_FACTOR if work_ceiling_rule == "bond" else 0.0
     if not (factor == 0 or (factor > 1 and work_ceiling_rule == "bond")):
         raise ValueError("bond_ceiling_factor must be 0, or > 1 with work_ceiling_rule='bond'")
+    if probe_hook is not None and parallel:
+        raise ValueError("probe_hook observes one probe sequence; it is not taken with parallel")
     if precision not in ("fp64", "mixed"):
         raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
     cp = _lib.PREC_FP32 if precision == "mixed" else _lib.PREC_FP64
@@ -746,73 +802,96 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     right_frames: List[Optional[BlockTensor]] = [None] * (L - 1)
     secs = {}
 
-    # ---- left half-sweep ----
-    ts = time.perf_counter()
+    # ---- the two half-sweeps (half_sweeps.jl:218-410) ----
+    # The left sweep works on a copy, so `psi` stays left of the root for the assembly; the right
+    # sweep works on `psi` itself.  Under `parallel` they run concurrently, the right one on a side
+    # thread with its own library context and HIP stream, its own probe stream split off the
+    # step's, and each with its own ledger, merged after the join (bug_step.jl:222-268).
     psiL = psi.copy()
-    canonical(psiL, 0)
-    renvs: List[Optional[BlockTensor]] = [None] * (L + 1)
-    renvs[L] = ev.boundary_right(mpo, psiL.bond_leg(L))
-    for i in range(L - 1, 1, -1):
-        renvs[i] = ev.push_right(renvs[i + 1], psiL[i], mpo[i], precision=cp)
-    lch = ev.boundary_left(mpo, psiL.bond_leg(0))
-    lroot = lch
-    carry = None
-    for b in range(rl + 1):
-        if b % 16 == 0:
-            _progress(f"left half-sweep bond {b}/{rl}")
-        carried = psiL[b] if carry is None else T.core_times_site(carry, psiL[b])
-        fr = frame_from(carried, psiL[b + 1])
-        xp = dd_cbe_expand(fr, mpo, b, lch, renvs[b + 2], "left", alloc, ledger, wcap[b], ex,
-                           stream, probe_hook, cp)
-        Kex = T.site_times_core(carried, T.overlap_right(psiL[b + 1], xp.V_ex))
-        rch1 = ev.push_right(renvs[b + 2], xp.V_ex, mpo[b + 1], precision=cp)
-        Hl, Hw, Hr = lch, mpo[b], rch1
-        Wk, kst = krylov_enrich(Kex, lambda v: ev.apply_one_site(Hl, Hw, Hr, v, precision=cp),
-                                xp.U_ex, "left", tau, alloc, wcap[b], recurrence, hermitian_tol=htol)
-        _record_krylov(ledger, kst)
-        left_frames[b] = Wk
-        expanded[b] = Wk.layout.col.total
-        carry = T.overlap_left(Wk, carried)
-        if b == rl:
-            lroot = lch
-        else:
-            lch = ev.push_left(lch, Wk, mpo[b], precision=cp)
-    secs["left_sweep"] = time.perf_counter() - ts
 
-    # ---- right half-sweep ----
-    ts = time.perf_counter()
-    psiR = psi
-    canonical(psiR, L - 1)
-    lenvs = [ev.boundary_left(mpo, psiR.bond_leg(0))]
-    for i in range(L - 2):
-        lenvs.append(ev.push_left(lenvs[-1], psiR[i], mpo[i], precision=cp))
-    rch = ev.boundary_right(mpo, psiR.bond_leg(L))
-    rroot = rch
-    carry = None
-    for b in range(L - 2, rl - 1, -1):
-        if b % 16 == 0:
-            _progress(f"right half-sweep bond {b}/{rl}")
-        carried = psiR[b + 1] if carry is None else T.site_times_core(psiR[b + 1], carry)
-        fr = frame_from(psiR[b], carried)
-        xp = dd_cbe_expand(fr, mpo, b, lenvs[b], rch, "right", alloc, ledger, wcap[b], ex,
-                           stream, probe_hook, cp)
-        Bex = T.core_times_site(T.overlap_left(xp.U_ex, psiR[b]), carried)
-        lch1 = ev.push_left(lenvs[b], xp.U_ex, mpo[b], precision=cp)
-        Hl, Hw, Hr = lch1, mpo[b + 1], rch
-        Zk, kst = krylov_enrich(Bex, lambda v: ev.apply_one_site(Hl, Hw, Hr, v, precision=cp),
-                                xp.V_ex, "right", tau, alloc, wcap[b], recurrence,
-                                hermitian_tol=htol)
-        _record_krylov(ledger, kst)
-        right_frames[b] = Zk
-        if b != rl:
-            expanded[b] = Zk.layout.row.total
-        carry = T.overlap_right(carried, Zk)
-        if b == rl:
-            rroot = rch
-        else:
-            rch = ev.push_right(rch, Zk, mpo[b + 1], precision=cp)
-    AR = carry
-    secs["right_sweep"] = time.perf_counter() - ts
+    def run_left(stream, ledger):
+        ts = time.perf_counter()
+        canonical(psiL, 0)
+        renvs: List[Optional[BlockTensor]] = [None] * (L + 1)
+        renvs[L] = ev.boundary_right(mpo, psiL.bond_leg(L))
+        for i in range(L - 1, 1, -1):
+            renvs[i] = ev.push_right(renvs[i + 1], psiL[i], mpo[i], precision=cp)
+        lch = ev.boundary_left(mpo, psiL.bond_leg(0))
+        lroot = lch
+        carry = None
+        for b in range(rl + 1):
+            if b % 16 == 0:
+                _progress(f"left half-sweep bond {b}/{rl}")
+            carried = psiL[b] if carry is None else T.core_times_site(carry, psiL[b])
+            fr = frame_from(carried, psiL[b + 1])
+            xp = dd_cbe_expand(fr, mpo, b, lch, renvs[b + 2], "left", alloc, ledger, wcap[b], ex,
+                               stream, probe_hook, cp)
+            Kex = T.site_times_core(carried, T.overlap_right(psiL[b + 1], xp.V_ex))
+            rch1 = ev.push_right(renvs[b + 2], xp.V_ex, mpo[b + 1], precision=cp)
+            renvs[b + 2] = None
+            Hl, Hw, Hr = lch, mpo[b], rch1
+            Wk, kst = krylov_enrich(Kex, lambda v: ev.apply_one_site(Hl, Hw, Hr, v, precision=cp),
+                                    xp.U_ex, "left", tau, alloc, wcap[b], recurrence,
+                                    hermitian_tol=htol)
+            _record_krylov(ledger, kst)
+            left_frames[b] = Wk
+            expanded[b] = Wk.layout.col.total
+            carry = T.overlap_left(Wk, carried)
+            if b == rl:
+                lroot = lch
+            else:
+                lch = ev.push_left(lch, Wk, mpo[b], precision=cp)
+        secs["left_sweep"] = time.perf_counter() - ts
+        return lroot
+
+    def run_right(stream, ledger):
+        ts = time.perf_counter()
+        psiR = psi
+        canonical(psiR, L - 1)
+        lenvs = [ev.boundary_left(mpo, psiR.bond_leg(0))]
+        for i in range(L - 2):
+            lenvs.append(ev.push_left(lenvs[-1], psiR[i], mpo[i], precision=cp))
+        rch = ev.boundary_right(mpo, psiR.bond_leg(L))
+        rroot = rch
+        carry = None
+        for b in range(L - 2, rl - 1, -1):
+            if b % 16 == 0:
+                _progress(f"right half-sweep bond {b}/{rl}")
+            carried = psiR[b + 1] if carry is None else T.site_times_core(psiR[b + 1], carry)
+            fr = frame_from(psiR[b], carried)
+            xp = dd_cbe_expand(fr, mpo, b, lenvs[b], rch, "right", alloc, ledger, wcap[b], ex,
+                               stream, probe_hook, cp)
+            Bex = T.core_times_site(T.overlap_left(xp.U_ex, psiR[b]), carried)
+            lch1 = ev.push_left(lenvs[b], xp.U_ex, mpo[b], precision=cp)
+            lenvs[b] = None
+            Hl, Hw, Hr = lch1, mpo[b + 1], rch
+            Zk, kst = krylov_enrich(Bex, lambda v: ev.apply_one_site(Hl, Hw, Hr, v, precision=cp),
+                                    xp.V_ex, "right", tau, alloc, wcap[b], recurrence,
+                                    hermitian_tol=htol)
+            _record_krylov(ledger, kst)
+            right_frames[b] = Zk
+            if b != rl:
+                expanded[b] = Zk.layout.row.total
+            carry = T.overlap_right(carried, Zk)
+            if b == rl:
+                rroot = rch
+            else:
+                rch = ev.push_right(rch, Zk, mpo[b + 1], precision=cp)
+        secs["right_sweep"] = time.perf_counter() - ts
+        return rroot, carry
+
+    if parallel:
+        streamR = stream.split()
+        ledgerL, ledgerR = AdaptiveLedger(), AdaptiveLedger()
+        keep = list(psi.tensors)        # read by the side stream until the join
+        lroot, (rroot, AR) = _run_beside(lambda: run_left(stream, ledgerL),
+                                         lambda: run_right(streamR, ledgerR))
+        del keep
+        ledger.merge(ledgerL)
+        ledger.merge(ledgerR)
+    else:
+        lroot = run_left(stream, ledger)
+        rroot, AR = run_right(stream, ledger)
 
     _progress("root start and projection miss")
     # ---- root start and projection miss ----

@@ -50,3 +50,13 @@ class ProbeStream:
 
     def skip(self, n: int):
         self.position += n
+
+    def split(self) -> "ProbeStream":
+        """A new stream seeded by this stream's next 64-bit word (the stream advances by one
+        number): the parallel right half-sweep's generator, ``MersenneTwister(rand(rng, UInt64))``
+        (src/BUG/TimeEvolution/bug_step.jl:238)."""
+        with np.errstate(over="ignore"):
+            word = _mix(np.array([np.uint64(2) * np.uint64(self.position) + self._key],
+                                 dtype=np.uint64))[0]
+        self.position += 1
+        return ProbeStream(int(word))

@@ -157,6 +157,7 @@ class StepOptions:
     tail_factor: float = 0.10
     krylov_mode: str = "auto"
     seed: int = 0x5EED
+    parallel: bool = False
 
 
 @dataclass
@@ -812,6 +813,16 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
     left_frames: List[Optional[Fam]] = [None] * (rl + 1)
     right_frames: List[Optional[Fam]] = [None] * (L - 1)
 
+    # ---- per-half-sweep generators and ledgers under `parallel` (bug_step.jl:234-268): the
+    # right sweep draws from a generator seeded by the step's next word, each sweep logs into its
+    # own ledger, merged left then right after both; run here one after the other ----
+    if opts.parallel:
+        rngL, rngR = rng, rng.split()
+        ledL, ledR = Ledger(), Ledger()
+    else:
+        rngL = rngR = rng
+        ledL = ledR = ledger
+
     # ---- left half-sweep (half_sweeps.jl:218-310) ----
     psiL = psi.copy()
     canonical(psiL, 0)
@@ -825,15 +836,15 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
     for b in range(rl + 1):
         carried = psiL.sites[b] if carry is None else la.core_times_site(carry, psiL.sites[b])
         fr = frame_from(carried, psiL.sites[b + 1])
-        ex = dd_cbe_expand(fr, mpo, b, lch, renvs[b + 2], "left", alloc, ledger,
-                           _cap_at(wcap, b), opts, rng, probe_hook)
+        ex = dd_cbe_expand(fr, mpo, b, lch, renvs[b + 2], "left", alloc, ledL,
+                           _cap_at(wcap, b), opts, rngL, probe_hook)
         right_projected = la.overlap_right(psiL.sites[b + 1], ex.V_ex)
         Kex = la.site_times_core(carried, right_projected)
         rch1 = ob.push_right(renvs[b + 2], ex.V_ex, mpo[b + 1])
         Hl, Hw, Hr = lch, mpo[b], rch1
         Wk, kst = krylov_enrich(Kex, lambda v: ob.apply_one_site(Hl, Hw, Hr, v), ex.U_ex, "left",
                                 tau, alloc, _cap_at(wcap, b), recurrence)
-        record_krylov(ledger, kst, alloc)
+        record_krylov(ledL, kst, alloc)
         left_frames[b] = Wk
         expanded[b] = sum(Wk.col.values())
         carry = la.overlap_left(Wk, carried)
@@ -854,15 +865,15 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
     for b in range(L - 2, rl - 1, -1):
         carried = psiR.sites[b + 1] if carry is None else la.site_times_core(psiR.sites[b + 1], carry)
         fr = frame_from(psiR.sites[b], carried)
-        ex = dd_cbe_expand(fr, mpo, b, lenvs[b], rch, "right", alloc, ledger,
-                           _cap_at(wcap, b), opts, rng, probe_hook)
+        ex = dd_cbe_expand(fr, mpo, b, lenvs[b], rch, "right", alloc, ledR,
+                           _cap_at(wcap, b), opts, rngR, probe_hook)
         left_projected = la.overlap_left(ex.U_ex, psiR.sites[b])
         Bex = la.core_times_site(left_projected, carried)
         lch1 = ob.push_left(lenvs[b], ex.U_ex, mpo[b])
         Hl, Hw, Hr = lch1, mpo[b + 1], rch
         Zk, kst = krylov_enrich(Bex, lambda v: ob.apply_one_site(Hl, Hw, Hr, v), ex.V_ex, "right",
                                 tau, alloc, _cap_at(wcap, b), recurrence)
-        record_krylov(ledger, kst, alloc)
+        record_krylov(ledR, kst, alloc)
         right_frames[b] = Zk
         if b != rl:
             expanded[b] = sum(Zk.row.values())
@@ -872,6 +883,10 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
         else:
             rch = ob.push_right(rch, Zk, mpo[b + 1])
     AR = carry
+    if opts.parallel:
+        from .closure import _merge
+        _merge(ledger, ledL)
+        _merge(ledger, ledR)
 
     # ---- root start, projection miss (bug_step.jl:278-327) ----
     AL = None

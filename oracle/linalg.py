@@ -448,3 +448,13 @@ class ProbeRNG:
 
     def skip(self, n: int):
         self.pos += n
+
+    def split(self) -> "ProbeRNG":
+        """A generator seeded by the next 64-bit word of this one, which advances by one number
+        (the parallel right half-sweep's ``MersenneTwister(rand(rng, UInt64))``,
+        src/BUG/TimeEvolution/bug_step.jl:238)."""
+        with np.errstate(over="ignore"):
+            word = _splitmix(np.array([self.key + np.uint64(2) * np.uint64(self.pos)],
+                                      dtype=np.uint64))[0]
+        self.pos += 1
+        return ProbeRNG(int(word))

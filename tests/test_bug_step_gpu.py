@@ -151,3 +151,30 @@ def test_mixed_precision_refuses_unknown_form(gpu):
 
     with pytest.raises(ValueError):
         B.cbe_bug_step(mps.neel_state(6), xxz_mpo(6), -0.1j, precision="bf16")
+
+
+@pytest.mark.parametrize("start,L,delta,rule,steps,order", [("neel", 10, 0.5, "maxdim", 3, 1),
+                                                             ("dw", 12, 0.0, "bond", 3, 2),
+                                                             ("neel", 10, 1.0, "maxdim", 2, 2)])
+def test_parallel_half_sweeps_match_oracle(gpu, start, L, delta, rule, steps, order):
+    """parallel = True: the right half-sweep on a side thread and stream with the split probe
+    stream, against the oracle's serial run of the same split (bug_step.jl:222-268)."""
+    from bondupdategalerkin_amd import bug_step as B
+    from bondupdategalerkin_amd import mps
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    psi_d = _device_state(start, L)
+    psi_o = _oracle_state(start, L)
+    mpo_d, mpo_o = xxz_mpo(L, delta=delta), om.xxz_mpo(L, delta=delta)
+    for n in range(steps):
+        info_d = B.cbe_bug_step(psi_d, mpo_d, -0.05j, work_ceiling_rule=rule, order=order,
+                                parallel=True)
+        info_o = obug.cbe_bug_step(psi_o, mpo_o, -0.05j,
+                                   obug.StepOptions(work_ceiling_rule=rule, order=order,
+                                                    parallel=True))
+        assert psi_d.bond_dims() == psi_o.bond_dims(), n
+        assert B.budget_status(info_d.ledger) == info_o.ledger.status, n
+        assert info_d.ledger.unmet == info_o.ledger.unmet, n
+        vd, vo = _dense(psi_d), om.mps_dense(psi_o.sites)
+        assert np.linalg.norm(vd - vo) / np.linalg.norm(vo) <= 1e-10, n
+        assert np.abs(mps.magnetisation(psi_d) - obug.sz_profile(psi_o)).max() <= 1e-9, n
