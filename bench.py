@@ -54,6 +54,10 @@ def parse():
     p.add_argument("--kernel-reps", type=int, default=20)
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--serving", type=int, default=12,
+                   help="requests of the seeded Poisson serving mix to replay (0: skip)")
+    p.add_argument("--serving-rate", type=float, default=1.0, help="arrivals per second")
+    p.add_argument("--slots", type=int, default=4, help="concurrent requests of the serving loop")
     p.add_argument("--kernel-only", action="store_true",
                    help="run only the dominant-kernel timing (for profiling)")
     return p.parse_args()
@@ -178,6 +182,20 @@ def main():
         dsz = float(np.abs(M.magnetisation(psi) - M.magnetisation(twin)).max())
         extra["fp64"]["max_abs_dSz_mixed_vs_fp64"] = dsz
         extra["fp64"]["speedup_mixed_vs_fp64"] = round(extra["fp64"]["ms_per_step"] / ms_per_step, 3)
+
+    if args.serving > 0 and args.workload == "bug_step":
+        from bondupdategalerkin_amd import serving as S
+
+        srv = {"mix": (f"{args.serving} requests, Poisson arrivals at {args.serving_rate}/s "
+                       "(seed 0): 60% XX/XXZ chains L in {16,24,32}, maxdim in {16,32,64}, 2-6 "
+                       "steps, order 2; 40% Heston QTT 2^5-2^7 per axis, maxdim 16-32, 2-4 steps"),
+               "slots": args.slots}
+        for form in (["mixed", "fp64"] if args.precision == "mixed" else ["fp64"]):
+            print(f"# serving mix ({form})", file=sys.stderr, flush=True)
+            srv[form] = S.serve_mix(S.poisson_mix(args.serving, args.serving_rate, seed=0),
+                                    args.slots, form)
+            print(f"# serving {form}: {srv[form]}", file=sys.stderr, flush=True)
+        extra["serving"] = srv
 
     cpu = None
     if not args.no_cpu:
