@@ -165,7 +165,8 @@ struct bug_ctx {
     // keyed by (stream, slot)
     std::unordered_map<void*, bug::Workspace> scratch[8];
     // SVD kernel choice (0 automatic, 1 plain one-sided Jacobi, 2 QR-preconditioned with one workgroup
-    // per matrix, 3 QR-preconditioned multi-workgroup block Jacobi) and an optional
+    // per matrix, 3 QR-preconditioned multi-workgroup block Jacobi, 4 QR-preconditioned Gram block
+    // Jacobi over row chunks) and an optional
     // device array receiving the sweeps used per matrix of the last batch (diagnostics)
     int svd_variant = 0;
     int* svd_sweeps_dev = nullptr;
@@ -175,6 +176,12 @@ struct bug_ctx {
     // and one-sided sweeps over the staged columns per task
     int bj_cols = 0;
     int bj_inner = 1;
+    // Gram block Jacobi: columns per block pair, inner sweeps per round, and the smallest matrix
+    // dimension of a batch for which the automatic choice takes it over the staged kernel (by
+    // default never: on the step's blocks, 100-512 wide, it runs no faster than the staged kernel)
+    int gbj_cols = 32;
+    int gbj_inner = 1;
+    int gbj_min_n = 1 << 30;
     // dynamic LDS bytes a workgroup may use on this device (queried on first use)
     int lds_limit = 0;
     // pinned host staging for small reductions

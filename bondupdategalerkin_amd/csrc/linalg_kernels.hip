@@ -1097,6 +1097,224 @@ __global__ __launch_bounds__(kBjThreads) void bj_round_kernel(const BjMat* __res
     if (tid == 0) flags_cur[tk.x] = 1;
 }
 
+// ---- Gram block Jacobi: the block pair's work split over row chunks --------------------------
+//
+// Block one-sided Jacobi where a round's pair (p, q) is not staged whole in one workgroup: the
+// rows are cut into chunks of kGbjRows, each handled by its own workgroup, so a round runs
+// (pairs x chunks) workgroups and the column block width no longer depends on the row count.
+// Per round two launches:
+//   gbj_gram_kernel   -- each chunk's partial Gram matrix G_k = X_k^H X_k of the pair's c <= 32
+//                        columns, written to a per-(pair, chunk) slot;
+//   gbj_rotate_kernel -- every chunk's workgroup sums the partials in chunk order (the same sum
+//                        in every workgroup), runs the pair's inner sweeps of two-sided rotations
+//                        on G itself (the one-sided rotations of the columns, applied to their
+//                        Gram matrix: G <- J^H G J, Z <- Z J), and applies the accumulated Z to its
+//                        row chunk of X and of V.
+// The rotation test and angles are those of the staged kernel (|g_ab|^2 > eps^2 n g_aa g_bb);
+// the Gram matrix is recomputed from the columns every round, so its updates within the round
+// never accumulate past one round, and convergence is judged on fresh inner products.
+constexpr int kGbjCols = 32;   // columns of a block pair (2w)
+constexpr int kGbjRows = 64;   // rows per chunk
+constexpr int kGbjThreads = 256;
+
+__global__ __launch_bounds__(kGbjThreads) void gbj_gram_kernel(const BjMat* __restrict__ mats,
+                                                               const int2* __restrict__ tasks,
+                                                               const double2* __restrict__ X,
+                                                               double2* __restrict__ gbuf, int round,
+                                                               const int* __restrict__ flags_prev,
+                                                               int maxch) {
+    __shared__ double2 xs[kGbjCols][kGbjRows + 1];
+    const int2 tk = tasks[blockIdx.x];
+    const BjMat mt = mats[tk.x];
+    if (round >= mt.N - 1 || flags_prev[tk.x] == 0) return;
+    const int r0 = blockIdx.y * kGbjRows;
+    if (r0 >= mt.n) return;
+    int bp, bq;
+    round_robin(mt.N, round, tk.y, bp, bq);
+    const int n = mt.n, w = mt.w;
+    const int cp = bp < mt.nb ? min(w, n - bp * w) : 0;
+    const int cq = bq < mt.nb ? min(w, n - bq * w) : 0;
+    const int c = cp + cq;
+    if (c < 2) return;
+    const int rows = min(kGbjRows, n - r0);
+    const double2* x = X + mt.x_off;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kGbjCols * kGbjRows; e += kGbjThreads) {
+        const int j = e / kGbjRows, i = e % kGbjRows;
+        double2 val = double2{0.0, 0.0};
+        if (j < c && i < rows) {
+            const int col = j < cp ? bp * w + j : bq * w + (j - cp);
+            val = x[(int64_t)col * n + r0 + i];
+        }
+        xs[j][i] = val;
+    }
+    __syncthreads();
+    // thread (bi, bj) owns the 2 x 2 block G[2bi:2bi+2, 2bj:2bj+2]
+    const int bi = tid >> 4, bj = tid & 15;
+    double2 g00{0.0, 0.0}, g01{0.0, 0.0}, g10{0.0, 0.0}, g11{0.0, 0.0};
+    for (int i = 0; i < rows; ++i) {
+        const double2 a0 = xs[2 * bi][i], a1 = xs[2 * bi + 1][i];
+        const double2 b0 = xs[2 * bj][i], b1 = xs[2 * bj + 1][i];
+        // conj(a) b
+        g00.x += a0.x * b0.x + a0.y * b0.y;  g00.y += a0.x * b0.y - a0.y * b0.x;
+        g01.x += a0.x * b1.x + a0.y * b1.y;  g01.y += a0.x * b1.y - a0.y * b1.x;
+        g10.x += a1.x * b0.x + a1.y * b0.y;  g10.y += a1.x * b0.y - a1.y * b0.x;
+        g11.x += a1.x * b1.x + a1.y * b1.y;  g11.y += a1.x * b1.y - a1.y * b1.x;
+    }
+    double2* g = gbuf + ((int64_t)blockIdx.x * maxch + blockIdx.y) * (kGbjCols * kGbjCols);
+    g[(2 * bi) * kGbjCols + 2 * bj] = g00;
+    g[(2 * bi) * kGbjCols + 2 * bj + 1] = g01;
+    g[(2 * bi + 1) * kGbjCols + 2 * bj] = g10;
+    g[(2 * bi + 1) * kGbjCols + 2 * bj + 1] = g11;
+}
+
+__global__ __launch_bounds__(kGbjThreads) void gbj_rotate_kernel(const BjMat* __restrict__ mats,
+                                                                 const int2* __restrict__ tasks,
+                                                                 double2* __restrict__ X,
+                                                                 double2* __restrict__ V,
+                                                                 const double2* __restrict__ gbuf,
+                                                                 int round,
+                                                                 const int* __restrict__ flags_prev,
+                                                                 int* __restrict__ flags_cur,
+                                                                 double eps, int inner, int maxch) {
+    __shared__ double2 G[kGbjCols][kGbjCols + 1];
+    __shared__ double2 Z[kGbjCols][kGbjCols + 1];
+    __shared__ double2 xs[kGbjRows / 2][kGbjCols + 1];
+    __shared__ double rot[kGbjCols / 2][4];
+    __shared__ int rab[kGbjCols / 2][2];
+    __shared__ int s_rot;
+    const int2 tk = tasks[blockIdx.x];
+    const BjMat mt = mats[tk.x];
+    if (round >= mt.N - 1 || flags_prev[tk.x] == 0) return;
+    const int r0 = blockIdx.y * kGbjRows;
+    if (r0 >= mt.n) return;
+    int bp, bq;
+    round_robin(mt.N, round, tk.y, bp, bq);
+    const int n = mt.n, w = mt.w;
+    const int cp = bp < mt.nb ? min(w, n - bp * w) : 0;
+    const int cq = bq < mt.nb ? min(w, n - bq * w) : 0;
+    const int c = cp + cq;
+    if (c < 2) return;
+    const int rows = min(kGbjRows, n - r0);
+    const int nch = (n + kGbjRows - 1) / kGbjRows;
+    const int tid = threadIdx.x;
+    const double2* gp = gbuf + (int64_t)blockIdx.x * maxch * (kGbjCols * kGbjCols);
+    for (int e = tid; e < c * c; e += kGbjThreads) {
+        const int a = e / c, b = e % c;
+        double sr = 0.0, si = 0.0;
+        for (int k = 0; k < nch; ++k) {
+            const double2 v = gp[(int64_t)k * (kGbjCols * kGbjCols) + a * kGbjCols + b];
+            sr += v.x;
+            si += v.y;
+        }
+        G[a][b] = double2{sr, si};
+        Z[a][b] = double2{a == b ? 1.0 : 0.0, 0.0};
+    }
+    if (tid == 0) s_rot = 0;
+    __syncthreads();
+    const double tol2 = eps * eps * (double)n;
+    const bool cross_only = round > 0 && cp > 0 && cq > 0;
+    const int big = cp >= cq ? cp : cq, small = cp >= cq ? cq : cp;
+    const int big0 = cp >= cq ? 0 : cp, small0 = cp >= cq ? cp : 0;
+    const int C = c + (c & 1);
+    const int np = cross_only ? small : C / 2;
+    const int nrounds = cross_only ? big : C - 1;
+    for (int pass = 0; pass < inner; ++pass) {
+        for (int r = 0; r < nrounds; ++r) {
+            if (tid < np) {
+                int a, b;
+                bool valid = true;
+                if (cross_only) {
+                    const int i1 = small0 + tid, i2 = big0 + (tid + r) % big;
+                    a = min(i1, i2);
+                    b = max(i1, i2);
+                } else {
+                    round_robin(C, r, tid, a, b);
+                    valid = b < c;
+                }
+                int ra = -1;
+                if (valid) {
+                    const double al = G[a][a].x, be = G[b][b].x;
+                    const double gr = G[a][b].x, gi = G[a][b].y;
+                    const double ag2 = gr * gr + gi * gi;
+                    if (ag2 > tol2 * (al * be) && ag2 != 0.0) {
+                        const double ag = sqrt(ag2);
+                        const double inv = 1.0 / ag;
+                        const double zeta = 0.5 * (be - al) * inv;
+                        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                        const double cs = 1.0 / sqrt(1.0 + t * t);
+                        rot[tid][0] = cs;
+                        rot[tid][1] = cs * t;
+                        rot[tid][2] = gr * inv;
+                        rot[tid][3] = gi * inv;
+                        ra = a;
+                        s_rot = 1;
+                    }
+                }
+                rab[tid][0] = ra;
+                rab[tid][1] = b;
+            }
+            __syncthreads();
+            // columns: M[:, a] <- c M[:, a] - s conj(e) M[:, b];  M[:, b] <- s e M[:, a] + c M[:, b]
+            for (int e = tid; e < np * c; e += kGbjThreads) {
+                const int g = e / c, k = e % c;
+                const int a = rab[g][0];
+                if (a < 0) continue;
+                const int b = rab[g][1];
+                const double cs = rot[g][0], sn = rot[g][1], er = rot[g][2], ei = rot[g][3];
+                double2 p = G[k][a], q = G[k][b];
+                G[k][a] = double2{cs * p.x - sn * (er * q.x + ei * q.y), cs * p.y - sn * (er * q.y - ei * q.x)};
+                G[k][b] = double2{sn * (er * p.x - ei * p.y) + cs * q.x, sn * (er * p.y + ei * p.x) + cs * q.y};
+                p = Z[k][a];
+                q = Z[k][b];
+                Z[k][a] = double2{cs * p.x - sn * (er * q.x + ei * q.y), cs * p.y - sn * (er * q.y - ei * q.x)};
+                Z[k][b] = double2{sn * (er * p.x - ei * p.y) + cs * q.x, sn * (er * p.y + ei * p.x) + cs * q.y};
+            }
+            __syncthreads();
+            // rows: M[a, :] <- c M[a, :] - s e M[b, :];  M[b, :] <- s conj(e) M[a, :] + c M[b, :]
+            for (int e = tid; e < np * c; e += kGbjThreads) {
+                const int g = e / c, k = e % c;
+                const int a = rab[g][0];
+                if (a < 0) continue;
+                const int b = rab[g][1];
+                const double cs = rot[g][0], sn = rot[g][1], er = rot[g][2], ei = rot[g][3];
+                const double2 p = G[a][k], q = G[b][k];
+                G[a][k] = double2{cs * p.x - sn * (er * q.x - ei * q.y), cs * p.y - sn * (er * q.y + ei * q.x)};
+                G[b][k] = double2{sn * (er * p.x + ei * p.y) + cs * q.x, sn * (er * p.y - ei * p.x) + cs * q.y};
+            }
+            __syncthreads();
+        }
+    }
+    if (!s_rot) return;                       // uniform: read after the last barrier
+    auto gcol = [&](int j) { return j < cp ? bp * w + j : bq * w + (j - cp); };
+    // M[rows, cols] <- M[rows, cols] Z for M = X, then V, half a chunk of rows at a time
+    constexpr int H = kGbjRows / 2;
+    for (int pass = 0; pass < 4; ++pass) {
+        double2* m = (pass < 2 ? X + mt.x_off : V + mt.v_off);
+        const int h0 = (pass & 1) * H;
+        const int hr = min(H, rows - h0);
+        if (hr <= 0) continue;                // uniform
+        for (int e = tid; e < c * H; e += kGbjThreads) {
+            const int j = e / H, i = e % H;
+            if (i < hr) xs[i][j] = m[(int64_t)gcol(j) * n + r0 + h0 + i];
+        }
+        __syncthreads();
+        for (int e = tid; e < c * H; e += kGbjThreads) {
+            const int j = e / H, i = e % H;
+            if (i >= hr) continue;
+            double sr = 0.0, si = 0.0;
+            for (int k = 0; k < c; ++k) {
+                const double2 p = xs[i][k], z = Z[k][j];
+                sr += p.x * z.x - p.y * z.y;
+                si += p.x * z.y + p.y * z.x;
+            }
+            m[(int64_t)gcol(j) * n + r0 + h0 + i] = double2{sr, si};
+        }
+        __syncthreads();
+    }
+    if (tid == 0) flags_cur[tk.x] = 1;
+}
+
 // V = I per matrix; flags row 0 = 1 (every matrix runs its first sweep), later rows = 0.
 __global__ void bj_setup_kernel(const BjMat* __restrict__ mats, int nmat, double2* __restrict__ V,
                                 int* __restrict__ flags, int nrows) {
@@ -1361,6 +1579,78 @@ void block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list,
         HIP_CHECK(hipStreamSynchronize(st));
     }
 }
+
+// The Gram block Jacobi (gbj_gram_kernel + gbj_rotate_kernel per round) of the n x n matrices X_i
+// at the running offsets sum_{k<i} n_k^2, with V_i at list[i].v_off (set to I here); sweeps as in
+// block_jacobi.
+void gram_block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list, double2* X,
+                       double2* V, int max_sweeps, int* sweeps_dev) {
+    const int nmat = (int)list.size();
+    const int c = std::max(2, std::min(kGbjCols, ctx->gbj_cols) & ~1);
+    const int w = c / 2;
+    std::vector<BjMat> mats(nmat);
+    std::vector<int2> tasks;
+    int64_t xo = 0;
+    int rounds = 0, maxch = 1;
+    for (int i = 0; i < nmat; ++i) {
+        const int n = list[i].n;
+        const int nb = (n + w - 1) / w;
+        const int N = nb + (nb & 1);
+        mats[i] = BjMat{xo, list[i].v_off, n, w, nb, N};
+        xo += (int64_t)n * n;
+        if (n >= 2) {
+            for (int p = 0; p < N / 2; ++p) tasks.push_back(make_int2(i, p));
+            rounds = std::max(rounds, N - 1);
+            maxch = std::max(maxch, (n + kGbjRows - 1) / kGbjRows);
+        }
+    }
+    const int nrows = max_sweeps + 1;
+    int* flags = static_cast<int*>(scratch(ctx, st, sizeof(int) * (size_t)nmat * nrows, 3));
+    std::vector<unsigned char> blob(sizeof(BjMat) * nmat + sizeof(int2) * tasks.size());
+    std::memcpy(blob.data(), mats.data(), sizeof(BjMat) * nmat);
+    if (!tasks.empty()) std::memcpy(blob.data() + sizeof(BjMat) * nmat, tasks.data(), sizeof(int2) * tasks.size());
+    unsigned char* dblob = upload_list(ctx, st, blob.data(), blob.size(), 2);
+    const BjMat* dm = reinterpret_cast<const BjMat*>(dblob);
+    const int2* dt = reinterpret_cast<const int2*>(dblob + sizeof(BjMat) * nmat);
+    hipLaunchKernelGGL(bj_setup_kernel, dim3(nmat, 16), dim3(256), 0, st, dm, nmat, V, flags, nrows);
+    HIP_CHECK(hipGetLastError());
+    int used = 0;
+    if (!tasks.empty()) {
+        double2* gbuf = static_cast<double2*>(scratch(
+            ctx, st, sizeof(double2) * (size_t)kGbjCols * kGbjCols * maxch * tasks.size(), 6));
+        int* hp = reinterpret_cast<int*>(pinned(ctx, sizeof(int) * nmat));
+        const dim3 grid((unsigned)tasks.size(), (unsigned)maxch);
+        for (int s = 0; s < max_sweeps; ++s) {
+            for (int r = 0; r < rounds; ++r) {
+                hipLaunchKernelGGL(gbj_gram_kernel, grid, dim3(kGbjThreads), 0, st, dm, dt, X, gbuf,
+                                   r, flags + (size_t)s * nmat, maxch);
+                hipLaunchKernelGGL(gbj_rotate_kernel, grid, dim3(kGbjThreads), 0, st, dm, dt, X, V,
+                                   gbuf, r, flags + (size_t)s * nmat, flags + (size_t)(s + 1) * nmat,
+                                   2.220446049250313e-16, std::max(1, ctx->gbj_inner), maxch);
+            }
+            HIP_CHECK(hipGetLastError());
+            ++used;
+            HIP_CHECK(hipMemcpyAsync(hp, flags + (size_t)(s + 1) * nmat, sizeof(int) * nmat,
+                                     hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipStreamSynchronize(st));
+            bool any = false;
+            for (int i = 0; i < nmat; ++i) any |= hp[i] != 0;
+            if (!any) break;
+        }
+    }
+    if (sweeps_dev) {
+        std::vector<int> all((size_t)nmat * nrows), host(nmat);
+        HIP_CHECK(hipMemcpyAsync(all.data(), flags, sizeof(int) * all.size(), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        for (int i = 0; i < nmat; ++i) {
+            int k = 0;
+            for (int s = 1; s <= used; ++s) k += all[(size_t)s * nmat + i] != 0;
+            host[i] = std::min(k + 1, used);
+        }
+        HIP_CHECK(hipMemcpyAsync(sweeps_dev, host.data(), sizeof(int) * nmat, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+    }
+}
 }  // namespace
 
 extern "C" {
@@ -1527,6 +1817,16 @@ int bug_set_jacobi_params(bug_ctx* ctx, int32_t cols, int32_t inner_sweeps) {
     });
 }
 
+int bug_set_gram_jacobi_params(bug_ctx* ctx, int32_t cols, int32_t inner_sweeps, int32_t min_n) {
+    return guarded(ctx, [&] {
+        if (cols < 2 || cols > kGbjCols || inner_sweeps < 1 || inner_sweeps > 8 || min_n < 0)
+            throw Error(BUG_EINVAL, "gram jacobi params: cols in 2..32, inner sweeps in 1..8, min_n >= 0");
+        ctx->gbj_cols = cols & ~1;
+        ctx->gbj_inner = inner_sweeps;
+        ctx->gbj_min_n = min_n;
+    });
+}
+
 int bug_set_qr_variant(bug_ctx* ctx, int32_t variant) {
     return guarded(ctx, [&] {
         if (variant < 0 || variant > 2) throw Error(BUG_EINVAL, "qr variant must be 0, 1 or 2");
@@ -1619,12 +1919,17 @@ int bug_block_svd(bug_ctx* ctx, int32_t nmat, const bug_svd_job* jobs, void* g, 
                                nullptr, 0);
             HIP_CHECK(hipGetLastError());
             const size_t lds = (size_t)nmax * (sizeof(double) + sizeof(int));
-            if (ctx->svd_variant == 2 || !block_jacobi_fits(ctx, nmax)) {
+            const bool gram = ctx->svd_variant == 4 ||
+                              (ctx->svd_variant == 0 && nmax >= ctx->gbj_min_n);
+            if (!gram && (ctx->svd_variant == 2 || !block_jacobi_fits(ctx, nmax))) {
                 SvdJob* dj = upload_list(ctx, st, jj.data(), jj.size(), 2);
                 hipLaunchKernelGGL(svd_jacobi_kernel, dim3(nmat), dim3(kSvdThreads), lds, st, dj,
                                    Xw, v, Uxw, s, Vxw, sweeps, sw);
             } else {
-                block_jacobi(ctx, st, list, Xw, v, sweeps, sw);
+                if (gram)
+                    gram_block_jacobi(ctx, st, list, Xw, v, sweeps, sw);
+                else
+                    block_jacobi(ctx, st, list, Xw, v, sweeps, sw);
                 SvdJob* dj = upload_list(ctx, st, jj.data(), jj.size(), 2);
                 hipLaunchKernelGGL(svd_finish_kernel, dim3(nmat), dim3(kSvdThreads), lds, st, dj,
                                    Xw, v, Uxw, s, Vxw);
@@ -1673,7 +1978,7 @@ extern "C" {
 
 int bug_set_svd_variant(bug_ctx* ctx, int32_t variant, int32_t record_sweeps) {
     return guarded(ctx, [&] {
-        if (variant < 0 || variant > 3) throw Error(BUG_EINVAL, "svd variant must be 0 .. 3");
+        if (variant < 0 || variant > 4) throw Error(BUG_EINVAL, "svd variant must be 0 .. 4");
         ctx->svd_variant = variant;
         if (record_sweeps && !ctx->svd_sweeps_dev) {
             HIP_CHECK(hipMalloc(&ctx->svd_sweeps_dev, sizeof(int) * 65536));

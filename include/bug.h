@@ -220,7 +220,8 @@ int bug_block_svd(bug_ctx* ctx, int32_t nmat, const bug_svd_job* jobs, void* g, 
 
 /* Diagnostics: force the SVD kernel (0 automatic, 1 one-sided Jacobi with one workgroup per
  * matrix, 2 QR-preconditioned one-workgroup Jacobi, 3 QR-preconditioned multi-workgroup block
- * Jacobi) and, with record_sweeps != 0, record the Jacobi sweeps each matrix of a batch used;
+ * Jacobi with the block pair staged in LDS, 4 QR-preconditioned Gram block Jacobi with the pair's
+ * rows split over workgroups) and, with record_sweeps != 0, record the Jacobi sweeps each matrix of a batch used;
  * bug_svd_sweeps copies the counts of the last batch (synchronizes the device). */
 int bug_set_svd_variant(bug_ctx* ctx, int32_t variant, int32_t record_sweeps);
 
@@ -233,6 +234,11 @@ int bug_set_qr_variant(bug_ctx* ctx, int32_t variant);
  * holds, at most 32; an explicit value only lowers it) and one-sided sweeps over the staged
  * columns per task (1..8). */
 int bug_set_jacobi_params(bug_ctx* ctx, int32_t cols, int32_t inner_sweeps);
+
+/* Tuning of the Gram block Jacobi: columns per block pair (2..32), rotation sweeps over the pair's
+ * Gram matrix per round (1..8), and the smallest matrix dimension of a batch from which the
+ * automatic SVD choice uses it. */
+int bug_set_gram_jacobi_params(bug_ctx* ctx, int32_t cols, int32_t inner_sweeps, int32_t min_n);
 int bug_svd_sweeps(bug_ctx* ctx, int32_t n, int32_t* out);
 
 #ifdef __cplusplus
