@@ -550,7 +550,10 @@ def dd_cbe_expand(fr: BondFrame, mpo: MPO, bond: int, lenv, renv, side: str, all
         Ysk = stream_right(lenv, W1, W2, renv, fr.U0, T.core_times_site(fr.S0, fr.V0), Om, prec)
         project = lambda t: T.perp_right(fr.V0, t)
     Y = project(Ysk)
-    if T.norm(Y) <= CBE_EPS:
+    ny = T.norm(Y)
+    if not math.isfinite(ny):
+        raise FloatingPointError(f"dd_cbe_expand: non-finite sketch at bond {bond} ({side})")
+    if ny <= CBE_EPS:
         return none
     rmax = min(cap, _width(Om, oside))
     cache = {}
@@ -794,6 +797,8 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     else:
         wcap = [max(d * maxdim, d * max(bd0 + [0]), 1)] * (L - 1)
     nrm = T.norm(psi[psi.center]) if 0 <= psi.center < L else 1.0
+    if not math.isfinite(nrm):
+        raise FloatingPointError("cbe_bug_step: the state is not finite")
     alloc = basis_allocation(tb, L - 1, tau, nrm, order, tol_krylov)
     ledger = AdaptiveLedger(r_old=max(bd0 + [0]))
     recurrence = ex.krylov_mode if ex.krylov_mode != "auto" else ("lanczos" if hermitian else "arnoldi")

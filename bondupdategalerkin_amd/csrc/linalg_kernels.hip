@@ -791,10 +791,16 @@ __device__ void svd_finish(const SvdJob& job, const double2* __restrict__ g,
         if (lane == 0) sig[j] = sqrt(s2);
     }
     __syncthreads();
+    // rank by a strict total order (descending value, ties by index, a NaN below everything) so
+    // that perm is a permutation whatever the input held: a non-finite block then yields NaN
+    // outputs for the caller to refuse, never an index outside the block
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
         int rank = 0;
-        const double sj = sig[j];
-        for (int i = 0; i < n; ++i) rank += (sig[i] > sj) || (sig[i] == sj && i < j);
+        const double sj = sig[j] == sig[j] ? sig[j] : -1.0;
+        for (int i = 0; i < n; ++i) {
+            const double si = sig[i] == sig[i] ? sig[i] : -1.0;
+            rank += (si > sj) || (si == sj && i < j);
+        }
         perm[rank] = j;
     }
     __syncthreads();

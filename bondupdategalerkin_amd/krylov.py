@@ -118,6 +118,8 @@ class Hessenberg:
         i = int(np.argmax(ex))
         sigma = float(self.vals[i].real) if ex[i] > SHIFT_TRIGGER else 0.0
         prop = sla.expm(tau * (self.Hm - sigma * np.eye(self.Hm.shape[0])))
+        if not np.isfinite(prop[:, 0]).all():
+            raise FloatingPointError("the Hessenberg exponential overflowed")
         return prop[:, 0], tau * sigma
 
 

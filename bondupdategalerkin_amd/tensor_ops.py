@@ -546,6 +546,9 @@ class _SvdBatch:
                 self.U.numel(), ctypes.cast(self.S.data_ptr(), _lib._f64p), self.S.numel(),
                 _ptr(self.Vh), self.Vh.numel(), 0, _stream()))
         host = self.S.cpu().numpy()
+        if not np.isfinite(host).all():
+            raise FloatingPointError("SVD of a block with non-finite entries: the state or the "
+                                     "operator is not finite")
         self.spectra = []
         for i, (m, n) in enumerate(self.mats):
             k = min(m, n)

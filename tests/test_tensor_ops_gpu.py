@@ -184,3 +184,23 @@ def test_qr_paths_on_wide_blocks(gpu, variant):
                 Qo, Ro = la._unique_qr(M)
                 assert np.abs(Qd - Qo).max() < 1e-11 and np.abs(Rd - Ro).max() < 1e-11 * np.abs(M).max(), q
     ctx.check(ctx._lib.bug_set_qr_variant(ctx.handle, 0))
+
+
+@pytest.mark.parametrize("m,n", [(40, 8), (120, 60)])
+def test_svd_refuses_non_finite_blocks(gpu, m, n):
+    """A block holding a NaN gives NaN singular values, which the host refuses; the device sort of
+    the spectrum stays a permutation, so nothing is read outside the block (the narrow block runs
+    the one-workgroup kernel, the wide one the preconditioned block Jacobi)."""
+    from bondupdategalerkin_amd import tensor_ops as T
+
+    rng = np.random.default_rng(n)
+    C = ob.Fam({0: m, 2: m}, {0: n, 2: n}, [0])
+    for q in (0, 2):
+        C.blocks[(0, q)] = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+    C.blocks[(0, 2)][3, 1] = np.nan
+    with pytest.raises(FloatingPointError):
+        T.svd_core(device_from_fam(C))
+    C.blocks[(0, 2)][3, 1] = 1.0
+    r = T.svd_core(device_from_fam(C))
+    o = la.svd_core(C)
+    assert np.allclose(r.S[2], o.S[2], rtol=1e-12)
