@@ -46,7 +46,7 @@ def parse():
     p.add_argument("--delta", type=float, default=0.5)
     p.add_argument("--order", type=int, default=2)
     p.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
-    p.add_argument("--half-sweeps", choices=["serial", "parallel"], default="serial",
+    p.add_argument("--half-sweeps", choices=["serial", "parallel"], default="parallel",
                    help="run the two half-sweeps one after the other or concurrently")
     p.add_argument("--no-fp64", action="store_true",
                    help="with --precision mixed: skip the FP64 comparison run")
@@ -58,6 +58,10 @@ def parse():
                    help="requests of the seeded Poisson serving mix to replay (0: skip)")
     p.add_argument("--serving-rate", type=float, default=1.0, help="arrivals per second")
     p.add_argument("--slots", type=int, default=4, help="concurrent requests of the serving loop")
+    p.add_argument("--heston", type=int, default=4,
+                   help="Heston instances (2^10 x 2^10 grid) stepped through the serving loop "
+                        "(0: skip)")
+    p.add_argument("--heston-steps", type=int, default=1)
     p.add_argument("--kernel-only", action="store_true",
                    help="run only the dominant-kernel timing (for profiling)")
     return p.parse_args()
@@ -65,6 +69,8 @@ def parse():
 
 def main():
     args = parse()
+    # the step's own progress lines (bonds, closure, root substeps) on stderr: long steps stay visible
+    os.environ.setdefault("BUG_PROGRESS", "1")
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}), flush=True)
@@ -110,8 +116,13 @@ def main():
     apply_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
     achieved = flops_apply / (apply_ms * 1e-3) / 1e12
     peak = PEAK[ckind]
+    # HBM bytes per apply, from the two PMC passes (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors
+    # from a plain copy of known size) recorded in profiles/heff_apply_L128_D1024_mixed.md; only
+    # for the shape and form they were measured on
+    traffic = (105081 * 1024 if (ckind == "fp32" and L == 128 and D == 1024 and args.delta == 0.5)
+               else None)
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
-                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
                 "kernel": f"grouped_zgemm {ckind} (one-site H_eff apply, 2 launches)",
                 "flops_per_apply": flops_apply, "apply_ms": round(apply_ms, 4)}
     if args.kernel_only:
@@ -196,6 +207,39 @@ def main():
                                     args.slots, form)
             print(f"# serving {form}: {srv[form]}", file=sys.stderr, flush=True)
         extra["serving"] = srv
+
+    if args.heston > 0 and args.workload == "bug_step":
+        from bondupdategalerkin_amd import serving as S
+
+        print(f"# heston batch: {args.heston} instances", file=sys.stderr, flush=True)
+        hrng = np.random.default_rng(0)
+        eng = S.Engine(slots=args.slots)
+        try:
+            t0 = time.perf_counter()
+            rids = []
+            for _ in range(args.heston):
+                prm = (hrng.uniform(0, 0.05), hrng.uniform(0, 0.05), hrng.uniform(0.5, 4),
+                       hrng.uniform(0.02, 0.15), hrng.uniform(0.1, 0.8), hrng.uniform(-0.9, 0))
+                rids.append(eng.submit(S.Request(
+                    "heston", {"bits": 10, "params": prm, "maxdim": 128,
+                               "steps": args.heston_steps, "dt": 0.02}, precision="fp64")))
+            res = eng.wait(rids, timeout=1200)
+            span = time.perf_counter() - t0
+        finally:
+            eng.close()
+        lat = np.array([r.latency for r in res])
+        extra["heston_batch"] = {
+            "workload": (f"{args.heston} Heston QTT instances, 2^10 x 2^10 grid (x in [-1,1], v in "
+                         "[0,0.5]), params drawn as the batched config (seed 0), psi0 = "
+                         f"exp(0.5x) exp(-v), order 2, maxdim 128, tau 0.02, hermitian=false, "
+                         f"{args.heston_steps} step(s) each, FP64, {args.slots} concurrent slots"),
+            "instance_steps_per_s": round(sum(r.steps for r in res) / span, 4),
+            "makespan_s": round(span, 2),
+            "p50_latency_s": round(float(np.percentile(lat, 50)), 2),
+            "errors": [r.error for r in res if r.error],
+            "budget_status": [s for r in res for s in r.statuses],
+        }
+        print(f"# heston batch: {extra['heston_batch']}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:

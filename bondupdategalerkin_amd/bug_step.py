@@ -936,12 +936,20 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         napply[0] += 1
         return ev.apply_zero_site(Lr, Rr, C, precision=cp)
 
+    _progress("root solve")
+
+    def substep(n):
+        if n % 50 == 0:
+            _progress(f"root solve substep {n}")
+
     if hermitian:
         result, rep = K.lanczos_expv(apply_root, tau, root_start, tol=1e-15,
-                                     maxiter=K.ROOT_MAXITER, reorth=reorth, conv_tol=te, substeps=0)
+                                     maxiter=K.ROOT_MAXITER, reorth=reorth, conv_tol=te, substeps=0,
+                                     on_substep=substep)
     else:
         result, rep = K.arnoldi_expv(apply_root, tau, root_start, tol=1e-15,
-                                     maxiter=K.ROOT_MAXITER, conv_tol=te, substeps=0)
+                                     maxiter=K.ROOT_MAXITER, conv_tol=te, substeps=0,
+                                     on_substep=substep)
     ledger.expv_err, ledger.expv_iters = rep.err, rep.iterations
     if rep.err > te * (1 + 1e-8):
         ledger.fail("expv")

@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass
-from typing import Callable, List
+from typing import Callable, List, Optional
 
 import numpy as np
 import scipy.linalg as sla
@@ -156,7 +156,8 @@ class ExpvReport:
 
 
 def lanczos_expv(apply, tau: complex, v0, *, tol: float = 1e-15, maxiter: int = ROOT_MAXITER,
-                 reorth: bool = True, conv_tol: float = 0.0, substeps: int = 0):
+                 reorth: bool = True, conv_tol: float = 0.0, substeps: int = 0,
+                 on_substep: Optional[Callable[[int], None]] = None):
     """exp(tau A) v0 for a Hermitian device action ``apply`` (``lanczos_expv``)."""
     beta0 = T.norm(v0)
     if beta0 == 0:
@@ -166,6 +167,8 @@ def lanczos_expv(apply, tau: complex, v0, *, tol: float = 1e-15, maxiter: int = 
     err = 0.0
     while True:
         nsub += 1
+        if on_substep is not None:
+            on_substep(nsub)
         cn = T.norm(cur)
         if cn == 0:
             break
@@ -225,7 +228,8 @@ def lanczos_expv(apply, tau: complex, v0, *, tol: float = 1e-15, maxiter: int = 
 
 
 def arnoldi_expv(apply, tau: complex, v0, *, tol: float = 1e-13, maxiter: int = ROOT_MAXITER,
-                 conv_tol: float = 0.0, substeps: int = 0):
+                 conv_tol: float = 0.0, substeps: int = 0,
+                 on_substep: Optional[Callable[[int], None]] = None):
     """exp(tau A) v0 for a non-Hermitian device action ``apply`` (``arnoldi_expv``,
     src/BUG/Core/LocalSolvers/krylov.jl:343-442): upper-Hessenberg Arnoldi with classical
     Gram-Schmidt applied twice per vector (each pass one batched inner-product launch against the
@@ -238,6 +242,8 @@ def arnoldi_expv(apply, tau: complex, v0, *, tol: float = 1e-13, maxiter: int = 
     err = 0.0
     while True:
         nsub += 1
+        if on_substep is not None:
+            on_substep(nsub)
         cn = T.norm(cur)
         if cn == 0:
             break
