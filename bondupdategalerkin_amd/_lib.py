@@ -138,6 +138,11 @@ SIGNATURES = {
         [_P, ctypes.c_int32, ctypes.POINTER(BugQrJob), _P, ctypes.c_int64, _P, ctypes.c_int64, _P,
          ctypes.c_int64, _P],
     ),
+    "bug_block_qrp": (
+        ctypes.c_int,
+        [_P, ctypes.c_int32, ctypes.POINTER(BugQrJob), _P, ctypes.c_int64, _P, ctypes.c_int64,
+         _f64p, ctypes.c_int64, _P],
+    ),
     "bug_set_svd_variant": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     "bug_set_qr_variant": (ctypes.c_int, [_P, ctypes.c_int32]),
     "bug_set_jacobi_params": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),

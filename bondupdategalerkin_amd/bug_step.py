@@ -62,12 +62,15 @@ class DDQRExpansion:
     fixed_width: int = 8
     tail_factor: float = 0.10
     krylov_mode: str = "auto"
+    holdout_width: int = 4
 
     def __post_init__(self):
-        if self.mode != "rank_tail":
-            raise ValueError(f"DD mode {self.mode!r} is not implemented (rank_tail only)")
-        if self.fixed_width < 1:
+        if self.mode not in ("rank_tail", "adaptive"):
+            raise ValueError(f"DD mode {self.mode!r} is not implemented (rank_tail, adaptive)")
+        if self.mode == "rank_tail" and self.fixed_width < 1:
             raise ValueError("rank_tail requires positive fixed_width")
+        if self.holdout_width < 1:
+            raise ValueError("holdout_width must be positive")
         if not 0 < self.tail_factor <= 1:
             raise ValueError("tail_factor must lie in (0, 1]")
         if self.krylov_mode not in ("auto", "lanczos", "arnoldi"):
@@ -90,6 +93,8 @@ class AdaptiveLedger:
     n_selected: int = 0
     n_krylov: int = 0
     n_h1: int = 0
+    n_holdout: int = 0
+    n_targeted: int = 0
     workcap_reached: bool = False
     tolerance_met: bool = True
     maxdim_reached: bool = False
@@ -121,6 +126,8 @@ class AdaptiveLedger:
         self.n_selected += other.n_selected
         self.n_krylov += other.n_krylov
         self.n_h1 += other.n_h1
+        self.n_holdout += other.n_holdout
+        self.n_targeted += other.n_targeted
         self.n_h1_closure += other.n_h1_closure
         self.n_closure_sweeps += other.n_closure_sweeps
         self.workcap_reached |= other.workcap_reached
@@ -312,16 +319,24 @@ def _alloc_columns(dims: Dict[int, int], width: int) -> Dict[int, int]:
     return {q: min(d, math.ceil(d / tot * width)) for q, d in dims.items()}
 
 
-def dd_probe(frame: BlockTensor, side: str, width: int, stream: ProbeStream) -> Optional[BlockTensor]:
+def dd_probe(frame: BlockTensor, side: str, width: int, stream: ProbeStream,
+             boost: Optional[Dict[int, int]] = None) -> Optional[BlockTensor]:
     """Sector-graded Gaussian probe in the discarded space of ``frame`` (_dd_probe, dd_cbe.jl:132-161):
-    columns per fused charge by its discarded dimension, the frame projected out afterwards."""
+    columns per fused charge by its discarded dimension (plus a sector ``boost``, ``_boost!``,
+    cbe.jl:206-216, capped by the discarded dimension), the frame projected out afterwards."""
     lay = frame.layout
     dfull = _fused_dims(lay, side)
     dT = _widths(lay, side)
     dC = {q: max(d - dT.get(q, 0), 0) for q, d in dfull.items()}
     if sum(dC.values()) == 0:
         return None
-    Om = _graded_draw(frame, side, dfull, _alloc_columns(dC, width), stream)
+    cols = _alloc_columns(dC, width)
+    if boost:
+        for q, extra in boost.items():
+            if q in dC:
+                have = cols.get(q, 0)
+                cols[q] = have + min(max(extra, 0), max(dC[q] - have, 0))
+    Om = _graded_draw(frame, side, dfull, cols, stream)
     proj = T.perp_left(frame, Om) if side == "left" else T.perp_right(frame, Om)
     return proj if T.norm(proj) > 0 else None
 
@@ -487,13 +502,20 @@ def _oplus(frame, new, side):
 
 def sample_residual(basis, project, Y: BlockTensor, Om: BlockTensor, side: str) -> float:
     """Test B's per-width-sector normalised residual (``_sample_residual``, sketch.jl:141-180)."""
+    return sample_residual_full(basis, project, Y, Om, side)[0]
+
+
+def sample_residual_full(basis, project, Y: BlockTensor, Om: BlockTensor, side: str):
+    """``_sample_residual`` -> (estimate, {width sector: squared residual / its columns},
+    {width sector: holdout columns})."""
     cols = Om.layout.row.as_dict() if side == "left" else Om.layout.col.as_dict()
     cols = {q: n for q, n in cols.items() if n > 0}
+    per: Dict[int, float] = {}
     if not cols:
-        return 0.0
+        return 0.0, per, cols
     res = project(Y)
     if T.norm(res) <= CBE_EPS:
-        return 0.0
+        return 0.0, per, cols
     if basis is not None:
         res = project(_perp(side)(basis, res))
     blks, keys = [], []
@@ -501,15 +523,145 @@ def sample_residual(basis, project, Y: BlockTensor, Om: BlockTensor, side: str) 
         blks.append((off, d, d, dc))
         keys.append(qc if side == "left" else q)
     sq = T.block_sqnorms(blks, res.data)
-    per: Dict[int, float] = {}
+    acc: Dict[int, float] = {}
     for w, v in zip(keys, sq):
-        per[w] = per.get(w, 0.0) + float(v)
-    tot = 0.0
-    for w, v in per.items():
+        acc[w] = acc.get(w, 0.0) + float(v)
+    for w, v in acc.items():
         if cols.get(w, 0) <= 0:
             raise RuntimeError("Test B: residual sector without a holdout column")
-        tot += v / cols[w]
-    return math.sqrt(tot)
+        per[w] = v / cols[w]
+    return math.sqrt(sum(per.values())), per, cols
+
+
+REORTH_KEEP = 1.0 / math.sqrt(2.0)      # _REORTH_KEEP, dd_cbe.jl:273
+ADAPTIVE_GROWTH = 1.2                  # basis_budget.jl:565
+
+
+def dd_select(frame: BlockTensor, sketch: BlockTensor, side: str) -> Optional[BlockTensor]:
+    """``_dd_select`` (dd_cbe.jl:287-321): projected = P^D sketch, its pivoted thin QR at the
+    floor 100 eps max(|sketch|, |projected|), then the thin QR of P^D(first pass) keeping the
+    pivots >= 1/sqrt 2 ("twice is enough")."""
+    perp = _perp(side)
+    projected = perp(frame, sketch)
+    nz, ny = T.norm(sketch), T.norm(projected)
+    if ny <= CBE_EPS * max(nz, 1.0):
+        return None
+    first = T.thin_qr(projected, side, CBE_EPS * max(nz, ny))
+    if first is None:
+        return None
+    return T.thin_qr(perp(frame, first), side, REORTH_KEEP)
+
+
+def target_sectors(boost: Dict[int, int], per: Dict[int, float], Om, side: str, alloc) -> int:
+    """``_target_sectors!`` (sketch.jl:188-201): a sector whose scaled residual exceeds its equal
+    share of the sample allowance gets as many extra probe columns as the holdout gave it."""
+    if not per:
+        return 0
+    have = {} if Om is None else (Om.layout.row.as_dict() if side == "left"
+                                  else Om.layout.col.as_dict())
+    share = alloc.sample ** 2 / len(per)
+    n = 0
+    for q, r2 in per.items():
+        if alloc.scale ** 2 * r2 > share:
+            boost[q] = boost.get(q, 0) + max(have.get(q, 0), 1)
+            n += 1
+    return n
+
+
+def dd_controller(frame: BlockTensor, side: str, draw, sketch, project, width0: int, wcap: int,
+                  ncap: int, alloc, ledger: "AdaptiveLedger", holdout_width: int = 4,
+                  acceptance_factor: float = 1.0):
+    """``_dd_controller`` (dd_cbe_controller.jl:26-212; ``batch_blocks = 1``, storage-order cap):
+    grow the DD basis block by block until an independent holdout finds the rest within the
+    side's share.  Returns (basis or None, status)."""
+    st = {"aug": frame, "dd": None, "count": 0, "discovered": 0}
+    boost: Dict[int, int] = {}
+
+    def width_of(t):
+        return 0 if t is None else _width(t, side)
+
+    def probe_width(Om):
+        return Om.layout.row.total if side == "left" else Om.layout.col.total
+
+    def absorb(sketched):
+        new = dd_select(st["aug"], sketched, side)
+        if new is None:
+            return 0
+        room = ncap - width_of(st["dd"])
+        if room <= 0:
+            return 0
+        if width_of(new) > room:
+            new = trim_total(new, side, room)
+            if new is None:
+                return 0
+        st["dd"] = new if st["dd"] is None else _oplus(st["dd"], new, side)
+        st["aug"] = _oplus(st["aug"], new, side)
+        return width_of(new)
+
+    def stream(Om, holdout):
+        w = probe_width(Om)
+        if holdout:
+            ledger.n_holdout += w
+        else:
+            st["discovered"] += w
+            ledger.n_probes += w
+        return sketch(Om)
+
+    def fresh(columns):
+        Om = draw(columns, None)
+        return None if Om is None else (Om, stream(Om, True))
+
+    def newsector():
+        if st["dd"] is None:
+            return False
+        fw = _widths(frame.layout, side)
+        return any(fw.get(q, 0) == 0 for q, n in _widths(st["dd"].layout, side).items() if n)
+
+    width = min(max(width0, 1), max(min(wcap, ncap), 1))
+    last = draw(width, None)
+    if last is None:
+        return None, "empty"
+    st["count"] += absorb(stream(last, False))
+    check = fresh(holdout_width)
+    escalated = False
+    while True:
+        if check is None:
+            return st["dd"], "met"
+        est, per, _ = sample_residual_full(st["dd"], project, check[1], check[0], side)
+        ind = alloc.scale * est
+        ok = ind <= acceptance_factor * alloc.sample
+        if ok and not escalated and (ind > alloc.sample / 2 or newsector()):
+            escalated = True
+            wide = fresh(2 * holdout_width)
+            if wide is not None:
+                check = wide
+                continue
+        capped = st["discovered"] >= wcap or st["count"] >= ncap
+        if ok or capped:
+            ledger.max_sample = max(ledger.max_sample, ind)
+            if ok:
+                return st["dd"], "met"
+            ledger.fail("sample")
+            ledger.workcap_reached = True
+            return st["dd"], "work_limited"
+        nt = target_sectors(boost, per, check[0], side, alloc)
+        ledger.n_targeted += nt
+        next_check = fresh(holdout_width)
+        added = absorb(check[1])
+        if nt == 0:
+            width = min(2 * width, wcap)
+        remaining = ncap - width_of(st["dd"])
+        if remaining > 0:
+            blk = draw(min(width, max(remaining + 2, 1)), boost if boost else None)
+            if blk is not None:
+                last = blk
+                added += absorb(stream(blk, False))
+        check = next_check
+        st["count"] += added
+        if added == 0:
+            ledger.max_sample = max(ledger.max_sample, ind)
+            ledger.fail("sample")
+            return st["dd"], "basis_limited"
 
 
 @dataclass
@@ -517,6 +669,45 @@ class _Expansion:
     U_ex: BlockTensor
     V_ex: BlockTensor
     n_new: int
+
+
+def _dd_cbe_adaptive(fr: BondFrame, frame, opp, side: str, oside: str, lenv, renv, W1, W2,
+                     alloc: _Alloc, ledger: "AdaptiveLedger", cap: int, wbind: bool, full_opp: int,
+                     workcap: int, ex: "DDQRExpansion", stream: ProbeStream,
+                     prec: int) -> "_Expansion":
+    """dd_cbe_expand's ``:adaptive`` branch (dd_cbe_expand.jl:88-118,362-433)."""
+    left = side == "left"
+    none = _Expansion(fr.U0, fr.V0, 0)
+    dmax = max(fr.U0.layout.row.total, fr.old_rank, fr.V0.layout.col.total)
+    width0 = math.ceil(1.2 * max(math.ceil(ADAPTIVE_GROWTH * dmax) - fr.old_rank, 1))
+    wcap = min(full_opp, workcap) if workcap > 0 else full_opp
+    if left:
+        A = T.site_times_core(fr.U0, fr.S0)
+        sketch = lambda Om: stream_left(lenv, W1, W2, renv, A, fr.V0, Om, prec)     # noqa: E731
+        project = lambda t: T.perp_left(fr.U0, t)                                  # noqa: E731
+    else:
+        Bv = T.core_times_site(fr.S0, fr.V0)
+        sketch = lambda Om: stream_right(lenv, W1, W2, renv, fr.U0, Bv, Om, prec)  # noqa: E731
+        project = lambda t: T.perp_right(fr.V0, t)                                 # noqa: E731
+    directions, _status = dd_controller(
+        frame, side, lambda w, b: dd_probe(opp, oside, w, stream, b), sketch, project, width0,
+        wcap, cap, alloc, ledger, holdout_width=ex.holdout_width)
+    if directions is None:
+        return none
+    count = _width(directions, side)
+    if count > cap:
+        directions = trim_total(directions, side, cap)
+        if directions is None:
+            return none
+    if wbind and count >= cap:
+        ledger.workcap_reached = True
+    directions = cap_to_local_room(frame, directions, side)
+    if directions is None:
+        return none
+    nq = _width(directions, side)
+    ledger.n_selected += nq
+    grown = _oplus(frame, directions, side)
+    return _Expansion(grown, fr.V0, nq) if left else _Expansion(fr.U0, grown, nq)
 
 
 def dd_cbe_expand(fr: BondFrame, mpo: MPO, bond: int, lenv, renv, side: str, alloc: _Alloc,
@@ -538,6 +729,11 @@ def dd_cbe_expand(fr: BondFrame, mpo: MPO, bond: int, lenv, renv, side: str, all
     if cap <= 0:
         return none
     W1, W2 = mpo[bond], mpo[bond + 1]
+    if ex.mode == "adaptive":
+        if probe_hook is not None:
+            raise ValueError("probe_hook observes the rank_tail probes only")
+        return _dd_cbe_adaptive(fr, frame, opp, side, oside, lenv, renv, W1, W2, alloc, ledger,
+                                cap, wbind, full_opp, workcap, ex, stream, prec)
     width = min(ex.fixed_width, full_opp)
     Om = probe_hook(opp, oside, width) if probe_hook else dd_probe(opp, oside, width, stream)
     if Om is None:

@@ -332,6 +332,155 @@ __global__ __launch_bounds__(kQrThreads) void qr_kernel(const MatJob* __restrict
     }
 }
 
+// Column-pivoted Householder QR of A (m x n, lda), overwritten: at step j the pivot is the
+// column of largest remaining norm over rows j..m-1, the first of equals (LAPACK geqp3's greedy
+// rule; the norms are recomputed at every step instead of downdated).  Outputs Q (m x k, ldq,
+// k = min(m, n)) with the R diagonal made real non-negative and rdiag[r_off + j] = R_jj.
+__global__ __launch_bounds__(kQrThreads) void qrp_kernel(const MatJob* __restrict__ jobs,
+                                                           double2* __restrict__ A,
+                                                           double2* __restrict__ Q,
+                                                           double* __restrict__ rdiag,
+                                                           double2* __restrict__ tau_ws) {
+    extern __shared__ double cn[];            // squared norms of the remaining columns (n)
+    const MatJob job = jobs[blockIdx.x];
+    const int m = job.m, n = job.n, k = min(m, n);
+    double2* a = A + job.a_off;
+    double2* q = Q + job.q_off;
+    double* rd = rdiag + job.r_off;
+    double2* tau = tau_ws + job.w_off;
+    const int64_t lda = job.lda, ldq = job.ldq;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int nw = kQrThreads / 64;
+    __shared__ double2 s_tau;
+    __shared__ int s_piv;
+
+    for (int j = 0; j < k; ++j) {
+        for (int c = j + wave; c < n; c += nw) {
+            const double2* ac = a + (int64_t)c * lda;
+            double s2 = 0.0;
+            for (int i = j + lane; i < m; i += 64) s2 += ac[i].x * ac[i].x + ac[i].y * ac[i].y;
+            s2 = wave_sum(s2);
+            if (lane == 0) cn[c] = s2;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int p = j;
+            double best = cn[j];
+            for (int c = j + 1; c < n; ++c)
+                if (cn[c] > best) {
+                    best = cn[c];
+                    p = c;
+                }
+            s_piv = p;
+        }
+        __syncthreads();
+        const int p = s_piv;
+        if (p != j) {
+            double2* aj = a + (int64_t)j * lda;
+            double2* ap = a + (int64_t)p * lda;
+            for (int i = threadIdx.x; i < m; i += kQrThreads) {
+                const double2 t = aj[i];
+                aj[i] = ap[i];
+                ap[i] = t;
+            }
+        }
+        __syncthreads();
+        double2* col = a + (int64_t)j * lda;
+        if (wave == 0) {
+            double x2 = 0.0;
+            for (int i = j + 1 + lane; i < m; i += 64) x2 += col[i].x * col[i].x + col[i].y * col[i].y;
+            x2 = wave_sum(x2);
+            const double2 alpha = col[j];
+            double2 t{0.0, 0.0};
+            if (!(x2 == 0.0 && alpha.y == 0.0)) {
+                const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + x2);
+                const double beta = alpha.x >= 0.0 ? -nrm : nrm;
+                t = double2{(beta - alpha.x) / beta, -alpha.y / beta};
+                const double dr = alpha.x - beta, di = alpha.y;
+                const double den = dr * dr + di * di;
+                const double sr = dr / den, si = -di / den;
+                for (int i = j + 1 + lane; i < m; i += 64) {
+                    const double2 v = col[i];
+                    col[i] = double2{v.x * sr - v.y * si, v.x * si + v.y * sr};
+                }
+                if (lane == 0) col[j] = double2{beta, 0.0};
+            }
+            if (lane == 0) {
+                s_tau = t;
+                tau[j] = t;
+            }
+        }
+        __syncthreads();
+        const double2 t = s_tau;
+        if (t.x != 0.0 || t.y != 0.0) {
+            for (int c = j + 1 + wave; c < n; c += nw) {
+                double2* ac = a + (int64_t)c * lda;
+                double wr = 0.0, wi = 0.0;
+                for (int i = j + lane; i < m; i += 64) {
+                    const double2 v = (i == j) ? double2{1.0, 0.0} : col[i];
+                    const double2 x = ac[i];
+                    wr += v.x * x.x + v.y * x.y;
+                    wi += v.x * x.y - v.y * x.x;
+                }
+                wr = wave_sum(wr);
+                wi = wave_sum(wi);
+                const double fr = t.x * wr + t.y * wi;
+                const double fi = t.x * wi - t.y * wr;
+                for (int i = j + lane; i < m; i += 64) {
+                    const double2 v = (i == j) ? double2{1.0, 0.0} : col[i];
+                    double2 x = ac[i];
+                    x.x -= fr * v.x - fi * v.y;
+                    x.y -= fr * v.y + fi * v.x;
+                    ac[i] = x;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int64_t e = threadIdx.x; e < (int64_t)m * k; e += kQrThreads) {
+        const int r = (int)(e % m), c = (int)(e / m);
+        q[r + c * ldq] = double2{r == c ? 1.0 : 0.0, 0.0};
+    }
+    __syncthreads();
+    for (int j = k - 1; j >= 0; --j) {
+        const double2 t = tau[j];
+        if (t.x == 0.0 && t.y == 0.0) continue;
+        const double2* col = a + (int64_t)j * lda;
+        for (int c = j + wave; c < k; c += nw) {
+            double2* qc = q + (int64_t)c * ldq;
+            double wr = 0.0, wi = 0.0;
+            for (int i = j + lane; i < m; i += 64) {
+                const double2 v = (i == j) ? double2{1.0, 0.0} : col[i];
+                const double2 x = qc[i];
+                wr += v.x * x.x + v.y * x.y;
+                wi += v.x * x.y - v.y * x.x;
+            }
+            wr = wave_sum(wr);
+            wi = wave_sum(wi);
+            const double fr = t.x * wr - t.y * wi;
+            const double fi = t.x * wi + t.y * wr;
+            for (int i = j + lane; i < m; i += 64) {
+                const double2 v = (i == j) ? double2{1.0, 0.0} : col[i];
+                double2 x = qc[i];
+                x.x -= fr * v.x - fi * v.y;
+                x.y -= fr * v.y + fi * v.x;
+                qc[i] = x;
+            }
+        }
+        __syncthreads();
+    }
+    // positive-diagonal gauge on Q, and the diagonal's magnitude
+    for (int j = wave; j < k; j += nw) {
+        const double d = a[j + (int64_t)j * lda].x;
+        if (d < 0.0)
+            for (int i = lane; i < m; i += 64) {
+                const double2 v = q[i + (int64_t)j * ldq];
+                q[i + (int64_t)j * ldq] = double2{-v.x, -v.y};
+            }
+        if (lane == 0) rd[j] = fabs(d);
+    }
+}
+
 // ---- blocked Householder QR (panels + grouped-GEMM updates) -------------------------------
 //
 // Panels of PW columns (16, or 8 for blocks of more than 512 rows so that a lane's RPT rows stay in
@@ -1810,6 +1959,42 @@ int bug_block_qr(bug_ctx* ctx, int32_t nmat, const bug_qr_job* jobs, void* a, in
         hipLaunchKernelGGL(qr_kernel, dim3(nmat), dim3(kQrThreads), 0, st, d,
                            static_cast<double2*>(a), static_cast<double2*>(q),
                            static_cast<double2*>(r), tau);
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int bug_block_qrp(bug_ctx* ctx, int32_t nmat, const bug_qr_job* jobs, void* a, int64_t a_nelem,
+                  void* q, int64_t q_nelem, double* rdiag, int64_t rdiag_nelem, void* stream) {
+    return guarded(ctx, [&] {
+        if (nmat < 0 || (nmat > 0 && (!jobs || !a || !q || !rdiag)))
+            throw Error(BUG_EINVAL, "bad pivoted qr jobs");
+        if (nmat == 0) return;
+        std::vector<MatJob> list(nmat);
+        int64_t wtot = 0;
+        int nmax = 1;
+        for (int i = 0; i < nmat; ++i) {
+            const bug_qr_job& j = jobs[i];
+            const int k = std::min(j.m, j.n);
+            if (j.m < 0 || j.n < 0 || j.lda < std::max(1, j.m) || j.ldq < std::max(1, j.m))
+                throw Error(BUG_EINVAL, "bad pivoted qr job dims");
+            if (j.n > 16384) throw Error(BUG_EINVAL, "pivoted qr block wider than 16384");
+            if (j.m > 0 && j.n > 0 &&
+                (j.a_off + (int64_t)(j.n - 1) * j.lda + j.m > a_nelem ||
+                 j.q_off + (int64_t)(k - 1) * j.ldq + j.m > q_nelem || j.r_off + k > rdiag_nelem))
+                throw Error(BUG_EINVARIANT, "pivoted qr job outside its buffers");
+            list[i] = MatJob{j.a_off, j.q_off, j.r_off, wtot, j.m, j.n, j.lda, j.ldq, 1};
+            wtot += k;
+            nmax = std::max(nmax, j.n);
+        }
+        hipStream_t st = as_stream(stream);
+        MatJob* d = upload_list(ctx, st, list.data(), list.size(), 2);
+        double2* tau = static_cast<double2*>(scratch(ctx, st, sizeof(double2) * std::max<int64_t>(wtot, 1), 1));
+        const size_t lds = sizeof(double) * (size_t)nmax;
+        if (lds > 65536)
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(qrp_kernel),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(qrp_kernel, dim3(nmat), dim3(kQrThreads), lds, st, d,
+                           static_cast<double2*>(a), static_cast<double2*>(q), rdiag, tau);
         HIP_CHECK(hipGetLastError());
     });
 }

@@ -446,6 +446,107 @@ def qr_right(A: BlockTensor) -> Tuple[BlockTensor, BlockTensor]:
     return Lc, Q
 
 
+def _run_qrp(mats: List[Tuple[int, int]], ws: torch.Tensor, offs: List[int]):
+    """Pivoted QR of the listed (m, n) matrices at offs in ws -> (Qbuf, qoffs, ks, |R| diagonals
+    on the host)."""
+    qoffs, doffs, ks = [], [], []
+    qp = dp = 0
+    for (m, n) in mats:
+        k = min(m, n)
+        qoffs.append(qp)
+        doffs.append(dp)
+        ks.append(k)
+        qp += m * k
+        dp += k
+    Qb = empty_buffer(qp)
+    rd = torch.zeros(max(dp, 1), dtype=torch.float64, device=device())
+    if mats:
+        arr = (_lib.BugQrJob * len(mats))()
+        for i, (m, n) in enumerate(mats):
+            arr[i] = _lib.BugQrJob(offs[i], qoffs[i], doffs[i], m, n, max(m, 1), max(m, 1), 1, 0)
+        ctx = _ctx()
+        ctx.check(ctx._lib.bug_block_qrp(ctx.handle, len(mats), arr, _ptr(ws), ws.numel(),
+                                         _ptr(Qb), Qb.numel(), ctypes.cast(rd.data_ptr(), _lib._f64p),
+                                         rd.numel(), _stream()))
+    host = rd.cpu().numpy()
+    return Qb, qoffs, ks, [host[doffs[i]: doffs[i] + ks[i]] for i in range(len(mats))]
+
+
+def thin_qr(A: BlockTensor, side: str, thr: float) -> Optional[BlockTensor]:
+    """Orthonormal basis of the span of the new-leg columns of ``A`` (``_thin_qr``,
+    src/BUG/Randomised/dd_cbe.jl:184-259): per sector of that leg, the column-pivoted QR of the
+    stacked blocks (left: the left matricisation; right: the conjugate transpose of the right
+    one), the first columns kept as many as pivots exceed ``thr``.  None when nothing is kept."""
+    lay = A.layout
+    gather, offs, mats, pos = CopyList(), [], [], 0
+    if side == "left":
+        secs = left_sectors(lay)
+        cold = lay.col.as_dict()
+        qlist = list(secs.keys())
+        for qc in qlist:
+            parts = secs[qc]
+            m, n = sum(p[2] for p in parts), cold[qc]
+            r0 = 0
+            for s, ql, rows, off in parts:
+                gather.add(off, rows, pos + r0, m, rows, n)
+                r0 += rows
+            offs.append(pos)
+            mats.append((m, n))
+            pos += m * n
+    else:
+        secs = right_sectors(lay)
+        rowd = lay.row.as_dict()
+        qlist = list(secs.keys())
+        for ql in qlist:
+            parts = secs[ql]
+            m, n = sum(p[2] for p in parts), rowd[ql]
+            c0 = 0
+            for s, qr, cols, off in parts:
+                gather.add(off, n, pos + c0, m, cols, n, mode=2)
+                c0 += cols
+            offs.append(pos)
+            mats.append((m, n))
+            pos += m * n
+    if not mats:
+        return None
+    ws = empty_buffer(pos)
+    gather.run(A.data, ws)
+    Qb, qo, ks, rdiag = _run_qrp(mats, ws, offs)
+    kept = [int(np.sum(r > thr)) for r in rdiag]
+    if not any(kept):
+        return None
+    bond = Leg.make((qlist[i], kept[i]) for i in range(len(mats)) if kept[i] > 0)
+    if side == "left":
+        out = zeros(Layout(lay.row, bond, lay.shifts))
+        tab = _block_table(out.layout)
+        cq = CopyList()
+        for i, qc in enumerate(qlist):
+            if kept[i] == 0:
+                continue
+            m = mats[i][0]
+            r0 = 0
+            for s, ql, rows, off in secs[qc]:
+                o = tab[(s, ql)]
+                cq.add(qo[i] + r0, m, o[0], o[1], rows, kept[i])
+                r0 += rows
+        cq.run(Qb, out.data)
+        return out
+    out = zeros(Layout(bond, lay.col, lay.shifts))
+    tab = _block_table(out.layout)
+    cq = CopyList()
+    for i, ql in enumerate(qlist):
+        if kept[i] == 0:
+            continue
+        m = mats[i][0]
+        c0 = 0
+        for s, qr, cols, off in secs[ql]:
+            o = tab[(s, ql)]
+            cq.add(qo[i] + c0, m, o[0], o[1], kept[i], cols, mode=2)
+            c0 += cols
+    cq.run(Qb, out.data)
+    return out
+
+
 def qr_core(C: BlockTensor) -> Tuple[BlockTensor, BlockTensor]:
     """C = Q R per sector of a one-slice core."""
     lay = C.layout

@@ -218,6 +218,13 @@ int bug_block_svd(bug_ctx* ctx, int32_t nmat, const bug_svd_job* jobs, void* g, 
                   void* u, int64_t u_nelem, double* s, int64_t s_nelem, void* vh, int64_t vh_nelem,
                   int32_t max_sweeps, void* stream);
 
+/* Column-pivoted thin QR of every job (the per-sector `qr(stacked, ColumnNorm())` of _thin_qr,
+ * src/BUG/Randomised/dd_cbe.jl:184-259, LAPACK geqp3's greedy pivot rule with norms recomputed at
+ * every step).  a is overwritten; q receives Q (m x k, ldq, k = min(m, n)) with the R diagonal
+ * real non-negative; rdiag[r_off .. r_off + k) receives that diagonal.  ldr is ignored. */
+int bug_block_qrp(bug_ctx* ctx, int32_t nmat, const bug_qr_job* jobs, void* a, int64_t a_nelem,
+                  void* q, int64_t q_nelem, double* rdiag, int64_t rdiag_nelem, void* stream);
+
 /* Diagnostics: force the SVD kernel (0 automatic, 1 one-sided Jacobi with one workgroup per
  * matrix, 2 QR-preconditioned one-workgroup Jacobi, 3 QR-preconditioned multi-workgroup block
  * Jacobi with the block pair staged in LDS, 4 QR-preconditioned Gram block Jacobi with the pair's

@@ -152,8 +152,9 @@ class StepOptions:
     root: Optional[int] = None          # 0-based bond of the root (reference root - 1)
     reorth: bool = True
     truncate: bool = True
-    mode: str = "rank_tail"
+    mode: str = "rank_tail"             # "rank_tail" (default) or "adaptive"
     fixed_width: int = 8
+    holdout_width: int = 4
     tail_factor: float = 0.10
     krylov_mode: str = "auto"
     seed: int = 0x5EED
@@ -183,6 +184,8 @@ class Ledger:
     n_selected: int = 0
     n_krylov: int = 0
     n_h1: int = 0
+    n_holdout: int = 0
+    n_targeted: int = 0
     workcap_reached: bool = False
     tolerance_met: bool = True
     maxdim_reached: bool = False
@@ -545,6 +548,11 @@ def dd_cbe_expand(fr: BondFrame, mpo, bond: int, lenv: Fam, renv: Fam, side: str
     if cap <= 0:
         return none
     W1, W2 = mpo[bond], mpo[bond + 1]
+    if opts.mode == "adaptive":
+        return _dd_cbe_adaptive(fr, frame, opp, side, oside, lenv, renv, W1, W2, alloc, ledger,
+                                cap, wbind, full_opp, workcap, opts, rng)
+    if opts.mode != "rank_tail":
+        raise ValueError(f"unknown DD mode {opts.mode!r}")
     width = min(opts.fixed_width, full_opp)
     Om = probe_hook(opp, oside, width) if probe_hook is not None else dd_probe(opp, oside, width, rng)
     if Om is None:
@@ -603,6 +611,51 @@ def dd_cbe_expand(fr: BondFrame, mpo, bond: int, lenv: Fam, renv: Fam, side: str
         count = 0 if directions is None else _width_of(directions, side)
     if directions is None:
         return none
+    if count > cap:
+        directions = trim_total(directions, side, cap)
+        if directions is None:
+            return none
+    if wbind and count >= cap:
+        ledger.workcap_reached = True
+    directions = cap_to_local_room(frame, directions, side)
+    if directions is None:
+        return none
+    nq = _width_of(directions, side)
+    ledger.n_selected += nq
+    grown = oplus(frame, directions, side)
+    return Expansion(grown, fr.V0, nq) if left else Expansion(fr.U0, grown, nq)
+
+
+def _dd_cbe_adaptive(fr: BondFrame, frame: Fam, opp: Fam, side: str, oside: str, lenv, renv, W1,
+                     W2, alloc: Alloc, ledger: Ledger, cap: int, wbind: bool, full_opp: int,
+                     workcap: int, opts: StepOptions, rng) -> Expansion:
+    """dd_cbe_expand's ``:adaptive`` branch (dd_cbe_expand.jl:88-118,362-433): the
+    holdout-certified controller (oracle/dd_adaptive.py) picks the new directions."""
+    from . import dd_adaptive as da
+
+    left = side == "left"
+    none = Expansion(fr.U0, fr.V0, 0)
+    width0 = da.initial_width(fr.U0, fr.V0, fr.old_rank)
+    wcap = full_opp
+    if workcap > 0:
+        wcap = min(wcap, workcap)
+    if left:
+        A = la.site_times_core(fr.U0, fr.S0)
+        sketch = lambda Om: stream_left(lenv, W1, W2, renv, A, fr.V0, Om)       # noqa: E731
+        project = lambda T: la.perp_left(fr.U0, T)                              # noqa: E731
+    else:
+        Bv = la.core_times_site(fr.S0, fr.V0)
+        sketch = lambda Om: stream_right(lenv, W1, W2, renv, fr.U0, Bv, Om)     # noqa: E731
+        project = lambda T: la.perp_right(fr.V0, T)                             # noqa: E731
+
+    def draw(width, boost):
+        return da.dd_probe(opp, oside, width, rng, boost)
+
+    directions, _status = da.dd_controller(frame, side, draw, sketch, project, width0, wcap, cap,
+                                           alloc, ledger, holdout_width=opts.holdout_width)
+    if directions is None:
+        return none
+    count = _width_of(directions, side)
     if count > cap:
         directions = trim_total(directions, side, cap)
         if directions is None:

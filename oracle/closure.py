@@ -160,6 +160,8 @@ def _merge(into, other):
     into.n_selected += other.n_selected
     into.n_krylov += other.n_krylov
     into.n_h1 += other.n_h1
+    into.n_holdout += other.n_holdout
+    into.n_targeted += other.n_targeted
     into.workcap_reached |= other.workcap_reached
     into.tolerance_met &= other.tolerance_met
     for u in other.unmet:

@@ -204,3 +204,35 @@ def test_svd_refuses_non_finite_blocks(gpu, m, n):
     r = T.svd_core(device_from_fam(C))
     o = la.svd_core(C)
     assert np.allclose(r.S[2], o.S[2], rtol=1e-12)
+
+
+@pytest.mark.parametrize("side", ["left", "right"])
+@pytest.mark.parametrize("seed,dmax,rank", [(0, 6, None), (1, 20, 3), (2, 40, 1)])
+def test_pivoted_thin_qr_matches_oracle(gpu, side, seed, dmax, rank):
+    """_thin_qr's sector-wise column-pivoted QR on the device against the oracle's: the same
+    kept widths and the same orthonormal columns (both with a non-negative R diagonal), on
+    full-rank and rank-deficient sketches."""
+    from bondupdategalerkin_amd import tensor_ops as T
+    from oracle import dd_adaptive as da
+
+    rng = np.random.default_rng(seed)
+    row, col, shifts = {-1: dmax, 1: dmax + 2}, {0: 7, 2: 5, -2: 9}, [1, -1]
+    if side == "right":
+        row, col = {0: 7, 2: 5, -2: 9}, {-1: dmax, 1: dmax + 2}
+    A = ob.Fam.zeros(row, col, shifts)
+    for key, b in A.blocks.items():
+        M = rng.standard_normal(b.shape) + 1j * rng.standard_normal(b.shape)
+        if rank is not None:
+            k = min(rank, *b.shape)
+            M = M[:, :k] @ (rng.standard_normal((k, b.shape[1])) + 0j) if side == "left" else \
+                (rng.standard_normal((b.shape[0], k)) + 0j) @ M[:k, :]
+        A.blocks[key] = M
+    thr = 1e-12 * A.norm()
+    o = da.thin_qr(A, side, thr)
+    d = T.thin_qr(device_from_fam(A), side, thr)
+    assert (o is None) == (d is None)
+    if o is None:
+        return
+    df = fam_from_device(d)
+    assert (df.col if side == "left" else df.row) == (o.col if side == "left" else o.row)
+    assert rel_err(df, o) < 1e-10
