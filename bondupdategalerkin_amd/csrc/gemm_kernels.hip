@@ -118,10 +118,15 @@ __global__ __launch_bounds__(64 * WM * WN) void grouped_zgemm(
         const double2* __restrict__ Bp = B + term.b_off;
         const int64_t lda = term.lda;
         const int64_t ldb = term.ldb;
-        for (int k0 = 0; k0 < term.k; k0 += kTK) {
-            // op_a(A)[m0 + i][k0 + kk] scaled by alpha -> LDS (k-major)
+        // software pipeline: the next stage's operands are fetched into registers while the
+        // current stage's MFMAs run from LDS
+        constexpr int LA = (TM * kTK) / NT;
+        constexpr int LB = (kTK * TN) / NT;
+        double2 ra[LA], rb[LB];
+        auto fetch = [&](int k0) {
 #pragma unroll
-            for (int e = tid; e < TM * kTK; e += NT) {
+            for (int u = 0; u < LA; ++u) {
+                const int e = tid + u * NT;
                 int i, kk;
                 if (a_plain) {
                     i = e % TM;
@@ -130,21 +135,13 @@ __global__ __launch_bounds__(64 * WM * WN) void grouped_zgemm(
                     kk = e % kTK;
                     i = e / kTK;
                 }
-                const int gi = m0 + i;
-                const int gk = k0 + kk;
-                double re = 0.0, im = 0.0;
-                if (gi < task.m && gk < term.k) {
-                    const double2 v = a_plain ? Ap[gi + gk * lda] : Ap[gk + gi * lda];
-                    const double vi = a_conj * v.y;
-                    re = term.alpha_re * v.x - term.alpha_im * vi;
-                    im = term.alpha_re * vi + term.alpha_im * v.x;
-                }
-                As_re[kk][i] = static_cast<T>(re);
-                As_im[kk][i] = static_cast<T>(im);
+                const int gi = m0 + i, gk = k0 + kk;
+                ra[u] = (gi < task.m && gk < term.k) ? (a_plain ? Ap[gi + gk * lda] : Ap[gk + gi * lda])
+                                                     : double2{0.0, 0.0};
             }
-            // op_b(B)[k0 + kk][n0 + j] -> LDS (k-major)
 #pragma unroll
-            for (int e = tid; e < kTK * TN; e += NT) {
+            for (int u = 0; u < LB; ++u) {
+                const int e = tid + u * NT;
                 int kk, j;
                 if (b_plain) {
                     kk = e % kTK;
@@ -153,18 +150,46 @@ __global__ __launch_bounds__(64 * WM * WN) void grouped_zgemm(
                     j = e % TN;
                     kk = e / TN;
                 }
-                const int gj = n0 + j;
-                const int gk = k0 + kk;
-                double re = 0.0, im = 0.0;
-                if (gj < task.n && gk < term.k) {
-                    const double2 v = b_plain ? Bp[gk + gj * ldb] : Bp[gj + gk * ldb];
-                    re = v.x;
-                    im = b_conj * v.y;
+                const int gj = n0 + j, gk = k0 + kk;
+                rb[u] = (gj < task.n && gk < term.k) ? (b_plain ? Bp[gk + gj * ldb] : Bp[gj + gk * ldb])
+                                                     : double2{0.0, 0.0};
+            }
+        };
+        fetch(0);
+        for (int k0 = 0; k0 < term.k; k0 += kTK) {
+            // op_a(A) scaled by alpha and op_b(B) -> LDS (k-major)
+#pragma unroll
+            for (int u = 0; u < LA; ++u) {
+                const int e = tid + u * NT;
+                int i, kk;
+                if (a_plain) {
+                    i = e % TM;
+                    kk = e / TM;
+                } else {
+                    kk = e % kTK;
+                    i = e / kTK;
                 }
-                Bs_re[kk][j] = static_cast<T>(re);
-                Bs_im[kk][j] = static_cast<T>(im);
+                const double2 v = ra[u];
+                const double vi = a_conj * v.y;
+                As_re[kk][i] = static_cast<T>(term.alpha_re * v.x - term.alpha_im * vi);
+                As_im[kk][i] = static_cast<T>(term.alpha_re * vi + term.alpha_im * v.x);
+            }
+#pragma unroll
+            for (int u = 0; u < LB; ++u) {
+                const int e = tid + u * NT;
+                int kk, j;
+                if (b_plain) {
+                    kk = e % kTK;
+                    j = e / kTK;
+                } else {
+                    j = e % TN;
+                    kk = e / TN;
+                }
+                Bs_re[kk][j] = static_cast<T>(rb[u].x);
+                Bs_im[kk][j] = static_cast<T>(b_conj * rb[u].y);
             }
             __syncthreads();
+            if (k0 + kTK < term.k) fetch(k0 + kTK);
 #pragma unroll
             for (int kk = 0; kk < kTK; kk += 4) {
                 T a_re[FM], a_im[FM], b_re[FN], b_im[FN];
