@@ -34,6 +34,10 @@ const int kNumTileConfigs = sizeof(kTileConfigs) / sizeof(kTileConfigs[0]);
 namespace {
 
 constexpr int kTK = 16;  // complex k per LDS stage
+// LDS row padding: a row stride of 16 (mod 64) 4-byte banks keeps the four k-rows that one
+// MFMA operand read touches in different bank quarters (conflict-free for FP32, the minimum
+// two passes for FP64)
+constexpr int kPad = 16;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -78,10 +82,10 @@ __global__ __launch_bounds__(64 * WM * WN) void grouped_zgemm(
     constexpr int TM = 16 * FM * WM;
     constexpr int TN = 16 * FN * WN;
     typedef typename Mfma<T>::acc_t acc_t;
-    __shared__ T As_re[kTK][TM];
-    __shared__ T As_im[kTK][TM];
-    __shared__ T Bs_re[kTK][TN];
-    __shared__ T Bs_im[kTK][TN];
+    __shared__ T As_re[kTK][TM + kPad];
+    __shared__ T As_im[kTK][TM + kPad];
+    __shared__ T Bs_re[kTK][TN + kPad];
+    __shared__ T Bs_im[kTK][TN + kPad];
 
     const int logical = xcd_remap(blockIdx.x, gridDim.x);
     const GemmTile tile = tiles[logical];
