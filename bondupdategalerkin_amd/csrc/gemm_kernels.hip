@@ -33,7 +33,12 @@ const int kNumTileConfigs = sizeof(kTileConfigs) / sizeof(kTileConfigs[0]);
 
 namespace {
 
-constexpr int kTK = 16;  // complex k per LDS stage
+// complex k per LDS stage: 32 for FP32 operands on 256-thread tiles, else 16 (the LDS of the
+// widest FP64 tile, the prefetch registers of the 64-thread tiles)
+template <typename T, int NT>
+struct Stage {
+    static constexpr int k = (sizeof(T) == 4 && NT >= 256) ? 32 : 16;
+};
 // LDS row padding: a row stride of 16 (mod 64) 4-byte banks keeps the four k-rows that one
 // MFMA operand read touches in different bank quarters (conflict-free for FP32, the minimum
 // two passes for FP64)
@@ -82,6 +87,7 @@ __global__ __launch_bounds__(64 * WM * WN) void grouped_zgemm(
     constexpr int TM = 16 * FM * WM;
     constexpr int TN = 16 * FN * WN;
     typedef typename Mfma<T>::acc_t acc_t;
+    constexpr int kTK = Stage<T, 64 * WM * WN>::k;
     __shared__ T As_re[kTK][TM + kPad];
     __shared__ T As_im[kTK][TM + kPad];
     __shared__ T Bs_re[kTK][TN + kPad];
