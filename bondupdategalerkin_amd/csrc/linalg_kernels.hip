@@ -181,14 +181,17 @@ __global__ void copy_blocks_kernel(const CopyOp* __restrict__ ops, const double2
     }
 }
 
-// squared Frobenius norm of each listed block (CopyOp fields: src_off, src_ld, rows, cols)
+// partial squared Frobenius norms of the listed blocks (CopyOp fields: src_off, src_ld, rows,
+// cols): workgroup (b, y) of a (nblk, chunks) grid sums the elements y, y + chunks, ... of block b
+// in units of its 256 threads and writes out[b * chunks + y]; the caller adds the chunks in order
 __global__ void block_sqnorm_kernel(const CopyOp* __restrict__ ops, const double2* __restrict__ src,
                                     double* __restrict__ out) {
     __shared__ double red[4];
     const CopyOp op = ops[blockIdx.x];
     const int64_t total = (int64_t)op.rows * op.cols;
     double acc = 0.0;
-    for (int64_t e = threadIdx.x; e < total; e += blockDim.x) {
+    for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.y * blockDim.x) {
         const int r = (int)(e % op.rows);
         const int c = (int)(e / op.rows);
         const double2 v = src[op.src_off + r + (int64_t)c * op.src_ld];
@@ -197,7 +200,8 @@ __global__ void block_sqnorm_kernel(const CopyOp* __restrict__ ops, const double
     acc = wave_sum(acc);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) out[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x == 0)
+        out[(int64_t)blockIdx.x * gridDim.y + blockIdx.y] = red[0] + red[1] + red[2] + red[3];
 }
 
 // ---- batched thin QR (Householder, unique gauge) ------------------------------------------
@@ -796,7 +800,8 @@ __global__ __launch_bounds__(1024) void qr_finish_kernel(const MatJob* __restric
     const int64_t lda = job.lda, ldq = job.ldq, ldr = job.ldr;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nw = blockDim.x / 64;
-    for (int j = wave; j < k; j += nw) {
+    // one wave per row of R (column of Q); the rows are spread over gridDim.y workgroups
+    for (int j = blockIdx.y * nw + wave; j < k; j += gridDim.y * nw) {
         const bool neg = a[j + (int64_t)j * lda].x < 0.0;
         for (int c = lane; c < n; c += 64) {
             double2 v = c >= j ? a[j + (int64_t)c * lda] : double2{0.0, 0.0};
@@ -1015,7 +1020,8 @@ __global__ void svd_phase_kernel(const SvdJob* __restrict__ jobs, double2* __res
     double2* u = U + job.u_off;
     double2* vh = Vh + job.vh_off;
     const int64_t ldu = job.ldu, ldvh = job.ldvh;
-    for (int r = wave; r < n; r += nw) {
+    // one wave per column; the columns of a matrix are spread over gridDim.y workgroups
+    for (int r = blockIdx.y * nw + wave; r < n; r += gridDim.y * nw) {
         const bool onv = job.phase_on_v != 0;
         const int len = onv ? n : m;
         double mx = 0.0;
@@ -1635,7 +1641,8 @@ void blocked_qr(bug_ctx* ctx, hipStream_t st, const std::vector<MatJob>& jobs, d
             hipLaunchKernelGGL(apply_reflector_kernel, dim3((unsigned)nq), dim3(kReflThreads), 0, st,
                                dt + qstart[p], Q, Yb, Vb);
     }
-    hipLaunchKernelGGL(qr_finish_kernel, dim3(nmat), dim3(1024), 0, st, dj, A, Q, R);
+    hipLaunchKernelGGL(qr_finish_kernel, dim3(nmat, std::min(32, std::max(1, (kmax + 15) / 16))),
+                       dim3(1024), 0, st, dj, A, Q, R);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -1914,16 +1921,26 @@ int bug_block_sqnorms(bug_ctx* ctx, int32_t nblk, const bug_copy* blocks, const 
                 throw Error(BUG_EINVARIANT, "block outside its buffer");
             list[i] = CopyOp{o.src_off, 0, o.src_ld, 1, o.rows, o.cols, 0, 0, 0};
         }
+        // enough workgroups per block to spread a large one over the device (16 elements per
+        // thread and chunk at least), at most 64 chunks
+        int64_t biggest = 1;
+        for (const CopyOp& o : list) biggest = std::max<int64_t>(biggest, (int64_t)o.rows * o.cols);
+        const int chunks = (int)std::min<int64_t>(64, std::max<int64_t>(1, biggest / (256 * 16)));
         hipStream_t st = as_stream(stream);
         CopyOp* d = upload_list(ctx, st, list.data(), list.size(), 2);
-        double* res = static_cast<double*>(scratch(ctx, st, sizeof(double) * nblk, 3));
-        hipLaunchKernelGGL(block_sqnorm_kernel, dim3(nblk), dim3(256), 0, st, d,
+        const size_t nres = (size_t)nblk * chunks;
+        double* res = static_cast<double*>(scratch(ctx, st, sizeof(double) * nres, 3));
+        hipLaunchKernelGGL(block_sqnorm_kernel, dim3(nblk, chunks), dim3(256), 0, st, d,
                            static_cast<const double2*>(src), res);
         HIP_CHECK(hipGetLastError());
-        double* h = pinned(ctx, sizeof(double) * nblk);
-        HIP_CHECK(hipMemcpyAsync(h, res, sizeof(double) * nblk, hipMemcpyDeviceToHost, st));
+        double* h = pinned(ctx, sizeof(double) * nres);
+        HIP_CHECK(hipMemcpyAsync(h, res, sizeof(double) * nres, hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipStreamSynchronize(st));
-        std::memcpy(out, h, sizeof(double) * nblk);
+        for (int i = 0; i < nblk; ++i) {
+            double acc = 0.0;
+            for (int y = 0; y < chunks; ++y) acc += h[(size_t)i * chunks + y];
+            out[i] = acc;
+        }
     });
 }
 
@@ -2156,7 +2173,8 @@ int bug_block_svd(bug_ctx* ctx, int32_t nmat, const bug_svd_job* jobs, void* g, 
                                static_cast<double2*>(vh), nullptr, 0);
             HIP_CHECK(hipGetLastError());
             SvdJob* dp = upload_list(ctx, st, list.data(), list.size(), 2);
-            hipLaunchKernelGGL(svd_phase_kernel, dim3(nmat), dim3(512), 0, st, dp,
+            const int pcg = std::min(64, std::max(1, (nmax + 7) / 8));
+            hipLaunchKernelGGL(svd_phase_kernel, dim3(nmat, pcg), dim3(512), 0, st, dp,
                                static_cast<double2*>(u), static_cast<double2*>(vh));
         }
         HIP_CHECK(hipGetLastError());
