@@ -32,6 +32,8 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 METRIC = "BUG steps/s L=128 D=1024 mixed vs FP64 @1e-6 err; p50/p99 request latency"
+# tile configurations of the grouped GEMM (waves in m, waves in n, 16 x 16 MFMA tiles per wave)
+TILES = [(2, 2, 2, 2), (1, 1, 2, 2), (2, 2, 1, 1), (2, 2, 4, 2), (1, 1, 1, 1)]
 PEAK = {"fp64": 78.6, "fp32": 157.3}   # dense MFMA TFLOP/s, MI355X datasheet
 HBM_PEAK = 8000.0                      # GB/s
 
@@ -62,6 +64,12 @@ def parse():
                    help="Heston instances (2^10 x 2^10 grid) stepped through the serving loop "
                         "(0: skip)")
     p.add_argument("--heston-steps", type=int, default=1)
+    p.add_argument("--autotune", choices=["tune", "off"], default="tune",
+                   help="tune: the grouped-GEMM autotuner times candidate schedules during the "
+                        "kernel warm-up and the warm-up steps, the timed steps use its table; "
+                        "off: heuristic schedules")
+    p.add_argument("--save-schedules", default=None,
+                   help="write the autotuner's table (plan class -> schedule) to this file")
     p.add_argument("--kernel-only", action="store_true",
                    help="run only the dominant-kernel timing (for profiling)")
     return p.parse_args()
@@ -87,12 +95,24 @@ def main():
     ckind = "fp64" if args.precision == "fp64" else "fp32"
     L, D = args.L, args.D
     t0 = time.time()
+    from bondupdategalerkin_amd import bug_step as B
+    from bondupdategalerkin_amd import tensor_ops as T
+
     psi = random_mps(L, D, seed=0)
     mpo = xxz_mpo(L, delta=args.delta)
+    mid = L // 2
+    # mixed-canonical about the centre and normalised: the environments below stay bounded
+    B.canonical(psi, mid)
+    T.scale_(1.0 / T.norm(psi[mid]), psi[mid])
     torch.cuda.synchronize()
     print(f"# state built in {time.time() - t0:.1f}s; bond dims max {max(psi.bond_dims())}",
           file=sys.stderr, flush=True)
-    mid = L // 2
+
+    def tuning(on):
+        if args.autotune == "tune":
+            _lib.set_autotune(_lib.AUTOTUNE_TUNE if on else _lib.AUTOTUNE_USE)
+
+    tuning(True)
     # right environments once (inputs of every apply)
     renvs = ev.right_env_stack(psi, mpo, downto=mid + 1, precision=prec)
     lenvs = ev.left_env_stack(psi, mpo, upto=mid, precision=prec)
@@ -105,6 +125,7 @@ def main():
     sample_sites = (psi[mid], psi[mid + 1])      # the tensors the environments were built from
     ev.apply_one_site(Lm, mpo[mid], Rm, X, precision=prec)
     flops_apply = ctx.last_flops()
+    schedules = ctx.last_schedules()
     stream = torch.cuda.current_stream()
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.kernel_reps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.kernel_reps)]
@@ -123,7 +144,9 @@ def main():
                else None)
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
                 "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
-                "kernel": f"grouped_zgemm {ckind} (one-site H_eff apply, 2 launches)",
+                "kernel": (f"grouped_zgemm {ckind} (one-site H_eff apply: two grouped-GEMM "
+                           "stages, each followed by its split-k reduce when split)"),
+                "schedules": [{"tile": list(TILES[c]), "splits": sp} for c, sp in schedules],
                 "flops_per_apply": flops_apply, "apply_ms": round(apply_ms, 4)}
     if args.kernel_only:
         print(json.dumps({"kernel_only": True, "roofline": roofline}), flush=True)
@@ -132,9 +155,7 @@ def main():
     # ---- the timed step ----
     statuses, extra = [], {}
     if args.workload == "bug_step":
-        from bondupdategalerkin_amd import bug_step as B
         from bondupdategalerkin_amd import mps as M
-        from bondupdategalerkin_amd import tensor_ops as T
 
         B.canonical(psi, L - 1)
         T.scale_(1.0 / T.norm(psi[L - 1]), psi[L - 1])
@@ -159,10 +180,12 @@ def main():
         if twin is not None:
             fp64_statuses = []
             step64 = make_step(twin, "fp64", fp64_statuses)
+            tuning(True)
             for _ in range(args.warmup):
                 step64()
             torch.cuda.synchronize()
             fp64_statuses.clear()
+            tuning(False)
             t1 = time.perf_counter()
             for _ in range(args.steps):
                 step64()
@@ -179,10 +202,12 @@ def main():
                     ev.apply_one_site(lenv, mpo[i], R, psi[i], precision=prec)
                 lenv = ev.push_left(lenv, psi[i], mpo[i], precision=prec)
 
+    tuning(True)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
     statuses.clear()
+    tuning(False)
     t1 = time.perf_counter()
     for _ in range(args.steps):
         step()
@@ -267,9 +292,14 @@ def main():
                    "parallelism": "single GPU",
                    "budget_status": statuses},
         **extra,
+        "autotune": ({"mode": "tuned in the kernel warm-up and warm-up steps, table used in the "
+                              "timed steps", "classes": _lib.autotune_count()[0]}
+                     if args.autotune == "tune" else {"mode": "off (heuristic schedules)"}),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if args.save_schedules:
+        _lib.autotune_save(args.save_schedules)
     print(json.dumps(out), flush=True)
 
 

@@ -106,6 +106,13 @@ SIGNATURES = {
     "bug_last_error": (ctypes.c_char_p, [_P]),
     "bug_set_tile_config": (ctypes.c_int, [_P, ctypes.c_int]),
     "bug_tile_config_count": (ctypes.c_int, []),
+    "bug_set_gemm_schedule": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
+    "bug_set_autotune": (ctypes.c_int, [ctypes.c_int32]),
+    "bug_autotune_count": (ctypes.c_int, [_i64p]),
+    "bug_autotune_clear": (ctypes.c_int, []),
+    "bug_autotune_save": (ctypes.c_int, [ctypes.c_char_p]),
+    "bug_autotune_load": (ctypes.c_int, [ctypes.c_char_p]),
+    "bug_last_schedules": (ctypes.c_int, [_P, ctypes.c_int32, _i32p, _i32p]),
     "bug_ctx_stats": (ctypes.c_int, [_P, _i64p]),
     "bug_last_op_flops": (ctypes.c_int, [_P, _f64p]),
     "bug_bsm_product": (
@@ -210,6 +217,13 @@ class Context:
         self.check(self._lib.bug_last_op_flops(self.handle, ctypes.byref(v)))
         return v.value
 
+    def last_schedules(self):
+        """[(tile configuration, split factor)] of the grouped-GEMM launches of the last call."""
+        out = (ctypes.c_int32 * 64)()
+        n = ctypes.c_int32()
+        self.check(self._lib.bug_last_schedules(self.handle, 32, out, ctypes.byref(n)))
+        return [(out[2 * i], out[2 * i + 1]) for i in range(min(n.value, 32))]
+
     def stats(self):
         out = (ctypes.c_int64 * 4)()
         self.check(self._lib.bug_ctx_stats(self.handle, out))
@@ -217,6 +231,11 @@ class Context:
 
     def set_tile_config(self, config: int):
         self.check(self._lib.bug_set_tile_config(self.handle, config))
+
+    def set_gemm_schedule(self, config: int = -1, splits: int = 0):
+        """Force this context's grouped-GEMM tile configuration and split-k factor (-1 / 0:
+        automatic)."""
+        self.check(self._lib.bug_set_gemm_schedule(self.handle, config, splits))
 
     def close(self):
         if self.handle:
@@ -228,6 +247,39 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+AUTOTUNE_OFF, AUTOTUNE_USE, AUTOTUNE_TUNE = 0, 1, 2
+
+
+def set_autotune(mode: int):
+    """Process-wide grouped-GEMM autotuning: AUTOTUNE_OFF (heuristic schedules), AUTOTUNE_USE
+    (schedules from the tuning table where it has the plan's class) or AUTOTUNE_TUNE (as USE, and
+    time every candidate on the first launch of a new class)."""
+    if lib().bug_set_autotune(mode) != BUG_OK:
+        raise ValueError(f"autotune mode must be 0, 1 or 2, not {mode}")
+
+
+def autotune_count():
+    """(classes in the tuning table, classes tuned by this process)."""
+    out = (ctypes.c_int64 * 2)()
+    lib().bug_autotune_count(out)
+    return int(out[0]), int(out[1])
+
+
+def autotune_clear():
+    lib().bug_autotune_clear()
+
+
+def autotune_save(path: str):
+    if lib().bug_autotune_save(os.fsencode(path)) != BUG_OK:
+        raise OSError(f"cannot write the tuning table to {path}")
+
+
+def autotune_load(path: str):
+    """Merge a saved tuning table (raises on a missing or malformed file)."""
+    if lib().bug_autotune_load(os.fsencode(path)) != BUG_OK:
+        raise ValueError(f"cannot read a tuning table from {path}")
 
 
 _local = threading.local()

@@ -232,7 +232,10 @@ void build_product_plan(Plan& plan, const Bsm& out, const Bsm& P, const Bsm& Q,
         }
 }
 
-// Choose a tile configuration: the largest tile that still puts enough workgroups on the chip.
+// Choose a tile configuration from block-size statistics: 16 x 16 or 32 x 32 tiles of one wave
+// when most of the work is in blocks 32 or fewer wide, else 32 x 32 tiles of four waves (on the
+// step's contractions the autotuner picks the 64 x 64 and 128 x 64 tiles for under 2 % of the
+// plan classes: their few workgroups leave most SIMDs idle).
 int choose_config(const Plan& plan) {
     double work = 0.0, small = 0.0;
     for (const GemmTask& t : plan.tasks) {
@@ -249,7 +252,6 @@ int choose_config(const Plan& plan) {
         return n;
     };
     if (small / work > 0.5) return tiles_for(4) > 4 * 256 ? 1 : 4;
-    if (tiles_for(0) >= 2 * 256) return 0;
     return 2;
 }
 
@@ -353,10 +355,14 @@ void launch_cached_plan(bug_ctx* ctx, uint64_t key, const std::function<void(Pla
             delete old;
         }
     }
-    const int config = ctx->forced_config >= 0 ? ctx->forced_config : choose_config(*plan);
-    upload_plan(ctx, *plan, config, stream);
+    const Schedule sc = pick_schedule(ctx, *plan, precision, A, B, stream);
+    upload_plan(ctx, *plan, sc.config, stream);
+    void* partial = nullptr;
+    if (sc.splits > 1)
+        partial = scratch(ctx, stream, split_partial_bytes(*plan, sc.config, sc.splits, precision), 6);
     ctx->last_flops += plan->flops;
-    HIP_CHECK(launch_grouped_gemm(*plan, config, precision, A, B, C, stream));
+    ctx->last_schedules.push_back(sc);
+    HIP_CHECK(launch_grouped_gemm(*plan, sc.config, sc.splits, precision, A, B, C, partial, stream));
 }
 
 void* scratch(bug_ctx* ctx, hipStream_t stream, size_t bytes, int slot) {
@@ -455,6 +461,8 @@ int bug_ctx_destroy(bug_ctx* ctx) {
             if (s.second.ptr) (void)hipFree(s.second.ptr);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->svd_sweeps_dev) (void)hipFree(ctx->svd_sweeps_dev);
+    for (hipEvent_t e : ctx->tune_ev)
+        if (e) (void)hipEventDestroy(e);
     delete ctx;
     return BUG_OK;
 }
@@ -481,6 +489,17 @@ int bug_ctx_stats(const bug_ctx* ctx, int64_t* out4) {
     out4[1] = (int64_t)ctx->plan_lru.size();
     out4[2] = ctx->plan_hits;
     out4[3] = ctx->plan_misses;
+    return BUG_OK;
+}
+
+int bug_last_schedules(const bug_ctx* ctx, int32_t cap, int32_t* out, int32_t* count) {
+    if (!ctx || !count || cap < 0 || (cap > 0 && !out)) return BUG_EINVAL;
+    const int32_t n = (int32_t)ctx->last_schedules.size();
+    for (int32_t i = 0; i < std::min(n, cap); ++i) {
+        out[2 * i] = ctx->last_schedules[i].config;
+        out[2 * i + 1] = ctx->last_schedules[i].splits;
+    }
+    *count = n;
     return BUG_OK;
 }
 

@@ -78,11 +78,28 @@ __device__ __forceinline__ int xcd_remap(int bid, int grid) {
     return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
 }
 
+// Partial tiles of a split-k launch: split s of tile t stores value e (of R = 8 FM FN per
+// thread: FM x FN fragments x 4 rows x re/im) of thread tid at ((s ntiles + t) R + e) NT + tid,
+// so each store instruction of a wave covers 64 consecutive elements.
+template <int FM, int FN>
+struct PartialLayout {
+    static constexpr int R = 8 * FM * FN;
+    static __device__ __forceinline__ int64_t at(int s, int ntiles, int t, int e, int nt, int tid) {
+        return (((int64_t)s * ntiles + t) * R + e) * nt + tid;
+    }
+};
+
+// grid = ntiles x splits.  splits = 1: each workgroup runs every k stage of its tile's terms and
+// writes C.  splits > 1: the k stages of a tile (its terms' stages in order, ceil(k / kTK) each)
+// are cut into `splits` contiguous windows, workgroup (tile, s) accumulates window s and stores
+// the partial tile; split_reduce then sums the windows in order, so the result does not depend
+// on how the windows were scheduled.
 template <typename T, int WM, int WN, int FM, int FN>
 __global__ __launch_bounds__(64 * WM * WN) void grouped_zgemm(
     const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks,
     const GemmTerm* __restrict__ terms, const double2* __restrict__ A,
-    const double2* __restrict__ B, double2* __restrict__ C) {
+    const double2* __restrict__ B, double2* __restrict__ C, int ntiles, int splits,
+    T* __restrict__ partial) {
     constexpr int NT = 64 * WM * WN;
     constexpr int TM = 16 * FM * WM;
     constexpr int TN = 16 * FN * WN;
@@ -94,7 +111,10 @@ __global__ __launch_bounds__(64 * WM * WN) void grouped_zgemm(
     __shared__ T Bs_im[kTK][TN + kPad];
 
     const int logical = xcd_remap(blockIdx.x, gridDim.x);
-    const GemmTile tile = tiles[logical];
+    // consecutive logical ids: consecutive tiles of one window (they share operand panels)
+    const int tile_id = logical % ntiles;
+    const int split = logical / ntiles;
+    const GemmTile tile = tiles[tile_id];
     const GemmTask task = tasks[tile.task];
     const int m0 = tile.tm * TM;
     const int n0 = tile.tn * TN;
@@ -116,8 +136,22 @@ __global__ __launch_bounds__(64 * WM * WN) void grouped_zgemm(
             acc_im[i][j] = acc_t{0, 0, 0, 0};
         }
 
+    // this workgroup's window [s0, s1) of the task's k stages
+    int s0 = 0, s1 = 1 << 30;
+    if (splits > 1) {
+        int total = 0;
+        for (int t = task.term_begin; t < task.term_end; ++t) total += (terms[t].k + kTK - 1) / kTK;
+        s0 = (int)((int64_t)total * split / splits);
+        s1 = (int)((int64_t)total * (split + 1) / splits);
+    }
+    int g = 0;                                // stage index of the current term's first stage
     for (int t = task.term_begin; t < task.term_end; ++t) {
         const GemmTerm term = terms[t];
+        const int nst = (term.k + kTK - 1) / kTK;
+        const int lo = max(s0 - g, 0), hi = min(s1 - g, nst);
+        g += nst;
+        if (lo >= hi) continue;
+        const int kbeg = lo * kTK, kend = min(hi * kTK, term.k);
         const int opa = term.ops & 3;
         const int opb = (term.ops >> 2) & 3;
         const bool a_plain = (opa == BUG_OP_N || opa == BUG_OP_R);
@@ -165,8 +199,8 @@ __global__ __launch_bounds__(64 * WM * WN) void grouped_zgemm(
                                                      : double2{0.0, 0.0};
             }
         };
-        fetch(0);
-        for (int k0 = 0; k0 < term.k; k0 += kTK) {
+        fetch(kbeg);
+        for (int k0 = kbeg; k0 < kend; k0 += kTK) {
             // op_a(A) scaled by alpha and op_b(B) -> LDS (k-major)
 #pragma unroll
             for (int u = 0; u < LA; ++u) {
@@ -199,7 +233,7 @@ __global__ __launch_bounds__(64 * WM * WN) void grouped_zgemm(
                 Bs_im[kk][j] = static_cast<T>(b_conj * rb[u].y);
             }
             __syncthreads();
-            if (k0 + kTK < term.k) fetch(k0 + kTK);
+            if (k0 + kTK < kend) fetch(k0 + kTK);
 #pragma unroll
             for (int kk = 0; kk < kTK; kk += 4) {
                 T a_re[FM], a_im[FM], b_re[FN], b_im[FN];
@@ -229,6 +263,20 @@ __global__ __launch_bounds__(64 * WM * WN) void grouped_zgemm(
         }
     }
 
+    if (splits > 1) {
+        typedef PartialLayout<FM, FN> PL;
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int e = ((i * FN + j) * 4 + r) * 2;
+                    partial[PL::at(split, ntiles, tile_id, e, NT, tid)] = acc_re[i][j][r];
+                    partial[PL::at(split, ntiles, tile_id, e + 1, NT, tid)] = acc_im[i][j][r];
+                }
+        return;
+    }
     double2* __restrict__ Cp = C + task.c_off;
     const int64_t ldc = task.ldc;
 #pragma unroll
@@ -254,36 +302,96 @@ __global__ __launch_bounds__(64 * WM * WN) void grouped_zgemm(
             }
 }
 
+// Sum the split windows of each tile in window order (in FP64) and write C like the one-pass
+// epilogue of grouped_zgemm (same lane -> element map, same beta rule).
 template <typename T, int WM, int WN, int FM, int FN>
-hipError_t launch_one(const Plan& plan, const void* A, const void* B, void* C,
-                      hipStream_t stream) {
-    hipLaunchKernelGGL((grouped_zgemm<T, WM, WN, FM, FN>), dim3(plan.ntiles),
-                       dim3(64 * WM * WN), 0, stream, plan.d_tiles, plan.d_tasks, plan.d_terms,
+__global__ __launch_bounds__(64 * WM * WN) void split_reduce(
+    const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks,
+    double2* __restrict__ C, int ntiles, int splits, const T* __restrict__ partial) {
+    constexpr int NT = 64 * WM * WN;
+    constexpr int TM = 16 * FM * WM;
+    constexpr int TN = 16 * FN * WN;
+    typedef PartialLayout<FM, FN> PL;
+    const int tile_id = blockIdx.x;
+    const GemmTile tile = tiles[tile_id];
+    const GemmTask task = tasks[tile.task];
+    const int m0 = tile.tm * TM, n0 = tile.tn * TN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    const int lrow = lane & 15, lk = lane >> 4;
+    double2* __restrict__ Cp = C + task.c_off;
+    const int64_t ldc = task.ldc;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int e = ((i * FN + j) * 4 + r) * 2;
+                double2 out{0.0, 0.0};
+                for (int s = 0; s < splits; ++s) {
+                    out.x += static_cast<double>(partial[PL::at(s, ntiles, tile_id, e, NT, tid)]);
+                    out.y += static_cast<double>(partial[PL::at(s, ntiles, tile_id, e + 1, NT, tid)]);
+                }
+                const int gm = m0 + wm * 16 * FM + i * 16 + lrow;
+                const int gn = n0 + wn * 16 * FN + j * 16 + Mfma<T>::drow(lk, r);
+                if (gm < task.m && gn < task.n) {
+                    double2* dst = Cp + gm + gn * ldc;
+                    if (task.beta) {
+                        const double2 old = *dst;
+                        out.x += old.x;
+                        out.y += old.y;
+                    }
+                    *dst = out;
+                }
+            }
+}
+
+template <typename T, int WM, int WN, int FM, int FN>
+hipError_t launch_one(const Plan& plan, int splits, const void* A, const void* B, void* C,
+                      void* partial, hipStream_t stream) {
+    const dim3 block(64 * WM * WN);
+    hipLaunchKernelGGL((grouped_zgemm<T, WM, WN, FM, FN>), dim3(plan.ntiles * splits), block, 0,
+                       stream, plan.d_tiles, plan.d_tasks, plan.d_terms,
                        static_cast<const double2*>(A), static_cast<const double2*>(B),
-                       static_cast<double2*>(C));
+                       static_cast<double2*>(C), plan.ntiles, splits, static_cast<T*>(partial));
+    if (splits > 1)
+        hipLaunchKernelGGL((split_reduce<T, WM, WN, FM, FN>), dim3(plan.ntiles), block, 0, stream,
+                           plan.d_tiles, plan.d_tasks, static_cast<double2*>(C), plan.ntiles,
+                           splits, static_cast<const T*>(partial));
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_typed(const Plan& plan, int config, const void* A, const void* B, void* C,
-                        hipStream_t stream) {
+hipError_t launch_typed(const Plan& plan, int config, int splits, const void* A, const void* B,
+                        void* C, void* partial, hipStream_t stream) {
     switch (config) {
-        case 0: return launch_one<T, 2, 2, 2, 2>(plan, A, B, C, stream);
-        case 1: return launch_one<T, 1, 1, 2, 2>(plan, A, B, C, stream);
-        case 2: return launch_one<T, 2, 2, 1, 1>(plan, A, B, C, stream);
-        case 3: return launch_one<T, 2, 2, 4, 2>(plan, A, B, C, stream);
-        case 4: return launch_one<T, 1, 1, 1, 1>(plan, A, B, C, stream);
+        case 0: return launch_one<T, 2, 2, 2, 2>(plan, splits, A, B, C, partial, stream);
+        case 1: return launch_one<T, 1, 1, 2, 2>(plan, splits, A, B, C, partial, stream);
+        case 2: return launch_one<T, 2, 2, 1, 1>(plan, splits, A, B, C, partial, stream);
+        case 3: return launch_one<T, 2, 2, 4, 2>(plan, splits, A, B, C, partial, stream);
+        case 4: return launch_one<T, 1, 1, 1, 1>(plan, splits, A, B, C, partial, stream);
         default: return hipErrorInvalidValue;
     }
 }
 
 }  // namespace
 
-hipError_t launch_grouped_gemm(const Plan& plan, int config, int precision, const void* A,
-                               const void* B, void* C, hipStream_t stream) {
+size_t split_partial_bytes(const Plan& plan, int config, int splits, int precision) {
+    if (splits <= 1) return 0;
+    const TileConfig& tc = kTileConfigs[config];
+    const size_t elem = precision == BUG_PREC_FP32 ? sizeof(float) : sizeof(double);
+    return (size_t)plan.ntiles * splits * tc.tm() * tc.tn() * 2 * elem;
+}
+
+hipError_t launch_grouped_gemm(const Plan& plan, int config, int splits, int precision,
+                               const void* A, const void* B, void* C, void* partial,
+                               hipStream_t stream) {
     if (plan.ntiles == 0) return hipSuccess;
-    if (precision == BUG_PREC_FP32) return launch_typed<float>(plan, config, A, B, C, stream);
-    return launch_typed<double>(plan, config, A, B, C, stream);
+    if (splits < 1 || (splits > 1 && !partial)) return hipErrorInvalidValue;
+    if (precision == BUG_PREC_FP32)
+        return launch_typed<float>(plan, config, splits, A, B, C, partial, stream);
+    return launch_typed<double>(plan, config, splits, A, B, C, partial, stream);
 }
 
 }  // namespace bug

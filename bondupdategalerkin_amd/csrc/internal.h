@@ -86,6 +86,14 @@ struct SvdJob {
     int32_t m, n, ldg, ldv, ldu, ldvh, phase_on_v;
 };
 
+// How one grouped-GEMM launch runs: tile configuration (index into kTileConfigs) and the number
+// of k windows each output tile is split into (1 = no split).
+struct Schedule {
+    int config = 0;
+    int splits = 1;
+};
+constexpr int kMaxSplits = 8;
+
 // A product plan: tasks and terms in host memory, plus their device copy.
 struct Plan {
     std::vector<GemmTask> tasks;
@@ -101,6 +109,11 @@ struct Plan {
     const GemmTile* d_tiles = nullptr;
     const GemmTask* d_tasks = nullptr;
     const GemmTerm* d_terms = nullptr;
+    // the schedule chosen for this plan, valid while the tuning table generation, the tuning
+    // mode and the precision it was chosen under are unchanged
+    Schedule sched;
+    uint64_t sched_gen = 0;
+    int sched_mode = -1, sched_prec = -1;
 };
 
 // Tile configurations of the grouped GEMM (waves in m, waves in n, 16x16 MFMA tiles per wave).
@@ -131,9 +144,23 @@ struct Hasher {
     void pod(const T& v) { bytes(&v, sizeof(T)); }
 };
 
-// Launch the grouped GEMM of `plan` (device image already uploaded for `config`).
-hipError_t launch_grouped_gemm(const Plan& plan, int config, int precision, const void* A,
-                               const void* B, void* C, hipStream_t stream);
+// Launch the grouped GEMM of `plan` (device image already uploaded for `config`) with its k
+// stages split into `splits` windows; `partial` holds split_partial_bytes() of scratch when
+// splits > 1.
+hipError_t launch_grouped_gemm(const Plan& plan, int config, int splits, int precision,
+                               const void* A, const void* B, void* C, void* partial,
+                               hipStream_t stream);
+size_t split_partial_bytes(const Plan& plan, int config, int splits, int precision);
+
+// Tile configuration by block-size statistics; device image of a plan for a configuration.
+int choose_config(const Plan& plan);
+void upload_plan(bug_ctx* ctx, Plan& plan, int config, hipStream_t stream);
+
+// The schedule of a launch of `plan`: forced by the context, from the process-wide tuning table
+// (timing the candidates first when tuning is on and the plan's class is new), or heuristic.
+Schedule pick_schedule(bug_ctx* ctx, Plan& plan, int precision, const void* A, const void* B,
+                       hipStream_t stream);
+uint64_t plan_class(const Plan& plan, int precision);
 
 struct Workspace {
     void* ptr = nullptr;
@@ -149,6 +176,8 @@ struct bug_ctx {
     int device = 0;
     std::string last_error;
     int forced_config = -1;
+    int forced_splits = 0;       // 0: the schedule's own split factor
+    hipEvent_t tune_ev[2] = {nullptr, nullptr};
     std::mutex mu;
     // plan cache: signature hash -> plan (LRU)
     std::unordered_map<uint64_t, std::list<std::pair<uint64_t, bug::Plan*>>::iterator> plan_index;
@@ -161,6 +190,7 @@ struct bug_ctx {
     size_t plan_arena_bytes = 0, plan_arena_used = 0;
     uint64_t plan_arena_gen = 1;
     double last_flops = 0.0;     // algorithmic flops of the last entry-point call
+    std::vector<bug::Schedule> last_schedules;   // grouped-GEMM launches of that call
     // per-stream scratch for intermediates of the two-stage contractions and the solvers,
     // keyed by (stream, slot)
     std::unordered_map<void*, bug::Workspace> scratch[8];
@@ -198,6 +228,7 @@ int guarded(bug_ctx* ctx, F&& f) {
     try {
         HIP_CHECK(hipSetDevice(ctx->device));
         ctx->last_flops = 0.0;
+        ctx->last_schedules.clear();
         f();
         return BUG_OK;
     } catch (const Error& e) {

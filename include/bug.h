@@ -121,6 +121,23 @@ const char* bug_last_error(const bug_ctx* ctx);
 /* Force a GEMM tile configuration (autotuner override); -1 restores automatic choice. */
 int bug_set_tile_config(bug_ctx* ctx, int config);
 int bug_tile_config_count(void);
+/* Grouped-GEMM schedules (tile configuration + split-k factor).  The reference has no
+ * counterpart: MKL's zgemm picks its own blocking (deps/Telum/src/contract.jl:1022-1036).
+ * bug_set_gemm_schedule forces both on one context (config -1 / splits 0: automatic; splits in
+ * 1..8).  bug_set_autotune sets the process-wide mode: 0 heuristic only, 1 use the tuning table
+ * where it holds the plan's class, 2 as 1 and time every candidate schedule on the first launch
+ * of a new class (the launch then synchronizes its stream).  The table (plan class -> schedule)
+ * is shared by all contexts; bug_autotune_count gives [entries, classes tuned by this process];
+ * save/load write and merge a text file; clear empties it. */
+int bug_set_gemm_schedule(bug_ctx* ctx, int32_t config, int32_t splits);
+int bug_set_autotune(int32_t mode);
+int bug_autotune_count(int64_t* out2);
+int bug_autotune_clear(void);
+int bug_autotune_save(const char* path);
+int bug_autotune_load(const char* path);
+/* The (tile configuration, split factor) pairs of the grouped-GEMM launches of the last
+ * entry-point call, in launch order: min(cap, count) pairs into out[2 cap], the total in *count. */
+int bug_last_schedules(const bug_ctx* ctx, int32_t cap, int32_t* out, int32_t* count);
 /* Device workspace and plan-cache statistics: [bytes held, plans cached, plan hits, misses]. */
 int bug_ctx_stats(const bug_ctx* ctx, int64_t* out4);
 /* Algorithmic flops (sum of 8 m n k over the block products actually multiplied) of the last

@@ -173,3 +173,100 @@ def test_invalid_layout_is_refused(gpu):
     arr = (_lib.BugTerm * 1)(_lib.BugTerm(0, 0, 0, 0, 0, 1.0, 0.0))
     rc = ctx._lib.bug_bsm_product(ctx.handle, d, d, d, 1, arr, 0, 0, None)
     assert rc == _lib.BUG_EINVARIANT
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_every_schedule_agrees(gpu, prec):
+    """Every (tile configuration, split-k factor) schedule against the oracle on the one-site
+    apply and on an accumulating product; unsplit configurations agree bit for bit, and each
+    schedule is deterministic (a repeat is bitwise identical)."""
+    from bondupdategalerkin_amd import _lib
+    from bondupdategalerkin_amd import environments as ev
+    from bondupdategalerkin_amd.blocks import Layout, Leg, bsm_product
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    L = 6
+    psi, rng = _chain(7, L, 90, "U1")
+    mpo_o = om.xxz_mpo(L, delta=1.0)
+    mpo_d = xxz_mpo(L, delta=1.0)
+    lenvs, renvs = _envs(psi, mpo_o)
+    site = 3
+    ref = ob.apply_one_site(lenvs[site], mpo_o[site], renvs[site + 1], psi[site])
+    Ld, Rd, Xd = (device_from_fam(lenvs[site]), device_from_fam(renvs[site + 1]),
+                  device_from_fam(psi[site]))
+    m, kk, n = {0: 70, 2: 5}, {1: 200, 3: 33}, {2: 40, 4: 90}
+    P, Q = _rand_fam(rng, m, kk, [1]), _rand_fam(rng, kk, n, [1])
+    C0 = _rand_fam(rng, m, n, [2])
+    outl = Layout(Leg.make(m.items()), Leg.make(n.items()), [2])
+    Pd, Qd = device_from_fam(P), device_from_fam(Q)
+    ref2 = C0.dense()[0] + (0.5 + 1j) * P.dense()[0] @ Q.dense()[0]
+    tol = TOL64 if prec == 0 else TOL32
+    ctx = _lib.context()
+    unsplit = None
+    try:
+        for cfg in range(_lib.lib().bug_tile_config_count()):
+            for splits in (1, 2, 3, 8):
+                ctx.set_gemm_schedule(cfg, splits)
+                runs = []
+                for _ in range(2):
+                    got = ev.apply_one_site(Ld, mpo_d[site], Rd, Xd, precision=prec)
+                    acc = bsm_product(outl, Pd, Qd, [(0, 0, 0, 0, 0, 0.5 + 1j)],
+                                      accumulate_into=device_from_fam(C0), precision=prec)
+                    runs.append((got.data.cpu().numpy(), acc.data.cpu().numpy()))
+                assert all(np.array_equal(a, b) for a, b in zip(runs[0], runs[1])), (cfg, splits)
+                assert rel_err(fam_from_device(got), ref) <= tol, (cfg, splits)
+                dense = fam_from_device(acc).dense()[0]
+                assert np.abs(dense - ref2).max() <= tol * np.abs(ref2).max(), (cfg, splits)
+                if splits == 1:
+                    if unsplit is None:
+                        unsplit = runs[0]
+                    assert all(np.array_equal(a, b) for a, b in zip(unsplit, runs[0])), cfg
+    finally:
+        ctx.set_gemm_schedule(-1, 0)
+
+
+def test_autotuner_table(gpu, tmp_path):
+    """Tuning mode times the candidate schedules of each new plan class and records the winner;
+    the table survives save / clear / load, and use mode applies it without timing."""
+    from bondupdategalerkin_amd import _lib
+    from bondupdategalerkin_amd import environments as ev
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    L = 6
+    psi, _ = _chain(9, L, 100, "U1")
+    mpo_o = om.xxz_mpo(L, delta=0.5)
+    mpo_d = xxz_mpo(L, delta=0.5)
+    lenvs, renvs = _envs(psi, mpo_o)
+    dev = [(device_from_fam(lenvs[s]), device_from_fam(renvs[s + 1]), device_from_fam(psi[s]))
+           for s in range(L)]
+    refs = [ob.apply_one_site(lenvs[s], mpo_o[s], renvs[s + 1], psi[s]) for s in range(L)]
+    path = str(tmp_path / "schedules.txt")
+    _lib.autotune_clear()
+    try:
+        _lib.set_autotune(_lib.AUTOTUNE_TUNE)
+        for s in range(L):
+            got = ev.apply_one_site(dev[s][0], mpo_d[s], dev[s][1], dev[s][2])
+            assert rel_err(fam_from_device(got), refs[s]) <= TOL64, s
+        entries, tuned = _lib.autotune_count()
+        assert entries >= 1 and tuned == entries
+        _lib.autotune_save(path)
+        lines = [ln for ln in open(path) if not ln.startswith("#")]
+        assert len(lines) == entries
+        _lib.autotune_clear()
+        assert _lib.autotune_count()[0] == 0
+        _lib.autotune_load(path)
+        assert _lib.autotune_count() == (entries, tuned)
+        _lib.set_autotune(_lib.AUTOTUNE_USE)
+        for s in range(L):
+            got = ev.apply_one_site(dev[s][0], mpo_d[s], dev[s][1], dev[s][2])
+            assert rel_err(fam_from_device(got), refs[s]) <= TOL64, s
+        assert _lib.autotune_count() == (entries, tuned)      # use mode never times
+        with open(path, "a") as f:
+            f.write("zz 1\n")
+        with pytest.raises(ValueError):
+            _lib.autotune_load(path)
+        with pytest.raises(ValueError):
+            _lib.set_autotune(3)
+    finally:
+        _lib.set_autotune(_lib.AUTOTUNE_OFF)
+        _lib.autotune_clear()
