@@ -64,10 +64,11 @@ def parse():
                    help="Heston instances (2^10 x 2^10 grid) stepped through the serving loop "
                         "(0: skip)")
     p.add_argument("--heston-steps", type=int, default=1)
-    p.add_argument("--autotune", choices=["tune", "off"], default="tune",
-                   help="tune: the grouped-GEMM autotuner times candidate schedules during the "
-                        "kernel warm-up and the warm-up steps, the timed steps use its table; "
-                        "off: heuristic schedules")
+    p.add_argument("--autotune", choices=["table", "tune", "off"], default="table",
+                   help="table: grouped-GEMM schedules from the package's tuning table for "
+                        "gfx950 (heuristic for classes it lacks); tune: the autotuner times "
+                        "candidate schedules during the kernel warm-up and the warm-up steps "
+                        "and the timed steps use its table; off: heuristic schedules")
     p.add_argument("--save-schedules", default=None,
                    help="write the autotuner's table (plan class -> schedule) to this file")
     p.add_argument("--kernel-only", action="store_true",
@@ -108,6 +109,14 @@ def main():
     print(f"# state built in {time.time() - t0:.1f}s; bond dims max {max(psi.bond_dims())}",
           file=sys.stderr, flush=True)
 
+    if args.autotune == "table":
+        if os.path.exists(_lib.SCHEDULE_TABLE):
+            _lib.autotune_load(_lib.SCHEDULE_TABLE)
+            _lib.set_autotune(_lib.AUTOTUNE_USE)
+        else:
+            print(f"# no tuning table at {_lib.SCHEDULE_TABLE}: heuristic schedules",
+                  file=sys.stderr, flush=True)
+
     def tuning(on):
         if args.autotune == "tune":
             _lib.set_autotune(_lib.AUTOTUNE_TUNE if on else _lib.AUTOTUNE_USE)
@@ -135,6 +144,19 @@ def main():
         ends[r].record(stream)
     torch.cuda.synchronize()
     apply_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    # the same apply captured as a hipGraph and replayed (reported beside the eager time)
+    from bondupdategalerkin_amd.graphs import CapturedApply
+
+    cap = CapturedApply(lambda x: ev.apply_one_site(Lm, mpo[mid], Rm, x, precision=prec), X)
+    y_eager = ev.apply_one_site(Lm, mpo[mid], Rm, X, precision=prec)
+    graph_equal = bool(torch.equal(cap.replay().data, y_eager.data))
+    for r in range(args.kernel_reps):
+        starts[r].record(stream)
+        cap.replay()
+        ends[r].record(stream)
+    torch.cuda.synchronize()
+    cap.close()
+    apply_ms_graph = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
     achieved = flops_apply / (apply_ms * 1e-3) / 1e12
     peak = PEAK[ckind]
     # HBM bytes per apply, from the two PMC passes (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors
@@ -147,7 +169,10 @@ def main():
                 "kernel": (f"grouped_zgemm {ckind} (one-site H_eff apply: two grouped-GEMM "
                            "stages, each followed by its split-k reduce when split)"),
                 "schedules": [{"tile": list(TILES[c]), "splits": sp} for c, sp in schedules],
-                "flops_per_apply": flops_apply, "apply_ms": round(apply_ms, 4)}
+                "flops_per_apply": flops_apply, "apply_ms": round(apply_ms, 4),
+                "timing": "HIP events on the launch stream around each eager apply (its 2-3 "
+                          "launches and the gaps between them), mean of kernel_reps",
+                "apply_ms_graph": round(apply_ms_graph, 4), "graph_equals_eager": graph_equal}
     if args.kernel_only:
         print(json.dumps({"kernel_only": True, "roofline": roofline}), flush=True)
         return
@@ -292,12 +317,18 @@ def main():
                    "parallelism": "single GPU",
                    "budget_status": statuses},
         **extra,
-        "autotune": ({"mode": "tuned in the kernel warm-up and warm-up steps, table used in the "
-                              "timed steps", "classes": _lib.autotune_count()[0]}
-                     if args.autotune == "tune" else {"mode": "off (heuristic schedules)"}),
+        "autotune": {"mode": {"table": "package tuning table for gfx950 "
+                                        "(bondupdategalerkin_amd/schedules_gfx950.txt)",
+                              "tune": "tuned in the kernel warm-up and warm-up steps, table "
+                                      "used in the timed steps",
+                              "off": "heuristic schedules"}[args.autotune],
+                     "classes": _lib.autotune_count()[0]},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    st = ctx.stats()
+    out["plan_cache"] = {"plans": st["plans"], "hits": st["hits"], "misses": st["misses"],
+                         "context": "main thread"}
     if args.save_schedules:
         _lib.autotune_save(args.save_schedules)
     print(json.dumps(out), flush=True)

@@ -113,6 +113,9 @@ SIGNATURES = {
     "bug_autotune_save": (ctypes.c_int, [ctypes.c_char_p]),
     "bug_autotune_load": (ctypes.c_int, [ctypes.c_char_p]),
     "bug_last_schedules": (ctypes.c_int, [_P, ctypes.c_int32, _i32p, _i32p]),
+    "bug_graph_capture_begin": (ctypes.c_int, [_P, _P]),
+    "bug_graph_capture_end": (ctypes.c_int, [_P]),
+    "bug_graph_release": (ctypes.c_int, [_P, _P]),
     "bug_ctx_stats": (ctypes.c_int, [_P, _i64p]),
     "bug_last_op_flops": (ctypes.c_int, [_P, _f64p]),
     "bug_bsm_product": (
@@ -250,6 +253,8 @@ class Context:
 
 
 AUTOTUNE_OFF, AUTOTUNE_USE, AUTOTUNE_TUNE = 0, 1, 2
+# schedules the autotuner chose on MI355X for the XXZ L = 128, D = 1024 step (mixed and FP64)
+SCHEDULE_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "schedules_gfx950.txt")
 
 
 def set_autotune(mode: int):

@@ -283,13 +283,24 @@ void upload_plan(bug_ctx* ctx, Plan& plan, int config, hipStream_t stream) {
         plan.off_terms = align(bt) + align(bk);
     }
     if (plan.gen == ctx->plan_arena_gen) return;
+    if (capturing_on(ctx, stream))
+        throw Error(BUG_EINVAL, "a plan would be uploaded during graph capture: run the operation "
+                                "once on the capture stream before capturing");
     const size_t total = plan.image.size();
     if (ctx->plan_arena_used + total > ctx->plan_arena_bytes) {
-        HIP_CHECK(hipDeviceSynchronize());
+        const size_t prev = ctx->plan_arena_bytes;
+        if (ctx->graph_pins > 0) {
+            // live graphs hold addresses in this arena: keep it, start a new one
+            if (ctx->plan_arena) ctx->retired.push_back(ctx->plan_arena);
+            ctx->plan_arena = nullptr;
+            ctx->plan_arena_bytes = 0;
+        } else {
+            HIP_CHECK(hipDeviceSynchronize());
+        }
         if (total > ctx->plan_arena_bytes) {
             if (ctx->plan_arena) HIP_CHECK(hipFree(ctx->plan_arena));
             ctx->plan_arena = nullptr;
-            const size_t want = std::max<size_t>(std::max<size_t>(2 * ctx->plan_arena_bytes, 4 * total),
+            const size_t want = std::max<size_t>(std::max<size_t>(total > prev ? 2 * prev : prev, 4 * total),
                                                  (size_t)64 << 20);
             if (hipMalloc(&ctx->plan_arena, want) != hipSuccess) {
                 (void)hipGetLastError();
@@ -323,7 +334,16 @@ void run_product(bug_ctx* ctx, const Bsm& out, void* out_dev, const Bsm& P, cons
         h.bsm(vp);
         h.bsm(vq);
     }
-    for (const bug_term& t : terms) h.pod(t);
+    // field by field: bug_term has padding after op_b, whose bytes are unspecified
+    for (const bug_term& t : terms) {
+        h.pod(t.o);
+        h.pod(t.p);
+        h.pod(t.q);
+        h.pod(t.op_a);
+        h.pod(t.op_b);
+        h.pod(t.alpha_re);
+        h.pod(t.alpha_im);
+    }
     h.pod(accumulate);
     launch_cached_plan(ctx, h.h, [&](Plan& p) { build_product_plan(p, out, P, Q, terms, accumulate); },
                        precision, p_dev, q_dev, out_dev, stream);
@@ -368,9 +388,18 @@ void launch_cached_plan(bug_ctx* ctx, uint64_t key, const std::function<void(Pla
 void* scratch(bug_ctx* ctx, hipStream_t stream, size_t bytes, int slot) {
     Workspace& w = ctx->scratch[slot][(void*)stream];
     if (w.bytes < bytes) {
+        if (capturing_on(ctx, stream))
+            throw Error(BUG_EINVAL, "device scratch (slot " + std::to_string(slot) + ", " +
+                                        std::to_string(w.bytes) + " -> " + std::to_string(bytes) +
+                                        " bytes) would grow during graph capture: run the "
+                                        "operation once on the capture stream before capturing");
         if (w.ptr) {
-            HIP_CHECK(hipStreamSynchronize(stream));
-            HIP_CHECK(hipFree(w.ptr));
+            if (ctx->graph_pins > 0) {
+                ctx->retired.push_back(w.ptr);      // a live graph may still use it
+            } else {
+                HIP_CHECK(hipStreamSynchronize(stream));
+                HIP_CHECK(hipFree(w.ptr));
+            }
             w.ptr = nullptr;
             w.bytes = 0;
         }
@@ -463,6 +492,7 @@ int bug_ctx_destroy(bug_ctx* ctx) {
     if (ctx->svd_sweeps_dev) (void)hipFree(ctx->svd_sweeps_dev);
     for (hipEvent_t e : ctx->tune_ev)
         if (e) (void)hipEventDestroy(e);
+    for (void* p : ctx->retired) (void)hipFree(p);
     delete ctx;
     return BUG_OK;
 }
@@ -490,6 +520,45 @@ int bug_ctx_stats(const bug_ctx* ctx, int64_t* out4) {
     out4[2] = ctx->plan_hits;
     out4[3] = ctx->plan_misses;
     return BUG_OK;
+}
+
+int bug_graph_capture_begin(bug_ctx* ctx, void* stream) {
+    return guarded(ctx, [&] {
+        if (!stream) throw Error(BUG_EINVAL, "graph capture needs a non-default stream");
+        if (ctx->capture_active) throw Error(BUG_EINVAL, "a graph capture is already in progress");
+        ctx->capture_active = true;
+        ctx->capturing = as_stream(stream);
+        ++ctx->graph_pins;
+    });
+}
+
+int bug_graph_capture_end(bug_ctx* ctx) {
+    return guarded(ctx, [&] {
+        if (!ctx->capture_active) throw Error(BUG_EINVAL, "no graph capture in progress");
+        ctx->capture_active = false;
+        ctx->capturing = nullptr;
+    });
+}
+
+int bug_graph_release(bug_ctx* ctx, void* stream) {
+    return guarded(ctx, [&] {
+        if (ctx->graph_pins <= 0) throw Error(BUG_EINVAL, "no live graph to release");
+        if (ctx->capture_active) throw Error(BUG_EINVAL, "release during a graph capture");
+        HIP_CHECK(hipDeviceSynchronize());
+        // the capture stream's scratch belonged to the graph
+        if (stream) {
+            for (auto& slot : ctx->scratch) {
+                auto it = slot.find(stream);
+                if (it == slot.end()) continue;
+                if (it->second.ptr) ctx->retired.push_back(it->second.ptr);
+                slot.erase(it);
+            }
+        }
+        if (--ctx->graph_pins == 0) {
+            for (void* p : ctx->retired) HIP_CHECK(hipFree(p));
+            ctx->retired.clear();
+        }
+    });
 }
 
 int bug_last_schedules(const bug_ctx* ctx, int32_t cap, int32_t* out, int32_t* count) {

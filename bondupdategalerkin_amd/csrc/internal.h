@@ -193,7 +193,7 @@ struct bug_ctx {
     std::vector<bug::Schedule> last_schedules;   // grouped-GEMM launches of that call
     // per-stream scratch for intermediates of the two-stage contractions and the solvers,
     // keyed by (stream, slot)
-    std::unordered_map<void*, bug::Workspace> scratch[8];
+    std::unordered_map<void*, bug::Workspace> scratch[10];
     // SVD kernel choice (0 automatic, 1 plain one-sided Jacobi, 2 QR-preconditioned with one workgroup
     // per matrix, 3 QR-preconditioned multi-workgroup block Jacobi, 4 QR-preconditioned Gram block
     // Jacobi over row chunks) and an optional
@@ -217,9 +217,23 @@ struct bug_ctx {
     // pinned host staging for small reductions
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
+    // graph capture: the stream being captured (no allocation, upload, tuning or host wait may
+    // happen on it), the number of live graphs that hold device addresses of this context's
+    // plan arena and scratch (while it is non-zero those buffers are never freed or reused in
+    // place: a buffer that must grow or reset is retired and freed when the count drops to 0)
+    bool capture_active = false;
+    hipStream_t capturing = nullptr;
+    int graph_pins = 0;
+    std::vector<void*> retired;
 };
 
 namespace bug {
+// True while `stream` is being captured into a graph (the default stream is handle 0, so the
+// capture flag, not a null handle, says whether a capture is on).
+inline bool capturing_on(const bug_ctx* ctx, hipStream_t stream) {
+    return ctx->capture_active && ctx->capturing == stream;
+}
+
 // Run f under the context lock, mapping exceptions to status codes and messages.
 template <typename F>
 int guarded(bug_ctx* ctx, F&& f) {

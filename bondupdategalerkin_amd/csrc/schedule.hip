@@ -32,6 +32,7 @@ namespace {
 // of the winner and of the heuristic's choice (zero for entries loaded from a file).
 struct TuneEntry {
     Schedule s;
+    int precision = -1;
     double m = 0, n = 0, k = 0, tasks = 0, flops = 0;
     float best_ms = 0.0f, heuristic_ms = 0.0f;
 };
@@ -122,7 +123,7 @@ TuneEntry tune(bug_ctx* ctx, Plan& plan, int precision, const void* A, const voi
         HIP_CHECK(hipEventCreate(&ctx->tune_ev[0]));
         HIP_CHECK(hipEventCreate(&ctx->tune_ev[1]));
     }
-    void* Cs = scratch(ctx, stream, sizeof(double2) * (size_t)output_extent(plan), 7);
+    void* Cs = scratch(ctx, stream, sizeof(double2) * (size_t)output_extent(plan), 8);
     const Schedule heur = heuristic_schedule(plan);
     Schedule best = heur;
     float best_ms = 1e30f, heur_ms = 0.0f;
@@ -134,7 +135,7 @@ TuneEntry tune(bug_ctx* ctx, Plan& plan, int precision, const void* A, const voi
             if (split > 1) part = scratch(ctx, stream, split_partial_bytes(plan, c, split, precision), 6);
             HIP_CHECK(launch_grouped_gemm(plan, c, split, precision, A, B, Cs, part, stream));
             HIP_CHECK(hipEventRecord(ctx->tune_ev[0], stream));
-            constexpr int kReps = 2;
+            constexpr int kReps = 3;
             for (int r = 0; r < kReps; ++r)
                 HIP_CHECK(launch_grouped_gemm(plan, c, split, precision, A, B, Cs, part, stream));
             HIP_CHECK(hipEventRecord(ctx->tune_ev[1], stream));
@@ -151,6 +152,7 @@ TuneEntry tune(bug_ctx* ctx, Plan& plan, int precision, const void* A, const voi
     }
     TuneEntry e;
     e.s = best;
+    e.precision = precision;
     class_stats(plan, e.m, e.n, e.k);
     e.tasks = (double)plan.tasks.size();
     e.flops = plan.flops;
@@ -182,7 +184,8 @@ Schedule pick_schedule(bug_ctx* ctx, Plan& plan, int precision, const void* A, c
                     found = true;
                 }
             }
-            if (!found && mode == 2) {
+            // no timing (it waits on the stream) while a graph is being captured on it
+            if (!found && mode == 2 && !capturing_on(ctx, stream)) {
                 const TuneEntry e = tune(ctx, plan, precision, A, B, stream);
                 std::lock_guard<std::mutex> lk(tt.mu);
                 // the first decision recorded for a class wins (a concurrent tuner may have
@@ -247,12 +250,12 @@ int bug_autotune_save(const char* path) {
     FILE* f = std::fopen(path, "w");
     if (!f) return BUG_EINVAL;
     std::fprintf(f, "# grouped-GEMM schedules: plan class, tile configuration, split factor "
-                    "# mean m, n, k, tasks, flops, best ms, heuristic ms\n");
+                    "# precision, mean m, n, k, tasks, flops, best ms, heuristic ms\n");
     for (const auto& e : tt.entries) {
         const TuneEntry& t = e.second;
-        std::fprintf(f, "%016llx %d %d # %.0f %.0f %.0f %.0f %.4g %.4f %.4f\n",
-                     (unsigned long long)e.first, t.s.config, t.s.splits, t.m, t.n, t.k, t.tasks,
-                     t.flops, t.best_ms, t.heuristic_ms);
+        std::fprintf(f, "%016llx %d %d # %d %.0f %.0f %.0f %.0f %.4g %.4f %.4f\n",
+                     (unsigned long long)e.first, t.s.config, t.s.splits, t.precision, t.m, t.n,
+                     t.k, t.tasks, t.flops, t.best_ms, t.heuristic_ms);
     }
     const bool ok = std::fclose(f) == 0;
     return ok ? BUG_OK : BUG_EINVAL;

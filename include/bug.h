@@ -135,6 +135,17 @@ int bug_autotune_count(int64_t* out2);
 int bug_autotune_clear(void);
 int bug_autotune_save(const char* path);
 int bug_autotune_load(const char* path);
+/* Graph capture of entry-point calls (no counterpart in the reference, whose Julia calls run
+ * eagerly).  Between begin and end the caller captures `stream` (hipStreamBeginCapture or a
+ * framework's graph object); calls on that stream then refuse (BUG_EINVAL) anything that cannot
+ * be captured: a plan upload, scratch growth, autotuner timing -- run the same call once on the
+ * stream before capturing.  From begin until the matching release the context keeps every plan
+ * image and scratch buffer that a replay may touch alive (buffers that must grow or reset are
+ * retired, not freed).  Release (after the graph is destroyed or will not be replayed again)
+ * synchronizes the device and drops the capture stream's scratch. */
+int bug_graph_capture_begin(bug_ctx* ctx, void* stream);
+int bug_graph_capture_end(bug_ctx* ctx);
+int bug_graph_release(bug_ctx* ctx, void* stream);
 /* The (tile configuration, split factor) pairs of the grouped-GEMM launches of the last
  * entry-point call, in launch order: min(cap, count) pairs into out[2 cap], the total in *count. */
 int bug_last_schedules(const bug_ctx* ctx, int32_t cap, int32_t* out, int32_t* count);
