@@ -162,8 +162,8 @@ def main():
     # HBM bytes per apply, from the two PMC passes (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors
     # from a plain copy of known size) recorded in profiles/heff_apply_L128_D1024_mixed.md; only
     # for the shape and form they were measured on
-    traffic = (105799 * 1024 if (ckind == "fp32" and L == 128 and D == 1024 and args.delta == 0.5)
-               else None)
+    traffic = (126611 * 1024 if (ckind == "fp32" and L == 128 and D == 1024 and args.delta == 0.5
+                                 and args.autotune == "table") else None)
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
                 "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
                 "kernel": (f"grouped_zgemm {ckind} (one-site H_eff apply: two grouped-GEMM "

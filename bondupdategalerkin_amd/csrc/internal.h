@@ -182,7 +182,7 @@ struct bug_ctx {
     // plan cache: signature hash -> plan (LRU)
     std::unordered_map<uint64_t, std::list<std::pair<uint64_t, bug::Plan*>>::iterator> plan_index;
     std::list<std::pair<uint64_t, bug::Plan*>> plan_lru;
-    size_t plan_capacity = 512;
+    size_t plan_capacity = 4096;
     int64_t plan_hits = 0, plan_misses = 0;
     // device arena of plan images: bump-allocated, copied in stream order without blocking the
     // host; when full the device is synchronized, the arena reset and its generation advanced
