@@ -492,6 +492,8 @@ int bug_ctx_destroy(bug_ctx* ctx) {
     if (ctx->svd_sweeps_dev) (void)hipFree(ctx->svd_sweeps_dev);
     for (hipEvent_t e : ctx->tune_ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->sync_ev)
+        if (e) (void)hipEventDestroy(e);
     for (void* p : ctx->retired) (void)hipFree(p);
     delete ctx;
     return BUG_OK;

@@ -178,6 +178,7 @@ struct bug_ctx {
     int forced_config = -1;
     int forced_splits = 0;       // 0: the schedule's own split factor
     hipEvent_t tune_ev[2] = {nullptr, nullptr};
+    hipEvent_t sync_ev[2] = {nullptr, nullptr};   // host waits of the Jacobi sweep loop
     std::mutex mu;
     // plan cache: signature hash -> plan (LRU)
     std::unordered_map<uint64_t, std::list<std::pair<uint64_t, bug::Plan*>>::iterator> plan_index;

@@ -1710,7 +1710,12 @@ void block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list,
         if (lds > 65536)
             HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(bj_round_kernel),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        int* hp = reinterpret_cast<int*>(pinned(ctx, sizeof(int) * nmat));
+        // The convergence read of sweep s overlaps sweep s + 1, which is launched before it:
+        // if sweep s rotated nothing, every round of sweep s + 1 returns at once (its matrices'
+        // flags from sweep s are 0) and writes no flag, so the extra sweep changes nothing.
+        int* hp = reinterpret_cast<int*>(pinned(ctx, sizeof(int) * (size_t)nmat * nrows));
+        for (hipEvent_t& e : ctx->sync_ev)
+            if (!e) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         for (int s = 0; s < max_sweeps; ++s) {
             for (int r = 0; r < rounds; ++r)
                 hipLaunchKernelGGL(bj_round_kernel, dim3((unsigned)tasks.size()), dim3(kBjThreads), lds,
@@ -1719,13 +1724,17 @@ void block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list,
                                    std::max(1, ctx->bj_inner));
             HIP_CHECK(hipGetLastError());
             ++used;
-            HIP_CHECK(hipMemcpyAsync(hp, flags + (size_t)(s + 1) * nmat, sizeof(int) * nmat,
-                                     hipMemcpyDeviceToHost, st));
-            HIP_CHECK(hipStreamSynchronize(st));
+            HIP_CHECK(hipMemcpyAsync(hp + (size_t)s * nmat, flags + (size_t)(s + 1) * nmat,
+                                     sizeof(int) * nmat, hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipEventRecord(ctx->sync_ev[s & 1], st));
+            if (s == 0) continue;
+            HIP_CHECK(hipEventSynchronize(ctx->sync_ev[(s - 1) & 1]));
             bool any = false;
-            for (int i = 0; i < nmat; ++i) any |= hp[i] != 0;
+            for (int i = 0; i < nmat; ++i) any |= hp[(size_t)(s - 1) * nmat + i] != 0;
             if (!any) break;
         }
+        // the last sweep's read (or the speculative sweep) must land before the flags are reused
+        HIP_CHECK(hipEventSynchronize(ctx->sync_ev[(used - 1) & 1]));
     }
     if (sweeps_dev) {
         // sweeps per matrix: the sweeps that rotated plus the one that found it converged

@@ -9,6 +9,10 @@ instead of generating them.
 """
 from __future__ import annotations
 
+import os
+import threading
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 _MUL1 = np.uint64(0xBF58476D1CE4E5B9)
@@ -24,6 +28,35 @@ def _mix(x: np.ndarray) -> np.ndarray:
         return z ^ (z >> np.uint64(31))
 
 
+def _values(key: np.uint64, i: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        two_i = np.uint64(2) * i + key
+        a = _mix(two_i)
+        b = _mix(two_i + np.uint64(1))
+    u1 = ((a >> np.uint64(11)).astype(np.float64) + 0.5) / 9007199254740992.0
+    u2 = (b >> np.uint64(11)).astype(np.float64) / 9007199254740992.0
+    rad = np.sqrt(-np.log(u1))
+    ang = 2.0 * np.pi * u2
+    return rad * np.cos(ang) + 1j * rad * np.sin(ang)
+
+
+# Large draws are cut into chunks evaluated by a small thread pool (numpy's element-wise loops
+# release the GIL); every number depends only on its own position, so the chunking does not
+# change any value.
+_CHUNK = 1 << 15
+_pool = None
+_pool_lock = threading.Lock()
+
+
+def _executor() -> ThreadPoolExecutor:
+    global _pool
+    with _pool_lock:
+        if _pool is None:
+            _pool = ThreadPoolExecutor(max_workers=max(1, min(8, (os.cpu_count() or 1) // 2)),
+                                       thread_name_prefix="probes")
+        return _pool
+
+
 class ProbeStream:
     """Seeded, skippable stream of complex Gaussian numbers."""
 
@@ -33,15 +66,18 @@ class ProbeStream:
 
     def at(self, i: np.ndarray) -> np.ndarray:
         """The numbers at stream positions ``i`` (uint64), without moving the stream."""
-        with np.errstate(over="ignore"):
-            two_i = np.uint64(2) * i + self._key
-            a = _mix(two_i)
-            b = _mix(two_i + np.uint64(1))
-        u1 = ((a >> np.uint64(11)).astype(np.float64) + 0.5) / 9007199254740992.0
-        u2 = (b >> np.uint64(11)).astype(np.float64) / 9007199254740992.0
-        rad = np.sqrt(-np.log(u1))
-        ang = 2.0 * np.pi * u2
-        return rad * np.cos(ang) + 1j * rad * np.sin(ang)
+        i = np.asarray(i, dtype=np.uint64)
+        n = i.size
+        if n <= 2 * _CHUNK:
+            return _values(self._key, i)
+        flat = i.reshape(-1)
+        out = np.empty(n, dtype=np.complex128)
+
+        def fill(lo):
+            out[lo: lo + _CHUNK] = _values(self._key, flat[lo: lo + _CHUNK])
+
+        list(_executor().map(fill, range(0, n, _CHUNK)))
+        return out.reshape(i.shape)
 
     def draw(self, n: int) -> np.ndarray:
         vals = self.at(np.arange(self.position, self.position + n, dtype=np.uint64))

@@ -50,3 +50,21 @@ def test_parallel_oracle_step_uses_split_stream():
         status.append(info.ledger.status)
     d = np.linalg.norm(out[0] - out[1]) / np.linalg.norm(out[0])
     assert d < 1e-4 and status[0] == status[1] == "met"
+
+
+def test_large_draws_are_chunk_independent():
+    """A draw large enough to be evaluated in chunks by the thread pool equals the oracle's
+    stream bit for bit, and so does every chunk-sized piece drawn alone."""
+    from bondupdategalerkin_amd import probes
+
+    n = 5 * probes._CHUNK + 123
+    dev = probes.ProbeStream(77)
+    ref = ProbeRNG(77)
+    got = dev.draw(n)
+    want = ref.complex_normal(n)
+    assert np.array_equal(got, want)
+    idx = np.arange(n, dtype=np.uint64)[::-1].copy()
+    assert np.array_equal(dev.at(idx), want[::-1])
+    small = probes.ProbeStream(77)
+    parts = [small.draw(1000) for _ in range(n // 1000)]
+    assert np.array_equal(np.concatenate(parts), want[: 1000 * (n // 1000)])
