@@ -29,6 +29,16 @@ import torch
 
 from . import _lib
 
+if hasattr(torch._C, "_cuda_getCurrentRawStream"):
+    def current_stream_handle() -> int:
+        """Raw handle of torch's current stream on the current device, the value of
+        ``torch.cuda.current_stream().cuda_stream`` without building a Stream object (the
+        entry-point wrappers ask for it on every call)."""
+        return torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice())
+else:                                                    # pragma: no cover
+    def current_stream_handle() -> int:
+        return int(torch.cuda.current_stream().cuda_stream)
+
 
 @dataclass(frozen=True)
 class Leg:
@@ -242,9 +252,7 @@ def bsm_product(out_layout: Layout, P: BlockTensor, Q: BlockTensor, terms, accum
     for n, (o, p, q, oa, ob, alpha) in enumerate(terms):
         alpha = complex(alpha)
         arr[n] = _lib.BugTerm(o, p, q, oa, ob, alpha.real, alpha.imag)
-    import torch as _t
     ctx.check(ctx._lib.bug_bsm_product(
         ctx.handle, out.desc(), P.desc(), Q.desc(), len(terms), arr,
-        1 if accumulate_into is not None else 0, precision,
-        int(_t.cuda.current_stream().cuda_stream)))
+        1 if accumulate_into is not None else 0, precision, current_stream_handle()))
     return out

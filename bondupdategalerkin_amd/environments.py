@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .blocks import BlockTensor, Layout, Leg
+from .blocks import BlockTensor, Layout, Leg, current_stream_handle
 from .mpo import MPO, MPOSite
 from .mps import SymMPS
 
@@ -43,7 +43,7 @@ def boundary_right(mpo: MPO, leg: Leg) -> BlockTensor:
 
 
 def _stream():
-    return ctypes_stream(torch.cuda.current_stream())
+    return current_stream_handle()
 
 
 def ctypes_stream(s) -> int:

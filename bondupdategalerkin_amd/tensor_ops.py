@@ -26,7 +26,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .blocks import BlockTensor, Layout, Leg, bsm_product, device
+from .blocks import BlockTensor, Layout, Leg, bsm_product, current_stream_handle, device
 
 OP_N, OP_T, OP_C, OP_R = _lib.OP_N, _lib.OP_T, _lib.OP_C, _lib.OP_R
 CBE_EPS = 100 * np.finfo(float).eps
@@ -41,7 +41,7 @@ def _ctx():
 
 
 def _stream() -> int:
-    return int(torch.cuda.current_stream().cuda_stream)
+    return current_stream_handle()
 
 
 def _ptr(t: torch.Tensor) -> int:
