@@ -880,15 +880,13 @@ def krylov_enrich(seed, applyH, start_frame, side, tau, alloc: _Alloc, workcap: 
             T.axpy_(-a.real, vs[k - 1], img)
             if k >= 20 and k % 2 == 0:
                 cs = T.inner_many(vs, img)
-                for c, v in zip(cs, vs):
-                    T.axpy_(-c, v, img)
+                T.axpy_many_([-c for c in cs], vs, img)
         else:
             for _p in range(2):
                 cs = T.inner_many(vs, img)
                 for row, c in enumerate(cs):
                     H[row, k - 1] += c
-                for c, v in zip(cs, vs):
-                    T.axpy_(-c, v, img)
+                T.axpy_many_([-c for c in cs], vs, img)
         beta = T.norm(img)
         if beta <= K.BREAKDOWN:
             closed = True

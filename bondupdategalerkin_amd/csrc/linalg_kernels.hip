@@ -84,6 +84,27 @@ struct DotPtrs {
     const double2* p[kMaxDots];
 };
 
+struct AxpyCoeffs {
+    double2 a[kMaxDots];
+};
+
+// y += sum_j a_j x_j, the terms added one after another in j order, each as zaxpy_kernel adds it
+// (the same per-element arithmetic as k zaxpy launches, in one pass over y).
+__global__ void zaxpy_many_kernel(int64_t n, int k, AxpyCoeffs c, DotPtrs xs,
+                                  double2* __restrict__ y) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double2 w = y[i];
+        for (int j = 0; j < k; ++j) {
+            const double ar = c.a[j].x, ai = c.a[j].y;
+            const double2 v = xs.p[j][i];
+            w.x += ar * v.x - ai * v.y;
+            w.y += ar * v.y + ai * v.x;
+        }
+        y[i] = w;
+    }
+}
+
 // partial[j * nblk + b] = sum over block b's rows of conj(x_j) y  (re, im interleaved)
 __global__ __launch_bounds__(kDotThreads) void multidot_partial(int64_t n, int k, DotPtrs xs,
                                                                  const double2* __restrict__ y,
@@ -1842,6 +1863,25 @@ int bug_scal(bug_ctx* ctx, int64_t n, double ar, double ai, void* x, void* strea
         if (n == 0) return;
         hipLaunchKernelGGL(zscal_kernel, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), n,
                            ar, ai, static_cast<double2*>(x));
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int bug_axpy_many(bug_ctx* ctx, int64_t n, int32_t k, const double* alphas, const void* const* xs,
+                  void* y, void* stream) {
+    return guarded(ctx, [&] {
+        if (n < 0 || k < 0 || k > kMaxDots || (k > 0 && (!alphas || !xs)) || (n > 0 && k > 0 && !y))
+            throw Error(BUG_EINVAL, "bad axpy_many arguments (k must be <= 64)");
+        if (n == 0 || k == 0) return;
+        AxpyCoeffs c;
+        DotPtrs p;
+        for (int j = 0; j < k; ++j) {
+            if (!xs[j]) throw Error(BUG_EINVAL, "null vector in axpy_many");
+            c.a[j] = double2{alphas[2 * j], alphas[2 * j + 1]};
+            p.p[j] = static_cast<const double2*>(xs[j]);
+        }
+        hipLaunchKernelGGL(zaxpy_many_kernel, dim3(grid_for(n, 256)), dim3(256), 0,
+                           as_stream(stream), n, k, c, p, static_cast<double2*>(y));
         HIP_CHECK(hipGetLastError());
     });
 }

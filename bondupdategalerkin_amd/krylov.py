@@ -31,10 +31,11 @@ MAX_NORM = 1e150
 def residual_integral(index: int, M: np.ndarray, tau: complex, relative: bool) -> float:
     """|tau| int_0^1 |(exp(sM) e1)[index]| ds by composite Simpson in log space.
 
-    The nodes c_j = S^j e1 (S = exp(h (M - sigma))) are produced a block at a time: with the
-    powers S, S^2, ..., S^B at hand, the integrand at the next B nodes is row `index` of those
-    powers times the block's first node, and the next block starts from S^B times it, so a stiff
-    generator's thousands of panels cost a few hundred small products, not one per node.  The integrand and the Simpson weights are those of the node-by-node recurrence;
+    The nodes c_j = S^j e1 (S = exp(h (M - sigma))) are produced a block of B = 64 at a time:
+    with row `index` of S, S^2, ..., S^B at hand (vector-matrix products) and S^B (by squaring),
+    the integrand at the next B nodes is those rows times the block's first node, and the next
+    block starts from S^B times it, so a stiff generator's thousands of panels cost a few hundred
+    small products, not one per node.  The integrand and the Simpson weights are those of the node-by-node recurrence;
     the nodes agree with it to rounding."""
     n = M.shape[0]
     unitary = np.linalg.norm(M + M.conj().T) <= SKEW_TOL * np.linalg.norm(M)
@@ -43,22 +44,28 @@ def residual_integral(index: int, M: np.ndarray, tau: complex, relative: bool) -
     panels = 2 * min(max(math.ceil(2 * np.abs(A).sum(axis=0).max()), 8), MAX_PANELS)
     h = 1.0 / panels
     step = sla.expm(h * A)
-    B = min(panels, 128 if n <= 32 else 32)
-    powers = np.empty((B, n, n), dtype=complex)
-    powers[0] = step
+    B = 64 if panels >= 64 else 1
+    # row `index` of S^1 .. S^B (vector-matrix products) and S^B (by squaring)
+    rows = np.empty((B, n), dtype=complex)
+    rows[0] = step[index]
     for b in range(1, B):
-        powers[b] = step @ powers[b - 1]
-    rows = np.ascontiguousarray(powers[:, index, :])      # row `index` of S^1 .. S^B
+        rows[b] = rows[b - 1] @ step
+    big = step
+    for _ in range(B.bit_length() - 1):
+        big = big @ big
     mag = np.empty(panels + 1)
     c = np.zeros(n, dtype=complex)
     c[0] = 1.0
     mag[0] = abs(c[index])
     j = 0
-    while j < panels:
-        nb = min(B, panels - j)
-        mag[j + 1: j + 1 + nb] = np.abs(rows[:nb] @ c)
-        c = powers[nb - 1] @ c
-        j += nb
+    while j + B <= panels:
+        mag[j + 1: j + 1 + B] = np.abs(rows @ c)
+        c = big @ c
+        j += B
+    while j < panels:                                     # the last partial block, node by node
+        c = step @ c
+        j += 1
+        mag[j] = abs(c[index])
     w = np.full(panels + 1, 2.0)
     w[1::2] = 4.0
     w[0] = w[panels] = 1.0
@@ -216,8 +223,7 @@ def lanczos_expv(apply, tau: complex, v0, *, tol: float = 1e-15, maxiter: int = 
             if reorth:
                 for _p in range(2):
                     cs = T.inner_many(basis, r)
-                    for c, p in zip(cs, basis):
-                        T.axpy_(-c, p, r)
+                    T.axpy_many_([-c for c in cs], basis, r)
         tri = Tridiagonal(alpha, beta)
         betak = T.norm(r)
         if substeps == 1:
@@ -229,8 +235,8 @@ def lanczos_expv(apply, tau: complex, v0, *, tol: float = 1e-15, maxiter: int = 
         coeff, ls = tri.coeffs(tau_s)
         err += tail_sup(tri, tau_s, betak) * cn / beta0
         out = T.scale(coeff[0] * cn, basis[0])
-        for c, p in zip(coeff[1:], basis[1:]):
-            T.axpy_(c * cn, p, out)
+        m = min(len(coeff), len(basis))
+        T.axpy_many_([c * cn for c in coeff[1:m]], basis[1:m], out)
         cur = out
         logscale += ls
         tau_rem -= tau_s
@@ -276,8 +282,7 @@ def arnoldi_expv(apply, tau: complex, v0, *, tol: float = 1e-13, maxiter: int = 
             for _p in range(2):
                 cs = T.inner_many(basis, r)
                 H[:len(basis), k - 1] += cs
-                for c, p in zip(cs, basis):
-                    T.axpy_(-c, p, r)
+                T.axpy_many_([-c for c in cs], basis, r)
             dim = k
             betak = T.norm(r)
             if betak < tol:
@@ -299,8 +304,7 @@ def arnoldi_expv(apply, tau: complex, v0, *, tol: float = 1e-13, maxiter: int = 
         coeff, ls = hess.coeffs(tau_s)
         err += tail_sup(hess, tau_s, betak) * cn / beta0
         out = T.scale(coeff[0] * cn, basis[0])
-        for c, p in zip(coeff[1:], basis[1:dim]):
-            T.axpy_(c * cn, p, out)
+        T.axpy_many_([c * cn for c in coeff[1:dim]], basis[1:dim], out)
         cur = out
         logscale += ls
         tau_rem -= tau_s

@@ -156,6 +156,28 @@ def axpy_(alpha: complex, x: BlockTensor, y: BlockTensor) -> BlockTensor:
     return y
 
 
+def axpy_many_(alphas: Sequence[complex], xs: Sequence[BlockTensor], y: BlockTensor) -> BlockTensor:
+    """y += sum_j alphas[j] xs[j] in place, the terms added in order (64 per launch), with the
+    per-element arithmetic of that many ``axpy_`` calls."""
+    if len(alphas) != len(xs):
+        raise ValueError("axpy_many_: as many coefficients as vectors")
+    for x in xs:
+        _same(x, y)
+    ctx = _ctx()
+    for lo in range(0, len(xs), 64):
+        chunk = xs[lo: lo + 64]
+        k = len(chunk)
+        coef = np.empty(2 * k)
+        for j, a in enumerate(alphas[lo: lo + 64]):
+            a = complex(a)
+            coef[2 * j], coef[2 * j + 1] = a.real, a.imag
+        ptrs = (ctypes.c_void_p * k)(*[_ptr(x.data) for x in chunk])
+        ctx.check(ctx._lib.bug_axpy_many(ctx.handle, y.layout.nelem, k,
+                                         coef.ctypes.data_as(_lib._f64p), ptrs, _ptr(y.data),
+                                         _stream()))
+    return y
+
+
 def axpy(alpha: complex, x: BlockTensor, y: BlockTensor) -> BlockTensor:
     """y + alpha x (new tensor)."""
     return axpy_(alpha, x, y.clone())

@@ -203,6 +203,11 @@ int bug_apply_zero_site(bug_ctx* ctx, const bug_bsm* L, const bug_bsm* R, const 
 
 /* y += (ar + i ai) x over n complex elements. */
 int bug_axpy(bug_ctx* ctx, int64_t n, double ar, double ai, const void* x, void* y, void* stream);
+/* y += sum_j (alphas[2j] + i alphas[2j+1]) xs[j] for j < k <= 64, added in j order (one launch
+ * for the Gram-Schmidt subtraction against a whole basis and the basis recombination,
+ * _cgs2!/_combine in krylov.jl:296-332). */
+int bug_axpy_many(bug_ctx* ctx, int64_t n, int32_t k, const double* alphas, const void* const* xs,
+                  void* y, void* stream);
 /* x *= (ar + i ai). */
 int bug_scal(bug_ctx* ctx, int64_t n, double ar, double ai, void* x, void* stream);
 /* out[2j], out[2j+1] = re, im of <xs[j] | y> for j < k <= 64 (host output; synchronizes the

@@ -236,3 +236,31 @@ def test_pivoted_thin_qr_matches_oracle(gpu, side, seed, dmax, rank):
     df = fam_from_device(d)
     assert (df.col if side == "left" else df.row) == (o.col if side == "left" else o.row)
     assert rel_err(df, o) < 1e-10
+
+
+def test_axpy_many_equals_chained_axpys(gpu):
+    """The fused y += sum_j a_j x_j (70 vectors: two launches) against one axpy per vector."""
+    import torch
+
+    from bondupdategalerkin_amd import tensor_ops as T
+    from bondupdategalerkin_amd.blocks import BlockTensor, Layout, Leg
+
+    rng = np.random.default_rng(12)
+    lay = Layout(Leg.make([(0, 37), (2, 5)]), Leg.make([(1, 19), (3, 40)]), [1])
+    def rand():
+        blocks = {}
+        for k, q, qc, d, dc, off in lay.blocks():
+            blocks[(k, q)] = rng.standard_normal((d, dc)) + 1j * rng.standard_normal((d, dc))
+        return BlockTensor.from_blocks(lay, blocks)
+    xs = [rand() for _ in range(70)]
+    alphas = list(rng.standard_normal(70) + 1j * rng.standard_normal(70))
+    y0 = rand()
+    fused = T.axpy_many_(alphas, xs, y0.clone())
+    chained = y0.clone()
+    for a, x in zip(alphas, xs):
+        T.axpy_(a, x, chained)
+    torch.cuda.synchronize()
+    diff = (fused.data - chained.data).abs().max().item()
+    assert diff <= 1e-14 * chained.data.abs().max().item()
+    with pytest.raises(ValueError):
+        T.axpy_many_(alphas[:3], xs[:2], y0)
