@@ -159,6 +159,7 @@ SIGNATURES = {
     "bug_set_qr_variant": (ctypes.c_int, [_P, ctypes.c_int32]),
     "bug_set_jacobi_params": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     "bug_set_gram_jacobi_params": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "bug_set_mixed_jacobi_params": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     "bug_svd_sweeps": (ctypes.c_int, [_P, ctypes.c_int32, _i32p]),
     "bug_block_svd": (
         ctypes.c_int,

@@ -213,6 +213,10 @@ struct bug_ctx {
     int gbj_cols = 32;
     int gbj_inner = 1;
     int gbj_min_n = 1 << 30;
+    // FP32-preconditioned block Jacobi (svd_variant 5): FP32 sweeps at most, and the smallest
+    // matrix dimension of a batch for which the automatic choice (variant 0) takes it
+    int mixed_fp32_sweeps = 12;
+    int mixed_min_n = 1 << 30;
     // dynamic LDS bytes a workgroup may use on this device (queried on first use)
     int lds_limit = 0;
     // pinned host staging for small reductions

@@ -46,6 +46,36 @@ __device__ __forceinline__ double wave_sum(double v) {
     return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
+template <int CTRL>
+__device__ __forceinline__ float dpp_movf(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float read_lanef(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// The FP32 wave sum, with the reduction pattern of the FP64 one.
+__device__ __forceinline__ float wave_sum(float v) {
+    v += dpp_movf<0xb1>(v);
+    v += dpp_movf<0x4e>(v);
+    v += dpp_movf<0x141>(v);
+    v += dpp_movf<0x140>(v);
+    return (read_lanef(v, 0) + read_lanef(v, 16)) + (read_lanef(v, 32) + read_lanef(v, 48));
+}
+
+// complex element type of a real scalar type
+template <typename R>
+struct Cplx;
+template <>
+struct Cplx<double> {
+    typedef double2 type;
+};
+template <>
+struct Cplx<float> {
+    typedef float2 type;
+};
+
 __device__ __forceinline__ double wave_max(double v) {
     v = fmax(v, dpp_mov<0xb1>(v));
     v = fmax(v, dpp_mov<0x4e>(v));
@@ -1113,15 +1143,21 @@ __device__ __forceinline__ void round_robin(int N, int round, int pi, int& p, in
     }
 }
 
+// R = double (the FP64 Jacobi) or float (the FP32 preconditioning sweeps of the mixed variant):
+// columns, rotations and the threshold eps in R.
+template <typename R>
 __global__ __launch_bounds__(kBjThreads) void bj_round_kernel(const BjMat* __restrict__ mats,
                                                               const int2* __restrict__ tasks,
-                                                              double2* __restrict__ X,
-                                                              double2* __restrict__ V, int round,
+                                                              typename Cplx<R>::type* __restrict__ X,
+                                                              typename Cplx<R>::type* __restrict__ V,
+                                                              int round,
                                                               const int* __restrict__ flags_prev,
                                                               int* __restrict__ flags_cur,
-                                                              double eps, int inner) {
-    extern __shared__ double2 xs[];           // n * c staged columns, then c * c Z
-    __shared__ double red[kBjThreads / 64][4];
+                                                              R eps, int inner) {
+    typedef typename Cplx<R>::type C2;
+    extern __shared__ __align__(16) unsigned char bj_lds[];
+    C2* xs = reinterpret_cast<C2*>(bj_lds);   // n * c staged columns, then c * c Z
+    __shared__ R red[kBjThreads / 64][4];
     __shared__ int s_rot;
     const int2 tk = tasks[blockIdx.x];
     const BjMat mt = mats[tk.x];
@@ -1137,21 +1173,21 @@ __global__ __launch_bounds__(kBjThreads) void bj_round_kernel(const BjMat* __res
     // round 0 of a sweep rotates every pair of the staged columns (so every pair of the matrix,
     // intra-block ones included, is visited once per sweep); later rounds only the cross pairs
     const bool cross_only = round > 0 && cp > 0 && cq > 0;
-    double2* x = X + mt.x_off;
-    double2* v = V + mt.v_off;
-    double2* Z = xs + (int64_t)n * c;
+    C2* x = X + mt.x_off;
+    C2* v = V + mt.v_off;
+    C2* Z = xs + (int64_t)n * c;
     auto gcol = [&](int j) { return j < cp ? bp * w + j : bq * w + (j - cp); };
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr int nw = kBjThreads / 64;
     for (int j = wave; j < c; j += nw) {
-        const double2* src = x + (int64_t)gcol(j) * n;
-        double2* dst = xs + (int64_t)j * n;
+        const C2* src = x + (int64_t)gcol(j) * n;
+        C2* dst = xs + (int64_t)j * n;
         for (int i = lane; i < n; i += 64) dst[i] = src[i];
     }
-    for (int e = tid; e < c * c; e += kBjThreads) Z[e] = double2{(e % c) == (e / c) ? 1.0 : 0.0, 0.0};
+    for (int e = tid; e < c * c; e += kBjThreads) Z[e] = C2{(e % c) == (e / c) ? R(1) : R(0), R(0)};
     if (tid == 0) s_rot = 0;
     __syncthreads();
-    const double tol2 = eps * eps * (double)n;
+    const R tol2 = eps * eps * (R)n;
     // pairs per inner round: round-robin over all c columns, or the cyclic shifts of the smaller
     // block against the larger
     const int big = cp >= cq ? cp : cq, small = cp >= cq ? cq : cp;
@@ -1179,12 +1215,12 @@ __global__ __launch_bounds__(kBjThreads) void bj_round_kernel(const BjMat* __res
                         valid = b < c;
                     }
                 }
-                double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+                R al = 0, be = 0, gr = 0, gi = 0;
                 if (valid) {
-                    const double2* xa = xs + (int64_t)a * n;
-                    const double2* xb = xs + (int64_t)b * n;
+                    const C2* xa = xs + (int64_t)a * n;
+                    const C2* xb = xs + (int64_t)b * n;
                     for (int i = gtid; i < n; i += gthreads) {
-                        const double2 p = xa[i], q = xb[i];
+                        const C2 p = xa[i], q = xb[i];
                         al += p.x * p.x + p.y * p.y;
                         be += q.x * q.x + q.y * q.y;
                         gr += p.x * q.x + p.y * q.y;
@@ -1205,7 +1241,7 @@ __global__ __launch_bounds__(kBjThreads) void bj_round_kernel(const BjMat* __res
                     }
                     __syncthreads();
                     if (valid) {
-                        al = be = gr = gi = 0.0;
+                        al = be = gr = gi = 0;
                         for (int k = 0; k < wpp; ++k) {
                             al += red[grp * wpp + k][0];
                             be += red[grp * wpp + k][1];
@@ -1214,33 +1250,33 @@ __global__ __launch_bounds__(kBjThreads) void bj_round_kernel(const BjMat* __res
                         }
                     }
                 }
-                const double ag2 = gr * gr + gi * gi;
+                const R ag2 = gr * gr + gi * gi;
                 // rotate when |gamma| > tol sqrt(al be), tol = eps sqrt(n) (squared: no roots)
-                if (valid && ag2 > tol2 * (al * be) && ag2 != 0.0) {
-                    const double ag = sqrt(ag2);
-                    const double inv = 1.0 / ag;
-                    const double zeta = 0.5 * (be - al) * inv;
-                    const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                    const double cs = 1.0 / sqrt(1.0 + t * t);
-                    const double sn = cs * t;
-                    const double er = gr * inv, ei = gi * inv;
+                if (valid && ag2 > tol2 * (al * be) && ag2 != R(0)) {
+                    const R ag = sqrt(ag2);
+                    const R inv = R(1) / ag;
+                    const R zeta = R(0.5) * (be - al) * inv;
+                    const R t = (zeta >= R(0) ? R(1) : R(-1)) / (fabs(zeta) + sqrt(R(1) + zeta * zeta));
+                    const R cs = R(1) / sqrt(R(1) + t * t);
+                    const R sn = cs * t;
+                    const R er = gr * inv, ei = gi * inv;
                     // x_a' = c x_a - s conj(e) x_b ; x_b' = s e x_a + c x_b
-                    double2* xa = xs + (int64_t)a * n;
-                    double2* xb = xs + (int64_t)b * n;
+                    C2* xa = xs + (int64_t)a * n;
+                    C2* xb = xs + (int64_t)b * n;
                     for (int i = gtid; i < n; i += gthreads) {
-                        const double2 p = xa[i], q = xb[i];
-                        xa[i] = double2{cs * p.x - sn * (er * q.x + ei * q.y),
+                        const C2 p = xa[i], q = xb[i];
+                        xa[i] = C2{cs * p.x - sn * (er * q.x + ei * q.y),
                                         cs * p.y - sn * (er * q.y - ei * q.x)};
-                        xb[i] = double2{sn * (er * p.x - ei * p.y) + cs * q.x,
+                        xb[i] = C2{sn * (er * p.x - ei * p.y) + cs * q.x,
                                         sn * (er * p.y + ei * p.x) + cs * q.y};
                     }
-                    double2* za = Z + (int64_t)a * c;
-                    double2* zb = Z + (int64_t)b * c;
+                    C2* za = Z + (int64_t)a * c;
+                    C2* zb = Z + (int64_t)b * c;
                     for (int i = gtid; i < c; i += gthreads) {
-                        const double2 p = za[i], q = zb[i];
-                        za[i] = double2{cs * p.x - sn * (er * q.x + ei * q.y),
+                        const C2 p = za[i], q = zb[i];
+                        za[i] = C2{cs * p.x - sn * (er * q.x + ei * q.y),
                                         cs * p.y - sn * (er * q.y - ei * q.x)};
-                        zb[i] = double2{sn * (er * p.x - ei * p.y) + cs * q.x,
+                        zb[i] = C2{sn * (er * p.x - ei * p.y) + cs * q.x,
                                         sn * (er * p.y + ei * p.x) + cs * q.y};
                     }
                     if (gtid == 0) s_rot = 1;
@@ -1251,29 +1287,29 @@ __global__ __launch_bounds__(kBjThreads) void bj_round_kernel(const BjMat* __res
     }
     if (!s_rot) return;                       // uniform: read after the last barrier
     for (int j = wave; j < c; j += nw) {
-        double2* dst = x + (int64_t)gcol(j) * n;
-        const double2* src = xs + (int64_t)j * n;
+        C2* dst = x + (int64_t)gcol(j) * n;
+        const C2* src = xs + (int64_t)j * n;
         for (int i = lane; i < n; i += 64) dst[i] = src[i];
     }
     __syncthreads();
     for (int j = wave; j < c; j += nw) {
-        const double2* src = v + (int64_t)gcol(j) * n;
-        double2* dst = xs + (int64_t)j * n;
+        const C2* src = v + (int64_t)gcol(j) * n;
+        C2* dst = xs + (int64_t)j * n;
         for (int i = lane; i < n; i += 64) dst[i] = src[i];
     }
     __syncthreads();
     // V(:, cols) <- V(:, cols) Z
     for (int j = wave; j < c; j += nw) {
-        double2* dst = v + (int64_t)gcol(j) * n;
-        const double2* zj = Z + (int64_t)j * c;
+        C2* dst = v + (int64_t)gcol(j) * n;
+        const C2* zj = Z + (int64_t)j * c;
         for (int i = lane; i < n; i += 64) {
-            double sr = 0.0, si = 0.0;
+            R sr = 0, si = 0;
             for (int k = 0; k < c; ++k) {
-                const double2 p = xs[i + (int64_t)k * n], z = zj[k];
+                const C2 p = xs[i + (int64_t)k * n], z = zj[k];
                 sr += p.x * z.x - p.y * z.y;
                 si += p.x * z.y + p.y * z.x;
             }
-            dst[i] = double2{sr, si};
+            dst[i] = C2{sr, si};
         }
     }
     if (tid == 0) flags_cur[tk.x] = 1;
@@ -1498,16 +1534,21 @@ __global__ __launch_bounds__(kGbjThreads) void gbj_rotate_kernel(const BjMat* __
 }
 
 // V = I per matrix; flags row 0 = 1 (every matrix runs its first sweep), later rows = 0.
-__global__ void bj_setup_kernel(const BjMat* __restrict__ mats, int nmat, double2* __restrict__ V,
+template <typename C2>
+__global__ void bj_setup_kernel(const BjMat* __restrict__ mats, int nmat, C2* __restrict__ V,
                                 int* __restrict__ flags, int nrows) {
     const int mi = blockIdx.x;
     if (mi < nmat) {
         const BjMat mt = mats[mi];
-        double2* v = V + mt.v_off;
+        C2* v = V + mt.v_off;
         const int64_t nn = (int64_t)mt.n * mt.n;
         for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < nn;
-             e += (int64_t)gridDim.y * blockDim.x)
-            v[e] = double2{(e % mt.n) == (e / mt.n) ? 1.0 : 0.0, 0.0};
+             e += (int64_t)gridDim.y * blockDim.x) {
+            C2 one;
+            one.x = (e % mt.n) == (e / mt.n) ? 1 : 0;
+            one.y = 0;
+            v[e] = one;
+        }
     }
     if (blockIdx.x == 0 && blockIdx.y == 0)
         for (int e = threadIdx.x; e < nmat * nrows; e += blockDim.x) flags[e] = e < nmat ? 1 : 0;
@@ -1680,9 +1721,10 @@ size_t bj_stage_bytes(bug_ctx* ctx) {
     return lim > reserve ? std::min(kBjStageBytes, lim - reserve) : 0;
 }
 
-// Columns staged per workgroup for an n-column matrix (even, 2 .. kBjMaxCols), 0 if 2 do not fit.
-int bj_cols(size_t stage_bytes, int n) {
-    const int64_t c = std::min<int64_t>(kBjMaxCols, (int64_t)(stage_bytes / sizeof(double2)) / std::max(n, 1));
+// Columns staged per workgroup for an n-column matrix of `elem`-byte elements (even,
+// 2 .. kBjMaxCols), 0 if 2 do not fit.
+int bj_cols(size_t stage_bytes, int n, size_t elem = sizeof(double2)) {
+    const int64_t c = std::min<int64_t>(kBjMaxCols, (int64_t)(stage_bytes / elem) / std::max(n, 1));
     return c >= 2 ? (int)(c & ~1) : 0;
 }
 
@@ -1691,7 +1733,11 @@ bool block_jacobi_fits(bug_ctx* ctx, int nmax) { return bj_cols(bj_stage_bytes(c
 // Block one-sided Jacobi of the n x n matrices X_i (at the running offsets sum_{k<i} n_k^2 of X,
 // leading dimension n) with V_i at list[i].v_off of V (set to I here): sweeps of round launches
 // until a sweep rotates nothing in any matrix (the host reads the per-sweep flags) or max_sweeps.
-void block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list, double2* X, double2* V, int max_sweeps, int* sweeps_dev) {
+template <typename R>
+void block_jacobi_t(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list,
+                    typename Cplx<R>::type* X, typename Cplx<R>::type* V, int max_sweeps,
+                    int* sweeps_dev, R eps) {
+    typedef typename Cplx<R>::type C2;
     const int nmat = (int)list.size();
     const size_t stage = bj_stage_bytes(ctx);
     std::vector<BjMat> mats(nmat);
@@ -1701,7 +1747,7 @@ void block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list,
     size_t lds = 0;
     for (int i = 0; i < nmat; ++i) {
         const int n = list[i].n;
-        int c = bj_cols(stage, n);
+        int c = bj_cols(stage, n, sizeof(C2));
         if (ctx->bj_cols >= 2) c = std::min(c, ctx->bj_cols & ~1);
         if (c < 2) throw Error(BUG_EINVAL, "block Jacobi: matrix too wide for LDS");
         const int w = c / 2;
@@ -1712,7 +1758,7 @@ void block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list,
         if (n >= 2) {
             for (int p = 0; p < N / 2; ++p) tasks.push_back(make_int2(i, p));
             rounds = std::max(rounds, N - 1);
-            lds = std::max(lds, sizeof(double2) * ((size_t)n * std::min(c, n) + (size_t)c * c));
+            lds = std::max(lds, sizeof(C2) * ((size_t)n * std::min(c, n) + (size_t)c * c));
         }
     }
     const int nrows = max_sweeps + 1;
@@ -1723,13 +1769,13 @@ void block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list,
     unsigned char* dblob = upload_list(ctx, st, blob.data(), blob.size(), 2);
     const BjMat* dm = reinterpret_cast<const BjMat*>(dblob);
     const int2* dt = reinterpret_cast<const int2*>(dblob + sizeof(BjMat) * nmat);
-    hipLaunchKernelGGL(bj_setup_kernel, dim3(nmat, 16), dim3(256), 0, st, dm, nmat, V, flags, nrows);
+    hipLaunchKernelGGL(bj_setup_kernel<C2>, dim3(nmat, 16), dim3(256), 0, st, dm, nmat, V, flags, nrows);
     HIP_CHECK(hipGetLastError());
     int used = 0;
     std::vector<int> host(nmat);
     if (!tasks.empty()) {
         if (lds > 65536)
-            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(bj_round_kernel),
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(bj_round_kernel<R>),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         // The convergence read of sweep s overlaps sweep s + 1, which is launched before it:
         // if sweep s rotated nothing, every round of sweep s + 1 returns at once (its matrices'
@@ -1739,9 +1785,9 @@ void block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list,
             if (!e) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         for (int s = 0; s < max_sweeps; ++s) {
             for (int r = 0; r < rounds; ++r)
-                hipLaunchKernelGGL(bj_round_kernel, dim3((unsigned)tasks.size()), dim3(kBjThreads), lds,
+                hipLaunchKernelGGL(bj_round_kernel<R>, dim3((unsigned)tasks.size()), dim3(kBjThreads), lds,
                                    st, dm, dt, X, V, r, flags + (size_t)s * nmat,
-                                   flags + (size_t)(s + 1) * nmat, 2.220446049250313e-16,
+                                   flags + (size_t)(s + 1) * nmat, eps,
                                    std::max(1, ctx->bj_inner));
             HIP_CHECK(hipGetLastError());
             ++used;
@@ -1771,6 +1817,143 @@ void block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list,
         HIP_CHECK(hipStreamSynchronize(st));
     }
 }
+
+void block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list, double2* X,
+                  double2* V, int max_sweeps, int* sweeps_dev) {
+    block_jacobi_t<double>(ctx, st, list, X, V, max_sweeps, sweeps_dev, 2.220446049250313e-16);
+}
+
+// ---- FP32-preconditioned block Jacobi --------------------------------------------------------
+//
+// Sweeps of the block Jacobi in FP32 (twice the staged columns per workgroup, so about half the
+// rounds per sweep) bring X's columns to orthogonality at FP32 accuracy; their rotation V32 is
+// made exactly unitary in FP64 by a QR (Q_v, positive-diagonal gauge), X is rotated by it in
+// FP64 (X2 = X Q_v, a grouped GEMM) and the FP64 block Jacobi finishes from there (V2); the
+// accumulated rotation is V = Q_v V2.  The FP64 sweeps start from couplings of order 1e-7 and
+// converge in a few sweeps; the result is an FP64 one-sided Jacobi SVD of X (the FP32 phase only
+// chooses the starting basis), to rounding the same as the FP64-only path.
+
+// One workgroup per matrix: X32 = s X with s = 2^-e, e the binary exponent of ||X||_F (so the
+// FP32 column norms stay below 1 and far from overflow).
+__global__ __launch_bounds__(1024) void bj_to_f32_kernel(const BjMat* __restrict__ mats,
+                                                         const double2* __restrict__ X,
+                                                         float2* __restrict__ X32) {
+    __shared__ double red[16];
+    __shared__ double s_scale;
+    const BjMat mt = mats[blockIdx.x];
+    const int64_t nn = (int64_t)mt.n * mt.n;
+    const double2* x = X + mt.x_off;
+    float2* y = X32 + mt.x_off;
+    double ss = 0.0;
+    for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) ss += x[e].x * x[e].x + x[e].y * x[e].y;
+    ss = wave_sum(ss);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+        int ex = 0;
+        if (t > 0.0 && isfinite(t)) frexp(sqrt(t), &ex);
+        s_scale = ldexp(1.0, -ex);
+    }
+    __syncthreads();
+    const double sc = s_scale;
+    for (int64_t e = threadIdx.x; e < nn; e += blockDim.x)
+        y[e] = float2{(float)(sc * x[e].x), (float)(sc * x[e].y)};
+}
+
+__global__ void f32_to_f64_kernel(int64_t n, const float2* __restrict__ in, double2* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = double2{(double)in[i].x, (double)in[i].y};
+}
+
+void mixed_block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list, double2* X,
+                        double2* V, int max_sweeps, int* sweeps_dev) {
+    const int nmat = (int)list.size();
+    std::vector<int64_t> off(nmat);
+    int64_t ntot = 0, ksum = 0;
+    for (int i = 0; i < nmat; ++i) {
+        off[i] = ntot;
+        ntot += (int64_t)list[i].n * list[i].n;
+        ksum += list[i].n;
+    }
+    const size_t bytes = (size_t)ntot * (2 * sizeof(float2) + 5 * sizeof(double2)) +
+                         sizeof(double2) * (size_t)std::max<int64_t>(ksum, 1) + 1024;
+    char* base = static_cast<char*>(scratch(ctx, st, bytes, 9));
+    auto carve = [&](size_t b) {
+        char* p = base;
+        base += (b + 255) & ~size_t(255);
+        return p;
+    };
+    float2* X32 = reinterpret_cast<float2*>(carve(sizeof(float2) * ntot));
+    float2* V32 = reinterpret_cast<float2*>(carve(sizeof(float2) * ntot));
+    double2* V64 = reinterpret_cast<double2*>(carve(sizeof(double2) * ntot));
+    double2* Qv = reinterpret_cast<double2*>(carve(sizeof(double2) * ntot));
+    double2* Rv = reinterpret_cast<double2*>(carve(sizeof(double2) * ntot));
+    double2* X2 = reinterpret_cast<double2*>(carve(sizeof(double2) * ntot));
+    double2* V2 = reinterpret_cast<double2*>(carve(sizeof(double2) * ntot));
+    double2* tau = reinterpret_cast<double2*>(carve(sizeof(double2) * std::max<int64_t>(ksum, 1)));
+    (void)carve;
+    // the FP32 and FP64 phases keep V at the same running offsets as X
+    std::vector<SvdJob> loc(list);
+    std::vector<BjMat> bm(nmat);
+    for (int i = 0; i < nmat; ++i) {
+        loc[i].v_off = off[i];
+        bm[i] = BjMat{off[i], off[i], list[i].n, 0, 0, 0};
+    }
+    const BjMat* dbm = upload_list(ctx, st, bm.data(), bm.size(), 2);
+    hipLaunchKernelGGL(bj_to_f32_kernel, dim3(nmat), dim3(1024), 0, st, dbm, X, X32);
+    HIP_CHECK(hipGetLastError());
+    block_jacobi_t<float>(ctx, st, loc, X32, V32, std::max(1, ctx->mixed_fp32_sweeps), nullptr,
+                          1.1920928955078125e-07f);
+    hipLaunchKernelGGL(f32_to_f64_kernel, dim3(grid_for(ntot, 256)), dim3(256), 0, st, ntot, V32, V64);
+    HIP_CHECK(hipGetLastError());
+    // Q_v: unitary factor of V32 in FP64
+    std::vector<MatJob> qj(nmat);
+    int64_t to = 0;
+    for (int i = 0; i < nmat; ++i) {
+        const int n = list[i].n;
+        qj[i] = MatJob{off[i], off[i], off[i], to, n, n, std::max(1, n), std::max(1, n), std::max(1, n)};
+        to += n;
+    }
+    if (use_blocked_qr(ctx, qj)) {
+        blocked_qr(ctx, st, qj, V64, Qv, Rv);
+    } else {
+        MatJob* dq = upload_list(ctx, st, qj.data(), qj.size(), 2);
+        hipLaunchKernelGGL(qr_kernel, dim3(nmat), dim3(kQrThreads), 0, st, dq, V64, Qv, Rv, tau);
+    }
+    HIP_CHECK(hipGetLastError());
+    // X2 = X Q_v
+    auto square_plan = [&](uint64_t tag, bool to_v) {
+        Hasher h;
+        h.pod(tag);
+        for (int i = 0; i < nmat; ++i) {
+            h.pod(list[i].n);
+            h.pod(off[i]);
+            h.pod(to_v ? list[i].v_off : off[i]);
+        }
+        return h.h;
+    };
+    auto build = [&](Plan& plan, bool to_v) {
+        for (int i = 0; i < nmat; ++i) {
+            const int n = list[i].n;
+            if (n <= 0) continue;
+            plan.tasks.push_back(GemmTask{to_v ? list[i].v_off : off[i], n, n, n,
+                                          (int32_t)plan.terms.size(), (int32_t)plan.terms.size() + 1, 0});
+            plan.terms.push_back(GemmTerm{off[i], off[i], n, n, n, BUG_OP_N | (BUG_OP_N << 2), 1.0, 0.0});
+            plan.flops += 8.0 * n * (double)n * n;
+        }
+    };
+    launch_cached_plan(ctx, square_plan(0x5F2A, false), [&](Plan& p) { build(p, false); },
+                       BUG_PREC_FP64, X, Qv, X2, st);
+    block_jacobi_t<double>(ctx, st, loc, X2, V2, max_sweeps, sweeps_dev, 2.220446049250313e-16);
+    // V = Q_v V2 at the caller's offsets; the rotated X back into X
+    launch_cached_plan(ctx, square_plan(0x5F2B, true), [&](Plan& p) { build(p, true); },
+                       BUG_PREC_FP64, Qv, V2, V, st);
+    HIP_CHECK(hipMemcpyAsync(X, X2, sizeof(double2) * (size_t)ntot, hipMemcpyDeviceToDevice, st));
+}
+
 
 // The Gram block Jacobi (gbj_gram_kernel + gbj_rotate_kernel per round) of the n x n matrices X_i
 // at the running offsets sum_{k<i} n_k^2, with V_i at list[i].v_off (set to I here); sweeps as in
@@ -1804,7 +1987,7 @@ void gram_block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& 
     unsigned char* dblob = upload_list(ctx, st, blob.data(), blob.size(), 2);
     const BjMat* dm = reinterpret_cast<const BjMat*>(dblob);
     const int2* dt = reinterpret_cast<const int2*>(dblob + sizeof(BjMat) * nmat);
-    hipLaunchKernelGGL(bj_setup_kernel, dim3(nmat, 16), dim3(256), 0, st, dm, nmat, V, flags, nrows);
+    hipLaunchKernelGGL(bj_setup_kernel<double2>, dim3(nmat, 16), dim3(256), 0, st, dm, nmat, V, flags, nrows);
     HIP_CHECK(hipGetLastError());
     int used = 0;
     if (!tasks.empty()) {
@@ -2084,6 +2267,15 @@ int bug_set_gram_jacobi_params(bug_ctx* ctx, int32_t cols, int32_t inner_sweeps,
     });
 }
 
+int bug_set_mixed_jacobi_params(bug_ctx* ctx, int32_t fp32_sweeps, int32_t min_n) {
+    return guarded(ctx, [&] {
+        if (fp32_sweeps < 1 || fp32_sweeps > 60 || min_n < 0)
+            throw Error(BUG_EINVAL, "mixed jacobi params: FP32 sweeps in 1..60, min_n >= 0");
+        ctx->mixed_fp32_sweeps = fp32_sweeps;
+        ctx->mixed_min_n = min_n;
+    });
+}
+
 int bug_set_qr_variant(bug_ctx* ctx, int32_t variant) {
     return guarded(ctx, [&] {
         if (variant < 0 || variant > 2) throw Error(BUG_EINVAL, "qr variant must be 0, 1 or 2");
@@ -2183,8 +2375,12 @@ int bug_block_svd(bug_ctx* ctx, int32_t nmat, const bug_svd_job* jobs, void* g, 
                 hipLaunchKernelGGL(svd_jacobi_kernel, dim3(nmat), dim3(kSvdThreads), lds, st, dj,
                                    Xw, v, Uxw, s, Vxw, sweeps, sw);
             } else {
+                const bool mixed = !gram && (ctx->svd_variant == 5 ||
+                                             (ctx->svd_variant == 0 && nmax >= ctx->mixed_min_n));
                 if (gram)
                     gram_block_jacobi(ctx, st, list, Xw, v, sweeps, sw);
+                else if (mixed)
+                    mixed_block_jacobi(ctx, st, list, Xw, v, sweeps, sw);
                 else
                     block_jacobi(ctx, st, list, Xw, v, sweeps, sw);
                 SvdJob* dj = upload_list(ctx, st, jj.data(), jj.size(), 2);
@@ -2236,7 +2432,7 @@ extern "C" {
 
 int bug_set_svd_variant(bug_ctx* ctx, int32_t variant, int32_t record_sweeps) {
     return guarded(ctx, [&] {
-        if (variant < 0 || variant > 4) throw Error(BUG_EINVAL, "svd variant must be 0 .. 4");
+        if (variant < 0 || variant > 5) throw Error(BUG_EINVAL, "svd variant must be 0 .. 5");
         ctx->svd_variant = variant;
         if (record_sweeps && !ctx->svd_sweeps_dev) {
             HIP_CHECK(hipMalloc(&ctx->svd_sweeps_dev, sizeof(int) * 65536));

@@ -261,7 +261,8 @@ int bug_block_qrp(bug_ctx* ctx, int32_t nmat, const bug_qr_job* jobs, void* a, i
 /* Diagnostics: force the SVD kernel (0 automatic, 1 one-sided Jacobi with one workgroup per
  * matrix, 2 QR-preconditioned one-workgroup Jacobi, 3 QR-preconditioned multi-workgroup block
  * Jacobi with the block pair staged in LDS, 4 QR-preconditioned Gram block Jacobi with the pair's
- * rows split over workgroups) and, with record_sweeps != 0, record the Jacobi sweeps each matrix of a batch used;
+ * rows split over workgroups, 5 as 3 started from FP32 sweeps) and, with record_sweeps != 0,
+ * record the (FP64) Jacobi sweeps each matrix of a batch used;
  * bug_svd_sweeps copies the counts of the last batch (synchronizes the device). */
 int bug_set_svd_variant(bug_ctx* ctx, int32_t variant, int32_t record_sweeps);
 
@@ -279,6 +280,10 @@ int bug_set_jacobi_params(bug_ctx* ctx, int32_t cols, int32_t inner_sweeps);
  * Gram matrix per round (1..8), and the smallest matrix dimension of a batch from which the
  * automatic SVD choice uses it. */
 int bug_set_gram_jacobi_params(bug_ctx* ctx, int32_t cols, int32_t inner_sweeps, int32_t min_n);
+/* FP32-preconditioned block Jacobi (svd_variant 5; no counterpart in the reference, whose gesdd
+ * runs in FP64 throughout): at most fp32_sweeps FP32 sweeps choose the starting basis of the
+ * FP64 sweeps; variant 0 takes it for batches whose widest block has min_n or more columns. */
+int bug_set_mixed_jacobi_params(bug_ctx* ctx, int32_t fp32_sweeps, int32_t min_n);
 int bug_svd_sweeps(bug_ctx* ctx, int32_t n, int32_t* out);
 
 #ifdef __cplusplus

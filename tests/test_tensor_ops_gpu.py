@@ -86,12 +86,12 @@ def test_projections_direct_sums_and_vectors(gpu):
     assert len(many) == 70 and all(abs(v - many[0]) < 1e-12 * abs(many[0]) for v in many)
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("m,n", [(2500, 6), (700, 40), (300, 17), (5, 1), (64, 64), (610, 455)])
 def test_svd_kernels_on_tall_blocks(gpu, m, n, variant):
-    """Every SVD path (plain one-sided Jacobi; QR-preconditioned Jacobi on R^H in one workgroup or
-    as multi-workgroup block Jacobi) must match LAPACK's spectrum, reconstruct the block and return
-    orthonormal singular vectors."""
+    """Every SVD path (plain one-sided Jacobi; QR-preconditioned Jacobi on R^H in one workgroup, as
+    multi-workgroup block Jacobi, Gram block Jacobi, or block Jacobi started from FP32 sweeps) must
+    match LAPACK's spectrum, reconstruct the block and return orthonormal singular vectors."""
     from bondupdategalerkin_amd import _lib
     from bondupdategalerkin_amd import tensor_ops as T
 
@@ -121,7 +121,7 @@ def test_svd_kernels_on_tall_blocks(gpu, m, n, variant):
     ctx.check(ctx._lib.bug_set_svd_variant(ctx.handle, 0, 0))
 
 
-@pytest.mark.parametrize("variant", [2, 3, 4])
+@pytest.mark.parametrize("variant", [2, 3, 4, 5])
 def test_svd_graded_spectrum_batch(gpu, variant):
     """A batch of blocks of different widths with geometrically graded spectra (1 .. 1e-12): the
     singular values are accurate to 1e-13 sigma_max and U S Vh reconstructs each block."""
