@@ -45,3 +45,39 @@ def test_ctx_create_without_gpu_fails_cleanly():
     out = ctypes.c_void_p()
     rc = h.bug_ctx_create(0, ctypes.byref(out))
     assert rc == _lib.BUG_EDEVICE and not out.value
+
+
+def test_schedule_table_file_roundtrip(tmp_path):
+    """The shipped gfx950 tuning table loads, saves and reloads without a device; a malformed
+    file and an out-of-range mode are refused; the process-wide mode is restored."""
+    _lib.autotune_clear()
+    try:
+        _lib.autotune_load(_lib.SCHEDULE_TABLE)
+        n, _ = _lib.autotune_count()
+        rows = [ln for ln in open(_lib.SCHEDULE_TABLE) if not ln.startswith("#")]
+        assert n == len(rows) > 0
+        out = str(tmp_path / "t.txt")
+        _lib.autotune_save(out)
+        _lib.autotune_clear()
+        assert _lib.autotune_count()[0] == 0
+        _lib.autotune_load(out)
+        assert _lib.autotune_count()[0] == n
+        for bad in ("0123 9 1\n", "0123 1 0\n", "xyz\n"):
+            p = tmp_path / "bad.txt"
+            p.write_text(bad)
+            try:
+                _lib.autotune_load(str(p))
+            except ValueError:
+                pass
+            else:
+                raise AssertionError(f"accepted {bad!r}")
+        try:
+            _lib.set_autotune(7)
+        except ValueError:
+            pass
+        else:
+            raise AssertionError("accepted autotune mode 7")
+        _lib.set_autotune(_lib.AUTOTUNE_USE)
+    finally:
+        _lib.set_autotune(_lib.AUTOTUNE_OFF)
+        _lib.autotune_clear()
