@@ -322,9 +322,26 @@ void upload_plan(bug_ctx* ctx, Plan& plan, int config, hipStream_t stream) {
 }
 
 // Look up (or build) the plan of one product stage and launch it.
+uint64_t product_key(const Bsm& out, const Bsm& P, const Bsm& Q, const std::vector<bug_term>& terms,
+                     bool accumulate, uint64_t tag);
+
 void run_product(bug_ctx* ctx, const Bsm& out, void* out_dev, const Bsm& P, const void* p_dev,
                  const Bsm& Q, const void* q_dev, const std::vector<bug_term>& terms,
                  bool accumulate, int precision, hipStream_t stream, uint64_t tag) {
+    launch_cached_plan(ctx, product_key(out, P, Q, terms, accumulate, tag),
+                       [&](Plan& p) { build_product_plan(p, out, P, Q, terms, accumulate); },
+                       precision, p_dev, q_dev, out_dev, stream);
+}
+
+// The plan of one product stage (built on a cache miss), not launched.
+Plan* cached_product_plan(bug_ctx* ctx, const Bsm& out, const Bsm& P, const Bsm& Q,
+                          const std::vector<bug_term>& terms, bool accumulate, uint64_t tag) {
+    return cached_plan(ctx, product_key(out, P, Q, terms, accumulate, tag),
+                       [&](Plan& p) { build_product_plan(p, out, P, Q, terms, accumulate); });
+}
+
+uint64_t product_key(const Bsm& out, const Bsm& P, const Bsm& Q, const std::vector<bug_term>& terms,
+                     bool accumulate, uint64_t tag) {
     BsmHasher h;
     h.pod(tag);
     {
@@ -345,12 +362,10 @@ void run_product(bug_ctx* ctx, const Bsm& out, void* out_dev, const Bsm& P, cons
         h.pod(t.alpha_im);
     }
     h.pod(accumulate);
-    launch_cached_plan(ctx, h.h, [&](Plan& p) { build_product_plan(p, out, P, Q, terms, accumulate); },
-                       precision, p_dev, q_dev, out_dev, stream);
+    return h.h;
 }
 
-void launch_cached_plan(bug_ctx* ctx, uint64_t key, const std::function<void(Plan&)>& build,
-                        int precision, const void* A, const void* B, void* C, hipStream_t stream) {
+Plan* cached_plan(bug_ctx* ctx, uint64_t key, const std::function<void(Plan&)>& build) {
     Plan* plan = nullptr;
     auto it = ctx->plan_index.find(key);
     if (it != ctx->plan_index.end()) {
@@ -375,14 +390,25 @@ void launch_cached_plan(bug_ctx* ctx, uint64_t key, const std::function<void(Pla
             delete old;
         }
     }
-    const Schedule sc = pick_schedule(ctx, *plan, precision, A, B, stream);
-    upload_plan(ctx, *plan, sc.config, stream);
+    return plan;
+}
+
+void launch_plan(bug_ctx* ctx, Plan& plan, int precision, const void* A, const void* B, void* C,
+                 hipStream_t stream) {
+    const Schedule sc = pick_schedule(ctx, plan, precision, A, B, stream);
+    upload_plan(ctx, plan, sc.config, stream);
     void* partial = nullptr;
     if (sc.splits > 1)
-        partial = scratch(ctx, stream, split_partial_bytes(*plan, sc.config, sc.splits, precision), 6);
-    ctx->last_flops += plan->flops;
+        partial = scratch(ctx, stream, split_partial_bytes(plan, sc.config, sc.splits, precision), 6);
+    ctx->last_flops += plan.flops;
     ctx->last_schedules.push_back(sc);
-    HIP_CHECK(launch_grouped_gemm(*plan, sc.config, sc.splits, precision, A, B, C, partial, stream));
+    HIP_CHECK(launch_grouped_gemm(plan, sc.config, sc.splits, precision, A, B, C, partial, stream));
+}
+
+void launch_cached_plan(bug_ctx* ctx, uint64_t key, const std::function<void(Plan&)>& build,
+                        int precision, const void* A, const void* B, void* C, hipStream_t stream) {
+    Plan* plan = cached_plan(ctx, key, build);
+    launch_plan(ctx, *plan, precision, A, B, C, stream);
 }
 
 void* scratch(bug_ctx* ctx, hipStream_t stream, size_t bytes, int slot) {
@@ -452,6 +478,63 @@ struct MpoSite {
     }
     int nent() const { return (int)a.size(); }
 };
+
+// The two product stages of a one-site apply: T[(a, sin)] = L_a X_sin, then
+// Y_sout = sum coeff T[(a, sin)] R_b^T (effective_operators.jl:39-64).
+struct OneSiteStages {
+    Bsm T;
+    std::vector<bug_term> t1, t2;
+};
+
+OneSiteStages one_site_stages(const Bsm& l, const MpoSite& w, const Bsm& r, const Bsm& x,
+                              const Bsm& y) {
+    if (l.nslice() != w.wl || r.nslice() != w.wr)
+        throw Error(BUG_EINVAL, "environment channels do not match the MPO site");
+    if (y.nslice() != x.nslice()) throw Error(BUG_EINVAL, "X/Y local dims differ");
+    OneSiteStages sg;
+    std::vector<int> slot(w.wl * x.nslice(), -1);
+    std::vector<int32_t> shifts;
+    for (int e = 0; e < w.nent(); ++e) {
+        int& s = slot[w.a[e] * x.nslice() + w.si[e]];
+        if (s >= 0) continue;
+        s = (int)shifts.size();
+        shifts.push_back(l.shift[w.a[e]] + x.shift[w.si[e]]);
+        sg.t1.push_back(bug_term{s, w.a[e], w.si[e], BUG_OP_N, BUG_OP_N, 1.0, 0.0});
+    }
+    sg.T = make_intermediate(l.row, x.col, shifts);
+    for (int e = 0; e < w.nent(); ++e)
+        sg.t2.push_back(bug_term{w.so[e], slot[w.a[e] * x.nslice() + w.si[e]], w.b[e], BUG_OP_N,
+                                 BUG_OP_T, w.coeff[2 * e], w.coeff[2 * e + 1]});
+    return sg;
+}
+
+// Y -= shift X (the H_eff - shift of the Krylov recurrences).
+void shift_into(const Bsm& x, const Bsm& y, const void* xd, void* yd, double shift, hipStream_t st) {
+    same_layout(x, y, "apply_one_site shift");
+    const int64_t n = x.nelem;
+    if (n > 0)
+        hipLaunchKernelGGL(axpy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                           static_cast<double2*>(yd), static_cast<const double2*>(xd), -shift, n);
+    HIP_CHECK(hipGetLastError());
+}
+
+// Append the tasks and terms of `src` to `dst` with C offsets moved by c_shift and the A and B
+// offsets of its terms by a_shift and b_shift (elements): several instances' plans as one.
+void append_plan(Plan& dst, const Plan& src, int64_t c_shift, int64_t a_shift, int64_t b_shift) {
+    const int32_t base = (int32_t)dst.terms.size();
+    for (GemmTask t : src.tasks) {
+        t.c_off += c_shift;
+        t.term_begin += base;
+        t.term_end += base;
+        dst.tasks.push_back(t);
+    }
+    for (GemmTerm u : src.terms) {
+        u.a_off += a_shift;
+        u.b_off += b_shift;
+        dst.terms.push_back(u);
+    }
+    dst.flops += src.flops;
+}
 
 }  // namespace bug
 
@@ -676,39 +759,98 @@ int bug_apply_one_site(bug_ctx* ctx, const bug_bsm* L, const bug_mpo_site* W, co
             throw Error(BUG_EINVAL, "unknown precision");
         Bsm l(L, "L"), r(R, "R"), x(X, "X"), y(Y, "Y");
         MpoSite w(W, x.nslice());
-        if (l.nslice() != w.wl || r.nslice() != w.wr)
-            throw Error(BUG_EINVAL, "environment channels do not match the MPO site");
-        if (y.nslice() != x.nslice()) throw Error(BUG_EINVAL, "X/Y local dims differ");
-        // stage 1: T[(a, sin)] = L_a X_sin
-        std::vector<int> slot(w.wl * x.nslice(), -1);
-        std::vector<int32_t> shifts;
-        std::vector<bug_term> t1;
-        for (int e = 0; e < w.nent(); ++e) {
-            int& s = slot[w.a[e] * x.nslice() + w.si[e]];
-            if (s >= 0) continue;
-            s = (int)shifts.size();
-            shifts.push_back(l.shift[w.a[e]] + x.shift[w.si[e]]);
-            t1.push_back(bug_term{s, w.a[e], w.si[e], BUG_OP_N, BUG_OP_N, 1.0, 0.0});
-        }
-        Bsm T = make_intermediate(l.row, x.col, shifts);
+        const OneSiteStages sg = one_site_stages(l, w, r, x, y);
         hipStream_t st = as_stream(stream);
-        void* tdev = scratch(ctx, st, sizeof(double2) * std::max<int64_t>(T.nelem, 1));
-        run_product(ctx, T, tdev, l, L->data, x, X->data, t1, false, precision, st, 0x31);
-        // stage 2: Y_sout = sum coeff T[(a, sin)] R_b^T
-        std::vector<bug_term> t2;
-        for (int e = 0; e < w.nent(); ++e)
-            t2.push_back(bug_term{w.so[e], slot[w.a[e] * x.nslice() + w.si[e]], w.b[e],
-                                  BUG_OP_N, BUG_OP_T, w.coeff[2 * e], w.coeff[2 * e + 1]});
-        run_product(ctx, y, Y->data, T, tdev, r, R->data, t2, false, precision, st, 0x32);
-        if (shift != 0.0) {
-            same_layout(x, y, "apply_one_site shift");
-            const int64_t n = x.nelem;
-            if (n > 0)
-                hipLaunchKernelGGL(axpy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                                   st, static_cast<double2*>(Y->data),
-                                   static_cast<const double2*>(X->data), -shift, n);
-            HIP_CHECK(hipGetLastError());
+        void* tdev = scratch(ctx, st, sizeof(double2) * std::max<int64_t>(sg.T.nelem, 1));
+        run_product(ctx, sg.T, tdev, l, L->data, x, X->data, sg.t1, false, precision, st, 0x31);
+        run_product(ctx, y, Y->data, sg.T, tdev, r, R->data, sg.t2, false, precision, st, 0x32);
+        if (shift != 0.0) shift_into(x, y, X->data, Y->data, shift, st);
+    });
+}
+
+int bug_apply_one_site_batch(bug_ctx* ctx, int32_t k, const bug_bsm* L, const bug_mpo_site* W,
+                             const bug_bsm* R, const bug_bsm* X, const bug_bsm* Y,
+                             const double* shifts, int32_t precision, void* stream) {
+    return guarded(ctx, [&] {
+        if (precision != BUG_PREC_FP64 && precision != BUG_PREC_FP32)
+            throw Error(BUG_EINVAL, "unknown precision");
+        if (k < 0 || (k > 0 && (!L || !W || !R || !X || !Y)))
+            throw Error(BUG_EINVAL, "bad batch arguments");
+        if (k == 0) return;
+        std::vector<Bsm> ls, rs, xs, ys;
+        std::vector<OneSiteStages> sg;
+        std::vector<int64_t> toff(k);
+        int64_t ttot = 0;
+        for (int i = 0; i < k; ++i) {
+            ls.emplace_back(&L[i], "L");
+            rs.emplace_back(&R[i], "R");
+            xs.emplace_back(&X[i], "X");
+            ys.emplace_back(&Y[i], "Y");
+            MpoSite w(&W[i], xs[i].nslice());
+            sg.push_back(one_site_stages(ls[i], w, rs[i], xs[i], ys[i]));
+            toff[i] = ttot;
+            ttot += std::max<int64_t>(sg[i].T.nelem, 1);
         }
+        hipStream_t st = as_stream(stream);
+        double2* tdev = static_cast<double2*>(scratch(ctx, st, sizeof(double2) * ttot));
+        // element offset of ptr from base (both device addresses of double2 data)
+        auto lowest = [&](const bug_bsm* d) {
+            const char* b = static_cast<const char*>(d[0].data);
+            for (int i = 1; i < k; ++i) b = std::min(b, static_cast<const char*>(d[i].data));
+            return b;
+        };
+        auto rel = [](const void* p, const char* base) {
+            const int64_t bytes = static_cast<const char*>(p) - base;
+            if (bytes % (int64_t)sizeof(double2) != 0)
+                throw Error(BUG_EINVAL, "batched operands must be 16-byte aligned");
+            return bytes / (int64_t)sizeof(double2);
+        };
+        // The merged plan of a stage is cached under the instances' plan keys and their operand
+        // offsets from the lowest base: callers that keep their operands in fixed buffers (a
+        // packed Krylov basis) reuse it call after call.
+        auto merged = [&](int stage, const char* cb, const char* ab, const char* bb) {
+            std::vector<uint64_t> keys(k);
+            std::vector<int64_t> co(k), ao(k), bo(k);
+            Hasher h;
+            h.pod((uint64_t)(0xBA7C0 + stage));
+            for (int i = 0; i < k; ++i) {
+                if (stage == 1) {
+                    keys[i] = product_key(sg[i].T, ls[i], xs[i], sg[i].t1, false, 0x31);
+                    co[i] = toff[i];
+                    ao[i] = rel(L[i].data, ab);
+                    bo[i] = rel(X[i].data, bb);
+                } else {
+                    keys[i] = product_key(ys[i], sg[i].T, rs[i], sg[i].t2, false, 0x32);
+                    co[i] = rel(Y[i].data, cb);
+                    ao[i] = toff[i];
+                    bo[i] = rel(R[i].data, bb);
+                }
+                h.pod(keys[i]);
+                h.pod(co[i]);
+                h.pod(ao[i]);
+                h.pod(bo[i]);
+            }
+            return cached_plan(ctx, h.h, [&](Plan& m) {
+                for (int i = 0; i < k; ++i) {
+                    const Plan* p = stage == 1
+                        ? cached_product_plan(ctx, sg[i].T, ls[i], xs[i], sg[i].t1, false, 0x31)
+                        : cached_product_plan(ctx, ys[i], sg[i].T, rs[i], sg[i].t2, false, 0x32);
+                    append_plan(m, *p, co[i], ao[i], bo[i]);
+                }
+            });
+        };
+        // stage 1 of every instance in one launch: A = L_i, B = X_i, C = T_i
+        const char* lb = lowest(L);
+        const char* xb = lowest(X);
+        Plan* m1 = merged(1, nullptr, lb, xb);
+        launch_plan(ctx, *m1, precision, lb, xb, tdev, st);
+        // stage 2: A = T_i, B = R_i, C = Y_i
+        const char* rb = lowest(R);
+        const char* yb = lowest(Y);
+        Plan* m2 = merged(2, yb, nullptr, rb);
+        launch_plan(ctx, *m2, precision, tdev, rb, const_cast<char*>(yb), st);
+        for (int i = 0; i < k; ++i)
+            if (shifts && shifts[i] != 0.0) shift_into(xs[i], ys[i], X[i].data, Y[i].data, shifts[i], st);
     });
 }
 

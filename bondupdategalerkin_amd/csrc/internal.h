@@ -129,6 +129,10 @@ extern const int kNumTileConfigs;
 // Launch a grouped-GEMM plan cached under `key` (built by `build` on a miss) on A, B -> C.
 void launch_cached_plan(bug_ctx* ctx, uint64_t key, const std::function<void(Plan&)>& build,
                         int precision, const void* A, const void* B, void* C, hipStream_t stream);
+// The cached plan under `key` (built on a miss); launching a plan (cached or not) on A, B -> C.
+Plan* cached_plan(bug_ctx* ctx, uint64_t key, const std::function<void(Plan&)>& build);
+void launch_plan(bug_ctx* ctx, Plan& plan, int precision, const void* A, const void* B, void* C,
+                 hipStream_t stream);
 
 // FNV-1a hash of plan signatures.
 struct Hasher {

@@ -189,6 +189,14 @@ int bug_env_push_right(bug_ctx* ctx, const bug_bsm* R, const bug_bsm* B, const b
 int bug_apply_one_site(bug_ctx* ctx, const bug_bsm* L, const bug_mpo_site* W, const bug_bsm* R,
                        const bug_bsm* X, const bug_bsm* Y, double shift, int32_t precision,
                        void* stream);
+/* k independent one-site applies Y_i = (H_eff,i - shifts[i]) X_i (shifts may be NULL) with one
+ * grouped-GEMM launch per product stage for all of them: the instances' block products are
+ * packed into one task list over the lowest of their base addresses (operands 16-byte aligned).
+ * The reference applies one H_eff at a time (effective_operators.jl:39-64); this entry point is for
+ * many small independent evolutions (parameter sets, serving requests) advanced together. */
+int bug_apply_one_site_batch(bug_ctx* ctx, int32_t k, const bug_bsm* L, const bug_mpo_site* W,
+                             const bug_bsm* R, const bug_bsm* X, const bug_bsm* Y,
+                             const double* shifts, int32_t precision, void* stream);
 
 /*
  * Zero-site (root) generator  Y = sum_w L_w C R_w^T - shift * C, with L, R the environments

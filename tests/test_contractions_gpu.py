@@ -270,3 +270,46 @@ def test_autotuner_table(gpu, tmp_path):
     finally:
         _lib.set_autotune(_lib.AUTOTUNE_OFF)
         _lib.autotune_clear()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batched_one_site_apply(gpu, prec):
+    """Independent instances (different chains, sites, layouts and shifts) packed into one launch
+    per stage: each result equals its single-instance apply to rounding and the oracle's."""
+    from bondupdategalerkin_amd import environments as ev
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    items, refs, singles = [], [], []
+    for seed, (L, dmax, site, delta, shift) in enumerate([(6, 60, 2, 0.5, 0.0), (8, 40, 5, 1.0, 0.3),
+                                                          (5, 90, 1, 0.0, -0.2), (6, 25, 4, 0.5, 0.1)]):
+        psi, _ = _chain(20 + seed, L, dmax, "U1")
+        mpo_o = om.xxz_mpo(L, delta=delta)
+        mpo_d = xxz_mpo(L, delta=delta)
+        lenvs, renvs = _envs(psi, mpo_o)
+        Ld, Rd, Xd = (device_from_fam(lenvs[site]), device_from_fam(renvs[site + 1]),
+                      device_from_fam(psi[site]))
+        items.append((Ld, mpo_d[site], Rd, Xd, shift))
+        refs.append(ob.apply_one_site(lenvs[site], mpo_o[site], renvs[site + 1], psi[site],
+                                      shift=shift))
+        singles.append(ev.apply_one_site(Ld, mpo_d[site], Rd, Xd, shift=shift, precision=prec))
+    got = ev.apply_one_site_batch(items, precision=prec)
+    tol = TOL64 if prec == 0 else TOL32
+    for i, (g, r, one) in enumerate(zip(got, refs, singles)):
+        assert rel_err(fam_from_device(g), r) <= tol, i
+        d = (g.data - one.data).abs().max().item()
+        assert d <= (1e-13 if prec == 0 else 1e-5) * one.data.abs().max().item(), i
+    assert ev.apply_one_site_batch([]) == []
+    # the packed operator: persistent input/output buffers, the merged plan reused
+    from bondupdategalerkin_amd import _lib
+
+    packed = ev.PackedApply([it[:3] for it in items], [it[3].layout for it in items],
+                            shifts=[it[4] for it in items], precision=prec)
+    ctx = _lib.context()
+    for rep in range(3):
+        out = packed.apply([it[3] for it in items])
+        before = ctx.stats()["misses"]
+        for i, (o, one) in enumerate(zip(out, singles)):
+            d = (o.data - one.data).abs().max().item()
+            assert d <= (1e-13 if prec == 0 else 1e-5) * one.data.abs().max().item(), (rep, i)
+    out = packed()
+    assert ctx.stats()["misses"] == before          # no plan built on a repeated call
