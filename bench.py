@@ -289,7 +289,8 @@ def main():
 
         srv = {"mix": (f"{args.serving} requests, Poisson arrivals at {args.serving_rate}/s "
                        "(seed 0): 60% XX/XXZ chains L in {16,24,32}, maxdim in {16,32,64}, 2-6 "
-                       "steps, order 2; 40% Heston QTT 2^5-2^7 per axis, maxdim 16-32, 2-4 steps"),
+                       "steps, order 2; 40% Heston QTT 2^5-2^7 per axis, maxdim 16-32, 2-4 steps; Heston "
+                       "requests run FP64 in both forms (non-Hermitian, stiff)"),
                "slots": args.slots}
         for form in (["mixed", "fp64"] if args.precision == "mixed" else ["fp64"]):
             print(f"# serving mix ({form})", file=sys.stderr, flush=True)

@@ -11,6 +11,12 @@ all running requests overlap on the device (different streams).  Each request's 
 the same alone and in the mix: the kernels are deterministic, a request's launches go to its own
 stream in program order, and nothing numerical is shared between slots.
 
+Precision policy: a request's ``precision`` ("mixed" or "fp64") applies to chains (Hermitian
+generators); Heston requests (non-Hermitian, stiff) always run FP64.  In FP32 their stiff
+generators turn contraction rounding into spurious rank growth (a 2^7 x 2^7 grid: bond dimension
+2 -> 14 and 3x the time over three steps) or overflow (2^10 x 2^10); the form actually used is
+reported on the result.
+
 Reported per request: queueing delay (arrival to admission), run time, and latency (arrival to
 completion); per run: p50/p99 latency and the aggregate step rate.
 """
@@ -63,6 +69,7 @@ class Result:
     observable: np.ndarray          # chain: S^z profile; heston: bond dimensions
     state: Optional[object] = None  # the final device state when ``keep_state``
     error: Optional[str] = None
+    precision: str = ""             # the arithmetic form the request ran in
 
     @property
     def latency(self) -> float:
@@ -109,14 +116,22 @@ def _build(req: Request):
     return psi, mpo, tau, herm
 
 
+def precision_used(req: Request) -> str:
+    """The form a request runs in: its own for chains, FP64 for Heston (see the module notes)."""
+    if req.precision not in ("mixed", "fp64"):
+        raise ValueError(f"unknown precision {req.precision!r}")
+    return req.precision if req.kind == "chain" else "fp64"
+
+
 def run_request(req: Request):
     """Run one request to completion on the calling thread's current stream."""
     psi, mpo, tau, herm = _build(req)
+    form = precision_used(req)
     statuses = []
     for _ in range(req.params["steps"]):
         info = B.cbe_bug_step(psi, mpo, tau, order=req.order, maxdim=req.params["maxdim"],
                               hermitian=herm, work_ceiling_rule="maxdim",
-                              precision=req.precision, rng=req.seed)
+                              precision=form, rng=req.seed)
         statuses.append(B.budget_status(info.ledger))
     obs = M.magnetisation(psi) if req.kind == "chain" else np.array(psi.bond_dims())
     torch.cuda.current_stream().synchronize()
@@ -225,9 +240,13 @@ class Engine:
                     psi, statuses, obs = run_request(req)
             except Exception as e:          # reported on the result, the slot stays alive
                 err = f"{type(e).__name__}: {e}"
+            try:
+                form = precision_used(req)
+            except ValueError:
+                form = req.precision
             res = Result(req.rid, req.kind, len(statuses), req.arrival, req._admitted,
                          time.perf_counter(), req._footprint, statuses, obs,
-                         psi if req.keep_state else None, err)
+                         psi if req.keep_state else None, err, form)
             with self._cv:
                 self.results[req.rid] = res
                 self.in_use -= req._footprint

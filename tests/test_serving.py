@@ -29,3 +29,19 @@ def test_poisson_mix_is_seeded_and_mixed():
     full = poisson_mix(50, rate=1.0, seed=1, scale="full")
     assert all(r.params["L"] in (32, 64, 96, 128) for _, r in full if r.kind == "chain")
     assert all(8 <= r.params["bits"] <= 11 for _, r in full if r.kind == "heston")
+
+
+def test_precision_policy():
+    """Chains run in the requested form; Heston requests always run FP64 (CPU only)."""
+    import pytest
+
+    from bondupdategalerkin_amd.serving import Request, precision_used
+
+    chain = {"L": 12, "delta": 0.0, "start": "dw", "maxdim": 16, "steps": 1, "dt": 0.05}
+    heston = {"bits": 4, "params": (0.03, 0.0, 1.0, 0.1, 0.3, -0.5), "maxdim": 8, "steps": 1,
+              "dt": 0.02}
+    assert precision_used(Request("chain", chain, precision="mixed")) == "mixed"
+    assert precision_used(Request("chain", chain, precision="fp64")) == "fp64"
+    assert precision_used(Request("heston", heston, precision="mixed")) == "fp64"
+    with pytest.raises(ValueError):
+        precision_used(Request("chain", chain, precision="fp16"))

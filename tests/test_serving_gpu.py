@@ -56,6 +56,7 @@ def test_results_alone_equal_results_in_the_mix(gpu):
     finally:
         eng.close()
     assert all(r.error is None for r in alone + mixed), [r.error for r in alone + mixed]
+    assert [r.precision for r in mixed] == ["mixed", "fp64", "fp64", "mixed", "fp64"]
     # the mix really overlapped: some request was admitted before another one finished
     spans = sorted((r.admitted, r.finished) for r in mixed)
     assert any(spans[i + 1][0] < spans[i][1] for i in range(len(spans) - 1))
