@@ -139,6 +139,10 @@ SIGNATURES = {
     ),
     "bug_axpy": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _P, _P, _P]),
     "bug_scal": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _P, _P]),
+    "bug_dotc_segments": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, _i64p, ctypes.c_int32,
+                                         ctypes.POINTER(_P), _P, _f64p, _P]),
+    "bug_axpy_segments": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, _i64p, ctypes.c_int32,
+                                         _f64p, ctypes.POINTER(_P), _P, _P]),
     "bug_axpy_many": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, _f64p, ctypes.POINTER(_P),
                                      _P, _P]),
     "bug_dotc": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_P), _P, _f64p, _P]),

@@ -135,6 +135,81 @@ __global__ void zaxpy_many_kernel(int64_t n, int k, AxpyCoeffs c, DotPtrs xs,
     }
 }
 
+// ---- segmented vector operations (K independent vectors packed in one buffer) -------------
+//
+// Segment i of a packed buffer is [off_i, off_i + len_i).  The multi-dot reduces each
+// (vector j, segment i) over fixed chunks of kSegChunk elements and then over the chunks in
+// order; the multi-axpy adds sum_j a[j][i] x_j to segment i of y, the terms in j order.
+
+constexpr int kSegChunk = 4096;
+constexpr int kSegThreads = 256;
+
+__global__ __launch_bounds__(kSegThreads) void seg_multidot_partial(
+    int nseg, const int64_t* __restrict__ segs, int k, DotPtrs xs, const double2* __restrict__ y,
+    int nchunk, double2* __restrict__ partial) {
+    __shared__ double red_re[kSegThreads / 64];
+    __shared__ double red_im[kSegThreads / 64];
+    const int i = blockIdx.x, ch = blockIdx.y;
+    const int64_t off = segs[2 * i], len = segs[2 * i + 1];
+    const int64_t lo = (int64_t)ch * kSegChunk, hi = min(len, lo + kSegChunk);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j = 0; j < k; ++j) {
+        double re = 0.0, im = 0.0;
+        for (int64_t e = lo + threadIdx.x; e < hi; e += kSegThreads) {
+            const double2 xv = xs.p[j][off + e], yv = y[off + e];
+            re += xv.x * yv.x + xv.y * yv.y;
+            im += xv.x * yv.y - xv.y * yv.x;
+        }
+        re = wave_sum(re);
+        im = wave_sum(im);
+        if (lane == 0) {
+            red_re[wave] = re;
+            red_im[wave] = im;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double sr = 0.0, si = 0.0;
+            for (int w = 0; w < kSegThreads / 64; ++w) {
+                sr += red_re[w];
+                si += red_im[w];
+            }
+            partial[((int64_t)j * nseg + i) * nchunk + ch] = double2{sr, si};
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void seg_multidot_final(int nchunk, const double2* __restrict__ partial,
+                                   double2* __restrict__ out) {
+    const int64_t r = blockIdx.x;       // (j, i) row
+    if (threadIdx.x == 0) {
+        double re = 0.0, im = 0.0;
+        for (int c = 0; c < nchunk; ++c) {
+            re += partial[r * nchunk + c].x;
+            im += partial[r * nchunk + c].y;
+        }
+        out[r] = double2{re, im};
+    }
+}
+
+__global__ void seg_axpy_many_kernel(int nseg, const int64_t* __restrict__ segs, int k,
+                                     const double2* __restrict__ a, DotPtrs xs,
+                                     double2* __restrict__ y) {
+    const int i = blockIdx.x;
+    const int64_t off = segs[2 * i], len = segs[2 * i + 1];
+    for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < len;
+         e += (int64_t)gridDim.y * blockDim.x) {
+        double2 w = y[off + e];
+        for (int j = 0; j < k; ++j) {
+            const double ar = a[(int64_t)j * nseg + i].x, ai = a[(int64_t)j * nseg + i].y;
+            const double2 v = xs.p[j][off + e];
+            w.x += ar * v.x - ai * v.y;
+            w.y += ar * v.y + ai * v.x;
+        }
+        y[off + e] = w;
+    }
+}
+
 // partial[j * nblk + b] = sum over block b's rows of conj(x_j) y  (re, im interleaved)
 __global__ __launch_bounds__(kDotThreads) void multidot_partial(int64_t n, int k, DotPtrs xs,
                                                                  const double2* __restrict__ y,
@@ -2065,6 +2140,74 @@ int bug_axpy_many(bug_ctx* ctx, int64_t n, int32_t k, const double* alphas, cons
         }
         hipLaunchKernelGGL(zaxpy_many_kernel, dim3(grid_for(n, 256)), dim3(256), 0,
                            as_stream(stream), n, k, c, p, static_cast<double2*>(y));
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+namespace {
+// Validate segments [off, off + len) of a buffer of n elements and upload them.
+const int64_t* upload_segments(bug_ctx* ctx, hipStream_t st, int32_t nseg, const int64_t* segs,
+                               int64_t n, int64_t& maxlen) {
+    if (nseg < 1 || nseg > 4096 || !segs) throw Error(BUG_EINVAL, "segments: 1 .. 4096 needed");
+    maxlen = 0;
+    for (int i = 0; i < nseg; ++i) {
+        const int64_t off = segs[2 * i], len = segs[2 * i + 1];
+        if (off < 0 || len < 0 || off + len > n) throw Error(BUG_EINVARIANT, "segment outside the buffer");
+        maxlen = std::max(maxlen, len);
+    }
+    return upload_list(ctx, st, segs, 2 * (size_t)nseg, 2);
+}
+}  // namespace
+
+int bug_dotc_segments(bug_ctx* ctx, int64_t n, int32_t nseg, const int64_t* segs, int32_t k,
+                      const void* const* xs, const void* y, double* out, void* stream) {
+    return guarded(ctx, [&] {
+        if (n < 0 || k < 0 || k > kMaxDots || (k > 0 && (!xs || !out || !y)))
+            throw Error(BUG_EINVAL, "bad dotc_segments arguments (k must be <= 64)");
+        if (k == 0) return;
+        hipStream_t st = as_stream(stream);
+        int64_t maxlen = 0;
+        const int64_t* dseg = upload_segments(ctx, st, nseg, segs, n, maxlen);
+        DotPtrs p;
+        for (int j = 0; j < k; ++j) {
+            if (!xs[j]) throw Error(BUG_EINVAL, "null vector in dotc_segments");
+            p.p[j] = static_cast<const double2*>(xs[j]);
+        }
+        const int nchunk = (int)std::max<int64_t>(1, (maxlen + kSegChunk - 1) / kSegChunk);
+        const size_t rows = (size_t)k * nseg;
+        double2* part = static_cast<double2*>(
+            scratch(ctx, st, sizeof(double2) * (rows * nchunk + rows), 3));
+        double2* res = part + rows * nchunk;
+        hipLaunchKernelGGL(seg_multidot_partial, dim3(nseg, nchunk), dim3(kSegThreads), 0, st, nseg,
+                           dseg, k, p, static_cast<const double2*>(y), nchunk, part);
+        hipLaunchKernelGGL(seg_multidot_final, dim3((unsigned)rows), dim3(64), 0, st, nchunk, part, res);
+        HIP_CHECK(hipGetLastError());
+        double* h = pinned(ctx, sizeof(double2) * rows);
+        HIP_CHECK(hipMemcpyAsync(h, res, sizeof(double2) * rows, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        std::memcpy(out, h, sizeof(double2) * rows);
+    });
+}
+
+int bug_axpy_segments(bug_ctx* ctx, int64_t n, int32_t nseg, const int64_t* segs, int32_t k,
+                      const double* alphas, const void* const* xs, void* y, void* stream) {
+    return guarded(ctx, [&] {
+        if (n < 0 || k < 0 || k > kMaxDots || (k > 0 && (!xs || !alphas || !y)))
+            throw Error(BUG_EINVAL, "bad axpy_segments arguments (k must be <= 64)");
+        if (k == 0) return;
+        hipStream_t st = as_stream(stream);
+        int64_t maxlen = 0;
+        const int64_t* dseg = upload_segments(ctx, st, nseg, segs, n, maxlen);
+        DotPtrs p;
+        for (int j = 0; j < k; ++j) {
+            if (!xs[j]) throw Error(BUG_EINVAL, "null vector in axpy_segments");
+            p.p[j] = static_cast<const double2*>(xs[j]);
+        }
+        const double2* da = upload_list(ctx, st, reinterpret_cast<const double2*>(alphas),
+                                        (size_t)k * nseg, 1);
+        const unsigned gy = (unsigned)std::max<int64_t>(1, std::min<int64_t>((maxlen + 255) / 256, 64));
+        hipLaunchKernelGGL(seg_axpy_many_kernel, dim3(nseg, gy), dim3(256), 0, st, nseg, dseg, k, da,
+                           p, static_cast<double2*>(y));
         HIP_CHECK(hipGetLastError());
     });
 }

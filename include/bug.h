@@ -216,6 +216,15 @@ int bug_axpy(bug_ctx* ctx, int64_t n, double ar, double ai, const void* x, void*
  * _cgs2!/_combine in krylov.jl:296-332). */
 int bug_axpy_many(bug_ctx* ctx, int64_t n, int32_t k, const double* alphas, const void* const* xs,
                   void* y, void* stream);
+/* Segmented forms for K independent vectors packed in one buffer of n elements, segment i at
+ * [segs[2i], segs[2i] + segs[2i+1]) (1 <= nseg <= 4096, no counterpart in the reference, which
+ * runs one Krylov recurrence at a time): out[2(j nseg + i)], out[2(j nseg + i) + 1] = re, im of
+ * <xs[j]_i | y_i> for j < k <= 64 (host output, synchronizes the stream); and
+ * y_i += sum_j (alphas[2(j nseg + i)] + i alphas[2(j nseg + i) + 1]) xs[j]_i, added in j order. */
+int bug_dotc_segments(bug_ctx* ctx, int64_t n, int32_t nseg, const int64_t* segs, int32_t k,
+                      const void* const* xs, const void* y, double* out, void* stream);
+int bug_axpy_segments(bug_ctx* ctx, int64_t n, int32_t nseg, const int64_t* segs, int32_t k,
+                      const double* alphas, const void* const* xs, void* y, void* stream);
 /* x *= (ar + i ai). */
 int bug_scal(bug_ctx* ctx, int64_t n, double ar, double ai, void* x, void* stream);
 /* out[2j], out[2j+1] = re, im of <xs[j] | y> for j < k <= 64 (host output; synchronizes the
