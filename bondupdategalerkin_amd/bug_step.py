@@ -228,6 +228,35 @@ def bond_work_ceilings(ranks, d, maxdim, factor):
     return [max(min(one(r), maxdim), 1) for r in ranks]
 
 
+def bond_work_ceilings_strict(ranks, maxdim, factor):
+    """min(ceil(f D_b), maxdim) per bond, no local-dimension factor (bond_work_ceilings_strict,
+    basis_budget.jl:505-512)."""
+    if not factor > 1:
+        raise ValueError("strict bond factor must exceed 1")
+    return [max(1, min(maxdim, math.ceil(round(factor * r, 9)))) for r in ranks]
+
+
+def bond_frame_root_ceilings(ranks, maxdim, root_bond, frame_factor=2.0, root_factor=1.2):
+    """Two tiers (bond_frame_root_ceilings, basis_budget.jl:526-549): ceil(frame_factor D_b)
+    away from the root bond and ceil(root_factor D_root) at it (0-based root_bond), each within
+    [1, maxdim], all from the bonds at the start of the step."""
+    if not 0 <= root_bond < len(ranks):
+        raise IndexError("root bond outside the chain")
+    if not frame_factor > 1:
+        raise ValueError("frame_factor must exceed 1")
+    if not root_factor > 1:
+        raise ValueError("root_factor must exceed 1")
+    if root_factor > frame_factor:
+        raise ValueError("root_factor cannot exceed frame_factor")
+    if maxdim < 1:
+        raise ValueError("maxdim must be positive")
+    out = []
+    for b, r in enumerate(ranks):
+        f = root_factor if b == root_bond else frame_factor
+        out.append(max(1, min(maxdim, math.ceil(round(f * r, 9)))))
+    return out
+
+
 # ---- gauge --------------------------------------------------------------------------------
 
 def canonical(psi: SymMPS, site: int) -> SymMPS:
@@ -929,6 +958,7 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
                  tol_root: Optional[float] = None, tol_krylov: Optional[float] = None,
                  order: int = 1, maxdim: Optional[int] = None, hermitian: bool = True,
                  work_ceiling_rule: str = "bond", bond_ceiling_factor: Optional[float] = None,
+                 root_ceiling_factor: float = 1.2,
                  root: Optional[int] = None, reorth: bool = True, truncate: bool = True,
                  expansion: Optional[DDQRExpansion] = None, rng: Optional[int] = None,
                  probe_hook: Optional[Callable] = None, precision: str = "fp64",
@@ -937,6 +967,12 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     enriching frontier sweep each, ``closure.py``); ``root`` is the 0-based root bond (the
     reference's ``root`` minus 1),
     ``rng`` the probe seed (default 0x5EED, fresh per call as the reference's default RNG).
+
+    ``work_ceiling_rule`` caps every bond's working rank from the bonds before the step
+    (bug_step_stages.jl:116-129): ``"bond"`` ceil(f d D_b) (f defaults to 1.2), ``"maxdim"`` one
+    cap d max(maxdim, D_max), ``"bond_strict"`` ceil(f D_b) (f > 1 required), ``"frame_root"``
+    ceil(f D_b) away from the root bond (f defaults to 2) and ceil(root_ceiling_factor D_root) at
+    it; each per-bond cap is clipped to [1, maxdim].
 
     ``precision = "mixed"`` runs the dense block contractions (environment pushes, one- and
     zero-site applies, streamed sketches) on FP32 MFMA with the complex products split into real
@@ -966,13 +1002,18 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         raise ValueError("tolerances must be positive (tol_trunc non-negative)")
     if order < 1:
         raise ValueError(f"order must be >= 1, got {order}")
-    if work_ceiling_rule not in ("bond", "maxdim"):
-        raise ValueError(f"work_ceiling_rule must be 'bond' or 'maxdim', got {work_ceiling_rule!r}")
+    rules = ("maxdim", "bond", "bond_strict", "frame_root")
+    if work_ceiling_rule not in rules:
+        raise ValueError(f"work_ceiling_rule must be one of {rules}, got {work_ceiling_rule!r}")
     factor = bond_ceiling_factor
     if factor is None:
         factor = DEFAULT_BOND_CEILING_FACTOR if work_ceiling_rule == "bond" else 0.0
-    if not (factor == 0 or (factor > 1 and work_ceiling_rule == "bond")):
-        raise ValueError("bond_ceiling_factor must be 0, or > 1 with work_ceiling_rule='bond'")
+    if not (factor == 0 or (factor > 1 and work_ceiling_rule != "maxdim")):
+        raise ValueError("bond_ceiling_factor must be 0, or > 1 with a per-bond work_ceiling_rule")
+    if not root_ceiling_factor > 1:
+        raise ValueError("root_ceiling_factor must exceed one")
+    if work_ceiling_rule == "frame_root" and root_ceiling_factor > (2.0 if factor == 0 else factor):
+        raise ValueError("root_ceiling_factor must not exceed the frame factor")
     if probe_hook is not None and parallel:
         raise ValueError("probe_hook observes one probe sequence; it is not taken with parallel")
     if precision not in ("fp64", "mixed"):
@@ -988,6 +1029,11 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     bd0 = psi.bond_dims()
     if work_ceiling_rule == "bond":
         wcap = bond_work_ceilings(bd0, d, maxdim, factor)
+    elif work_ceiling_rule == "bond_strict":
+        wcap = bond_work_ceilings_strict(bd0, maxdim, factor)
+    elif work_ceiling_rule == "frame_root":
+        wcap = bond_frame_root_ceilings(bd0, maxdim, rl, 2.0 if factor == 0 else factor,
+                                        root_ceiling_factor)
     else:
         wcap = [max(d * maxdim, d * max(bd0 + [0]), 1)] * (L - 1)
     nrm = T.norm(psi[psi.center]) if 0 <= psi.center < L else 1.0

@@ -149,6 +149,7 @@ class StepOptions:
     hermitian: bool = True
     work_ceiling_rule: str = "bond"
     bond_ceiling_factor: Optional[float] = None
+    root_ceiling_factor: float = 1.2
     root: Optional[int] = None          # 0-based bond of the root (reference root - 1)
     reorth: bool = True
     truncate: bool = True
@@ -221,6 +222,57 @@ def bond_work_ceilings(ranks, d, maxdim, factor):
     def ceil_of(r):
         return math.ceil(round(factor * d * r, 9)) if factor > 0 else 2 * d * r
     return [max(min(ceil_of(r), maxdim), 1) for r in ranks]
+
+
+def bond_work_ceilings_strict(ranks, maxdim, factor):
+    """bond_work_ceilings_strict (basis_budget.jl:505-512): min(ceil(f D_b), maxdim) per bond,
+    no local-dimension factor."""
+    if not factor > 1:
+        raise ValueError("strict bond factor must exceed 1")
+    return [max(1, min(maxdim, math.ceil(round(factor * r, 9)))) for r in ranks]
+
+
+def bond_frame_root_ceilings(ranks, maxdim, root_bond, frame_factor=2.0, root_factor=1.2):
+    """bond_frame_root_ceilings (basis_budget.jl:526-549): ceil(frame_factor D_b) away from the
+    root bond, ceil(root_factor D_root) at it, each within [1, maxdim]; root_bond is 0-based."""
+    if not 0 <= root_bond < len(ranks):
+        raise IndexError("root bond outside the chain")
+    if not frame_factor > 1:
+        raise ValueError("frame_factor must exceed 1")
+    if not root_factor > 1:
+        raise ValueError("root_factor must exceed 1")
+    if root_factor > frame_factor:
+        raise ValueError("root_factor cannot exceed frame_factor")
+    if maxdim < 1:
+        raise ValueError("maxdim must be positive")
+    return [max(1, min(maxdim, math.ceil(round((root_factor if b == root_bond else frame_factor) * r,
+                                                9))))
+            for b, r in enumerate(ranks)]
+
+
+def work_ceilings(rule, bd0, d, maxdim, factor, root_bond, root_factor):
+    """The step's work ceiling from the bonds before the step (bug_step_stages.jl:63-85,116-129):
+    one number for every bond (rule "maxdim") or one per bond ("bond", "bond_strict",
+    "frame_root")."""
+    if rule not in ("maxdim", "bond", "bond_strict", "frame_root"):
+        raise ValueError(f"work_ceiling_rule must be maxdim, bond, bond_strict or frame_root, "
+                         f"got {rule!r}")
+    if factor is None:
+        factor = DEFAULT_BOND_CEILING_FACTOR if rule == "bond" else 0.0
+    if not (factor == 0 or (factor > 1 and rule in ("bond", "bond_strict", "frame_root"))):
+        raise ValueError("bond_ceiling_factor must be 0, or > 1 with a per-bond rule")
+    if not root_factor > 1:
+        raise ValueError("root_ceiling_factor must exceed one")
+    if rule == "frame_root" and root_factor > (2.0 if factor == 0 else factor):
+        raise ValueError("root_ceiling_factor must not exceed the frame factor")
+    if rule == "bond":
+        return bond_work_ceilings(bd0, d, maxdim, factor)
+    if rule == "bond_strict":
+        return bond_work_ceilings_strict(bd0, maxdim, factor)
+    if rule == "frame_root":
+        return bond_frame_root_ceilings(bd0, maxdim, root_bond, 2.0 if factor == 0 else factor,
+                                        root_factor)
+    return max(d * maxdim, d * max(bd0 + [0]), 1)
 
 
 # ---- environments -------------------------------------------------------------------------
@@ -847,16 +899,9 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
     d = len(psi.sites[0].shifts)
     tau = complex(tau)
     rng = la.ProbeRNG(opts.seed)
-    factor = opts.bond_ceiling_factor
-    if factor is None:
-        factor = DEFAULT_BOND_CEILING_FACTOR if opts.work_ceiling_rule == "bond" else 0.0
     bd0 = psi.bond_dims()
-    if opts.work_ceiling_rule == "bond":
-        wcap = bond_work_ceilings(bd0, d, opts.maxdim, factor)
-    elif opts.work_ceiling_rule == "maxdim":
-        wcap = max(d * opts.maxdim, d * max(bd0 + [0]), 1)
-    else:
-        raise ValueError(f"unknown work_ceiling_rule {opts.work_ceiling_rule}")
+    wcap = work_ceilings(opts.work_ceiling_rule, bd0, d, opts.maxdim, opts.bond_ceiling_factor,
+                         rl, opts.root_ceiling_factor)
     nrm = psi.sites[psi.center].norm() if 0 <= psi.center < L else 1.0
     alloc = basis_allocation(opts.tol_basis, L - 1, tau, nrm, opts.order, opts.tol_krylov)
     ledger = Ledger(r_old=max(bd0 + [0]))

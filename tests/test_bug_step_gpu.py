@@ -98,6 +98,28 @@ def test_frozen_order2_run_on_device(gpu):
         assert B.budget_status(info.ledger) == row["status"], n
 
 
+@pytest.mark.parametrize("start,delta,rule,factor,root_factor,order", [
+    ("dw", 0.5, "bond_strict", 1.5, 1.2, 1), ("neel", 0.0, "bond_strict", 2.0, 1.2, 2),
+    ("dw", 0.5, "frame_root", None, 1.2, 2), ("neel", 1.0, "frame_root", 1.8, 1.5, 1)])
+def test_ceiling_rules_match_oracle(gpu, start, delta, rule, factor, root_factor, order):
+    """:bond_strict and :frame_root work ceilings (basis_budget.jl:497-549,
+    bug_step_stages.jl:116-129) on L = 10: same bonds, verdicts and states as the oracle."""
+    from bondupdategalerkin_amd import bug_step as B
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    psi_d, psi_o = _device_state(start, 10), _oracle_state(start, 10)
+    mpo_d, mpo_o = xxz_mpo(10, delta=delta), om.xxz_mpo(10, delta=delta)
+    kw = dict(work_ceiling_rule=rule, bond_ceiling_factor=factor, root_ceiling_factor=root_factor,
+              order=order)
+    for n in range(3):
+        info_d = B.cbe_bug_step(psi_d, mpo_d, -0.05j, **kw)
+        info_o = obug.cbe_bug_step(psi_o, mpo_o, -0.05j, obug.StepOptions(**kw))
+        assert psi_d.bond_dims() == psi_o.bond_dims(), n
+        assert B.budget_status(info_d.ledger) == info_o.ledger.status, n
+        vd, vo = _dense(psi_d), om.mps_dense(psi_o.sites)
+        assert np.linalg.norm(vd - vo) / np.linalg.norm(vo) <= 1e-10, n
+
+
 def test_invalid_options_refused_before_touching_state(gpu):
     from bondupdategalerkin_amd import bug_step as B
     from bondupdategalerkin_amd import mps
@@ -113,6 +135,11 @@ def test_invalid_options_refused_before_touching_state(gpu):
         B.cbe_bug_step(psi, xxz_mpo(6), -0.1j, tol_root=1e-8, tol_expv=1e-8)
     with pytest.raises(ValueError):
         B.cbe_bug_step(psi, xxz_mpo(6), -0.1j, order=0)
+    with pytest.raises(ValueError):
+        B.cbe_bug_step(psi, xxz_mpo(6), -0.1j, work_ceiling_rule="bond_strict")
+    with pytest.raises(ValueError):
+        B.cbe_bug_step(psi, xxz_mpo(6), -0.1j, work_ceiling_rule="frame_root",
+                       root_ceiling_factor=2.5)
     for a, t in zip(before, psi.tensors):
         assert bool((a == t.data).all())
 

@@ -137,3 +137,61 @@ def test_invalid_options_are_refused_before_touching_the_state():
         obug.cbe_bug_step(psi, om.xxz_mpo(5), -0.1j)
     for a, b in zip(before, psi.sites):
         assert a.blocks.keys() == b.blocks.keys()
+
+
+def test_strict_and_frame_root_ceilings_closed_forms():
+    """bond_work_ceilings_strict and bond_frame_root_ceilings (basis_budget.jl:497-549) against
+    hand-worked values: ceil(f D_b) clipped to [1, maxdim], no local-dimension factor, the root
+    bond on its own factor; 1.1 * 10 must give 11, not 12 (the product is rounded to 9 digits
+    before the ceiling)."""
+    assert obug.bond_work_ceilings_strict([1, 3, 10, 100], 64, 1.5) == [2, 5, 15, 64]
+    assert obug.bond_work_ceilings_strict([10, 0], 64, 1.1) == [11, 1]
+    assert obug.bond_frame_root_ceilings([4, 8, 8, 4], 100, 1, 2.0, 1.2) == [8, 10, 16, 8]
+    assert obug.bond_frame_root_ceilings([4, 8, 8, 4], 12, 2, 2.0, 1.2) == [8, 12, 10, 8]
+    with pytest.raises(ValueError):
+        obug.bond_work_ceilings_strict([4], 64, 1.0)
+    with pytest.raises(ValueError):
+        obug.bond_frame_root_ceilings([4, 4], 64, 0, 1.5, 1.6)
+    with pytest.raises(IndexError):
+        obug.bond_frame_root_ceilings([4, 4], 64, 2)
+    # Rule resolution (bug_step_stages.jl:63-85, 116-129): strict needs an explicit factor above
+    # one; frame_root takes 2 as its frame factor when none is given, and refuses a root factor
+    # above the frame factor.
+    with pytest.raises(ValueError):
+        obug.work_ceilings("bond_strict", [2, 2], 2, 64, None, 0, 1.2)
+    assert obug.work_ceilings("frame_root", [2, 3], 2, 64, None, 1, 1.5) == [4, 5]
+    with pytest.raises(ValueError):
+        obug.work_ceilings("frame_root", [2, 3], 2, 64, None, 1, 2.5)
+    with pytest.raises(ValueError):
+        obug.work_ceilings("maxdim", [2, 3], 2, 64, 1.5, 1, 1.2)
+    assert obug.work_ceilings("maxdim", [2, 3], 2, 64, None, 1, 1.2) == 128
+
+
+def test_product_ceiling_helpers_equal_the_oracle():
+    from bondupdategalerkin_amd import bug_step as B
+
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        ranks = [int(x) for x in rng.integers(0, 300, size=int(rng.integers(1, 12)))]
+        maxdim = int(rng.integers(1, 400))
+        f = float(rng.choice([1.05, 1.1, 1.2, 1.5, 2.0, 2.5]))
+        rf = min(f, float(rng.choice([1.01, 1.1, 1.2])))
+        root = int(rng.integers(0, len(ranks)))
+        assert B.bond_work_ceilings_strict(ranks, maxdim, f) == \
+            obug.bond_work_ceilings_strict(ranks, maxdim, f)
+        assert B.bond_frame_root_ceilings(ranks, maxdim, root, f, rf) == \
+            obug.bond_frame_root_ceilings(ranks, maxdim, root, f, rf)
+
+
+def test_ceiling_rules_bound_the_step():
+    """A step under :bond_strict or :frame_root keeps every bond within the ceiling computed
+    from the bonds before it."""
+    for rule, f, rf in [("bond_strict", 1.5, 1.2), ("frame_root", None, 1.2),
+                        ("frame_root", 1.8, 1.5)]:
+        psi = obug.MPS(om.domain_wall_state(10, "U1"))
+        mpo = om.xxz_mpo(10, delta=0.5)
+        for _ in range(3):
+            cap = obug.work_ceilings(rule, psi.bond_dims(), 2, 128, f, 4, rf)
+            obug.cbe_bug_step(psi, mpo, -0.05j, obug.StepOptions(
+                work_ceiling_rule=rule, bond_ceiling_factor=f, root_ceiling_factor=rf, order=2))
+            assert all(b <= c for b, c in zip(psi.bond_dims(), cap))
