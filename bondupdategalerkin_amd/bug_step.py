@@ -207,18 +207,24 @@ class _Alloc:
     tail: float
     krylov: float
     scale: float
+    krylov_rank: float = 0.0             # > 0: ranked Krylov selection (basis_budget.jl:37-48)
 
 
-def basis_allocation(tol_basis, nbonds, tau, nrm, nfront, tol_krylov=None) -> _Alloc:
-    """The ``:linear`` rule the step uses (basis_budget.jl:75-104)."""
+def basis_allocation(tol_basis, nbonds, tau, nrm, nfront, tol_krylov=None,
+                     krylov_rank=0.0) -> _Alloc:
+    """The ``:linear`` rule the step uses (basis_budget.jl:75-104); ``krylov_rank`` (tol_krylov's
+    units) is apportioned by the same rule into the tail a ranked Krylov stage may discard."""
     nb, nf = max(nbonds, 1), max(nfront, 1)
     kr = 1.0 if tol_krylov is None else tol_krylov / tol_basis
     if kr <= 0:
         raise ValueError("tol_krylov must be positive")
+    if not krylov_rank >= 0:
+        raise ValueError("krylov_rank_tol must be >= 0")
     at = abs(tau)
     scale = 1.0 if at == 0 else at / max(nrm, np.finfo(float).eps)
     per_test = tol_basis / (2 * (nb + 1) * nf)
-    return _Alloc(2 * per_test, per_test, per_test, kr * per_test, scale)
+    return _Alloc(2 * per_test, per_test, per_test, kr * per_test, scale,
+                  krylov_rank / tol_basis * per_test)
 
 
 def bond_work_ceilings(ranks, d, maxdim, factor):
@@ -851,6 +857,21 @@ def extend_frame(frame, residual, direction, side, weight, allowance, room):
     return _oplus(frame, new, side), _width(new, side), fits
 
 
+def append_ranked(frame, candidates, side, allowance, workcap):
+    """``_append_ranked`` (krylov_enrichment.jl:185-196) -> (frame, added, fits): one SVD of the
+    direct sum of the weighted residuals, all outside the same frame, keeps the fewest top
+    directions whose discarded tail fits ``allowance``."""
+    if not candidates:
+        return frame, 0, True
+    room = workcap - _width(frame, side) if workcap > 0 else 1 << 60
+    block = candidates[0]
+    for c in candidates[1:]:
+        block = _oplus(block, c, side)
+    if room <= 0:
+        return frame, 0, T.norm(block) <= allowance
+    return extend_frame(frame, block, block, side, 1.0, allowance, room)
+
+
 @dataclass
 class _KrylovStats:
     eta_c: float = 0.0
@@ -869,6 +890,8 @@ def krylov_enrich(seed, applyH, start_frame, side, tau, alloc: _Alloc, workcap: 
     ``amp`` scales every weight (a frontier seed's norm relative to the state)."""
     perp = _perp(side)
     allow = alloc.krylov
+    ranked = alloc.krylov_rank > 0
+    candidates = []
     st = _KrylovStats()
     frame = [start_frame]
 
@@ -880,6 +903,9 @@ def krylov_enrich(seed, applyH, start_frame, side, tau, alloc: _Alloc, workcap: 
         else:
             st.eta_d = max(st.eta_d, eta)
         if not eta > allow:
+            return True
+        if ranked:                       # measured against start_frame, selected at the end
+            candidates.append(T.scale(weight, residual))
             return True
         room = workcap - _width(frame[0], side) if workcap > 0 else 1 << 60
         if room <= 0:
@@ -934,6 +960,10 @@ def krylov_enrich(seed, applyH, start_frame, side, tau, alloc: _Alloc, workcap: 
         prev_beta = beta
     if not closed:
         capped = True
+    if ranked:
+        frame[0], st.added, fits = append_ranked(start_frame, candidates, side, alloc.krylov_rank,
+                                                 workcap)
+        capped = capped or not fits
     st.capped = capped
     st.met = closed and not capped
     return frame[0], st
@@ -956,6 +986,7 @@ def _record_krylov(ledger: AdaptiveLedger, st: _KrylovStats):
 def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[float] = None,
                  tol_trunc: Optional[float] = None, tol_expv: Optional[float] = None,
                  tol_root: Optional[float] = None, tol_krylov: Optional[float] = None,
+                 krylov_rank_tol: float = 0.0,
                  order: int = 1, maxdim: Optional[int] = None, hermitian: bool = True,
                  work_ceiling_rule: str = "bond", bond_ceiling_factor: Optional[float] = None,
                  root_ceiling_factor: float = 1.2,
@@ -967,6 +998,11 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     enriching frontier sweep each, ``closure.py``); ``root`` is the 0-based root bond (the
     reference's ``root`` minus 1),
     ``rng`` the probe seed (default 0x5EED, fresh per call as the reference's default RNG).
+
+    ``krylov_rank_tol > 0`` ranks the local Krylov directions instead of appending them in chain
+    order (krylov_enrichment.jl:174-196,209-211): every direction above the allowance is collected
+    against the bond's starting frame, and one SVD of their weighted direct sum keeps the fewest
+    top ones whose discarded tail fits the apportioned ``krylov_rank_tol``.
 
     ``work_ceiling_rule`` caps every bond's working rank from the bonds before the step
     (bug_step_stages.jl:116-129): ``"bond"`` ceil(f d D_b) (f defaults to 1.2), ``"maxdim"`` one
@@ -1002,6 +1038,8 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         raise ValueError("tolerances must be positive (tol_trunc non-negative)")
     if order < 1:
         raise ValueError(f"order must be >= 1, got {order}")
+    if not krylov_rank_tol >= 0:
+        raise ValueError(f"krylov_rank_tol must be >= 0, got {krylov_rank_tol}")
     rules = ("maxdim", "bond", "bond_strict", "frame_root")
     if work_ceiling_rule not in rules:
         raise ValueError(f"work_ceiling_rule must be one of {rules}, got {work_ceiling_rule!r}")
@@ -1039,7 +1077,7 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     nrm = T.norm(psi[psi.center]) if 0 <= psi.center < L else 1.0
     if not math.isfinite(nrm):
         raise FloatingPointError("cbe_bug_step: the state is not finite")
-    alloc = basis_allocation(tb, L - 1, tau, nrm, order, tol_krylov)
+    alloc = basis_allocation(tb, L - 1, tau, nrm, order, tol_krylov, krylov_rank_tol)
     ledger = AdaptiveLedger(r_old=max(bd0 + [0]))
     recurrence = ex.krylov_mode if ex.krylov_mode != "auto" else ("lanczos" if hermitian else "arnoldi")
     expanded = [0] * (L - 1)

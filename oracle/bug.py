@@ -144,6 +144,7 @@ class StepOptions:
     tol_trunc: float = TOL_DEFAULTS["trunc"]
     tol_expv: float = TOL_DEFAULTS["expv"]
     tol_krylov: Optional[float] = None
+    krylov_rank_tol: float = 0.0        # > 0: ranked Krylov selection (bug_step.jl:58)
     order: int = 1
     maxdim: int = 128
     hermitian: bool = True
@@ -169,6 +170,7 @@ class Alloc:
     tail: float
     krylov: float
     scale: float
+    krylov_rank: float = 0.0            # > 0: ranked Krylov selection (basis_budget.jl:37-48)
 
 
 @dataclass
@@ -208,14 +210,19 @@ class Ledger:
         return "met" if self.tolerance_met else "budget_unmet"
 
 
-def basis_allocation(tol_basis, nbonds, tau, nrm, nfront, tol_krylov=None) -> Alloc:
-    """:linear rule of basis_allocation (basis_budget.jl:75-104)."""
+def basis_allocation(tol_basis, nbonds, tau, nrm, nfront, tol_krylov=None,
+                     krylov_rank=0.0) -> Alloc:
+    """:linear rule of basis_allocation (basis_budget.jl:75-104); ``krylov_rank`` (tol_krylov's
+    units) is apportioned by the same rule into the tail a ranked Krylov stage may discard."""
     nb, nf = max(nbonds, 1), max(nfront, 1)
     kr = 1.0 if tol_krylov is None else tol_krylov / tol_basis
+    if not krylov_rank >= 0:
+        raise ValueError("krylov_rank_tol must be >= 0")
     at = abs(tau)
     scale = 1.0 if at == 0 else at / max(nrm, np.finfo(float).eps)
     per_test = tol_basis / (2 * (nb + 1) * nf)
-    return Alloc(2 * per_test, per_test, per_test, kr * per_test, scale)
+    return Alloc(2 * per_test, per_test, per_test, kr * per_test, scale,
+                 krylov_rank / tol_basis * per_test)
 
 
 def bond_work_ceilings(ranks, d, maxdim, factor):
@@ -746,6 +753,20 @@ def extend_frame(frame: Fam, residual: Fam, direction: Fam, side: str, weight: f
     return oplus(frame, newb, side), _width_of(newb, side), fits
 
 
+def append_ranked(frame: Fam, candidates: List[Fam], side: str, allowance: float, workcap: int):
+    """_append_ranked (krylov_enrichment.jl:185-196) -> (frame, n_added, fits): one SVD of the
+    direct sum of the weighted residuals keeps the fewest top directions whose tail fits."""
+    if not candidates:
+        return frame, 0, True
+    room = workcap - _width_of(frame, side) if workcap > 0 else 1 << 60
+    block = candidates[0]
+    for c in candidates[1:]:
+        block = oplus(block, c, side)
+    if room <= 0:
+        return frame, 0, block.norm() <= allowance
+    return extend_frame(frame, block, block, side, 1.0, allowance, room)
+
+
 @dataclass
 class KrylovStats:
     eta_c: float = 0.0
@@ -765,6 +786,8 @@ def krylov_enrich(seed: Fam, applyH, start_frame: Fam, side: str, tau: complex, 
     ``state_seed = False`` weighs a non-unitary step absolutely (_residual_integral)."""
     perp = la.perp_left if side == "left" else la.perp_right
     allow = alloc.krylov
+    ranked = alloc.krylov_rank > 0
+    candidates: List[Fam] = []
     st = KrylovStats()
     state = {"frame": start_frame}
 
@@ -776,6 +799,9 @@ def krylov_enrich(seed: Fam, applyH, start_frame: Fam, side: str, tau: complex, 
         else:
             st.eta_d = max(st.eta_d, eta)
         if not eta > allow:
+            return True
+        if ranked:                       # measured against start_frame, selected at the end
+            candidates.append(residual.scale(weight))
             return True
         room = workcap - _width_of(state["frame"], side) if workcap > 0 else 1 << 60
         if room <= 0:
@@ -851,6 +877,10 @@ def krylov_enrich(seed: Fam, applyH, start_frame: Fam, side: str, tau: complex, 
                 break
     if not closed:
         capped = True
+    if ranked:
+        state["frame"], st.added, fits = append_ranked(start_frame, candidates, side,
+                                                       alloc.krylov_rank, workcap)
+        capped = capped or not fits
     st.capped = capped
     st.met = closed and not capped
     return state["frame"], st
@@ -903,7 +933,8 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
     wcap = work_ceilings(opts.work_ceiling_rule, bd0, d, opts.maxdim, opts.bond_ceiling_factor,
                          rl, opts.root_ceiling_factor)
     nrm = psi.sites[psi.center].norm() if 0 <= psi.center < L else 1.0
-    alloc = basis_allocation(opts.tol_basis, L - 1, tau, nrm, opts.order, opts.tol_krylov)
+    alloc = basis_allocation(opts.tol_basis, L - 1, tau, nrm, opts.order, opts.tol_krylov,
+                             opts.krylov_rank_tol)
     ledger = Ledger(r_old=max(bd0 + [0]))
     recurrence = "lanczos" if (opts.krylov_mode == "auto" and opts.hermitian) else (
         "arnoldi" if opts.krylov_mode == "auto" else opts.krylov_mode)
