@@ -109,6 +109,9 @@ class AdaptiveLedger:
     n_h1_closure: int = 0
     n_closure_sweeps: int = 0
     unmet: List[str] = field(default_factory=list)
+    # one entry per half-sweep bond visit (0-based bond), written by log_bond; after the
+    # compression each entry also holds the bond's r_old, r_final and relative discarded norm
+    bond_log: List[dict] = field(default_factory=list)
 
     def fail(self, what: str):
         self.tolerance_met = False
@@ -135,6 +138,19 @@ class AdaptiveLedger:
         for u in other.unmet:
             if u not in self.unmet:
                 self.unmet.append(u)
+        self.bond_log.extend(other.bond_log)
+
+
+def log_bond(ledger: AdaptiveLedger, bl: AdaptiveLedger, bond: int, side: str, width: int):
+    """Close one half-sweep bond visit (``_log_bond!``, basis_budget.jl:390-418): ``bl`` is the
+    fresh ledger the bond's Tests A/B/C/D wrote into; its summary is logged and it is merged
+    into ``ledger``, so the step's maxima and sums are the aggregate of the logged bonds."""
+    bl.bond_log.append(dict(
+        bond=bond, side=side, sample=bl.max_sample, tail=bl.max_tail, krylov_c=bl.max_krylov_c,
+        krylov_d=bl.max_krylov_d, n_probes=bl.n_probes, n_holdout=bl.n_holdout,
+        n_selected=bl.n_selected, n_targeted=bl.n_targeted, n_krylov=bl.n_krylov, n_h1=bl.n_h1,
+        workcap=bl.workcap_reached, met=bl.tolerance_met, unmet=list(bl.unmet), width=width))
+    ledger.merge(bl)
 
 
 _SIDE = {}
@@ -1107,7 +1123,8 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
                 _progress(f"left half-sweep bond {b}/{rl}")
             carried = psiL[b] if carry is None else T.core_times_site(carry, psiL[b])
             fr = frame_from(carried, psiL[b + 1])
-            xp = dd_cbe_expand(fr, mpo, b, lch, renvs[b + 2], "left", alloc, ledger, wcap[b], ex,
+            bl = AdaptiveLedger()
+            xp = dd_cbe_expand(fr, mpo, b, lch, renvs[b + 2], "left", alloc, bl, wcap[b], ex,
                                stream, probe_hook, cp)
             Kex = T.site_times_core(carried, T.overlap_right(psiL[b + 1], xp.V_ex))
             rch1 = ev.push_right(renvs[b + 2], xp.V_ex, mpo[b + 1], precision=cp)
@@ -1116,9 +1133,10 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
             Wk, kst = krylov_enrich(Kex, lambda v: ev.apply_one_site(Hl, Hw, Hr, v, precision=cp),
                                     xp.U_ex, "left", tau, alloc, wcap[b], recurrence,
                                     hermitian_tol=htol)
-            _record_krylov(ledger, kst)
+            _record_krylov(bl, kst)
             left_frames[b] = Wk
             expanded[b] = Wk.layout.col.total
+            log_bond(ledger, bl, b, "left", expanded[b])
             carry = T.overlap_left(Wk, carried)
             if b == rl:
                 lroot = lch
@@ -1142,7 +1160,8 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
                 _progress(f"right half-sweep bond {b}/{rl}")
             carried = psiR[b + 1] if carry is None else T.site_times_core(psiR[b + 1], carry)
             fr = frame_from(psiR[b], carried)
-            xp = dd_cbe_expand(fr, mpo, b, lenvs[b], rch, "right", alloc, ledger, wcap[b], ex,
+            bl = AdaptiveLedger()
+            xp = dd_cbe_expand(fr, mpo, b, lenvs[b], rch, "right", alloc, bl, wcap[b], ex,
                                stream, probe_hook, cp)
             Bex = T.core_times_site(T.overlap_left(xp.U_ex, psiR[b]), carried)
             lch1 = ev.push_left(lenvs[b], xp.U_ex, mpo[b], precision=cp)
@@ -1151,7 +1170,8 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
             Zk, kst = krylov_enrich(Bex, lambda v: ev.apply_one_site(Hl, Hw, Hr, v, precision=cp),
                                     xp.V_ex, "right", tau, alloc, wcap[b], recurrence,
                                     hermitian_tol=htol)
-            _record_krylov(ledger, kst)
+            _record_krylov(bl, kst)
+            log_bond(ledger, bl, b, "right", Zk.layout.row.total)
             right_frames[b] = Zk
             if b != rl:
                 expanded[b] = Zk.layout.row.total
@@ -1247,13 +1267,18 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         psi[b + 1] = right_frames[b]
     psi.center = rl + 1
     root_rank = psi[rl].layout.col.total
+    bwts = [0.0] * (L - 1)
     if truncate:
-        wtr, cumw, capped = truncate_recursive(psi, rl + 1, maxdim, tt)
+        wtr, cumw, capped = truncate_recursive(psi, rl + 1, maxdim, tt, bwts)
         discarded = max(discarded, wtr)
         ledger.discarded, ledger.maxdim_reached = cumw, capped
         if cumw > tt * (1 + 1e-8):
             ledger.fail("trunc")
-    ledger.r_final = max(psi.bond_dims() + [0])
+    bd1 = psi.bond_dims()
+    ledger.r_final = max(bd1 + [0])
+    # close the bond log with what exists only after the compression (diagnostics.jl:328-335)
+    for e in ledger.bond_log:
+        e.update(r_old=bd0[e["bond"]], r_final=bd1[e["bond"]], discarded=bwts[e["bond"]])
     secs["truncation"] = time.perf_counter() - ts
     secs["step"] = time.perf_counter() - t0
     return CBEBugSweepInfo(ledger, expanded, rl, root_rank, discarded, napply[0], secs)
@@ -1284,9 +1309,13 @@ def projection_miss(psi: SymMPS, left_frames, right_frames, rl: int) -> float:
     return math.sqrt(miss2)
 
 
-def truncate_recursive(psi: SymMPS, root: int, maxdim: int, weight_tol: float):
+def truncate_recursive(psi: SymMPS, root: int, maxdim: int, weight_tol: float,
+                       bond_weights: Optional[List[float]] = None):
     """Budgeted root-to-leaves compression (``truncate_recursive!``, truncation.jl:59-158);
-    ``root`` is the 0-based root site.  Returns (worst, cumulative, capped)."""
+    ``root`` is the 0-based root site.  Returns (worst, cumulative, capped); ``bond_weights``
+    (L - 1 entries) receives each bond's discarded norm relative to the state's."""
+    if bond_weights is not None and len(bond_weights) != len(psi) - 1:
+        raise ValueError("bond_weights needs one entry per bond")
     L = len(psi)
     sdiv = math.sqrt(root) + math.sqrt(L - 1 - root)
     st = {"nrm2": -1.0, "worst": 0.0, "capped": False}
@@ -1308,23 +1337,28 @@ def truncate_recursive(psi: SymMPS, root: int, maxdim: int, weight_tol: float):
             res = fac.select(0.0, keep)
         return res
 
-    def account(res, side):
+    def account(res, side, bond):
         st["worst"] = max(st["worst"], T.trunc_weight(res))
         cum[side] += T.sq_weight(res.trunc)
+        if bond_weights is not None:
+            bond_weights[bond] = T.sq_weight(res.trunc)
 
     for site in range(root, 0, -1):
         res = cut(psi[site], False)
-        account(res, 0)
+        account(res, 0, site - 1)
         psi[site] = res.Vh
         psi[site - 1] = T.site_times_core(psi[site - 1], T.diag_scale(res.U, res.S, "cols"))
         psi.center = site - 1
     canonical(psi, root)
     for site in range(root, L - 1):
         res = cut(psi[site], True)
-        account(res, 1)
+        account(res, 1, site)
         psi[site] = res.U
         psi[site + 1] = T.core_times_site(T.diag_scale(res.Vh, res.S, "rows"), psi[site + 1])
         psi.center = site + 1
     n2 = st["nrm2"]
     cumulative = (math.sqrt(cum[0]) + math.sqrt(cum[1])) / math.sqrt(n2) if n2 > 0 else 0.0
+    if bond_weights is not None:
+        for i, w in enumerate(bond_weights):
+            bond_weights[i] = math.sqrt(w / n2) if n2 > 0 else 0.0
     return st["worst"], cumulative, st["capped"]

@@ -199,6 +199,7 @@ class Ledger:
     leak: List[float] = field(default_factory=list)
     defect: float = 0.0
     unmet: List[str] = field(default_factory=list)
+    bond_log: List[dict] = field(default_factory=list)   # half-sweep bond visits, 0-based bonds
 
     def unmet_(self, what):
         self.tolerance_met = False
@@ -886,6 +887,18 @@ def krylov_enrich(seed: Fam, applyH, start_frame: Fam, side: str, tau: complex, 
     return state["frame"], st
 
 
+def log_bond(ledger: Ledger, bl: Ledger, bond: int, side: str, width: int):
+    """_log_bond! (basis_budget.jl:390-418): log the bond's own indicators and counts from its
+    fresh ledger ``bl``, then merge ``bl`` into ``ledger``."""
+    from .closure import _merge
+    bl.bond_log.append(dict(
+        bond=bond, side=side, sample=bl.max_sample, tail=bl.max_tail, krylov_c=bl.max_krylov_c,
+        krylov_d=bl.max_krylov_d, n_probes=bl.n_probes, n_holdout=bl.n_holdout,
+        n_selected=bl.n_selected, n_targeted=bl.n_targeted, n_krylov=bl.n_krylov, n_h1=bl.n_h1,
+        workcap=bl.workcap_reached, met=bl.tolerance_met, unmet=list(bl.unmet), width=width))
+    _merge(ledger, bl)
+
+
 def record_krylov(ledger: Ledger, st: KrylovStats, alloc: Alloc):
     ledger.max_krylov_c = max(ledger.max_krylov_c, st.eta_c)
     ledger.max_krylov_d = max(ledger.max_krylov_d, st.eta_d)
@@ -965,7 +978,8 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
     for b in range(rl + 1):
         carried = psiL.sites[b] if carry is None else la.core_times_site(carry, psiL.sites[b])
         fr = frame_from(carried, psiL.sites[b + 1])
-        ex = dd_cbe_expand(fr, mpo, b, lch, renvs[b + 2], "left", alloc, ledL,
+        bl = Ledger()
+        ex = dd_cbe_expand(fr, mpo, b, lch, renvs[b + 2], "left", alloc, bl,
                            _cap_at(wcap, b), opts, rngL, probe_hook)
         right_projected = la.overlap_right(psiL.sites[b + 1], ex.V_ex)
         Kex = la.site_times_core(carried, right_projected)
@@ -973,9 +987,10 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
         Hl, Hw, Hr = lch, mpo[b], rch1
         Wk, kst = krylov_enrich(Kex, lambda v: ob.apply_one_site(Hl, Hw, Hr, v), ex.U_ex, "left",
                                 tau, alloc, _cap_at(wcap, b), recurrence)
-        record_krylov(ledL, kst, alloc)
+        record_krylov(bl, kst, alloc)
         left_frames[b] = Wk
         expanded[b] = sum(Wk.col.values())
+        log_bond(ledL, bl, b, "left", expanded[b])
         carry = la.overlap_left(Wk, carried)
         if b == rl:
             lroot = lch
@@ -994,7 +1009,8 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
     for b in range(L - 2, rl - 1, -1):
         carried = psiR.sites[b + 1] if carry is None else la.site_times_core(psiR.sites[b + 1], carry)
         fr = frame_from(psiR.sites[b], carried)
-        ex = dd_cbe_expand(fr, mpo, b, lenvs[b], rch, "right", alloc, ledR,
+        bl = Ledger()
+        ex = dd_cbe_expand(fr, mpo, b, lenvs[b], rch, "right", alloc, bl,
                            _cap_at(wcap, b), opts, rngR, probe_hook)
         left_projected = la.overlap_left(ex.U_ex, psiR.sites[b])
         Bex = la.core_times_site(left_projected, carried)
@@ -1002,7 +1018,8 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
         Hl, Hw, Hr = lch1, mpo[b + 1], rch
         Zk, kst = krylov_enrich(Bex, lambda v: ob.apply_one_site(Hl, Hw, Hr, v), ex.V_ex, "right",
                                 tau, alloc, _cap_at(wcap, b), recurrence)
-        record_krylov(ledR, kst, alloc)
+        record_krylov(bl, kst, alloc)
+        log_bond(ledR, bl, b, "right", sum(Zk.row.values()))
         right_frames[b] = Zk
         if b != rl:
             expanded[b] = sum(Zk.row.values())
@@ -1078,14 +1095,18 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
         psi.sites[b + 1] = right_frames[b]
     psi.center = rl + 1
     root_rank = sum(psi.sites[rl].col.values())
+    bwts = [0.0] * (L - 1)
     if opts.truncate:
-        wtr, cumw, capped = truncate_recursive(psi, rl + 1, opts.maxdim, opts.tol_trunc)
+        wtr, cumw, capped = truncate_recursive(psi, rl + 1, opts.maxdim, opts.tol_trunc, bwts)
         discarded = max(discarded, wtr)
         ledger.discarded = cumw
         ledger.maxdim_reached = capped
         if cumw > opts.tol_trunc * (1 + 1e-8):
             ledger.unmet_("trunc")
-    ledger.r_final = max(psi.bond_dims() + [0])
+    bd1 = psi.bond_dims()
+    ledger.r_final = max(bd1 + [0])
+    for e in ledger.bond_log:            # diagnostics.jl:328-335
+        e.update(r_old=bd0[e["bond"]], r_final=bd1[e["bond"]], discarded=bwts[e["bond"]])
     return StepInfo(ledger, expanded, root_rank, discarded, napply[0])
 
 
@@ -1133,8 +1154,9 @@ def qr_core(C: Fam):
     return Fam(C.row, bond, [0], qb), Fam(bond, C.col, [0], rb)
 
 
-def truncate_recursive(psi: MPS, root: int, maxdim: int, weight_tol: float):
-    """truncate_recursive! with the budgeted rule (truncation.jl:59-158); root = 0-based site."""
+def truncate_recursive(psi: MPS, root: int, maxdim: int, weight_tol: float, bond_weights=None):
+    """truncate_recursive! with the budgeted rule (truncation.jl:59-158); root = 0-based site;
+    ``bond_weights`` (L - 1 entries) receives each bond's relative discarded norm."""
     L = len(psi)
     sdiv = math.sqrt(root) + math.sqrt(L - 1 - root)
     state = {"nrm2": -1.0, "worst": 0.0, "capped": False}
@@ -1156,13 +1178,15 @@ def truncate_recursive(psi: MPS, root: int, maxdim: int, weight_tol: float):
             res = svd(T, cutoff=0.0, nkeep=keep)
         return res
 
-    def account(res, side):
+    def account(res, side, bond):
         state["worst"] = max(state["worst"], la.trunc_weight(res))
         cum[side] += la.sq_weight(res.trunc)
+        if bond_weights is not None:
+            bond_weights[bond] = la.sq_weight(res.trunc)
 
     for site in range(root, 0, -1):
         res = cut(psi.sites[site], False)
-        account(res, 0)
+        account(res, 0, site - 1)
         psi.sites[site] = res.Vh
         US = la.diag_times(res.S, res.U, "right")
         psi.sites[site - 1] = la.site_times_core(psi.sites[site - 1], US)
@@ -1170,11 +1194,14 @@ def truncate_recursive(psi: MPS, root: int, maxdim: int, weight_tol: float):
     canonical(psi, root)
     for site in range(root, L - 1):
         res = cut(psi.sites[site], True)
-        account(res, 1)
+        account(res, 1, site)
         psi.sites[site] = res.U
         SV = la.diag_times(res.S, res.Vh, "left")
         psi.sites[site + 1] = la.core_times_site(SV, psi.sites[site + 1])
         psi.center = site + 1
     nrm2 = state["nrm2"]
     cumulative = (math.sqrt(cum[0]) + math.sqrt(cum[1])) / math.sqrt(nrm2) if nrm2 > 0 else 0.0
+    if bond_weights is not None:
+        for i, w in enumerate(bond_weights):
+            bond_weights[i] = math.sqrt(w / nrm2) if nrm2 > 0 else 0.0
     return state["worst"], cumulative, state["capped"]

@@ -167,6 +167,7 @@ def _merge(into, other):
     for u in other.unmet:
         if u not in into.unmet:
             into.unmet.append(u)
+    into.bond_log.extend(other.bond_log)
 
 
 def frontier_left(Fm, mpo, fr: KLFrames, tau, alloc, opts, enrich: bool, wcap, amp: float, rng,

@@ -30,6 +30,20 @@ def _dense(psi_dev):
     return om.mps_dense([fam_from_device(t) for t in psi_dev.tensors])
 
 
+_LOG_EXACT = ("bond", "side", "width", "n_probes", "n_holdout", "n_selected", "n_targeted",
+              "n_krylov", "n_h1", "workcap", "met", "unmet", "r_old", "r_final")
+
+
+def _same_bond_log(dev, ora):
+    """Per-bond logs agree: every count, verdict and rank exactly, the indicators and the
+    discarded norms to 1e-6 relative (FP64 on both sides, the same probes)."""
+    assert len(dev) == len(ora)
+    for d, o in zip(dev, ora):
+        assert {k: d[k] for k in _LOG_EXACT} == {k: o[k] for k in _LOG_EXACT}
+        for k in ("sample", "tail", "krylov_c", "krylov_d", "discarded"):
+            assert abs(d[k] - o[k]) <= 1e-6 * max(abs(d[k]), abs(o[k])) + 1e-12, (d["bond"], k)
+
+
 @pytest.mark.parametrize("start,L,delta,rule,steps,order", [("neel", 10, 0.0, "maxdim", 3, 1),
                                                              ("dw", 10, 0.0, "bond", 3, 1),
                                                              ("neel", 8, 0.5, "bond", 3, 1),
@@ -51,6 +65,7 @@ def test_step_matches_oracle(gpu, start, L, delta, rule, steps, order):
                                    obug.StepOptions(work_ceiling_rule=rule, order=order))
         assert psi_d.bond_dims() == psi_o.bond_dims(), n
         assert B.budget_status(info_d.ledger) == info_o.ledger.status, n
+        _same_bond_log(info_d.ledger.bond_log, info_o.ledger.bond_log)
         vd, vo = _dense(psi_d), om.mps_dense(psi_o.sites)
         assert np.linalg.norm(vd - vo) / np.linalg.norm(vo) <= 1e-10, n
         assert np.abs(mps.magnetisation(psi_d) - obug.sz_profile(psi_o)).max() <= 1e-9, n

@@ -1,5 +1,6 @@
 """Pin the oracle's whole CBE-BUG step against closed forms, ED and the reference's frozen runs."""
 import json
+import math
 import os
 
 import numpy as np
@@ -262,3 +263,44 @@ def test_ranked_step_tracks_free_fermions():
     occupied = [j for j in range(L // 2)]
     ref = free_fermion_sz(L, 6 * dt, occupied)
     assert np.abs(obug.sz_profile(psi) - ref).max() < 1e-7
+
+
+@pytest.mark.parametrize("start,delta,order", [("neel", 0.5, 1), ("dw", 0.0, 1), ("neel", 1.0, 2)])
+def test_bond_log_aggregates_to_the_step_ledger(start, delta, order):
+    """The per-bond log (_log_bond!, basis_budget.jl:390-418; diagnostics.jl:328-335): one entry
+    per half-sweep bond visit (the root bond from both sides), the step's maxima and sums are
+    exactly the aggregate of the entries, r_old/r_final are the bonds before and after, and the
+    per-bond discarded norms recombine into the cumulative one, sqrt(sum over the left subtree)
+    + sqrt(sum over the right subtree)."""
+    L = 10
+    st = {"neel": om.neel_state, "dw": om.domain_wall_state}[start](L, "U1")
+    psi = obug.MPS(st)
+    mpo = om.xxz_mpo(L, delta=delta)
+    for _ in range(3):
+        bd0 = psi.bond_dims()
+        info = obug.cbe_bug_step(psi, mpo, -0.05j, obug.StepOptions(order=order, maxdim=6))
+        led, log = info.ledger, info.ledger.bond_log
+        rl = L // 2 - 1
+        assert [(e["bond"], e["side"]) for e in log] == \
+            [(b, "left") for b in range(rl + 1)] + [(b, "right") for b in range(L - 2, rl - 1, -1)]
+        for key, agg in (("n_probes", led.n_probes), ("n_selected", led.n_selected),
+                         ("n_krylov", led.n_krylov), ("n_holdout", led.n_holdout),
+                         ("n_targeted", led.n_targeted)):
+            assert sum(e[key] for e in log) <= agg        # order > 1: closure sweeps add
+            if order == 1:
+                assert sum(e[key] for e in log) == agg, key
+        if order == 1:
+            assert max(e["sample"] for e in log) == led.max_sample
+            assert max(e["krylov_c"] for e in log) == led.max_krylov_c
+            bond_unmet = {u for e in log for u in e["unmet"]}
+            assert bond_unmet == set(led.unmet) - {"trunc", "projection", "expv"}
+            assert all(e["met"] == (not e["unmet"]) for e in log)
+        bd1 = psi.bond_dims()
+        assert [e["r_old"] for e in log] == [bd0[e["bond"]] for e in log]
+        assert [e["r_final"] for e in log] == [bd1[e["bond"]] for e in log]
+        w = {e["bond"]: e["discarded"] for e in log}
+        left = math.sqrt(sum(w[b] ** 2 for b in range(rl + 1)))
+        right = math.sqrt(sum(w[b] ** 2 for b in range(rl + 1, L - 1)))
+        assert left + right == pytest.approx(led.discarded, rel=1e-12, abs=1e-15)
+        widths = {(e["bond"], e["side"]): e["width"] for e in log}
+        assert all(widths[(b, "left")] == info.expanded[b] for b in range(rl + 1))
