@@ -728,7 +728,6 @@ def _dd_cbe_adaptive(fr: BondFrame, frame, opp, side: str, oside: str, lenv, ren
                      prec: int) -> "_Expansion":
     """dd_cbe_expand's ``:adaptive`` branch (dd_cbe_expand.jl:88-118,362-433)."""
     left = side == "left"
-    none = _Expansion(fr.U0, fr.V0, 0)
     dmax = max(fr.U0.layout.row.total, fr.old_rank, fr.V0.layout.col.total)
     width0 = math.ceil(1.2 * max(math.ceil(ADAPTIVE_GROWTH * dmax) - fr.old_rank, 1))
     wcap = min(full_opp, workcap) if workcap > 0 else full_opp
@@ -743,6 +742,60 @@ def _dd_cbe_adaptive(fr: BondFrame, frame, opp, side: str, oside: str, lenv, ren
     directions, _status = dd_controller(
         frame, side, lambda w, b: dd_probe(opp, oside, w, stream, b), sketch, project, width0,
         wcap, cap, alloc, ledger, holdout_width=ex.holdout_width)
+    return _finish_expansion(fr, frame, side, directions, cap, wbind, ledger)
+
+
+def exact_probe(frame: BlockTensor, side: str) -> Optional[BlockTensor]:
+    """``_dd_probe`` with exact = true (dd_cbe.jl:141-148,155-160): the whole fused local space
+    of ``frame`` (its fusion basis: per fused charge the slices stacked in ascending order, one
+    identity block each) with the frame projected out.  No random numbers."""
+    lay = frame.layout
+    dfull = _fused_dims(lay, side)
+    dT = _widths(lay, side)
+    if sum(max(d - dT.get(q, 0), 0) for q, d in dfull.items()) == 0:
+        return None
+    link = lay.row if side == "left" else lay.col
+    wl = sorted((q, d) for q, d in dfull.items() if d > 0)
+    wleg = Leg.make(wl)
+    play = Layout(link, wleg, lay.shifts) if side == "left" else Layout(wleg, link, lay.shifts)
+    blocks = {}
+    for q, df in wl:
+        off = 0
+        for s, sh in enumerate(lay.shifts):
+            lq = q - sh if side == "left" else q + sh
+            dl = link.dim_of(lq)
+            if dl == 0:
+                continue
+            blk = np.zeros((dl, df), dtype=complex)
+            blk[:, off: off + dl] = np.eye(dl)
+            off += dl
+            blocks[(s, lq) if side == "left" else (s, q)] = blk if side == "left" else blk.T.copy()
+    Om = BlockTensor.from_blocks(play, blocks)
+    proj = T.perp_left(frame, Om) if side == "left" else T.perp_right(frame, Om)
+    return proj if T.norm(proj) > 0 else None
+
+
+def _dd_cbe_exact(fr: BondFrame, frame, opp, side: str, oside: str, lenv, renv, W1, W2,
+                  ledger: "AdaptiveLedger", cap: int, wbind: bool, prec: int) -> "_Expansion":
+    """dd_cbe_expand with exact = true (dd_cbe_expand.jl:91-97,346-360): ``_certified_range``
+    without an allocation is ``_range_find`` (range_finder.jl:40-48, sketch.jl:119-132), whose one
+    pass at the full width sketches the exact probe and keeps ``_dd_select`` of it; no probes or
+    holdouts are counted."""
+    Om = exact_probe(opp, oside)
+    if Om is None:
+        return _Expansion(fr.U0, fr.V0, 0)
+    if side == "left":
+        Ysk = stream_left(lenv, W1, W2, renv, T.site_times_core(fr.U0, fr.S0), fr.V0, Om, prec)
+    else:
+        Ysk = stream_right(lenv, W1, W2, renv, fr.U0, T.core_times_site(fr.S0, fr.V0), Om, prec)
+    return _finish_expansion(fr, frame, side, dd_select(frame, Ysk, side), cap, wbind, ledger)
+
+
+def _finish_expansion(fr: BondFrame, frame, side: str, directions, cap: int, wbind: bool,
+                      ledger: "AdaptiveLedger") -> "_Expansion":
+    """The common end of dd_cbe_expand (dd_cbe_expand.jl:383-419)."""
+    left = side == "left"
+    none = _Expansion(fr.U0, fr.V0, 0)
     if directions is None:
         return none
     count = _width(directions, side)
@@ -763,8 +816,9 @@ def _dd_cbe_adaptive(fr: BondFrame, frame, opp, side: str, oside: str, lenv, ren
 
 def dd_cbe_expand(fr: BondFrame, mpo: MPO, bond: int, lenv, renv, side: str, alloc: _Alloc,
                   ledger: AdaptiveLedger, workcap: int, ex: DDQRExpansion, stream: ProbeStream,
-                  probe_hook=None, prec: int = _lib.PREC_FP64) -> _Expansion:
-    """One side of one bond, ``:rank_tail`` (dd_cbe_expand.jl:19-87,252-301,383-419)."""
+                  probe_hook=None, prec: int = _lib.PREC_FP64, exact: bool = False) -> _Expansion:
+    """One side of one bond, ``:rank_tail`` (dd_cbe_expand.jl:19-87,252-301,383-419), the
+    adaptive controller, or the deterministic ``exact`` reference."""
     left = side == "left"
     frame, opp = (fr.U0, fr.V0) if left else (fr.V0, fr.U0)
     oside = "right" if left else "left"
@@ -780,6 +834,9 @@ def dd_cbe_expand(fr: BondFrame, mpo: MPO, bond: int, lenv, renv, side: str, all
     if cap <= 0:
         return none
     W1, W2 = mpo[bond], mpo[bond + 1]
+    if exact:
+        return _dd_cbe_exact(fr, frame, opp, side, oside, lenv, renv, W1, W2, ledger, cap, wbind,
+                             prec)
     if ex.mode == "adaptive":
         if probe_hook is not None:
             raise ValueError("probe_hook observes the rank_tail probes only")
@@ -1002,7 +1059,7 @@ def _record_krylov(ledger: AdaptiveLedger, st: _KrylovStats):
 def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[float] = None,
                  tol_trunc: Optional[float] = None, tol_expv: Optional[float] = None,
                  tol_root: Optional[float] = None, tol_krylov: Optional[float] = None,
-                 krylov_rank_tol: float = 0.0,
+                 krylov_rank_tol: float = 0.0, exact: bool = False,
                  order: int = 1, maxdim: Optional[int] = None, hermitian: bool = True,
                  work_ceiling_rule: str = "bond", bond_ceiling_factor: Optional[float] = None,
                  root_ceiling_factor: float = 1.2,
@@ -1019,6 +1076,11 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     order (krylov_enrichment.jl:174-196,209-211): every direction above the allowance is collected
     against the bond's starting frame, and one SVD of their weighted direct sum keeps the fewest
     top ones whose discarded tail fits the apportioned ``krylov_rank_tol``.
+
+    ``exact = True`` is the reference's deterministic debug path (bug_step.jl:117-119): each
+    bond's DD expansion sketches the whole discarded space of the opposite frame instead of
+    Gaussian probes, so the step draws no random numbers and serial and ``parallel`` half-sweeps
+    agree; order 1 only.
 
     ``work_ceiling_rule`` caps every bond's working rank from the bonds before the step
     (bug_step_stages.jl:116-129): ``"bond"`` ceil(f d D_b) (f defaults to 1.2), ``"maxdim"`` one
@@ -1056,6 +1118,9 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         raise ValueError(f"order must be >= 1, got {order}")
     if not krylov_rank_tol >= 0:
         raise ValueError(f"krylov_rank_tol must be >= 0, got {krylov_rank_tol}")
+    if exact and order > 1:
+        raise NotImplementedError("exact = true with order > 1 measures the unfolded H*Theta leak "
+                                  "(bug_step_stages.jl:67), which this backend does not build")
     rules = ("maxdim", "bond", "bond_strict", "frame_root")
     if work_ceiling_rule not in rules:
         raise ValueError(f"work_ceiling_rule must be one of {rules}, got {work_ceiling_rule!r}")
@@ -1125,7 +1190,7 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
             fr = frame_from(carried, psiL[b + 1])
             bl = AdaptiveLedger()
             xp = dd_cbe_expand(fr, mpo, b, lch, renvs[b + 2], "left", alloc, bl, wcap[b], ex,
-                               stream, probe_hook, cp)
+                               stream, probe_hook, cp, exact)
             Kex = T.site_times_core(carried, T.overlap_right(psiL[b + 1], xp.V_ex))
             rch1 = ev.push_right(renvs[b + 2], xp.V_ex, mpo[b + 1], precision=cp)
             renvs[b + 2] = None
@@ -1162,7 +1227,7 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
             fr = frame_from(psiR[b], carried)
             bl = AdaptiveLedger()
             xp = dd_cbe_expand(fr, mpo, b, lenvs[b], rch, "right", alloc, bl, wcap[b], ex,
-                               stream, probe_hook, cp)
+                               stream, probe_hook, cp, exact)
             Bex = T.core_times_site(T.overlap_left(xp.U_ex, psiR[b]), carried)
             lch1 = ev.push_left(lenvs[b], xp.U_ex, mpo[b], precision=cp)
             lenvs[b] = None

@@ -145,6 +145,7 @@ class StepOptions:
     tol_expv: float = TOL_DEFAULTS["expv"]
     tol_krylov: Optional[float] = None
     krylov_rank_tol: float = 0.0        # > 0: ranked Krylov selection (bug_step.jl:58)
+    exact: bool = False                 # deterministic DD reference (bug_step.jl:117-119)
     order: int = 1
     maxdim: int = 128
     hermitian: bool = True
@@ -608,6 +609,8 @@ def dd_cbe_expand(fr: BondFrame, mpo, bond: int, lenv: Fam, renv: Fam, side: str
     if cap <= 0:
         return none
     W1, W2 = mpo[bond], mpo[bond + 1]
+    if opts.exact:
+        return _dd_cbe_exact(fr, frame, opp, side, oside, lenv, renv, W1, W2, ledger, cap, wbind)
     if opts.mode == "adaptive":
         return _dd_cbe_adaptive(fr, frame, opp, side, oside, lenv, renv, W1, W2, alloc, ledger,
                                 cap, wbind, full_opp, workcap, opts, rng)
@@ -694,7 +697,6 @@ def _dd_cbe_adaptive(fr: BondFrame, frame: Fam, opp: Fam, side: str, oside: str,
     from . import dd_adaptive as da
 
     left = side == "left"
-    none = Expansion(fr.U0, fr.V0, 0)
     width0 = da.initial_width(fr.U0, fr.V0, fr.old_rank)
     wcap = full_opp
     if workcap > 0:
@@ -713,6 +715,62 @@ def _dd_cbe_adaptive(fr: BondFrame, frame: Fam, opp: Fam, side: str, oside: str,
 
     directions, _status = da.dd_controller(frame, side, draw, sketch, project, width0, wcap, cap,
                                            alloc, ledger, holdout_width=opts.holdout_width)
+    return _finish_expansion(fr, frame, side, directions, cap, wbind, ledger)
+
+
+def exact_probe(frame: Fam, side: str) -> Optional[Fam]:
+    """_dd_probe with exact = true (dd_cbe.jl:141-148,155-160): the whole fused local space of
+    ``frame`` (its fusion basis: per fused charge the slices stacked in ascending order, one
+    identity block each) with the frame projected out.  No random numbers."""
+    dfull = fused_dims(frame, side)
+    dT = frame_widths(frame, side)
+    if sum(max(d - dT.get(q, 0), 0) for q, d in dfull.items()) == 0:
+        return None
+    link = frame.row if side == "left" else frame.col
+    wl = {q: d for q, d in sorted(dfull.items()) if d > 0}
+    Om = Fam(frame.row, wl, frame.shifts) if side == "left" else Fam(wl, frame.col, frame.shifts)
+    for q, df in wl.items():
+        off = 0
+        for s, sh in enumerate(frame.shifts):
+            lq = q - sh if side == "left" else q + sh
+            dl = link.get(lq, 0)
+            if dl == 0:
+                continue
+            blk = np.zeros((dl, df), dtype=complex)
+            blk[:, off: off + dl] = np.eye(dl)
+            off += dl
+            if side == "left":
+                Om.blocks[(s, lq)] = blk
+            else:
+                Om.blocks[(s, q)] = blk.T.copy()
+    proj = la.perp_left(frame, Om) if side == "left" else la.perp_right(frame, Om)
+    return proj if proj.norm() > 0 else None
+
+
+def _dd_cbe_exact(fr: BondFrame, frame: Fam, opp: Fam, side: str, oside: str, lenv: Fam,
+                  renv: Fam, W1, W2, ledger: Ledger, cap: int, wbind: bool) -> Expansion:
+    """dd_cbe_expand with exact = true (dd_cbe_expand.jl:91-97,346-360): _certified_range without
+    an allocation is _range_find (range_finder.jl:40-48, sketch.jl:119-132), whose one pass at
+    the full width sketches the exact probe and keeps _dd_select of it; no probes or holdouts
+    are counted."""
+    from . import dd_adaptive as da
+
+    Om = exact_probe(opp, oside)
+    if Om is None:
+        return Expansion(fr.U0, fr.V0, 0)
+    if side == "left":
+        Ysk = stream_left(lenv, W1, W2, renv, la.site_times_core(fr.U0, fr.S0), fr.V0, Om)
+    else:
+        Ysk = stream_right(lenv, W1, W2, renv, fr.U0, la.core_times_site(fr.S0, fr.V0), Om)
+    directions = da.dd_select(frame, Ysk, side)
+    return _finish_expansion(fr, frame, side, directions, cap, wbind, ledger)
+
+
+def _finish_expansion(fr: BondFrame, frame: Fam, side: str, directions: Optional[Fam], cap: int,
+                      wbind: bool, ledger: Ledger) -> Expansion:
+    """The common end of dd_cbe_expand (dd_cbe_expand.jl:383-419)."""
+    left = side == "left"
+    none = Expansion(fr.U0, fr.V0, 0)
     if directions is None:
         return none
     count = _width_of(directions, side)
@@ -938,6 +996,9 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
         raise ValueError("cbe_bug_step needs at least two sites")
     if opts.order < 1:
         raise ValueError("order must be >= 1")
+    if opts.exact and opts.order > 1:
+        raise NotImplementedError("exact = true with order > 1 measures the unfolded H*Theta "
+                                  "leak (bug_step_stages.jl:67), which is not restated")
     rl = (max(1, min(L - 1, L // 2)) - 1) if opts.root is None else opts.root
     d = len(psi.sites[0].shifts)
     tau = complex(tau)

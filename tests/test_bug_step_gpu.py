@@ -160,6 +160,32 @@ def test_ranked_krylov_selection_matches_oracle(gpu, start, delta, order, krt):
         assert np.linalg.norm(vd - vo) / np.linalg.norm(vo) <= 2e-9, n
 
 
+@pytest.mark.parametrize("start,L,delta,rule", [("neel", 10, 0.5, "bond"), ("dw", 12, 0.0, "maxdim"),
+                                              ("neel", 8, 1.0, "bond")])
+def test_exact_step_matches_oracle_and_is_deterministic(gpu, start, L, delta, rule):
+    """exact = true (bug_step.jl:117-119; the exact probe dd_cbe.jl:147-148 and one _range_find
+    pass, range_finder.jl:40-48): the device step equals the oracle's (bonds, per-bond logs,
+    state to 1e-10), and the parallel half-sweeps with another seed give the serial state to
+    1e-12 (no random numbers are drawn)."""
+    from bondupdategalerkin_amd import bug_step as B
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    psi_d, psi_p, psi_o = _device_state(start, L), _device_state(start, L), _oracle_state(start, L)
+    mpo_d, mpo_o = xxz_mpo(L, delta=delta), om.xxz_mpo(L, delta=delta)
+    for n in range(3):
+        info_d = B.cbe_bug_step(psi_d, mpo_d, -0.05j, exact=True, work_ceiling_rule=rule)
+        B.cbe_bug_step(psi_p, mpo_d, -0.05j, exact=True, work_ceiling_rule=rule, parallel=True,
+                       rng=1234)
+        info_o = obug.cbe_bug_step(psi_o, mpo_o, -0.05j,
+                                   obug.StepOptions(exact=True, work_ceiling_rule=rule))
+        assert psi_d.bond_dims() == psi_o.bond_dims() == psi_p.bond_dims(), n
+        assert info_d.ledger.n_probes == 0 and info_d.ledger.n_selected == info_o.ledger.n_selected
+        _same_bond_log(info_d.ledger.bond_log, info_o.ledger.bond_log)
+        vd, vo, vp = _dense(psi_d), om.mps_dense(psi_o.sites), _dense(psi_p)
+        assert np.linalg.norm(vd - vo) / np.linalg.norm(vo) <= 1e-10, n
+        assert np.linalg.norm(vp - vd) / np.linalg.norm(vd) <= 1e-12, n
+
+
 def test_invalid_options_refused_before_touching_state(gpu):
     from bondupdategalerkin_amd import bug_step as B
     from bondupdategalerkin_amd import mps
@@ -177,6 +203,8 @@ def test_invalid_options_refused_before_touching_state(gpu):
         B.cbe_bug_step(psi, xxz_mpo(6), -0.1j, order=0)
     with pytest.raises(ValueError):
         B.cbe_bug_step(psi, xxz_mpo(6), -0.1j, krylov_rank_tol=-1e-6)
+    with pytest.raises(NotImplementedError):
+        B.cbe_bug_step(psi, xxz_mpo(6), -0.1j, exact=True, order=2)
     with pytest.raises(ValueError):
         B.cbe_bug_step(psi, xxz_mpo(6), -0.1j, work_ceiling_rule="bond_strict")
     with pytest.raises(ValueError):

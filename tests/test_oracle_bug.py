@@ -304,3 +304,62 @@ def test_bond_log_aggregates_to_the_step_ledger(start, delta, order):
         assert left + right == pytest.approx(led.discarded, rel=1e-12, abs=1e-15)
         widths = {(e["bond"], e["side"]): e["width"] for e in log}
         assert all(widths[(b, "left")] == info.expanded[b] for b in range(rl + 1))
+
+
+def test_exact_probe_spans_the_discarded_space():
+    """exact = true probes (dd_cbe.jl:141-160): the fusion basis with the frame projected out is
+    the orthogonal projector onto the frame's complement, Om Om^dag = I - F F^dag on the fused
+    local space (left frames) and Om^dag Om = I - F^dag F (right frames)."""
+    from oracle import linalg as la
+
+    rng = np.random.default_rng(5)
+    row, shifts = {-1: 2, 1: 3}, [1, -1]
+    A = obug.Fam.zeros(row, {0: 2, 2: 1}, shifts)
+    for blk in A.blocks.values():
+        blk[...] = rng.standard_normal(blk.shape) + 1j * rng.standard_normal(blk.shape)
+    F = la.svd_left(A).U
+    Om = obug.exact_probe(F, "left")
+    O, Fd = _left_dense(Om), _left_dense(F)
+    assert np.abs(O @ O.conj().T - (np.eye(O.shape[0]) - Fd @ Fd.conj().T)).max() < 1e-13
+    Vr = la.svd_right(obug.Fam.zeros({0: 2, -2: 1}, row, [-1, 1]).axpy(
+        1.0, _random_like(obug.Fam.zeros({0: 2, -2: 1}, row, [-1, 1]), rng))).Vh
+    Omr = obug.exact_probe(Vr, "right")
+    dr, dv = Omr.dense(), Vr.dense()
+    Or = np.concatenate(list(dr), axis=1)          # (width, slices * link)
+    Vd = np.concatenate(list(dv), axis=1)
+    assert np.abs(Or.conj().T @ Or - (np.eye(Or.shape[1]) - Vd.conj().T @ Vd)).max() < 1e-13
+
+
+def _random_like(f, rng):
+    for blk in f.blocks.values():
+        blk[...] = rng.standard_normal(blk.shape) + 1j * rng.standard_normal(blk.shape)
+    return f
+
+
+def test_exact_step_is_deterministic_and_tracks_free_fermions():
+    """exact = true (bug_step.jl:117-119) draws no random numbers: serial and parallel half-sweeps
+    give the identical state (test_parallel_half_sweeps.jl:6-9), any seed gives the same step,
+    no probe is counted, and on the XX domain wall it gives the Gaussian-probe step's state (both
+    span the discarded space the sketch needs) and its free-fermion error, 1.8e-5 after 6
+    order-1 steps of dt = 0.05; order > 1 is refused."""
+    L, dt = 10, 0.05
+    runs = []
+    for par, seed in ((False, 1), (True, 2)):
+        psi = obug.MPS(om.domain_wall_state(L, "U1"))
+        mpo = om.xxz_mpo(L, delta=0.0)
+        for _ in range(6):
+            info = obug.cbe_bug_step(psi, mpo, -1j * dt, obug.StepOptions(
+                exact=True, parallel=par, seed=seed, tol_basis=1e-6))
+            assert info.ledger.n_probes == 0 and info.ledger.n_holdout == 0
+        runs.append(om.mps_dense(psi.sites))
+    assert np.array_equal(runs[0], runs[1])
+    ref = free_fermion_sz(L, 6 * dt, list(range(L // 2)))
+    assert np.abs(obug.sz_profile(psi) - ref).max() < 2e-5
+    gauss = obug.MPS(om.domain_wall_state(L, "U1"))
+    for _ in range(6):
+        obug.cbe_bug_step(gauss, om.xxz_mpo(L, delta=0.0), -1j * dt,
+                          obug.StepOptions(tol_basis=1e-6))
+    assert np.abs(obug.sz_profile(psi) - obug.sz_profile(gauss)).max() < 1e-12
+    with pytest.raises(NotImplementedError):
+        obug.cbe_bug_step(obug.MPS(om.neel_state(6)), om.xxz_mpo(6), -0.1j,
+                          obug.StepOptions(exact=True, order=2))
