@@ -745,17 +745,16 @@ def _dd_cbe_adaptive(fr: BondFrame, frame, opp, side: str, oside: str, lenv, ren
     return _finish_expansion(fr, frame, side, directions, cap, wbind, ledger)
 
 
-def exact_probe(frame: BlockTensor, side: str) -> Optional[BlockTensor]:
-    """``_dd_probe`` with exact = true (dd_cbe.jl:141-148,155-160): the whole fused local space
-    of ``frame`` (its fusion basis: per fused charge the slices stacked in ascending order, one
-    identity block each) with the frame projected out.  No random numbers."""
+def fusion_probe(frame: BlockTensor, side: str) -> Optional[BlockTensor]:
+    """The fusion basis of ``frame``'s fused local space (``fusion_basis``, dd_cbe.jl:143): per
+    fused charge the slices stacked in ascending order, one identity block each, in the probe
+    layout of ``_graded_draw`` (a unitary change of basis of the fused legs)."""
     lay = frame.layout
     dfull = _fused_dims(lay, side)
-    dT = _widths(lay, side)
-    if sum(max(d - dT.get(q, 0), 0) for q, d in dfull.items()) == 0:
-        return None
     link = lay.row if side == "left" else lay.col
     wl = sorted((q, d) for q, d in dfull.items() if d > 0)
+    if not wl:
+        return None
     wleg = Leg.make(wl)
     play = Layout(link, wleg, lay.shifts) if side == "left" else Layout(wleg, link, lay.shifts)
     blocks = {}
@@ -770,7 +769,18 @@ def exact_probe(frame: BlockTensor, side: str) -> Optional[BlockTensor]:
             blk[:, off: off + dl] = np.eye(dl)
             off += dl
             blocks[(s, lq) if side == "left" else (s, q)] = blk if side == "left" else blk.T.copy()
-    Om = BlockTensor.from_blocks(play, blocks)
+    return BlockTensor.from_blocks(play, blocks)
+
+
+def exact_probe(frame: BlockTensor, side: str) -> Optional[BlockTensor]:
+    """``_dd_probe`` with exact = true (dd_cbe.jl:141-148,155-160): the fusion basis with the
+    frame projected out, i.e. the whole discarded space.  No random numbers."""
+    lay = frame.layout
+    dfull = _fused_dims(lay, side)
+    dT = _widths(lay, side)
+    if sum(max(d - dT.get(q, 0), 0) for q, d in dfull.items()) == 0:
+        return None
+    Om = fusion_probe(frame, side)
     proj = T.perp_left(frame, Om) if side == "left" else T.perp_right(frame, Om)
     return proj if T.norm(proj) > 0 else None
 
@@ -1060,6 +1070,7 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
                  tol_trunc: Optional[float] = None, tol_expv: Optional[float] = None,
                  tol_root: Optional[float] = None, tol_krylov: Optional[float] = None,
                  krylov_rank_tol: float = 0.0, exact: bool = False,
+                 fold_omega: Optional[bool] = None,
                  order: int = 1, maxdim: Optional[int] = None, hermitian: bool = True,
                  work_ceiling_rule: str = "bond", bond_ceiling_factor: Optional[float] = None,
                  root_ceiling_factor: float = 1.2,
@@ -1080,7 +1091,9 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     ``exact = True`` is the reference's deterministic debug path (bug_step.jl:117-119): each
     bond's DD expansion sketches the whole discarded space of the opposite frame instead of
     Gaussian probes, so the step draws no random numbers and serial and ``parallel`` half-sweeps
-    agree; order 1 only.
+    agree.  ``fold_omega`` (default: ``order > 1 and not exact``, bug_step_stages.jl:67) picks the
+    closure's frontier leak: folded (a Gaussian sketch of the two-site action, never formed) or
+    unfolded (the exact two-site action's outside-space norm).
 
     ``work_ceiling_rule`` caps every bond's working rank from the bonds before the step
     (bug_step_stages.jl:116-129): ``"bond"`` ceil(f d D_b) (f defaults to 1.2), ``"maxdim"`` one
@@ -1118,9 +1131,7 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         raise ValueError(f"order must be >= 1, got {order}")
     if not krylov_rank_tol >= 0:
         raise ValueError(f"krylov_rank_tol must be >= 0, got {krylov_rank_tol}")
-    if exact and order > 1:
-        raise NotImplementedError("exact = true with order > 1 measures the unfolded H*Theta leak "
-                                  "(bug_step_stages.jl:67), which this backend does not build")
+    fold = (order > 1 and not exact) if fold_omega is None else bool(fold_omega)
     rules = ("maxdim", "bond", "bond_strict", "frame_root")
     if work_ceiling_rule not in rules:
         raise ValueError(f"work_ceiling_rule must be one of {rules}, got {work_ceiling_rule!r}")
@@ -1275,7 +1286,7 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         ts = time.perf_counter()
         _progress("frontier closure")
         fr = KLFrames(left_frames, right_frames, rl, lroot, rroot)
-        cx = _Ctx(mpo, tau, alloc, wcap, ex, stream, recurrence, cp, htol)
+        cx = _Ctx(mpo, tau, alloc, wcap, ex, stream, recurrence, cp, htol, fold, exact)
         root_start, _ = single_pass_closure(fr, psi, root_start, cx, nrm, ledger, order)
         lroot, rroot = fr.lenv, fr.renv
         for b in range(rl):

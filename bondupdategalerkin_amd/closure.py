@@ -63,10 +63,12 @@ class SweepResult:
 class _Ctx:
     """What every bond of a frontier sweep shares: the step's settings and the probe stream."""
 
-    def __init__(self, mpo, tau, alloc, wcap, ex, stream, recurrence, prec, htol):
+    def __init__(self, mpo, tau, alloc, wcap, ex, stream, recurrence, prec, htol, fold=True,
+                 exact=False):
         self.mpo, self.tau, self.alloc, self.wcap = mpo, tau, alloc, wcap
         self.ex, self.stream, self.recurrence = ex, stream, recurrence
         self.prec, self.htol = prec, htol
+        self.fold, self.exact = fold, exact
 
 
 def root_generator(mpo, fr: KLFrames, prec: int):
@@ -181,6 +183,32 @@ def sampled_leak(fr, cx: _Ctx, bond: int, lenv, renv, side: str, count: int) -> 
     return sample_residual(None, project, Y, Om, side)
 
 
+def exact_leak(fr, cx: _Ctx, bond: int, lenv, renv, side: str) -> float:
+    """The unfolded leak (``_bond_action`` with fold = false, ``_left_leak``/``_right_leak``,
+    krylov_frontier_bonds.jl:57-106): ||(I - U0 U0^dag) X|| on the left, ||X (I - V0^dag V0)|| on
+    the right, of the exact two-site action X = H Theta_F, formed by streaming the full fusion
+    basis of the opposite frame (a unitary change of basis of its legs) through the action."""
+    from .bug_step import fusion_probe, stream_left, stream_right
+    W1, W2 = cx.mpo[bond], cx.mpo[bond + 1]
+    if side == "left":
+        Om = fusion_probe(fr.V0, "right")
+        if Om is None:
+            return 0.0
+        Y = stream_left(lenv, W1, W2, renv, T.site_times_core(fr.U0, fr.S0), fr.V0, Om, cx.prec)
+        return T.norm(T.perp_left(fr.U0, Y))
+    Om = fusion_probe(fr.U0, "left")
+    if Om is None:
+        return 0.0
+    Y = stream_right(lenv, W1, W2, renv, fr.U0, T.core_times_site(fr.S0, fr.V0), Om, cx.prec)
+    return T.norm(T.perp_right(fr.V0, Y))
+
+
+def _leak(tsf, cx: _Ctx, bond: int, lenv, renv, side: str, width: int) -> float:
+    if cx.fold:
+        return sampled_leak(tsf, cx, bond, lenv, renv, side, max(8, -(-width // 4)))
+    return exact_leak(tsf, cx, bond, lenv, renv, side)
+
+
 def _enrich_bond(tsf, cx: _Ctx, bond: int, lch, rch, side: str, Fsite, site_core, amp: float,
                  ledger):
     """CBE + local Krylov on one leaking frontier bond (``_frontier_expand`` + ``_krylov_enrich``
@@ -189,7 +217,7 @@ def _enrich_bond(tsf, cx: _Ctx, bond: int, lch, rch, side: str, Fsite, site_core
     bl = AdaptiveLedger()
     cap = cx.wcap[bond]
     xp = dd_cbe_expand(tsf, cx.mpo, bond, lch, rch, side, cx.alloc, bl, cap, cx.ex, cx.stream,
-                       None, cx.prec)
+                       None, cx.prec, cx.exact)
     prec = cx.prec
     if side == "left":
         seed = T.site_times_core(site_core, T.overlap_right(Fsite, xp.V_ex))
@@ -231,7 +259,7 @@ def frontier_left(Fm: SymMPS, fr: KLFrames, cx: _Ctx, amp: float, ledger):
         tsf = frame_onto_left(old, site_core, Fm[b + 1])
         rch = renv[b + 2]
         width = old.layout.col.total
-        leak[b] = sampled_leak(tsf, cx, b, lch, rch, "left", max(8, -(-width // 4)))
+        leak[b] = _leak(tsf, cx, b, lch, rch, "left", width)
         if cx.alloc.scale * leak[b] <= cx.alloc.per_bond:
             Wi = old
         else:
@@ -267,7 +295,7 @@ def frontier_right(Fm: SymMPS, fr: KLFrames, cx: _Ctx, amp: float, ledger):
         tsf = frame_onto_right(Fm[b], site_core, old)
         lch = lenvs[b]
         width = old.layout.row.total
-        leak[b] = sampled_leak(tsf, cx, b, lch, rch, "right", max(8, -(-width // 4)))
+        leak[b] = _leak(tsf, cx, b, lch, rch, "right", width)
         if cx.alloc.scale * leak[b] <= cx.alloc.per_bond:
             Zj = old
         else:

@@ -146,6 +146,7 @@ class StepOptions:
     tol_krylov: Optional[float] = None
     krylov_rank_tol: float = 0.0        # > 0: ranked Krylov selection (bug_step.jl:58)
     exact: bool = False                 # deterministic DD reference (bug_step.jl:117-119)
+    fold_omega: Optional[bool] = None   # None: order > 1 and not exact (bug_step_stages.jl:67)
     order: int = 1
     maxdim: int = 128
     hermitian: bool = True
@@ -718,16 +719,15 @@ def _dd_cbe_adaptive(fr: BondFrame, frame: Fam, opp: Fam, side: str, oside: str,
     return _finish_expansion(fr, frame, side, directions, cap, wbind, ledger)
 
 
-def exact_probe(frame: Fam, side: str) -> Optional[Fam]:
-    """_dd_probe with exact = true (dd_cbe.jl:141-148,155-160): the whole fused local space of
-    ``frame`` (its fusion basis: per fused charge the slices stacked in ascending order, one
-    identity block each) with the frame projected out.  No random numbers."""
+def fusion_probe(frame: Fam, side: str) -> Optional[Fam]:
+    """fusion_basis (dd_cbe.jl:143) of ``frame``'s fused local space: per fused charge the
+    slices stacked in ascending order, one identity block each, in the probe layout of
+    ``_graded_draw`` (a unitary change of basis of the fused legs)."""
     dfull = fused_dims(frame, side)
-    dT = frame_widths(frame, side)
-    if sum(max(d - dT.get(q, 0), 0) for q, d in dfull.items()) == 0:
-        return None
     link = frame.row if side == "left" else frame.col
     wl = {q: d for q, d in sorted(dfull.items()) if d > 0}
+    if not wl:
+        return None
     Om = Fam(frame.row, wl, frame.shifts) if side == "left" else Fam(wl, frame.col, frame.shifts)
     for q, df in wl.items():
         off = 0
@@ -743,8 +743,24 @@ def exact_probe(frame: Fam, side: str) -> Optional[Fam]:
                 Om.blocks[(s, lq)] = blk
             else:
                 Om.blocks[(s, q)] = blk.T.copy()
+    return Om
+
+
+def exact_probe(frame: Fam, side: str) -> Optional[Fam]:
+    """_dd_probe with exact = true (dd_cbe.jl:141-148,155-160): the fusion basis with the frame
+    projected out, i.e. the whole discarded space.  No random numbers."""
+    dfull = fused_dims(frame, side)
+    dT = frame_widths(frame, side)
+    if sum(max(d - dT.get(q, 0), 0) for q, d in dfull.items()) == 0:
+        return None
+    Om = fusion_probe(frame, side)
     proj = la.perp_left(frame, Om) if side == "left" else la.perp_right(frame, Om)
     return proj if proj.norm() > 0 else None
+
+
+def resolved_fold(opts) -> bool:
+    """fold_omega's default, order > 1 and not exact (bug_step_stages.jl:67)."""
+    return (opts.order > 1 and not opts.exact) if opts.fold_omega is None else bool(opts.fold_omega)
 
 
 def _dd_cbe_exact(fr: BondFrame, frame: Fam, opp: Fam, side: str, oside: str, lenv: Fam,
@@ -996,9 +1012,6 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
         raise ValueError("cbe_bug_step needs at least two sites")
     if opts.order < 1:
         raise ValueError("order must be >= 1")
-    if opts.exact and opts.order > 1:
-        raise NotImplementedError("exact = true with order > 1 measures the unfolded H*Theta "
-                                  "leak (bug_step_stages.jl:67), which is not restated")
     rl = (max(1, min(L - 1, L // 2)) - 1) if opts.root is None else opts.root
     d = len(psi.sites[0].shifts)
     tau = complex(tau)

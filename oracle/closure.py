@@ -170,6 +170,32 @@ def _merge(into, other):
     into.bond_log.extend(other.bond_log)
 
 
+def exact_leak(fr, mpo, bond: int, lenv, renv, side: str) -> float:
+    """Unfolded leak (``_bond_action`` with fold = false, ``_left_leak``/``_right_leak``,
+    krylov_frontier_bonds.jl:57-106): the outside-space norm of the exact two-site action
+    X = H Theta_F, formed by streaming the opposite frame's full fusion basis (unitary)."""
+    from .bug import fusion_probe, stream_left, stream_right
+    W1, W2 = mpo[bond], mpo[bond + 1]
+    if side == "left":
+        Om = fusion_probe(fr.V0, "right")
+        if Om is None:
+            return 0.0
+        Y = stream_left(lenv, W1, W2, renv, la.site_times_core(fr.U0, fr.S0), fr.V0, Om)
+        return la.perp_left(fr.U0, Y).norm()
+    Om = fusion_probe(fr.U0, "left")
+    if Om is None:
+        return 0.0
+    Y = stream_right(lenv, W1, W2, renv, fr.U0, la.core_times_site(fr.S0, fr.V0), Om)
+    return la.perp_right(fr.V0, Y).norm()
+
+
+def _leak(tsf, mpo, bond, lenv, renv, side, width, rng, opts) -> float:
+    from .bug import resolved_fold
+    if resolved_fold(opts):
+        return sampled_leak(tsf, mpo, bond, lenv, renv, side, max(8, -(-width // 4)), rng)
+    return exact_leak(tsf, mpo, bond, lenv, renv, side)
+
+
 def frontier_left(Fm, mpo, fr: KLFrames, tau, alloc, opts, enrich: bool, wcap, amp: float, rng,
                   recurrence: str):
     """``_frontier_left`` (frontier_sweeps.jl:46-199), enriching, folded leaks."""
@@ -194,7 +220,7 @@ def frontier_left(Fm, mpo, fr: KLFrames, tau, alloc, opts, enrich: bool, wcap, a
         tsf = frame_onto_left(old, site_core, Fm.sites[b + 1])
         rch = renv[b + 2]
         width = sum(old.col.values())
-        leak[b] = sampled_leak(tsf, mpo, b, lch, rch, "left", max(8, -(-width // 4)), rng)
+        leak[b] = _leak(tsf, mpo, b, lch, rch, "left", width, rng, opts)
         quiet = alloc.scale * leak[b] <= alloc.per_bond
         if enrich and not quiet:
             bl = Ledger()
@@ -245,7 +271,7 @@ def frontier_right(Fm, mpo, fr: KLFrames, tau, alloc, opts, enrich: bool, wcap, 
         tsf = frame_onto_right(Fm.sites[b], site_core, old)
         lch = lenvs[b]
         width = sum(old.row.values())
-        leak[b] = sampled_leak(tsf, mpo, b, lch, rch, "right", max(8, -(-width // 4)), rng)
+        leak[b] = _leak(tsf, mpo, b, lch, rch, "right", width, rng, opts)
         quiet = alloc.scale * leak[b] <= alloc.per_bond
         if enrich and not quiet:
             bl = Ledger()

@@ -341,7 +341,7 @@ def test_exact_step_is_deterministic_and_tracks_free_fermions():
     give the identical state (test_parallel_half_sweeps.jl:6-9), any seed gives the same step,
     no probe is counted, and on the XX domain wall it gives the Gaussian-probe step's state (both
     span the discarded space the sketch needs) and its free-fermion error, 1.8e-5 after 6
-    order-1 steps of dt = 0.05; order > 1 is refused."""
+    order-1 steps of dt = 0.05."""
     L, dt = 10, 0.05
     runs = []
     for par, seed in ((False, 1), (True, 2)):
@@ -360,6 +360,40 @@ def test_exact_step_is_deterministic_and_tracks_free_fermions():
         obug.cbe_bug_step(gauss, om.xxz_mpo(L, delta=0.0), -1j * dt,
                           obug.StepOptions(tol_basis=1e-6))
     assert np.abs(obug.sz_profile(psi) - obug.sz_profile(gauss)).max() < 1e-12
-    with pytest.raises(NotImplementedError):
-        obug.cbe_bug_step(obug.MPS(om.neel_state(6)), om.xxz_mpo(6), -0.1j,
-                          obug.StepOptions(exact=True, order=2))
+
+
+@pytest.mark.parametrize("kw", [dict(exact=True), dict(fold_omega=False)])
+def test_unfolded_closure_steps_track_free_fermions(kw):
+    """Order 2 with the unfolded frontier leak (fold_omega = false, the default under exact,
+    bug_step_stages.jl:67; _bond_action, krylov_frontier_bonds.jl:80-106): the XX domain wall
+    within 1e-7 of free fermions after 6 steps of dt = 0.05 (8.5e-9 measured, as folded); under
+    exact the serial and parallel runs with different seeds are identical."""
+    L, dt = 10, 0.05
+    states = []
+    for par, seed in ((False, 1), (True, 2)):
+        psi = obug.MPS(om.domain_wall_state(L, "U1"))
+        mpo = om.xxz_mpo(L, delta=0.0)
+        for _ in range(6):
+            info = obug.cbe_bug_step(psi, mpo, -1j * dt, obug.StepOptions(
+                order=2, tol_basis=1e-6, parallel=par, seed=seed, **kw))
+            assert np.isfinite(info.ledger.leak[1])
+        ref = free_fermion_sz(L, 6 * dt, list(range(L // 2)))
+        assert np.abs(obug.sz_profile(psi) - ref).max() < 1e-7
+        states.append(om.mps_dense(psi.sites))
+    if kw.get("exact"):
+        assert np.array_equal(states[0], states[1])
+
+
+def test_fusion_probe_is_unitary():
+    """fusion_basis (dd_cbe.jl:143) in the probe layouts: a unitary change of basis of the fused
+    legs on both sides, so a sketch with it is the two-site action itself up to that rotation
+    (which is what makes the unfolded leak exact)."""
+    row, shifts = {-1: 2, 1: 3}, [1, -1]
+    F = obug.Fam.zeros(row, {0: 1, 2: 1, -2: 1}, shifts)
+    O = _left_dense(obug.fusion_probe(F, "left"))
+    assert O.shape[0] == O.shape[1] == 10
+    assert np.abs(O @ O.conj().T - np.eye(10)).max() == 0.0
+    V = obug.Fam.zeros({0: 1, -2: 1}, row, [-1, 1])
+    Or = np.concatenate(list(obug.fusion_probe(V, "right").dense()), axis=1)
+    assert Or.shape[0] == Or.shape[1]
+    assert np.abs(Or.conj().T @ Or - np.eye(Or.shape[1])).max() == 0.0
