@@ -28,7 +28,7 @@ import sys
 import threading
 import time
 from dataclasses import dataclass, field
-from typing import Callable, Dict, List, Optional
+from typing import Callable, Dict, List, Optional, Sequence
 
 import numpy as np
 import torch
@@ -940,6 +940,40 @@ def extend_frame(frame, residual, direction, side, weight, allowance, room):
     return _oplus(frame, new, side), _width(new, side), fits
 
 
+UNION_KEEP = 0.5
+
+
+def union_fresh(frame, cand, side, tol=0.0, room=1 << 60):
+    """``_union_fresh`` (half_sweeps.jl:117-176): orthonormal directions of ``cand`` outside
+    ``frame`` that keep at least half of themselves under a second projection (twice is enough),
+    or None.  ``tol = 0`` admits everything above the numerical floor; ``tol > 0`` only
+    directions carrying about ``tol ||cand||``; at most ``room``, the strongest first."""
+    if room <= 0:
+        return None
+    perp = _perp(side)
+    factor = T.factor_left if side == "left" else T.factor_right
+    pick = (lambda r: r.U) if side == "left" else (lambda r: r.Vh)          # noqa: E731
+    residual = perp(frame, cand)
+    rn, cn = T.norm(residual), T.norm(cand)
+    if rn <= CBE_EPS * max(cn, 1.0) or rn <= tol * cn:
+        return None
+    first_cut = max(CBE_EPS * max(cn, rn) / rn, tol * cn / rn)
+    basis = pick(factor(T.scale(1.0 / rn, residual)).select(first_cut))
+    second = perp(frame, basis)
+    fac = factor(second)
+    smax = max((e[0] for e in fac.select(1e-13).kept), default=0.0)
+    if smax < UNION_KEEP:
+        return None
+    return pick(fac.select(UNION_KEEP / smax, None if room >= 1 << 60 else room))
+
+
+def union_frame(frame, cand, side, tol=0.0, room=1 << 60):
+    """``_union_left``/``_union_right`` (half_sweeps.jl:184-207): the frame verbatim, then the
+    fresh directions of ``cand`` appended."""
+    fresh = union_fresh(frame, cand, side, tol, room)
+    return frame if fresh is None else _oplus(frame, fresh, side)
+
+
 def append_ranked(frame, candidates, side, allowance, workcap):
     """``_append_ranked`` (krylov_enrichment.jl:185-196) -> (frame, added, fits): one SVD of the
     direct sum of the weighted residuals, all outside the same frame, keeps the fewest top
@@ -1070,7 +1104,8 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
                  tol_trunc: Optional[float] = None, tol_expv: Optional[float] = None,
                  tol_root: Optional[float] = None, tol_krylov: Optional[float] = None,
                  krylov_rank_tol: float = 0.0, exact: bool = False,
-                 fold_omega: Optional[bool] = None,
+                 fold_omega: Optional[bool] = None, keep_states: Sequence[SymMPS] = (),
+                 keep_tols: Sequence[float] = (),
                  order: int = 1, maxdim: Optional[int] = None, hermitian: bool = True,
                  work_ceiling_rule: str = "bond", bond_ceiling_factor: Optional[float] = None,
                  root_ceiling_factor: float = 1.2,
@@ -1094,6 +1129,12 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     agree.  ``fold_omega`` (default: ``order > 1 and not exact``, bug_step_stages.jl:67) picks the
     closure's frontier leak: folded (a Gaussian sketch of the two-site action, never formed) or
     unfolded (the exact two-site action's outside-space norm).
+
+    ``keep_states`` (with ``keep_tols``, default 0 each) are states the trial space must contain
+    (half_sweeps.jl:117-207,280-296,384-395): after each bond's Krylov stage their carried site
+    tensors are unioned into the frame, exactly at tol 0 (the MPDO identity, a mass state), at
+    tol > 0 only directions carrying about tol of them, within the bond's work ceiling under the
+    per-bond rules.
 
     ``work_ceiling_rule`` caps every bond's working rank from the bonds before the step
     (bug_step_stages.jl:116-129): ``"bond"`` ceil(f d D_b) (f defaults to 1.2), ``"maxdim"`` one
@@ -1132,6 +1173,19 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     if not krylov_rank_tol >= 0:
         raise ValueError(f"krylov_rank_tol must be >= 0, got {krylov_rank_tol}")
     fold = (order > 1 and not exact) if fold_omega is None else bool(fold_omega)
+    keep_states = list(keep_states)
+    if keep_tols and len(keep_tols) != len(keep_states):
+        raise ValueError(f"keep_tols has {len(keep_tols)} entries for {len(keep_states)} "
+                         "keep_states")
+    keep_tols = [float(t) for t in keep_tols] if keep_tols else [0.0] * len(keep_states)
+    if any(not t >= 0 for t in keep_tols):
+        raise ValueError("keep_tols must be >= 0")
+    if any(len(k) != L for k in keep_states):
+        raise ValueError("every kept state needs the chain's length")
+    cap_keep = work_ceiling_rule != "maxdim"
+
+    def keep_room(b, width, tol):
+        return wcap[b] - width if cap_keep and tol > 0 else 1 << 60
     rules = ("maxdim", "bond", "bond_strict", "frame_root")
     if work_ceiling_rule not in rules:
         raise ValueError(f"work_ceiling_rule must be one of {rules}, got {work_ceiling_rule!r}")
@@ -1194,6 +1248,7 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         lch = ev.boundary_left(mpo, psiL.bond_leg(0))
         lroot = lch
         carry = None
+        kcar = [None] * len(keep_states)
         for b in range(rl + 1):
             if b % 16 == 0:
                 _progress(f"left half-sweep bond {b}/{rl}")
@@ -1210,6 +1265,12 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
                                     xp.U_ex, "left", tau, alloc, wcap[b], recurrence,
                                     hermitian_tol=htol)
             _record_krylov(bl, kst)
+            # every kept state unioned before any of their carries is taken
+            kxs = [ks[b] if kc is None else T.core_times_site(kc, ks[b])
+                   for ks, kc in zip(keep_states, kcar)]
+            for kx, tol in zip(kxs, keep_tols):
+                Wk = union_frame(Wk, kx, "left", tol, keep_room(b, Wk.layout.col.total, tol))
+            kcar = [T.overlap_left(Wk, kx) for kx in kxs]
             left_frames[b] = Wk
             expanded[b] = Wk.layout.col.total
             log_bond(ledger, bl, b, "left", expanded[b])
@@ -1231,6 +1292,7 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         rch = ev.boundary_right(mpo, psiR.bond_leg(L))
         rroot = rch
         carry = None
+        kcar = [None] * len(keep_states)
         for b in range(L - 2, rl - 1, -1):
             if b % 16 == 0:
                 _progress(f"right half-sweep bond {b}/{rl}")
@@ -1247,6 +1309,11 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
                                     xp.V_ex, "right", tau, alloc, wcap[b], recurrence,
                                     hermitian_tol=htol)
             _record_krylov(bl, kst)
+            bxs = [ks[b + 1] if kc is None else T.site_times_core(ks[b + 1], kc)
+                   for ks, kc in zip(keep_states, kcar)]
+            for bx, tol in zip(bxs, keep_tols):
+                Zk = union_frame(Zk, bx, "right", tol, keep_room(b, Zk.layout.row.total, tol))
+            kcar = [T.overlap_right(bx, Zk) for bx in bxs]
             log_bond(ledger, bl, b, "right", Zk.layout.row.total)
             right_frames[b] = Zk
             if b != rl:

@@ -147,6 +147,8 @@ class StepOptions:
     krylov_rank_tol: float = 0.0        # > 0: ranked Krylov selection (bug_step.jl:58)
     exact: bool = False                 # deterministic DD reference (bug_step.jl:117-119)
     fold_omega: Optional[bool] = None   # None: order > 1 and not exact (bug_step_stages.jl:67)
+    keep_states: list = field(default_factory=list)   # MPS the trial space must contain
+    keep_tols: list = field(default_factory=list)     # per kept state, default 0 (exact)
     order: int = 1
     maxdim: int = 128
     hermitian: bool = True
@@ -828,6 +830,36 @@ def extend_frame(frame: Fam, residual: Fam, direction: Fam, side: str, weight: f
     return oplus(frame, newb, side), _width_of(newb, side), fits
 
 
+UNION_KEEP = 0.5
+
+
+def union_fresh(frame: Fam, cand: Fam, side: str, tol: float = 0.0, room: int = 1 << 60):
+    """_union_fresh (half_sweeps.jl:117-176): orthonormal directions of ``cand`` outside
+    ``frame`` that keep at least UNION_KEEP of themselves under a second projection, or None."""
+    if room <= 0:
+        return None
+    perp = la.perp_left if side == "left" else la.perp_right
+    svd = la.svd_left if side == "left" else la.svd_right
+    pick = (lambda r: r.U) if side == "left" else (lambda r: r.Vh)          # noqa: E731
+    residual = perp(frame, cand)
+    rn, cn = residual.norm(), cand.norm()
+    if rn <= CBE_EPS * max(cn, 1.0) or rn <= tol * cn:
+        return None
+    first_cut = max(CBE_EPS * max(cn, rn) / rn, tol * cn / rn)
+    basis = pick(svd(residual.scale(1.0 / rn), cutoff=first_cut))
+    second = perp(frame, basis)
+    smax = max((e[0] for e in svd(second, cutoff=1e-13).kept), default=0.0)
+    if smax < UNION_KEEP:
+        return None
+    return pick(svd(second, cutoff=UNION_KEEP / smax, nkeep=None if room >= 1 << 60 else room))
+
+
+def union_frame(frame: Fam, cand: Fam, side: str, tol: float = 0.0, room: int = 1 << 60) -> Fam:
+    """_union_left/_union_right (half_sweeps.jl:184-207)."""
+    fresh = union_fresh(frame, cand, side, tol, room)
+    return frame if fresh is None else oplus(frame, fresh, side)
+
+
 def append_ranked(frame: Fam, candidates: List[Fam], side: str, allowance: float, workcap: int):
     """_append_ranked (krylov_enrichment.jl:185-196) -> (frame, n_added, fits): one SVD of the
     direct sum of the weighted residuals keeps the fewest top directions whose tail fits."""
@@ -1013,6 +1045,13 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
     if opts.order < 1:
         raise ValueError("order must be >= 1")
     rl = (max(1, min(L - 1, L // 2)) - 1) if opts.root is None else opts.root
+    keeps = list(opts.keep_states)
+    if opts.keep_tols and len(opts.keep_tols) != len(keeps):
+        raise ValueError("keep_tols and keep_states differ in length")
+    ktols = [float(t) for t in opts.keep_tols] if opts.keep_tols else [0.0] * len(keeps)
+    if any(not t >= 0 for t in ktols):
+        raise ValueError("keep_tols must be >= 0")
+    cap_keep = opts.work_ceiling_rule != "maxdim"
     d = len(psi.sites[0].shifts)
     tau = complex(tau)
     rng = la.ProbeRNG(opts.seed)
@@ -1049,6 +1088,7 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
     lch = left_boundary(mpo, psiL)
     lroot = lch
     carry = None
+    kcar = [None] * len(keeps)
     for b in range(rl + 1):
         carried = psiL.sites[b] if carry is None else la.core_times_site(carry, psiL.sites[b])
         fr = frame_from(carried, psiL.sites[b + 1])
@@ -1062,6 +1102,13 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
         Wk, kst = krylov_enrich(Kex, lambda v: ob.apply_one_site(Hl, Hw, Hr, v), ex.U_ex, "left",
                                 tau, alloc, _cap_at(wcap, b), recurrence)
         record_krylov(bl, kst, alloc)
+        # half_sweeps.jl:280-296: every kept state unioned before any carry is taken
+        kxs = [ks.sites[b] if kc is None else la.core_times_site(kc, ks.sites[b])
+               for ks, kc in zip(keeps, kcar)]
+        for kx, tol in zip(kxs, ktols):
+            room = _cap_at(wcap, b) - sum(Wk.col.values()) if cap_keep and tol > 0 else 1 << 60
+            Wk = union_frame(Wk, kx, "left", tol, room)
+        kcar = [la.overlap_left(Wk, kx) for kx in kxs]
         left_frames[b] = Wk
         expanded[b] = sum(Wk.col.values())
         log_bond(ledL, bl, b, "left", expanded[b])
@@ -1080,6 +1127,7 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
     rch = right_boundary(mpo, psiR)
     rroot = rch
     carry = None
+    kcar = [None] * len(keeps)
     for b in range(L - 2, rl - 1, -1):
         carried = psiR.sites[b + 1] if carry is None else la.site_times_core(psiR.sites[b + 1], carry)
         fr = frame_from(psiR.sites[b], carried)
@@ -1093,6 +1141,12 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
         Zk, kst = krylov_enrich(Bex, lambda v: ob.apply_one_site(Hl, Hw, Hr, v), ex.V_ex, "right",
                                 tau, alloc, _cap_at(wcap, b), recurrence)
         record_krylov(bl, kst, alloc)
+        bxs = [ks.sites[b + 1] if kc is None else la.site_times_core(ks.sites[b + 1], kc)
+               for ks, kc in zip(keeps, kcar)]                       # half_sweeps.jl:384-395
+        for bx, tol in zip(bxs, ktols):
+            room = _cap_at(wcap, b) - sum(Zk.row.values()) if cap_keep and tol > 0 else 1 << 60
+            Zk = union_frame(Zk, bx, "right", tol, room)
+        kcar = [la.overlap_right(bx, Zk) for bx in bxs]
         log_bond(ledR, bl, b, "right", sum(Zk.row.values()))
         right_frames[b] = Zk
         if b != rl:

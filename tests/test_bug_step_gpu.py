@@ -223,6 +223,41 @@ def test_unfolded_leak_step_matches_oracle(gpu, start, L, delta):
         assert np.linalg.norm(vd - vo) / np.linalg.norm(vo) <= 1e-10, n
 
 
+def test_kept_states_match_oracle(gpu):
+    """keep_states at tol 0 and 1e-6 (half_sweeps.jl:117-207,280-296,384-395) on the reference's
+    own fixture (test_step_keep_states.jl:11-25: Heisenberg L = 8, exact, order 2): the device
+    step from the same start and kept states gives the oracle's frame widths and bonds, and its
+    state to the exact order-2 bar of 1e-8."""
+    from bondupdategalerkin_amd import bug_step as B
+    from bondupdategalerkin_amd import mps
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+    from tests.helpers import device_from_fam
+
+    mpo_o = om.xxz_mpo(8, delta=1.0)
+    kw = dict(order=2, tol_basis=1e-8, tol_trunc=1e-12, maxdim=64, exact=True)
+    start = obug.MPS(om.neel_state(8, "U1"))
+    obug.cbe_bug_step(start, mpo_o, -0.2j, obug.StepOptions(**kw))
+    kept = start.copy()
+    for _ in range(3):
+        obug.cbe_bug_step(kept, mpo_o, -0.3j, obug.StepOptions(**kw))
+
+    def on_device(m):
+        d = mps.neel_state(8)
+        return mps.SymMPS([device_from_fam(f) for f in m.sites], d.charges, m.center)
+
+    mpo_d = xxz_mpo(8, delta=1.0)
+    for tols in ([], [1e-6]):
+        psi_o, psi_d = start.copy(), on_device(start)
+        info_o = obug.cbe_bug_step(psi_o, mpo_o, -0.2j, obug.StepOptions(
+            **kw, keep_states=[kept], keep_tols=tols))
+        info_d = B.cbe_bug_step(psi_d, mpo_d, -0.2j, **kw, keep_states=[on_device(kept)],
+                                keep_tols=tols)
+        assert list(info_d.expanded) == list(info_o.expanded), tols
+        assert psi_d.bond_dims() == psi_o.bond_dims(), tols
+        vd, vo = _dense(psi_d), om.mps_dense(psi_o.sites)
+        assert np.linalg.norm(vd - vo) / np.linalg.norm(vo) <= 1e-8, tols
+
+
 def test_invalid_options_refused_before_touching_state(gpu):
     from bondupdategalerkin_amd import bug_step as B
     from bondupdategalerkin_amd import mps
@@ -240,6 +275,8 @@ def test_invalid_options_refused_before_touching_state(gpu):
         B.cbe_bug_step(psi, xxz_mpo(6), -0.1j, order=0)
     with pytest.raises(ValueError):
         B.cbe_bug_step(psi, xxz_mpo(6), -0.1j, krylov_rank_tol=-1e-6)
+    with pytest.raises(ValueError):
+        B.cbe_bug_step(psi, xxz_mpo(6), -0.1j, keep_states=[psi], keep_tols=[0.0, 0.0])
     with pytest.raises(ValueError):
         B.cbe_bug_step(psi, xxz_mpo(6), -0.1j, work_ceiling_rule="bond_strict")
     with pytest.raises(ValueError):

@@ -397,3 +397,59 @@ def test_fusion_probe_is_unitary():
     Or = np.concatenate(list(obug.fusion_probe(V, "right").dense()), axis=1)
     assert Or.shape[0] == Or.shape[1]
     assert np.abs(Or.conj().T @ Or - np.eye(Or.shape[1])).max() == 0.0
+
+
+def _keep_fixture():
+    """test_step_keep_states.jl:11-25: Heisenberg L = 8, exact selection, order 2; the start
+    state one step of -0.2i from Neel, the kept state three more steps of -0.3i."""
+    mpo = om.xxz_mpo(8, delta=1.0)
+    kw = dict(order=2, tol_basis=1e-8, tol_trunc=1e-12, maxdim=64, exact=True)
+    start = obug.MPS(om.neel_state(8, "U1"))
+    obug.cbe_bug_step(start, mpo, -0.2j, obug.StepOptions(**kw))
+    kept = start.copy()
+    for _ in range(3):
+        obug.cbe_bug_step(kept, mpo, -0.3j, obug.StepOptions(**kw))
+    return mpo, kw, start, kept
+
+
+def test_kept_states_widen_the_frames_within_their_rules():
+    """keep_states / keep_tols (half_sweeps.jl:117-207,280-296,384-395), the reference's own
+    checks (test_step_keep_states.jl:27-57): a kept state at tol 0 widens some frame, admitted at
+    a tolerance it widens no frame more than exactly, under :bond with a small maxdim no frame
+    exceeds its ceiling; and at tol 0 the kept state lies in the step's trial space (its
+    projection miss onto the frames is at rounding level)."""
+    mpo, kw, start, kept = _keep_fixture()
+    kw = dict(kw, truncate=False)
+
+    def widths(**extra):
+        info = obug.cbe_bug_step(start.copy(), mpo, -0.2j, obug.StepOptions(**dict(kw, **extra)))
+        return np.array(info.expanded)
+
+    plain = widths()
+    exact = widths(keep_states=[kept])
+    admitted = widths(keep_states=[kept], keep_tols=[1e-6])
+    assert (exact > plain).any()
+    assert (admitted <= exact).all()
+    bonds = start.bond_dims()
+    small = max(max(bonds), 2)
+    ceil = obug.bond_work_ceilings(bonds, 2, small, obug.DEFAULT_BOND_CEILING_FACTOR)
+    capped = widths(keep_states=[kept], keep_tols=[1e-12], maxdim=small, work_ceiling_rule="bond")
+    assert (capped <= np.array(ceil)).all()
+    with pytest.raises(ValueError):
+        widths(keep_states=[kept], keep_tols=[1e-6, 1e-6])
+
+
+def test_union_fresh_contains_the_candidate():
+    """_union_fresh at tol 0: the grown frame is an isometry whose span holds the candidate to
+    rounding, and a candidate already inside the frame adds nothing."""
+    from oracle import linalg as la
+
+    rng = np.random.default_rng(11)
+    row, shifts = {-1: 2, 1: 3}, [1, -1]
+    F = la.svd_left(_random_like(obug.Fam.zeros(row, {0: 2, 2: 1}, shifts), rng)).U
+    C = _random_like(obug.Fam.zeros(row, {0: 1, -2: 1}, shifts), rng)
+    G = obug.union_frame(F, C, "left")
+    Gd, Cd = _left_dense(G), _left_dense(C)
+    assert np.abs(Gd.conj().T @ Gd - np.eye(Gd.shape[1])).max() < 1e-13
+    assert np.linalg.norm(Cd - Gd @ (Gd.conj().T @ Cd)) < 1e-13 * np.linalg.norm(Cd)
+    assert obug.union_frame(G, C, "left") is G
