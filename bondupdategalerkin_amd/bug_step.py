@@ -1105,7 +1105,8 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
                  tol_root: Optional[float] = None, tol_krylov: Optional[float] = None,
                  krylov_rank_tol: float = 0.0, exact: bool = False,
                  fold_omega: Optional[bool] = None, keep_states: Sequence[SymMPS] = (),
-                 keep_tols: Sequence[float] = (),
+                 keep_tols: Sequence[float] = (), frames_only: bool = False,
+                 frames_report: Optional[dict] = None,
                  order: int = 1, maxdim: Optional[int] = None, hermitian: bool = True,
                  work_ceiling_rule: str = "bond", bond_ceiling_factor: Optional[float] = None,
                  root_ceiling_factor: float = 1.2,
@@ -1135,6 +1136,10 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     tensors are unioned into the frame, exactly at tol 0 (the MPDO identity, a mass state), at
     tol > 0 only directions carrying about tol of them, within the bond's work ceiling under the
     per-bond rules.
+
+    ``frames_only = True`` (with ``truncate = False``) skips the root solve: the state comes back
+    as itself in the step's frames, for a caller that solves its own root; ``frames_report``, a
+    dict, receives the frames (``left``, ``right``, ``root_link``).
 
     ``work_ceiling_rule`` caps every bond's working rank from the bonds before the step
     (bug_step_stages.jl:116-129): ``"bond"`` ceil(f d D_b) (f defaults to 1.2), ``"maxdim"`` one
@@ -1183,6 +1188,8 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     if any(len(k) != L for k in keep_states):
         raise ValueError("every kept state needs the chain's length")
     cap_keep = work_ceiling_rule != "maxdim"
+    if frames_only and truncate:
+        raise ValueError("frames_only needs truncate = False")
 
     def keep_room(b, width, tol):
         return wcap[b] - width if cap_keep and tol > 0 else 1 << 60
@@ -1383,7 +1390,9 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         if n % 50 == 0:
             _progress(f"root solve substep {n}")
 
-    if hermitian:
+    if frames_only:                      # the caller solves its own root (bug_step.jl:381-382)
+        result = root_start
+    elif hermitian:
         result, rep = K.lanczos_expv(apply_root, tau, root_start, tol=1e-15,
                                      maxiter=K.ROOT_MAXITER, reorth=reorth, conv_tol=te, substeps=0,
                                      on_substep=substep)
@@ -1391,9 +1400,12 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         result, rep = K.arnoldi_expv(apply_root, tau, root_start, tol=1e-15,
                                      maxiter=K.ROOT_MAXITER, conv_tol=te, substeps=0,
                                      on_substep=substep)
-    ledger.expv_err, ledger.expv_iters = rep.err, rep.iterations
-    if rep.err > te * (1 + 1e-8):
-        ledger.fail("expv")
+    if not frames_only:
+        ledger.expv_err, ledger.expv_iters = rep.err, rep.iterations
+        if rep.err > te * (1 + 1e-8):
+            ledger.fail("expv")
+    if frames_report is not None:        # bug_step.jl:455-467
+        frames_report.update(left=list(left_frames), right=list(right_frames), root_link=rl)
     secs["root"] = time.perf_counter() - ts
 
     _progress("assembly and compression")

@@ -149,6 +149,7 @@ class StepOptions:
     fold_omega: Optional[bool] = None   # None: order > 1 and not exact (bug_step_stages.jl:67)
     keep_states: list = field(default_factory=list)   # MPS the trial space must contain
     keep_tols: list = field(default_factory=list)     # per kept state, default 0 (exact)
+    frames_only: bool = False           # skip the root solve (bug_step.jl:114-116,381-382)
     order: int = 1
     maxdim: int = 128
     hermitian: bool = True
@@ -1052,6 +1053,8 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
     if any(not t >= 0 for t in ktols):
         raise ValueError("keep_tols must be >= 0")
     cap_keep = opts.work_ceiling_rule != "maxdim"
+    if opts.frames_only and opts.truncate:
+        raise ValueError("frames_only needs truncate = False")
     d = len(psi.sites[0].shifts)
     tau = complex(tau)
     rng = la.ProbeRNG(opts.seed)
@@ -1197,7 +1200,9 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
 
     vec_ops = dict(inner=ob.inner, norm=lambda x: x.norm(), axpy=lambda a, x, y: y.axpy(a, x),
                    scale=lambda a, x: x.scale(a))
-    if opts.hermitian:
+    if opts.frames_only:
+        result = root_start
+    elif opts.hermitian:
         result, rep = ok.lanczos_expv(apply_root, tau, root_start, tol=1e-15,
                                       maxiter=ok.ROOT_MAXITER, reorth=opts.reorth,
                                       conv_tol=opts.tol_expv, substeps=0, **vec_ops)
@@ -1205,10 +1210,11 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
         result, rep = ok.arnoldi_expv(apply_root, tau, root_start, tol=1e-15,
                                       maxiter=ok.ROOT_MAXITER, conv_tol=opts.tol_expv,
                                       substeps=0, **vec_ops)
-    ledger.expv_err = rep.err
-    ledger.expv_iters = rep.iterations
-    if rep.err > opts.tol_expv * (1 + 1e-8):
-        ledger.unmet_("expv")
+    if not opts.frames_only:
+        ledger.expv_err = rep.err
+        ledger.expv_iters = rep.iterations
+        if rep.err > opts.tol_expv * (1 + 1e-8):
+            ledger.unmet_("expv")
 
     # ---- assembly and compression (diagnostics.jl:269-337) ----
     res = la.svd_core(result, cutoff=0.0)

@@ -258,6 +258,30 @@ def test_kept_states_match_oracle(gpu):
         assert np.linalg.norm(vd - vo) / np.linalg.norm(vo) <= 1e-8, tols
 
 
+def test_frames_only_matches_oracle(gpu):
+    """frames_only with frames_report (bug_step.jl:114-116,381-382,455-467): the device returns
+    the input state in its frames (overlap ratio 1 to 1e-12) with the oracle's frame widths, and
+    reports frames whose widths are those widths."""
+    from bondupdategalerkin_amd import bug_step as B
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    psi_d, psi_o = _device_state("neel", 10), _oracle_state("neel", 10)
+    mpo_d, mpo_o = xxz_mpo(10, delta=0.5), om.xxz_mpo(10, delta=0.5)
+    B.cbe_bug_step(psi_d, mpo_d, -0.1j)
+    obug.cbe_bug_step(psi_o, mpo_o, -0.1j)
+    before = _dense(psi_d)
+    rep = {}
+    info_d = B.cbe_bug_step(psi_d, mpo_d, -0.05j, truncate=False, frames_only=True,
+                            frames_report=rep)
+    info_o = obug.cbe_bug_step(psi_o, mpo_o, -0.05j,
+                               obug.StepOptions(truncate=False, frames_only=True))
+    after = _dense(psi_d)
+    ratio = abs(np.vdot(after, before)) / (np.linalg.norm(after) * np.linalg.norm(before))
+    assert 1 - ratio < 1e-12
+    assert list(info_d.expanded) == list(info_o.expanded)
+    assert [f.layout.col.total for f in rep["left"]] == list(info_d.expanded[: rep["root_link"] + 1])
+
+
 def test_invalid_options_refused_before_touching_state(gpu):
     from bondupdategalerkin_amd import bug_step as B
     from bondupdategalerkin_amd import mps

@@ -453,3 +453,20 @@ def test_union_fresh_contains_the_candidate():
     assert np.abs(Gd.conj().T @ Gd - np.eye(Gd.shape[1])).max() < 1e-13
     assert np.linalg.norm(Cd - Gd @ (Gd.conj().T @ Cd)) < 1e-13 * np.linalg.norm(Cd)
     assert obug.union_frame(G, C, "left") is G
+
+
+def test_frames_only_returns_the_state_in_the_frames():
+    """frames_only (bug_step.jl:114-116,381-382; test_step_keep_states.jl:59-76): no root solve,
+    so the state comes back as the input in the frames (overlap ratio 1 to 1e-12), with the
+    full step's frame widths; a truncation with it is refused."""
+    mpo, kw, start, _ = _keep_fixture()
+    kw = dict(kw, truncate=False)
+    full = obug.cbe_bug_step(start.copy(), mpo, -0.2j, obug.StepOptions(**kw))
+    psi = start.copy()
+    info = obug.cbe_bug_step(psi, mpo, -0.2j, obug.StepOptions(**kw, frames_only=True))
+    a, b = om.mps_dense(psi.sites), om.mps_dense(start.sites)
+    assert 1 - abs(np.vdot(a, b)) / (np.linalg.norm(a) * np.linalg.norm(b)) < 1e-12
+    assert info.expanded == full.expanded
+    with pytest.raises(ValueError):
+        obug.cbe_bug_step(start.copy(), mpo, -0.2j, obug.StepOptions(**dict(kw, truncate=True),
+                                                                     frames_only=True))
