@@ -14,6 +14,12 @@ Follows, stage by stage (0-based sites; bond b joins sites b and b+1; link j sit
 * projection miss           krylov_closure_frames.jl:242-280
 * root generator and expv   effective_operators.jl:91-107, krylov.jl:128-263, root_methods.jl
 * assembly + compression    diagnostics.jl:269-337, truncation.jl:49-158
+* work ceilings             basis_budget.jl:454-549 (:bond, :maxdim, :bond_strict, :frame_root)
+* ranked Krylov selection   krylov_enrichment.jl:174-211 (krylov_rank_tol)
+* exact DD reference        dd_cbe.jl:141-160, dd_cbe_expand.jl:91-97,346-360, sketch.jl:119-132
+* kept states               half_sweeps.jl:117-207,280-296,384-395 (keep_states, keep_tols)
+* per-bond log              basis_budget.jl:390-418, diagnostics.jl:328-335
+* frames only               bug_step.jl:114-116,381-382
 
 The probe stream is this package's counter-based generator, not Julia's MersenneTwister, and the
 probe can be injected; the order-p frontier closure (``order > 1``) is not restated here.
@@ -21,7 +27,10 @@ probe can be injected; the order-p frontier closure (``order > 1``) is not resta
 Pinned by tests/test_oracle_bug.py against the exact free-fermion magnetisation of the XX chain
 (test/common/free_fermion.jl:25-47), against 2^L exact diagonalisation of the XXZ chain, and
 against the reference's frozen runs (test/fixtures/b155550/xx_u1_bug_order1.json) at the
-tolerances of test_fixtures_b155550.jl:61-64.
+tolerances of test_fixtures_b155550.jl:61-64.  The option paths are pinned there too: the
+ceilings on hand-worked values, the ranked selection on a dense SVD of the candidates, the exact
+probe on the complement projector, the kept states on the reference's own width checks
+(test_step_keep_states.jl:27-57), the bond log on its aggregation into the step's ledger.
 """
 from __future__ import annotations
 
