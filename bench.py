@@ -1,31 +1,41 @@
 """Benchmark of the BUG hot path on one MI355X.
 
-Prints ONE JSON line (rank 0).  Workload: the XXZ chain (delta = 0.5) at L = 128 on a seeded
-synthetic U(1) state with bond dimension up to D = 1024 (Gaussian charge-sector weights, sigma 4),
-canonicalised and normalised on the device.
+Prints ONE JSON line.  Workload: the XXZ chain (delta = 0.5) at L = 128 on a seeded synthetic U(1)
+state with bond dimension up to D = 1024 (Gaussian charge-sector weights, sigma 4; right-bond
+columns weighted so that the Schmidt values fall by ``--tail`` across each bond), canonicalised
+and normalised on the device.
 
 ``--workload bug_step`` (default): one step = one ``cbe_bug_step`` (order 2, tau = -0.02 i,
-maxdim = D, work_ceiling_rule = maxdim, default tolerances) on the device-resident state.  The
-headline form is ``--precision mixed`` (FP32-MFMA block contractions, FP64 Krylov / QR / SVD);
-the FP64 drop-in is stepped from the same state beside it, timed on its own, and the largest
-|<S^z_j>| difference between the two runs after the timed steps is reported as the error of the
-mixed form.  ``--precision fp64`` times the drop-in alone.
+maxdim = D, work_ceiling_rule = maxdim, default tolerances, parallel half-sweeps) on the
+device-resident state.  The headline form is ``--precision mixed`` (FP32-MFMA block contractions,
+FP64 Krylov / QR / SVD): ``--warmup`` untimed steps, then exactly ``--steps`` timed steps.  The
+FP64 drop-in is then stepped from the same starting state for as many steps as the time budget
+holds (at most ``--fp64-steps`` after one warm-up) and the largest |<S^z_j>| difference between
+the two forms after the same number of steps is reported as the error of the mixed form.
 ``--workload heff_sweep``: one step = a left-to-right pass doing the one-site effective-Hamiltonian
 apply and the left environment push at every site (the contraction core of a half-sweep).
 
+Everything after the timed steps (FP64 twin, serving mix, Heston batch, CPU baseline) runs under
+one wall-clock budget (``--budget`` seconds from the start of the process); a phase that does not
+fit is cut or skipped and the line says so.  A last-resort alarm prints the line with what exists
+if the budget is overrun anyway.
+
 The dominant kernel (grouped complex block GEMM of the one-site apply) is timed separately with
-HIP events on its own stream for the roofline; the CPU baseline times the oracle's one-site apply
-on the same environments (downloaded) with numpy/OpenBLAS on the host cores.
+HIP events on its own stream for the roofline; the CPU baseline times the oracle on bounded
+samples of the same step's phases on the host cores (one thread and all threads).
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import signal
 import sys
 import time
 
-import numpy as np
+T_START = time.time()
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
@@ -41,24 +51,31 @@ HBM_PEAK = 8000.0                      # GB/s
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=2)
+    p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--L", type=int, default=128)
     p.add_argument("--D", type=int, default=1024)
     p.add_argument("--delta", type=float, default=0.5)
     p.add_argument("--order", type=int, default=2)
+    p.add_argument("--tail", type=float, default=1e-9,
+                   help="Schmidt-value fall across each bond of the synthetic state (0: flat)")
     p.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     p.add_argument("--half-sweeps", choices=["serial", "parallel"], default="parallel",
                    help="run the two half-sweeps one after the other or concurrently")
-    p.add_argument("--no-fp64", action="store_true",
-                   help="with --precision mixed: skip the FP64 comparison run")
+    p.add_argument("--budget", type=float, default=500.0,
+                   help="wall-clock seconds from process start for the whole run; phases after "
+                        "the timed steps are cut to fit")
+    p.add_argument("--fp64-steps", type=int, default=2,
+                   help="FP64 comparison steps after one warm-up (0: skip the FP64 twin)")
     p.add_argument("--workload", choices=["bug_step", "heff_sweep"], default="bug_step")
     p.add_argument("--kernel-reps", type=int, default=20)
-    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--cpu-seconds", type=float, default=8.0,
+                   help="seconds of oracle work per CPU sample (two samples: 1 thread, all)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--serving", type=int, default=12,
                    help="requests of the seeded Poisson serving mix to replay (0: skip)")
     p.add_argument("--serving-rate", type=float, default=1.0, help="arrivals per second")
+    p.add_argument("--serving-scale", choices=["small", "full"], default="small")
     p.add_argument("--slots", type=int, default=4, help="concurrent requests of the serving loop")
     p.add_argument("--heston", type=int, default=4,
                    help="Heston instances (2^10 x 2^10 grid) stepped through the serving loop "
@@ -78,14 +95,44 @@ def parse():
     return p.parse_args()
 
 
+def left() -> float:
+    """Seconds left of the run's budget."""
+    return ARGS.budget - (time.time() - T_START)
+
+
+def note(msg: str):
+    print(f"# {time.time() - T_START:7.1f}s {msg}", file=sys.stderr, flush=True)
+
+
+OUT: dict = {}
+_PRINTED = [False]
+
+
+def emit():
+    """Print the JSON line once (from the normal end or from the budget alarm)."""
+    if _PRINTED[0]:
+        return
+    _PRINTED[0] = True
+    print(json.dumps(OUT), flush=True)
+
+
+def _alarm(signum, frame):
+    OUT.setdefault("cut", []).append("budget alarm: the line was printed before the run ended")
+    emit()
+    os._exit(0 if "value" in OUT else 3)
+
+
 def main():
-    args = parse()
+    global ARGS
+    args = ARGS = parse()
     # the step's own progress lines (bonds, closure, root substeps) on stderr: long steps stay visible
     os.environ.setdefault("BUG_PROGRESS", "1")
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
+    signal.signal(signal.SIGALRM, _alarm)
+    signal.alarm(max(int(args.budget + 40 - (time.time() - T_START)), 1))
     import torch
 
     from bondupdategalerkin_amd import _lib
@@ -94,30 +141,35 @@ def main():
     from bondupdategalerkin_amd.mps import random_mps
 
     torch.cuda.set_device(0)
+    note("torch and the library loaded")
     prec = _lib.PREC_FP64 if args.precision == "fp64" else _lib.PREC_FP32
     ckind = "fp64" if args.precision == "fp64" else "fp32"
     L, D = args.L, args.D
-    t0 = time.time()
     from bondupdategalerkin_amd import bug_step as B
     from bondupdategalerkin_amd import tensor_ops as T
 
-    psi = random_mps(L, D, seed=0)
+    OUT.update({
+        "metric": METRIC, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128" if args.precision == "fp64" else "c64 contractions / c128 Krylov, QR, SVD",
+        "data": "synthetic",
+    })
+    t0 = time.time()
+    psi = random_mps(L, D, seed=0, tail=args.tail if args.tail > 0 else None)
     mpo = xxz_mpo(L, delta=args.delta)
     mid = L // 2
     # mixed-canonical about the centre and normalised: the environments below stay bounded
     B.canonical(psi, mid)
     T.scale_(1.0 / T.norm(psi[mid]), psi[mid])
     torch.cuda.synchronize()
-    print(f"# state built in {time.time() - t0:.1f}s; bond dims max {max(psi.bond_dims())}",
-          file=sys.stderr, flush=True)
+    note(f"state built in {time.time() - t0:.1f}s; bond dims max {max(psi.bond_dims())}")
 
     if args.autotune == "table":
         if os.path.exists(_lib.SCHEDULE_TABLE):
             _lib.autotune_load(_lib.SCHEDULE_TABLE)
             _lib.set_autotune(_lib.AUTOTUNE_USE)
         else:
-            print(f"# no tuning table at {_lib.SCHEDULE_TABLE}: heuristic schedules",
-                  file=sys.stderr, flush=True)
+            note(f"no tuning table at {_lib.SCHEDULE_TABLE}: heuristic schedules")
 
     def tuning(on):
         if args.autotune == "tune":
@@ -165,73 +217,40 @@ def main():
     # from a plain copy of known size) recorded in profiles/heff_apply_L128_D1024_mixed.md; only
     # for the shape and form they were measured on
     traffic = (126611 * 1024 if (ckind == "fp32" and L == 128 and D == 1024 and args.delta == 0.5
-                                 and args.autotune == "table") else None)
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
-                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
-                "kernel": (f"grouped_zgemm {ckind} (one-site H_eff apply: two grouped-GEMM "
-                           "stages, each followed by its split-k reduce when split)"),
-                "schedules": [{"tile": list(TILES[c]), "splits": sp} for c, sp in schedules],
-                "flops_per_apply": flops_apply, "apply_ms": round(apply_ms, 4),
-                "timing": "HIP events on the launch stream around each eager apply (its 2-3 "
-                          "launches and the gaps between them), mean of kernel_reps",
-                "apply_ms_graph": round(apply_ms_graph, 4), "graph_equals_eager": graph_equal}
+                                 and args.autotune == "table" and args.tail == 0) else None)
+    OUT["roofline"] = {
+        "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
+        "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
+        "kernel": (f"grouped_zgemm {ckind} (one-site H_eff apply at the centre site: two "
+                   "grouped-GEMM stages, each followed by its split-k reduce when split)"),
+        "schedules": [{"tile": list(TILES[c]), "splits": sp} for c, sp in schedules],
+        "flops_per_apply": flops_apply, "apply_ms": round(apply_ms, 4),
+        "timing": "HIP events on the launch stream around each eager apply (its 2-3 "
+                  "launches and the gaps between them), mean of kernel_reps",
+        "apply_ms_graph": round(apply_ms_graph, 4), "graph_equals_eager": graph_equal}
+    note(f"roofline: {OUT['roofline']['achieved']} TFLOP/s ({apply_ms:.4f} ms per apply)")
     if args.kernel_only:
-        print(json.dumps({"kernel_only": True, "roofline": roofline}), flush=True)
+        OUT["kernel_only"] = True
+        emit()
         return
 
     # ---- instance packing: many small independent applies in one launch per stage ----
-    packing = None
     if args.packing > 0:
-        small = []
-        for s in range(args.packing):
-            st = random_mps(24, 64, seed=100 + s)
-            B.canonical(st, 12)
-            ms = xxz_mpo(24, delta=args.delta)
-            le = ev.left_env_stack(st, ms, upto=12, precision=prec)
-            re = ev.right_env_stack(st, ms, downto=13, precision=prec)
-            small.append((le[12], ms[12], re[13], st[12], 0.0))
-        torch.cuda.synchronize()
-
-        def time_it(fn, reps=10):
-            fn()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            for _ in range(reps):
-                fn()
-            e1.record(stream)
-            torch.cuda.synchronize()
-            return e0.elapsed_time(e1) / reps
-
-        packed = ev.PackedApply([it[:3] for it in small], [it[3].layout for it in small],
-                                precision=prec)
-        for dst, it in zip(packed.x, small):
-            dst.data.copy_(it[3].data)
-        seq_ms = time_it(lambda: [ev.apply_one_site(a, w, b, x, precision=prec)
-                                  for (a, w, b, _, _), x in zip(small, packed.x)])
-        bat_ms = time_it(packed)
-        # the same instances' Krylov exponentials (12 Lanczos vectors each) in lockstep
-        from bondupdategalerkin_amd import krylov as KR
-        from bondupdategalerkin_amd.packed import lanczos_expv_packed
-
-        v0s = [it[3] for it in small]
-        lz_seq = time_it(lambda: [KR.lanczos_expv(
-            lambda v, a=a, w=w, b=b: ev.apply_one_site(a, w, b, v, precision=prec), -0.05j, x,
-            maxiter=12, substeps=1, conv_tol=0.0) for (a, w, b, x, _) in small], reps=3)
-        lz_pk = time_it(lambda: lanczos_expv_packed(packed, -0.05j, v0s, maxiter=12), reps=3)
-        packing = {"workload": (f"{args.packing} independent one-site H_eff applies (XXZ L=24, "
-                                "D=64 states, centre site) on vectors held in one packed buffer: "
-                                "one apply call after another vs one PackedApply call (one grouped "
-                                "launch per stage, merged plan reused); and their Krylov "
-                                "exponentials (12 Lanczos vectors, tau=-0.05i) one after another vs "
-                                "in lockstep over packed vectors"),
-                   "sequential_ms": round(seq_ms, 4), "packed_ms": round(bat_ms, 4),
-                   "speedup": round(seq_ms / bat_ms, 2),
-                   "lanczos_sequential_ms": round(lz_seq, 3), "lanczos_packed_ms": round(lz_pk, 3),
-                   "lanczos_speedup": round(lz_seq / lz_pk, 2)}
-        print(f"# instance packing: {packing}", file=sys.stderr, flush=True)
+        OUT["instance_packing"] = packing_phase(args, prec, stream)
 
     # ---- the timed step ----
-    statuses, extra = [], {}
+    statuses = []
+    OUT["config"] = {
+        "workload": (f"cbe_bug_step order {args.order}, tau=-0.02i, maxdim=D, "
+                     "work_ceiling_rule=maxdim, default tolerances, synthetic canonical U(1) "
+                     f"state (Schmidt values falling by {args.tail:g} across each bond)")
+        if args.workload == "bug_step" else
+        "heff_sweep: one-site H_eff apply + left env push at every site",
+        "L": L, "D": D, "model": f"xxz delta={args.delta}", "symmetry": "U1",
+        "precision": args.precision, "half_sweeps": args.half_sweeps,
+        "parallelism": "single GPU", "budget_status": statuses}
+    snap, twin = None, None
+    ncmp = min(1 + args.fp64_steps, args.warmup + args.steps)
     if args.workload == "bug_step":
         from bondupdategalerkin_amd import mps as M
 
@@ -239,38 +258,24 @@ def main():
         T.scale_(1.0 / T.norm(psi[L - 1]), psi[L - 1])
         B.canonical(psi, 0)
         torch.cuda.synchronize()
-        twin = psi.copy() if (args.precision == "mixed" and not args.no_fp64) else None
+        if args.precision == "mixed" and args.fp64_steps > 0:
+            twin = psi.copy()
+        done = [0]
+        phases_log = []
 
-        def make_step(state, form, record):
-            def step():
-                info = B.cbe_bug_step(state, mpo, -0.02j, maxdim=D, work_ceiling_rule="maxdim",
-                                      precision=form, order=args.order,
-                                      parallel=args.half_sweeps == "parallel")
-                if record is not None:
-                    record.append(B.budget_status(info.ledger))
-                phases = " ".join(f"{k} {v:.2f}" for k, v in info.seconds.items() if k != "step")
-                print(f"# {form} step: {info.seconds['step']:.2f}s {B.budget_status(info.ledger)} "
-                      f"{info.ledger.unmet} max bond {max(state.bond_dims())} ({phases})",
-                      file=sys.stderr, flush=True)
-            return step
-
-        step = make_step(psi, args.precision, statuses)
-        if twin is not None:
-            fp64_statuses = []
-            step64 = make_step(twin, "fp64", fp64_statuses)
-            tuning(True)
-            for _ in range(args.warmup):
-                step64()
-            torch.cuda.synchronize()
-            fp64_statuses.clear()
-            tuning(False)
-            t1 = time.perf_counter()
-            for _ in range(args.steps):
-                step64()
-            torch.cuda.synchronize()
-            ms64 = 1e3 * (time.perf_counter() - t1) / args.steps
-            extra["fp64"] = {"steps_per_s": round(1e3 / ms64, 4), "ms_per_step": round(ms64, 3),
-                             "budget_status": fp64_statuses}
+        def step():
+            info = B.cbe_bug_step(psi, mpo, -0.02j, maxdim=D, work_ceiling_rule="maxdim",
+                                  precision=args.precision, order=args.order,
+                                  parallel=args.half_sweeps == "parallel")
+            done[0] += 1
+            statuses.append(B.budget_status(info.ledger))
+            phases_log.append({k: round(v, 3) for k, v in info.seconds.items()})
+            note(f"{args.precision} step {done[0]}: {info.seconds['step']:.2f}s "
+                 f"{statuses[-1]} {info.ledger.unmet} max bond {max(psi.bond_dims())} "
+                 f"({' '.join(f'{k} {v:.2f}' for k, v in info.seconds.items() if k != 'step')})")
+            if twin is not None and done[0] == ncmp:
+                nonlocal snap
+                snap = psi.copy()
     else:
         def step():
             lenv = lenvs[0]
@@ -284,120 +289,256 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    statuses.clear()
     tuning(False)
+    warm_statuses = list(statuses)
+    statuses.clear()
     t1 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t1
     ms_per_step = 1e3 * elapsed / max(args.steps, 1)
-    if args.workload == "bug_step" and twin is not None:
-        dsz = float(np.abs(M.magnetisation(psi) - M.magnetisation(twin)).max())
-        extra["fp64"]["max_abs_dSz_mixed_vs_fp64"] = dsz
-        extra["fp64"]["speedup_mixed_vs_fp64"] = round(extra["fp64"]["ms_per_step"] / ms_per_step, 3)
+    OUT["value"] = round(1e3 / ms_per_step, 4)
+    OUT["ms_per_step"] = round(ms_per_step, 3)
+    if args.workload == "bug_step":
+        OUT["config"]["warmup_budget_status"] = warm_statuses
+        OUT["config"]["max_bond_after"] = max(psi.bond_dims())
+        OUT["phase_seconds_per_step"] = {
+            k: round(float(np.mean([p.get(k, 0.0) for p in phases_log[args.warmup:]])), 3)
+            for k in phases_log[-1]} if phases_log[args.warmup:] else None
+    note(f"headline: {OUT['value']} steps/s ({ms_per_step:.1f} ms per step)")
+
+    # ---- FP64 twin: the drop-in form from the same start, as far as the budget holds ----
+    if twin is not None:
+        OUT["fp64"] = fp64_phase(args, B, M, mpo, twin, snap, ncmp, ms_per_step)
 
     if args.serving > 0 and args.workload == "bug_step":
-        from bondupdategalerkin_amd import serving as S
-
-        srv = {"mix": (f"{args.serving} requests, Poisson arrivals at {args.serving_rate}/s "
-                       "(seed 0): 60% XX/XXZ chains L in {16,24,32}, maxdim in {16,32,64}, 2-6 "
-                       "steps, order 2; 40% Heston QTT 2^5-2^7 per axis, maxdim 16-32, 2-4 steps; Heston "
-                       "requests run FP64 in both forms (non-Hermitian, stiff)"),
-               "slots": args.slots}
-        for form in (["mixed", "fp64"] if args.precision == "mixed" else ["fp64"]):
-            print(f"# serving mix ({form})", file=sys.stderr, flush=True)
-            srv[form] = S.serve_mix(S.poisson_mix(args.serving, args.serving_rate, seed=0),
-                                    args.slots, form)
-            print(f"# serving {form}: {srv[form]}", file=sys.stderr, flush=True)
-        extra["serving"] = srv
+        OUT["serving"] = serving_phase(args)
 
     if args.heston > 0 and args.workload == "bug_step":
-        from bondupdategalerkin_amd import serving as S
+        OUT["heston_batch"] = heston_phase(args)
 
-        print(f"# heston batch: {args.heston} instances", file=sys.stderr, flush=True)
-        hrng = np.random.default_rng(0)
-        eng = S.Engine(slots=args.slots)
-        try:
-            t0 = time.perf_counter()
-            rids = []
-            for _ in range(args.heston):
-                prm = (hrng.uniform(0, 0.05), hrng.uniform(0, 0.05), hrng.uniform(0.5, 4),
-                       hrng.uniform(0.02, 0.15), hrng.uniform(0.1, 0.8), hrng.uniform(-0.9, 0))
-                rids.append(eng.submit(S.Request(
-                    "heston", {"bits": 10, "params": prm, "maxdim": 128,
-                               "steps": args.heston_steps, "dt": 0.02}, precision="fp64")))
-            res = eng.wait(rids, timeout=1200)
-            span = time.perf_counter() - t0
-        finally:
-            eng.close()
-        lat = np.array([r.latency for r in res])
-        extra["heston_batch"] = {
-            "workload": (f"{args.heston} Heston QTT instances, 2^10 x 2^10 grid (x in [-1,1], v in "
-                         "[0,0.5]), params drawn as the batched config (seed 0), psi0 = "
-                         f"exp(0.5x) exp(-v), order 2, maxdim 128, tau 0.02, hermitian=false, "
-                         f"{args.heston_steps} step(s) each, FP64, {args.slots} concurrent slots"),
-            "instance_steps_per_s": round(sum(r.steps for r in res) / span, 4),
-            "makespan_s": round(span, 2),
-            "p50_latency_s": round(float(np.percentile(lat, 50)), 2),
-            "errors": [r.error for r in res if r.error],
-            "budget_status": [s for r in res for s in r.statuses],
-        }
-        print(f"# heston batch: {extra['heston_batch']}", file=sys.stderr, flush=True)
-
-    cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(sample_sites, mpo, lenvs, renvs, mid, args)
+        if left() > 2.5 * args.cpu_seconds + 30:
+            OUT["cpu_baseline"] = cpu_baseline(sample_sites, mpo, lenvs, renvs, mid, args)
+        else:
+            OUT["cpu_baseline"] = None
+            OUT.setdefault("cut", []).append("cpu_baseline: no budget left")
 
-    out = {
-        "metric": METRIC,
-        "value": round(1e3 / ms_per_step, 4),
-        "unit": "steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "c128" if args.precision == "fp64" else "c64 contractions / c128 Krylov, QR, SVD",
-        "data": "synthetic",
-        "config": {"workload": (f"cbe_bug_step order {args.order}, tau=-0.02i, maxdim=D, "
-                                "work_ceiling_rule=maxdim, synthetic canonical U(1) state")
-                   if args.workload == "bug_step" else
-                   "heff_sweep: one-site H_eff apply + left env push at every site",
-                   "L": L, "D": D, "model": f"xxz delta={args.delta}", "symmetry": "U1",
-                   "precision": args.precision, "half_sweeps": args.half_sweeps,
-                   "parallelism": "single GPU",
-                   "budget_status": statuses},
-        **extra,
-        "autotune": {"mode": {"table": "package tuning table for gfx950 "
-                                        "(bondupdategalerkin_amd/schedules_gfx950.txt)",
-                              "tune": "tuned in the kernel warm-up and warm-up steps, table "
-                                      "used in the timed steps",
-                              "off": "heuristic schedules"}[args.autotune],
-                     "classes": _lib.autotune_count()[0]},
-        "instance_packing": packing,
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
+    OUT["autotune"] = {"mode": {"table": "package tuning table for gfx950 "
+                                         "(bondupdategalerkin_amd/schedules_gfx950.txt)",
+                                "tune": "tuned in the kernel warm-up and warm-up steps, table "
+                                        "used in the timed steps",
+                                "off": "heuristic schedules"}[args.autotune],
+                       "classes": _lib.autotune_count()[0]}
     st = ctx.stats()
-    out["plan_cache"] = {"plans": st["plans"], "hits": st["hits"], "misses": st["misses"],
+    OUT["plan_cache"] = {"plans": st["plans"], "hits": st["hits"], "misses": st["misses"],
                          "context": "main thread"}
+    OUT["run_seconds"] = round(time.time() - T_START, 1)
     if args.save_schedules:
         _lib.autotune_save(args.save_schedules)
-    print(json.dumps(out), flush=True)
+    signal.alarm(0)
+    emit()
+
+
+def packing_phase(args, prec, stream):
+    import torch
+
+    from bondupdategalerkin_amd import bug_step as B
+    from bondupdategalerkin_amd import environments as ev
+    from bondupdategalerkin_amd import krylov as KR
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+    from bondupdategalerkin_amd.mps import random_mps
+    from bondupdategalerkin_amd.packed import lanczos_expv_packed
+
+    small = []
+    for s in range(args.packing):
+        st = random_mps(24, 64, seed=100 + s)
+        B.canonical(st, 12)
+        ms = xxz_mpo(24, delta=args.delta)
+        le = ev.left_env_stack(st, ms, upto=12, precision=prec)
+        re = ev.right_env_stack(st, ms, downto=13, precision=prec)
+        small.append((le[12], ms[12], re[13], st[12], 0.0))
+    torch.cuda.synchronize()
+
+    def time_it(fn, reps=10):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    packed = ev.PackedApply([it[:3] for it in small], [it[3].layout for it in small],
+                            precision=prec)
+    for dst, it in zip(packed.x, small):
+        dst.data.copy_(it[3].data)
+    seq_ms = time_it(lambda: [ev.apply_one_site(a, w, b, x, precision=prec)
+                              for (a, w, b, _, _), x in zip(small, packed.x)])
+    bat_ms = time_it(packed)
+    v0s = [it[3] for it in small]
+    lz_seq = time_it(lambda: [KR.lanczos_expv(
+        lambda v, a=a, w=w, b=b: ev.apply_one_site(a, w, b, v, precision=prec), -0.05j, x,
+        maxiter=12, substeps=1, conv_tol=0.0) for (a, w, b, x, _) in small], reps=3)
+    lz_pk = time_it(lambda: lanczos_expv_packed(packed, -0.05j, v0s, maxiter=12), reps=3)
+    out = {"workload": (f"{args.packing} independent one-site H_eff applies (XXZ L=24, "
+                        "D=64 states, centre site) on vectors held in one packed buffer: "
+                        "one apply call after another vs one PackedApply call (one grouped "
+                        "launch per stage, merged plan reused); and their Krylov "
+                        "exponentials (12 Lanczos vectors, tau=-0.05i) one after another vs "
+                        "in lockstep over packed vectors"),
+           "sequential_ms": round(seq_ms, 4), "packed_ms": round(bat_ms, 4),
+           "speedup": round(seq_ms / bat_ms, 2),
+           "lanczos_sequential_ms": round(lz_seq, 3), "lanczos_packed_ms": round(lz_pk, 3),
+           "lanczos_speedup": round(lz_seq / lz_pk, 2)}
+    note(f"instance packing: {out}")
+    return out
+
+
+def fp64_phase(args, B, M, mpo, twin, snap, ncmp, ms_mixed):
+    """FP64 steps from the mixed run's starting state: one warm-up, then up to --fp64-steps timed
+    ones while the budget holds (an FP64 step is taken to cost 1.3x a mixed one when planning)."""
+    import torch
+
+    out = {"warmup": 1, "steps": 0}
+    est = 1.3 * ms_mixed / 1e3
+    if left() < 2 * est + 60:
+        OUT.setdefault("cut", []).append("fp64 twin: no budget left")
+        return {"skipped": "budget"}
+    sts = []
+
+    def step():
+        info = B.cbe_bug_step(twin, mpo, -0.02j, maxdim=ARGS.D, work_ceiling_rule="maxdim",
+                              precision="fp64", order=ARGS.order,
+                              parallel=ARGS.half_sweeps == "parallel")
+        sts.append(B.budget_status(info.ledger))
+        note(f"fp64 step: {info.seconds['step']:.2f}s {sts[-1]} {info.ledger.unmet}")
+
+    step()
+    torch.cuda.synchronize()
+    times = []
+    while out["steps"] < ncmp - 1 and left() > est + 60:
+        t1 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t1)
+        out["steps"] += 1
+    if times:
+        ms64 = 1e3 * float(np.mean(times))
+        out.update(steps_per_s=round(1e3 / ms64, 4), ms_per_step=round(ms64, 3),
+                   speedup_mixed_vs_fp64=round(ms64 / ms_mixed, 3))
+    out["budget_status"] = sts
+    if snap is not None and 1 + out["steps"] == ncmp:
+        dsz = float(np.abs(M.magnetisation(snap) - M.magnetisation(twin)).max())
+        out["max_abs_dSz_mixed_vs_fp64"] = dsz
+        out["compared_after_steps"] = ncmp
+    else:
+        OUT.setdefault("cut", []).append(
+            f"fp64 twin: {1 + out['steps']} of {ncmp} steps in budget, no S^z comparison")
+    note(f"fp64: {out}")
+    return out
+
+
+def serving_phase(args):
+    from bondupdategalerkin_amd import serving as S
+
+    mixdesc = {"small": ("60% XX/XXZ chains L in {16,24,32}, maxdim in {16,32,64}, 2-6 steps, "
+                         "order 2; 40% Heston QTT 2^5-2^7 per axis, maxdim 16-32, 2-4 steps"),
+               "full": ("60% XX/XXZ chains L in {32,64,96,128}, maxdim in {64,256,512,1024}, "
+                        "20-200 steps; 40% Heston QTT 2^8-2^11 per axis, maxdim 32-128, 10-50 "
+                        "steps")}[args.serving_scale]
+    srv = {"mix": (f"{args.serving} requests, Poisson arrivals at {args.serving_rate}/s (seed 0): "
+                   f"{mixdesc}; Heston requests run FP64 in both forms (non-Hermitian, stiff)"),
+           "slots": args.slots}
+    forms = ["mixed", "fp64"] if ARGS.precision == "mixed" else ["fp64"]
+    for i, form in enumerate(forms):
+        # an equal share of what is left for each form, keeping 60 s for the later phases
+        share = (left() - 60) / (len(forms) - i)
+        if share < 20:
+            srv[form] = {"skipped": "budget"}
+            OUT.setdefault("cut", []).append(f"serving {form}: no budget left")
+            continue
+        note(f"serving mix ({form}), {share:.0f}s at most")
+        srv[form] = S.serve_mix(S.poisson_mix(args.serving, args.serving_rate, seed=0,
+                                              scale=args.serving_scale),
+                                args.slots, form, timeout=share)
+        if srv[form].get("cancelled"):
+            OUT.setdefault("cut", []).append(
+                f"serving {form}: {srv[form]['cancelled']} requests cancelled at the budget")
+        note(f"serving {form}: {srv[form]}")
+    return srv
+
+
+def heston_phase(args):
+    from bondupdategalerkin_amd import serving as S
+
+    if left() < 90:
+        OUT.setdefault("cut", []).append("heston batch: no budget left")
+        return {"skipped": "budget"}
+    note(f"heston batch: {args.heston} instances")
+    hrng = np.random.default_rng(0)
+    eng = S.Engine(slots=args.slots)
+    res = []
+    try:
+        t0 = time.perf_counter()
+        rids = []
+        for _ in range(args.heston):
+            prm = (hrng.uniform(0, 0.05), hrng.uniform(0, 0.05), hrng.uniform(0.5, 4),
+                   hrng.uniform(0.02, 0.15), hrng.uniform(0.1, 0.8), hrng.uniform(-0.9, 0))
+            rids.append(eng.submit(S.Request(
+                "heston", {"bits": 10, "params": prm, "maxdim": 128,
+                           "steps": args.heston_steps, "dt": 0.02}, precision="fp64")))
+        try:
+            eng.wait(rids, timeout=max(left() - 45, 1.0))
+        except TimeoutError:
+            eng.cancel()
+            OUT.setdefault("cut", []).append("heston batch: cancelled at the budget")
+    finally:
+        eng.close()
+    span = time.perf_counter() - t0
+    res = [eng.results[r] for r in rids if r in eng.results]
+    ok = [r for r in res if r.error is None]
+    out = {
+        "workload": (f"{args.heston} Heston QTT instances, 2^10 x 2^10 grid (x in [-1,1], v in "
+                     "[0,0.5]), params drawn as the batched config (seed 0), psi0 = "
+                     f"exp(0.5x) exp(-v), order 2, maxdim 128, tau 0.02, hermitian=false, "
+                     f"{args.heston_steps} step(s) each, FP64, {args.slots} concurrent slots"),
+        "instance_steps_per_s": round(sum(r.steps for r in res) / span, 4) if span > 0 else None,
+        "completed": len(ok),
+        "makespan_s": round(span, 2),
+        "p50_latency_s": round(float(np.percentile([r.latency for r in ok], 50)), 2) if ok else None,
+        "errors": [r.error for r in res if r.error and not r.error.startswith("Cancelled")],
+        "budget_status": [s for r in res for s in r.statuses],
+    }
+    note(f"heston batch: {out}")
+    return out
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(sites, mpo, lenvs, renvs, mid, args):
-    """The oracle on the host cores, on a bounded sample of the same step, scaled to steps/s.
+    """The oracle on the host cores, on bounded samples of the same step, scaled to steps/s.
 
-    Sample: whole bond visits of the left half-sweep at the centre bond (unique-QR frame, DD-QR CBE
-    with its probe, streamed sketch and rank bisection, local Lanczos enrichment) on the device
-    tensors downloaded, repeated for about ``--cpu-seconds``; plus one truncation cut (SVD of the
-    centre site).  Estimate: an order-2 step visits every bond twice (half-sweeps, frontier sweep)
-    and cuts every site once, so t_step ~ 2 (L-1) t_visit + L t_cut (an upper estimate: bonds near
-    the ends are smaller than the centre one)."""
+    Samples (each repeated for about ``--cpu-seconds``, once with the BLAS pool held to one thread
+    and once with every host thread): whole bond visits of the left half-sweep at the centre bond
+    (unique-QR frame, DD-QR CBE with its probe, streamed sketch and rank bisection, local Lanczos
+    enrichment) on the device tensors downloaded; and one truncation cut (SVD of the centre
+    site).  Scale: an order-2 step visits every bond twice (half-sweeps, frontier sweep) and cuts
+    every site once, so t_step ~ 2 (L-1) t_visit + L t_cut (an upper estimate: bonds near the
+    ends are smaller than the centre one)."""
+    from threadpoolctl import threadpool_info, threadpool_limits
+
     from oracle import blocks as ob
     from oracle import bug as obug
     from oracle import linalg as ola
@@ -413,44 +554,58 @@ def cpu_baseline(sites, mpo, lenvs, renvs, mid, args):
     tau = -0.02j
     W = {mid: site(mpo[mid]), mid + 1: site(mpo[mid + 1])}
     Lf, Rf = fam(lenvs[mid]), fam(renvs[mid + 2])
-    A, B = fam(sites[0]), fam(sites[1])
+    A, Bs = fam(sites[0]), fam(sites[1])
     opts = obug.StepOptions(order=2, maxdim=D, work_ceiling_rule="maxdim")
     alloc = obug.basis_allocation(opts.tol_basis, L - 1, tau, 1.0, 2)
     wcap = 2 * D
-    rng = ola.ProbeRNG(0x5EED)
 
-    def visit():
+    def visit(rng):
         ledger = obug.Ledger()
-        fr = obug.frame_from(A, B)
+        fr = obug.frame_from(A, Bs)
         ex = obug.dd_cbe_expand(fr, W, mid, Lf, Rf, "left", alloc, ledger, wcap, opts, rng)
-        Kex = ola.site_times_core(A, ola.overlap_right(B, ex.V_ex))
+        Kex = ola.site_times_core(A, ola.overlap_right(Bs, ex.V_ex))
         rch1 = ob.push_right(Rf, ex.V_ex, W[mid + 1])
         obug.krylov_enrich(Kex, lambda v: ob.apply_one_site(Lf, W[mid], rch1, v), ex.U_ex, "left",
                            tau, alloc, wcap, "lanczos")
 
-    n, t0 = 0, time.perf_counter()
-    while True:
-        visit()
-        n += 1
-        if time.perf_counter() - t0 > args.cpu_seconds:
-            break
-    t_visit = (time.perf_counter() - t0) / n
-    t1 = time.perf_counter()
-    ola.svd_left(A, cutoff=0.0, nkeep=D)
-    t_cut = time.perf_counter() - t1
-    t_step = 2 * (L - 1) * t_visit + L * t_cut
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
-    except Exception:
-        threads = os.cpu_count()
-    return {"value": round(1.0 / t_step, 6), "unit": "steps/s (estimated from a sample)",
-            "cores": threads, "kind": "port",
-            "sample": (f"{n} oracle bond visits at the centre bond (frame, DD-QR CBE, Lanczos "
-                       f"enrichment) of the L={L}, D={D} state, {1e3 * t_visit:.1f} ms each, and "
-                       f"one centre truncation SVD, {1e3 * t_cut:.1f} ms; step ~ 2 (L-1) t_visit "
-                       f"+ L t_cut = {t_step:.2f} s")}
+    def sample():
+        rng = ola.ProbeRNG(0x5EED)
+        n, t0 = 0, time.perf_counter()
+        while True:
+            visit(rng)
+            n += 1
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        t_visit = (time.perf_counter() - t0) / n
+        t1 = time.perf_counter()
+        ola.svd_left(A, cutoff=0.0, nkeep=D)
+        t_cut = time.perf_counter() - t1
+        return n, t_visit, t_cut
 
+    threads = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
+    with threadpool_limits(limits=1):
+        n1, v1, c1 = sample()
+    nall, vall, call = sample()
+    step1 = 2 * (L - 1) * v1 + L * c1
+    stepall = 2 * (L - 1) * vall + L * call
+    out = {"value": round(1.0 / stepall, 6), "unit": "steps/s (scaled from a sample)",
+           "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+           "host_threads": os.cpu_count(),
+           "one_thread": {"value": round(1.0 / step1, 6), "cores": 1,
+                          "visit_ms": round(1e3 * v1, 1), "cut_ms": round(1e3 * c1, 1),
+                          "visits": n1},
+           "all_threads": {"value": round(1.0 / stepall, 6), "cores": threads,
+                           "visit_ms": round(1e3 * vall, 1), "cut_ms": round(1e3 * call, 1),
+                           "visits": nall},
+           "sample": (f"oracle (numpy + OpenBLAS) bond visits at the centre bond (frame, DD-QR "
+                      f"CBE, Lanczos enrichment) of the L={L}, D={D} state, repeated for "
+                      f"{args.cpu_seconds:g} s, and one centre truncation SVD, on 1 thread and on "
+                      f"{threads} threads; step ~ 2 (L-1) t_visit + L t_cut")}
+    note(f"cpu baseline: {out}")
+    return out
+
+
+ARGS = None
 
 if __name__ == "__main__":
     main()

@@ -123,22 +123,46 @@ def synthetic_bond_legs(L: int, dmax: int, sigma: float = 4.0) -> List[Leg]:
     return legs
 
 
+def column_weights(leg: Leg, tail: float) -> Dict[int, np.ndarray]:
+    """Per-sector column weights that fall geometrically from 1 to ``tail`` across the leg.
+
+    Sectors are interleaved in proportion to their size: column j of a sector of dimension d_q on
+    a leg of total dimension D sits at rank (j + 1/2) D / d_q of the whole leg and gets the weight
+    tail ** (rank / D).  Scaling the right-bond columns of a random site tensor by these weights
+    gives the state a Schmidt spectrum that decays about as fast (the columns with small weight
+    carry little of the state), instead of the flat spectrum of a plain Gaussian MPS."""
+    out = {}
+    for q, d in zip(leg.charges, leg.dims):
+        rank = (np.arange(d) + 0.5) * (leg.total / d)
+        out[q] = tail ** (rank / leg.total)
+    return out
+
+
 def random_mps(L: int, dmax: int, seed: int = 0, sigma: float = 4.0,
-               symmetry: str = "U1") -> SymMPS:
-    """Seeded complex-Gaussian U(1) MPS on the synthetic bond legs (not canonicalised)."""
+               symmetry: str = "U1", tail: Optional[float] = None) -> SymMPS:
+    """Seeded complex-Gaussian U(1) MPS on the synthetic bond legs (not canonicalised).
+
+    ``tail`` (e.g. 1e-8): the columns of every right bond are scaled by ``column_weights``, so the
+    Schmidt values fall by about that factor across each bond; None keeps the flat spectrum."""
     rng = np.random.default_rng(seed)
     charges = SPIN_CHARGES_U1 if symmetry == "U1" else SPIN_CHARGES_NONE
     if symmetry == "U1":
         legs = synthetic_bond_legs(L, dmax, sigma)
     else:
         legs = [Leg.dense(min(2 ** min(b, L - b), dmax)) for b in range(L + 1)]
+    if tail is not None and not 0 < tail <= 1:
+        raise ValueError("tail must lie in (0, 1]")
     tensors = []
     for i in range(L):
         lay = Layout(legs[i], legs[i + 1], charges)
+        cw = column_weights(legs[i + 1], tail) if tail is not None and i < L - 1 else None
         blocks = {}
         for k, q, qc, d, dc, off in lay.blocks():
-            blocks[(k, q)] = (rng.standard_normal((d, dc)) + 1j * rng.standard_normal((d, dc))) \
+            blk = (rng.standard_normal((d, dc)) + 1j * rng.standard_normal((d, dc))) \
                 / np.sqrt(2.0 * max(1, legs[i].total))
+            if cw is not None:
+                blk = blk * cw[qc][None, :]
+            blocks[(k, q)] = blk
         tensors.append(BlockTensor.from_blocks(lay, blocks))
     return SymMPS(tensors, charges, center=0)
 

@@ -123,12 +123,19 @@ def precision_used(req: Request) -> str:
     return req.precision if req.kind == "chain" else "fp64"
 
 
-def run_request(req: Request):
-    """Run one request to completion on the calling thread's current stream."""
+class Cancelled(RuntimeError):
+    """A request stopped between two steps because its engine was cancelled."""
+
+
+def run_request(req: Request, stop: Optional[threading.Event] = None):
+    """Run one request to completion on the calling thread's current stream; with ``stop`` set
+    (``Engine.cancel``) it ends between two steps with ``Cancelled``."""
     psi, mpo, tau, herm = _build(req)
     form = precision_used(req)
     statuses = []
     for _ in range(req.params["steps"]):
+        if stop is not None and stop.is_set():
+            raise Cancelled(f"cancelled after {len(statuses)} of {req.params['steps']} steps")
         info = B.cbe_bug_step(psi, mpo, tau, order=req.order, maxdim=req.params["maxdim"],
                               hermitian=herm, work_ceiling_rule="maxdim",
                               precision=form, rng=req.seed)
@@ -158,6 +165,7 @@ class Engine:
         self._running = 0
         self._next = 0
         self._closed = False
+        self._stop = threading.Event()
         self.results: Dict[int, Result] = {}
         self._threads = [threading.Thread(target=self._worker, name=f"bug-slot-{i}", daemon=True)
                          for i in range(slots)]
@@ -194,6 +202,14 @@ class Engine:
                 if left is not None and left <= 0:
                     raise TimeoutError("requests still running")
                 self._cv.wait(left)
+
+    def cancel(self):
+        """Drop the requests still waiting for admission (they get no result) and make the running
+        ones stop after their current step (their results carry a ``Cancelled`` error)."""
+        with self._cv:
+            self._pending.clear()
+            self._stop.set()
+            self._cv.notify_all()
 
     def close(self):
         """Refuse new requests, let the admitted and waiting ones finish, stop the slots."""
@@ -237,7 +253,7 @@ class Engine:
             err, psi, statuses, obs = None, None, [], np.zeros(0)
             try:
                 with torch.cuda.stream(stream):
-                    psi, statuses, obs = run_request(req)
+                    psi, statuses, obs = run_request(req, self._stop)
             except Exception as e:          # reported on the result, the slot stays alive
                 err = f"{type(e).__name__}: {e}"
             try:
@@ -310,21 +326,35 @@ def poisson_mix(n: int, rate: float, seed: int = 0, scale: str = "small") -> Lis
 
 def serve_mix(mix: Sequence[tuple], slots: int, precision: str, timeout: float = 900.0) -> Dict:
     """Replay a timed mix (``poisson_mix``) against a fresh engine: each request is submitted at
-    its arrival offset; returns ``summarize`` of the run."""
+    its arrival offset; returns ``summarize`` of the requests that completed.  When ``timeout``
+    seconds pass first, the engine is cancelled (waiting requests dropped, running ones stopped
+    after their current step) and ``cancelled`` counts the requests left out of the summary."""
     eng = Engine(slots=slots)
+    t0 = time.perf_counter()
+    deadline = t0 + timeout
+    rids = []
     try:
-        t0 = time.perf_counter()
-        rids = []
         for offset, req in mix:
             req.precision = precision
             delay = t0 + offset - time.perf_counter()
             if delay > 0:
+                if t0 + offset > deadline:
+                    break
                 time.sleep(delay)
             rids.append(eng.submit(req))
-        res = eng.wait(rids, timeout=timeout)
-        makespan = max(r.finished for r in res) - t0
+        try:
+            eng.wait(rids, timeout=max(deadline - time.perf_counter(), 0.0))
+        except TimeoutError:
+            eng.cancel()
     finally:
         eng.close()
-    out = summarize(res, makespan)
-    out["statuses_met"] = sum(s == "met" for r in res for s in r.statuses)
+    done = [eng.results[r] for r in rids if r in eng.results and eng.results[r].error is None]
+    failed = [eng.results[r] for r in rids
+              if r in eng.results and eng.results[r].error is not None
+              and not eng.results[r].error.startswith("Cancelled")]
+    makespan = (max(r.finished for r in done) - t0) if done else 0.0
+    out = summarize(done, makespan) if done else {"requests": 0, "steps": 0}
+    out["errors"] = len(failed)
+    out["cancelled"] = len(mix) - len(done) - len(failed)
+    out["statuses_met"] = sum(s == "met" for r in done for s in r.statuses)
     return out
