@@ -169,6 +169,7 @@ SIGNATURES = {
     "bug_set_gram_jacobi_params": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "bug_set_mixed_jacobi_params": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     "bug_svd_sweeps": (ctypes.c_int, [_P, ctypes.c_int32, _i32p]),
+    "bug_set_svd_floor": (ctypes.c_int, [_P, ctypes.c_double]),
     "bug_block_svd": (
         ctypes.c_int,
         [_P, ctypes.c_int32, ctypes.POINTER(BugSvdJob), _P, ctypes.c_int64, _P, ctypes.c_int64,

@@ -1360,7 +1360,7 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         ts = time.perf_counter()
         _progress("frontier closure")
         fr = KLFrames(left_frames, right_frames, rl, lroot, rroot)
-        cx = _Ctx(mpo, tau, alloc, wcap, ex, stream, recurrence, cp, htol, fold, exact)
+        cx = _Ctx(mpo, tau, alloc, wcap, ex, stream, recurrence, cp, htol, fold, exact, parallel)
         root_start, _ = single_pass_closure(fr, psi, root_start, cx, nrm, ledger, order)
         lroot, rroot = fr.lenv, fr.renv
         for b in range(rl):

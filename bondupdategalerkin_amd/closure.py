@@ -64,11 +64,17 @@ class _Ctx:
     """What every bond of a frontier sweep shares: the step's settings and the probe stream."""
 
     def __init__(self, mpo, tau, alloc, wcap, ex, stream, recurrence, prec, htol, fold=True,
-                 exact=False):
+                 exact=False, parallel=False):
         self.mpo, self.tau, self.alloc, self.wcap = mpo, tau, alloc, wcap
         self.ex, self.stream, self.recurrence = ex, stream, recurrence
         self.prec, self.htol = prec, htol
         self.fold, self.exact = fold, exact
+        self.parallel = parallel
+
+    def with_stream(self, stream) -> "_Ctx":
+        """The same settings drawing from another probe stream."""
+        return _Ctx(self.mpo, self.tau, self.alloc, self.wcap, self.ex, stream, self.recurrence,
+                    self.prec, self.htol, self.fold, self.exact, self.parallel)
 
 
 def root_generator(mpo, fr: KLFrames, prec: int):
@@ -324,9 +330,20 @@ def frontier_sweep(fr: KLFrames, psi: SymMPS, core: BlockTensor, cx: _Ctx, nrm: 
     Fm = frontier_mps(psi, fr, core)
     FmR = Fm.copy()
     amp = nF / max(nrm, eps)
-    lg = AdaptiveLedger()
-    Wn, lenv, leak_l, add_l = frontier_left(Fm, fr, cx, amp, lg)
-    Zn, renv, leak_r, add_r = frontier_right(FmR, fr, cx, amp, lg)
+    lg, lg_l, lg_r = AdaptiveLedger(), AdaptiveLedger(), AdaptiveLedger()
+    if cx.parallel:
+        # the two halves concurrently, the right one on the side thread and stream with a probe
+        # stream split off the step's (frontier_closure.jl:125,137-140)
+        from .bug_step import _run_beside
+        cxR = cx.with_stream(cx.stream.split())
+        (Wn, lenv, leak_l, add_l), (Zn, renv, leak_r, add_r) = _run_beside(
+            lambda: frontier_left(Fm, fr, cx, amp, lg_l),
+            lambda: frontier_right(FmR, fr, cxR, amp, lg_r))
+    else:
+        Wn, lenv, leak_l, add_l = frontier_left(Fm, fr, cx, amp, lg_l)
+        Zn, renv, leak_r, add_r = frontier_right(FmR, fr, cx, amp, lg_r)
+    lg.merge(lg_l)
+    lg.merge(lg_r)
     lg.n_closure_sweeps = 1
     ledger.merge(lg)
     lL = math.sqrt(sum(x * x for x in leak_l))

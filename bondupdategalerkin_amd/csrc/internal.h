@@ -212,11 +212,19 @@ struct bug_ctx {
     int bj_cols = 0;
     int bj_inner = 1;
     // Gram block Jacobi: columns per block pair, inner sweeps per round, and the smallest matrix
-    // dimension of a batch for which the automatic choice takes it over the staged kernel (by
-    // default never: on the step's blocks, 100-512 wide, it runs no faster than the staged kernel)
+    // dimension of a batch for which the automatic choice takes it over the staged kernel (128:
+    // replayed on the SVD batches of an L = 128, D <= 1024 step it took the truncation cuts, 250-510
+    // wide, in 0.77x the staged kernel's time and the 128-256 wide Krylov residuals in 0.95x; on
+    // narrower blocks the staged kernel is faster)
     int gbj_cols = 32;
     int gbj_inner = 1;
-    int gbj_min_n = 1 << 30;
+    int gbj_min_n = 128;
+    // fused Gram block Jacobi (svd_variant 6): the smallest widest-block width of a batch from
+    // which the automatic choice takes it (never by default: on the same replay it took 1.4x the
+    // staged kernel's time)
+    int fbj_min_n = 1 << 30;
+    // Jacobi pairs whose two columns hold less than svd_floor^2 of ||X||_F^2 are not rotated
+    double svd_floor = 1e-16;
     // FP32-preconditioned block Jacobi (svd_variant 5): FP32 sweeps at most, and the smallest
     // matrix dimension of a batch for which the automatic choice (variant 0) takes it
     int mixed_fp32_sweeps = 12;

@@ -1228,7 +1228,9 @@ __global__ __launch_bounds__(kBjThreads) void bj_round_kernel(const BjMat* __res
                                                               int round,
                                                               const int* __restrict__ flags_prev,
                                                               int* __restrict__ flags_cur,
-                                                              R eps, int inner) {
+                                                              R eps, int inner,
+                                                              const R* __restrict__ fro2,
+                                                              R floor_rel2) {
     typedef typename Cplx<R>::type C2;
     extern __shared__ __align__(16) unsigned char bj_lds[];
     C2* xs = reinterpret_cast<C2*>(bj_lds);   // n * c staged columns, then c * c Z
@@ -1263,6 +1265,11 @@ __global__ __launch_bounds__(kBjThreads) void bj_round_kernel(const BjMat* __res
     if (tid == 0) s_rot = 0;
     __syncthreads();
     const R tol2 = eps * eps * (R)n;
+    // A pair whose two columns together hold less than floor_rel2 of the matrix's squared
+    // Frobenius norm is left alone: rotations among such columns only reshape directions whose
+    // singular values lie below floor_rel2^(1/2) ||X||_F, which no selection keeps, and a column
+    // can only grow by rotating against a larger partner, which is never skipped.
+    const R floor2 = floor_rel2 * fro2[tk.x];
     // pairs per inner round: round-robin over all c columns, or the cyclic shifts of the smaller
     // block against the larger
     const int big = cp >= cq ? cp : cq, small = cp >= cq ? cq : cp;
@@ -1327,7 +1334,7 @@ __global__ __launch_bounds__(kBjThreads) void bj_round_kernel(const BjMat* __res
                 }
                 const R ag2 = gr * gr + gi * gi;
                 // rotate when |gamma| > tol sqrt(al be), tol = eps sqrt(n) (squared: no roots)
-                if (valid && ag2 > tol2 * (al * be) && ag2 != R(0)) {
+                if (valid && al + be >= floor2 && ag2 > tol2 * (al * be) && ag2 != R(0)) {
                     const R ag = sqrt(ag2);
                     const R inv = R(1) / ag;
                     const R zeta = R(0.5) * (be - al) * inv;
@@ -1608,6 +1615,220 @@ __global__ __launch_bounds__(kGbjThreads) void gbj_rotate_kernel(const BjMat* __
     if (tid == 0) flags_cur[tk.x] = 1;
 }
 
+// ---- Fused Gram block Jacobi: one workgroup per block pair, columns streamed --------------------
+//
+// The block one-sided Jacobi of the staged kernel, with the pair's columns never staged whole: a
+// round's workgroup streams its c <= 32 columns of X through LDS in chunks of kFbjRows rows to form
+// their Gram matrix G = X_c^H X_c (4 x 4 complex register tiles per lane, one wave per quarter of
+// each chunk's rows, the four partial sums added in a fixed order), runs the pair's sweep of
+// two-sided rotations on G (G <- J^H G J, Z <- Z J; the rotation test and angles of the staged
+// kernel), and, when it rotated, applies Z to the columns of X and of V chunk by chunk in place.
+// The block width no longer depends on the row count (the staged kernel holds n x c in LDS, so its
+// blocks narrow as n grows and a sweep needs ~n^2 / 4096 rounds at n >= 256); a round costs three
+// streamed passes over n x c.  The Gram is rebuilt from the columns every round, so convergence is
+// judged on fresh inner products, as in the staged kernel.
+constexpr int kFbjThreads = 256;
+constexpr int kFbjCols = 32;                  // columns of a block pair (2w)
+constexpr int kFbjRows = 64;                  // rows per streamed chunk
+constexpr int kFbjLd = kFbjRows + 2;          // chunk stored column-major, padded
+constexpr int kFbjGld = kFbjCols + 1;         // G and Z row stride
+constexpr size_t kFbjLds = sizeof(double2) * ((size_t)kFbjCols * kFbjLd + 2 * (size_t)kFbjCols * kFbjGld);
+
+__global__ __launch_bounds__(kFbjThreads) void fbj_round_kernel(const BjMat* __restrict__ mats,
+                                                                const int2* __restrict__ tasks,
+                                                                double2* __restrict__ X,
+                                                                double2* __restrict__ V, int round,
+                                                                const int* __restrict__ flags_prev,
+                                                                int* __restrict__ flags_cur,
+                                                                double eps,
+                                                                const double* __restrict__ fro2,
+                                                                double floor_rel2) {
+    extern __shared__ __align__(16) unsigned char fbj_lds[];
+    double2* xs = reinterpret_cast<double2*>(fbj_lds);              // [kFbjCols][kFbjLd]
+    double2* G = xs + (size_t)kFbjCols * kFbjLd;                    // [kFbjCols][kFbjGld]
+    double2* Z = G + (size_t)kFbjCols * kFbjGld;                    // [kFbjCols][kFbjGld]
+    __shared__ double rot[kFbjCols / 2][4];
+    __shared__ int rab[kFbjCols / 2][2];
+    __shared__ int s_rot;
+    const int2 tk = tasks[blockIdx.x];
+    const BjMat mt = mats[tk.x];
+    if (round >= mt.N - 1 || flags_prev[tk.x] == 0) return;
+    int bp, bq;
+    round_robin(mt.N, round, tk.y, bp, bq);
+    const int n = mt.n, w = mt.w;
+    const int cp = bp < mt.nb ? min(w, n - bp * w) : 0;
+    const int cq = bq < mt.nb ? min(w, n - bq * w) : 0;
+    const int c = cp + cq;
+    if (c < 2) return;
+    auto gcol = [&](int j) { return j < cp ? bp * w + j : bq * w + (j - cp); };
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // ---- Gram: lane (ta, tb) owns G[4ta .. 4ta+3][4tb .. 4tb+3], wave w the rows w, w+4, ... ----
+    const int ta = lane >> 3, tb = lane & 7;
+    double2 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = double2{0.0, 0.0};
+    const double2* x = X + mt.x_off;
+    for (int r0 = 0; r0 < n; r0 += kFbjRows) {
+        const int rows = min(kFbjRows, n - r0);
+        __syncthreads();
+        for (int e = tid; e < kFbjCols * kFbjRows; e += kFbjThreads) {
+            const int j = e / kFbjRows, i = e % kFbjRows;
+            double2 v = double2{0.0, 0.0};
+            if (j < c && i < rows) v = x[(int64_t)gcol(j) * n + r0 + i];
+            xs[j * kFbjLd + i] = v;
+        }
+        __syncthreads();
+        for (int i = wave; i < rows; i += kFbjThreads / 64) {
+            double2 av[4], bv[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                av[k] = xs[(4 * ta + k) * kFbjLd + i];
+                bv[k] = xs[(4 * tb + k) * kFbjLd + i];
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    acc[a][b].x += av[a].x * bv[b].x + av[a].y * bv[b].y;
+                    acc[a][b].y += av[a].x * bv[b].y - av[a].y * bv[b].x;
+                }
+        }
+    }
+    __syncthreads();
+    // partial sums of waves 0, 1 into the chunk buffer, of waves 2, 3 into G and Z; summed in order
+    {
+        double2* P = wave == 0 ? xs : wave == 1 ? xs + (size_t)kFbjCols * kFbjGld : wave == 2 ? G : Z;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) P[(4 * ta + a) * kFbjGld + 4 * tb + b] = acc[a][b];
+    }
+    __syncthreads();
+    for (int e = tid; e < kFbjCols * kFbjCols; e += kFbjThreads) {
+        const int a = e / kFbjCols, b = e % kFbjCols, o = a * kFbjGld + b;
+        const double2 p0 = xs[o], p1 = xs[(size_t)kFbjCols * kFbjGld + o], p2 = G[o], p3 = Z[o];
+        G[o] = double2{(p0.x + p1.x) + (p2.x + p3.x), (p0.y + p1.y) + (p2.y + p3.y)};
+    }
+    __syncthreads();
+    for (int e = tid; e < kFbjCols * kFbjCols; e += kFbjThreads) {
+        const int a = e / kFbjCols, b = e % kFbjCols;
+        Z[a * kFbjGld + b] = double2{a == b ? 1.0 : 0.0, 0.0};
+    }
+    if (tid == 0) s_rot = 0;
+    __syncthreads();
+    // ---- the pair's sweep of rotations on G (round 0 of a sweep: every pair of the c columns,
+    // so intra-block pairs are visited once per sweep; later rounds: the cross pairs) ----
+    const double tol2 = eps * eps * (double)n;
+    const double floor2 = floor_rel2 * fro2[tk.x];
+    const bool cross_only = round > 0 && cp > 0 && cq > 0;
+    const int big = cp >= cq ? cp : cq, small = cp >= cq ? cq : cp;
+    const int big0 = cp >= cq ? 0 : cp, small0 = cp >= cq ? cp : 0;
+    const int C = c + (c & 1);
+    const int np = cross_only ? small : C / 2;
+    const int nsteps = cross_only ? big : C - 1;
+    for (int r = 0; r < nsteps; ++r) {
+        if (tid < np) {
+            int a, b;
+            bool valid = true;
+            if (cross_only) {
+                const int i1 = small0 + tid, i2 = big0 + (tid + r) % big;
+                a = min(i1, i2);
+                b = max(i1, i2);
+            } else {
+                round_robin(C, r, tid, a, b);
+                valid = b < c;
+            }
+            int ra = -1;
+            if (valid) {
+                const double al = G[a * kFbjGld + a].x, be = G[b * kFbjGld + b].x;
+                const double gr = G[a * kFbjGld + b].x, gi = G[a * kFbjGld + b].y;
+                const double ag2 = gr * gr + gi * gi;
+                if (al + be >= floor2 && ag2 > tol2 * (al * be) && ag2 != 0.0) {
+                    const double ag = sqrt(ag2);
+                    const double inv = 1.0 / ag;
+                    const double zeta = 0.5 * (be - al) * inv;
+                    const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    const double cs = 1.0 / sqrt(1.0 + t * t);
+                    rot[tid][0] = cs;
+                    rot[tid][1] = cs * t;
+                    rot[tid][2] = gr * inv;
+                    rot[tid][3] = gi * inv;
+                    ra = a;
+                    s_rot = 1;
+                }
+            }
+            rab[tid][0] = ra;
+            rab[tid][1] = b;
+        }
+        __syncthreads();
+        // columns: M[:, a] <- c M[:, a] - s conj(e) M[:, b];  M[:, b] <- s e M[:, a] + c M[:, b]
+        for (int e = tid; e < np * c; e += kFbjThreads) {
+            const int g = e / c, k = e % c;
+            const int a = rab[g][0];
+            if (a < 0) continue;
+            const int b = rab[g][1];
+            const double cs = rot[g][0], sn = rot[g][1], er = rot[g][2], ei = rot[g][3];
+            double2 p = G[k * kFbjGld + a], q = G[k * kFbjGld + b];
+            G[k * kFbjGld + a] = double2{cs * p.x - sn * (er * q.x + ei * q.y), cs * p.y - sn * (er * q.y - ei * q.x)};
+            G[k * kFbjGld + b] = double2{sn * (er * p.x - ei * p.y) + cs * q.x, sn * (er * p.y + ei * p.x) + cs * q.y};
+            p = Z[k * kFbjGld + a];
+            q = Z[k * kFbjGld + b];
+            Z[k * kFbjGld + a] = double2{cs * p.x - sn * (er * q.x + ei * q.y), cs * p.y - sn * (er * q.y - ei * q.x)};
+            Z[k * kFbjGld + b] = double2{sn * (er * p.x - ei * p.y) + cs * q.x, sn * (er * p.y + ei * p.x) + cs * q.y};
+        }
+        __syncthreads();
+        // rows: M[a, :] <- c M[a, :] - s e M[b, :];  M[b, :] <- s conj(e) M[a, :] + c M[b, :]
+        for (int e = tid; e < np * c; e += kFbjThreads) {
+            const int g = e / c, k = e % c;
+            const int a = rab[g][0];
+            if (a < 0) continue;
+            const int b = rab[g][1];
+            const double cs = rot[g][0], sn = rot[g][1], er = rot[g][2], ei = rot[g][3];
+            const double2 p = G[a * kFbjGld + k], q = G[b * kFbjGld + k];
+            G[a * kFbjGld + k] = double2{cs * p.x - sn * (er * q.x - ei * q.y), cs * p.y - sn * (er * q.y + ei * q.x)};
+            G[b * kFbjGld + k] = double2{sn * (er * p.x + ei * p.y) + cs * q.x, sn * (er * p.y - ei * p.x) + cs * q.y};
+        }
+        __syncthreads();
+    }
+    if (!s_rot) return;                       // uniform: read after the last barrier
+    // ---- M[:, cols] <- M[:, cols] Z for M = X, then V: lane = row of the chunk, wave w the output
+    // columns 8w .. 8w+7 ----
+    for (int pass = 0; pass < 2; ++pass) {
+        double2* m = pass == 0 ? X + mt.x_off : V + mt.v_off;
+        for (int r0 = 0; r0 < n; r0 += kFbjRows) {
+            const int rows = min(kFbjRows, n - r0);
+            for (int e = tid; e < kFbjCols * kFbjRows; e += kFbjThreads) {
+                const int j = e / kFbjRows, i = e % kFbjRows;
+                if (j < c && i < rows) xs[j * kFbjLd + i] = m[(int64_t)gcol(j) * n + r0 + i];
+            }
+            __syncthreads();
+            if (lane < rows) {
+                double2 out[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) out[q] = double2{0.0, 0.0};
+                for (int k = 0; k < c; ++k) {
+                    const double2 p = xs[k * kFbjLd + lane];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const double2 z = Z[k * kFbjGld + 8 * wave + q];
+                        out[q].x += p.x * z.x - p.y * z.y;
+                        out[q].y += p.x * z.y + p.y * z.x;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int j = 8 * wave + q;
+                    if (j < c) m[(int64_t)gcol(j) * n + r0 + lane] = out[q];
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0) flags_cur[tk.x] = 1;
+}
+
 // V = I per matrix; flags row 0 = 1 (every matrix runs its first sweep), later rows = 0.
 template <typename C2>
 __global__ void bj_setup_kernel(const BjMat* __restrict__ mats, int nmat, C2* __restrict__ V,
@@ -1627,6 +1848,24 @@ __global__ void bj_setup_kernel(const BjMat* __restrict__ mats, int nmat, C2* __
     }
     if (blockIdx.x == 0 && blockIdx.y == 0)
         for (int e = threadIdx.x; e < nmat * nrows; e += blockDim.x) flags[e] = e < nmat ? 1 : 0;
+}
+
+// Squared Frobenius norm of every n x n matrix X_i (one workgroup per matrix, a fixed summation
+// order, so repeated runs agree bit for bit).
+template <typename R>
+__global__ __launch_bounds__(256) void bj_norm_kernel(const BjMat* __restrict__ mats,
+                                                      const typename Cplx<R>::type* __restrict__ X,
+                                                      R* __restrict__ fro2) {
+    __shared__ R part[256 / 64];
+    const BjMat mt = mats[blockIdx.x];
+    const typename Cplx<R>::type* x = X + mt.x_off;
+    const int64_t nn = (int64_t)mt.n * mt.n;
+    R acc = 0;
+    for (int64_t e = threadIdx.x; e < nn; e += 256) acc += x[e].x * x[e].x + x[e].y * x[e].y;
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) fro2[blockIdx.x] = (part[0] + part[1]) + (part[2] + part[3]);
 }
 
 // Singular values, order, U / S / Vh of converged Jacobi factors (one workgroup per matrix).
@@ -1837,7 +2076,10 @@ void block_jacobi_t(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& lis
         }
     }
     const int nrows = max_sweeps + 1;
-    int* flags = static_cast<int*>(scratch(ctx, st, sizeof(int) * (size_t)nmat * nrows, 3));
+    const size_t flag_bytes = (sizeof(int) * (size_t)nmat * nrows + 255) & ~size_t(255);
+    char* fbase = static_cast<char*>(scratch(ctx, st, flag_bytes + sizeof(R) * (size_t)nmat, 3));
+    int* flags = reinterpret_cast<int*>(fbase);
+    R* fro2 = reinterpret_cast<R*>(fbase + flag_bytes);
     std::vector<unsigned char> blob(sizeof(BjMat) * nmat + sizeof(int2) * tasks.size());
     std::memcpy(blob.data(), mats.data(), sizeof(BjMat) * nmat);
     if (!tasks.empty()) std::memcpy(blob.data() + sizeof(BjMat) * nmat, tasks.data(), sizeof(int2) * tasks.size());
@@ -1846,6 +2088,12 @@ void block_jacobi_t(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& lis
     const int2* dt = reinterpret_cast<const int2*>(dblob + sizeof(BjMat) * nmat);
     hipLaunchKernelGGL(bj_setup_kernel<C2>, dim3(nmat, 16), dim3(256), 0, st, dm, nmat, V, flags, nrows);
     HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(bj_norm_kernel<R>, dim3(nmat), dim3(256), 0, st, dm, X, fro2);
+    HIP_CHECK(hipGetLastError());
+    // FP64 sweeps skip pairs below 1e-32 ||X||_F^2 (singular values under 1e-16 ||X||_F, below the
+    // 1e-14 sigma_max floor of every selection for blocks up to 4096 wide); the FP32 sweeps, which
+    // only precondition, rotate everything
+    const R floor_rel2 = sizeof(R) == sizeof(double) ? R(ctx->svd_floor * ctx->svd_floor) : R(0);
     int used = 0;
     std::vector<int> host(nmat);
     if (!tasks.empty()) {
@@ -1863,7 +2111,7 @@ void block_jacobi_t(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& lis
                 hipLaunchKernelGGL(bj_round_kernel<R>, dim3((unsigned)tasks.size()), dim3(kBjThreads), lds,
                                    st, dm, dt, X, V, r, flags + (size_t)s * nmat,
                                    flags + (size_t)(s + 1) * nmat, eps,
-                                   std::max(1, ctx->bj_inner));
+                                   std::max(1, ctx->bj_inner), fro2, floor_rel2);
             HIP_CHECK(hipGetLastError());
             ++used;
             HIP_CHECK(hipMemcpyAsync(hp + (size_t)s * nmat, flags + (size_t)(s + 1) * nmat,
@@ -1896,6 +2144,84 @@ void block_jacobi_t(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& lis
 void block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list, double2* X,
                   double2* V, int max_sweeps, int* sweeps_dev) {
     block_jacobi_t<double>(ctx, st, list, X, V, max_sweeps, sweeps_dev, 2.220446049250313e-16);
+}
+
+// Fused Gram block Jacobi (fbj_round_kernel per round) of the n x n matrices X_i at the running
+// offsets sum_{k<i} n_k^2 (leading dimension n), V_i at list[i].v_off (set to I here); blocks of
+// 16 columns whatever n; sweeps and the overlapped convergence read as in block_jacobi_t.
+void fused_block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list, double2* X,
+                        double2* V, int max_sweeps, int* sweeps_dev) {
+    const int nmat = (int)list.size();
+    std::vector<BjMat> mats(nmat);
+    std::vector<int2> tasks;
+    int64_t xo = 0;
+    int rounds = 0;
+    for (int i = 0; i < nmat; ++i) {
+        const int n = list[i].n;
+        const int w = std::max(1, std::min(kFbjCols / 2, (n + 1) / 2));
+        const int nb = (n + w - 1) / w;
+        const int N = nb + (nb & 1);
+        mats[i] = BjMat{xo, list[i].v_off, n, w, nb, N};
+        xo += (int64_t)n * n;
+        if (n >= 2) {
+            for (int p = 0; p < N / 2; ++p) tasks.push_back(make_int2(i, p));
+            rounds = std::max(rounds, N - 1);
+        }
+    }
+    const int nrows = max_sweeps + 1;
+    const size_t flag_bytes = (sizeof(int) * (size_t)nmat * nrows + 255) & ~size_t(255);
+    char* fbase = static_cast<char*>(scratch(ctx, st, flag_bytes + sizeof(double) * (size_t)nmat, 3));
+    int* flags = reinterpret_cast<int*>(fbase);
+    double* fro2 = reinterpret_cast<double*>(fbase + flag_bytes);
+    std::vector<unsigned char> blob(sizeof(BjMat) * nmat + sizeof(int2) * tasks.size());
+    std::memcpy(blob.data(), mats.data(), sizeof(BjMat) * nmat);
+    if (!tasks.empty())
+        std::memcpy(blob.data() + sizeof(BjMat) * nmat, tasks.data(), sizeof(int2) * tasks.size());
+    unsigned char* dblob = upload_list(ctx, st, blob.data(), blob.size(), 2);
+    const BjMat* dm = reinterpret_cast<const BjMat*>(dblob);
+    const int2* dt = reinterpret_cast<const int2*>(dblob + sizeof(BjMat) * nmat);
+    hipLaunchKernelGGL(bj_setup_kernel<double2>, dim3(nmat, 16), dim3(256), 0, st, dm, nmat, V, flags, nrows);
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(bj_norm_kernel<double>, dim3(nmat), dim3(256), 0, st, dm, X, fro2);
+    HIP_CHECK(hipGetLastError());
+    int used = 0;
+    if (!tasks.empty()) {
+        HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(fbj_round_kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFbjLds));
+        int* hp = reinterpret_cast<int*>(pinned(ctx, sizeof(int) * (size_t)nmat * nrows));
+        for (hipEvent_t& e : ctx->sync_ev)
+            if (!e) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        for (int s = 0; s < max_sweeps; ++s) {
+            for (int r = 0; r < rounds; ++r)
+                hipLaunchKernelGGL(fbj_round_kernel, dim3((unsigned)tasks.size()), dim3(kFbjThreads),
+                                   kFbjLds, st, dm, dt, X, V, r, flags + (size_t)s * nmat,
+                                   flags + (size_t)(s + 1) * nmat, 2.220446049250313e-16, fro2,
+                                   ctx->svd_floor * ctx->svd_floor);
+            HIP_CHECK(hipGetLastError());
+            ++used;
+            HIP_CHECK(hipMemcpyAsync(hp + (size_t)s * nmat, flags + (size_t)(s + 1) * nmat,
+                                     sizeof(int) * nmat, hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipEventRecord(ctx->sync_ev[s & 1], st));
+            if (s == 0) continue;
+            HIP_CHECK(hipEventSynchronize(ctx->sync_ev[(s - 1) & 1]));
+            bool any = false;
+            for (int i = 0; i < nmat; ++i) any |= hp[(size_t)(s - 1) * nmat + i] != 0;
+            if (!any) break;
+        }
+        HIP_CHECK(hipEventSynchronize(ctx->sync_ev[(used - 1) & 1]));
+    }
+    if (sweeps_dev) {
+        std::vector<int> all((size_t)nmat * nrows), host(nmat);
+        HIP_CHECK(hipMemcpyAsync(all.data(), flags, sizeof(int) * all.size(), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        for (int i = 0; i < nmat; ++i) {
+            int k = 0;
+            for (int s = 1; s <= used; ++s) k += all[(size_t)s * nmat + i] != 0;
+            host[i] = std::min(k + 1, used);
+        }
+        HIP_CHECK(hipMemcpyAsync(sweeps_dev, host.data(), sizeof(int) * nmat, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+    }
 }
 
 // ---- FP32-preconditioned block Jacobi --------------------------------------------------------
@@ -2432,6 +2758,8 @@ int bug_block_svd(bug_ctx* ctx, int32_t nmat, const bug_svd_job* jobs, void* g, 
     return guarded(ctx, [&] {
         if (nmat < 0 || (nmat > 0 && (!jobs || !g || !u || !s || !vh)))
             throw Error(BUG_EINVAL, "bad svd jobs");
+        if (ctx->svd_sweeps_dev && nmat > 65536)
+            throw Error(BUG_EINVAL, "sweep recording holds 65536 matrices; the batch has more");
         if (nmat == 0) return;
         std::vector<SvdJob> list(nmat);
         int64_t vtot = 0;
@@ -2511,16 +2839,20 @@ int bug_block_svd(bug_ctx* ctx, int32_t nmat, const bug_svd_job* jobs, void* g, 
                                nullptr, 0);
             HIP_CHECK(hipGetLastError());
             const size_t lds = (size_t)nmax * (sizeof(double) + sizeof(int));
-            const bool gram = ctx->svd_variant == 4 ||
-                              (ctx->svd_variant == 0 && nmax >= ctx->gbj_min_n);
-            if (!gram && (ctx->svd_variant == 2 || !block_jacobi_fits(ctx, nmax))) {
+            const bool fused = ctx->svd_variant == 6 ||
+                               (ctx->svd_variant == 0 && nmax >= ctx->fbj_min_n);
+            const bool gram = !fused && (ctx->svd_variant == 4 ||
+                                         (ctx->svd_variant == 0 && nmax >= ctx->gbj_min_n));
+            if (!fused && !gram && (ctx->svd_variant == 2 || !block_jacobi_fits(ctx, nmax))) {
                 SvdJob* dj = upload_list(ctx, st, jj.data(), jj.size(), 2);
                 hipLaunchKernelGGL(svd_jacobi_kernel, dim3(nmat), dim3(kSvdThreads), lds, st, dj,
                                    Xw, v, Uxw, s, Vxw, sweeps, sw);
             } else {
-                const bool mixed = !gram && (ctx->svd_variant == 5 ||
+                const bool mixed = !fused && !gram && (ctx->svd_variant == 5 ||
                                              (ctx->svd_variant == 0 && nmax >= ctx->mixed_min_n));
-                if (gram)
+                if (fused)
+                    fused_block_jacobi(ctx, st, list, Xw, v, sweeps, sw);
+                else if (gram)
                     gram_block_jacobi(ctx, st, list, Xw, v, sweeps, sw);
                 else if (mixed)
                     mixed_block_jacobi(ctx, st, list, Xw, v, sweeps, sw);
@@ -2575,7 +2907,7 @@ extern "C" {
 
 int bug_set_svd_variant(bug_ctx* ctx, int32_t variant, int32_t record_sweeps) {
     return guarded(ctx, [&] {
-        if (variant < 0 || variant > 5) throw Error(BUG_EINVAL, "svd variant must be 0 .. 5");
+        if (variant < 0 || variant > 6) throw Error(BUG_EINVAL, "svd variant must be 0 .. 6");
         ctx->svd_variant = variant;
         if (record_sweeps && !ctx->svd_sweeps_dev) {
             HIP_CHECK(hipMalloc(&ctx->svd_sweeps_dev, sizeof(int) * 65536));
@@ -2585,6 +2917,13 @@ int bug_set_svd_variant(bug_ctx* ctx, int32_t variant, int32_t record_sweeps) {
             HIP_CHECK(hipFree(ctx->svd_sweeps_dev));
             ctx->svd_sweeps_dev = nullptr;
         }
+    });
+}
+
+int bug_set_svd_floor(bug_ctx* ctx, double rel) {
+    return guarded(ctx, [&] {
+        if (!(rel >= 0.0 && rel < 1e-6)) throw Error(BUG_EINVAL, "svd floor must lie in [0, 1e-6)");
+        ctx->svd_floor = rel;
     });
 }
 

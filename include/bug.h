@@ -302,6 +302,12 @@ int bug_set_gram_jacobi_params(bug_ctx* ctx, int32_t cols, int32_t inner_sweeps,
  * FP64 sweeps; variant 0 takes it for batches whose widest block has min_n or more columns. */
 int bug_set_mixed_jacobi_params(bug_ctx* ctx, int32_t fp32_sweeps, int32_t min_n);
 int bug_svd_sweeps(bug_ctx* ctx, int32_t n, int32_t* out);
+/* The Jacobi noise floor (no counterpart in the reference, whose gesdd has no such knob): pairs
+ * of columns holding together less than rel^2 ||X||_F^2 are not rotated, so singular values under
+ * about rel ||X||_F are inexact (their sum of squares stays exact) and their vectors need not be
+ * orthonormal.  Default 1e-16, below the 1e-14 sigma_max floor of every selection here; rel in
+ * [0, 1e-6). */
+int bug_set_svd_floor(bug_ctx* ctx, double rel);
 
 #ifdef __cplusplus
 }

@@ -311,10 +311,13 @@ def frontier_sweep(fr: KLFrames, psi, mpo, core: ob.Fam, tau, alloc, opts, wcap,
     Fm = frontier_mps(psi, fr, core)
     FmR = Fm.copy()
     amp = nF / max(nrm, eps)
+    # under `parallel` the right half draws from a generator split off the step's
+    # (frontier_closure.jl:125); run one after the other here, they share nothing else
+    rngR = rng.split() if opts.parallel else rng
     Wn, lenv, leak_l, add_l, lg_l = frontier_left(Fm, mpo, fr, tau, alloc, opts, enrich, wcap,
                                                   amp, rng, recurrence)
     Zn, renv, leak_r, add_r, lg_r = frontier_right(FmR, mpo, fr, tau, alloc, opts, enrich, wcap,
-                                                   amp, rng, recurrence)
+                                                   amp, rngR, recurrence)
     from .bug import Ledger
     lg = Ledger()
     _merge(lg, lg_l)

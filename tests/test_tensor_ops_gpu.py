@@ -86,7 +86,7 @@ def test_projections_direct_sums_and_vectors(gpu):
     assert len(many) == 70 and all(abs(v - many[0]) < 1e-12 * abs(many[0]) for v in many)
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("m,n", [(2500, 6), (700, 40), (300, 17), (5, 1), (64, 64), (610, 455)])
 def test_svd_kernels_on_tall_blocks(gpu, m, n, variant):
     """Every SVD path (plain one-sided Jacobi; QR-preconditioned Jacobi on R^H in one workgroup, as
@@ -121,7 +121,7 @@ def test_svd_kernels_on_tall_blocks(gpu, m, n, variant):
     ctx.check(ctx._lib.bug_set_svd_variant(ctx.handle, 0, 0))
 
 
-@pytest.mark.parametrize("variant", [2, 3, 4, 5])
+@pytest.mark.parametrize("variant", [2, 3, 4, 5, 6])
 def test_svd_graded_spectrum_batch(gpu, variant):
     """A batch of blocks of different widths with geometrically graded spectra (1 .. 1e-12): the
     singular values are accurate to 1e-13 sigma_max and U S Vh reconstructs each block."""
