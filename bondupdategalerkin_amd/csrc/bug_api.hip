@@ -282,6 +282,9 @@ void upload_plan(bug_ctx* ctx, Plan& plan, int config, hipStream_t stream) {
         plan.off_tasks = align(bt);
         plan.off_terms = align(bt) + align(bk);
     }
+    if (std::find(ctx->arena_streams.begin(), ctx->arena_streams.end(), stream) ==
+        ctx->arena_streams.end())
+        ctx->arena_streams.push_back(stream);
     if (plan.gen == ctx->plan_arena_gen) return;
     if (capturing_on(ctx, stream))
         throw Error(BUG_EINVAL, "a plan would be uploaded during graph capture: run the operation "
@@ -295,8 +298,10 @@ void upload_plan(bug_ctx* ctx, Plan& plan, int config, hipStream_t stream) {
             ctx->plan_arena = nullptr;
             ctx->plan_arena_bytes = 0;
         } else {
-            HIP_CHECK(hipDeviceSynchronize());
+            // every launch that may read the arena went to one of these streams
+            for (hipStream_t s : ctx->arena_streams) HIP_CHECK(hipStreamSynchronize(s));
         }
+        ctx->arena_streams.assign(1, stream);
         if (total > ctx->plan_arena_bytes) {
             if (ctx->plan_arena) HIP_CHECK(hipFree(ctx->plan_arena));
             ctx->plan_arena = nullptr;
@@ -314,7 +319,7 @@ void upload_plan(bug_ctx* ctx, Plan& plan, int config, hipStream_t stream) {
     }
     char* dev = static_cast<char*>(ctx->plan_arena) + ctx->plan_arena_used;
     ctx->plan_arena_used += align(total);
-    HIP_CHECK(hipMemcpyAsync(dev, plan.image.data(), total, hipMemcpyHostToDevice, stream));
+    stage_upload(ctx, stream, dev, plan.image.data(), total);
     plan.d_tiles = reinterpret_cast<const GemmTile*>(dev);
     plan.d_tasks = reinterpret_cast<const GemmTask*>(dev + plan.off_tasks);
     plan.d_terms = reinterpret_cast<const GemmTerm*>(dev + plan.off_terms);
@@ -409,6 +414,38 @@ void launch_cached_plan(bug_ctx* ctx, uint64_t key, const std::function<void(Pla
                         int precision, const void* A, const void* B, void* C, hipStream_t stream) {
     Plan* plan = cached_plan(ctx, key, build);
     launch_plan(ctx, *plan, precision, A, B, C, stream);
+}
+
+void stage_upload(bug_ctx* ctx, hipStream_t stream, void* dev, const void* host, size_t bytes) {
+    if (bytes == 0) return;
+    if (capturing_on(ctx, stream))
+        throw Error(BUG_EINVAL, "a host-to-device upload would be recorded during graph capture: "
+                                "run the operation once on the capture stream before capturing");
+    constexpr size_t kRing = (size_t)32 << 20;
+    if (bytes > kRing / 4) {                 // rare large tables: a plain (pageable) copy
+        HIP_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, stream));
+        return;
+    }
+    if (!ctx->ring) {
+        void* p = nullptr;
+        HIP_CHECK(hipHostMalloc(&p, kRing));
+        ctx->ring = static_cast<char*>(p);
+        ctx->ring_bytes = kRing;
+        ctx->ring_head = 0;
+    }
+    const size_t need = (bytes + 255) & ~size_t(255);
+    if (ctx->ring_head + need > ctx->ring_bytes) {
+        // the ring's start is about to be overwritten: the copies queued out of it must be done
+        for (hipStream_t s : ctx->ring_streams) HIP_CHECK(hipStreamSynchronize(s));
+        ctx->ring_streams.clear();
+        ctx->ring_head = 0;
+    }
+    if (std::find(ctx->ring_streams.begin(), ctx->ring_streams.end(), stream) == ctx->ring_streams.end())
+        ctx->ring_streams.push_back(stream);
+    char* src = ctx->ring + ctx->ring_head;
+    std::memcpy(src, host, bytes);
+    ctx->ring_head += need;
+    HIP_CHECK(hipMemcpyAsync(dev, src, bytes, hipMemcpyHostToDevice, stream));
 }
 
 void* scratch(bug_ctx* ctx, hipStream_t stream, size_t bytes, int slot) {
@@ -572,6 +609,7 @@ int bug_ctx_destroy(bug_ctx* ctx) {
         for (auto& s : slot)
             if (s.second.ptr) (void)hipFree(s.second.ptr);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->ring) (void)hipHostFree(ctx->ring);
     if (ctx->svd_sweeps_dev) (void)hipFree(ctx->svd_sweeps_dev);
     for (hipEvent_t e : ctx->tune_ev)
         if (e) (void)hipEventDestroy(e);

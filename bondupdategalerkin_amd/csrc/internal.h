@@ -173,6 +173,9 @@ struct Workspace {
 
 // Grow-only per-stream device scratch of the context (slot selects one of several buffers).
 void* scratch(bug_ctx* ctx, hipStream_t stream, size_t bytes, int slot = 0);
+// Copy `bytes` of host memory to device memory `dev` in the order of `stream` without making the
+// host wait for the stream: the bytes go through the context's pinned staging ring first.
+void stage_upload(bug_ctx* ctx, hipStream_t stream, void* dev, const void* host, size_t bytes);
 
 }  // namespace bug
 
@@ -190,10 +193,18 @@ struct bug_ctx {
     size_t plan_capacity = 4096;
     int64_t plan_hits = 0, plan_misses = 0;
     // device arena of plan images: bump-allocated, copied in stream order without blocking the
-    // host; when full the device is synchronized, the arena reset and its generation advanced
+    // host; when full, the streams that launched from it since its last reset are synchronized
+    // (not the device: other contexts' streams keep running), the arena reset and its
+    // generation advanced
     void* plan_arena = nullptr;
     size_t plan_arena_bytes = 0, plan_arena_used = 0;
     uint64_t plan_arena_gen = 1;
+    std::vector<hipStream_t> arena_streams;
+    // pinned host staging ring of stage_upload; wrapping it first synchronizes the streams that
+    // copied out of it since the previous wrap
+    char* ring = nullptr;
+    size_t ring_bytes = 0, ring_head = 0;
+    std::vector<hipStream_t> ring_streams;
     double last_flops = 0.0;     // algorithmic flops of the last entry-point call
     std::vector<bug::Schedule> last_schedules;   // grouped-GEMM launches of that call
     // per-stream scratch for intermediates of the two-stage contractions and the solvers,

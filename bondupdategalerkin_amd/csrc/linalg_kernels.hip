@@ -1909,7 +1909,7 @@ double* pinned(bug_ctx* ctx, size_t bytes) {
 template <typename T>
 T* upload_list(bug_ctx* ctx, hipStream_t st, const T* host, size_t count, int slot) {
     void* dev = scratch(ctx, st, sizeof(T) * std::max<size_t>(count, 1), slot);
-    HIP_CHECK(hipMemcpyAsync(dev, host, sizeof(T) * count, hipMemcpyHostToDevice, st));
+    stage_upload(ctx, st, dev, host, sizeof(T) * count);
     return static_cast<T*>(dev);
 }
 

@@ -130,12 +130,15 @@ def column_weights(leg: Leg, tail: float) -> Dict[int, np.ndarray]:
     a leg of total dimension D sits at rank (j + 1/2) D / d_q of the whole leg and gets the weight
     tail ** (rank / D).  Scaling the right-bond columns of a random site tensor by these weights
     gives the state a Schmidt spectrum that decays about as fast (the columns with small weight
-    carry little of the state), instead of the flat spectrum of a plain Gaussian MPS."""
+    carry little of the state), instead of the flat spectrum of a plain Gaussian MPS.  The weights
+    are scaled to a mean square of one (a uniform factor per site, which leaves the normalised
+    state unchanged and keeps products along the chain clear of underflow)."""
     out = {}
     for q, d in zip(leg.charges, leg.dims):
         rank = (np.arange(d) + 0.5) * (leg.total / d)
         out[q] = tail ** (rank / leg.total)
-    return out
+    ms = sum(float(np.sum(w * w)) for w in out.values()) / max(leg.total, 1)
+    return {q: w / np.sqrt(ms) for q, w in out.items()}
 
 
 def random_mps(L: int, dmax: int, seed: int = 0, sigma: float = 4.0,
