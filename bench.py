@@ -65,7 +65,7 @@ def parse():
     p.add_argument("--budget", type=float, default=500.0,
                    help="wall-clock seconds from process start for the whole run; phases after "
                         "the timed steps are cut to fit")
-    p.add_argument("--fp64-steps", type=int, default=2,
+    p.add_argument("--fp64-steps", type=int, default=1,
                    help="FP64 comparison steps after one warm-up (0: skip the FP64 twin)")
     p.add_argument("--workload", choices=["bug_step", "heff_sweep"], default="bug_step")
     p.add_argument("--kernel-reps", type=int, default=20)
@@ -132,7 +132,7 @@ def main():
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
     signal.signal(signal.SIGALRM, _alarm)
-    signal.alarm(max(int(args.budget + 40 - (time.time() - T_START)), 1))
+    signal.alarm(max(int(args.budget + 70 - (time.time() - T_START)), 1))
     import torch
 
     from bondupdategalerkin_amd import _lib
@@ -312,18 +312,18 @@ def main():
     if twin is not None:
         OUT["fp64"] = fp64_phase(args, B, M, mpo, twin, snap, ncmp, ms_per_step)
 
+    if not args.no_cpu:
+        if left() > 2.5 * args.cpu_seconds + 15:
+            OUT["cpu_baseline"] = cpu_baseline(sample_sites, mpo, lenvs, renvs, mid, args)
+        else:
+            OUT["cpu_baseline"] = None
+            OUT.setdefault("cut", []).append("cpu_baseline: no budget left")
+
     if args.serving > 0 and args.workload == "bug_step":
         OUT["serving"] = serving_phase(args)
 
     if args.heston > 0 and args.workload == "bug_step":
         OUT["heston_batch"] = heston_phase(args)
-
-    if not args.no_cpu:
-        if left() > 2.5 * args.cpu_seconds + 30:
-            OUT["cpu_baseline"] = cpu_baseline(sample_sites, mpo, lenvs, renvs, mid, args)
-        else:
-            OUT["cpu_baseline"] = None
-            OUT.setdefault("cut", []).append("cpu_baseline: no budget left")
 
     OUT["autotune"] = {"mode": {"table": "package tuning table for gfx950 "
                                          "(bondupdategalerkin_amd/schedules_gfx950.txt)",
@@ -454,8 +454,8 @@ def serving_phase(args):
            "slots": args.slots}
     forms = ["mixed", "fp64"] if ARGS.precision == "mixed" else ["fp64"]
     for i, form in enumerate(forms):
-        # an equal share of what is left for each form, keeping 60 s for the later phases
-        share = (left() - 60) / (len(forms) - i)
+        # an equal share of what is left for each form, keeping 10 s for the later phases
+        share = (left() - 10) / (len(forms) - i)
         if share < 20:
             srv[form] = {"skipped": "budget"}
             OUT.setdefault("cut", []).append(f"serving {form}: no budget left")

@@ -374,10 +374,12 @@ def test_parallel_half_sweeps_match_oracle(gpu, start, L, delta, rule, steps, or
 
 
 def test_xx_l64_drop_in_run_matches_oracle_and_free_fermions(gpu):
-    """The FP64 drop-in configuration: XX L = 64 domain wall, dt 0.05, order 2, tol_basis 1e-3,
-    maxdim 256, bond_ceiling_factor 1.2.  Every step: identical bonds and verdict, the device
-    state within 1e-10 of the oracle's (direct-sum distance, no densifying), and S^z no further
-    from the free-fermion solution than the oracle's plus 1e-9."""
+    """The FP64 drop-in configuration: XX L = 64 domain wall, dt 0.05, 100 steps (T = 5), order 2,
+    tol_basis 1e-3, maxdim 256, bond_ceiling_factor 1.2.  Every step: identical bonds and verdict,
+    the device state within 1e-10 of the oracle's (direct-sum distance, no densifying), and S^z no
+    further from the free-fermion solution than the oracle's plus 1e-9; at T = 5 the error against
+    free fermions stays below 1e-5 (the oracle's is 4.6e-6; the reference's published worst error
+    at tol_basis 1e-3 and dt 0.1 is 8.4e-6 to 1.1e-5)."""
     from bondupdategalerkin_amd import bug_step as B
     from bondupdategalerkin_amd import mps
     from bondupdategalerkin_amd.mpo import xxz_mpo
@@ -388,7 +390,7 @@ def test_xx_l64_drop_in_run_matches_oracle_and_free_fermions(gpu):
     psi_o = _oracle_state("dw", L)
     mpo_d, mpo_o = xxz_mpo(L, delta=0.0), om.xxz_mpo(L, delta=0.0)
     kw = dict(order=2, tol_basis=1e-3, maxdim=256, bond_ceiling_factor=1.2)
-    for n in range(1, 21):
+    for n in range(1, 101):
         info_d = B.cbe_bug_step(psi_d, mpo_d, -1j * dt, **kw)
         info_o = obug.cbe_bug_step(psi_o, mpo_o, -1j * dt, obug.StepOptions(**kw))
         assert psi_d.bond_dims() == psi_o.bond_dims(), n
@@ -399,7 +401,7 @@ def test_xx_l64_drop_in_run_matches_oracle_and_free_fermions(gpu):
         err_d = np.abs(mps.magnetisation(psi_d) - ff).max()
         err_o = np.abs(obug.sz_profile(psi_o) - ff).max()
         assert err_d <= err_o + 1e-9, n
-    assert err_d < 1e-6
+    assert err_d < 1e-5
 
 
 def test_xx_l20_reference_configuration_on_device(gpu):
