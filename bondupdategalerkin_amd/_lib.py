@@ -96,6 +96,54 @@ class BugSvdJob(ctypes.Structure):
                 ("phase_on_v", ctypes.c_int32)]
 
 
+class BugStepOpts(ctypes.Structure):
+    _fields_ = [("order", ctypes.c_int32), ("maxdim", ctypes.c_int32),
+                ("hermitian", ctypes.c_int32), ("work_ceiling_rule", ctypes.c_int32),
+                ("bond_ceiling_factor", ctypes.c_double), ("root_ceiling_factor", ctypes.c_double),
+                ("tol_basis", ctypes.c_double), ("tol_trunc", ctypes.c_double),
+                ("tol_expv", ctypes.c_double), ("tol_krylov", ctypes.c_double),
+                ("root", ctypes.c_int32), ("reorth", ctypes.c_int32),
+                ("truncate", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("parallel", ctypes.c_int32), ("seed", ctypes.c_uint64),
+                ("fixed_width", ctypes.c_int32), ("tail_factor", ctypes.c_double),
+                ("krylov_mode", ctypes.c_int32), ("mode", ctypes.c_int32),
+                ("exact", ctypes.c_int32), ("fold_omega", ctypes.c_int32),
+                ("krylov_rank_tol", ctypes.c_double)]
+
+
+class BugBondLog(ctypes.Structure):
+    _fields_ = [("bond", ctypes.c_int32), ("side", ctypes.c_int32),
+                ("sample", ctypes.c_double), ("tail", ctypes.c_double),
+                ("krylov_c", ctypes.c_double), ("krylov_d", ctypes.c_double),
+                ("n_probes", ctypes.c_int32), ("n_holdout", ctypes.c_int32),
+                ("n_selected", ctypes.c_int32), ("n_targeted", ctypes.c_int32),
+                ("n_krylov", ctypes.c_int32), ("n_h1", ctypes.c_int32),
+                ("workcap", ctypes.c_int32), ("met", ctypes.c_int32),
+                ("unmet", ctypes.c_int32), ("width", ctypes.c_int32),
+                ("r_old", ctypes.c_int32), ("r_final", ctypes.c_int32),
+                ("discarded", ctypes.c_double)]
+
+
+class BugStepInfo(ctypes.Structure):
+    _fields_ = [("tolerance_met", ctypes.c_int32), ("unmet", ctypes.c_int32),
+                ("max_sample", ctypes.c_double), ("max_tail", ctypes.c_double),
+                ("max_krylov", ctypes.c_double), ("max_krylov_c", ctypes.c_double),
+                ("max_krylov_d", ctypes.c_double), ("discarded", ctypes.c_double),
+                ("worst_discarded", ctypes.c_double), ("expv_err", ctypes.c_double),
+                ("expv_iters", ctypes.c_int32), ("proj_err", ctypes.c_double),
+                ("defect", ctypes.c_double), ("leak", ctypes.c_double * 4),
+                ("n_probes", ctypes.c_int64), ("n_selected", ctypes.c_int64),
+                ("n_krylov", ctypes.c_int64), ("n_h1", ctypes.c_int64),
+                ("n_h1_closure", ctypes.c_int64), ("n_closure_sweeps", ctypes.c_int64),
+                ("workcap_reached", ctypes.c_int32), ("maxdim_reached", ctypes.c_int32),
+                ("r_old", ctypes.c_int32), ("r_aug", ctypes.c_int32), ("r_final", ctypes.c_int32),
+                ("root_link", ctypes.c_int32), ("root_rank", ctypes.c_int32),
+                ("root_applications", ctypes.c_int32), ("seconds", ctypes.c_double * 6),
+                ("expanded", _i32p), ("expanded_cap", ctypes.c_int32),
+                ("log", ctypes.POINTER(BugBondLog)), ("log_cap", ctypes.c_int32),
+                ("log_count", ctypes.c_int32)]
+
+
 # name -> (restype, argtypes); the list is the ABI of include/bug.h
 _P = ctypes.c_void_p
 _BSMP = ctypes.POINTER(BugBsm)
@@ -170,6 +218,18 @@ SIGNATURES = {
     "bug_set_mixed_jacobi_params": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     "bug_svd_sweeps": (ctypes.c_int, [_P, ctypes.c_int32, _i32p]),
     "bug_set_svd_floor": (ctypes.c_int, [_P, ctypes.c_double]),
+    "bug_step_opts_default": (None, [ctypes.POINTER(BugStepOpts)]),
+    "bug_mpo_create": (ctypes.c_int, [ctypes.c_int32, _MPOP, ctypes.POINTER(_P)]),
+    "bug_mpo_destroy": (ctypes.c_int, [_P]),
+    "bug_state_create": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _i32p, _BSMP,
+                                        ctypes.c_int32, _P, ctypes.POINTER(_P)]),
+    "bug_state_destroy": (ctypes.c_int, [_P]),
+    "bug_state_info": (ctypes.c_int, [_P, _i32p, _i32p, _i32p]),
+    "bug_state_site_shape": (ctypes.c_int, [_P, ctypes.c_int32, _i32p, _i32p, _i64p]),
+    "bug_state_download_site": (ctypes.c_int, [_P, _P, ctypes.c_int32, _i32p, _i32p, _i32p, _i32p,
+                                               _i64p, _P]),
+    "bug_step": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_double,
+                                ctypes.POINTER(BugStepOpts), ctypes.POINTER(BugStepInfo)]),
     "bug_block_svd": (
         ctypes.c_int,
         [_P, ctypes.c_int32, ctypes.POINTER(BugSvdJob), _P, ctypes.c_int64, _P, ctypes.c_int64,

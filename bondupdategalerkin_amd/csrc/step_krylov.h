@@ -1,0 +1,56 @@
+// Krylov pieces of the step-level entry point (see step_krylov.hip).
+#pragma once
+
+#include <functional>
+
+#include "step_core.h"
+
+namespace bug {
+namespace host {
+
+constexpr double kBreakdown = 1e-13;      // invariant-subspace threshold of the local chains
+constexpr int kMaxDepth = 64;             // local Krylov depth bound
+constexpr int kRootMaxIter = 30;          // root basis per substep
+constexpr double kSkewTol = 1e-12;        // |M + M^dag| below which exp(M) counts as unitary
+constexpr int kMaxPanels = 4096;          // Simpson panels of one residual integral
+constexpr int kTailNodes = 8;             // nodes of the real-time tail guard
+constexpr double kShiftTrigger = 30.0;    // exponent past which the spectral shift switches on
+constexpr double kMaxNorm = 1e150;
+
+// smallest rank <= kmax whose scaled discarded tail fits; returns the scaled tail
+double tail_rank(const std::vector<SvEntry>& kept, double scale, double allowance, int kmax,
+                 double base, int& rank);
+// |tau| int_0^1 |(exp(sM) e1)[index]| ds (composite Simpson in log space), M row-major n x n
+double residual_integral(int index, const std::vector<cplx>& M, int n, cplx tau, bool relative);
+// weight of the (k+1)-th Krylov vector in exp(tau H) v1, H row-major with leading dimension ld
+double direction_weight(const std::vector<cplx>& H, int ld, int k, cplx tau, bool state_seed);
+
+struct Tridiagonal {
+    int n = 0;
+    std::vector<double> vals, vecs;       // vecs column-major
+    Tridiagonal(const std::vector<double>& alpha, const std::vector<double>& beta);
+    std::vector<cplx> coeffs(cplx tau, cplx& logscale) const;
+};
+
+struct Hessenberg {
+    std::vector<cplx> Hm;                 // row-major n x n
+    int n = 0;
+    std::vector<cplx> vals;
+    Hessenberg(const std::vector<cplx>& H, int n);
+    std::vector<cplx> coeffs(cplx tau, cplx& logscale) const;
+};
+
+struct ExpvReport {
+    int iterations = 0;
+    int substeps = 0;
+    double err = 0.0;
+};
+
+Tensor lanczos_expv(const Exec& ex, const std::function<Tensor(const Tensor&)>& apply, cplx tau,
+                    const Tensor& v0, double tol, int maxiter, bool reorth, double conv_tol,
+                    ExpvReport& rep);
+Tensor arnoldi_expv(const Exec& ex, const std::function<Tensor(const Tensor&)>& apply, cplx tau,
+                    const Tensor& v0, double tol, int maxiter, double conv_tol, ExpvReport& rep);
+
+}  // namespace host
+}  // namespace bug

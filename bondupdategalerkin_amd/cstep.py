@@ -1,0 +1,193 @@
+"""The step-level C entry point ``bug_step`` from Python.
+
+``bug_step`` is the whole ``cbe_bug_step!`` (src/BUG/TimeEvolution/bug_step.jl:17-148) behind the
+C ABI of ``libbug.so``: the state and the MPO are opaque library handles, the options a POD with
+the reference's keywords, the report ``CBEBugSweepInfo`` plus the ledger verdict.  This module
+turns the package's ``SymMPS`` / ``MPO`` into those handles and back; the step itself runs in
+C++ inside the library, on the stream the state was created on (torch's current stream here).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Dict, Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .blocks import BlockTensor, Layout, Leg, current_stream_handle, device
+from .mpo import MPO
+from .mps import SymMPS
+
+CEIL_RULES = {"bond": 0, "maxdim": 1, "bond_strict": 2, "frame_root": 3}
+KRYLOV_MODES = {"auto": 0, "lanczos": 1, "arnoldi": 2}
+UNMET_BITS = [(1, "sample"), (2, "krylov"), (4, "closure"), (8, "projection"), (16, "expv"),
+              (32, "trunc")]
+
+
+class CMPO:
+    """An MPO handle (the library keeps its own copy of the sites)."""
+
+    def __init__(self, mpo: MPO):
+        lib = _lib.lib()
+        descs = (_lib.BugMpoSite * len(mpo))(*[mpo[i].descriptor() for i in range(len(mpo))])
+        h = ctypes.c_void_p()
+        rc = lib.bug_mpo_create(len(mpo), descs, ctypes.byref(h))
+        if rc != _lib.BUG_OK:
+            raise _lib.BugError(rc, "bug_mpo_create refused the MPO")
+        self.handle = h
+        self.L = len(mpo)
+
+    def close(self):
+        if self.handle:
+            _lib.lib().bug_mpo_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CState:
+    """A device-resident state handle, created from a ``SymMPS`` on the current stream."""
+
+    def __init__(self, psi: SymMPS):
+        ctx = _lib.context()
+        L = len(psi)
+        self._keep = []
+        descs = (_lib.BugBsm * L)()
+        for i in range(L):
+            t = psi[i]
+            host = np.ascontiguousarray(t.data.detach().cpu().numpy()[:max(t.layout.nelem, 1)])
+            self._keep.append(host)
+            descs[i] = t.layout.descriptor(host.ctypes.data, t.layout.nelem)
+        charges = _lib.i32(psi.charges)
+        h = ctypes.c_void_p()
+        ctx.check(ctx._lib.bug_state_create(ctx.handle, L, len(psi.charges), _lib.ptr32(charges), descs,
+                                            psi.center, current_stream_handle(), ctypes.byref(h)))
+        self._keep = None
+        self.handle = h
+        self.charges = tuple(psi.charges)
+        self.L = L
+
+    def center(self) -> int:
+        c = ctypes.c_int32()
+        _lib.lib().bug_state_info(self.handle, None, None, ctypes.byref(c))
+        return int(c.value)
+
+    def to_mps(self) -> SymMPS:
+        """Download the state into a ``SymMPS`` (device tensors in the package's layout)."""
+        ctx = _lib.context()
+        lib = ctx._lib
+        tensors = []
+        for i in range(self.L):
+            rn, cn, ne = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+            ctx.check(lib.bug_state_site_shape(self.handle, i, ctypes.byref(rn), ctypes.byref(cn),
+                                               ctypes.byref(ne)))
+            rq, rd = np.zeros(rn.value, np.int32), np.zeros(rn.value, np.int32)
+            cq, cd = np.zeros(cn.value, np.int32), np.zeros(cn.value, np.int32)
+            off = np.zeros(len(self.charges) * rn.value, np.int64)
+            data = np.zeros(max(ne.value, 1), np.complex128)
+            ctx.check(lib.bug_state_download_site(ctx.handle, self.handle, i, _lib.ptr32(rq),
+                                                  _lib.ptr32(rd), _lib.ptr32(cq), _lib.ptr32(cd),
+                                                  _lib.ptr64(off), data.ctypes.data))
+            lay = Layout(Leg(tuple(int(x) for x in rq), tuple(int(x) for x in rd)),
+                         Leg(tuple(int(x) for x in cq), tuple(int(x) for x in cd)), self.charges)
+            if not np.array_equal(lay.offsets.reshape(-1), off):
+                raise RuntimeError("downloaded block offsets differ from the package layout")
+            tensors.append(BlockTensor(lay, torch.from_numpy(data).to(device())))
+        return SymMPS(tensors, self.charges, self.center())
+
+    def close(self):
+        if self.handle:
+            _lib.lib().bug_state_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def step_options(*, order: int = 1, maxdim: Optional[int] = None, hermitian: bool = True,
+                 work_ceiling_rule: str = "bond", bond_ceiling_factor: Optional[float] = None,
+                 root_ceiling_factor: float = 1.2, tol_basis: Optional[float] = None,
+                 tol_trunc: Optional[float] = None, tol_expv: Optional[float] = None,
+                 tol_root: Optional[float] = None, tol_krylov: Optional[float] = None,
+                 root: Optional[int] = None, reorth: bool = True, truncate: bool = True,
+                 precision: str = "fp64", parallel: bool = False, rng: Optional[int] = None,
+                 expansion=None, exact: bool = False, fold_omega: Optional[bool] = None,
+                 krylov_rank_tol: float = 0.0) -> _lib.BugStepOpts:
+    """The options POD from the reference's keywords (the Python step's names)."""
+    if tol_root is not None and tol_expv is not None:
+        raise ValueError("tol_root and tol_expv name the same tolerance; pass one")
+    if work_ceiling_rule not in CEIL_RULES:
+        raise ValueError(f"work_ceiling_rule must be one of {tuple(CEIL_RULES)}")
+    if precision not in ("fp64", "mixed"):
+        raise ValueError("precision must be 'fp64' or 'mixed'")
+    o = _lib.BugStepOpts()
+    _lib.lib().bug_step_opts_default(ctypes.byref(o))
+    o.order = order
+    if maxdim is not None:
+        o.maxdim = maxdim
+    o.hermitian = 1 if hermitian else 0
+    o.work_ceiling_rule = CEIL_RULES[work_ceiling_rule]
+    if bond_ceiling_factor is not None:
+        o.bond_ceiling_factor = bond_ceiling_factor
+    o.root_ceiling_factor = root_ceiling_factor
+    te = tol_root if tol_root is not None else tol_expv
+    for name, v in (("tol_basis", tol_basis), ("tol_trunc", tol_trunc), ("tol_expv", te),
+                    ("tol_krylov", tol_krylov)):
+        if v is not None:
+            if not v >= 0:
+                raise ValueError(f"{name} must be non-negative")
+            setattr(o, name, v)
+    if root is not None:
+        o.root = root
+    o.reorth = 1 if reorth else 0
+    o.truncate = 1 if truncate else 0
+    o.precision = _lib.PREC_FP32 if precision == "mixed" else _lib.PREC_FP64
+    o.parallel = 1 if parallel else 0
+    if rng is not None:
+        o.seed = int(rng)
+    if expansion is not None:
+        o.mode = 0 if expansion.mode == "rank_tail" else 1
+        o.fixed_width = expansion.fixed_width
+        o.tail_factor = expansion.tail_factor
+        o.krylov_mode = KRYLOV_MODES[expansion.krylov_mode]
+    o.exact = 1 if exact else 0
+    o.fold_omega = -1 if fold_omega is None else (1 if fold_omega else 0)
+    o.krylov_rank_tol = krylov_rank_tol
+    return o
+
+
+def bug_step(state: CState, mpo: CMPO, tau: complex, **kw) -> Dict:
+    """One ``cbe_bug_step!`` through the C entry point; returns the report as a dict (the verdict,
+    the ledger maxima and counts, ``unmet`` names, the expanded widths and the per-bond log)."""
+    ctx = _lib.context()
+    opts = step_options(**kw)
+    tau = complex(tau)
+    info = _lib.BugStepInfo()
+    exp = np.zeros(max(state.L - 1, 1), np.int32)
+    logs = (_lib.BugBondLog * (4 * state.L))()
+    info.expanded = _lib.ptr32(exp)
+    info.expanded_cap = state.L - 1
+    info.log = logs
+    info.log_cap = 4 * state.L
+    ctx.check(ctx._lib.bug_step(ctx.handle, state.handle, mpo.handle, tau.real, tau.imag,
+                                ctypes.byref(opts), ctypes.byref(info)))
+    out = {f: getattr(info, f) for f, _ in _lib.BugStepInfo._fields_
+           if f not in ("expanded", "expanded_cap", "log", "log_cap", "leak", "seconds")}
+    out["leak"] = [info.leak[j] for j in range(4)]
+    out["seconds"] = dict(zip(("left_sweep", "right_sweep", "closure", "root", "truncation",
+                               "step"), [info.seconds[j] for j in range(6)]))
+    out["expanded"] = [int(x) for x in exp[:state.L - 1]]
+    out["unmet_names"] = [name for bit, name in UNMET_BITS if info.unmet & bit]
+    out["status"] = "met" if info.tolerance_met else "budget_unmet"
+    out["bond_log"] = [{f: getattr(logs[i], f) for f, _ in _lib.BugBondLog._fields_}
+                       for i in range(info.log_count)]
+    return out

@@ -309,6 +309,128 @@ int bug_svd_sweeps(bug_ctx* ctx, int32_t n, int32_t* out);
  * [0, 1e-6). */
 int bug_set_svd_floor(bug_ctx* ctx, double rel);
 
+/* ---- the whole step ------------------------------------------------------------------------ */
+
+/*
+ * One rank-adaptive CBE-BUG step in place, the step-level drop-in for
+ *   cbe_bug_step!(psi, mpo, tau; order, maxdim, hermitian, work_ceiling_rule, ...) -> CBEBugSweepInfo
+ * (src/BUG/TimeEvolution/bug_step.jl:17-148, the Number overload :512-513).  The state and the MPO
+ * are opaque handles; the state stays resident on the device between steps.  Options are a POD with
+ * one field per keyword (bug_step_opts_default fills the reference's defaults); the report mirrors
+ * CBEBugSweepInfo and the AdaptiveLedger verdict (src/BUG/Core/diagnostics.jl:209-255,
+ * src/BUG/Core/RankAdaptation/basis_budget.jl:144-299).  Invalid options return BUG_EINVAL before
+ * the state is touched (bug_step_stages.jl:41-111); a numerical shortfall is not an error but
+ * tolerance_met = 0 with the failed tests in `unmet`.  Options this entry point does not carry
+ * (the adaptive controller, exact mode, kept states, unfolded leaks, ranked Krylov selection) are
+ * refused with BUG_EINVAL.
+ */
+typedef struct bug_state bug_state;
+typedef struct bug_mpo bug_mpo;
+
+#define BUG_CEIL_BOND 0         /* work_ceiling_rule = :bond */
+#define BUG_CEIL_MAXDIM 1       /* :maxdim */
+#define BUG_CEIL_BOND_STRICT 2  /* :bond_strict */
+#define BUG_CEIL_FRAME_ROOT 3   /* :frame_root */
+
+#define BUG_KRYLOV_AUTO 0       /* Lanczos for hermitian steps, Arnoldi otherwise */
+#define BUG_KRYLOV_LANCZOS 1
+#define BUG_KRYLOV_ARNOLDI 2
+
+/* bits of bug_step_info.unmet: which tests of the step's budget failed (_unmet!, basis_budget.jl) */
+#define BUG_UNMET_SAMPLE 1
+#define BUG_UNMET_KRYLOV 2
+#define BUG_UNMET_CLOSURE 4
+#define BUG_UNMET_PROJECTION 8
+#define BUG_UNMET_EXPV 16
+#define BUG_UNMET_TRUNC 32
+
+typedef struct bug_step_opts {
+    int32_t order;                /* >= 1 */
+    int32_t maxdim;
+    int32_t hermitian;            /* 1: Lanczos root solve; 0: Arnoldi */
+    int32_t work_ceiling_rule;    /* BUG_CEIL_* */
+    double bond_ceiling_factor;   /* < 0: the rule's default (1.2 for :bond, 0 otherwise) */
+    double root_ceiling_factor;   /* :frame_root's root tier */
+    double tol_basis;             /* < 0: 1e-5 */
+    double tol_trunc;             /* < 0: 1e-7 */
+    double tol_expv;              /* < 0: 1e-10 (tol_root) */
+    double tol_krylov;            /* < 0: tol_basis */
+    int32_t root;                 /* 0-based root bond; -1: L/2 - 1 */
+    int32_t reorth;
+    int32_t truncate;
+    int32_t precision;            /* BUG_PREC_FP64 (the drop-in) or BUG_PREC_FP32 (mixed) */
+    int32_t parallel;             /* the two half-sweeps (and frontier halves) concurrently */
+    uint64_t seed;                /* probe stream seed (the reference's rng) */
+    int32_t fixed_width;          /* DDQRExpansion(:rank_tail) probe width */
+    double tail_factor;
+    int32_t krylov_mode;          /* BUG_KRYLOV_* */
+    int32_t mode;                 /* 0 :rank_tail; anything else is refused here */
+    int32_t exact;                /* must be 0 here */
+    int32_t fold_omega;           /* -1 default (folded leaks for order > 1), 1 folded; 0 refused */
+    double krylov_rank_tol;       /* must be 0 here */
+} bug_step_opts;
+
+/* One half-sweep bond visit of the step (_log_bond!, basis_budget.jl:399-436). */
+typedef struct bug_bond_log {
+    int32_t bond;                 /* 0-based */
+    int32_t side;                 /* 0 left half-sweep, 1 right */
+    double sample, tail, krylov_c, krylov_d;
+    int32_t n_probes, n_holdout, n_selected, n_targeted, n_krylov, n_h1;
+    int32_t workcap, met, unmet, width;
+    int32_t r_old, r_final;
+    double discarded;             /* the bond's discarded norm relative to the state's */
+} bug_bond_log;
+
+typedef struct bug_step_info {
+    int32_t tolerance_met;        /* budget_status === :met */
+    int32_t unmet;                /* BUG_UNMET_* bits */
+    double max_sample, max_tail, max_krylov, max_krylov_c, max_krylov_d;
+    double discarded;             /* the compression's cumulative relative discarded norm */
+    double worst_discarded;       /* CBEBugSweepInfo.discarded: the worst single cut */
+    double expv_err;
+    int32_t expv_iters;
+    double proj_err;
+    double defect;
+    double leak[4];               /* closure leaks by degree (NaN: not measured) */
+    int64_t n_probes, n_selected, n_krylov, n_h1, n_h1_closure, n_closure_sweeps;
+    int32_t workcap_reached, maxdim_reached;
+    int32_t r_old, r_aug, r_final;
+    int32_t root_link, root_rank, root_applications;
+    double seconds[6];            /* left sweep, right sweep, closure, root, truncation, step */
+    int32_t* expanded;            /* caller array of expanded_cap >= L - 1 entries, or NULL */
+    int32_t expanded_cap;
+    bug_bond_log* log;            /* caller array of log_cap entries, or NULL */
+    int32_t log_cap;
+    int32_t log_count;            /* entries the step wrote (at most log_cap) */
+} bug_step_info;
+
+/* The reference's defaults (bug_step.jl:17-148, TOL_DEFAULTS basis_budget.jl:568). */
+void bug_step_opts_default(bug_step_opts* opts);
+
+/* MPO handle from L sites (copied; the caller keeps its arrays). */
+int bug_mpo_create(int32_t L, const bug_mpo_site* sites, bug_mpo** out);
+int bug_mpo_destroy(bug_mpo* mpo);
+
+/* State handle from L site tensors A(l, s, r) given as families with HOST data (block offsets in
+ * the bug_bsm convention); each site's slices are the d local states with shifts local_charges.
+ * The step runs on `stream` (NULL: the default stream).  copy(psi) semantics: the handle owns its
+ * device copy. */
+int bug_state_create(bug_ctx* ctx, int32_t L, int32_t d, const int32_t* local_charges,
+                     const bug_bsm* sites, int32_t center, void* stream, bug_state** out);
+int bug_state_destroy(bug_state* state);
+/* L, d and the canonical centre (0-based). */
+int bug_state_info(const bug_state* state, int32_t* L, int32_t* d, int32_t* center);
+/* Sector counts of the site's row and column legs and its element count. */
+int bug_state_site_shape(const bug_state* state, int32_t site, int32_t* row_nsec, int32_t* col_nsec,
+                         int64_t* nelem);
+/* Download one site: leg tables, block offsets [d * row_nsec] (-1 absent) and the complex data
+ * (column-major blocks, Julia's order) into host arrays sized by bug_state_site_shape. */
+int bug_state_download_site(bug_ctx* ctx, const bug_state* state, int32_t site,
+                            int32_t* row_charge, int32_t* row_dim, int32_t* col_charge,
+                            int32_t* col_dim, int64_t* offset, void* host_data);
+int bug_step(bug_ctx* ctx, bug_state* state, const bug_mpo* mpo, double tau_re, double tau_im,
+             const bug_step_opts* opts, bug_step_info* info);
+
 #ifdef __cplusplus
 }
 #endif
