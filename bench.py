@@ -68,6 +68,9 @@ def parse():
     p.add_argument("--fp64-steps", type=int, default=1,
                    help="FP64 comparison steps after one warm-up (0: skip the FP64 twin)")
     p.add_argument("--workload", choices=["bug_step", "heff_sweep"], default="bug_step")
+    p.add_argument("--driver", choices=["c", "python"], default="c",
+                   help="c: the step-level C entry point bug_step (C++ orchestration); python: "
+                        "the package's Python orchestration of the same kernels")
     p.add_argument("--kernel-reps", type=int, default=20)
     p.add_argument("--cpu-seconds", type=float, default=8.0,
                    help="seconds of oracle work per CPU sample (two samples: 1 thread, all)")
@@ -258,24 +261,46 @@ def main():
         T.scale_(1.0 / T.norm(psi[L - 1]), psi[L - 1])
         B.canonical(psi, 0)
         torch.cuda.synchronize()
-        if args.precision == "mixed" and args.fp64_steps > 0:
-            twin = psi.copy()
         done = [0]
         phases_log = []
+        kw = dict(maxdim=D, work_ceiling_rule="maxdim", precision=args.precision,
+                  order=args.order, parallel=args.half_sweeps == "parallel")
+        if args.driver == "c":
+            from bondupdategalerkin_amd import cstep as C
 
-        def step():
-            info = B.cbe_bug_step(psi, mpo, -0.02j, maxdim=D, work_ceiling_rule="maxdim",
-                                  precision=args.precision, order=args.order,
-                                  parallel=args.half_sweeps == "parallel")
-            done[0] += 1
-            statuses.append(B.budget_status(info.ledger))
-            phases_log.append({k: round(v, 3) for k, v in info.seconds.items()})
-            note(f"{args.precision} step {done[0]}: {info.seconds['step']:.2f}s "
-                 f"{statuses[-1]} {info.ledger.unmet} max bond {max(psi.bond_dims())} "
-                 f"({' '.join(f'{k} {v:.2f}' for k, v in info.seconds.items() if k != 'step')})")
-            if twin is not None and done[0] == ncmp:
+            cstate, cmpo = C.CState(psi), C.CMPO(mpo)
+            if args.precision == "mixed" and args.fp64_steps > 0:
+                twin = cstate.clone()
+            OUT["config"]["driver"] = ("bug_step, the step-level C entry point of libbug.so "
+                                       "(the whole step orchestrated in C++)")
+
+            def step():
                 nonlocal snap
-                snap = psi.copy()
+                info = C.bug_step(cstate, cmpo, -0.02j, **kw)
+                done[0] += 1
+                statuses.append(info["status"])
+                phases_log.append({k: round(v, 3) for k, v in info["seconds"].items()})
+                note(f"{args.precision} step {done[0]}: {info['seconds']['step']:.2f}s "
+                     f"{statuses[-1]} {info['unmet_names']} max bond {info['r_final']} "
+                     f"({' '.join(f'{k} {v:.2f}' for k, v in info['seconds'].items() if k != 'step')})")
+                if twin is not None and done[0] == ncmp:
+                    snap = cstate.clone()
+        else:
+            if args.precision == "mixed" and args.fp64_steps > 0:
+                twin = psi.copy()
+            OUT["config"]["driver"] = "cbe_bug_step, the package's Python orchestration"
+
+            def step():
+                nonlocal snap
+                info = B.cbe_bug_step(psi, mpo, -0.02j, **kw)
+                done[0] += 1
+                statuses.append(B.budget_status(info.ledger))
+                phases_log.append({k: round(v, 3) for k, v in info.seconds.items()})
+                note(f"{args.precision} step {done[0]}: {info.seconds['step']:.2f}s "
+                     f"{statuses[-1]} {info.ledger.unmet} max bond {max(psi.bond_dims())} "
+                     f"({' '.join(f'{k} {v:.2f}' for k, v in info.seconds.items() if k != 'step')})")
+                if twin is not None and done[0] == ncmp:
+                    snap = psi.copy()
     else:
         def step():
             lenv = lenvs[0]
@@ -302,7 +327,8 @@ def main():
     OUT["ms_per_step"] = round(ms_per_step, 3)
     if args.workload == "bug_step":
         OUT["config"]["warmup_budget_status"] = warm_statuses
-        OUT["config"]["max_bond_after"] = max(psi.bond_dims())
+        OUT["config"]["max_bond_after"] = max(cstate.bond_dims() if args.driver == "c"
+                                              else psi.bond_dims())
         OUT["phase_seconds_per_step"] = {
             k: round(float(np.mean([p.get(k, 0.0) for p in phases_log[args.warmup:]])), 3)
             for k in phases_log[-1]} if phases_log[args.warmup:] else None
@@ -310,6 +336,8 @@ def main():
 
     # ---- FP64 twin: the drop-in form from the same start, as far as the budget holds ----
     if twin is not None:
+        if args.driver == "c":
+            CMPO_HOLD[0] = cmpo
         OUT["fp64"] = fp64_phase(args, B, M, mpo, twin, snap, ncmp, ms_per_step)
 
     if not args.no_cpu:
@@ -408,11 +436,18 @@ def fp64_phase(args, B, M, mpo, twin, snap, ncmp, ms_mixed):
         OUT.setdefault("cut", []).append("fp64 twin: no budget left")
         return {"skipped": "budget"}
     sts = []
+    kw = dict(maxdim=ARGS.D, work_ceiling_rule="maxdim", precision="fp64", order=ARGS.order,
+              parallel=ARGS.half_sweeps == "parallel")
 
     def step():
-        info = B.cbe_bug_step(twin, mpo, -0.02j, maxdim=ARGS.D, work_ceiling_rule="maxdim",
-                              precision="fp64", order=ARGS.order,
-                              parallel=ARGS.half_sweeps == "parallel")
+        if ARGS.driver == "c":
+            from bondupdategalerkin_amd import cstep as C
+
+            info = C.bug_step(twin, CMPO_HOLD[0], -0.02j, **kw)
+            sts.append(info["status"])
+            note(f"fp64 step: {info['seconds']['step']:.2f}s {sts[-1]} {info['unmet_names']}")
+            return
+        info = B.cbe_bug_step(twin, mpo, -0.02j, **kw)
         sts.append(B.budget_status(info.ledger))
         note(f"fp64 step: {info.seconds['step']:.2f}s {sts[-1]} {info.ledger.unmet}")
 
@@ -431,7 +466,10 @@ def fp64_phase(args, B, M, mpo, twin, snap, ncmp, ms_mixed):
                    speedup_mixed_vs_fp64=round(ms64 / ms_mixed, 3))
     out["budget_status"] = sts
     if snap is not None and 1 + out["steps"] == ncmp:
-        dsz = float(np.abs(M.magnetisation(snap) - M.magnetisation(twin)).max())
+        if ARGS.driver == "c":
+            dsz = float(np.abs(snap.expect_diagonal() - twin.expect_diagonal()).max())
+        else:
+            dsz = float(np.abs(M.magnetisation(snap) - M.magnetisation(twin)).max())
         out["max_abs_dSz_mixed_vs_fp64"] = dsz
         out["compared_after_steps"] = ncmp
     else:
@@ -606,6 +644,7 @@ def cpu_baseline(sites, mpo, lenvs, renvs, mid, args):
 
 
 ARGS = None
+CMPO_HOLD = [None]
 
 if __name__ == "__main__":
     main()

@@ -223,6 +223,10 @@ SIGNATURES = {
     "bug_mpo_destroy": (ctypes.c_int, [_P]),
     "bug_state_create": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _i32p, _BSMP,
                                         ctypes.c_int32, _P, ctypes.POINTER(_P)]),
+    "bug_state_create_device": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _i32p, _BSMP,
+                                               ctypes.c_int32, _P, ctypes.POINTER(_P)]),
+    "bug_state_clone": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "bug_state_expect_diagonal": (ctypes.c_int, [_P, _P, _f64p, _f64p]),
     "bug_state_destroy": (ctypes.c_int, [_P]),
     "bug_state_info": (ctypes.c_int, [_P, _i32p, _i32p, _i32p]),
     "bug_state_site_shape": (ctypes.c_int, [_P, ctypes.c_int32, _i32p, _i32p, _i64p]),

@@ -1557,6 +1557,88 @@ int bug_state_create(bug_ctx* ctx, int32_t L, int32_t d, const int32_t* local_ch
     });
 }
 
+int bug_state_create_device(bug_ctx* ctx, int32_t L, int32_t d, const int32_t* local_charges,
+                            const bug_bsm* sites, int32_t center, void* stream, bug_state** out) {
+    return step_guard(ctx, [&] {
+        if (!out || L < 2 || d < 1 || !local_charges || !sites || center < 0 || center >= L)
+            throw StepError(BUG_EINVAL, "bad state arguments");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        set_thread_stream(st);
+        const Exec ex{ctx, st, BUG_PREC_FP64};
+        std::unique_ptr<bug_state> s(new bug_state());
+        s->stream = st;
+        s->psi.charges.assign(local_charges, local_charges + d);
+        s->psi.center = center;
+        for (int i = 0; i < L; ++i) {
+            const bug_bsm& b = sites[i];
+            if (b.nslice != d) throw StepError(BUG_EINVAL, "site " + std::to_string(i) + " has the wrong number of slices");
+            for (int k = 0; k < d; ++k)
+                if (b.shift[k] != local_charges[k]) throw StepError(BUG_EINVAL, "slice shifts differ from the local charges");
+            const Layout lay(leg_from(b.row), leg_from(b.col), s->psi.charges);
+            if (i > 0 && lay.row != s->psi.t[i - 1].L().col)
+                throw StepError(BUG_EINVAL, "bond " + std::to_string(i) + ": legs of neighbouring sites differ");
+            Tensor t = t_zeros(ex, lay);
+            CopyList cl;
+            for (const Blk& blk : lay.blks) {
+                int idx = -1;
+                for (int r = 0; r < b.row.nsec; ++r)
+                    if (b.row.charge[r] == blk.q) idx = r;
+                const int64_t so = idx < 0 ? -1 : b.offset[(size_t)blk.k * b.row.nsec + idx];
+                if (so < 0) continue;
+                cl.add(so, blk.d, blk.off, blk.d, blk.d, blk.dc);
+            }
+            cl.run(ex, b.data, b.nelem, t.buf->p, t.nelem());
+            s->psi.t.push_back(t);
+        }
+        *out = s.release();
+    });
+}
+
+int bug_state_clone(const bug_state* src, bug_state** out) {
+    if (!src || !out) return BUG_EINVAL;
+    try {
+        bug_state* s = new bug_state();
+        s->psi = src->psi;
+        s->stream = src->stream;
+        *out = s;
+        return BUG_OK;
+    } catch (...) {
+        return BUG_ENOMEM;
+    }
+}
+
+int bug_state_expect_diagonal(bug_ctx* ctx, const bug_state* state, const double* diag, double* out) {
+    return step_guard(ctx, [&] {
+        if (!state || !diag || !out) throw StepError(BUG_EINVAL, "bad expectation arguments");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        set_thread_stream(state->stream);
+        const Exec ex{ctx, state->stream, BUG_PREC_FP64};
+        MPS work = state->psi;
+        canonical(ex, work, 0);
+        const int d = (int)work.charges.size();
+        for (int j = 0; j < work.size(); ++j) {
+            if (j > 0) canonical(ex, work, j);
+            const Tensor& A = work.t[j];
+            std::vector<bug_copy> blks;
+            std::vector<int> slices;
+            for (const Blk& b : A.L().blks) {
+                blks.push_back(bug_copy{b.off, 0, std::max(b.d, 1), 1, b.d, b.dc, 0, 0, 0});
+                slices.push_back(b.k);
+            }
+            const std::vector<double> w = block_sqnorms(ex, blks, A.buf->p, A.nelem());
+            std::vector<double> per(d, 0.0);
+            for (size_t i = 0; i < w.size(); ++i) per[slices[i]] += w[i];
+            double tot = 0.0, num = 0.0;
+            for (int s = 0; s < d; ++s) {
+                tot += per[s];
+                num += per[s] * diag[s];
+            }
+            out[j] = tot > 0 ? num / tot : 0.0;
+        }
+    });
+}
+
 int bug_state_destroy(bug_state* state) {
     if (!state) return BUG_OK;
     set_thread_stream(state->stream);

@@ -52,26 +52,66 @@ class CMPO:
 
 
 class CState:
-    """A device-resident state handle, created from a ``SymMPS`` on the current stream."""
+    """A device-resident state handle, created from a ``SymMPS`` on the current stream (from the
+    device tensors directly, or with ``via_host`` through host copies, the path a caller holding
+    the state in host memory takes)."""
 
-    def __init__(self, psi: SymMPS):
+    def __init__(self, psi: Optional[SymMPS] = None, via_host: bool = False, _handle=None,
+                 _charges=None, _L=None):
+        if _handle is not None:
+            self.handle, self.charges, self.L = _handle, _charges, _L
+            return
         ctx = _lib.context()
         L = len(psi)
-        self._keep = []
+        keep = []
         descs = (_lib.BugBsm * L)()
         for i in range(L):
             t = psi[i]
-            host = np.ascontiguousarray(t.data.detach().cpu().numpy()[:max(t.layout.nelem, 1)])
-            self._keep.append(host)
-            descs[i] = t.layout.descriptor(host.ctypes.data, t.layout.nelem)
+            if via_host:
+                host = np.ascontiguousarray(t.data.detach().cpu().numpy()[:max(t.layout.nelem, 1)])
+                keep.append(host)
+                descs[i] = t.layout.descriptor(host.ctypes.data, t.layout.nelem)
+            else:
+                descs[i] = t.layout.descriptor(t.data.data_ptr(), t.layout.nelem)
         charges = _lib.i32(psi.charges)
         h = ctypes.c_void_p()
-        ctx.check(ctx._lib.bug_state_create(ctx.handle, L, len(psi.charges), _lib.ptr32(charges), descs,
-                                            psi.center, current_stream_handle(), ctypes.byref(h)))
-        self._keep = None
+        create = ctx._lib.bug_state_create if via_host else ctx._lib.bug_state_create_device
+        ctx.check(create(ctx.handle, L, len(psi.charges), _lib.ptr32(charges), descs, psi.center,
+                         current_stream_handle(), ctypes.byref(h)))
         self.handle = h
         self.charges = tuple(psi.charges)
         self.L = L
+
+    def clone(self) -> "CState":
+        """A second handle sharing the site tensors (they evolve independently afterwards)."""
+        h = ctypes.c_void_p()
+        rc = _lib.lib().bug_state_clone(self.handle, ctypes.byref(h))
+        if rc != _lib.BUG_OK:
+            raise _lib.BugError(rc, "bug_state_clone failed")
+        return CState(_handle=h, _charges=self.charges, _L=self.L)
+
+    def expect_diagonal(self, diag=(0.5, -0.5)) -> np.ndarray:
+        """<O_j> / <psi|psi> of a diagonal local operator at every site (default S^z)."""
+        ctx = _lib.context()
+        dg = np.ascontiguousarray(diag, dtype=np.float64)
+        out = np.zeros(self.L)
+        ctx.check(ctx._lib.bug_state_expect_diagonal(ctx.handle, self.handle,
+                                                     dg.ctypes.data_as(_lib._f64p),
+                                                     out.ctypes.data_as(_lib._f64p)))
+        return out
+
+    def bond_dims(self):
+        dims = []
+        for i in range(self.L - 1):
+            rn, cn, ne = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+            _lib.lib().bug_state_site_shape(self.handle, i, ctypes.byref(rn), ctypes.byref(cn),
+                                            ctypes.byref(ne))
+            cq, cd = np.zeros(cn.value, np.int32), np.zeros(cn.value, np.int32)
+            ctx = _lib.context()
+            ctx.check(ctx._lib.bug_state_download_site(ctx.handle, self.handle, i, None, None,
+                                                       _lib.ptr32(cq), _lib.ptr32(cd), None, None))
+            dims.append(int(cd.sum()))
+        return dims
 
     def center(self) -> int:
         c = ctypes.c_int32()

@@ -417,7 +417,17 @@ int bug_mpo_destroy(bug_mpo* mpo);
  * device copy. */
 int bug_state_create(bug_ctx* ctx, int32_t L, int32_t d, const int32_t* local_charges,
                      const bug_bsm* sites, int32_t center, void* stream, bug_state** out);
+/* The same from DEVICE data (sites[i].data are device pointers of this context's device). */
+int bug_state_create_device(bug_ctx* ctx, int32_t L, int32_t d, const int32_t* local_charges,
+                            const bug_bsm* sites, int32_t center, void* stream, bug_state** out);
+/* A second handle on the same state (the site tensors are shared: a step replaces tensors and
+ * never writes into one, so the two evolve independently afterwards). */
+int bug_state_clone(const bug_state* src, bug_state** out);
 int bug_state_destroy(bug_state* state);
+/* <O_j> / <psi|psi> at every site j of a diagonal local operator O = sum_s diag[s] |s><s| (S^z:
+ * diag = {0.5, -0.5}); out[L] on the host (magnetisation / sz_expectation,
+ * src/QMB/Observables/local_observables.jl:11-31).  The state is not changed. */
+int bug_state_expect_diagonal(bug_ctx* ctx, const bug_state* state, const double* diag, double* out);
 /* L, d and the canonical centre (0-based). */
 int bug_state_info(const bug_state* state, int32_t* L, int32_t* d, int32_t* center);
 /* Sector counts of the site's row and column legs and its element count. */

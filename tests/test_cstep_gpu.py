@@ -189,3 +189,27 @@ def test_c_step_refuses_options_before_touching_the_state(gpu):
     assert np.array_equal(_dense(st.to_mps()), before)
     with pytest.raises(_lib.BugError):
         C.bug_step(st, C.CMPO(xxz_mpo(L + 2)), -0.05j)
+
+
+def test_state_handles_device_host_clone_and_magnetisation(gpu):
+    """A state handle made from device tensors equals one made through host copies; a clone shares
+    the tensors and evolves independently; the handle's S^z equals the package's magnetisation."""
+    from bondupdategalerkin_amd import bug_step as B
+    from bondupdategalerkin_amd import cstep as C
+    from bondupdategalerkin_amd import mps
+    from bondupdategalerkin_amd import tensor_ops as T
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    L = 12
+    psi = mps.random_mps(L, 16, seed=3, tail=1e-4)
+    B.canonical(psi, 0)
+    T.scale_(1.0 / T.norm(psi[0]), psi[0])
+    a, b = C.CState(psi), C.CState(psi, via_host=True)
+    assert np.array_equal(_dense(a.to_mps()), _dense(b.to_mps()))
+    assert np.abs(a.expect_diagonal() - mps.magnetisation(psi)).max() <= 1e-12
+    c = a.clone()
+    mp = C.CMPO(xxz_mpo(L, delta=0.5))
+    C.bug_step(a, mp, -0.05j, order=2)
+    assert np.array_equal(_dense(c.to_mps()), _dense(b.to_mps()))   # the clone did not move
+    assert np.abs(a.expect_diagonal() - mps.magnetisation(a.to_mps())).max() <= 1e-12
+    assert a.bond_dims() == a.to_mps().bond_dims()
