@@ -177,6 +177,10 @@ SIGNATURES = {
         ctypes.c_int,
         [_P, _BSMP, _MPOP, _BSMP, _BSMP, _BSMP, ctypes.c_double, ctypes.c_int32, _P],
     ),
+    "bug_apply_one_site_noalloc": (
+        ctypes.c_int,
+        [_P, _BSMP, _MPOP, _BSMP, _BSMP, _BSMP, ctypes.c_double, ctypes.c_int32, _P],
+    ),
     "bug_apply_one_site_batch": (
         ctypes.c_int,
         [_P, ctypes.c_int32, _BSMP, _MPOP, _BSMP, _BSMP, _BSMP, _f64p, ctypes.c_int32, _P],

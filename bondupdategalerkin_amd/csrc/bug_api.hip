@@ -257,7 +257,7 @@ int choose_config(const Plan& plan) {
 
 // Make the device image of `plan` for tile configuration `config` resident in the plan arena:
 // build it on the host once per configuration, copy it in stream order (no host wait).
-void upload_plan(bug_ctx* ctx, Plan& plan, int config, hipStream_t stream) {
+void build_plan_image(Plan& plan, int config) {
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
     if (plan.config != config || plan.image.empty()) {
         const TileConfig& tc = kTileConfigs[config];
@@ -282,6 +282,11 @@ void upload_plan(bug_ctx* ctx, Plan& plan, int config, hipStream_t stream) {
         plan.off_tasks = align(bt);
         plan.off_terms = align(bt) + align(bk);
     }
+}
+
+void upload_plan(bug_ctx* ctx, Plan& plan, int config, hipStream_t stream) {
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    build_plan_image(plan, config);
     if (std::find(ctx->arena_streams.begin(), ctx->arena_streams.end(), stream) ==
         ctx->arena_streams.end())
         ctx->arena_streams.push_back(stream);
@@ -573,6 +578,45 @@ void append_plan(Plan& dst, const Plan& src, int64_t c_shift, int64_t a_shift, i
     dst.flops += src.flops;
 }
 
+// ---- the allocation-free entry point's workspace ------------------------------------------------
+//
+// A file-scope device array, one copy per GPU, that bug_apply_one_site_noalloc carves into the
+// stage-1 intermediate, the two plan images and the split-k partial tiles of each call.  Its device
+// address is looked up once per device (with that device current) when a context is created there;
+// host code uses that address, never the array's name.  Every call on one GPU shares the array and
+// leaves its contents behind: the intermediate and the partials are written in full before they are
+// read, so nothing needs clearing, and calls that overlap on one GPU (different streams or threads)
+// race on it.
+namespace noalloc {
+constexpr size_t kBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kBytes];
+constexpr int kMaxDevices = 64;
+std::mutex mu;
+void* ptr[kMaxDevices] = {};
+bool looked_up[kMaxDevices] = {};
+
+// With `device` current: the array's address on it (once per device; null if the lookup failed).
+void setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lk(mu);
+    if (looked_up[device]) return;
+    looked_up[device] = true;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) {
+        ptr[device] = p;
+    } else {
+        (void)hipGetLastError();
+        ptr[device] = nullptr;
+    }
+}
+
+void* address(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    return ptr[device];
+}
+}  // namespace noalloc
+
 }  // namespace bug
 
 using namespace bug;
@@ -595,6 +639,16 @@ int bug_ctx_create(int device, bug_ctx** out) {
     }
     bug_ctx* ctx = new bug_ctx();
     ctx->device = device;
+    {
+        // the allocation-free entry point's workspace address on this device, with it current
+        int prev = 0;
+        if (hipGetDevice(&prev) == hipSuccess && hipSetDevice(device) == hipSuccess) {
+            noalloc::setup(device);
+            (void)hipSetDevice(prev);
+        } else {
+            (void)hipGetLastError();
+        }
+    }
     *out = ctx;
     return BUG_OK;
 }
@@ -802,6 +856,69 @@ int bug_apply_one_site(bug_ctx* ctx, const bug_bsm* L, const bug_mpo_site* W, co
         void* tdev = scratch(ctx, st, sizeof(double2) * std::max<int64_t>(sg.T.nelem, 1));
         run_product(ctx, sg.T, tdev, l, L->data, x, X->data, sg.t1, false, precision, st, 0x31);
         run_product(ctx, y, Y->data, sg.T, tdev, r, R->data, sg.t2, false, precision, st, 0x32);
+        if (shift != 0.0) shift_into(x, y, X->data, Y->data, shift, st);
+    });
+}
+
+int bug_apply_one_site_noalloc(bug_ctx* ctx, const bug_bsm* L, const bug_mpo_site* W,
+                               const bug_bsm* R, const bug_bsm* X, const bug_bsm* Y, double shift,
+                               int32_t precision, void* stream) {
+    return guarded(ctx, [&] {
+        if (precision != BUG_PREC_FP64 && precision != BUG_PREC_FP32)
+            throw Error(BUG_EINVAL, "unknown precision");
+        char* base = static_cast<char*>(noalloc::address(ctx->device));
+        if (!base) throw Error(BUG_EDEVICE, "the static workspace has no device address on this GPU");
+        Bsm l(L, "L"), r(R, "R"), x(X, "X"), y(Y, "Y");
+        MpoSite w(W, x.nslice());
+        const OneSiteStages sg = one_site_stages(l, w, r, x, y);
+        hipStream_t st = as_stream(stream);
+        struct NoTune {
+            bug_ctx* c;
+            ~NoTune() { c->no_tune = false; }
+        } reset{ctx};
+        ctx->no_tune = true;
+        // the two stages' plans and schedules exactly as bug_apply_one_site takes them
+        Plan* p1 = cached_product_plan(ctx, sg.T, l, x, sg.t1, false, 0x31);
+        Plan* p2 = cached_product_plan(ctx, y, sg.T, r, sg.t2, false, 0x32);
+        const Schedule s1 = pick_schedule(ctx, *p1, precision, L->data, X->data, st);
+        const Schedule s2 = pick_schedule(ctx, *p2, precision, nullptr, R->data, st);
+        build_plan_image(*p1, s1.config);
+        build_plan_image(*p2, s2.config);
+        auto align = [](size_t b) { return (b + 255) & ~size_t(255); };
+        const size_t tb = align(sizeof(double2) * (size_t)std::max<int64_t>(sg.T.nelem, 1));
+        const size_t i1 = align(p1->image.size()), i2 = align(p2->image.size());
+        const size_t q1 = s1.splits > 1 ? align(split_partial_bytes(*p1, s1.config, s1.splits, precision)) : 0;
+        const size_t q2 = s2.splits > 1 ? align(split_partial_bytes(*p2, s2.config, s2.splits, precision)) : 0;
+        const size_t need = tb + i1 + i2 + q1 + q2;
+        if (need > noalloc::kBytes)
+            throw Error(BUG_EINVAL, "this apply needs " + std::to_string(need) +
+                                        " bytes of workspace; the allocation-free entry point holds " +
+                                        std::to_string(noalloc::kBytes));
+        char* tdev = base;
+        char* img1 = tdev + tb;
+        char* img2 = img1 + i1;
+        char* part1 = q1 ? img2 + i2 : nullptr;
+        char* part2 = q2 ? img2 + i2 + q1 : nullptr;
+        HIP_CHECK(hipMemcpyAsync(img1, p1->image.data(), p1->image.size(), hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(img2, p2->image.data(), p2->image.size(), hipMemcpyHostToDevice, st));
+        auto launch = [&](Plan& p, const Schedule& sc, char* img, const void* A, const void* B, void* C,
+                          void* part) {
+            const GemmTile* tiles = p.d_tiles;
+            const GemmTask* tasks = p.d_tasks;
+            const GemmTerm* terms = p.d_terms;
+            p.d_tiles = reinterpret_cast<const GemmTile*>(img);
+            p.d_tasks = reinterpret_cast<const GemmTask*>(img + p.off_tasks);
+            p.d_terms = reinterpret_cast<const GemmTerm*>(img + p.off_terms);
+            const hipError_t e = launch_grouped_gemm(p, sc.config, sc.splits, precision, A, B, C, part, st);
+            p.d_tiles = tiles;
+            p.d_tasks = tasks;
+            p.d_terms = terms;
+            HIP_CHECK(e);
+            ctx->last_flops += p.flops;
+            ctx->last_schedules.push_back(sc);
+        };
+        launch(*p1, s1, img1, L->data, X->data, tdev, part1);
+        launch(*p2, s2, img2, tdev, R->data, Y->data, part2);
         if (shift != 0.0) shift_into(x, y, X->data, Y->data, shift, st);
     });
 }

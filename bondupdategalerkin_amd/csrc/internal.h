@@ -159,6 +159,9 @@ size_t split_partial_bytes(const Plan& plan, int config, int splits, int precisi
 // Tile configuration by block-size statistics; device image of a plan for a configuration.
 int choose_config(const Plan& plan);
 void upload_plan(bug_ctx* ctx, Plan& plan, int config, hipStream_t stream);
+// The host image [tiles | tasks | terms] of `plan` for tile configuration `config` (built once per
+// configuration; a rebuild drops the plan's arena residency).
+void build_plan_image(Plan& plan, int config);
 
 // The schedule of a launch of `plan`: forced by the context, from the process-wide tuning table
 // (timing the candidates first when tuning is on and the plan's class is new), or heuristic.
@@ -215,6 +218,9 @@ struct bug_ctx {
     // Jacobi over row chunks) and an optional
     // device array receiving the sweeps used per matrix of the last batch (diagnostics)
     int svd_variant = 0;
+    // set during an allocation-free call: schedules come from the tuning table or the heuristic,
+    // never from timing candidates (which needs scratch memory)
+    bool no_tune = false;
     int* svd_sweeps_dev = nullptr;
     // QR kernel choice (0 automatic, 1 one workgroup per matrix, 2 blocked panels where rows fit)
     int qr_variant = 0;

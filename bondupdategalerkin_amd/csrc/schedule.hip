@@ -185,6 +185,13 @@ Schedule pick_schedule(bug_ctx* ctx, Plan& plan, int precision, const void* A, c
                 }
             }
             // no timing (it waits on the stream) while a graph is being captured on it
+            if (!found && mode == 2 && ctx->no_tune) {
+                // an allocation-free call takes the heuristic for an untuned class and leaves
+                // the class to be tuned by the next ordinary call
+                if (ctx->forced_config >= 0) s.config = ctx->forced_config;
+                if (ctx->forced_splits >= 1) s.splits = ctx->forced_splits;
+                return s;
+            }
             if (!found && mode == 2 && !capturing_on(ctx, stream)) {
                 const TuneEntry e = tune(ctx, plan, precision, A, B, stream);
                 std::lock_guard<std::mutex> lk(tt.mu);

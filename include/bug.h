@@ -189,6 +189,16 @@ int bug_env_push_right(bug_ctx* ctx, const bug_bsm* R, const bug_bsm* B, const b
 int bug_apply_one_site(bug_ctx* ctx, const bug_bsm* L, const bug_mpo_site* W, const bug_bsm* R,
                        const bug_bsm* X, const bug_bsm* Y, double shift, int32_t precision,
                        void* stream);
+/* The same apply with no device allocation: no counterpart in the reference, it is for callers
+ * that cannot allocate (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The stage-1 intermediate, the plan images and the split-k partials go into one
+ * static device workspace of 64 MiB per GPU (applies that need more are refused with BUG_EINVAL
+ * before anything runs); schedules come from the tuning table or the heuristic, never from timing.
+ * Calls that overlap on one GPU (different streams or threads) race on the workspace.  Results
+ * equal bug_apply_one_site's. */
+int bug_apply_one_site_noalloc(bug_ctx* ctx, const bug_bsm* L, const bug_mpo_site* W,
+                               const bug_bsm* R, const bug_bsm* X, const bug_bsm* Y, double shift,
+                               int32_t precision, void* stream);
 /* k independent one-site applies Y_i = (H_eff,i - shifts[i]) X_i (shifts may be NULL) with one
  * grouped-GEMM launch per product stage for all of them: the instances' block products are
  * packed into one task list over the lowest of their base addresses (operands 16-byte aligned).
