@@ -144,6 +144,16 @@ class BugStepInfo(ctypes.Structure):
                 ("log_count", ctypes.c_int32)]
 
 
+class BugExpvReport(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("substeps", ctypes.c_int32),
+                ("err", ctypes.c_double)]
+
+
+# int (*bug_apply_fn)(void* user, const void* x, void* y, void* stream)
+APPLY_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                            ctypes.c_void_p)
+
+
 # name -> (restype, argtypes); the list is the ABI of include/bug.h
 _P = ctypes.c_void_p
 _BSMP = ctypes.POINTER(BugBsm)
@@ -223,6 +233,15 @@ SIGNATURES = {
     "bug_svd_sweeps": (ctypes.c_int, [_P, ctypes.c_int32, _i32p]),
     "bug_set_svd_floor": (ctypes.c_int, [_P, ctypes.c_double]),
     "bug_step_opts_default": (None, [ctypes.POINTER(BugStepOpts)]),
+    "bug_two_site_sketch_left": (ctypes.c_int, [_P, _BSMP, _MPOP, _MPOP, _BSMP, _BSMP, _BSMP, _BSMP,
+                                                _BSMP, ctypes.c_int32, _P]),
+    "bug_two_site_sketch_right": (ctypes.c_int, [_P, _BSMP, _MPOP, _MPOP, _BSMP, _BSMP, _BSMP,
+                                                 _BSMP, _BSMP, ctypes.c_int32, _P]),
+    "bug_expv": (ctypes.c_int, [_P, APPLY_FN, _P, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                                _P, _P, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
+                                ctypes.c_int32, ctypes.c_double, ctypes.POINTER(BugExpvReport), _P]),
+    "bug_svd_select": (ctypes.c_int, [ctypes.c_int32, _i32p, _f64p, ctypes.c_double, ctypes.c_int32,
+                                      _i32p, _i32p, _i32p, _f64p]),
     "bug_mpo_create": (ctypes.c_int, [ctypes.c_int32, _MPOP, ctypes.POINTER(_P)]),
     "bug_mpo_destroy": (ctypes.c_int, [_P]),
     "bug_state_create": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _i32p, _BSMP,

@@ -1436,6 +1436,76 @@ int step_guard(bug_ctx* ctx, F&& f) {
     }
 }
 
+Leg leg_from(const bug_leg& l);
+
+// The caller's family as a tensor: a view when its blocks sit where the library's packing puts
+// them, else a gathered copy.
+Tensor tensor_in(const Exec& ex, const bug_bsm& b) {
+    if (!b.data || b.nslice < 1) throw StepError(BUG_EINVAL, "family without data or slices");
+    const Layout lay(leg_from(b.row), leg_from(b.col), std::vector<int32_t>(b.shift, b.shift + b.nslice));
+    bool same = b.nelem >= lay.nelem;
+    std::vector<int64_t> src(lay.blks.size(), -1);
+    for (size_t i = 0; i < lay.blks.size(); ++i) {
+        const Blk& blk = lay.blks[i];
+        int idx = -1;
+        for (int r = 0; r < b.row.nsec; ++r)
+            if (b.row.charge[r] == blk.q) idx = r;
+        src[i] = idx < 0 ? -1 : b.offset[(size_t)blk.k * b.row.nsec + idx];
+        if (src[i] >= 0 && src[i] + (int64_t)blk.d * blk.dc > b.nelem)
+            throw StepError(BUG_EINVARIANT, "block outside its family's data");
+        same &= src[i] == blk.off;
+    }
+    if (same) {
+        Tensor t;
+        t.lay = std::make_shared<const Layout>(lay);
+        // the view does not own the caller's memory: the deleter detaches it before the buffer
+        // is destroyed, so it never reaches the pool
+        t.buf = std::shared_ptr<DevBuf>(new DevBuf(0), [](DevBuf* d) {
+            d->p = nullptr;
+            delete d;
+        });
+        dev_free(t.buf->p, t.buf->bytes);
+        t.buf->p = b.data;
+        t.buf->bytes = sizeof(double2) * (size_t)b.nelem;
+        return t;
+    }
+    Tensor t = t_zeros(ex, lay);
+    CopyList cl;
+    for (size_t i = 0; i < lay.blks.size(); ++i)
+        if (src[i] >= 0) cl.add(src[i], lay.blks[i].d, lay.blks[i].off, lay.blks[i].d, lay.blks[i].d, lay.blks[i].dc);
+    cl.run(ex, b.data, b.nelem, t.buf->p, t.nelem());
+    return t;
+}
+
+// Scatter `t` into the caller's family (same legs and slices required).
+void tensor_out(const Exec& ex, const Tensor& t, const bug_bsm& b) {
+    const Layout want(leg_from(b.row), leg_from(b.col), std::vector<int32_t>(b.shift, b.shift + b.nslice));
+    if (want != t.L()) throw StepError(BUG_EINVAL, "output family has the wrong legs or slices");
+    CopyList cl;
+    for (const Blk& blk : t.L().blks) {
+        int idx = -1;
+        for (int r = 0; r < b.row.nsec; ++r)
+            if (b.row.charge[r] == blk.q) idx = r;
+        const int64_t dst = idx < 0 ? -1 : b.offset[(size_t)blk.k * b.row.nsec + idx];
+        if (dst < 0) throw StepError(BUG_EINVARIANT, "output family lacks a block of the result");
+        if (dst + (int64_t)blk.d * blk.dc > b.nelem) throw StepError(BUG_EINVARIANT, "output block outside its data");
+        cl.add(blk.off, blk.d, dst, blk.d, blk.d, blk.dc);
+    }
+    cl.run(ex, t.buf->p, t.nelem(), b.data, b.nelem);
+}
+
+MpoSite mpo_in(const bug_mpo_site& w) {
+    MpoSite s;
+    s.cl.assign(w.cl, w.cl + w.wl);
+    s.cr.assign(w.cr, w.cr + w.wr);
+    s.a.assign(w.a, w.a + w.nent);
+    s.b.assign(w.b, w.b + w.nent);
+    s.sout.assign(w.sout, w.sout + w.nent);
+    s.sin.assign(w.sin, w.sin + w.nent);
+    s.coeff.assign(w.coeff, w.coeff + 2 * w.nent);
+    return s;
+}
+
 Leg leg_from(const bug_leg& l) {
     Leg out;
     for (int i = 0; i < l.nsec; ++i) {
@@ -1637,6 +1707,113 @@ int bug_state_expect_diagonal(bug_ctx* ctx, const bug_state* state, const double
             out[j] = tot > 0 ? num / tot : 0.0;
         }
     });
+}
+
+int bug_two_site_sketch_left(bug_ctx* ctx, const bug_bsm* L, const bug_mpo_site* W1,
+                             const bug_mpo_site* W2, const bug_bsm* R, const bug_bsm* A,
+                             const bug_bsm* V, const bug_bsm* Om, const bug_bsm* out,
+                             int32_t precision, void* stream) {
+    return step_guard(ctx, [&] {
+        if (!L || !W1 || !W2 || !R || !A || !V || !Om || !out) throw StepError(BUG_EINVAL, "null argument");
+        if (precision != BUG_PREC_FP64 && precision != BUG_PREC_FP32) throw StepError(BUG_EINVAL, "unknown precision");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        set_thread_stream(st);
+        const Exec ex{ctx, st, precision};
+        Tensor y = stream_left(ex, tensor_in(ex, *L), mpo_in(*W1), mpo_in(*W2), tensor_in(ex, *R),
+                               tensor_in(ex, *A), tensor_in(ex, *V), tensor_in(ex, *Om), precision);
+        tensor_out(ex, y, *out);
+    });
+}
+
+int bug_two_site_sketch_right(bug_ctx* ctx, const bug_bsm* L, const bug_mpo_site* W1,
+                              const bug_mpo_site* W2, const bug_bsm* R, const bug_bsm* U,
+                              const bug_bsm* B, const bug_bsm* Om, const bug_bsm* out,
+                              int32_t precision, void* stream) {
+    return step_guard(ctx, [&] {
+        if (!L || !W1 || !W2 || !R || !U || !B || !Om || !out) throw StepError(BUG_EINVAL, "null argument");
+        if (precision != BUG_PREC_FP64 && precision != BUG_PREC_FP32) throw StepError(BUG_EINVAL, "unknown precision");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        set_thread_stream(st);
+        const Exec ex{ctx, st, precision};
+        Tensor y = stream_right(ex, tensor_in(ex, *L), mpo_in(*W1), mpo_in(*W2), tensor_in(ex, *R),
+                                tensor_in(ex, *U), tensor_in(ex, *B), tensor_in(ex, *Om), precision);
+        tensor_out(ex, y, *out);
+    });
+}
+
+int bug_expv(bug_ctx* ctx, bug_apply_fn apply, void* user, int64_t n, double tau_re, double tau_im,
+             const void* v, void* out, int32_t hermitian, double tol, int32_t maxiter, int32_t reorth,
+             double conv_tol, bug_expv_report* report, void* stream) {
+    return step_guard(ctx, [&] {
+        if (!apply || !v || !out || n < 1 || maxiter < 1 || maxiter > 64 || !(tol >= 0))
+            throw StepError(BUG_EINVAL, "bad expv arguments");
+        if (v == out) throw StepError(BUG_EINVAL, "expv output may not alias its input");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        set_thread_stream(st);
+        const Exec ex{ctx, st, BUG_PREC_FP64};
+        const Layout lay(Leg::dense((int)std::min<int64_t>(n, INT32_MAX)), Leg::dense(1), {0});
+        if (lay.nelem != n) throw StepError(BUG_EINVAL, "expv vectors hold at most 2^31 - 1 elements");
+        Tensor v0 = t_empty(ex, lay);
+        HIP_CHECK(hipMemcpyAsync(v0.buf->p, v, sizeof(double2) * (size_t)n, hipMemcpyDeviceToDevice, st));
+        auto act = [&](const Tensor& x) {
+            Tensor y = t_empty(ex, lay);
+            const int rc = apply(user, x.buf->p, y.buf->p, stream);
+            if (rc != 0) throw StepError(BUG_EINVAL, "the apply callback returned " + std::to_string(rc));
+            return y;
+        };
+        ExpvReport rep;
+        const cplx tau(tau_re, tau_im);
+        Tensor r = hermitian ? lanczos_expv(ex, act, tau, v0, tol, maxiter, reorth != 0, conv_tol, rep)
+                             : arnoldi_expv(ex, act, tau, v0, tol, maxiter, conv_tol, rep);
+        HIP_CHECK(hipMemcpyAsync(out, r.buf->p, sizeof(double2) * (size_t)n, hipMemcpyDeviceToDevice, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        if (report) {
+            report->iterations = rep.iterations;
+            report->substeps = rep.substeps;
+            report->err = rep.err;
+        }
+    });
+}
+
+int bug_svd_select(int32_t nsec, const int32_t* lens, const double* spectra, double cutoff,
+                   int32_t nkeep, int32_t* keep, int32_t* order, int32_t* nkept, double* weights) {
+    if (nsec < 0 || (nsec > 0 && (!lens || !keep)) || !nkept || !(cutoff >= 0)) return BUG_EINVAL;
+    std::vector<std::vector<double>> spec(nsec);
+    std::vector<int> tags(nsec);
+    int64_t pos = 0;
+    for (int i = 0; i < nsec; ++i) {
+        if (lens[i] < 0 || (lens[i] > 0 && !spectra)) return BUG_EINVAL;
+        spec[i].assign(spectra + pos, spectra + pos + lens[i]);
+        for (double sv : spec[i])
+            if (!std::isfinite(sv)) return BUG_ENONFINITE;
+        pos += lens[i];
+        tags[i] = i;
+    }
+    std::vector<SvEntry> full;
+    const int n = global_selection(spec, tags, cutoff, nkeep, full);
+    double wk = 0.0, wt = 0.0;
+    for (int i = 0; i < nsec; ++i) keep[i] = 0;
+    for (int e = 0; e < (int)full.size(); ++e) {
+        if (e < n) {
+            keep[full[e].q] += 1;
+            wk += full[e].s * full[e].s;
+        } else {
+            wt += full[e].s * full[e].s;
+        }
+        if (order) {
+            order[2 * e] = full[e].q;
+            order[2 * e + 1] = full[e].j;
+        }
+    }
+    *nkept = n;
+    if (weights) {
+        weights[0] = wk;
+        weights[1] = wt;
+    }
+    return BUG_OK;
 }
 
 int bug_state_destroy(bug_state* state) {

@@ -189,6 +189,10 @@ struct SvEntry {
     double s;
     int q, j;
 };
+// Telum's global selection over per-sector spectra (sector i tagged tags[i]): `full` receives every
+// value sorted descending (stable, in sector order); returns how many of them are kept
+int global_selection(const std::vector<std::vector<double>>& spectra, const std::vector<int>& tags,
+                     double cutoff, int nkeep, std::vector<SvEntry>& full);
 struct SVDResult {
     Tensor U, Vh;
     std::map<int, std::vector<double>> S;

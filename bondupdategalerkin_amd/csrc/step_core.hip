@@ -743,32 +743,34 @@ struct SvdRun {
 };
 }  // namespace
 
-SVDResult BlockSVD::select(const Exec& ex, double cutoff, int nkeep) const {
+int global_selection(const std::vector<std::vector<double>>& spectra, const std::vector<int>& tags,
+                     double cutoff, int nkeep, std::vector<SvEntry>& full) {
     // Telum's global selection: sv > max(cutoff, floor) sv_max over all sectors, then the largest
-    // nkeep by a stable descending sort in sector order
-    std::vector<SvEntry> entries;
+    // nkeep by a stable descending sort in sector order; the kept values lead the sorted list
+    full.clear();
     double smax = 0.0;
-    for (size_t i = 0; i < qlist.size(); ++i)
+    for (size_t i = 0; i < spectra.size(); ++i)
         for (size_t j = 0; j < spectra[i].size(); ++j) {
             smax = std::max(smax, spectra[i][j]);
-            entries.push_back({spectra[i][j], qlist[i], (int)j});
+            full.push_back({spectra[i][j], tags[i], (int)j});
         }
-    std::vector<SvEntry> full = entries;
     std::stable_sort(full.begin(), full.end(), [](const SvEntry& a, const SvEntry& b) { return a.s > b.s; });
     const double thresh = std::max(cutoff, kZeroRel) * smax;
-    std::vector<SvEntry> sel;
-    for (const SvEntry& e : full)
-        if (e.s > thresh) sel.push_back(e);
-    if (nkeep >= 0 && (int)sel.size() > nkeep) sel.resize(nkeep);
+    int n = 0;
+    while (n < (int)full.size() && full[n].s > thresh) ++n;
+    if (nkeep >= 0) n = std::min(n, nkeep);
+    return n;
+}
+
+SVDResult BlockSVD::select(const Exec& ex, double cutoff, int nkeep) const {
+    std::vector<SvEntry> full;
+    const int n = global_selection(spectra, qlist, cutoff, nkeep, full);
     std::map<int, int> keep;
     for (int q : qlist) keep[q] = 0;
-    std::map<std::pair<int, int>, bool> chosen;
-    for (const SvEntry& e : sel) {
-        keep[e.q] += 1;
-        chosen[{e.q, e.j}] = true;
-    }
     SVDResult r;
-    for (const SvEntry& e : full) (chosen.count({e.q, e.j}) ? r.kept : r.trunc).push_back(e);
+    r.kept.assign(full.begin(), full.begin() + n);
+    r.trunc.assign(full.begin() + n, full.end());
+    for (const SvEntry& e : r.kept) keep[e.q] += 1;
     std::vector<std::pair<int, int>> bp;
     for (int q : qlist)
         if (keep[q] > 0) bp.emplace_back(q, keep[q]);

@@ -5,12 +5,16 @@ C ABI of ``libbug.so``: the state and the MPO are opaque library handles, the op
 the reference's keywords, the report ``CBEBugSweepInfo`` plus the ledger verdict.  This module
 turns the package's ``SymMPS`` / ``MPO`` into those handles and back; the step itself runs in
 C++ inside the library, on the stream the state was created on (torch's current stream here).
+
+The step's operator actions are exported beside it: the streamed two-site sketches
+(``apply_two_site_sketch!``), ``expv`` over a caller-supplied action, and the global
+singular-value selection.
 """
 from __future__ import annotations
 
 import ctypes
 import math
-from typing import Dict, Optional
+from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -231,3 +235,77 @@ def bug_step(state: CState, mpo: CMPO, tau: complex, **kw) -> Dict:
     out["bond_log"] = [{f: getattr(logs[i], f) for f, _ in _lib.BugBondLog._fields_}
                        for i in range(info.log_count)]
     return out
+
+
+def two_site_sketch(side: str, L: BlockTensor, W1, W2, R: BlockTensor, A: BlockTensor,
+                    B: BlockTensor, Om: BlockTensor, out_layout: Layout,
+                    precision: str = "fp64") -> BlockTensor:
+    """The streamed two-site sketch of Theta = A B through ``bug_two_site_sketch_left/right``
+    (apply_two_site_sketch!, dd_cbe.jl:70-89): side "left" gives (H Theta) Om^dag with Om in B's
+    layout, side "right" Om^dag (H Theta) with Om in A's layout; the result in ``out_layout``."""
+    if side not in ("left", "right"):
+        raise ValueError("side must be 'left' or 'right'")
+    if precision not in ("fp64", "mixed"):
+        raise ValueError("precision must be 'fp64' or 'mixed'")
+    ctx = _lib.context()
+    out = BlockTensor(out_layout)
+    fn = ctx._lib.bug_two_site_sketch_left if side == "left" else ctx._lib.bug_two_site_sketch_right
+    prec = _lib.PREC_FP32 if precision == "mixed" else _lib.PREC_FP64
+    ctx.check(fn(ctx.handle, L.desc(), W1.descriptor(), W2.descriptor(), R.desc(), A.desc(),
+                 B.desc(), Om.desc(), out.desc(), prec, current_stream_handle()))
+    return out
+
+
+def expv(apply: Callable[[int, int, int], None], v: torch.Tensor, tau: complex, *,
+         hermitian: bool = True, tol: float = 1e-15, maxiter: int = 30, reorth: bool = True,
+         conv_tol: float = 0.0) -> Tuple[torch.Tensor, Dict]:
+    """exp(tau A) v through ``bug_expv`` for a flat complex128 device vector ``v``; ``apply(x, y,
+    stream)`` receives device addresses of n-element vectors and enqueues y = A x on ``stream``."""
+    if v.dtype != torch.complex128 or not v.is_cuda or v.dim() != 1:
+        raise ValueError("v must be a 1-d complex128 device tensor")
+    v = v.contiguous()
+    ctx = _lib.context()
+    errors: List[BaseException] = []
+
+    def _cb(user, x, y, stream):
+        try:
+            apply(int(x or 0), int(y or 0), int(stream or 0))
+            return 0
+        except BaseException as e:        # reported through the return code, re-raised below
+            errors.append(e)
+            return 1
+
+    cb = _lib.APPLY_FN(_cb)
+    out = torch.empty_like(v)
+    rep = _lib.BugExpvReport()
+    tau = complex(tau)
+    rc = ctx._lib.bug_expv(ctx.handle, cb, None, v.numel(), tau.real, tau.imag, v.data_ptr(),
+                           out.data_ptr(), 1 if hermitian else 0, tol, maxiter,
+                           1 if reorth else 0, conv_tol, ctypes.byref(rep),
+                           current_stream_handle())
+    if errors:
+        raise errors[0]
+    ctx.check(rc)
+    return out, {"iterations": rep.iterations, "substeps": rep.substeps, "err": rep.err}
+
+
+def svd_select(spectra: Sequence[np.ndarray], cutoff: float = 0.0, nkeep: Optional[int] = None):
+    """Telum's global selection through ``bug_svd_select``: (kept count per sector, the (sector,
+    index) pairs of every value in descending order, how many of them are kept, [kept weight,
+    truncated weight])."""
+    lens = _lib.i32([len(s) for s in spectra])
+    flat = np.ascontiguousarray(np.concatenate([np.asarray(s, np.float64) for s in spectra])
+                                if len(spectra) else np.zeros(0), dtype=np.float64)
+    total = int(lens.sum())
+    keep = np.zeros(max(len(spectra), 1), np.int32)
+    order = np.zeros(max(2 * total, 1), np.int32)
+    nk = ctypes.c_int32()
+    w = np.zeros(2)
+    rc = _lib.lib().bug_svd_select(len(spectra), _lib.ptr32(lens), flat.ctypes.data_as(_lib._f64p),
+                                   float(cutoff), -1 if nkeep is None else int(nkeep),
+                                   _lib.ptr32(keep), _lib.ptr32(order), ctypes.byref(nk),
+                                   w.ctypes.data_as(_lib._f64p))
+    if rc != _lib.BUG_OK:
+        raise _lib.BugError(rc, "bug_svd_select refused its arguments")
+    pairs = [(int(order[2 * e]), int(order[2 * e + 1])) for e in range(total)]
+    return [int(k) for k in keep[:len(spectra)]], pairs, int(nk.value), [float(w[0]), float(w[1])]

@@ -319,6 +319,51 @@ int bug_svd_sweeps(bug_ctx* ctx, int32_t n, int32_t* out);
  * [0, 1e-6). */
 int bug_set_svd_floor(bug_ctx* ctx, double rel);
 
+/* ---- operator actions of the step (dd_cbe.jl:9-117, root_methods.jl:359-402) -------------- */
+
+/* The streamed two-site sketch (apply_two_site_sketch!, dd_cbe.jl:70-89; _stream_left,
+ * mpo_environments.jl:409-444): out = (H Theta) Om^dag for Theta = A V, without forming the
+ * two-site tensor.  L: environment left of site A, R: right of site V, W1/W2 the MPO sites of A
+ * and V, Om in V's layout (probe width on its row leg).  out (caller-allocated) has rows = L's
+ * bra leg, columns = Om's row leg, slices = A's local states. */
+int bug_two_site_sketch_left(bug_ctx* ctx, const bug_bsm* L, const bug_mpo_site* W1,
+                             const bug_mpo_site* W2, const bug_bsm* R, const bug_bsm* A,
+                             const bug_bsm* V, const bug_bsm* Om, const bug_bsm* out,
+                             int32_t precision, void* stream);
+/* The mirror (_stream_right, :446-481): out = Om^dag (H Theta) for Theta = U B, Om in U's layout
+ * (probe width on its column leg); out has rows = Om's column leg, columns = R's bra leg, slices
+ * = B's local states. */
+int bug_two_site_sketch_right(bug_ctx* ctx, const bug_bsm* L, const bug_mpo_site* W1,
+                              const bug_mpo_site* W2, const bug_bsm* R, const bug_bsm* U,
+                              const bug_bsm* B, const bug_bsm* Om, const bug_bsm* out,
+                              int32_t precision, void* stream);
+
+/* A linear action on n-element complex device vectors: y = A x, enqueued on `stream`; returns 0
+ * on success.  `user` is passed through. */
+typedef int (*bug_apply_fn)(void* user, const void* x, void* y, void* stream);
+typedef struct bug_expv_report {
+    int32_t iterations;
+    int32_t substeps;
+    double err;                   /* the accumulated tail estimate (Saad), relative */
+} bug_expv_report;
+/* out = exp(tau A) v (expv, root_methods.jl:359-402; lanczos_expv / arnoldi_expv,
+ * krylov.jl:128-263,343-442) with the basis of at most maxiter vectors per substep, the substep
+ * search holding the tail estimate to conv_tol, full reorthogonalisation (reorth) for the Lanczos
+ * form; hermitian selects Lanczos (A Hermitian) or Arnoldi.  v and out are device vectors of n
+ * complex FP64 elements (out may not alias v). */
+int bug_expv(bug_ctx* ctx, bug_apply_fn apply, void* user, int64_t n, double tau_re, double tau_im,
+             const void* v, void* out, int32_t hermitian, double tol, int32_t maxiter,
+             int32_t reorth, double conv_tol, bug_expv_report* report, void* stream);
+
+/* Telum's global singular-value selection (svd.jl:1813-1921) on per-sector spectra (each sorted
+ * descending; sector i has lens[i] values at spectra[sum_{k<i} lens[k]]): keep sv > max(cutoff,
+ * 1e-14) sv_max over all sectors, then the largest nkeep (nkeep < 0: no limit) by a stable
+ * descending sort in sector order.  keep[nsec] receives the kept count per sector, order[2 total]
+ * the (sector, index) pairs of all values in that descending order (the first *nkept are the kept
+ * list, the rest the truncated list), and weights[2] the kept and truncated sums of squares. */
+int bug_svd_select(int32_t nsec, const int32_t* lens, const double* spectra, double cutoff,
+                   int32_t nkeep, int32_t* keep, int32_t* order, int32_t* nkept, double* weights);
+
 /* ---- the whole step ------------------------------------------------------------------------ */
 
 /*
