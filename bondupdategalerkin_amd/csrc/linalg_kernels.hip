@@ -1480,8 +1480,6 @@ __global__ __launch_bounds__(kGbjThreads) void gbj_rotate_kernel(const BjMat* __
     __shared__ double2 G[kGbjCols][kGbjCols + 1];
     __shared__ double2 Z[kGbjCols][kGbjCols + 1];
     __shared__ double2 xs[kGbjRows / 2][kGbjCols + 1];
-    __shared__ double rot[kGbjCols / 2][4];
-    __shared__ int rab[kGbjCols / 2][2];
     __shared__ int s_rot;
     const int2 tk = tasks[blockIdx.x];
     const BjMat mt = mats[tk.x];
@@ -1519,76 +1517,99 @@ __global__ __launch_bounds__(kGbjThreads) void gbj_rotate_kernel(const BjMat* __
     const int C = c + (c & 1);
     const int np = cross_only ? small : C / 2;
     const int nrounds = cross_only ? big : C - 1;
+    // Thread t works on pair g = t / kGbjCols, entry k = t % kGbjCols of the pair's two columns
+    // (then of its two rows), kGbjThreads / kGbjCols pairs at a time.  Every thread of a pair
+    // computes the pair's rotation itself from G as the round found it: the pair's kGbjCols
+    // threads sit in one wavefront and read G[a][a], G[b][b], G[a][b] before any of them writes,
+    // and no other pair writes columns a or b, so a round needs only the barrier between its
+    // column and row updates and the one after them.
+    constexpr int kPairsAtOnce = kGbjThreads / kGbjCols;
+    constexpr int kSlots = (kGbjCols / 2 + kPairsAtOnce - 1) / kPairsAtOnce;
+    const int gl = tid / kGbjCols, k = tid % kGbjCols;
+    bool any = false;
     for (int pass = 0; pass < inner; ++pass) {
         for (int r = 0; r < nrounds; ++r) {
-            if (tid < np) {
+            double cs[kSlots], sn[kSlots], er[kSlots], ei[kSlots];
+            int ra[kSlots], rb[kSlots];
+#pragma unroll
+            for (int sl = 0; sl < kSlots; ++sl) {
+                const int g = gl + sl * kPairsAtOnce;
+                ra[sl] = -1;
+                if (g >= np) continue;
                 int a, b;
                 bool valid = true;
                 if (cross_only) {
-                    const int i1 = small0 + tid, i2 = big0 + (tid + r) % big;
+                    const int i1 = small0 + g, i2 = big0 + (g + r) % big;
                     a = min(i1, i2);
                     b = max(i1, i2);
                 } else {
-                    round_robin(C, r, tid, a, b);
+                    round_robin(C, r, g, a, b);
                     valid = b < c;
                 }
-                int ra = -1;
-                if (valid) {
-                    const double al = G[a][a].x, be = G[b][b].x;
-                    const double gr = G[a][b].x, gi = G[a][b].y;
-                    const double ag2 = gr * gr + gi * gi;
-                    if (ag2 > tol2 * (al * be) && ag2 != 0.0) {
-                        const double ag = sqrt(ag2);
-                        const double inv = 1.0 / ag;
-                        const double zeta = 0.5 * (be - al) * inv;
-                        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                        const double cs = 1.0 / sqrt(1.0 + t * t);
-                        rot[tid][0] = cs;
-                        rot[tid][1] = cs * t;
-                        rot[tid][2] = gr * inv;
-                        rot[tid][3] = gi * inv;
-                        ra = a;
-                        s_rot = 1;
-                    }
-                }
-                rab[tid][0] = ra;
-                rab[tid][1] = b;
+                if (!valid) continue;
+                const double al = G[a][a].x, be = G[b][b].x;
+                const double gr = G[a][b].x, gi = G[a][b].y;
+                const double ag2 = gr * gr + gi * gi;
+                if (!(ag2 > tol2 * (al * be) && ag2 != 0.0)) continue;
+                const double ag = sqrt(ag2);
+                const double inv = 1.0 / ag;
+                const double zeta = 0.5 * (be - al) * inv;
+                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                cs[sl] = 1.0 / sqrt(1.0 + t * t);
+                sn[sl] = cs[sl] * t;
+                er[sl] = gr * inv;
+                ei[sl] = gi * inv;
+                ra[sl] = a;
+                rb[sl] = b;
             }
-            __syncthreads();
-            // columns: M[:, a] <- c M[:, a] - s conj(e) M[:, b];  M[:, b] <- s e M[:, a] + c M[:, b]
-            for (int e = tid; e < np * c; e += kGbjThreads) {
-                const int g = e / c, k = e % c;
-                const int a = rab[g][0];
-                if (a < 0) continue;
-                const int b = rab[g][1];
-                const double cs = rot[g][0], sn = rot[g][1], er = rot[g][2], ei = rot[g][3];
+            // the reads above complete in every lane before the writes below (one wavefront
+            // per pair), and the compiler keeps them in order
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+#pragma unroll
+            for (int sl = 0; sl < kSlots; ++sl) {
+                const int a = ra[sl];
+                if (a < 0 || k >= c) continue;
+                const int b = rb[sl];
+                any = true;
+                // columns: M[:, a] <- c M[:, a] - s conj(e) M[:, b];  M[:, b] <- s e M[:, a] + c M[:, b]
                 double2 p = G[k][a], q = G[k][b];
-                G[k][a] = double2{cs * p.x - sn * (er * q.x + ei * q.y), cs * p.y - sn * (er * q.y - ei * q.x)};
-                G[k][b] = double2{sn * (er * p.x - ei * p.y) + cs * q.x, sn * (er * p.y + ei * p.x) + cs * q.y};
+                G[k][a] = double2{cs[sl] * p.x - sn[sl] * (er[sl] * q.x + ei[sl] * q.y),
+                                  cs[sl] * p.y - sn[sl] * (er[sl] * q.y - ei[sl] * q.x)};
+                G[k][b] = double2{sn[sl] * (er[sl] * p.x - ei[sl] * p.y) + cs[sl] * q.x,
+                                  sn[sl] * (er[sl] * p.y + ei[sl] * p.x) + cs[sl] * q.y};
                 p = Z[k][a];
                 q = Z[k][b];
-                Z[k][a] = double2{cs * p.x - sn * (er * q.x + ei * q.y), cs * p.y - sn * (er * q.y - ei * q.x)};
-                Z[k][b] = double2{sn * (er * p.x - ei * p.y) + cs * q.x, sn * (er * p.y + ei * p.x) + cs * q.y};
+                Z[k][a] = double2{cs[sl] * p.x - sn[sl] * (er[sl] * q.x + ei[sl] * q.y),
+                                  cs[sl] * p.y - sn[sl] * (er[sl] * q.y - ei[sl] * q.x)};
+                Z[k][b] = double2{sn[sl] * (er[sl] * p.x - ei[sl] * p.y) + cs[sl] * q.x,
+                                  sn[sl] * (er[sl] * p.y + ei[sl] * p.x) + cs[sl] * q.y};
             }
             __syncthreads();
             // rows: M[a, :] <- c M[a, :] - s e M[b, :];  M[b, :] <- s conj(e) M[a, :] + c M[b, :]
-            for (int e = tid; e < np * c; e += kGbjThreads) {
-                const int g = e / c, k = e % c;
-                const int a = rab[g][0];
-                if (a < 0) continue;
-                const int b = rab[g][1];
-                const double cs = rot[g][0], sn = rot[g][1], er = rot[g][2], ei = rot[g][3];
+#pragma unroll
+            for (int sl = 0; sl < kSlots; ++sl) {
+                const int a = ra[sl];
+                if (a < 0 || k >= c) continue;
+                const int b = rb[sl];
                 const double2 p = G[a][k], q = G[b][k];
-                G[a][k] = double2{cs * p.x - sn * (er * q.x - ei * q.y), cs * p.y - sn * (er * q.y + ei * q.x)};
-                G[b][k] = double2{sn * (er * p.x + ei * p.y) + cs * q.x, sn * (er * p.y - ei * p.x) + cs * q.y};
+                G[a][k] = double2{cs[sl] * p.x - sn[sl] * (er[sl] * q.x - ei[sl] * q.y),
+                                  cs[sl] * p.y - sn[sl] * (er[sl] * q.y + ei[sl] * q.x)};
+                G[b][k] = double2{sn[sl] * (er[sl] * p.x + ei[sl] * p.y) + cs[sl] * q.x,
+                                  sn[sl] * (er[sl] * p.y - ei[sl] * p.x) + cs[sl] * q.y};
             }
             __syncthreads();
         }
     }
+    if (any) s_rot = 1;
+    __syncthreads();
     if (!s_rot) return;                       // uniform: read after the last barrier
     auto gcol = [&](int j) { return j < cp ? bp * w + j : bq * w + (j - cp); };
-    // M[rows, cols] <- M[rows, cols] Z for M = X, then V, half a chunk of rows at a time
+    // M[rows, cols] <- M[rows, cols] Z for M = X, then V, half a chunk of rows at a time; each
+    // thread forms a 2 x 2 tile of the result (two rows, two columns), so every LDS value it
+    // loads feeds two products
     constexpr int H = kGbjRows / 2;
+    static_assert((H / 2) * (kGbjCols / 2) == kGbjThreads, "one 2 x 2 tile per thread");
+    const int ti = (tid % (H / 2)) * 2, tj = (tid / (H / 2)) * 2;   // tile origin
     for (int pass = 0; pass < 4; ++pass) {
         double2* m = (pass < 2 ? X + mt.x_off : V + mt.v_off);
         const int h0 = (pass & 1) * H;
@@ -1599,16 +1620,25 @@ __global__ __launch_bounds__(kGbjThreads) void gbj_rotate_kernel(const BjMat* __
             if (i < hr) xs[i][j] = m[(int64_t)gcol(j) * n + r0 + h0 + i];
         }
         __syncthreads();
-        for (int e = tid; e < c * H; e += kGbjThreads) {
-            const int j = e / H, i = e % H;
-            if (i >= hr) continue;
-            double sr = 0.0, si = 0.0;
-            for (int k = 0; k < c; ++k) {
-                const double2 p = xs[i][k], z = Z[k][j];
-                sr += p.x * z.x - p.y * z.y;
-                si += p.x * z.y + p.y * z.x;
+        if (tj < c) {
+            double2 o00{0.0, 0.0}, o01{0.0, 0.0}, o10{0.0, 0.0}, o11{0.0, 0.0};
+            const bool j1 = tj + 1 < c;
+            for (int kk = 0; kk < c; ++kk) {
+                const double2 p0 = xs[ti][kk], p1 = xs[ti + 1][kk];
+                const double2 z0 = Z[kk][tj], z1 = j1 ? Z[kk][tj + 1] : double2{0.0, 0.0};
+                o00.x += p0.x * z0.x - p0.y * z0.y;  o00.y += p0.x * z0.y + p0.y * z0.x;
+                o01.x += p0.x * z1.x - p0.y * z1.y;  o01.y += p0.x * z1.y + p0.y * z1.x;
+                o10.x += p1.x * z0.x - p1.y * z0.y;  o10.y += p1.x * z0.y + p1.y * z0.x;
+                o11.x += p1.x * z1.x - p1.y * z1.y;  o11.y += p1.x * z1.y + p1.y * z1.x;
             }
-            m[(int64_t)gcol(j) * n + r0 + h0 + i] = double2{sr, si};
+            double2* c0 = m + (int64_t)gcol(tj) * n + r0 + h0;
+            if (ti < hr) c0[ti] = o00;
+            if (ti + 1 < hr) c0[ti + 1] = o10;
+            if (j1) {
+                double2* c1 = m + (int64_t)gcol(tj + 1) * n + r0 + h0;
+                if (ti < hr) c1[ti] = o01;
+                if (ti + 1 < hr) c1[ti + 1] = o11;
+            }
         }
         __syncthreads();
     }
