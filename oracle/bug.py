@@ -175,6 +175,11 @@ class StepOptions:
     krylov_mode: str = "auto"
     seed: int = 0x5EED
     parallel: bool = False
+    eshift: float = 0.0                 # the step integrates H - eshift (bug_step.jl:88-91)
+    closure_checks: bool = False        # measured closure loop (krylov_closure_loop.jl:18-194)
+    closure_probe: int = 0              # order 1: measure frontiers 0..m-1 (bug_step.jl:33-38)
+    root_report: bool = False           # the root's account (bug_step_stages.jl:242-398)
+    outside_residual: bool = False      # with root_report: measure and re-enrich (I - P) H psi_1
 
 
 @dataclass
@@ -212,6 +217,10 @@ class Ledger:
     r_final: int = 0
     leak: List[float] = field(default_factory=list)
     defect: float = 0.0
+    leak_initial: List[float] = field(default_factory=list)
+    rho_initial: List[float] = field(default_factory=list)
+    n_passes: int = 0
+    n_root_frontier: int = 0
     unmet: List[str] = field(default_factory=list)
     bond_log: List[dict] = field(default_factory=list)   # half-sweep bond visits, 0-based bonds
 
@@ -1036,6 +1045,7 @@ class StepInfo:
     root_rank: int
     discarded: float
     n_root: int
+    root_report: Optional[dict] = None
 
 
 def _cap_at(wcap, b):
@@ -1043,9 +1053,11 @@ def _cap_at(wcap, b):
 
 
 def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None,
-                 probe_hook=None) -> StepInfo:
+                 probe_hook=None, rng: Optional[la.ProbeRNG] = None) -> StepInfo:
     """One order-p CBE-BUG step in place (bug_step.jl:17-510); order > 1 closes the root
-    frontiers by one enriching sweep each (oracle/closure.py)."""
+    frontiers by one enriching sweep each (oracle/closure.py), or by the measured loop under
+    ``closure_checks``.  ``rng`` is a probe stream the caller keeps across steps (the reference's
+    ``rng`` keyword); None draws from a fresh stream seeded by ``opts.seed``."""
     opts = opts or StepOptions()
     L = len(psi)
     if len(mpo) != L:
@@ -1062,11 +1074,14 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
     if any(not t >= 0 for t in ktols):
         raise ValueError("keep_tols must be >= 0")
     cap_keep = opts.work_ceiling_rule != "maxdim"
-    if opts.frames_only and opts.truncate:
-        raise ValueError("frames_only needs truncate = False")
+    if opts.frames_only and (opts.truncate or opts.root_report):
+        raise ValueError("frames_only needs root_report off and truncate = False")
+    if opts.closure_probe < 0:
+        raise ValueError("closure_probe must be a non-negative count")
     d = len(psi.sites[0].shifts)
     tau = complex(tau)
-    rng = la.ProbeRNG(opts.seed)
+    es = float(opts.eshift)
+    rng = la.ProbeRNG(opts.seed) if rng is None else rng
     bd0 = psi.bond_dims()
     wcap = work_ceilings(opts.work_ceiling_rule, bd0, d, opts.maxdim, opts.bond_ceiling_factor,
                          rl, opts.root_ceiling_factor)
@@ -1111,8 +1126,8 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
         Kex = la.site_times_core(carried, right_projected)
         rch1 = ob.push_right(renvs[b + 2], ex.V_ex, mpo[b + 1])
         Hl, Hw, Hr = lch, mpo[b], rch1
-        Wk, kst = krylov_enrich(Kex, lambda v: ob.apply_one_site(Hl, Hw, Hr, v), ex.U_ex, "left",
-                                tau, alloc, _cap_at(wcap, b), recurrence)
+        Wk, kst = krylov_enrich(Kex, lambda v: ob.apply_one_site(Hl, Hw, Hr, v, shift=es), ex.U_ex,
+                                "left", tau, alloc, _cap_at(wcap, b), recurrence)
         record_krylov(bl, kst, alloc)
         # half_sweeps.jl:280-296: every kept state unioned before any carry is taken
         kxs = [ks.sites[b] if kc is None else la.core_times_site(kc, ks.sites[b])
@@ -1150,8 +1165,8 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
         Bex = la.core_times_site(left_projected, carried)
         lch1 = ob.push_left(lenvs[b], ex.U_ex, mpo[b])
         Hl, Hw, Hr = lch1, mpo[b + 1], rch
-        Zk, kst = krylov_enrich(Bex, lambda v: ob.apply_one_site(Hl, Hw, Hr, v), ex.V_ex, "right",
-                                tau, alloc, _cap_at(wcap, b), recurrence)
+        Zk, kst = krylov_enrich(Bex, lambda v: ob.apply_one_site(Hl, Hw, Hr, v, shift=es), ex.V_ex,
+                                "right", tau, alloc, _cap_at(wcap, b), recurrence)
         record_krylov(bl, kst, alloc)
         bxs = [ks.sites[b + 1] if kc is None else la.site_times_core(ks.sites[b + 1], kc)
                for ks, kc in zip(keeps, kcar)]                       # half_sweeps.jl:384-395
@@ -1182,16 +1197,26 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
     root_start = la.product(AL.row, AR.col, [0], AL, AR, [(0, 0, 0, la.OP_N, la.OP_N, 1.0)])
 
     # ---- order p > 1: close the root frontiers j = 1..p-1 (bug_step_stages.jl:172-240) ----
+    from .closure import KLFrames
     if opts.order > 1:
-        from .closure import KLFrames, single_pass_closure
-        fr = KLFrames(left_frames, right_frames, rl, lroot, rroot)
-        root_start, _ = single_pass_closure(fr, psi, mpo, root_start, tau, alloc, opts, wcap, nrm,
-                                            rng, recurrence, ledger, opts.order)
+        from .closure import checked_closure, single_pass_closure
+        fr = KLFrames(left_frames, right_frames, rl, lroot, rroot, eshift=es)
+        close = checked_closure if opts.closure_checks else single_pass_closure
+        root_start, _ = close(fr, psi, mpo, root_start, tau, alloc, opts, wcap, nrm, rng,
+                              recurrence, ledger, opts.order)
         lroot, rroot = fr.lenv, fr.renv
-        for b in range(rl):
-            expanded[b] = sum(left_frames[b].col.values())
-        for b in range(rl + 1, L - 1):
-            expanded[b] = sum(right_frames[b].row.values())
+        _refresh_expanded(expanded, left_frames, right_frames, rl)
+    elif opts.closure_probe > 0:
+        # measure only, on a copy of the stream: the step itself is unchanged
+        import copy
+        from .closure import _merge, measure_frontiers
+        fr = KLFrames(left_frames, right_frames, rl, lroot, rroot, eshift=es)
+        *_, plg, _ = measure_frontiers(fr, psi, mpo, root_start, tau, alloc, opts, wcap, nrm,
+                                       copy.copy(rng), recurrence, opts.closure_probe)
+        _merge(ledger, plg)
+        ledger.leak_initial = list(ledger.leak_initial) + list(plg.leak_initial)
+        ledger.rho_initial = list(ledger.rho_initial) + list(plg.rho_initial)
+        ledger.n_root_frontier += plg.n_root_frontier
     npsi = psi.sites[psi.center].norm()
     ledger.r_aug = max(max(expanded + [0]), sum(right_frames[rl].row.values()))
     ledger.proj_err = projection_miss(psi, left_frames, right_frames, rl) / npsi if npsi > 0 else 0.0
@@ -1202,23 +1227,30 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
     Lr = ob.push_left(lroot, left_frames[rl], mpo[rl])
     Rr = ob.push_right(rroot, right_frames[rl], mpo[rl + 1])
     napply = [0]
-
-    def apply_root(C):
-        napply[0] += 1
-        return ob.apply_zero_site(Lr, Rr, C)
-
     vec_ops = dict(inner=ob.inner, norm=lambda x: x.norm(), axpy=lambda a, x, y: y.axpy(a, x),
                    scale=lambda a, x: x.scale(a))
+
+    def solve_with(Lr, Rr, start):
+        def apply_root(C):
+            napply[0] += 1
+            return ob.apply_zero_site(Lr, Rr, C, shift=es)
+        if opts.hermitian:
+            return ok.lanczos_expv(apply_root, tau, start, tol=1e-15, maxiter=ok.ROOT_MAXITER,
+                                   reorth=opts.reorth, conv_tol=opts.tol_expv, substeps=0,
+                                   **vec_ops)
+        return ok.arnoldi_expv(apply_root, tau, start, tol=1e-15, maxiter=ok.ROOT_MAXITER,
+                               conv_tol=opts.tol_expv, substeps=0, **vec_ops)
+
     if opts.frames_only:
         result = root_start
-    elif opts.hermitian:
-        result, rep = ok.lanczos_expv(apply_root, tau, root_start, tol=1e-15,
-                                      maxiter=ok.ROOT_MAXITER, reorth=opts.reorth,
-                                      conv_tol=opts.tol_expv, substeps=0, **vec_ops)
     else:
-        result, rep = ok.arnoldi_expv(apply_root, tau, root_start, tol=1e-15,
-                                      maxiter=ok.ROOT_MAXITER, conv_tol=opts.tol_expv,
-                                      substeps=0, **vec_ops)
+        result, rep = solve_with(Lr, Rr, root_start)
+    report = None
+    if opts.root_report:
+        fr = KLFrames(left_frames, right_frames, rl, lroot, rroot, eshift=es)
+        root_start, result, rep, report = report_root(
+            psi, mpo, fr, Lr, Rr, root_start, result, rep, solve_with, tau, alloc, opts, wcap,
+            nrm, rng, recurrence, ledger, expanded)
     if not opts.frames_only:
         ledger.expv_err = rep.err
         ledger.expv_iters = rep.iterations
@@ -1250,7 +1282,77 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
     ledger.r_final = max(bd1 + [0])
     for e in ledger.bond_log:            # diagnostics.jl:328-335
         e.update(r_old=bd0[e["bond"]], r_final=bd1[e["bond"]], discarded=bwts[e["bond"]])
-    return StepInfo(ledger, expanded, root_rank, discarded, napply[0])
+    return StepInfo(ledger, expanded, root_rank, discarded, napply[0], report)
+
+
+def _refresh_expanded(expanded, left_frames, right_frames, rl):
+    """``_refresh_expanded!``: the frame widths off the root bond."""
+    for b in range(rl):
+        expanded[b] = sum(left_frames[b].col.values())
+    for b in range(rl + 1, len(expanded)):
+        expanded[b] = sum(right_frames[b].row.values())
+
+
+def report_root(psi, mpo, fr, Lr, Rr, root_start, result, rep, solve_with, tau, alloc, opts, wcap,
+                nrm, rng, recurrence, ledger, expanded):
+    """``_report_root!`` (bug_step_stages.jl:242-398): the energies before and after the root
+    solve (Rayleigh quotients in the Galerkin space plus eshift, exact energies of the state
+    since it lies in the frames' span) and, with ``outside_residual``, the outside-Galerkin
+    defect |tau| ||(I - P) H psi_1|| / ||psi_1|| of the evolved root vector
+    (``complement_action``).  While the defect exceeds tol_basis the frames are re-enriched by
+    the evolved vector's own frontier sweep and the root re-solved (at most 2p+2 passes, stopping
+    when a pass adds nothing or binds the work ceiling).  Returns the possibly re-solved
+    root_start, result and solve report, and the report."""
+    from .closure import closure_maxpass, complement_action, frontier_sweep, project_root
+    es = fr.eshift
+
+    def rayleigh(S):
+        n2 = S.norm() ** 2
+        if not n2 > 0:
+            return float("nan")
+        return float(ob.inner(S, ob.apply_zero_site(Lr, Rr, S, shift=es)).real) / n2 + es
+
+    e_pre = rayleigh(root_start)
+    outside = (complement_action(psi, mpo, fr, result, alloc, opts, wcap, rng, recurrence)
+               if opts.outside_residual else None)
+
+    def resid(act, S):
+        return float("nan") if act is None else abs(tau) * act["bound"] / max(
+            S.norm(), np.finfo(float).eps)
+
+    initial = resid(outside, result)
+    passes = added = 0
+    if opts.outside_residual:
+        while not (resid(outside, result) <= opts.tol_basis) and passes < closure_maxpass(
+                opts.order):
+            passes += 1
+            sr = frontier_sweep(fr, psi, mpo, result, tau, alloc, opts, wcap, nrm, rng, recurrence,
+                                ledger)
+            if sr.added == 0:
+                break
+            added += sr.added
+            root_start = project_root(psi, fr)
+            Lr = ob.push_left(fr.lenv, fr.left[fr.rl], mpo[fr.rl])
+            Rr = ob.push_right(fr.renv, fr.right[fr.rl], mpo[fr.rl + 1])
+            result, rep = solve_with(Lr, Rr, root_start)
+            outside = complement_action(psi, mpo, fr, result, alloc, opts, wcap, rng, recurrence)
+            if sr.capped:
+                break
+        if added > 0:
+            _refresh_expanded(expanded, fr.left, fr.right, fr.rl)
+            ledger.r_aug = max(max(expanded + [0]), sum(fr.right[fr.rl].row.values()))
+    e_root = rayleigh(result)
+    rn = max(result.norm(), np.finfo(float).eps)
+    report = dict(
+        eshift=es, e_pre=e_pre, e_root=e_root, resid_initial=initial, outside_passes=passes,
+        outside_added=added,
+        outside=float("nan") if outside is None else outside["bound"] / rn,
+        outside_left=float("nan") if outside is None else outside["left"] / rn,
+        outside_right=float("nan") if outside is None else outside["right"] / rn,
+        per_left=[] if outside is None else [x / rn for x in outside["per_left"]],
+        per_right=[] if outside is None else [x / rn for x in outside["per_right"]],
+        resid=float("nan") if outside is None else abs(tau) * outside["bound"] / rn)
+    return root_start, result, rep, report
 
 
 def projection_miss(psi: MPS, left_frames, right_frames, rl: int) -> float:

@@ -1,13 +1,23 @@
 """Oracle (test infrastructure only): the order-p closure of the root frontiers, restated in numpy.
 
 Only the tests, the smoke check and the benchmark's CPU baseline use this module, and only as the
-checker.  It follows the default order-p path of the reference step (``closure_checks = false``,
+checker.  The default order-p path of the reference step (``closure_checks = false``,
 ``frontier_tol = 0``, folded sketches): after the two half-sweeps, ONE enriching frontier sweep per
 degree j = 1 .. p-1 grows the frames by the directions the projected Krylov frontier
-F_j = (tau^j / j!) B^j S0 needs outside them.
+F_j = (tau^j / j!) B^j S0 needs outside them.  ``closure_checks = true`` first measures every
+frontier on the incoming frames and loops enriching passes until one adds nothing; measuring
+(non-enriching) sweeps also give the outside-Galerkin action of a root vector.
+
+The reference runs some measurements concurrently when Julia has four or more threads
+(``_frontier_parallel``); this restatement takes the reference's path with fewer threads, where
+every measurement runs in order on the step's own stream.
 
 Reference files (madhavmen/BondUpdateGalerkin):
+  - src/BUG/Core/RankAdaptation/krylov_closure_loop.jl:18-194    _krylov_closure! (checks = true)
   - src/BUG/Core/RankAdaptation/krylov_closure_loop.jl:203-281   _single_pass_closure!
+  - src/BUG/Core/Sweeps/frontier_closure.jl:238-287              estimate_global_complement_action
+  - src/BUG/Core/Sweeps/frontier_closure.jl:289-358              _measure_frontiers
+  - src/BUG/Core/Sweeps/frontier_closure.jl:370-410              _closure_loop!
   - src/BUG/Core/RankAdaptation/krylov_closure_frames.jl         KLFrames, _projected_frontiers,
         _frontier_mps (:331-356), _project_root (:284-290), _root_generator (:292-299),
         _FRONTIER_FLOOR (:9)
@@ -48,6 +58,7 @@ class KLFrames:
     lenv: ob.Fam
     renv: ob.Fam
     gen: int = 0
+    eshift: float = 0.0          # the generators' energy shift: B - eshift, H_eff - eshift
 
 
 @dataclass
@@ -57,13 +68,21 @@ class SweepResult:
     met: bool
     leaks_left: List[float] = field(default_factory=list)
     leaks_right: List[float] = field(default_factory=list)
+    capped: bool = False         # an enrichment of the sweep stopped at the work ceiling
+
+
+def closure_maxpass(order: int) -> int:
+    """``_closure_maxpass`` (krylov_closure_frames.jl:12)."""
+    return 2 * order + 2
 
 
 def root_generator(mpo, fr: KLFrames):
-    """B = P H P at the root (``_root_generator``): two environment pushes, then zero-site applies."""
+    """B = P H P - eshift at the root (``_root_generator``): two environment pushes, then
+    zero-site applies."""
     Lr = ob.push_left(fr.lenv, fr.left[fr.rl], mpo[fr.rl])
     Rr = ob.push_right(fr.renv, fr.right[fr.rl], mpo[fr.rl + 1])
-    return lambda C: ob.apply_zero_site(Lr, Rr, C)
+    es = fr.eshift
+    return lambda C: ob.apply_zero_site(Lr, Rr, C, shift=es)
 
 
 def project_root(psi, fr: KLFrames) -> ob.Fam:
@@ -189,15 +208,14 @@ def exact_leak(fr, mpo, bond: int, lenv, renv, side: str) -> float:
     return la.perp_right(fr.V0, Y).norm()
 
 
-def _leak(tsf, mpo, bond, lenv, renv, side, width, rng, opts) -> float:
-    from .bug import resolved_fold
-    if resolved_fold(opts):
+def _leak(tsf, mpo, bond, lenv, renv, side, width, rng, fold: bool) -> float:
+    if fold:
         return sampled_leak(tsf, mpo, bond, lenv, renv, side, max(8, -(-width // 4)), rng)
     return exact_leak(tsf, mpo, bond, lenv, renv, side)
 
 
 def frontier_left(Fm, mpo, fr: KLFrames, tau, alloc, opts, enrich: bool, wcap, amp: float, rng,
-                  recurrence: str):
+                  recurrence: str, fold: bool):
     """``_frontier_left`` (frontier_sweeps.jl:46-199), enriching, folded leaks."""
     from .bug import (Ledger, _cap_at, canonical, dd_cbe_expand, krylov_enrich, left_boundary,
                       record_krylov, right_boundary)
@@ -220,7 +238,7 @@ def frontier_left(Fm, mpo, fr: KLFrames, tau, alloc, opts, enrich: bool, wcap, a
         tsf = frame_onto_left(old, site_core, Fm.sites[b + 1])
         rch = renv[b + 2]
         width = sum(old.col.values())
-        leak[b] = _leak(tsf, mpo, b, lch, rch, "left", width, rng, opts)
+        leak[b] = _leak(tsf, mpo, b, lch, rch, "left", width, rng, fold)
         quiet = alloc.scale * leak[b] <= alloc.per_bond
         if enrich and not quiet:
             bl = Ledger()
@@ -228,8 +246,9 @@ def frontier_left(Fm, mpo, fr: KLFrames, tau, alloc, opts, enrich: bool, wcap, a
                                rng)
             Kex = la.site_times_core(site_core, la.overlap_right(Fm.sites[b + 1], ex.V_ex))
             rch1 = ob.push_right(rch, ex.V_ex, mpo[b + 1])
-            Hl, Hw = lch, mpo[b]
-            Wi, st = krylov_enrich(Kex, lambda v: ob.apply_one_site(Hl, Hw, rch1, v), ex.U_ex,
+            Hl, Hw, es = lch, mpo[b], fr.eshift
+            Wi, st = krylov_enrich(Kex, lambda v: ob.apply_one_site(Hl, Hw, rch1, v, shift=es),
+                                   ex.U_ex,
                                    "left", tau, alloc, _cap_at(wcap, b), recurrence, amp=amp,
                                    state_seed=False)
             record_krylov(bl, st, alloc)
@@ -243,13 +262,14 @@ def frontier_left(Fm, mpo, fr: KLFrames, tau, alloc, opts, enrich: bool, wcap, a
         carry = la.overlap_left(Wn[b], site_core)
         if b == rl:
             lenv = lch
-        elif enrich:
+        else:
+            # a measuring sweep reads the frames' own environment chain (frontier_sweeps.jl:104)
             lch = ob.push_left(lch, Wn[b], mpo[b])
     return Wn, lenv, leak, added, lg
 
 
 def frontier_right(Fm, mpo, fr: KLFrames, tau, alloc, opts, enrich: bool, wcap, amp: float, rng,
-                   recurrence: str):
+                   recurrence: str, fold: bool):
     """``_frontier_right`` (frontier_sweeps.jl:201-354), the mirror of ``frontier_left``."""
     from .bug import (Ledger, _cap_at, canonical, dd_cbe_expand, krylov_enrich, left_boundary,
                       record_krylov, right_boundary)
@@ -271,7 +291,7 @@ def frontier_right(Fm, mpo, fr: KLFrames, tau, alloc, opts, enrich: bool, wcap, 
         tsf = frame_onto_right(Fm.sites[b], site_core, old)
         lch = lenvs[b]
         width = sum(old.row.values())
-        leak[b] = _leak(tsf, mpo, b, lch, rch, "right", width, rng, opts)
+        leak[b] = _leak(tsf, mpo, b, lch, rch, "right", width, rng, fold)
         quiet = alloc.scale * leak[b] <= alloc.per_bond
         if enrich and not quiet:
             bl = Ledger()
@@ -279,8 +299,9 @@ def frontier_right(Fm, mpo, fr: KLFrames, tau, alloc, opts, enrich: bool, wcap, 
                                rng)
             Bex = la.core_times_site(la.overlap_left(ex.U_ex, Fm.sites[b]), site_core)
             lch1 = ob.push_left(lch, ex.U_ex, mpo[b])
-            Hw, Hr = mpo[b + 1], rch
-            Zj, st = krylov_enrich(Bex, lambda v: ob.apply_one_site(lch1, Hw, Hr, v), ex.V_ex,
+            Hw, Hr, es = mpo[b + 1], rch, fr.eshift
+            Zj, st = krylov_enrich(Bex, lambda v: ob.apply_one_site(lch1, Hw, Hr, v, shift=es),
+                                   ex.V_ex,
                                    "right", tau, alloc, _cap_at(wcap, b), recurrence, amp=amp,
                                    state_seed=False)
             record_krylov(bl, st, alloc)
@@ -294,20 +315,24 @@ def frontier_right(Fm, mpo, fr: KLFrames, tau, alloc, opts, enrich: bool, wcap, 
         carry = la.overlap_right(site_core, Zn[b])
         if b == rl:
             renv = rch
-        elif enrich:
+        else:
             rch = ob.push_right(rch, Zn[b], mpo[b + 1])
     return Zn, renv, leak, added, lg
 
 
 def frontier_sweep(fr: KLFrames, psi, mpo, core: ob.Fam, tau, alloc, opts, wcap, nrm: float, rng,
-                   recurrence: str, ledger, enrich: bool = True) -> SweepResult:
+                   recurrence: str, ledger, enrich: bool = True,
+                   fold: Optional[bool] = None) -> SweepResult:
     """``_frontier_sweep!`` (frontier_closure.jl:42-236): both half-sweeps of one frontier, the
-    frames replaced (and their generation advanced) only if something was added."""
+    frames replaced (and their generation advanced) only if something was added; ``enrich =
+    False`` only measures.  ``fold`` None takes the step's (``resolved_fold``)."""
+    from .bug import resolved_fold
+    fold = resolved_fold(opts) if fold is None else fold
     L, rl = len(psi), fr.rl
     nF = core.norm()
     eps = np.finfo(float).eps
     if nF <= FRONTIER_FLOOR * max(nrm, eps):
-        return SweepResult(0, 0.0, True)
+        return SweepResult(0, 0.0, True, [0.0] * (rl + 1), [0.0] * (L - 1))
     Fm = frontier_mps(psi, fr, core)
     FmR = Fm.copy()
     amp = nF / max(nrm, eps)
@@ -315,9 +340,9 @@ def frontier_sweep(fr: KLFrames, psi, mpo, core: ob.Fam, tau, alloc, opts, wcap,
     # (frontier_closure.jl:125); run one after the other here, they share nothing else
     rngR = rng.split() if opts.parallel else rng
     Wn, lenv, leak_l, add_l, lg_l = frontier_left(Fm, mpo, fr, tau, alloc, opts, enrich, wcap,
-                                                  amp, rng, recurrence)
+                                                  amp, rng, recurrence, fold)
     Zn, renv, leak_r, add_r, lg_r = frontier_right(FmR, mpo, fr, tau, alloc, opts, enrich, wcap,
-                                                   amp, rngR, recurrence)
+                                                   amp, rngR, recurrence, fold)
     from .bug import Ledger
     lg = Ledger()
     _merge(lg, lg_l)
@@ -335,7 +360,7 @@ def frontier_sweep(fr: KLFrames, psi, mpo, core: ob.Fam, tau, alloc, opts, wcap,
         for b in range(rl, L - 1):
             fr.right[b] = Zn[b]
         fr.lenv, fr.renv, fr.gen = lenv, renv, fr.gen + 1
-    return SweepResult(added, leak, met, leak_l, leak_r)
+    return SweepResult(added, leak, met, leak_l, leak_r, enrich and lg.workcap_reached)
 
 
 def single_pass_closure(fr: KLFrames, psi, mpo, root_start, tau, alloc, opts, wcap, nrm: float,
@@ -367,3 +392,144 @@ def single_pass_closure(fr: KLFrames, psi, mpo, root_start, tau, alloc, opts, wc
     if not met:
         ledger.unmet_("closure")
     return current, napply[0]
+
+
+def measure_frontiers(fr: KLFrames, psi, mpo, root_start, tau, alloc, opts, wcap, nrm: float, rng,
+                      recurrence: str, count: int):
+    """``_measure_frontiers`` (frontier_closure.jl:289-358): non-enriching sweeps of the frontiers
+    j = 0 .. count-1 of ``root_start`` on the frames (left untouched).  Returns (leak, rho, met,
+    met0, ledger, n_root): the time-weighted leaks, the scale-free ||(I - P) A F_j|| / ||F_j||,
+    whether every frontier j >= 1 met its shares, the j = 0 verdict."""
+    from .bug import Ledger
+    napply = [0]
+    gen = root_generator(mpo, fr)
+
+    def apply_B(C):
+        napply[0] += 1
+        return gen(C)
+
+    cores = [root_start] + (projected_frontiers(apply_B, root_start, tau, count - 1)
+                            if count > 1 else [])
+    lg = Ledger()
+    leak, rho = [0.0] * count, [0.0] * count
+    met, met0 = True, True
+    for j in range(count):
+        r = frontier_sweep(fr, psi, mpo, cores[j], tau, alloc, opts, wcap, nrm, rng, recurrence,
+                           lg, enrich=False)
+        nF = cores[j].norm()
+        raw = math.sqrt(sum(x * x for x in r.leaks_left) + sum(x * x for x in r.leaks_right))
+        leak[j] = r.leak
+        rho[j] = raw / nF if nF > 0 else 0.0
+        if j >= 1:
+            met &= r.met
+        else:
+            met0 = r.met
+    lg.leak_initial, lg.rho_initial = list(leak), list(rho)
+    lg.n_root_frontier = napply[0]
+    return leak, rho, met, met0, lg, napply[0]
+
+
+def closure_loop(frontier, sweep, order: int, maxpass: int, include_zero: bool = True):
+    """``_closure_loop!`` (frontier_closure.jl:370-410): pass 1 sweeps degrees 0..p-1 (1..p-1
+    without degree 0), later passes 1..p-1; converged when a pass adds nothing and meets every
+    share, stopped as capped when a pass misses a share with a sweep at the work ceiling.
+    Returns (passes, converged, exhausted, capped)."""
+    later = list(range(1, order))
+    initial = list(range(order)) if include_zero else later
+    if not initial:
+        return 0, True, False, False
+    ok0 = True
+    for p in range(1, maxpass + 1):
+        grew, allmet, capped = False, True, False
+        for degree in (initial if p == 1 else later):
+            r = sweep(frontier(degree), degree, p)
+            grew |= r.added > 0
+            allmet &= r.met
+            capped |= r.capped
+            if degree == 0:
+                ok0 = r.met
+        if not grew or not later:
+            return p, allmet and ok0, False, False
+        if capped and not allmet:
+            return p, False, False, True
+    return maxpass, False, True, False
+
+
+def checked_closure(fr: KLFrames, psi, mpo, root_start, tau, alloc, opts, wcap, nrm: float, rng,
+                    recurrence: str, ledger, order: int, include_zero: bool = False):
+    """``_krylov_closure!`` with ``checks = true`` (krylov_closure_loop.jl:18-194): measure every
+    frontier on the incoming frames first; if they all close, the frames stay as the half-sweeps
+    built them.  Otherwise loop enriching passes (``closure_loop``) and measure degree 0 on the
+    final frames.  Returns the root coefficients in the final frames and the root applications."""
+    from .bug import Ledger
+    lg = Ledger()
+    pleak, _, pmet, pmet0, plg, pn = measure_frontiers(fr, psi, mpo, root_start, tau, alloc, opts,
+                                                       wcap, nrm, rng, recurrence, order)
+    _merge(lg, plg)
+    lg.leak_initial, lg.rho_initial = plg.leak_initial, plg.rho_initial
+    napply = [pn]
+    current = [root_start]
+    final_leak = {j: pleak[j] for j in range(order)}
+    gen0 = fr.gen
+    if pmet and (pmet0 or not include_zero):
+        converged = True
+        passes = 0
+    else:
+        op = [root_generator(mpo, fr)]
+
+        def apply_B(C):
+            napply[0] += 1
+            return op[0](C)
+
+        def frontier(degree):
+            if degree == 0:
+                return current[0]
+            return projected_frontiers(apply_B, current[0], tau, degree)[degree - 1]
+
+        def sweep(core, degree, p):
+            r = frontier_sweep(fr, psi, mpo, core, tau, alloc, opts, wcap, nrm, rng, recurrence,
+                               lg)
+            final_leak[degree] = r.leak
+            if r.added > 0:
+                op[0] = root_generator(mpo, fr)
+                current[0] = project_root(psi, fr)
+            return r
+
+        passes, converged, _, _ = closure_loop(frontier, sweep, order, closure_maxpass(order),
+                                               include_zero)
+        if fr.gen != gen0:
+            r0 = frontier_sweep(fr, psi, mpo, current[0], tau, alloc, opts, wcap, nrm, rng,
+                                recurrence, lg, enrich=False)
+            final_leak[0] = r0.leak
+    lg.n_passes = passes
+    lg.leak = [final_leak[j] for j in range(order)]
+    lg.defect = sum(lg.leak[j] / (j + 1) for j in range(order))
+    lg.n_root_frontier = napply[0]
+    if not converged:
+        lg.unmet_("closure")
+    _merge(ledger, lg)
+    ledger.leak = list(lg.leak)
+    ledger.defect = max(ledger.defect, lg.defect)
+    ledger.leak_initial = list(ledger.leak_initial) + list(lg.leak_initial)
+    ledger.rho_initial = list(ledger.rho_initial) + list(lg.rho_initial)
+    ledger.n_passes += lg.n_passes
+    return current[0], napply[0]
+
+
+def complement_action(psi, mpo, fr: KLFrames, core: ob.Fam, alloc, opts, wcap, rng,
+                      recurrence: str):
+    """``estimate_global_complement_action`` (frontier_closure.jl:238-287): the outside-Galerkin
+    action (I - P) H v of v = (W (x) Z) core from one measuring sweep with exact (unfolded) bond
+    actions.  For a nearest-neighbour H the per-bond pieces are orthogonal, so
+    max(left, right) <= ||(I - P) H v|| <= sqrt(left^2 + right^2) = bound.  Returns a dict with
+    left, right, bound and the per-bond actions."""
+    from .bug import Ledger
+    L, rl = len(psi), fr.rl
+    eps = np.finfo(float).eps
+    r = frontier_sweep(fr, psi, mpo, core, 0j, alloc, opts, wcap, max(core.norm(), eps), rng,
+                       recurrence, Ledger(), enrich=False, fold=False)
+    pl, pr = list(r.leaks_left), list(r.leaks_right[rl:L - 1])
+    left = math.sqrt(sum(x * x for x in pl))
+    right = math.sqrt(sum(x * x for x in pr))
+    return dict(left=left, right=right, bound=math.sqrt(left * left + right * right),
+                per_left=pl, per_right=pr)
