@@ -108,6 +108,12 @@ class AdaptiveLedger:
     defect: float = 0.0
     n_h1_closure: int = 0
     n_closure_sweeps: int = 0
+    # measured closure (closure_checks) and closure_probe: the frontiers' leaks on the incoming
+    # frames (time-weighted and scale-free), the enriching passes, the root applications spent
+    leak_initial: List[float] = field(default_factory=list)
+    rho_initial: List[float] = field(default_factory=list)
+    n_passes: int = 0
+    n_root_frontier: int = 0
     unmet: List[str] = field(default_factory=list)
     # one entry per half-sweep bond visit (0-based bond), written by log_bond; after the
     # compression each entry also holds the bond's r_old, r_final and relative discarded norm
@@ -1111,13 +1117,28 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
                  work_ceiling_rule: str = "bond", bond_ceiling_factor: Optional[float] = None,
                  root_ceiling_factor: float = 1.2,
                  root: Optional[int] = None, reorth: bool = True, truncate: bool = True,
-                 expansion: Optional[DDQRExpansion] = None, rng: Optional[int] = None,
+                 expansion: Optional[DDQRExpansion] = None, rng=None,
                  probe_hook: Optional[Callable] = None, precision: str = "fp64",
-                 parallel: bool = False) -> CBEBugSweepInfo:
+                 parallel: bool = False, eshift: float = 0.0, closure_checks: bool = False,
+                 closure_probe: int = 0, root_report: Optional[dict] = None,
+                 outside_residual: bool = False) -> CBEBugSweepInfo:
     """One order-p step in place (``order > 1`` closes the root frontiers j = 1..p-1 by one
     enriching frontier sweep each, ``closure.py``); ``root`` is the 0-based root bond (the
     reference's ``root`` minus 1),
-    ``rng`` the probe seed (default 0x5EED, fresh per call as the reference's default RNG).
+    ``rng`` the probe seed (default 0x5EED, fresh per call as the reference's default RNG), or a
+    ``ProbeStream`` the caller keeps drawing from across steps.
+
+    ``eshift`` makes the step integrate H - eshift (bug_step.jl:88-91): every local Krylov
+    generator and the root generator are shifted, so the basis weights measure the change of
+    the normalised state; 0 leaves the step unchanged.  ``closure_checks`` (order > 1) measures
+    every frontier on the incoming frames first and loops enriching passes until one adds nothing
+    (krylov_closure_loop.jl:18-194); ``closure_probe = m`` (order 1) only measures frontiers
+    0..m-1 into the ledger's ``leak_initial``/``rho_initial``, on a copy of the probe stream.
+    ``root_report``, a dict, receives the root's account (bug_step_stages.jl:242-398): the
+    energies before and after the root solve and, with ``outside_residual``, the
+    outside-Galerkin defect |tau| ||(I - P) H psi_1|| / ||psi_1|| of the evolved vector, the
+    frames re-enriched by its own frontier sweep and the root re-solved while it exceeds
+    tol_basis.
 
     ``krylov_rank_tol > 0`` ranks the local Krylov directions instead of appending them in chain
     order (krylov_enrichment.jl:174-196,209-211): every direction above the allowance is collected
@@ -1188,8 +1209,11 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     if any(len(k) != L for k in keep_states):
         raise ValueError("every kept state needs the chain's length")
     cap_keep = work_ceiling_rule != "maxdim"
-    if frames_only and truncate:
-        raise ValueError("frames_only needs truncate = False")
+    if frames_only and (truncate or root_report is not None):
+        raise ValueError("frames_only needs root_report = None and truncate = False")
+    if closure_probe < 0:
+        raise ValueError(f"closure_probe must be a non-negative count, got {closure_probe}")
+    eshift = float(eshift)
 
     def keep_room(b, width, tol):
         return wcap[b] - width if cap_keep and tol > 0 else 1 << 60
@@ -1216,7 +1240,7 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     tau = complex(tau)
     d = psi.d
     t0 = time.perf_counter()
-    stream = ProbeStream(0x5EED if rng is None else rng)
+    stream = rng if isinstance(rng, ProbeStream) else ProbeStream(0x5EED if rng is None else rng)
     bd0 = psi.bond_dims()
     if work_ceiling_rule == "bond":
         wcap = bond_work_ceilings(bd0, d, maxdim, factor)
@@ -1268,7 +1292,8 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
             rch1 = ev.push_right(renvs[b + 2], xp.V_ex, mpo[b + 1], precision=cp)
             renvs[b + 2] = None
             Hl, Hw, Hr = lch, mpo[b], rch1
-            Wk, kst = krylov_enrich(Kex, lambda v: ev.apply_one_site(Hl, Hw, Hr, v, precision=cp),
+            Wk, kst = krylov_enrich(Kex, lambda v: ev.apply_one_site(Hl, Hw, Hr, v, shift=eshift,
+                                                                     precision=cp),
                                     xp.U_ex, "left", tau, alloc, wcap[b], recurrence,
                                     hermitian_tol=htol)
             _record_krylov(bl, kst)
@@ -1312,7 +1337,8 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
             lch1 = ev.push_left(lenvs[b], xp.U_ex, mpo[b], precision=cp)
             lenvs[b] = None
             Hl, Hw, Hr = lch1, mpo[b + 1], rch
-            Zk, kst = krylov_enrich(Bex, lambda v: ev.apply_one_site(Hl, Hw, Hr, v, precision=cp),
+            Zk, kst = krylov_enrich(Bex, lambda v: ev.apply_one_site(Hl, Hw, Hr, v, shift=eshift,
+                                                                     precision=cp),
                                     xp.V_ex, "right", tau, alloc, wcap[b], recurrence,
                                     hermitian_tol=htol)
             _record_krylov(bl, kst)
@@ -1355,19 +1381,30 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     root_start = T.core_times_core(AL, AR)
 
     # ---- order p > 1: close the root frontiers j = 1..p-1 (bug_step_stages.jl:172-240) ----
+    from .closure import KLFrames, _Ctx
+    cx = _Ctx(mpo, tau, alloc, wcap, ex, stream, recurrence, cp, htol, fold, exact, parallel,
+              eshift)
     if order > 1:
-        from .closure import KLFrames, _Ctx, single_pass_closure
+        from .closure import checked_closure, single_pass_closure
         ts = time.perf_counter()
         _progress("frontier closure")
-        fr = KLFrames(left_frames, right_frames, rl, lroot, rroot)
-        cx = _Ctx(mpo, tau, alloc, wcap, ex, stream, recurrence, cp, htol, fold, exact, parallel)
-        root_start, _ = single_pass_closure(fr, psi, root_start, cx, nrm, ledger, order)
+        fr = KLFrames(left_frames, right_frames, rl, lroot, rroot, eshift=eshift)
+        close = checked_closure if closure_checks else single_pass_closure
+        root_start, _ = close(fr, psi, root_start, cx, nrm, ledger, order)
         lroot, rroot = fr.lenv, fr.renv
-        for b in range(rl):
-            expanded[b] = left_frames[b].layout.col.total
-        for b in range(rl + 1, L - 1):
-            expanded[b] = right_frames[b].layout.row.total
+        _refresh_expanded(expanded, left_frames, right_frames, rl)
         secs["closure"] = time.perf_counter() - ts
+    elif closure_probe > 0:
+        # measure only, on a copy of the probe stream: the step itself is unchanged
+        import copy
+        from .closure import measure_frontiers
+        fr = KLFrames(left_frames, right_frames, rl, lroot, rroot, eshift=eshift)
+        *_, plg, _ = measure_frontiers(fr, psi, root_start, cx.with_stream(copy.copy(stream)),
+                                       nrm, closure_probe)
+        ledger.merge(plg)
+        ledger.leak_initial = list(ledger.leak_initial) + list(plg.leak_initial)
+        ledger.rho_initial = list(ledger.rho_initial) + list(plg.rho_initial)
+        ledger.n_root_frontier += plg.n_root_frontier
     npsi = T.norm(psi[psi.center])
     ledger.r_aug = max(max(expanded + [0]), right_frames[rl].layout.row.total)
     ledger.proj_err = projection_miss(psi, left_frames, right_frames, rl) / npsi if npsi > 0 else 0.0
@@ -1379,27 +1416,33 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     Lr = ev.push_left(lroot, left_frames[rl], mpo[rl], precision=cp)
     Rr = ev.push_right(rroot, right_frames[rl], mpo[rl + 1], precision=cp)
     napply = [0]
-
-    def apply_root(C):
-        napply[0] += 1
-        return ev.apply_zero_site(Lr, Rr, C, precision=cp)
-
     _progress("root solve")
 
     def substep(n):
         if n % 50 == 0:
             _progress(f"root solve substep {n}")
 
+    def solve_with(Lr, Rr, start):
+        def apply_root(C):
+            napply[0] += 1
+            return ev.apply_zero_site(Lr, Rr, C, shift=eshift, precision=cp)
+        if hermitian:
+            return K.lanczos_expv(apply_root, tau, start, tol=1e-15, maxiter=K.ROOT_MAXITER,
+                                  reorth=reorth, conv_tol=te, substeps=0, on_substep=substep)
+        return K.arnoldi_expv(apply_root, tau, start, tol=1e-15, maxiter=K.ROOT_MAXITER,
+                              conv_tol=te, substeps=0, on_substep=substep)
+
     if frames_only:                      # the caller solves its own root (bug_step.jl:381-382)
         result = root_start
-    elif hermitian:
-        result, rep = K.lanczos_expv(apply_root, tau, root_start, tol=1e-15,
-                                     maxiter=K.ROOT_MAXITER, reorth=reorth, conv_tol=te, substeps=0,
-                                     on_substep=substep)
     else:
-        result, rep = K.arnoldi_expv(apply_root, tau, root_start, tol=1e-15,
-                                     maxiter=K.ROOT_MAXITER, conv_tol=te, substeps=0,
-                                     on_substep=substep)
+        result, rep = solve_with(Lr, Rr, root_start)
+    if root_report is not None:
+        fr = KLFrames(left_frames, right_frames, rl, lroot, rroot, eshift=eshift)
+        root_start, result, rep, report = report_root(
+            psi, fr, Lr, Rr, root_start, result, rep, solve_with, cx, nrm, ledger, expanded,
+            outside_residual, tb, order)
+        root_report.clear()
+        root_report.update(report)
     if not frames_only:
         ledger.expv_err, ledger.expv_iters = rep.err, rep.iterations
         if rep.err > te * (1 + 1e-8):
@@ -1437,6 +1480,75 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     secs["truncation"] = time.perf_counter() - ts
     secs["step"] = time.perf_counter() - t0
     return CBEBugSweepInfo(ledger, expanded, rl, root_rank, discarded, napply[0], secs)
+
+
+def _refresh_expanded(expanded, left_frames, right_frames, rl):
+    """``_refresh_expanded!``: the frame widths off the root bond."""
+    for b in range(rl):
+        expanded[b] = left_frames[b].layout.col.total
+    for b in range(rl + 1, len(expanded)):
+        expanded[b] = right_frames[b].layout.row.total
+
+
+def report_root(psi: SymMPS, fr, Lr, Rr, root_start, result, rep, solve_with, cx, nrm: float,
+                ledger: AdaptiveLedger, expanded, outside_residual: bool, tol_basis: float,
+                order: int):
+    """``_report_root!`` (bug_step_stages.jl:242-398): the energies before and after the root
+    solve (Rayleigh quotients of the shifted root generator plus eshift: exact energies, since the
+    state lies in the frames' span) and, with ``outside_residual``, the outside-Galerkin defect
+    |tau| ||(I - P) H psi_1|| / ||psi_1|| of the evolved root vector.  While the defect exceeds
+    tol_basis the frames are re-enriched by the evolved vector's own frontier sweep and the root
+    is re-solved (at most 2p+2 passes; a pass that adds nothing or binds the work ceiling ends the
+    loop).  Returns the possibly re-solved root start, result and solve report, and the report."""
+    from .closure import closure_maxpass, complement_action, frontier_sweep, project_root
+    es, tau, prec = fr.eshift, cx.tau, cx.prec
+    eps = np.finfo(float).eps
+
+    def rayleigh(S):
+        n2 = T.norm(S) ** 2
+        if not n2 > 0:
+            return float("nan")
+        return float(T.inner(S, ev.apply_zero_site(Lr, Rr, S, shift=es, precision=prec)).real) \
+            / n2 + es
+
+    e_pre = rayleigh(root_start)
+    outside = complement_action(psi, fr, result, cx) if outside_residual else None
+
+    def resid(act, S):
+        return float("nan") if act is None else abs(tau) * act["bound"] / max(T.norm(S), eps)
+
+    initial = resid(outside, result)
+    passes = added = 0
+    if outside_residual:
+        while not (resid(outside, result) <= tol_basis) and passes < closure_maxpass(order):
+            passes += 1
+            sr = frontier_sweep(fr, psi, result, cx, nrm, ledger)
+            if sr.added == 0:
+                break
+            added += sr.added
+            root_start = project_root(psi, fr)
+            Lr = ev.push_left(fr.lenv, fr.left[fr.rl], cx.mpo[fr.rl], precision=prec)
+            Rr = ev.push_right(fr.renv, fr.right[fr.rl], cx.mpo[fr.rl + 1], precision=prec)
+            result, rep = solve_with(Lr, Rr, root_start)
+            outside = complement_action(psi, fr, result, cx)
+            if sr.capped:
+                break
+        if added > 0:
+            _refresh_expanded(expanded, fr.left, fr.right, fr.rl)
+            ledger.r_aug = max(max(expanded + [0]), fr.right[fr.rl].layout.row.total)
+    e_root = rayleigh(result)
+    rn = max(T.norm(result), eps)
+    nan = float("nan")
+    report = dict(
+        eshift=es, e_pre=e_pre, e_root=e_root, resid_initial=initial, outside_passes=passes,
+        outside_added=added,
+        outside=nan if outside is None else outside["bound"] / rn,
+        outside_left=nan if outside is None else outside["left"] / rn,
+        outside_right=nan if outside is None else outside["right"] / rn,
+        per_left=[] if outside is None else [x / rn for x in outside["per_left"]],
+        per_right=[] if outside is None else [x / rn for x in outside["per_right"]],
+        resid=nan if outside is None else abs(tau) * outside["bound"] / rn)
+    return root_start, result, rep, report
 
 
 def projection_miss(psi: SymMPS, left_frames, right_frames, rl: int) -> float:

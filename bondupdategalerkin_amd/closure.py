@@ -5,13 +5,20 @@ the frontiers F_j = (tau^j / j!) B^j S0 (B = P H P at the root) for j = 1 .. p-1
 operator: one enriching frontier sweep per degree measures, bond by bond, the frontier's
 outside-space action with a folded sketch (the two-site action is never formed), grows the bonds
 whose leak exceeds their share by the same DD-QR CBE and local Krylov stage as the half-sweeps,
-and re-orthonormalises every grown frame by its polar factor.  Every contraction is a grouped
+and re-orthonormalises every grown frame by its polar factor.  Under ``closure_checks`` every
+frontier is first measured on the incoming frames (non-enriching sweeps) and enriching passes
+loop until one adds nothing; a measuring sweep with exact bond actions also gives the
+outside-Galerkin action of a root vector (the imaginary-time driver's defect).  Every contraction is a grouped
 block GEMM of the library; factorizations are the batched QR / SVD kernels; the probes are drawn
 on the host from the step's counter-based stream and uploaded.
 
-Reference (madhavmen/BondUpdateGalerkin), the default path (``closure_checks = false``,
-``frontier_tol = 0``, ``fold_omega`` unset so folded for order > 1):
+Reference (madhavmen/BondUpdateGalerkin), ``frontier_tol = 0``, ``fold_omega`` unset so folded
+for order > 1, measurements run in order (the reference's path with fewer than four Julia
+threads, ``_frontier_parallel`` false):
   - _single_pass_closure!      src/BUG/Core/RankAdaptation/krylov_closure_loop.jl:203-281
+  - _krylov_closure! (checks)  src/BUG/Core/RankAdaptation/krylov_closure_loop.jl:18-194
+  - _measure_frontiers, _closure_loop!, estimate_global_complement_action
+                               src/BUG/Core/Sweeps/frontier_closure.jl:238-410
   - KLFrames, _projected_frontiers, _frontier_mps, _project_root, _root_generator,
     _FRONTIER_FLOOR            src/BUG/Core/RankAdaptation/krylov_closure_frames.jl
   - _frontier_sweep!           src/BUG/Core/Sweeps/frontier_closure.jl:42-236
@@ -25,8 +32,9 @@ Sites are 0-based, bond b joins sites b and b+1, ``rl`` is the root bond.
 """
 from __future__ import annotations
 
+import copy
 import math
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from typing import List, Optional
 
 import numpy as np
@@ -51,6 +59,7 @@ class KLFrames:
     lenv: BlockTensor
     renv: BlockTensor
     gen: int = 0
+    eshift: float = 0.0          # the generators' energy shift: B - eshift, H_eff - eshift
 
 
 @dataclass
@@ -58,30 +67,40 @@ class SweepResult:
     added: int
     leak: float
     met: bool
+    leaks_left: List[float] = field(default_factory=list)
+    leaks_right: List[float] = field(default_factory=list)
+    capped: bool = False         # an enrichment of the sweep stopped at the work ceiling
+
+
+def closure_maxpass(order: int) -> int:
+    """``_closure_maxpass`` (krylov_closure_frames.jl:12)."""
+    return 2 * order + 2
 
 
 class _Ctx:
     """What every bond of a frontier sweep shares: the step's settings and the probe stream."""
 
     def __init__(self, mpo, tau, alloc, wcap, ex, stream, recurrence, prec, htol, fold=True,
-                 exact=False, parallel=False):
+                 exact=False, parallel=False, eshift=0.0):
         self.mpo, self.tau, self.alloc, self.wcap = mpo, tau, alloc, wcap
         self.ex, self.stream, self.recurrence = ex, stream, recurrence
         self.prec, self.htol = prec, htol
         self.fold, self.exact = fold, exact
         self.parallel = parallel
+        self.eshift = eshift
 
     def with_stream(self, stream) -> "_Ctx":
         """The same settings drawing from another probe stream."""
         return _Ctx(self.mpo, self.tau, self.alloc, self.wcap, self.ex, stream, self.recurrence,
-                    self.prec, self.htol, self.fold, self.exact, self.parallel)
+                    self.prec, self.htol, self.fold, self.exact, self.parallel, self.eshift)
 
 
 def root_generator(mpo, fr: KLFrames, prec: int):
-    """B = P H P at the root: two environment pushes, then zero-site applies."""
+    """B = P H P - eshift at the root: two environment pushes, then zero-site applies."""
     Lr = ev.push_left(fr.lenv, fr.left[fr.rl], mpo[fr.rl], precision=prec)
     Rr = ev.push_right(fr.renv, fr.right[fr.rl], mpo[fr.rl + 1], precision=prec)
-    return lambda C: ev.apply_zero_site(Lr, Rr, C, precision=prec)
+    es = fr.eshift
+    return lambda C: ev.apply_zero_site(Lr, Rr, C, shift=es, precision=prec)
 
 
 def project_root(psi: SymMPS, fr: KLFrames) -> BlockTensor:
@@ -209,8 +228,8 @@ def exact_leak(fr, cx: _Ctx, bond: int, lenv, renv, side: str) -> float:
     return T.norm(T.perp_right(fr.V0, Y))
 
 
-def _leak(tsf, cx: _Ctx, bond: int, lenv, renv, side: str, width: int) -> float:
-    if cx.fold:
+def _leak(tsf, cx: _Ctx, bond: int, lenv, renv, side: str, width: int, fold: bool) -> float:
+    if fold:
         return sampled_leak(tsf, cx, bond, lenv, renv, side, max(8, -(-width // 4)))
     return exact_leak(tsf, cx, bond, lenv, renv, side)
 
@@ -233,7 +252,9 @@ def _enrich_bond(tsf, cx: _Ctx, bond: int, lch, rch, side: str, Fsite, site_core
         seed = T.core_times_site(T.overlap_left(xp.U_ex, Fsite), site_core)
         Hl, Hw, Hr = ev.push_left(lch, xp.U_ex, cx.mpo[bond], precision=prec), cx.mpo[bond + 1], rch
         start = xp.V_ex
-    frame, st = krylov_enrich(seed, lambda v: ev.apply_one_site(Hl, Hw, Hr, v, precision=prec),
+    es = cx.eshift
+    frame, st = krylov_enrich(seed, lambda v: ev.apply_one_site(Hl, Hw, Hr, v, shift=es,
+                                                              precision=prec),
                               start, side, cx.tau, cx.alloc, cap, cx.recurrence,
                               hermitian_tol=cx.htol, amp=amp, state_seed=False)
     _record_krylov(bl, st)
@@ -242,10 +263,13 @@ def _enrich_bond(tsf, cx: _Ctx, bond: int, lch, rch, side: str, Fsite, site_core
     return frame
 
 
-def frontier_left(Fm: SymMPS, fr: KLFrames, cx: _Ctx, amp: float, ledger):
+def frontier_left(Fm: SymMPS, fr: KLFrames, cx: _Ctx, amp: float, ledger, enrich: bool = True,
+                  fold: Optional[bool] = None):
     """The left half-sweep of one frontier: per bond the old frame carried into the grown basis,
     the folded leak, and where it exceeds the bond's share CBE + Krylov growth and a polar
-    re-orthonormalisation.  Returns (frames, root environment, leaks, added)."""
+    re-orthonormalisation (``enrich = False`` only measures, on the frames as they are).
+    Returns (frames, root environment, leaks, added)."""
+    fold = cx.fold if fold is None else fold
     from .bug_step import canonical
     L, rl, mpo, prec = len(Fm), fr.rl, cx.mpo, cx.prec
     canonical(Fm, 0)
@@ -265,14 +289,17 @@ def frontier_left(Fm: SymMPS, fr: KLFrames, cx: _Ctx, amp: float, ledger):
         tsf = frame_onto_left(old, site_core, Fm[b + 1])
         rch = renv[b + 2]
         width = old.layout.col.total
-        leak[b] = _leak(tsf, cx, b, lch, rch, "left", width)
-        if cx.alloc.scale * leak[b] <= cx.alloc.per_bond:
-            Wi = old
+        leak[b] = _leak(tsf, cx, b, lch, rch, "left", width, fold)
+        if not enrich:
+            Wn[b] = old
         else:
-            Wi = _enrich_bond(tsf, cx, b, lch, rch, "left", Fm[b + 1], site_core, amp, ledger)
-        Wn[b] = polar_frame(Wi, "left")
-        added += Wn[b].layout.col.total - width
-        Pm = T.overlap_left(Wn[b], old)
+            if cx.alloc.scale * leak[b] <= cx.alloc.per_bond:
+                Wi = old
+            else:
+                Wi = _enrich_bond(tsf, cx, b, lch, rch, "left", Fm[b + 1], site_core, amp, ledger)
+            Wn[b] = polar_frame(Wi, "left")
+            added += Wn[b].layout.col.total - width
+            Pm = T.overlap_left(Wn[b], old)
         carry = T.overlap_left(Wn[b], site_core)
         if b == rl:
             lenv = lch
@@ -281,8 +308,10 @@ def frontier_left(Fm: SymMPS, fr: KLFrames, cx: _Ctx, amp: float, ledger):
     return Wn, lenv, leak, added
 
 
-def frontier_right(Fm: SymMPS, fr: KLFrames, cx: _Ctx, amp: float, ledger):
+def frontier_right(Fm: SymMPS, fr: KLFrames, cx: _Ctx, amp: float, ledger, enrich: bool = True,
+                   fold: Optional[bool] = None):
     """The mirror of ``frontier_left`` over bonds L-2 .. rl."""
+    fold = cx.fold if fold is None else fold
     from .bug_step import canonical
     L, rl, mpo, prec = len(Fm), fr.rl, cx.mpo, cx.prec
     canonical(Fm, L - 1)
@@ -301,14 +330,17 @@ def frontier_right(Fm: SymMPS, fr: KLFrames, cx: _Ctx, amp: float, ledger):
         tsf = frame_onto_right(Fm[b], site_core, old)
         lch = lenvs[b]
         width = old.layout.row.total
-        leak[b] = _leak(tsf, cx, b, lch, rch, "right", width)
-        if cx.alloc.scale * leak[b] <= cx.alloc.per_bond:
-            Zj = old
+        leak[b] = _leak(tsf, cx, b, lch, rch, "right", width, fold)
+        if not enrich:
+            Zn[b] = old
         else:
-            Zj = _enrich_bond(tsf, cx, b, lch, rch, "right", Fm[b], site_core, amp, ledger)
-        Zn[b] = polar_frame(Zj, "right")
-        added += Zn[b].layout.row.total - width
-        Pm = T.overlap_right(old, Zn[b])
+            if cx.alloc.scale * leak[b] <= cx.alloc.per_bond:
+                Zj = old
+            else:
+                Zj = _enrich_bond(tsf, cx, b, lch, rch, "right", Fm[b], site_core, amp, ledger)
+            Zn[b] = polar_frame(Zj, "right")
+            added += Zn[b].layout.row.total - width
+            Pm = T.overlap_right(old, Zn[b])
         carry = T.overlap_right(site_core, Zn[b])
         if b == rl:
             renv = rch
@@ -318,15 +350,16 @@ def frontier_right(Fm: SymMPS, fr: KLFrames, cx: _Ctx, amp: float, ledger):
 
 
 def frontier_sweep(fr: KLFrames, psi: SymMPS, core: BlockTensor, cx: _Ctx, nrm: float,
-                   ledger) -> SweepResult:
-    """One enriching sweep of a frontier on both sides; the frames are replaced (and their
-    generation advanced) only if something was added."""
+                   ledger, enrich: bool = True, fold: Optional[bool] = None) -> SweepResult:
+    """One sweep of a frontier on both sides; enriching, the frames are replaced (and their
+    generation advanced) only if something was added; ``enrich = False`` only measures.
+    ``fold`` None takes the step's leak form."""
     from .bug_step import AdaptiveLedger
     L, rl = len(psi), fr.rl
     nF = T.norm(core)
     eps = np.finfo(float).eps
     if nF <= FRONTIER_FLOOR * max(nrm, eps):
-        return SweepResult(0, 0.0, True)
+        return SweepResult(0, 0.0, True, [0.0] * (rl + 1), [0.0] * (L - 1))
     Fm = frontier_mps(psi, fr, core)
     FmR = Fm.copy()
     amp = nF / max(nrm, eps)
@@ -337,25 +370,28 @@ def frontier_sweep(fr: KLFrames, psi: SymMPS, core: BlockTensor, cx: _Ctx, nrm: 
         from .bug_step import _run_beside
         cxR = cx.with_stream(cx.stream.split())
         (Wn, lenv, leak_l, add_l), (Zn, renv, leak_r, add_r) = _run_beside(
-            lambda: frontier_left(Fm, fr, cx, amp, lg_l),
-            lambda: frontier_right(FmR, fr, cxR, amp, lg_r))
+            lambda: frontier_left(Fm, fr, cx, amp, lg_l, enrich, fold),
+            lambda: frontier_right(FmR, fr, cxR, amp, lg_r, enrich, fold))
     else:
-        Wn, lenv, leak_l, add_l = frontier_left(Fm, fr, cx, amp, lg_l)
-        Zn, renv, leak_r, add_r = frontier_right(FmR, fr, cx, amp, lg_r)
+        Wn, lenv, leak_l, add_l = frontier_left(Fm, fr, cx, amp, lg_l, enrich, fold)
+        Zn, renv, leak_r, add_r = frontier_right(FmR, fr, cx, amp, lg_r, enrich, fold)
     lg.merge(lg_l)
     lg.merge(lg_r)
-    lg.n_closure_sweeps = 1
+    lg.n_closure_sweeps = 1 if enrich else 0
     ledger.merge(lg)
     lL = math.sqrt(sum(x * x for x in leak_l))
     lR = math.sqrt(sum(x * x for x in leak_r))
     added = add_l + add_r
-    if added > 0:
+    if enrich and added > 0:
         for b in range(rl + 1):
             fr.left[b] = Wn[b]
         for b in range(rl, L - 1):
             fr.right[b] = Zn[b]
         fr.lenv, fr.renv, fr.gen = lenv, renv, fr.gen + 1
-    return SweepResult(added, cx.alloc.scale * math.sqrt(lL * lL + lR * lR), lg.tolerance_met)
+    met = lg.tolerance_met if enrich else all(
+        cx.alloc.scale * x <= cx.alloc.per_bond for x in leak_l + leak_r)
+    return SweepResult(added, cx.alloc.scale * math.sqrt(lL * lL + lR * lR), met, leak_l, leak_r,
+                       enrich and lg.workcap_reached)
 
 
 def single_pass_closure(fr: KLFrames, psi: SymMPS, root_start: BlockTensor, cx: _Ctx, nrm: float,
@@ -386,3 +422,137 @@ def single_pass_closure(fr: KLFrames, psi: SymMPS, root_start: BlockTensor, cx: 
     if not met:
         ledger.fail("closure")
     return current, napply[0]
+
+
+def measure_frontiers(fr: KLFrames, psi: SymMPS, root_start: BlockTensor, cx: _Ctx, nrm: float,
+                      count: int):
+    """Non-enriching sweeps of the frontiers j = 0 .. count-1 of ``root_start`` on the frames
+    (left untouched).  Returns (leak, rho, met, met0, ledger, n_root): the time-weighted leaks,
+    the scale-free ||(I - P) A F_j|| / ||F_j||, whether every frontier j >= 1 met its shares, the
+    j = 0 verdict, the ledger of the sweeps and the root applications."""
+    from .bug_step import AdaptiveLedger
+    napply = [0]
+    gen = root_generator(cx.mpo, fr, cx.prec)
+
+    def apply_B(C):
+        napply[0] += 1
+        return gen(C)
+
+    cores = [root_start] + (projected_frontiers(apply_B, root_start, cx.tau, count - 1)
+                            if count > 1 else [])
+    lg = AdaptiveLedger()
+    leak, rho = [0.0] * count, [0.0] * count
+    met = met0 = True
+    for j in range(count):
+        r = frontier_sweep(fr, psi, cores[j], cx, nrm, lg, enrich=False)
+        nF = T.norm(cores[j])
+        raw = math.sqrt(sum(x * x for x in r.leaks_left) + sum(x * x for x in r.leaks_right))
+        leak[j] = r.leak
+        rho[j] = raw / nF if nF > 0 else 0.0
+        if j >= 1:
+            met &= r.met
+        else:
+            met0 = r.met
+    lg.leak_initial, lg.rho_initial = list(leak), list(rho)
+    lg.n_root_frontier = napply[0]
+    return leak, rho, met, met0, lg, napply[0]
+
+
+def closure_loop(frontier, sweep, order: int, maxpass: int, include_zero: bool = True):
+    """The termination logic of the measured closure: pass 1 sweeps degrees 0..p-1 (1..p-1
+    without degree 0), later passes 1..p-1; converged when a pass adds nothing and meets every
+    share, stopped as capped when a pass misses a share with a sweep at the work ceiling.
+    Returns (passes, converged, exhausted, capped)."""
+    later = list(range(1, order))
+    initial = list(range(order)) if include_zero else later
+    if not initial:
+        return 0, True, False, False
+    ok0 = True
+    for p in range(1, maxpass + 1):
+        grew, allmet, capped = False, True, False
+        for degree in (initial if p == 1 else later):
+            r = sweep(frontier(degree), degree, p)
+            grew |= r.added > 0
+            allmet &= r.met
+            capped |= r.capped
+            if degree == 0:
+                ok0 = r.met
+        if not grew or not later:
+            return p, allmet and ok0, False, False
+        if capped and not allmet:
+            return p, False, False, True
+    return maxpass, False, True, False
+
+
+def checked_closure(fr: KLFrames, psi: SymMPS, root_start: BlockTensor, cx: _Ctx, nrm: float,
+                    ledger, order: int, include_zero: bool = False):
+    """The closure with ``closure_checks``: measure every frontier on the incoming frames; if
+    they all close, the frames stay as the half-sweeps built them, otherwise loop enriching
+    passes (``closure_loop``) and measure degree 0 on the final frames.  Returns the root
+    coefficients in the final frames and the root applications."""
+    from .bug_step import AdaptiveLedger
+    lg = AdaptiveLedger()
+    pleak, _, pmet, pmet0, plg, pn = measure_frontiers(fr, psi, root_start, cx, nrm, order)
+    lg.merge(plg)
+    lg.leak_initial, lg.rho_initial = list(plg.leak_initial), list(plg.rho_initial)
+    napply = [pn]
+    current = [root_start]
+    final_leak = {j: pleak[j] for j in range(order)}
+    gen0 = fr.gen
+    if pmet and (pmet0 or not include_zero):
+        converged, passes = True, 0
+    else:
+        op = [root_generator(cx.mpo, fr, cx.prec)]
+
+        def apply_B(C):
+            napply[0] += 1
+            return op[0](C)
+
+        def frontier(degree):
+            if degree == 0:
+                return current[0]
+            return projected_frontiers(apply_B, current[0], cx.tau, degree)[degree - 1]
+
+        def sweep(core, degree, p):
+            r = frontier_sweep(fr, psi, core, cx, nrm, lg)
+            final_leak[degree] = r.leak
+            if r.added > 0:
+                op[0] = root_generator(cx.mpo, fr, cx.prec)
+                current[0] = project_root(psi, fr)
+            return r
+
+        passes, converged, _, _ = closure_loop(frontier, sweep, order, closure_maxpass(order),
+                                               include_zero)
+        if fr.gen != gen0:
+            final_leak[0] = frontier_sweep(fr, psi, current[0], cx, nrm, lg, enrich=False).leak
+    lg.n_passes = passes
+    lg.leak = [final_leak[j] for j in range(order)]
+    lg.defect = sum(lg.leak[j] / (j + 1) for j in range(order))
+    lg.n_root_frontier = napply[0]
+    if not converged:
+        lg.fail("closure")
+    ledger.merge(lg)
+    ledger.leak = list(lg.leak)
+    ledger.defect = max(ledger.defect, lg.defect)
+    ledger.leak_initial = list(ledger.leak_initial) + list(lg.leak_initial)
+    ledger.rho_initial = list(ledger.rho_initial) + list(lg.rho_initial)
+    ledger.n_passes += lg.n_passes
+    return current[0], napply[0]
+
+
+def complement_action(psi: SymMPS, fr: KLFrames, core: BlockTensor, cx: _Ctx) -> dict:
+    """The outside-Galerkin action (I - P) H v of v = (W (x) Z) core from one measuring sweep
+    with exact (unfolded) bond actions; for a nearest-neighbour H,
+    max(left, right) <= ||(I - P) H v|| <= sqrt(left^2 + right^2) = bound.  Returns left, right,
+    bound and the per-bond actions."""
+    from .bug_step import AdaptiveLedger
+    L, rl = len(psi), fr.rl
+    eps = np.finfo(float).eps
+    # on a copy of the stream, so the measurement leaves the step's draws untouched
+    r = frontier_sweep(fr, psi, core, cx.with_stream(copy.copy(cx.stream)), max(T.norm(core), eps),
+                       AdaptiveLedger(), enrich=False, fold=False)
+    pl, pr = list(r.leaks_left), list(r.leaks_right[rl:L - 1])
+    left = math.sqrt(sum(x * x for x in pl))
+    right = math.sqrt(sum(x * x for x in pr))
+    return dict(left=left, right=right, bound=math.sqrt(left * left + right * right),
+                per_left=pl, per_right=pr)

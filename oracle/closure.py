@@ -35,6 +35,7 @@ stream shared with the device path.
 """
 from __future__ import annotations
 
+import copy
 import math
 from dataclasses import dataclass, field
 from typing import List, Optional
@@ -526,8 +527,9 @@ def complement_action(psi, mpo, fr: KLFrames, core: ob.Fam, alloc, opts, wcap, r
     from .bug import Ledger
     L, rl = len(psi), fr.rl
     eps = np.finfo(float).eps
-    r = frontier_sweep(fr, psi, mpo, core, 0j, alloc, opts, wcap, max(core.norm(), eps), rng,
-                       recurrence, Ledger(), enrich=False, fold=False)
+    # on a copy of the stream, so the measurement leaves the step's draws untouched
+    r = frontier_sweep(fr, psi, mpo, core, 0j, alloc, opts, wcap, max(core.norm(), eps),
+                       copy.copy(rng), recurrence, Ledger(), enrich=False, fold=False)
     pl, pr = list(r.leaks_left), list(r.leaks_right[rl:L - 1])
     left = math.sqrt(sum(x * x for x in pl))
     right = math.sqrt(sum(x * x for x in pr))
