@@ -232,6 +232,10 @@ def main():
                   "launches and the gaps between them), mean of kernel_reps",
         "apply_ms_graph": round(apply_ms_graph, 4), "graph_equals_eager": graph_equal}
     note(f"roofline: {OUT['roofline']['achieved']} TFLOP/s ({apply_ms:.4f} ms per apply)")
+    # the step's largest kernel family by time: the batched block SVD of a centre cut
+    OUT["roofline_svd"] = svd_roofline(psi[mid], args.kernel_reps, stream)
+    note(f"svd roofline: {OUT['roofline_svd']['achieved']} TFLOP/s "
+         f"({OUT['roofline_svd']['svd_ms']:.2f} ms per factorization)")
     if args.kernel_only:
         OUT["kernel_only"] = True
         emit()
@@ -367,6 +371,56 @@ def main():
         _lib.autotune_save(args.save_schedules)
     signal.alarm(0)
     emit()
+
+
+def svd_roofline(A, reps, stream):
+    """The batched block SVD of one truncation cut (the centre site's left matricisation, every
+    charge sector in one call: QR preconditioning, block-Jacobi sweeps, U = Q V), timed with HIP
+    events on its launch stream.  Algorithmic flops per sector, M x N after transposition to
+    M >= N: 16 (2 M N^2 - 2 N^3 / 3) for the Householder QR (R and Q), 44 N^3 per one-sided Jacobi
+    sweep (N (N - 1) / 2 rotations, each three length-N inner products and the update of two
+    columns of X and of the accumulated V) times the sector's recorded sweep count, and 8 M N^2
+    for U = Q V."""
+    import torch
+
+    from bondupdategalerkin_amd import _lib
+    from bondupdategalerkin_amd import tensor_ops as T
+
+    ctx = _lib.context()
+    ctx.check(ctx._lib.bug_set_svd_variant(ctx.handle, 0, 1))
+    fac = T.factor_left(A)
+    torch.cuda.synchronize()
+    mats = list(fac.mats)
+    sweeps = np.zeros(max(len(mats), 1), np.int32)
+    ctx.check(ctx._lib.bug_svd_sweeps(ctx.handle, len(mats), _lib.ptr32(sweeps)))
+    ctx.check(ctx._lib.bug_set_svd_variant(ctx.handle, 0, 0))
+    flops = 0.0
+    for (m, n), sw in zip(mats, sweeps):
+        M, N = max(m, n), min(m, n)
+        flops += 16.0 * (2.0 * M * N * N - 2.0 * N ** 3 / 3) + 44.0 * N ** 3 * int(sw) \
+            + 8.0 * M * N * N
+    reps = max(2, min(reps, 5))
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+    for r in range(reps):
+        starts[r].record(stream)
+        T.factor_left(A)
+        ends[r].record(stream)
+    torch.cuda.synchronize()
+    ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    achieved = flops / (ms * 1e-3) / 1e12
+    peak = PEAK["fp64"]
+    widths = sorted((min(m, n) for m, n in mats), reverse=True)
+    return {
+        "bound": "fp64 (Jacobi rotations on the vector ALUs, QR reflectors and U = Q V on MFMA)",
+        "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+        "frac": round(achieved / peak, 4), "traffic": None,
+        "kernel": ("batched block SVD of the centre site's left matricisation: QR panels and "
+                   "reflectors, Gram block-Jacobi rounds (gbj_gram_kernel + gbj_rotate_kernel) "
+                   "for sectors >= 128 wide, svd_finish, U = Q V, phase"),
+        "sectors": len(mats), "widths_top": widths[:6], "sweeps_max": int(sweeps.max()),
+        "flops_per_svd": flops, "svd_ms": round(ms, 3),
+        "timing": "HIP events on the launch stream around each factorization, mean of reps"}
 
 
 def packing_phase(args, prec, stream):
