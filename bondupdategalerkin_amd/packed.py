@@ -7,7 +7,9 @@ stage), one segmented multi-dot with one host read for the coefficients, and seg
 multi-axpys, instead of K of each.  Each instance follows the recurrence of ``lanczos_expv``
 (src/BUG/Core/LocalSolvers/krylov.jl:128-263: one Krylov pass, substeps = 1, classical
 Gram-Schmidt applied twice against its whole basis) and stops growing its basis at its own
-breakdown; the others go on.
+breakdown; the others go on.  ``arnoldi_expv_packed`` does the same for non-Hermitian problems
+(the Heston roots, ``arnoldi_expv``, krylov.jl:343-442): upper-Hessenberg Arnoldi with CGS2, the
+Hessenberg exponential per instance on the host.
 """
 from __future__ import annotations
 
@@ -21,7 +23,7 @@ import torch
 from . import _lib
 from .blocks import BlockTensor, current_stream_handle
 from .environments import PackedApply
-from .krylov import MAX_NORM, ROOT_MAXITER, Tridiagonal
+from .krylov import MAX_NORM, ROOT_MAXITER, Hessenberg, Tridiagonal
 
 
 class Segments:
@@ -65,16 +67,7 @@ class Segments:
                 coef.ctypes.data_as(_lib._f64p), ptrs, y.data_ptr(), current_stream_handle()))
 
 
-def lanczos_expv_packed(packed: PackedApply, tau: complex, v0s: Sequence[BlockTensor], *,
-                        tol: float = 1e-15, maxiter: int = ROOT_MAXITER,
-                        reorth: bool = True) -> Tuple[List[BlockTensor], List[int]]:
-    """exp(tau H_i) v0_i for every instance of ``packed`` (its shifts included in H_i); returns
-    the results (new tensors) and each instance's Krylov dimension."""
-    seg = Segments(packed)
-    K = seg.nseg
-    if len(v0s) != K:
-        raise ValueError("one start vector per instance")
-
+def _pack_helpers(packed: PackedApply, seg: Segments):
     def pack(vs):
         buf = torch.zeros_like(packed._xbuf)
         for t, v in zip(packed.x, vs):
@@ -94,6 +87,19 @@ def lanczos_expv_packed(packed: PackedApply, tau: complex, v0s: Sequence[BlockTe
         seg.axpy(np.asarray(s, dtype=complex)[None, :], [v], out)
         return out
 
+    return pack, apply, scaled
+
+
+def lanczos_expv_packed(packed: PackedApply, tau: complex, v0s: Sequence[BlockTensor], *,
+                        tol: float = 1e-15, maxiter: int = ROOT_MAXITER,
+                        reorth: bool = True) -> Tuple[List[BlockTensor], List[int]]:
+    """exp(tau H_i) v0_i for every instance of ``packed`` (its shifts included in H_i); returns
+    the results (new tensors) and each instance's Krylov dimension."""
+    seg = Segments(packed)
+    K = seg.nseg
+    if len(v0s) != K:
+        raise ValueError("one start vector per instance")
+    pack, apply, scaled = _pack_helpers(packed, seg)
     v0 = pack(v0s)
     beta0 = np.sqrt(np.maximum(seg.dots([v0], v0)[0].real, 0.0))
     active = beta0 > 0
@@ -136,6 +142,63 @@ def lanczos_expv_packed(packed: PackedApply, tau: complex, v0s: Sequence[BlockTe
         if np.isfinite(abs(sc)) and sc != 0 and abs(sc) * beta0[i] * np.linalg.norm(c) <= MAX_NORM:
             scale = beta0[i] * sc
         coef[: len(c), i] = c * scale
+    out = torch.zeros_like(v0)
+    seg.axpy(coef, basis, out)
+    res = []
+    for t, v, i in zip(packed.x, v0s, range(K)):
+        off = (t.data.data_ptr() - packed._xbuf.data_ptr()) // 16
+        y = BlockTensor(t.layout, out[off: off + t.layout.nelem].clone())
+        res.append(v.clone() if beta0[i] == 0 else y)
+    return res, dims
+
+
+def arnoldi_expv_packed(packed: PackedApply, tau: complex, v0s: Sequence[BlockTensor], *,
+                        tol: float = 1e-13, maxiter: int = ROOT_MAXITER
+                        ) -> Tuple[List[BlockTensor], List[int]]:
+    """exp(tau A_i) v0_i for every instance of ``packed`` by one Arnoldi pass each, in lockstep:
+    per iteration one packed apply, two segmented Gram-Schmidt passes against the whole basis
+    (one multi-dot launch and one host read each) and segmented multi-axpys; an instance stops
+    growing its basis at its own breakdown (residual below ``tol``).  Returns the results (new
+    tensors) and each instance's Krylov dimension, as ``arnoldi_expv`` with substeps = 1 gives
+    them one instance at a time."""
+    seg = Segments(packed)
+    K = seg.nseg
+    if len(v0s) != K:
+        raise ValueError("one start vector per instance")
+    pack, apply, scaled = _pack_helpers(packed, seg)
+    v0 = pack(v0s)
+    beta0 = np.sqrt(np.maximum(seg.dots([v0], v0)[0].real, 0.0))
+    active = beta0 > 0
+    H = np.zeros((K, maxiter + 1, maxiter), dtype=complex)
+    dims = [0] * K
+    basis = [scaled(v0, np.where(active, 1.0 / np.where(active, beta0, 1.0), 0.0))]
+    for k in range(1, maxiter + 1):
+        r = apply(basis[k - 1])
+        for _p in range(2):
+            cs = seg.dots(basis, r)
+            cs[:, ~active] = 0.0
+            H[:, :len(basis), k - 1] += cs.T
+            seg.axpy(-cs, basis, r)
+        for i in np.flatnonzero(active):
+            dims[i] = k
+        b = np.sqrt(np.maximum(seg.dots([r], r)[0].real, 0.0))
+        active &= b >= tol
+        if k == maxiter or not active.any():
+            break
+        for i in np.flatnonzero(active):
+            H[i, k, k - 1] = b[i]
+        basis.append(scaled(r, np.where(active, 1.0 / np.where(active, b, 1.0), 0.0)))
+    coef = np.zeros((len(basis), K), dtype=complex)
+    for i in range(K):
+        d = dims[i]
+        if beta0[i] == 0 or d == 0:
+            continue
+        c, ls = Hessenberg(H[i, :d, :d].copy()).coeffs(tau)
+        sc = np.exp(ls)
+        scale = beta0[i]
+        if np.isfinite(abs(sc)) and sc != 0 and abs(sc) * beta0[i] * np.linalg.norm(c) <= MAX_NORM:
+            scale = beta0[i] * sc
+        coef[:d, i] = c[:d] * scale
     out = torch.zeros_like(v0)
     seg.axpy(coef, basis, out)
     res = []

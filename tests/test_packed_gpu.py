@@ -1,5 +1,6 @@
 """Many independent instances in lockstep (GPU): the segmented vector operations against their
-per-instance forms, and the packed Lanczos exponential against one lanczos_expv per instance."""
+per-instance forms, and the packed Lanczos and Arnoldi exponentials against one lanczos_expv /
+arnoldi_expv per instance."""
 import numpy as np
 import pytest
 
@@ -86,3 +87,26 @@ def test_packed_lanczos_matches_per_instance(gpu, tau):
         assert diff <= 1e-11 * T.norm(want), (i, diff)
         if tau.real == 0:                             # unitary: the norm is kept
             assert abs(T.norm(got[i]) - T.norm(x)) <= 1e-10 * T.norm(x)
+
+
+@pytest.mark.parametrize("tau", [-0.3j, -0.2 + 0.1j])
+def test_packed_arnoldi_matches_per_instance(gpu, tau):
+    """The lockstep Arnoldi (a shifted, hence non-Hermitian-looking, generator: tau complex) equals
+    one arnoldi_expv with substeps = 1 per instance to 1e-11, with the same Krylov dimensions."""
+    from bondupdategalerkin_amd import environments as ev
+    from bondupdategalerkin_amd import krylov as K
+    from bondupdategalerkin_amd import tensor_ops as T
+    from bondupdategalerkin_amd.environments import PackedApply
+    from bondupdategalerkin_amd.packed import arnoldi_expv_packed
+
+    ops, xs = _instances(SPECS)
+    packed = PackedApply(ops, [x.layout for x in xs])
+    gain = max(T.norm(ev.apply_one_site(L, W, R, x)) / T.norm(x) for (L, W, R), x in zip(ops, xs))
+    tau = tau / gain
+    got, dims = arnoldi_expv_packed(packed, tau, xs, maxiter=12)
+    for i, ((L, W, R), x) in enumerate(zip(ops, xs)):
+        want, rep = K.arnoldi_expv(lambda v, L=L, W=W, R=R: ev.apply_one_site(L, W, R, v), tau, x,
+                                   maxiter=12, substeps=1, conv_tol=0.0)
+        assert dims[i] == rep.iterations, i
+        diff = T.norm(T.axpy(-1.0, want, got[i]))
+        assert diff <= 1e-11 * T.norm(want), (i, diff)
