@@ -18,7 +18,8 @@ Stages (0-based sites; bond b joins sites b and b+1; link j sits left of site j)
 * root solve               effective_operators.jl:91-107; krylov.jl:128-263
 * assembly + compression   diagnostics.jl:269-337; truncation.jl:49-158
 
-``order > 1`` (the frontier closure) is not implemented and is refused.
+* order-p closure (order > 1)  krylov_closure_loop.jl:18-281 (``closure.single_pass_closure``,
+                           and the measured loop under ``closure_checks``)
 """
 from __future__ import annotations
 
@@ -235,7 +236,7 @@ class _Alloc:
 def basis_allocation(tol_basis, nbonds, tau, nrm, nfront, tol_krylov=None,
                      krylov_rank=0.0) -> _Alloc:
     """The ``:linear`` rule the step uses (basis_budget.jl:75-104); ``krylov_rank`` (tol_krylov's
-    units) is apportioned by the same rule into the tail a ranked Krylov stage may discard."""
+    units) is split over the bonds the same way; it bounds what a ranked Krylov stage may drop."""
     nb, nf = max(nbonds, 1), max(nfront, 1)
     kr = 1.0 if tol_krylov is None else tol_krylov / tol_basis
     if kr <= 0:
@@ -1448,7 +1449,8 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         if rep.err > te * (1 + 1e-8):
             ledger.fail("expv")
     if frames_report is not None:        # bug_step.jl:455-467
-        frames_report.update(left=list(left_frames), right=list(right_frames), root_link=rl)
+        frames_report.update(left=list(left_frames), right=list(right_frames), root_link=rl,
+                             lenv=lroot, renv=rroot, eshift=float(eshift))
     secs["root"] = time.perf_counter() - ts
 
     _progress("assembly and compression")

@@ -351,8 +351,9 @@ def frontier_right(Fm: SymMPS, fr: KLFrames, cx: _Ctx, amp: float, ledger, enric
 
 def frontier_sweep(fr: KLFrames, psi: SymMPS, core: BlockTensor, cx: _Ctx, nrm: float,
                    ledger, enrich: bool = True, fold: Optional[bool] = None) -> SweepResult:
-    """One sweep of a frontier on both sides; enriching, the frames are replaced (and their
-    generation advanced) only if something was added; ``enrich = False`` only measures.
+    """One sweep of a frontier on both sides.  With ``enrich`` a sweep that finds nothing to add
+    leaves the frames and their generation counter as they were; ``enrich = False`` measures the
+    leaks and changes nothing.
     ``fold`` None takes the step's leak form."""
     from .bug_step import AdaptiveLedger
     L, rl = len(psi), fr.rl

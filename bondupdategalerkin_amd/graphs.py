@@ -6,7 +6,7 @@ and replays it: one launch call per application instead of one host round trip p
 grouped-GEMM stage, so the stages run back to back with no host gaps.  While the graph lives the
 library keeps the plan images and scratch buffers it reads alive (``bug_graph_capture_begin`` /
 ``bug_graph_release`` in include/bug.h).  The reference has no counterpart: its
-``expv``/``lanczos_expv`` (src/Core/Time/BUG/krylov.jl:128-442) call the action eagerly.
+``expv``/``lanczos_expv`` (src/BUG/Core/LocalSolvers/krylov.jl:128-442) call the action eagerly.
 """
 from typing import Callable
 

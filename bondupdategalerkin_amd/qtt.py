@@ -72,7 +72,7 @@ def qtt_grid(nx: int, ny: int, *, x=(-1.0, 1.0), y=(-1.0, 1.0), periodic_x: bool
     if order != "grouped" or endian != "lsb_first":
         raise ValueError(f"only the grouped lsb-first ordering is built, got {order}/{endian}")
     if nx < 1 or ny < 1:
-        raise ValueError("qtt_grid needs at least one bit per coordinate")
+        raise ValueError("every coordinate of a quantics grid must have n >= 1 bits")
     if not (x[1] > x[0] and y[1] > y[0]):
         raise ValueError(f"empty domain {x} x {y}")
     dx = (x[1] - x[0]) / (2 ** nx if periodic_x else 2 ** nx - 1)
@@ -138,7 +138,8 @@ def op_transpose(A: Chain) -> Chain:
 def op_shift(n: int, periodic: bool) -> Chain:
     """``(S f)_i = f_{i+1}`` on n bits: the carry automaton, +1 entering at bit 0.  State c is the
     carry into a bit: the input bit is (out + c) mod 2 and the carry leaving is (out + c) div 2.
-    The carry out of the top bit wraps (periodic) or leaves the box (row 2^n - 1 zero)."""
+    Past the top bit the carry either re-enters at bit 0 (periodic) or is dropped, which zeroes
+    row 2^n - 1 (open boundary)."""
     if n < 1:
         raise ValueError("a shift needs at least one bit")
     core = np.zeros((2, 2, 2, 2))
@@ -253,7 +254,7 @@ def qtt_derivative(n: int, h: float, order: int = 1, bc: str = "periodic") -> Ch
         if N < 4:
             raise ValueError(f"onesided stencils need at least 4 points, got {N}")
         e = N - 1
-        # (one-sided stencil) - (the truncated central row) on rows 0 and N - 1
+        # rows 0 and N - 1: replace what the central stencil left there by the one-sided row
         if order == 1:
             patch = [(0, 0, -3.0), (0, 1, 3.0), (0, 2, -1.0),
                      (e, e, 3.0), (e, e - 1, -3.0), (e, e - 2, 1.0)]

@@ -237,7 +237,7 @@ class Ledger:
 def basis_allocation(tol_basis, nbonds, tau, nrm, nfront, tol_krylov=None,
                      krylov_rank=0.0) -> Alloc:
     """:linear rule of basis_allocation (basis_budget.jl:75-104); ``krylov_rank`` (tol_krylov's
-    units) is apportioned by the same rule into the tail a ranked Krylov stage may discard."""
+    units) is split over the bonds the same way; it bounds what a ranked Krylov stage may drop."""
     nb, nf = max(nbonds, 1), max(nfront, 1)
     kr = 1.0 if tol_krylov is None else tol_krylov / tol_basis
     if not krylov_rank >= 0:

@@ -282,6 +282,52 @@ def test_frames_only_matches_oracle(gpu):
     assert [f.layout.col.total for f in rep["left"]] == list(info_d.expanded[: rep["root_link"] + 1])
 
 
+def _root_core(psi, left, right, rl):
+    """The root core of a state that lies in the frames: left frames contracted from site 0,
+    right frames from site L - 1, met at the root link."""
+    from bondupdategalerkin_amd import tensor_ops as T
+
+    AL = None
+    for i in range(rl + 1):
+        AL = T.overlap_left(left[i], psi[i] if AL is None else T.core_times_site(AL, psi[i]))
+    AR = None
+    for b in range(len(psi) - 2, rl - 1, -1):
+        c = psi[b + 1] if AR is None else T.site_times_core(psi[b + 1], AR)
+        AR = T.overlap_right(c, right[b])
+    return T.core_times_core(AL, AR)
+
+
+def test_frames_report_solves_the_root(gpu):
+    """A caller that solves its own root from frames_report (bug_step.jl:455-467: frames, root
+    link, lenv_root/renv_root, eshift) reproduces the normal step: the root generator built from
+    the reported environments, exponentiated from the frames-only state's root core, equals the
+    root core of the normal (untruncated) step from the same state and probe stream."""
+    from bondupdategalerkin_amd import bug_step as B
+    from bondupdategalerkin_amd import environments as ev
+    from bondupdategalerkin_amd import krylov as K
+    from bondupdategalerkin_amd import tensor_ops as T
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    tau = -0.05j
+    psi_f = _device_state("neel", 10)
+    mpo = xxz_mpo(10, delta=0.5)
+    B.cbe_bug_step(psi_f, mpo, -0.1j)
+    psi_n = psi_f.copy()
+    rep = {}
+    B.cbe_bug_step(psi_f, mpo, tau, truncate=False, frames_only=True, frames_report=rep)
+    B.cbe_bug_step(psi_n, mpo, tau, truncate=False)
+    rl = rep["root_link"]
+    assert rep["eshift"] == 0.0
+    Lr = ev.push_left(rep["lenv"], rep["left"][rl], mpo[rl])
+    Rr = ev.push_right(rep["renv"], rep["right"][rl], mpo[rl + 1])
+    C0 = _root_core(psi_f, rep["left"], rep["right"], rl)
+    mine, _ = K.lanczos_expv(lambda C: ev.apply_zero_site(Lr, Rr, C), tau, C0, tol=1e-15,
+                             maxiter=K.ROOT_MAXITER)
+    ref = _root_core(psi_n, rep["left"], rep["right"], rl)
+    diff = T.norm(T.axpy(-1.0, ref, mine)) / T.norm(ref)
+    assert diff <= 1e-10, diff
+
+
 def test_invalid_options_refused_before_touching_state(gpu):
     from bondupdategalerkin_amd import bug_step as B
     from bondupdategalerkin_amd import mps
