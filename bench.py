@@ -80,10 +80,12 @@ def parse():
     p.add_argument("--serving-rate", type=float, default=1.0, help="arrivals per second")
     p.add_argument("--serving-scale", choices=["small", "full"], default="small")
     p.add_argument("--slots", type=int, default=4, help="concurrent requests of the serving loop")
-    p.add_argument("--heston", type=int, default=4,
-                   help="Heston instances (2^10 x 2^10 grid) stepped through the serving loop "
-                        "(0: skip)")
-    p.add_argument("--heston-steps", type=int, default=1)
+    p.add_argument("--heston", type=int, default=64,
+                   help="Heston instances (2^10 x 2^10 grid) stepped together through "
+                        "bug_step_batch (0: skip)")
+    p.add_argument("--heston-steps", type=int, default=2, help="batched Heston steps")
+    p.add_argument("--heston-workers", type=int, default=8,
+                   help="worker threads of the batched step's per-instance stages")
     p.add_argument("--autotune", choices=["table", "tune", "off"], default="table",
                    help="table: grouped-GEMM schedules from the package's tuning table for "
                         "gfx950 (heuristic for classes it lacks); tune: the autotuner times "
@@ -564,46 +566,69 @@ def serving_phase(args):
 
 
 def heston_phase(args):
-    from bondupdategalerkin_amd import serving as S
+    """The batched Heston form: ``--heston`` instances on the 2^10 x 2^10 grid (parameter sets
+    drawn as the batched configuration describes, seed 0) advanced together through
+    ``bug_step_batch`` (stages before the root per instance on worker threads, the root
+    exponentials as one lockstep Krylov recurrence over grouped launches), ``--heston-steps``
+    batched steps within the budget; and one instance stepped alone through ``bug_step`` for the
+    sequential rate."""
+    import torch
 
-    if left() < 90:
+    from bondupdategalerkin_amd import cstep as C
+    from bondupdategalerkin_amd import qtt as Q
+
+    if left() < 60:
         OUT.setdefault("cut", []).append("heston batch: no budget left")
         return {"skipped": "budget"}
     note(f"heston batch: {args.heston} instances")
     hrng = np.random.default_rng(0)
-    eng = S.Engine(slots=args.slots)
-    res = []
-    try:
+    g = Q.qtt_grid(10, 10, x=(-1.0, 1.0), y=(0.0, 0.5), periodic_x=False, periodic_y=False)
+    init = Q.qtt_separable(g, lambda x: float(np.exp(0.5 * x)), lambda v: float(np.exp(-v)))
+    prms = [(hrng.uniform(0, 0.05), hrng.uniform(0, 0.05), hrng.uniform(0.5, 4),
+             hrng.uniform(0.02, 0.15), hrng.uniform(0.1, 0.8), hrng.uniform(-0.9, 0))
+            for _ in range(args.heston)]
+    kw = dict(order=2, maxdim=128, hermitian=False, work_ceiling_rule="maxdim",
+              precision="fp64")
+    states = [C.CState(Q.qtt_mps(init)) for _ in prms]
+    mpos = [C.CMPO(Q.heston_mpo(g, Q.HestonParams(*p))) for p in prms]
+    torch.cuda.synchronize()
+    out = {"workload": (f"{args.heston} Heston QTT instances, 2^10 x 2^10 grid (x in [-1,1], v in "
+                        "[0,0.5], non-periodic), params drawn as the batched config (seed 0), "
+                        "psi0 = exp(0.5x) exp(-v), order 2, maxdim 128, tau 0.02, "
+                        "hermitian=false, FP64, one bug_step_batch call per step "
+                        f"({args.heston_workers} worker threads)"),
+           "steps": 0, "seconds": [], "budget_status": [], "phase_seconds": []}
+    for _ in range(args.heston_steps):
+        if out["seconds"] and left() < 1.5 * out["seconds"][-1] + 30:
+            OUT.setdefault("cut", []).append(f"heston batch: {out['steps']} of "
+                                             f"{args.heston_steps} batched steps in budget")
+            break
         t0 = time.perf_counter()
-        rids = []
-        for _ in range(args.heston):
-            prm = (hrng.uniform(0, 0.05), hrng.uniform(0, 0.05), hrng.uniform(0.5, 4),
-                   hrng.uniform(0.02, 0.15), hrng.uniform(0.1, 0.8), hrng.uniform(-0.9, 0))
-            rids.append(eng.submit(S.Request(
-                "heston", {"bits": 10, "params": prm, "maxdim": 128,
-                           "steps": args.heston_steps, "dt": 0.02}, precision="fp64")))
-        try:
-            eng.wait(rids, timeout=max(left() - 45, 1.0))
-        except TimeoutError:
-            eng.cancel()
-            OUT.setdefault("cut", []).append("heston batch: cancelled at the budget")
-    finally:
-        eng.close()
-    span = time.perf_counter() - t0
-    res = [eng.results[r] for r in rids if r in eng.results]
-    ok = [r for r in res if r.error is None]
-    out = {
-        "workload": (f"{args.heston} Heston QTT instances, 2^10 x 2^10 grid (x in [-1,1], v in "
-                     "[0,0.5]), params drawn as the batched config (seed 0), psi0 = "
-                     f"exp(0.5x) exp(-v), order 2, maxdim 128, tau 0.02, hermitian=false, "
-                     f"{args.heston_steps} step(s) each, FP64, {args.slots} concurrent slots"),
-        "instance_steps_per_s": round(sum(r.steps for r in res) / span, 4) if span > 0 else None,
-        "completed": len(ok),
-        "makespan_s": round(span, 2),
-        "p50_latency_s": round(float(np.percentile([r.latency for r in ok], 50)), 2) if ok else None,
-        "errors": [r.error for r in res if r.error and not r.error.startswith("Cancelled")],
-        "budget_status": [s for r in res for s in r.statuses],
-    }
+        infos = C.bug_step_batch(states, mpos, 0.02, workers=args.heston_workers, **kw)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out["steps"] += 1
+        out["seconds"].append(round(dt, 2))
+        out["budget_status"].append(sorted({i["status"] for i in infos}))
+        out["phase_seconds"].append({"root_lockstep": round(infos[0]["seconds"]["root"], 2),
+                                     "max_step": round(max(i["seconds"]["step"] for i in infos), 2)})
+        note(f"heston batched step {out['steps']}: {dt:.1f}s, root lockstep "
+             f"{infos[0]['seconds']['root']:.2f}s, expv iters max "
+             f"{max(i['expv_iters'] for i in infos)}")
+    if out["seconds"]:
+        out["instance_steps_per_s"] = round(args.heston / float(np.mean(out["seconds"])), 4)
+        out["max_bond_after"] = max(max(s.bond_dims()) for s in states)
+    # one instance alone through bug_step, for the sequential rate
+    if left() > 40:
+        one = C.CState(Q.qtt_mps(init))
+        t0 = time.perf_counter()
+        info = C.bug_step(one, mpos[0], 0.02, **kw)
+        torch.cuda.synchronize()
+        out["single_instance_step_s"] = round(time.perf_counter() - t0, 2)
+        out["single_instance_status"] = info["status"]
+        if "instance_steps_per_s" in out:
+            out["speedup_vs_one_at_a_time"] = round(
+                out["instance_steps_per_s"] * out["single_instance_step_s"], 2)
     note(f"heston batch: {out}")
     return out
 

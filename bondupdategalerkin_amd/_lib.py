@@ -199,6 +199,10 @@ SIGNATURES = {
         ctypes.c_int,
         [_P, _BSMP, _BSMP, _BSMP, _BSMP, ctypes.c_double, ctypes.c_int32, _P],
     ),
+    "bug_apply_zero_site_batch": (
+        ctypes.c_int,
+        [_P, ctypes.c_int32, _BSMP, _BSMP, _BSMP, _BSMP, _f64p, ctypes.c_int32, _P],
+    ),
     "bug_axpy": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _P, _P, _P]),
     "bug_scal": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _P, _P]),
     "bug_dotc_segments": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, _i64p, ctypes.c_int32,
@@ -257,6 +261,9 @@ SIGNATURES = {
                                                _i64p, _P]),
     "bug_step": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_double,
                                 ctypes.POINTER(BugStepOpts), ctypes.POINTER(BugStepInfo)]),
+    "bug_step_batch": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _f64p,
+                                      ctypes.POINTER(BugStepOpts), ctypes.POINTER(BugStepInfo),
+                                      ctypes.c_int32]),
     "bug_block_svd": (
         ctypes.c_int,
         [_P, ctypes.c_int32, ctypes.POINTER(BugSvdJob), _P, ctypes.c_int64, _P, ctypes.c_int64,

@@ -1009,6 +1009,99 @@ int bug_apply_one_site_batch(bug_ctx* ctx, int32_t k, const bug_bsm* L, const bu
     });
 }
 
+int bug_apply_zero_site_batch(bug_ctx* ctx, int32_t k, const bug_bsm* L, const bug_bsm* R,
+                              const bug_bsm* C, const bug_bsm* Y, const double* shifts,
+                              int32_t precision, void* stream) {
+    return guarded(ctx, [&] {
+        if (precision != BUG_PREC_FP64 && precision != BUG_PREC_FP32)
+            throw Error(BUG_EINVAL, "unknown precision");
+        if (k < 0 || (k > 0 && (!L || !R || !C || !Y)))
+            throw Error(BUG_EINVAL, "bad batch arguments");
+        if (k == 0) return;
+        std::vector<Bsm> ls, rs, cs, ys, ts;
+        std::vector<std::vector<bug_term>> t1(k), t2(k);
+        std::vector<int64_t> toff(k);
+        int64_t ttot = 0;
+        for (int i = 0; i < k; ++i) {
+            ls.emplace_back(&L[i], "L");
+            rs.emplace_back(&R[i], "R");
+            cs.emplace_back(&C[i], "C");
+            ys.emplace_back(&Y[i], "Y");
+            if (ls[i].nslice() != rs[i].nslice())
+                throw Error(BUG_EINVAL, "root environments have different channel counts");
+            if (cs[i].nslice() != 1 || ys[i].nslice() != 1)
+                throw Error(BUG_EINVAL, "a zero-site core has one slice");
+            std::vector<int32_t> sh;
+            for (int w = 0; w < ls[i].nslice(); ++w) {
+                sh.push_back(ls[i].shift[w] + cs[i].shift[0]);
+                t1[i].push_back(bug_term{w, w, 0, BUG_OP_N, BUG_OP_N, 1.0, 0.0});
+                t2[i].push_back(bug_term{0, w, w, BUG_OP_N, BUG_OP_T, 1.0, 0.0});
+            }
+            ts.push_back(make_intermediate(ls[i].row, cs[i].col, sh));
+            toff[i] = ttot;
+            ttot += std::max<int64_t>(ts[i].nelem, 1);
+        }
+        hipStream_t st = as_stream(stream);
+        double2* tdev = static_cast<double2*>(scratch(ctx, st, sizeof(double2) * ttot));
+        auto lowest = [&](const bug_bsm* d) {
+            const char* b = static_cast<const char*>(d[0].data);
+            for (int i = 1; i < k; ++i) b = std::min(b, static_cast<const char*>(d[i].data));
+            return b;
+        };
+        auto rel = [](const void* p, const char* base) {
+            const int64_t bytes = static_cast<const char*>(p) - base;
+            if (bytes % (int64_t)sizeof(double2) != 0)
+                throw Error(BUG_EINVAL, "batched operands must be 16-byte aligned");
+            return bytes / (int64_t)sizeof(double2);
+        };
+        // merged plans keyed by the instances' plan keys and operand offsets from the lowest base,
+        // as for the one-site batch: fixed packed buffers reuse them call after call
+        auto merged = [&](int stage, const char* cb, const char* ab, const char* bb) {
+            std::vector<int64_t> co(k), ao(k), bo(k);
+            std::vector<uint64_t> keys(k);
+            Hasher h;
+            h.pod((uint64_t)(0x2E50 + stage));
+            for (int i = 0; i < k; ++i) {
+                if (stage == 1) {
+                    keys[i] = product_key(ts[i], ls[i], cs[i], t1[i], false, 0x41);
+                    co[i] = toff[i];
+                    ao[i] = rel(L[i].data, ab);
+                    bo[i] = rel(C[i].data, bb);
+                } else {
+                    keys[i] = product_key(ys[i], ts[i], rs[i], t2[i], false, 0x42);
+                    co[i] = rel(Y[i].data, cb);
+                    ao[i] = toff[i];
+                    bo[i] = rel(R[i].data, bb);
+                }
+                h.pod(keys[i]);
+                h.pod(co[i]);
+                h.pod(ao[i]);
+                h.pod(bo[i]);
+            }
+            return cached_plan(ctx, h.h, [&](Plan& m) {
+                for (int i = 0; i < k; ++i) {
+                    const Plan* p = stage == 1
+                        ? cached_product_plan(ctx, ts[i], ls[i], cs[i], t1[i], false, 0x41)
+                        : cached_product_plan(ctx, ys[i], ts[i], rs[i], t2[i], false, 0x42);
+                    append_plan(m, *p, co[i], ao[i], bo[i]);
+                }
+            });
+        };
+        // stage 1 of every instance in one launch: A = L_i, B = C_i, out T_i
+        const char* lb = lowest(L);
+        const char* cb = lowest(C);
+        Plan* m1 = merged(1, nullptr, lb, cb);
+        launch_plan(ctx, *m1, precision, lb, cb, tdev, st);
+        // stage 2: A = T_i, B = R_i, out Y_i
+        const char* rb = lowest(R);
+        const char* yb = lowest(Y);
+        Plan* m2 = merged(2, yb, nullptr, rb);
+        launch_plan(ctx, *m2, precision, tdev, rb, const_cast<char*>(yb), st);
+        for (int i = 0; i < k; ++i)
+            if (shifts && shifts[i] != 0.0) shift_into(cs[i], ys[i], C[i].data, Y[i].data, shifts[i], st);
+    });
+}
+
 int bug_apply_zero_site(bug_ctx* ctx, const bug_bsm* L, const bug_bsm* R, const bug_bsm* C,
                         const bug_bsm* Y, double shift, int32_t precision, void* stream) {
     return guarded(ctx, [&] {

@@ -20,6 +20,7 @@
 // cutoff-0 SVDs; both give the same isometries' spans and the same represented tensors.  The probes
 // come from a counter-based stream (splitmix64 + Box-Muller) consumed in the reference's order.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -1176,8 +1177,20 @@ void validate(const bug_step_opts& o, int L, int& rl) {
     }
 }
 
-void run_step(const Exec& ex, MPS& psi, const std::vector<MpoSite>& mpo, cplx tau, const bug_step_opts& o,
-              StepOut& out) {
+// What the step has built when it reaches the root solve, and what its last stage needs.
+struct RootProblem {
+    int rl = 0;
+    double t0 = 0.0, te = 0.0, tt = 0.0;
+    bool hermitian = true;
+    std::vector<int> bd0;
+    std::vector<Tensor> left_frames, right_frames;
+    Tensor Lr, Rr, start;                  // root environments and the root start core
+};
+
+// Stages up to the root solve: options, ceilings, half-sweeps, closure, projection miss, and the
+// root generator's environments (bug_step.jl:17-380).
+void step_prepare(const Exec& ex, MPS& psi, const std::vector<MpoSite>& mpo, cplx tau,
+                  const bug_step_opts& o, StepOut& out, RootProblem& rp) {
     const int L = psi.size();
     int rl = 0;
     validate(o, L, rl);
@@ -1342,25 +1355,42 @@ void run_step(const Exec& ex, MPS& psi, const std::vector<MpoSite>& mpo, cplx ta
     ledger.proj_err = npsi > 0 ? projection_miss(ex, psi, left_frames, right_frames, rl) / npsi : 0.0;
     if (ledger.proj_err > tb) ledger.fail(BUG_UNMET_PROJECTION);
 
-    // ---- root solve ----
-    double ts = now();
-    Tensor Lr = push_left(ex, lroot, left_frames[rl], mpo[rl], ex.prec);
-    Tensor Rr = push_right(ex, rroot, right_frames[rl], mpo[rl + 1], ex.prec);
-    int napply = 0;
-    auto apply_root = [&](const Tensor& C) {
-        ++napply;
-        return apply_zero_site(ex, Lr, Rr, C, ex.prec);
-    };
-    ExpvReport rep;
-    Tensor result = hermitian ? lanczos_expv(ex, apply_root, tau, root_start, 1e-15, kRootMaxIter, o.reorth != 0, te, rep)
-                              : arnoldi_expv(ex, apply_root, tau, root_start, 1e-15, kRootMaxIter, te, rep);
-    ledger.expv_err = rep.err;
-    ledger.expv_iters = rep.iterations;
-    if (rep.err > te * (1 + 1e-8)) ledger.fail(BUG_UNMET_EXPV);
-    out.seconds[3] = now() - ts;
+    // ---- the root generator's environments ----
+    rp.rl = rl;
+    rp.t0 = t0;
+    rp.te = te;
+    rp.tt = tt;
+    rp.hermitian = hermitian;
+    rp.bd0 = bd0;
+    rp.Lr = push_left(ex, lroot, left_frames[rl], mpo[rl], ex.prec);
+    rp.Rr = push_right(ex, rroot, right_frames[rl], mpo[rl + 1], ex.prec);
+    rp.start = root_start;
+    rp.left_frames = std::move(left_frames);
+    rp.right_frames = std::move(right_frames);
+}
 
-    // ---- assembly and compression ----
-    ts = now();
+// Record the root solve's report on the ledger (bug_step.jl:383-395).
+void step_root_report(const RootProblem& rp, const ExpvReport& rep, int napply, double seconds,
+                      StepOut& out) {
+    out.ledger.expv_err = rep.err;
+    out.ledger.expv_iters = rep.iterations;
+    if (rep.err > rp.te * (1 + 1e-8)) out.ledger.fail(BUG_UNMET_EXPV);
+    out.root_applications = napply;
+    out.seconds[3] = seconds;
+}
+
+// Assembly of the solved root into the state and the budgeted compression (bug_step.jl:396-510).
+void step_finish(const Exec& ex, MPS& psi, RootProblem& rp, const Tensor& result,
+                 const bug_step_opts& o, StepOut& out) {
+    const int L = psi.size();
+    const int rl = rp.rl;
+    const int maxdim = o.maxdim;
+    const double tt = rp.tt;
+    Ledger& ledger = out.ledger;
+    const std::vector<Tensor>& left_frames = rp.left_frames;
+    const std::vector<Tensor>& right_frames = rp.right_frames;
+    const std::vector<int>& bd0 = rp.bd0;
+    const double ts = now();
     SVDResult res = factor_core(ex, result).select(ex, 0.0);
     out.expanded[rl] = left_frames[rl].L().col.total();
     double discarded = trunc_weight(res);
@@ -1390,10 +1420,27 @@ void run_step(const Exec& ex, MPS& psi, const std::vector<MpoSite>& mpo, cplx ta
         e.discarded = bwts[e.bond];
     }
     out.seconds[4] = now() - ts;
-    out.seconds[5] = now() - t0;
+    out.seconds[5] = now() - rp.t0;
     out.root_link = rl;
-    out.root_applications = napply;
     out.discarded = discarded;
+}
+
+void run_step(const Exec& ex, MPS& psi, const std::vector<MpoSite>& mpo, cplx tau, const bug_step_opts& o,
+              StepOut& out) {
+    RootProblem rp;
+    step_prepare(ex, psi, mpo, tau, o, out, rp);
+    const double ts = now();
+    int napply = 0;
+    auto apply_root = [&](const Tensor& C) {
+        ++napply;
+        return apply_zero_site(ex, rp.Lr, rp.Rr, C, ex.prec);
+    };
+    ExpvReport rep;
+    Tensor result = rp.hermitian
+        ? lanczos_expv(ex, apply_root, tau, rp.start, 1e-15, kRootMaxIter, o.reorth != 0, rp.te, rep)
+        : arnoldi_expv(ex, apply_root, tau, rp.start, 1e-15, kRootMaxIter, rp.te, rep);
+    step_root_report(rp, rep, napply, now() - ts, out);
+    step_finish(ex, psi, rp, result, o, out);
 }
 
 }  // namespace host
@@ -1413,6 +1460,192 @@ struct bug_state {
 namespace {
 
 using namespace bug::host;
+
+// the step's report into the caller's info POD
+void fill_info(const StepOut& out, bug_step_info* info) {
+    const Ledger& lg = out.ledger;
+    info->tolerance_met = lg.tolerance_met ? 1 : 0;
+    info->unmet = lg.unmet;
+    info->max_sample = lg.max_sample;
+    info->max_tail = lg.max_tail;
+    info->max_krylov = lg.max_krylov;
+    info->max_krylov_c = lg.max_krylov_c;
+    info->max_krylov_d = lg.max_krylov_d;
+    info->discarded = lg.discarded;
+    info->worst_discarded = out.discarded;
+    info->expv_err = lg.expv_err;
+    info->expv_iters = lg.expv_iters;
+    info->proj_err = lg.proj_err;
+    info->defect = lg.defect;
+    for (int j = 0; j < 4; ++j)
+        info->leak[j] = j < (int)lg.leak.size() ? lg.leak[j] : std::numeric_limits<double>::quiet_NaN();
+    info->n_probes = lg.n_probes;
+    info->n_selected = lg.n_selected;
+    info->n_krylov = lg.n_krylov;
+    info->n_h1 = lg.n_h1;
+    info->n_h1_closure = lg.n_h1_closure;
+    info->n_closure_sweeps = lg.n_closure_sweeps;
+    info->workcap_reached = lg.workcap_reached;
+    info->maxdim_reached = lg.maxdim_reached;
+    info->r_old = lg.r_old;
+    info->r_aug = lg.r_aug;
+    info->r_final = lg.r_final;
+    info->root_link = out.root_link;
+    info->root_rank = out.root_rank;
+    info->root_applications = out.root_applications;
+    for (int j = 0; j < 6; ++j) info->seconds[j] = out.seconds[j];
+    if (info->expanded)
+        for (int b = 0; b < std::min<int>(info->expanded_cap, (int)out.expanded.size()); ++b)
+            info->expanded[b] = out.expanded[b];
+    info->log_count = 0;
+    if (info->log)
+        for (const BondLogEntry& e : lg.log) {
+            if (info->log_count >= info->log_cap) break;
+            bug_bond_log& g = info->log[info->log_count++];
+            g.bond = e.bond;
+            g.side = e.side;
+            g.sample = e.sample;
+            g.tail = e.tail;
+            g.krylov_c = e.krylov_c;
+            g.krylov_d = e.krylov_d;
+            g.n_probes = e.n_probes;
+            g.n_holdout = e.n_holdout;
+            g.n_selected = e.n_selected;
+            g.n_targeted = e.n_targeted;
+            g.n_krylov = e.n_krylov;
+            g.n_h1 = e.n_h1;
+            g.workcap = e.workcap;
+            g.met = e.met;
+            g.unmet = e.unmet;
+            g.width = e.width;
+            g.r_old = e.r_old;
+            g.r_final = e.r_final;
+            g.discarded = e.discarded;
+        }
+}
+
+// ---- many instances stepped together ----------------------------------------------------------------
+
+// Worker contexts and streams of a main context's batched steps (created on first use, kept).
+struct BatchWorker {
+    bug_ctx* ctx = nullptr;
+    hipStream_t st = nullptr;
+};
+std::mutex g_batch_mu;
+std::map<bug_ctx*, std::vector<BatchWorker>> g_batch;
+
+std::vector<BatchWorker> batch_workers(bug_ctx* main, int n) {
+    std::lock_guard<std::mutex> lk(g_batch_mu);
+    std::vector<BatchWorker>& ws = g_batch[main];
+    while ((int)ws.size() < n) {
+        BatchWorker w;
+        if (bug_ctx_create(main->device, &w.ctx) != BUG_OK)
+            throw StepError(BUG_EDEVICE, "batch worker context creation failed");
+        if (hipStreamCreateWithFlags(&w.st, hipStreamNonBlocking) != hipSuccess)
+            throw StepError(BUG_EDEVICE, "batch worker stream creation failed");
+        ws.push_back(w);
+    }
+    return std::vector<BatchWorker>(ws.begin(), ws.begin() + n);
+}
+
+// f(i, exec) for every instance i < k on `workers` threads, each with its own context and stream;
+// every thread drains its stream before it takes the next instance and before it returns.
+template <typename F>
+void on_workers(bug_ctx* main, int k, int workers, int prec, F&& f) {
+    const int n = std::max(1, std::min(workers, k));
+    const std::vector<BatchWorker> ws = batch_workers(main, n);
+    std::atomic<int> next{0};
+    std::vector<int> rc(n, BUG_OK);
+    std::vector<std::string> msg(n);
+    std::vector<std::thread> th;
+    for (int w = 0; w < n; ++w)
+        th.emplace_back([&, w] {
+            (void)hipSetDevice(main->device);
+            set_thread_stream(ws[w].st);
+            const Exec e{ws[w].ctx, ws[w].st, prec};
+            for (int i = next.fetch_add(1); i < k && rc[w] == BUG_OK; i = next.fetch_add(1)) {
+                try {
+                    f(i, e);
+                    STEP_HIP(hipStreamSynchronize(ws[w].st));
+                } catch (const StepError& x) {
+                    rc[w] = x.code;
+                    msg[w] = "instance " + std::to_string(i) + ": " + x.what();
+                } catch (const bug::Error& x) {
+                    rc[w] = x.code;
+                    msg[w] = "instance " + std::to_string(i) + ": " + x.what();
+                } catch (const std::exception& x) {
+                    rc[w] = BUG_EDEVICE;
+                    msg[w] = "instance " + std::to_string(i) + ": " + x.what();
+                }
+                if (rc[w] != BUG_OK) next.store(k);     // the others stop after their instance
+            }
+            (void)hipStreamSynchronize(ws[w].st);
+        });
+    for (auto& t : th) t.join();
+    for (int w = 0; w < n; ++w)
+        if (rc[w] != BUG_OK) throw StepError(rc[w], msg[w]);
+}
+
+// K steps with one option set: each instance's stages up to the root solve on the worker threads,
+// every root exponential in one lockstep recurrence (grouped zero-site applies, segmented
+// Gram-Schmidt) on the caller's stream, then each instance's assembly and compression.
+void run_step_batch(bug_ctx* ctx, hipStream_t st, std::vector<MPS>& work,
+                    const std::vector<const std::vector<MpoSite>*>& mpos, const std::vector<cplx>& taus,
+                    const bug_step_opts& o, std::vector<StepOut>& outs, int workers) {
+    const int k = (int)work.size();
+    std::vector<RootProblem> rps(k);
+    outs.assign(k, StepOut());
+    on_workers(ctx, k, workers, o.precision, [&](int i, const Exec& e) {
+        step_prepare(e, work[i], *mpos[i], taus[i], o, outs[i], rps[i]);
+    });
+    // ---- the K root exponentials in lockstep ----
+    set_thread_stream(st);
+    const Exec ex{ctx, st, o.precision};
+    const double ts = now();
+    PackedSegs sg;
+    for (int i = 0; i < k; ++i) {
+        sg.table.push_back(sg.n);
+        sg.table.push_back(rps[i].start.nelem());
+        sg.n += rps[i].start.nelem();
+    }
+    DevBuf v0(sizeof(double2) * (size_t)std::max<int64_t>(sg.n, 1)), res(sizeof(double2) * (size_t)std::max<int64_t>(sg.n, 1));
+    for (int i = 0; i < k; ++i)
+        if (rps[i].start.nelem() > 0)
+            STEP_HIP(hipMemcpyAsync(static_cast<double2*>(v0.p) + sg.table[2 * i], rps[i].start.data(),
+                                    sizeof(double2) * (size_t)rps[i].start.nelem(), hipMemcpyDeviceToDevice, st));
+    std::vector<bug_bsm> Ld(k), Rd(k), Cd(k), Yd(k);
+    for (int i = 0; i < k; ++i) {
+        Ld[i] = rps[i].Lr.desc();
+        Rd[i] = rps[i].Rr.desc();
+        Cd[i] = rps[i].start.desc();
+        Yd[i] = Cd[i];
+    }
+    auto apply = [&](const double2* x, double2* y) {
+        for (int i = 0; i < k; ++i) {
+            Cd[i].data = const_cast<double2*>(x) + sg.table[2 * i];
+            Yd[i].data = y + sg.table[2 * i];
+        }
+        ex.check(bug_apply_zero_site_batch(ctx, k, Ld.data(), Rd.data(), Cd.data(), Yd.data(), nullptr,
+                                           o.precision, st));
+    };
+    std::vector<ExpvReport> reps;
+    const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    expv_lockstep(ex, sg, apply, taus, o.hermitian != 0, static_cast<const double2*>(v0.p), 1e-15,
+                  kRootMaxIter, o.reorth != 0, rps[0].te, static_cast<double2*>(res.p), reps, hw);
+    std::vector<Tensor> results(k);
+    for (int i = 0; i < k; ++i) {
+        results[i] = t_empty(ex, rps[i].start.L());
+        if (rps[i].start.nelem() > 0)
+            STEP_HIP(hipMemcpyAsync(results[i].data(), static_cast<double2*>(res.p) + sg.table[2 * i],
+                                    sizeof(double2) * (size_t)rps[i].start.nelem(), hipMemcpyDeviceToDevice, st));
+    }
+    STEP_HIP(hipStreamSynchronize(st));
+    const double troot = now() - ts;
+    for (int i = 0; i < k; ++i) step_root_report(rps[i], reps[i], reps[i].iterations, troot, outs[i]);
+    on_workers(ctx, k, workers, o.precision, [&](int i, const Exec& e) {
+        step_finish(e, work[i], rps[i], results[i], o, outs[i]);
+    });
+}
 
 int fail_with(bug_ctx* ctx, int code, const std::string& msg) {
     if (ctx) ctx->last_error = msg;
@@ -1878,66 +2111,40 @@ int bug_step(bug_ctx* ctx, bug_state* state, const bug_mpo* mpo, double tau_re, 
         StepOut out;
         run_step(ex, work, mpo->sites, cplx(tau_re, tau_im), *opts, out);
         state->psi = work;
-        if (info) {
-            const Ledger& lg = out.ledger;
-            info->tolerance_met = lg.tolerance_met ? 1 : 0;
-            info->unmet = lg.unmet;
-            info->max_sample = lg.max_sample;
-            info->max_tail = lg.max_tail;
-            info->max_krylov = lg.max_krylov;
-            info->max_krylov_c = lg.max_krylov_c;
-            info->max_krylov_d = lg.max_krylov_d;
-            info->discarded = lg.discarded;
-            info->worst_discarded = out.discarded;
-            info->expv_err = lg.expv_err;
-            info->expv_iters = lg.expv_iters;
-            info->proj_err = lg.proj_err;
-            info->defect = lg.defect;
-            for (int j = 0; j < 4; ++j)
-                info->leak[j] = j < (int)lg.leak.size() ? lg.leak[j] : std::numeric_limits<double>::quiet_NaN();
-            info->n_probes = lg.n_probes;
-            info->n_selected = lg.n_selected;
-            info->n_krylov = lg.n_krylov;
-            info->n_h1 = lg.n_h1;
-            info->n_h1_closure = lg.n_h1_closure;
-            info->n_closure_sweeps = lg.n_closure_sweeps;
-            info->workcap_reached = lg.workcap_reached;
-            info->maxdim_reached = lg.maxdim_reached;
-            info->r_old = lg.r_old;
-            info->r_aug = lg.r_aug;
-            info->r_final = lg.r_final;
-            info->root_link = out.root_link;
-            info->root_rank = out.root_rank;
-            info->root_applications = out.root_applications;
-            for (int j = 0; j < 6; ++j) info->seconds[j] = out.seconds[j];
-            if (info->expanded)
-                for (int b = 0; b < std::min<int>(info->expanded_cap, (int)out.expanded.size()); ++b)
-                    info->expanded[b] = out.expanded[b];
-            info->log_count = 0;
-            if (info->log)
-                for (const BondLogEntry& e : lg.log) {
-                    if (info->log_count >= info->log_cap) break;
-                    bug_bond_log& g = info->log[info->log_count++];
-                    g.bond = e.bond;
-                    g.side = e.side;
-                    g.sample = e.sample;
-                    g.tail = e.tail;
-                    g.krylov_c = e.krylov_c;
-                    g.krylov_d = e.krylov_d;
-                    g.n_probes = e.n_probes;
-                    g.n_holdout = e.n_holdout;
-                    g.n_selected = e.n_selected;
-                    g.n_targeted = e.n_targeted;
-                    g.n_krylov = e.n_krylov;
-                    g.n_h1 = e.n_h1;
-                    g.workcap = e.workcap;
-                    g.met = e.met;
-                    g.unmet = e.unmet;
-                    g.width = e.width;
-                    g.r_old = e.r_old;
-                    g.r_final = e.r_final;
-                    g.discarded = e.discarded;
-                }
+        if (info) fill_info(out, info);
+    });
+}
+
+int bug_step_batch(bug_ctx* ctx, int32_t k, bug_state* const* states, const bug_mpo* const* mpos,
+                   const double* tau, const bug_step_opts* opts, bug_step_info* infos, int32_t workers) {
+    return step_guard(ctx, [&] {
+        if (k < 0 || (k > 0 && (!states || !mpos || !tau || !opts)))
+            throw StepError(BUG_EINVAL, "bug_step_batch needs k states, k MPOs, 2k tau values and options");
+        if (k == 0) return;
+        if (workers < 1) throw StepError(BUG_EINVAL, "bug_step_batch needs at least one worker");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        std::vector<MPS> work;
+        std::vector<const std::vector<MpoSite>*> ms;
+        std::vector<cplx> taus;
+        for (int i = 0; i < k; ++i) {
+            if (!states[i] || !mpos[i]) throw StepError(BUG_EINVAL, "null state or MPO in the batch");
+            for (int j = 0; j < i; ++j)
+                if (states[j] == states[i]) throw StepError(BUG_EINVAL, "a state appears twice in the batch");
+            int rl = 0;
+            validate(*opts, states[i]->psi.size(), rl);
+            if ((int)mpos[i]->sites.size() != states[i]->psi.size())
+                throw StepError(BUG_EINVAL, "MPO and state lengths differ at instance " + std::to_string(i));
+            work.push_back(states[i]->psi);
+            ms.push_back(&mpos[i]->sites);
+            taus.emplace_back(tau[2 * i], tau[2 * i + 1]);
+        }
+        // every instance's pending work is ordered before the workers and the lockstep stream
+        for (int i = 0; i < k; ++i) HIP_CHECK(hipStreamSynchronize(states[i]->stream));
+        std::vector<StepOut> outs;
+        run_step_batch(ctx, states[0]->stream, work, ms, taus, *opts, outs, workers);
+        for (int i = 0; i < k; ++i) {
+            states[i]->psi = work[i];
+            if (infos) fill_info(outs[i], &infos[i]);
         }
     });
 }

@@ -27,6 +27,15 @@ struct StepError : std::runtime_error {
     StepError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
 
+// a HIP runtime call whose failure ends the step with BUG_EDEVICE
+#define STEP_HIP(expr)                                                                     \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            throw StepError(BUG_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+
 // ---- legs and layouts (the block structure of a family of shifted block-diagonal matrices) ----
 
 struct Leg {

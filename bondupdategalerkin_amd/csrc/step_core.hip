@@ -1,4 +1,5 @@
 // Host tensor algebra of the step-level entry point (see step_core.h).
+#include <limits>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -10,13 +11,6 @@
 
 namespace bug {
 namespace host {
-
-#define STEP_HIP(expr)                                                                     \
-    do {                                                                                   \
-        hipError_t _e = (expr);                                                            \
-        if (_e != hipSuccess)                                                              \
-            throw StepError(BUG_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
-    } while (0)
 
 void Exec::check(int rc) const {
     if (rc != BUG_OK) throw StepError(rc, bug_last_error(ctx));
@@ -1021,7 +1015,9 @@ void tridiag_eigen(const std::vector<double>& diag, const std::vector<double>& o
             int m = l;
             for (; m + 1 < n; ++m) {
                 const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
-                if (std::fabs(e[m]) <= 1e-17 * dd) break;
+                // negligible at one ulp of its diagonal neighbours (a smaller threshold can
+                // stall on entries that rounding keeps at the ulp level)
+                if (std::fabs(e[m]) <= std::numeric_limits<double>::epsilon() * dd) break;
             }
             if (m == l) break;
             if (++iter > 200) throw StepError(BUG_ENONFINITE, "tridiagonal eigensolver did not converge");
@@ -1207,6 +1203,38 @@ std::vector<cplx> eigvals(std::vector<cplx> A, int n) {
     std::vector<cplx> w;
     if (n == 0) return w;
     auto at = [&](int i, int j) -> cplx& { return A[(size_t)i * n + j]; };
+    auto cabs1 = [](cplx z) { return std::abs(z.real()) + std::abs(z.imag()); };
+    // balancing (Parlett-Reinsch): a diagonal similarity by powers of two that brings each
+    // row's and column's off-diagonal norms within a factor of two, so that the rounding of the
+    // iteration below is relative to the balanced norm and graded matrices lose no accuracy
+    for (bool again = true; again;) {
+        again = false;
+        for (int i = 0; i < n; ++i) {
+            double c = 0.0, r = 0.0;
+            for (int j = 0; j < n; ++j)
+                if (j != i) {
+                    c += cabs1(at(j, i));
+                    r += cabs1(at(i, j));
+                }
+            if (c == 0.0 || r == 0.0) continue;
+            double g = r / 2.0, f = 1.0;
+            const double sum = c + r;
+            while (c < g) {
+                f *= 2.0;
+                c *= 4.0;
+            }
+            g = r * 2.0;
+            while (c > g) {
+                f /= 2.0;
+                c /= 4.0;
+            }
+            if ((c + r) / f < 0.95 * sum) {
+                again = true;
+                for (int j = 0; j < n; ++j) at(i, j) /= f;
+                for (int j = 0; j < n; ++j) at(j, i) *= f;
+            }
+        }
+    }
     // Householder reduction to upper Hessenberg form
     for (int k = 0; k + 2 < n; ++k) {
         double alpha = 0.0;
@@ -1235,7 +1263,29 @@ std::vector<cplx> eigvals(std::vector<cplx> A, int n) {
             for (int j = k + 1; j < n; ++j) at(i, j) -= s * std::conj(v[j]);
         }
     }
-    // shifted QR on the Hessenberg matrix (Givens rotations, Wilkinson shifts, deflation)
+    // shifted QR on the Hessenberg matrix (Givens rotations, Wilkinson shifts, deflation).
+    // A subdiagonal entry is negligible at one ulp of its diagonal neighbours, refined by the
+    // Ahues-Tisseur test on the 2 x 2 block it couples (the criterion of LAPACK's xLAHQR): a
+    // threshold below one ulp can stall on entries that rounding keeps at that level.
+    const double ulp = std::numeric_limits<double>::epsilon();
+    const double smlnum = std::numeric_limits<double>::min() * ((double)n / ulp);
+    auto negligible = [&](int k) {
+        const cplx h = at(k, k - 1);
+        if (cabs1(h) <= smlnum) return true;
+        double tst = cabs1(at(k - 1, k - 1)) + cabs1(at(k, k));
+        if (tst == 0.0) {
+            if (k - 2 >= 0) tst += std::abs(at(k - 1, k - 2).real());
+            if (k + 1 < n) tst += std::abs(at(k + 1, k).real());
+        }
+        if (std::abs(h.real()) > ulp * tst) return false;
+        const double ab = std::max(cabs1(h), cabs1(at(k - 1, k)));
+        const double ba = std::min(cabs1(h), cabs1(at(k - 1, k)));
+        const double aa = std::max(cabs1(at(k, k)), cabs1(at(k - 1, k - 1) - at(k, k)));
+        const double bb = std::min(cabs1(at(k, k)), cabs1(at(k - 1, k - 1) - at(k, k)));
+        const double s = aa + ab;
+        return ba * (ab / s) <= std::max(smlnum, ulp * (bb * (aa / s)));
+    };
+    const int itmax = 30 * std::max(10, n);
     int hi = n - 1, iter = 0;
     while (hi >= 0) {
         if (hi == 0) {
@@ -1244,8 +1294,7 @@ std::vector<cplx> eigvals(std::vector<cplx> A, int n) {
         }
         int lo = hi;
         for (; lo > 0; --lo) {
-            const double sc = std::abs(at(lo, lo)) + std::abs(at(lo - 1, lo - 1));
-            if (std::abs(at(lo, lo - 1)) <= 1e-16 * (sc > 0 ? sc : 1.0)) {
+            if (negligible(lo)) {
                 at(lo, lo - 1) = 0.0;
                 break;
             }
@@ -1256,14 +1305,46 @@ std::vector<cplx> eigvals(std::vector<cplx> A, int n) {
             iter = 0;
             continue;
         }
-        if (++iter > 300) throw StepError(BUG_ENONFINITE, "Hessenberg QR did not converge");
-        // Wilkinson shift from the trailing 2 x 2 block (exceptional shift every 10 iterations)
-        const cplx a = at(hi - 1, hi - 1), b = at(hi - 1, hi), c = at(hi, hi - 1), d = at(hi, hi);
-        const cplx tr = a + d, det = a * d - b * c;
-        const cplx disc = std::sqrt(tr * tr - 4.0 * det);
-        const cplx l1 = 0.5 * (tr + disc), l2 = 0.5 * (tr - disc);
-        cplx mu = std::abs(l1 - d) < std::abs(l2 - d) ? l1 : l2;
-        if (iter % 10 == 0) mu = d + std::abs(at(hi, hi - 1));
+        if (++iter > itmax) {
+            // a block that rounding keeps from meeting the coupling test: split it at its
+            // relatively smallest subdiagonal entry (already below one ulp of its neighbours
+            // where the first test passed), which perturbs the eigenvalues by that entry
+            int kmin = lo + 1;
+            double best = std::numeric_limits<double>::infinity();
+            for (int k = lo + 1; k <= hi; ++k) {
+                const double tst = cabs1(at(k - 1, k - 1)) + cabs1(at(k, k));
+                const double rel = cabs1(at(k, k - 1)) / (tst > 0.0 ? tst : 1.0);
+                if (rel < best) {
+                    best = rel;
+                    kmin = k;
+                }
+            }
+            if (!std::isfinite(best)) throw StepError(BUG_ENONFINITE, "Hessenberg QR on a non-finite matrix");
+            at(kmin, kmin - 1) = 0.0;
+            iter = 0;
+            continue;
+        }
+        // shift: the eigenvalue of the trailing 2 x 2 block nearer its last diagonal entry, in the
+        // cancellation-free form of xLAHQR; exceptional shifts at the 10th and 20th iteration
+        // of a block (from its top, then its bottom subdiagonal)
+        cplx mu;
+        if (iter == 10) {
+            mu = 0.75 * std::abs(at(lo + 1, lo).real()) + at(lo, lo);
+        } else if (iter == 20) {
+            mu = 0.75 * std::abs(at(hi, hi - 1).real()) + at(hi, hi);
+        } else {
+            mu = at(hi, hi);
+            const cplx u = std::sqrt(at(hi - 1, hi)) * std::sqrt(at(hi, hi - 1));
+            double sc = cabs1(u);
+            if (sc != 0.0) {
+                const cplx x = 0.5 * (at(hi - 1, hi - 1) - mu);
+                const double sx = cabs1(x);
+                sc = std::max(sc, sx);
+                cplx y = sc * std::sqrt((x / sc) * (x / sc) + (u / sc) * (u / sc));
+                if (sx > 0.0 && (x / sx).real() * y.real() + (x / sx).imag() * y.imag() < 0.0) y = -y;
+                mu -= u * (u / (x + y));
+            }
+        }
         for (int i = lo; i <= hi; ++i) at(i, i) -= mu;
         std::vector<double> cs(hi - lo);
         std::vector<cplx> sn(hi - lo);

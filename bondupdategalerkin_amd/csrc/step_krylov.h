@@ -46,6 +46,25 @@ struct ExpvReport {
     double err = 0.0;
 };
 
+// K independent vectors packed in one device buffer of n elements: instance i occupies
+// [table[2i], table[2i] + table[2i+1]).
+struct PackedSegs {
+    std::vector<int64_t> table;
+    int64_t n = 0;
+    int K() const { return (int)(table.size() / 2); }
+};
+// y_i = A_i x_i for every instance, on packed buffers of PackedSegs::n elements
+using PackedApplyFn = std::function<void(const double2* x, double2* y)>;
+
+// exp(tau_i A_i) v0_i for K instances advanced in lockstep (lanczos_expv / arnoldi_expv per instance,
+// their Krylov passes side by side): one packed apply, one segmented multi-dot and one segmented
+// multi-axpy per recurrence step for all instances; each instance keeps its own substep search,
+// stopping rule and scale bookkeeping, and the host-side k x k work runs on `host_threads` threads.
+void expv_lockstep(const Exec& ex, const PackedSegs& sg, const PackedApplyFn& apply,
+                   const std::vector<cplx>& tau, bool hermitian, const double2* v0, double tol,
+                   int maxiter, bool reorth, double conv_tol, double2* out,
+                   std::vector<ExpvReport>& reps, int host_threads);
+
 Tensor lanczos_expv(const Exec& ex, const std::function<Tensor(const Tensor&)>& apply, cplx tau,
                     const Tensor& v0, double tol, int maxiter, bool reorth, double conv_tol,
                     ExpvReport& rep);

@@ -7,8 +7,13 @@
 // Krylov direction weights, src/BUG/Core/RankAdaptation/krylov_enrichment.jl:58-120; Test A's tail
 // rank, truncation.jl:262-278.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
+#include <exception>
 #include <limits>
+#include <mutex>
+#include <thread>
 
 #include "step_krylov.h"
 
@@ -388,6 +393,351 @@ Tensor arnoldi_expv(const Exec& ex, const std::function<Tensor(const Tensor&)>& 
         if (rep.substeps >= 10000) throw StepError(BUG_EINVARIANT, "expv: substep search failed to advance");
     }
     return finish_scale(ex, cur, logscale);
+}
+
+// ---- many instances in lockstep ---------------------------------------------------------------------
+
+namespace {
+
+// A fixed set of host threads running index ranges: run(count, f) calls f(i) for i < count on the
+// caller and the workers and returns when all calls are done (the first exception is rethrown).
+class HostPool {
+  public:
+    explicit HostPool(int threads) {
+        for (int t = 1; t < std::max(1, threads); ++t) workers_.emplace_back([this] { loop(); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& w : workers_) w.join();
+    }
+    template <typename F>
+    void run(int count, F&& f) {
+        if (count <= 0) return;
+        if (workers_.empty() || count == 1) {
+            for (int i = 0; i < count; ++i) f(i);
+            return;
+        }
+        std::function<void(int)> job = f;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &job;
+            count_ = count;
+            next_.store(0);
+            done_ = 0;
+            err_ = nullptr;
+            ++gen_;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return done_ == count_; });
+        job_ = nullptr;
+        if (err_) std::rethrow_exception(err_);
+    }
+
+  private:
+    void work() {
+        int did = 0;
+        std::exception_ptr err;
+        for (int i = next_.fetch_add(1); i < count_; i = next_.fetch_add(1)) {
+            try {
+                (*job_)(i);
+            } catch (...) {
+                if (!err) err = std::current_exception();
+            }
+            ++did;
+        }
+        std::lock_guard<std::mutex> lk(mu_);
+        if (err && !err_) err_ = err;
+        done_ += did;
+        if (done_ == count_) done_cv_.notify_all();
+    }
+    void loop() {
+        uint64_t seen = 0;
+        while (true) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || (gen_ != seen && job_); });
+                if (stop_) return;
+                seen = gen_;
+            }
+            work();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    std::function<void(int)>* job_ = nullptr;
+    int count_ = 0, done_ = 0;
+    std::atomic<int> next_{0};
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+    std::exception_ptr err_;
+};
+
+// packed vectors on the step's stream
+struct Packed {
+    const Exec& ex;
+    const PackedSegs& sg;
+    int K;
+    std::shared_ptr<DevBuf> fresh(bool zero) const {
+        auto b = std::make_shared<DevBuf>(sizeof(double2) * (size_t)std::max<int64_t>(sg.n, 1));
+        if (zero) STEP_HIP(hipMemsetAsync(b->p, 0, sizeof(double2) * (size_t)std::max<int64_t>(sg.n, 1), ex.st));
+        return b;
+    }
+    // out[j][i] = <xs[j]_i | y_i>
+    std::vector<std::vector<cplx>> dots(const std::vector<const double2*>& xs, const double2* y) const {
+        std::vector<std::vector<cplx>> out(xs.size(), std::vector<cplx>(K));
+        std::vector<double> res;
+        for (size_t lo = 0; lo < xs.size(); lo += 64) {
+            const int k = (int)std::min<size_t>(64, xs.size() - lo);
+            std::vector<const void*> ptrs(xs.begin() + lo, xs.begin() + lo + k);
+            res.assign((size_t)2 * k * K, 0.0);
+            ex.check(bug_dotc_segments(ex.ctx, sg.n, K, sg.table.data(), k, ptrs.data(), y, res.data(), ex.st));
+            for (int j = 0; j < k; ++j)
+                for (int i = 0; i < K; ++i)
+                    out[lo + j][i] = cplx(res[2 * ((size_t)j * K + i)], res[2 * ((size_t)j * K + i) + 1]);
+        }
+        return out;
+    }
+    std::vector<double> norms(const double2* y) const {
+        const auto d = dots({y}, y);
+        std::vector<double> n(K);
+        for (int i = 0; i < K; ++i) n[i] = std::sqrt(std::max(d[0][i].real(), 0.0));
+        return n;
+    }
+    // y_i += sum_j a[j][i] xs[j]_i
+    void axpy(const std::vector<std::vector<cplx>>& a, const std::vector<const double2*>& xs, double2* y) const {
+        std::vector<double> coef;
+        for (size_t lo = 0; lo < xs.size(); lo += 64) {
+            const int k = (int)std::min<size_t>(64, xs.size() - lo);
+            std::vector<const void*> ptrs(xs.begin() + lo, xs.begin() + lo + k);
+            coef.assign((size_t)2 * k * K, 0.0);
+            for (int j = 0; j < k; ++j)
+                for (int i = 0; i < K; ++i) {
+                    coef[2 * ((size_t)j * K + i)] = a[lo + j][i].real();
+                    coef[2 * ((size_t)j * K + i) + 1] = a[lo + j][i].imag();
+                }
+            ex.check(bug_axpy_segments(ex.ctx, sg.n, K, sg.table.data(), k, coef.data(), ptrs.data(), y, ex.st));
+        }
+    }
+    // a new vector with segment i equal to s[i] x_i
+    std::shared_ptr<DevBuf> scaled(const double2* x, const std::vector<cplx>& s) const {
+        auto b = fresh(true);
+        axpy({s}, {x}, static_cast<double2*>(b->p));
+        return b;
+    }
+};
+
+}  // namespace
+
+void expv_lockstep(const Exec& ex, const PackedSegs& sg, const PackedApplyFn& apply,
+                   const std::vector<cplx>& tau, bool hermitian, const double2* v0, double tol,
+                   int maxiter, bool reorth, double conv_tol, double2* out,
+                   std::vector<ExpvReport>& reps, int host_threads) {
+    const int K = sg.K();
+    if ((int)tau.size() != K) throw StepError(BUG_EINVAL, "one tau per instance");
+    reps.assign(K, ExpvReport());
+    if (K == 0) return;
+    const Packed pk{ex, sg, K};
+    HostPool pool(std::min(host_threads, K));
+    const std::vector<double> beta0 = pk.norms(v0);
+    std::vector<char> done(K, 0);
+    for (int i = 0; i < K; ++i) done[i] = beta0[i] == 0.0;
+    std::vector<cplx> tau_rem(tau), logscale(K, 0.0);
+    auto cur = pk.fresh(true);
+    pk.axpy({std::vector<cplx>(K, 1.0)}, {v0}, static_cast<double2*>(cur->p));
+    const int ld = maxiter;
+    auto any = [](const std::vector<char>& m) { return std::find(m.begin(), m.end(), 1) != m.end(); };
+    while (true) {
+        std::vector<char> act(K, 0);
+        for (int i = 0; i < K; ++i) act[i] = !done[i];
+        if (!any(act)) break;
+        for (int i = 0; i < K; ++i)
+            if (act[i]) reps[i].substeps += 1;
+        const std::vector<double> cn = pk.norms(static_cast<double2*>(cur->p));
+        for (int i = 0; i < K; ++i)
+            if (act[i] && cn[i] == 0.0) act[i] = 0, done[i] = 1;
+        if (!any(act)) break;
+        std::vector<cplx> inv(K, 0.0);
+        for (int i = 0; i < K; ++i)
+            if (act[i]) inv[i] = 1.0 / cn[i];
+        std::vector<std::shared_ptr<DevBuf>> basis{pk.scaled(static_cast<double2*>(cur->p), inv)};
+        auto bptr = [&](size_t j) { return static_cast<const double2*>(basis[j]->p); };
+        auto all_basis = [&] {
+            std::vector<const double2*> v;
+            for (size_t j = 0; j < basis.size(); ++j) v.push_back(bptr(j));
+            return v;
+        };
+        std::vector<char> pass(act);
+        std::vector<double> betak(K, 0.0);
+        std::vector<int> dim(K, 0);
+        std::vector<std::vector<double>> alpha(K), beta(K);
+        std::vector<std::vector<cplx>> H(hermitian ? 0 : K);
+        if (!hermitian)
+            for (int i = 0; i < K; ++i) H[i].assign((size_t)(maxiter + 1) * ld, 0.0);
+        auto masked = [&](std::vector<cplx> c) {
+            for (int i = 0; i < K; ++i)
+                if (!pass[i]) c[i] = 0.0;
+            return c;
+        };
+        auto r = pk.fresh(false);
+        double2* rp = static_cast<double2*>(r->p);
+        if (hermitian) {
+            apply(bptr(0), rp);
+            for (int i = 0; i < K; ++i)
+                if (pass[i]) reps[i].iterations += 1;
+            const auto a = pk.dots({bptr(0)}, rp)[0];
+            std::vector<cplx> na(K);
+            for (int i = 0; i < K; ++i) {
+                if (pass[i]) alpha[i].push_back(a[i].real());
+                na[i] = -a[i].real();
+            }
+            pk.axpy({masked(na)}, {bptr(0)}, rp);
+            for (int it = 2; it <= maxiter; ++it) {
+                const std::vector<double> b = pk.norms(rp);
+                pool.run(K, [&](int i) {
+                    if (!pass[i]) return;
+                    if (b[i] < tol ||
+                        (conv_tol > 0 && tail_sup(Tridiagonal(alpha[i], beta[i]), tau_rem[i], b[i], conv_tol) < conv_tol)) {
+                        pass[i] = 0;
+                        betak[i] = b[i];
+                    }
+                });
+                if (!any(pass)) break;
+                std::vector<cplx> ib(K, 0.0), nb(K, 0.0);
+                for (int i = 0; i < K; ++i)
+                    if (pass[i]) {
+                        beta[i].push_back(b[i]);
+                        ib[i] = 1.0 / b[i];
+                        nb[i] = -b[i];
+                    }
+                basis.push_back(pk.scaled(rp, ib));
+                const size_t last = basis.size() - 1;
+                apply(bptr(last), rp);
+                for (int i = 0; i < K; ++i)
+                    if (pass[i]) reps[i].iterations += 1;
+                const auto a2 = pk.dots({bptr(last)}, rp)[0];
+                std::vector<cplx> na2(K, 0.0);
+                for (int i = 0; i < K; ++i)
+                    if (pass[i]) {
+                        alpha[i].push_back(a2[i].real());
+                        na2[i] = -a2[i].real();
+                    }
+                pk.axpy({masked(na2), masked(nb)}, {bptr(last), bptr(last - 1)}, rp);
+                if (reorth)
+                    for (int p = 0; p < 2; ++p) {
+                        auto cs = pk.dots(all_basis(), rp);
+                        for (auto& row : cs) row = masked(row);
+                        for (auto& row : cs)
+                            for (auto& c : row) c = -c;
+                        pk.axpy(cs, all_basis(), rp);
+                    }
+            }
+            // instances whose pass ran to maxiter: the tail of their last residual
+            if (any(pass)) {
+                const std::vector<double> b = pk.norms(rp);
+                for (int i = 0; i < K; ++i)
+                    if (pass[i]) betak[i] = b[i], pass[i] = 0;
+            }
+            for (int i = 0; i < K; ++i) dim[i] = (int)alpha[i].size();
+        } else {
+            for (int k = 1; k <= maxiter; ++k) {
+                apply(bptr(k - 1), rp);
+                for (int i = 0; i < K; ++i)
+                    if (pass[i]) reps[i].iterations += 1;
+                for (int p = 0; p < 2; ++p) {
+                    auto cs = pk.dots(all_basis(), rp);
+                    for (auto& row : cs) row = masked(row);
+                    for (size_t j = 0; j < cs.size(); ++j)
+                        for (int i = 0; i < K; ++i) H[i][j * ld + (k - 1)] += cs[j][i];
+                    for (auto& row : cs)
+                        for (auto& c : row) c = -c;
+                    pk.axpy(cs, all_basis(), rp);
+                }
+                const std::vector<double> b = pk.norms(rp);
+                pool.run(K, [&](int i) {
+                    if (!pass[i]) return;
+                    dim[i] = k;
+                    betak[i] = b[i];
+                    bool stop = b[i] < tol;
+                    if (!stop && conv_tol > 0) {
+                        std::vector<cplx> S((size_t)k * k);
+                        for (int a = 0; a < k; ++a)
+                            for (int c = 0; c < k; ++c) S[(size_t)a * k + c] = H[i][(size_t)a * ld + c];
+                        stop = tail_sup(Hessenberg(S, k), tau_rem[i], b[i], conv_tol) < conv_tol;
+                    }
+                    if (!stop) {
+                        H[i][(size_t)k * ld + (k - 1)] = b[i];
+                        stop = k == maxiter;
+                    }
+                    if (stop) pass[i] = 0;
+                });
+                if (!any(pass)) break;
+                std::vector<cplx> ib(K, 0.0);
+                for (int i = 0; i < K; ++i)
+                    if (pass[i]) ib[i] = 1.0 / b[i];
+                basis.push_back(pk.scaled(rp, ib));
+            }
+        }
+        // each instance's substep: step size, coefficients, error bound, new current vector
+        std::vector<std::vector<cplx>> coef(basis.size(), std::vector<cplx>(K, 0.0));
+        std::vector<cplx> ls(K, 0.0), tau_s(K, 0.0);
+        std::vector<double> err(K, 0.0);
+        pool.run(K, [&](int i) {
+            if (!act[i]) return;
+            std::vector<cplx> c;
+            if (hermitian) {
+                const Tridiagonal tri(alpha[i], beta[i]);
+                tau_s[i] = pick_tau(tri, tau_rem[i], betak[i], conv_tol);
+                c = tri.coeffs(tau_s[i], ls[i]);
+                err[i] = tail_sup(tri, tau_s[i], betak[i], std::numeric_limits<double>::infinity());
+            } else {
+                const int d = dim[i];
+                std::vector<cplx> S((size_t)d * d);
+                for (int a = 0; a < d; ++a)
+                    for (int b = 0; b < d; ++b) S[(size_t)a * d + b] = H[i][(size_t)a * ld + b];
+                const Hessenberg hess(S, d);
+                tau_s[i] = pick_tau(hess, tau_rem[i], betak[i], conv_tol);
+                c = hess.coeffs(tau_s[i], ls[i]);
+                err[i] = tail_sup(hess, tau_s[i], betak[i], std::numeric_limits<double>::infinity());
+            }
+            const size_t m = std::min<size_t>(std::min<size_t>(c.size(), (size_t)dim[i]), basis.size());
+            for (size_t j = 0; j < m; ++j) coef[j][i] = c[j] * cn[i];
+        });
+        auto next = pk.fresh(true);
+        pk.axpy(coef, all_basis(), static_cast<double2*>(next->p));
+        // instances that took no substep keep their current vector
+        std::vector<cplx> keep(K, 0.0);
+        bool any_keep = false;
+        for (int i = 0; i < K; ++i)
+            if (!act[i]) keep[i] = 1.0, any_keep = true;
+        if (any_keep) pk.axpy({keep}, {static_cast<const double2*>(cur->p)}, static_cast<double2*>(next->p));
+        cur = next;
+        for (int i = 0; i < K; ++i) {
+            if (!act[i]) continue;
+            reps[i].err += err[i] * cn[i] / beta0[i];
+            logscale[i] += ls[i];
+            tau_rem[i] -= tau_s[i];
+            if (std::abs(tau_rem[i]) <= 1e-14 * std::max(std::abs(tau[i]), 1.0)) done[i] = 1;
+            if (!done[i] && reps[i].substeps >= 10000)
+                throw StepError(BUG_EINVARIANT, "expv: substep search failed to advance");
+        }
+    }
+    // the accumulated scale, where it neither vanishes nor overflows (finish_scale per instance)
+    const std::vector<double> nrm = pk.norms(static_cast<double2*>(cur->p));
+    std::vector<cplx> sc(K, 1.0);
+    for (int i = 0; i < K; ++i) {
+        const cplx e = std::exp(logscale[i]);
+        if (beta0[i] == 0.0 || !(std::isfinite(std::abs(e)) && e != 0.0)) continue;
+        if (!(std::isfinite(nrm[i]) && nrm[i] > 0 && std::abs(e) * nrm[i] > kMaxNorm)) sc[i] = e;
+    }
+    STEP_HIP(hipMemsetAsync(out, 0, sizeof(double2) * (size_t)std::max<int64_t>(sg.n, 1), ex.st));
+    pk.axpy({sc}, {static_cast<const double2*>(cur->p)}, out);
 }
 
 }  // namespace host

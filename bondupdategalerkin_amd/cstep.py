@@ -209,6 +209,17 @@ def step_options(*, order: int = 1, maxdim: Optional[int] = None, hermitian: boo
     return o
 
 
+def _info_buffers(info, L: int):
+    """Attach the expanded-width and bond-log arrays the library fills to an info POD."""
+    exp = np.zeros(max(L - 1, 1), np.int32)
+    logs = (_lib.BugBondLog * (4 * L))()
+    info.expanded = _lib.ptr32(exp)
+    info.expanded_cap = L - 1
+    info.log = logs
+    info.log_cap = 4 * L
+    return exp, logs
+
+
 def bug_step(state: CState, mpo: CMPO, tau: complex, **kw) -> Dict:
     """One ``cbe_bug_step!`` through the C entry point; returns the report as a dict (the verdict,
     the ledger maxima and counts, ``unmet`` names, the expanded widths and the per-bond log)."""
@@ -216,20 +227,46 @@ def bug_step(state: CState, mpo: CMPO, tau: complex, **kw) -> Dict:
     opts = step_options(**kw)
     tau = complex(tau)
     info = _lib.BugStepInfo()
-    exp = np.zeros(max(state.L - 1, 1), np.int32)
-    logs = (_lib.BugBondLog * (4 * state.L))()
-    info.expanded = _lib.ptr32(exp)
-    info.expanded_cap = state.L - 1
-    info.log = logs
-    info.log_cap = 4 * state.L
+    exp, logs = _info_buffers(info, state.L)
     ctx.check(ctx._lib.bug_step(ctx.handle, state.handle, mpo.handle, tau.real, tau.imag,
                                 ctypes.byref(opts), ctypes.byref(info)))
+    return _report(info, exp, logs, state.L)
+
+
+def bug_step_batch(states: Sequence[CState], mpos: Sequence[CMPO], taus, workers: int = 8,
+                   **kw) -> List[Dict]:
+    """K independent steps with one option set through ``bug_step_batch``: the stages before the
+    root solve per instance on ``workers`` host threads, the K root exponentials as one lockstep
+    Krylov recurrence over grouped launches, then each instance's compression.  ``taus``: one
+    step per instance (a scalar applies to all).  Returns each instance's report."""
+    K = len(states)
+    if len(mpos) != K:
+        raise ValueError("one MPO per state")
+    if np.isscalar(taus):
+        taus = [taus] * K
+    if len(taus) != K:
+        raise ValueError("one tau per state")
+    ctx = _lib.context()
+    opts = step_options(**kw)
+    tv = np.zeros(2 * max(K, 1))
+    tv[0::2] = [complex(t).real for t in taus]
+    tv[1::2] = [complex(t).imag for t in taus]
+    infos = (_lib.BugStepInfo * max(K, 1))()
+    bufs = [_info_buffers(infos[i], states[i].L) for i in range(K)]
+    sh = (ctypes.c_void_p * max(K, 1))(*[s.handle for s in states])
+    mh = (ctypes.c_void_p * max(K, 1))(*[m.handle for m in mpos])
+    ctx.check(ctx._lib.bug_step_batch(ctx.handle, K, sh, mh, tv.ctypes.data_as(_lib._f64p),
+                                      ctypes.byref(opts), infos, workers))
+    return [_report(infos[i], *bufs[i], states[i].L) for i in range(K)]
+
+
+def _report(info, exp, logs, L: int) -> Dict:
     out = {f: getattr(info, f) for f, _ in _lib.BugStepInfo._fields_
            if f not in ("expanded", "expanded_cap", "log", "log_cap", "leak", "seconds")}
     out["leak"] = [info.leak[j] for j in range(4)]
     out["seconds"] = dict(zip(("left_sweep", "right_sweep", "closure", "root", "truncation",
                                "step"), [info.seconds[j] for j in range(6)]))
-    out["expanded"] = [int(x) for x in exp[:state.L - 1]]
+    out["expanded"] = [int(x) for x in exp[:L - 1]]
     out["unmet_names"] = [name for bit, name in UNMET_BITS if info.unmet & bit]
     out["status"] = "met" if info.tolerance_met else "budget_unmet"
     out["bond_log"] = [{f: getattr(logs[i], f) for f, _ in _lib.BugBondLog._fields_}

@@ -33,6 +33,7 @@ import numpy as np
 import torch
 
 from . import bug_step as B
+from . import cstep as C
 from . import mps as M
 from . import qtt as Q
 from .blocks import device
@@ -44,7 +45,9 @@ class Request:
     """One evolution.  ``kind`` "chain": ``L``, ``delta``, ``start`` ("dw" or "neel"),
     ``maxdim``, ``steps``, ``dt`` (tau = -i dt); ``kind`` "heston": ``bits`` per axis,
     ``params`` (r, q, kappa, theta, sigma, rho), ``maxdim``, ``steps``, ``dt`` (tau = dt, real,
-    hermitian = False).  ``order``, ``precision`` and ``seed`` apply to both."""
+    hermitian = False).  ``order``, ``precision`` and ``seed`` apply to both.  ``driver`` "c"
+    steps through the library's step-level entry point ``bug_step`` (the step orchestrated in
+    C++), "python" through the package's ``cbe_bug_step``; both run the same kernels."""
 
     kind: str
     params: Dict
@@ -52,6 +55,7 @@ class Request:
     precision: str = "mixed"
     seed: Optional[int] = None
     keep_state: bool = False
+    driver: str = "c"
     rid: int = -1
     arrival: float = 0.0
 
@@ -132,16 +136,34 @@ def run_request(req: Request, stop: Optional[threading.Event] = None):
     (``Engine.cancel``) it ends between two steps with ``Cancelled``."""
     psi, mpo, tau, herm = _build(req)
     form = precision_used(req)
+    if req.driver not in ("c", "python"):
+        raise ValueError(f"unknown driver {req.driver!r}")
+    kw = dict(order=req.order, maxdim=req.params["maxdim"], hermitian=herm,
+              work_ceiling_rule="maxdim", precision=form, rng=req.seed)
     statuses = []
-    for _ in range(req.params["steps"]):
-        if stop is not None and stop.is_set():
-            raise Cancelled(f"cancelled after {len(statuses)} of {req.params['steps']} steps")
-        info = B.cbe_bug_step(psi, mpo, tau, order=req.order, maxdim=req.params["maxdim"],
-                              hermitian=herm, work_ceiling_rule="maxdim",
-                              precision=form, rng=req.seed)
-        statuses.append(B.budget_status(info.ledger))
-    obs = M.magnetisation(psi) if req.kind == "chain" else np.array(psi.bond_dims())
-    torch.cuda.current_stream().synchronize()
+    cstate = cmpo = None
+    if req.driver == "c":
+        cstate, cmpo = C.CState(psi), C.CMPO(mpo)
+    try:
+        for _ in range(req.params["steps"]):
+            if stop is not None and stop.is_set():
+                raise Cancelled(f"cancelled after {len(statuses)} of {req.params['steps']} steps")
+            if cstate is not None:
+                statuses.append(C.bug_step(cstate, cmpo, tau, **kw)["status"])
+            else:
+                info = B.cbe_bug_step(psi, mpo, tau, **kw)
+                statuses.append(B.budget_status(info.ledger))
+        if cstate is not None:
+            obs = (cstate.expect_diagonal() if req.kind == "chain"
+                   else np.array(cstate.bond_dims()))
+            psi = cstate.to_mps() if req.keep_state else None
+        else:
+            obs = M.magnetisation(psi) if req.kind == "chain" else np.array(psi.bond_dims())
+        torch.cuda.current_stream().synchronize()
+    finally:
+        if cstate is not None:
+            cstate.close()
+            cmpo.close()
     return psi, statuses, obs
 
 

@@ -216,6 +216,13 @@ int bug_apply_one_site_batch(bug_ctx* ctx, int32_t k, const bug_bsm* L, const bu
  */
 int bug_apply_zero_site(bug_ctx* ctx, const bug_bsm* L, const bug_bsm* R, const bug_bsm* C,
                         const bug_bsm* Y, double shift, int32_t precision, void* stream);
+/* k independent zero-site (root) applies Y_i = (H0_i - shifts[i]) C_i (shifts may be NULL) with one
+ * grouped-GEMM launch per product stage for all of them, packed as in bug_apply_one_site_batch.
+ * The reference solves one root at a time (effective_operators.jl:91-107, root_methods.jl:
+ * 359-402); this entry point carries the lockstep root solve of bug_step_batch. */
+int bug_apply_zero_site_batch(bug_ctx* ctx, int32_t k, const bug_bsm* L, const bug_bsm* R,
+                              const bug_bsm* C, const bug_bsm* Y, const double* shifts,
+                              int32_t precision, void* stream);
 
 /* ---- vectors (Krylov recurrences: tensor_inner, _cgs2!, _combine; krylov.jl:296-332) ------- */
 
@@ -495,6 +502,18 @@ int bug_state_download_site(bug_ctx* ctx, const bug_state* state, int32_t site,
                             int32_t* col_dim, int64_t* offset, void* host_data);
 int bug_step(bug_ctx* ctx, bug_state* state, const bug_mpo* mpo, double tau_re, double tau_im,
              const bug_step_opts* opts, bug_step_info* info);
+/* k independent steps with one option set, advanced together (cbe_bug_step! per instance,
+ * bug_step.jl:17-510; the reference steps one state at a time and has no counterpart): the
+ * stages up to the root solve run per instance on `workers` host threads, each with its own
+ * library context and HIP stream; the k root exponentials run as ONE lockstep Krylov recurrence
+ * (bug_apply_zero_site_batch for all roots per iteration, segmented Gram-Schmidt, each instance
+ * with its own substep search and stopping rule); then each instance's assembly and
+ * compression.  tau[2i], tau[2i+1] = re, im of instance i's step; infos (k entries) may be NULL.
+ * Each instance's result equals its own bug_step to rounding (the lockstep sums its inner
+ * products per segment).  On an error no state is changed. */
+int bug_step_batch(bug_ctx* ctx, int32_t k, bug_state* const* states, const bug_mpo* const* mpos,
+                   const double* tau, const bug_step_opts* opts, bug_step_info* infos,
+                   int32_t workers);
 
 #ifdef __cplusplus
 }

@@ -172,7 +172,7 @@ def _dense_apply(M):
     return apply
 
 
-def _dense_operator(n, seed, hermitian):
+def _dense_operator(n, seed, hermitian, stiff=False):
     import torch
 
     from bondupdategalerkin_amd.blocks import BlockTensor, Layout, Leg, device
@@ -181,6 +181,11 @@ def _dense_operator(n, seed, hermitian):
     X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
     M = (X + X.conj().T) / 2 if hermitian else X
     M = M / np.linalg.norm(M, 2) * 3.0
+    if stiff:
+        # a stiff, strongly non-normal generator like a discretised drift-diffusion: decay rates
+        # over three decades on the diagonal, one-sided transport above it, a dense weak coupling
+        M = (np.diag(-np.exp(np.linspace(0.0, 7.0, n))) + np.diag(np.full(n - 1, 60.0), 1)
+             + 0.3 * M)
     lay = Layout(Leg((0,), (n,)), Leg((0,), (n,)), (0,))
     v = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     return M, BlockTensor.from_blocks(lay, {(0, 0): M}), v, torch.from_numpy(v).to(device())
@@ -192,11 +197,15 @@ def _dense_operator(n, seed, hermitian):
     (64, True, -2.0j, 12, 1e-13),
     (40, False, -0.3 + 0.2j, 30, 0.0),
     (40, False, -1.5j, 10, 1e-12),
+    (80, False, 0.05, 30, 1e-10),
 ])
 def test_expv_matches_the_matrix_exponential(gpu, n, hermitian, tau, maxiter, conv_tol):
+    """``bug_expv`` against scipy's expm; the last case is stiff (||tau M|| ~ 55 with a
+    non-normal drift) and needs the substep search and the Hessenberg eigenvalues of many
+    graded Krylov blocks."""
     from bondupdategalerkin_amd import cstep as C
 
-    M, Md, v, vd = _dense_operator(n, 7 + n, hermitian)
+    M, Md, v, vd = _dense_operator(n, 7 + n, hermitian, stiff=n == 80)
     print(f"case n={n} hermitian={hermitian} tau={tau} maxiter={maxiter}", flush=True)
     out, rep = C.expv(_dense_apply(Md), vd, tau, hermitian=hermitian,
                       tol=1e-15 if hermitian else 1e-13, maxiter=maxiter, conv_tol=conv_tol)

@@ -88,3 +88,22 @@ def test_admission_holds_requests_beyond_the_budget(gpu):
     spans = sorted((r.admitted, r.finished) for r in res)
     assert all(spans[i + 1][0] >= spans[i][1] for i in range(len(spans) - 1))
     assert all(r.error is None and r.statuses for r in res)
+
+
+def test_c_and_python_drivers_agree(gpu):
+    """A request stepped through the library's ``bug_step`` (the default driver) ends where the
+    package's ``cbe_bug_step`` ends: the same statuses, states within 1e-10 (both orchestrate
+    the same kernels; their host-side eigen- and exponential solvers round differently)."""
+    from bondupdategalerkin_amd.serving import run_request
+
+    for r in _mix()[:3]:
+        outs = []
+        for drv in ("c", "python"):
+            r.driver = drv
+            psi, statuses, obs = run_request(r)
+            res = type("R", (), {"state": psi, "kind": r.kind})
+            outs.append((statuses, _dense(res), obs))
+        (sa, va, oa), (sb, vb, ob) = outs
+        assert sa == sb, r.kind
+        assert np.linalg.norm(va - vb) <= 1e-10 * np.linalg.norm(vb), r.kind
+        assert np.allclose(oa, ob, atol=1e-10), r.kind

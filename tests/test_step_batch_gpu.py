@@ -1,0 +1,110 @@
+"""``bug_step_batch``: K independent instances stepped together (the stages before the root solve
+per instance on worker threads, the K root exponentials as one lockstep Krylov recurrence over
+grouped launches) against each instance's own ``bug_step`` (GPU).
+
+The bar is the step's: relative state difference <= 1e-10 per instance and step, identical
+verdicts, bond dimensions and root Krylov iteration counts (the lockstep sums inner products per
+segment, so the two differ by rounding only)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import models as om
+from tests.helpers import fam_from_device
+
+pytestmark = pytest.mark.gpu
+
+
+def _heston(bits, prm):
+    from bondupdategalerkin_amd import qtt as Q
+
+    g = Q.qtt_grid(bits, bits, x=(-1.0, 1.0), y=(0.0, 0.5), periodic_x=False, periodic_y=False)
+    init = Q.qtt_separable(g, lambda x: math.exp(0.5 * x), lambda v: math.exp(-v))
+    return Q.qtt_mps(init), Q.heston_mpo(g, Q.HestonParams(*prm))
+
+
+_PARAMS = [(0.03, 0.01, 1.5, 0.09, 0.5, -0.7), (0.0, 0.02, 3.2, 0.04, 0.2, -0.1),
+           (0.05, 0.0, 0.7, 0.12, 0.75, -0.85), (0.01, 0.04, 2.4, 0.06, 0.35, -0.4),
+           (0.02, 0.03, 1.1, 0.15, 0.6, 0.0)]
+
+
+def _dense(cs):
+    from bondupdategalerkin_amd import qtt as Q
+
+    return Q.tt_dense(Q.qtt_cores(cs.to_mps()))
+
+
+def _chain_dense(cs):
+    return om.mps_dense([fam_from_device(t) for t in cs.to_mps().tensors])
+
+
+@pytest.mark.parametrize("bits,workers,order", [(4, 3, 2), (5, 2, 2), (4, 5, 1)])
+def test_heston_batch_equals_single_steps(gpu, bits, workers, order):
+    """Non-Hermitian roots (Arnoldi with substeps): five Heston parameter sets, two steps."""
+    from bondupdategalerkin_amd import cstep as C
+
+    kw = dict(order=order, maxdim=32, hermitian=False, work_ceiling_rule="maxdim", tol_basis=1e-8,
+              tol_trunc=1e-12)
+    built = [_heston(bits, p) for p in _PARAMS]
+    single = [(C.CState(psi), C.CMPO(mpo)) for psi, mpo in built]
+    batch = [(C.CState(psi), C.CMPO(mpo)) for psi, mpo in built]
+    taus = [0.02, 0.015, 0.02, 0.025, 0.01]
+    for step in range(2):
+        print(f"case bits={bits} workers={workers} order={order} step={step}", flush=True)
+        ref = [C.bug_step(s, m, t, **kw) for (s, m), t in zip(single, taus)]
+        got = C.bug_step_batch([s for s, _ in batch], [m for _, m in batch], taus,
+                               workers=workers, **kw)
+        for i, (r, g) in enumerate(zip(ref, got)):
+            assert g["status"] == r["status"] and g["unmet"] == r["unmet"], i
+            assert g["expv_iters"] == r["expv_iters"], (i, g["expv_iters"], r["expv_iters"])
+            assert g["expanded"] == r["expanded"], i
+            assert batch[i][0].bond_dims() == single[i][0].bond_dims(), i
+            a, b = _dense(batch[i][0]), _dense(single[i][0])
+            assert np.linalg.norm(a - b) <= 1e-10 * np.linalg.norm(b), (i, step)
+            assert abs(g["expv_err"] - r["expv_err"]) <= 1e-6 * max(r["expv_err"], 1e-300) + 1e-18
+
+
+@pytest.mark.parametrize("parallel,precision", [(False, "fp64"), (True, "fp64"),
+                                                (False, "mixed")])
+def test_chain_batch_equals_single_steps(gpu, parallel, precision):
+    """Hermitian roots (Lanczos): XXZ chains of different anisotropy and start, one option set."""
+    from bondupdategalerkin_amd import cstep as C
+    from bondupdategalerkin_amd import mps
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    specs = [(mps.neel_state, 10, 0.5), (mps.domain_wall_state, 10, 0.0),
+             (mps.neel_state, 10, 1.0), (mps.domain_wall_state, 10, 0.3)]
+    kw = dict(order=2, maxdim=32, work_ceiling_rule="maxdim", parallel=parallel,
+              precision=precision)
+    single = [(C.CState(make(L)), C.CMPO(xxz_mpo(L, delta=d))) for make, L, d in specs]
+    batch = [(C.CState(make(L)), C.CMPO(xxz_mpo(L, delta=d))) for make, L, d in specs]
+    for step in range(3):
+        print(f"case chains parallel={parallel} {precision} step={step}", flush=True)
+        ref = [C.bug_step(s, m, -0.05j, **kw) for s, m in single]
+        got = C.bug_step_batch([s for s, _ in batch], [m for _, m in batch], -0.05j, workers=2,
+                               **kw)
+        for i, (r, g) in enumerate(zip(ref, got)):
+            assert g["status"] == r["status"] and g["expv_iters"] == r["expv_iters"], i
+            assert batch[i][0].bond_dims() == single[i][0].bond_dims(), i
+            a, b = _chain_dense(batch[i][0]), _chain_dense(single[i][0])
+            assert np.linalg.norm(a - b) <= 1e-10 * np.linalg.norm(b), (i, step)
+            assert np.abs(batch[i][0].expect_diagonal() - single[i][0].expect_diagonal()).max() <= 1e-9
+
+
+def test_batch_refuses_bad_input_and_leaves_states(gpu):
+    from bondupdategalerkin_amd import _lib
+    from bondupdategalerkin_amd import cstep as C
+    from bondupdategalerkin_amd import mps
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    a = C.CState(mps.neel_state(8))
+    m8, m6 = C.CMPO(xxz_mpo(8)), C.CMPO(xxz_mpo(6))
+    before = _chain_dense(a)
+    with pytest.raises(_lib.BugError):
+        C.bug_step_batch([a, a], [m8, m8], -0.05j, order=2)          # the same state twice
+    with pytest.raises(_lib.BugError):
+        C.bug_step_batch([a], [m6], -0.05j, order=2)                 # lengths differ
+    with pytest.raises(_lib.BugError):
+        C.bug_step_batch([a], [m8], -0.05j, workers=0)
+    assert np.array_equal(_chain_dense(a), before)
