@@ -84,7 +84,7 @@ def parse():
                    help="Heston instances (2^10 x 2^10 grid) stepped together through "
                         "bug_step_batch (0: skip)")
     p.add_argument("--heston-steps", type=int, default=2, help="batched Heston steps")
-    p.add_argument("--heston-workers", type=int, default=8,
+    p.add_argument("--heston-workers", type=int, default=16,
                    help="worker threads of the batched step's per-instance stages")
     p.add_argument("--autotune", choices=["table", "tune", "off"], default="table",
                    help="table: grouped-GEMM schedules from the package's tuning table for "

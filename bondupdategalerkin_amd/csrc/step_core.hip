@@ -1075,14 +1075,32 @@ void tridiag_eigen(const std::vector<double>& diag, const std::vector<double>& o
 namespace {
 using CM = std::vector<cplx>;   // row-major n x n
 
+// C = A B with the real and imaginary parts of B and C held apart, so that the inner loop over a
+// row is plain real arithmetic the compiler vectorizes (the host side of every Krylov step runs
+// many of these on k x k blocks, k <= 65)
 CM matmul(const CM& A, const CM& B, int n) {
-    CM C((size_t)n * n, 0.0);
-    for (int i = 0; i < n; ++i)
+    const size_t nn = (size_t)n * n;
+    std::vector<double> br(nn), bi(nn), cr(nn, 0.0), ci(nn, 0.0);
+    for (size_t e = 0; e < nn; ++e) {
+        br[e] = B[e].real();
+        bi[e] = B[e].imag();
+    }
+    for (int i = 0; i < n; ++i) {
+        double* __restrict r = &cr[(size_t)i * n];
+        double* __restrict m = &ci[(size_t)i * n];
         for (int k = 0; k < n; ++k) {
-            const cplx a = A[(size_t)i * n + k];
-            if (a == 0.0) continue;
-            for (int j = 0; j < n; ++j) C[(size_t)i * n + j] += a * B[(size_t)k * n + j];
+            const double ar = A[(size_t)i * n + k].real(), ai = A[(size_t)i * n + k].imag();
+            if (ar == 0.0 && ai == 0.0) continue;
+            const double* __restrict xr = &br[(size_t)k * n];
+            const double* __restrict xi = &bi[(size_t)k * n];
+            for (int j = 0; j < n; ++j) {
+                r[j] += ar * xr[j] - ai * xi[j];
+                m[j] += ar * xi[j] + ai * xr[j];
+            }
         }
+    }
+    CM C(nn);
+    for (size_t e = 0; e < nn; ++e) C[e] = cplx(cr[e], ci[e]);
     return C;
 }
 
@@ -1237,10 +1255,11 @@ std::vector<cplx> eigvals(std::vector<cplx> A, int n) {
     }
     // Householder reduction to upper Hessenberg form
     for (int k = 0; k + 2 < n; ++k) {
-        double alpha = 0.0;
-        for (int i = k + 1; i < n; ++i) alpha += std::norm(at(i, k));
+        double below = 0.0;                  // the column under the subdiagonal
+        for (int i = k + 2; i < n; ++i) below += std::norm(at(i, k));
+        if (below == 0.0) continue;          // already Hessenberg there (Arnoldi's H always is)
+        double alpha = std::norm(at(k + 1, k)) + below;
         alpha = std::sqrt(alpha);
-        if (alpha == 0.0) continue;
         const cplx x0 = at(k + 1, k);
         const cplx ph = std::abs(x0) > 0 ? x0 / std::abs(x0) : cplx(1.0);
         std::vector<cplx> v(n, 0.0);
@@ -1360,19 +1379,31 @@ std::vector<cplx> eigvals(std::vector<cplx> A, int n) {
             }
             cs[k - lo] = cc;
             sn[k - lo] = ss;
-            for (int j = k; j < n; ++j) {
-                const cplx p = at(k, j), q = at(k + 1, j);
-                at(k, j) = cc * p + ss * q;
-                at(k + 1, j) = -std::conj(ss) * p + cc * q;
+            // rows k, k+1 <- [c s; -conj(s) c] rows k, k+1 within the active block (only the
+            // eigenvalues are wanted, so the parts of the Schur form outside it are not kept)
+            const double sr = ss.real(), si = ss.imag();
+            double* __restrict rk = reinterpret_cast<double*>(&at(k, 0));
+            double* __restrict rk1 = reinterpret_cast<double*>(&at(k + 1, 0));
+            for (int j = k; j <= hi; ++j) {
+                const double pr = rk[2 * j], pi = rk[2 * j + 1], qr = rk1[2 * j], qi = rk1[2 * j + 1];
+                rk[2 * j] = cc * pr + (sr * qr - si * qi);
+                rk[2 * j + 1] = cc * pi + (sr * qi + si * qr);
+                rk1[2 * j] = -(sr * pr + si * pi) + cc * qr;
+                rk1[2 * j + 1] = -(sr * pi - si * pr) + cc * qi;
             }
         }
         for (int k = lo; k < hi; ++k) {
             const double cc = cs[k - lo];
-            const cplx ss = sn[k - lo];
-            for (int i = 0; i <= std::min(k + 2, hi); ++i) {
-                const cplx p = at(i, k), q = at(i, k + 1);
-                at(i, k) = cc * p + std::conj(ss) * q;
-                at(i, k + 1) = -ss * p + cc * q;
+            const double sr = sn[k - lo].real(), si = sn[k - lo].imag();
+            // columns k, k+1 <- columns k, k+1 [c -s; conj(s) c]
+            for (int i = lo; i <= std::min(k + 2, hi); ++i) {
+                double* a0 = reinterpret_cast<double*>(&at(i, k));
+                double* a1 = reinterpret_cast<double*>(&at(i, k + 1));
+                const double pr = a0[0], pi = a0[1], qr = a1[0], qi = a1[1];
+                a0[0] = cc * pr + (sr * qr + si * qi);
+                a0[1] = cc * pi + (sr * qi - si * qr);
+                a1[0] = -(sr * pr - si * pi) + cc * qr;
+                a1[1] = -(sr * pi + si * pr) + cc * qi;
             }
         }
         for (int i = lo; i <= hi; ++i) at(i, i) += mu;

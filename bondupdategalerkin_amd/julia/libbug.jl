@@ -381,27 +381,18 @@ function download_site(st::DeviceState, site::Integer)
     return (; row = collect(zip(rq, rd)), col = collect(zip(cq, cd)), offset, data)
 end
 
-"""
-    cbe_bug_step_device!(st, mpo, tau; kwargs...) -> NamedTuple
-
-`cbe_bug_step!(psi, mpo, tau; kwargs...)` (bug_step.jl:17-148) on a `DeviceState`: the same
-keywords (the `DDQRExpansion(:rank_tail)` expansion; the adaptive controller, `exact`,
-`keep_states`, `fold_omega = false` and `krylov_rank_tol > 0` are refused by the library with
-`ArgumentError` before the state is touched), the ledger's verdict as `status` (`:met` or
-`:budget_unmet`) with the failed tests in `unmet`, and the `CBEBugSweepInfo` fields.
-"""
-function cbe_bug_step_device!(st::DeviceState, mpo::DeviceMPO, tau::Number;
-                              order::Int = 1, maxdim::Union{Nothing,Int} = nothing,
-                              hermitian::Bool = true, work_ceiling_rule::Symbol = :bond,
-                              bond_ceiling_factor = nothing, root_ceiling_factor::Real = 1.2,
-                              tol_basis = nothing, tol_trunc = nothing, tol_expv = nothing,
-                              tol_root = nothing, tol_krylov = nothing,
-                              root::Union{Nothing,Int} = nothing, reorth::Bool = true,
-                              truncate::Bool = true, precision::Symbol = :fp64,
-                              parallel::Bool = false, rng::Union{Nothing,Integer} = nothing,
-                              fixed_width::Union{Nothing,Int} = nothing,
-                              tail_factor::Union{Nothing,Real} = nothing,
-                              krylov_mode::Symbol = :auto)
+"The options POD from `cbe_bug_step!`'s keywords (`root` 1-based, as in the reference)."
+function step_options(; order::Int = 1, maxdim::Union{Nothing,Int} = nothing,
+                      hermitian::Bool = true, work_ceiling_rule::Symbol = :bond,
+                      bond_ceiling_factor = nothing, root_ceiling_factor::Real = 1.2,
+                      tol_basis = nothing, tol_trunc = nothing, tol_expv = nothing,
+                      tol_root = nothing, tol_krylov = nothing,
+                      root::Union{Nothing,Int} = nothing, reorth::Bool = true,
+                      truncate::Bool = true, precision::Symbol = :fp64,
+                      parallel::Bool = false, rng::Union{Nothing,Integer} = nothing,
+                      fixed_width::Union{Nothing,Int} = nothing,
+                      tail_factor::Union{Nothing,Real} = nothing,
+                      krylov_mode::Symbol = :auto)
     tol_expv !== nothing && tol_root !== nothing &&
         throw(ArgumentError("tol_root and tol_expv name the same tolerance; pass one"))
     haskey(BUG_CEIL_RULES, work_ceiling_rule) ||
@@ -429,7 +420,20 @@ function cbe_bug_step_device!(st::DeviceState, mpo::DeviceMPO, tau::Number;
     fixed_width === nothing || (o.fixed_width = fixed_width)
     tail_factor === nothing || (o.tail_factor = tail_factor)
     o.krylov_mode = BUG_KRYLOV_MODES[krylov_mode]
+    return o
+end
 
+"""
+    cbe_bug_step_device!(st, mpo, tau; kwargs...) -> NamedTuple
+
+`cbe_bug_step!(psi, mpo, tau; kwargs...)` (bug_step.jl:17-148) on a `DeviceState`: the same
+keywords (the `DDQRExpansion(:rank_tail)` expansion; the adaptive controller, `exact`,
+`keep_states`, `fold_omega = false` and `krylov_rank_tol > 0` are refused by the library with
+`ArgumentError` before the state is touched), the ledger's verdict as `status` (`:met` or
+`:budget_unmet`) with the failed tests in `unmet`, and the `CBEBugSweepInfo` fields.
+"""
+function cbe_bug_step_device!(st::DeviceState, mpo::DeviceMPO, tau::Number; kwargs...)
+    o = step_options(; kwargs...)
     expanded = zeros(Int32, max(st.L - 1, 1))
     logs = Vector{BugBondLog}(undef, 4 * st.L)
     info = BugStepInfo()
@@ -456,6 +460,38 @@ function cbe_bug_step_device!(st::DeviceState, mpo::DeviceMPO, tau::Number;
             seconds = info.seconds)
 end
 
+"""
+    cbe_bug_step_batch!(states, mpos, taus; workers = 8, kwargs...)
+
+Many independent `cbe_bug_step!` calls with one keyword set, advanced together through
+`bug_step_batch`: each instance's stages before the root solve on `workers` host threads, the
+root exponentials as one lockstep Krylov recurrence over grouped launches (e.g. a batch of Heston
+parameter sets).  `taus` holds one step per state (a number applies to all).  Returns each
+instance's `(status, expv_iters, root_link)`; full reports come from `cbe_bug_step_device!`.
+"""
+function cbe_bug_step_batch!(states::AbstractVector{DeviceState}, mpos::AbstractVector{DeviceMPO},
+                             taus; workers::Int = 8, kwargs...)
+    k = length(states)
+    length(mpos) == k || throw(ArgumentError("one MPO per state"))
+    ts = taus isa Number ? fill(ComplexF64(taus), k) : ComplexF64.(collect(taus))
+    length(ts) == k || throw(ArgumentError("one tau per state"))
+    k == 0 && return NamedTuple[]
+    o = step_options(; kwargs...)
+    tv = Vector{Float64}(undef, 2k)
+    for i in 1:k
+        tv[2i-1], tv[2i] = real(ts[i]), imag(ts[i])
+    end
+    infos = [BugStepInfo() for _ in 1:k]
+    sh = [s.h for s in states]
+    mh = [m.h for m in mpos]
+    ctx = states[1].ctx
+    bugcheck(ctx, ccall((:bug_step_batch, LIBBUG), Cint,
+        (Ptr{Cvoid}, Int32, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Float64}, Ref{BugStepOpts},
+         Ptr{BugStepInfo}, Int32), ctx.h, k, sh, mh, tv, o, infos, workers))
+    return [(; status = inf.tolerance_met != 0 ? :met : :budget_unmet, expv_iters = inf.expv_iters,
+             root_link = inf.root_link + 1) for inf in infos]
+end
+
 # ── the operator-action seams over caller-owned device buffers ─────────────────────────────
 
 "`apply_one_site` (effective_operators.jl:39-64,118-129): Y = (H_eff - shift) X."
@@ -471,6 +507,15 @@ apply_zero_site!(ctx::BugCtx, L::BugBsm, R::BugBsm, C::BugBsm, Y::BugBsm;
     bugcheck(ctx, ccall((:bug_apply_zero_site, LIBBUG), Cint,
         (Ptr{Cvoid}, Ref{BugBsm}, Ref{BugBsm}, Ref{BugBsm}, Ref{BugBsm}, Cdouble, Int32,
          Ptr{Cvoid}), ctx.h, L, R, C, Y, shift, precision, stream))
+
+"K root generators at once (`bug_apply_zero_site_batch`): Y[i] = (H_0,i - shifts[i]) C[i]."
+apply_zero_site_batch!(ctx::BugCtx, L::Vector{BugBsm}, R::Vector{BugBsm}, C::Vector{BugBsm},
+                       Y::Vector{BugBsm}; shifts = nothing, precision = BUG_PREC_FP64,
+                       stream = C_NULL) =
+    bugcheck(ctx, ccall((:bug_apply_zero_site_batch, LIBBUG), Cint,
+        (Ptr{Cvoid}, Int32, Ptr{BugBsm}, Ptr{BugBsm}, Ptr{BugBsm}, Ptr{BugBsm}, Ptr{Float64},
+         Int32, Ptr{Cvoid}), ctx.h, length(L), L, R, C, Y,
+        shifts === nothing ? Ptr{Float64}(C_NULL) : Vector{Float64}(shifts), precision, stream))
 
 "`_mpo_left_step` (mpo_environments.jl:65-81): the environment on the next link."
 env_push_left!(ctx::BugCtx, L::BugBsm, A::BugBsm, W::BugMpoSite, Lout::BugBsm;
