@@ -67,7 +67,12 @@ def parse():
                         "the timed steps are cut to fit")
     p.add_argument("--fp64-steps", type=int, default=1,
                    help="FP64 comparison steps after one warm-up (0: skip the FP64 twin)")
-    p.add_argument("--workload", choices=["bug_step", "heff_sweep"], default="bug_step")
+    p.add_argument("--workload", choices=["bug_step", "heff_sweep", "heston_batch", "c3_quench"],
+                   default="bug_step",
+                   help="bug_step: the headline; heff_sweep: the contraction core of a "
+                        "half-sweep; heston_batch: --heston Heston instances stepped together "
+                        "for --steps batched steps; c3_quench: the XXZ L=128 Neel quench "
+                        "(maxdim 1024) for --steps steps in each form")
     p.add_argument("--driver", choices=["c", "python"], default="c",
                    help="c: the step-level C entry point bug_step (C++ orchestration); python: "
                         "the package's Python orchestration of the same kernels")
@@ -139,6 +144,13 @@ def main():
     signal.signal(signal.SIGALRM, _alarm)
     signal.alarm(max(int(args.budget + 70 - (time.time() - T_START)), 1))
     import torch
+
+    if args.workload in ("heston_batch", "c3_quench"):
+        torch.cuda.set_device(0)
+        (heston_batch_run if args.workload == "heston_batch" else c3_quench_run)(args)
+        signal.alarm(0)
+        emit()
+        return
 
     from bondupdategalerkin_amd import _lib
     from bondupdategalerkin_amd import environments as ev
@@ -631,6 +643,104 @@ def heston_phase(args):
                 out["instance_steps_per_s"] * out["single_instance_step_s"], 2)
     note(f"heston batch: {out}")
     return out
+
+
+def heston_batch_run(args):
+    """The batched Heston configuration on its own: ``--heston`` instances (2^10 x 2^10 grid,
+    parameter sets drawn as the configuration describes, seed 0), ``--warmup`` untimed and
+    ``--steps`` timed ``bug_step_batch`` calls; value = instance-steps/s."""
+    import torch
+
+    from bondupdategalerkin_amd import cstep as C
+    from bondupdategalerkin_amd import qtt as Q
+
+    hrng = np.random.default_rng(0)
+    g = Q.qtt_grid(10, 10, x=(-1.0, 1.0), y=(0.0, 0.5), periodic_x=False, periodic_y=False)
+    init = Q.qtt_separable(g, lambda x: float(np.exp(0.5 * x)), lambda v: float(np.exp(-v)))
+    prms = [(hrng.uniform(0, 0.05), hrng.uniform(0, 0.05), hrng.uniform(0.5, 4),
+             hrng.uniform(0.02, 0.15), hrng.uniform(0.1, 0.8), hrng.uniform(-0.9, 0))
+            for _ in range(args.heston)]
+    kw = dict(order=2, maxdim=128, hermitian=False, work_ceiling_rule="maxdim",
+              precision="fp64")
+    states = [C.CState(Q.qtt_mps(init)) for _ in prms]
+    mpos = [C.CMPO(Q.heston_mpo(g, Q.HestonParams(*p))) for p in prms]
+    torch.cuda.synchronize()
+    OUT.update({"metric": "Heston QTT instance-steps/s (batched)", "unit": "instance-steps/s",
+                "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "c128", "data": "synthetic",
+                "config": {"workload": (f"{args.heston} Heston QTT instances, 2^10 x 2^10 grid, "
+                                        "order 2, maxdim 128, tau 0.02, hermitian=false, one "
+                                        "bug_step_batch call per step"),
+                           "instances": args.heston, "workers": args.heston_workers}})
+    statuses, secs = [], []
+    for k in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        infos = C.bug_step_batch(states, mpos, 0.02, workers=args.heston_workers, **kw)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        if k >= args.warmup:
+            secs.append(dt)
+            statuses.append(sum(i["status"] == "met" for i in infos))
+        note(f"batched step {k + 1}: {dt:.2f}s root {infos[0]['seconds']['root']:.2f}s "
+             f"met {sum(i['status'] == 'met' for i in infos)}/{len(infos)} max bond "
+             f"{max(i['r_final'] for i in infos)}")
+    OUT["ms_per_step"] = round(1e3 * float(np.mean(secs)), 1)
+    OUT["value"] = round(args.heston / float(np.mean(secs)), 4)
+    OUT["met_per_step"] = statuses
+
+
+def c3_quench_run(args):
+    """The C3 configuration: XXZ (delta = 0.5) L = 128 Neel quench, maxdim 1024, work ceiling
+    rule maxdim, dt 0.02, order 2, parallel half-sweeps, stepped ``--steps`` times in the mixed
+    form and the FP64 drop-in side by side; reports the verdicts, the largest |dS^z| between the
+    two forms along the way, the bond dimensions and the time of each form."""
+    import torch
+
+    from bondupdategalerkin_amd import cstep as C
+    from bondupdategalerkin_amd import mps as M
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    L = 128
+    mpo = C.CMPO(xxz_mpo(L, delta=args.delta))
+    kw = dict(order=2, maxdim=1024, work_ceiling_rule="maxdim", parallel=True)
+    sm, sf = C.CState(M.neel_state(L)), C.CState(M.neel_state(L))
+    tm = tf = 0.0
+    unmet, dsz_log, bonds = [], [], []
+    done = 0
+    for s in range(args.steps):
+        if left() < 30:
+            OUT.setdefault("cut", []).append(f"c3 quench: {done} of {args.steps} steps in budget")
+            break
+        t0 = time.perf_counter()
+        im = C.bug_step(sm, mpo, -0.02j, precision="mixed", **kw)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        i6 = C.bug_step(sf, mpo, -0.02j, precision="fp64", **kw)
+        torch.cuda.synchronize()
+        tm += t1 - t0
+        tf += time.perf_counter() - t1
+        done += 1
+        if im["status"] != "met" or i6["status"] != "met":
+            unmet.append({"step": s + 1, "mixed": im["unmet_names"], "fp64": i6["unmet_names"]})
+        dsz = float(np.abs(sm.expect_diagonal() - sf.expect_diagonal()).max())
+        dsz_log.append(dsz)
+        bonds.append((im["r_final"], i6["r_final"]))
+        note(f"c3 step {s + 1}: mixed {t1 - t0:.2f}s {im['status']} fp64 "
+             f"{time.perf_counter() - t1:.2f}s {i6['status']} bonds {bonds[-1]} dSz {dsz:.2e}")
+    OUT.update({"metric": "BUG steps/s, C3 Neel quench (XXZ L=128, maxdim 1024)",
+                "unit": "steps/s", "n_gpus": 1, "steps": done, "warmup": 0,
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "c64 contractions / c128 Krylov, QR, SVD", "data": "Neel product state",
+                "value": round(done / tm, 4) if tm > 0 else None,
+                "fp64_steps_per_s": round(done / tf, 4) if tf > 0 else None,
+                "config": {"workload": "XXZ delta=0.5 L=128 Neel quench, maxdim 1024, "
+                                       "work_ceiling_rule maxdim, dt 0.02, order 2, parallel"},
+                "unmet_steps": unmet, "first_unmet_step": unmet[0]["step"] if unmet else None,
+                "max_abs_dSz_mixed_vs_fp64": max(dsz_log) if dsz_log else None,
+                "dSz_every_10": [round(x, 10) for x in dsz_log[9::10]],
+                "bonds_every_10": bonds[9::10],
+                "seconds": {"mixed": round(tm, 1), "fp64": round(tf, 1)}})
 
 
 def cpu_model() -> str:

@@ -1122,7 +1122,8 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
                  probe_hook: Optional[Callable] = None, precision: str = "fp64",
                  parallel: bool = False, eshift: float = 0.0, closure_checks: bool = False,
                  closure_probe: int = 0, root_report: Optional[dict] = None,
-                 outside_residual: bool = False) -> CBEBugSweepInfo:
+                 outside_residual: bool = False, root_method: str = "exp",
+                 root_shift: bool = True, root_maxiter: int = K.ROOT_MAXITER) -> CBEBugSweepInfo:
     """One order-p step in place (``order > 1`` closes the root frontiers j = 1..p-1 by one
     enriching frontier sweep each, ``closure.py``); ``root`` is the 0-based root bond (the
     reference's ``root`` minus 1),
@@ -1139,7 +1140,10 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     energies before and after the root solve and, with ``outside_residual``, the
     outside-Galerkin defect |tau| ||(I - P) H psi_1|| / ||psi_1|| of the evolved vector, the
     frames re-enriched by its own frontier sweep and the root re-solved while it exceeds
-    tol_basis.
+    tol_basis.  ``root_method`` (bug_step.jl:76-87) is the root integrator: "exp" (the Krylov
+    exponential) or the stability function of one implicit Runge-Kutta step, "trapezoid",
+    "radau2" or "radau3", applied in one stage on a basis of at most ``root_maxiter`` vectors;
+    ``root_shift = False`` applies R(tau A_G) unshifted, its decay kept (a linear PDE flow).
 
     ``krylov_rank_tol > 0`` ranks the local Krylov directions instead of appending them in chain
     order (krylov_enrichment.jl:174-196,209-211): every direction above the allowance is collected
@@ -1212,6 +1216,9 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
     cap_keep = work_ceiling_rule != "maxdim"
     if frames_only and (truncate or root_report is not None):
         raise ValueError("frames_only needs root_report = None and truncate = False")
+    K.check_root_method(root_method)
+    if root_maxiter < 1:
+        raise ValueError("root_maxiter must be positive")
     if closure_probe < 0:
         raise ValueError(f"closure_probe must be a non-negative count, got {closure_probe}")
     eshift = float(eshift)
@@ -1427,11 +1434,13 @@ def cbe_bug_step(psi: SymMPS, mpo: MPO, tau: complex, *, tol_basis: Optional[flo
         def apply_root(C):
             napply[0] += 1
             return ev.apply_zero_site(Lr, Rr, C, shift=eshift, precision=cp)
+        rk = dict(root_method=root_method, rational_shift=root_shift)
         if hermitian:
-            return K.lanczos_expv(apply_root, tau, start, tol=1e-15, maxiter=K.ROOT_MAXITER,
-                                  reorth=reorth, conv_tol=te, substeps=0, on_substep=substep)
-        return K.arnoldi_expv(apply_root, tau, start, tol=1e-15, maxiter=K.ROOT_MAXITER,
-                              conv_tol=te, substeps=0, on_substep=substep)
+            return K.lanczos_expv(apply_root, tau, start, tol=1e-15, maxiter=root_maxiter,
+                                  reorth=reorth, conv_tol=te, substeps=0, on_substep=substep,
+                                  **rk)
+        return K.arnoldi_expv(apply_root, tau, start, tol=1e-15, maxiter=root_maxiter,
+                              conv_tol=te, substeps=0, on_substep=substep, **rk)
 
     if frames_only:                      # the caller solves its own root (bug_step.jl:381-382)
         result = root_start

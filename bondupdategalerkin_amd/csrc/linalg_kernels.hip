@@ -1512,11 +1512,27 @@ __global__ __launch_bounds__(kGbjThreads) void gbj_rotate_kernel(const BjMat* __
     __syncthreads();
     const double tol2 = eps * eps * (double)n;
     const bool cross_only = round > 0 && cp > 0 && cq > 0;
+    // Every pair the inner sweep visits, tested on G as summed: if none passes the rotation
+    // test, the first inner round rotates nothing, G stays as it is, and so do all later ones;
+    // the sweep is skipped (the result is the one the sweep would give).
+    for (int e = tid; e < c * c; e += kGbjThreads) {
+        const int a = e / c, b = e % c;
+        if (a >= b) continue;
+        if (cross_only && !(a < cp && b >= cp)) continue;
+        const double gr = G[a][b].x, gi = G[a][b].y;
+        const double ag2 = gr * gr + gi * gi;
+        if (ag2 > tol2 * (G[a][a].x * G[b][b].x) && ag2 != 0.0) s_rot = 1;
+    }
+    __syncthreads();
+    const bool needed = s_rot != 0;
+    __syncthreads();
+    if (tid == 0) s_rot = 0;
+    __syncthreads();
     const int big = cp >= cq ? cp : cq, small = cp >= cq ? cq : cp;
     const int big0 = cp >= cq ? 0 : cp, small0 = cp >= cq ? cp : 0;
     const int C = c + (c & 1);
     const int np = cross_only ? small : C / 2;
-    const int nrounds = cross_only ? big : C - 1;
+    const int nrounds = needed ? (cross_only ? big : C - 1) : 0;
     // Thread t works on pair g = t / kGbjCols, entry k = t % kGbjCols of the pair's two columns
     // (then of its two rows), kGbjThreads / kGbjCols pairs at a time.  Every thread of a pair
     // computes the pair's rotation itself from G as the round found it: the pair's kGbjCols

@@ -28,6 +28,12 @@ SHIFT_TRIGGER = 30.0         # exponent past which the spectral shift switches o
 MAX_NORM = 1e150
 
 
+def _exp(x: float) -> float:
+    """exp that saturates to inf instead of raising (a weight past the double range is infinite,
+    as the IEEE exponential gives it)."""
+    return math.inf if x > 709.0 else math.exp(x)
+
+
 def residual_integral(index: int, M: np.ndarray, tau: complex, relative: bool) -> float:
     """|tau| int_0^1 |(exp(sM) e1)[index]| ds by composite Simpson in log space.
 
@@ -77,8 +83,8 @@ def residual_integral(index: int, M: np.ndarray, tau: complex, relative: bool) -
         return 0.0
     lw = math.log(abs(tau)) + top + math.log(np.exp(logw - top).sum())
     if unitary or not relative:
-        return math.exp(lw)
-    return math.exp(lw - sigma - math.log(max(np.linalg.norm(c), np.finfo(float).tiny)))
+        return _exp(lw)
+    return _exp(lw - sigma - math.log(max(np.linalg.norm(c), np.finfo(float).tiny)))
 
 
 def direction_weight(H: np.ndarray, k: int, tau: complex, state_seed: bool = True) -> float:
@@ -104,10 +110,60 @@ def tail_rank(kept, scale: float, allowance: float, kmax: int, base: float = 0.0
     return r, scale * math.sqrt(tail)
 
 
-class Tridiagonal:
-    """Eigendecomposition of the Lanczos tridiagonal, evaluated for any trial step."""
+# The rational roots (root_methods.jl:5-71,143-157): one implicit Runge-Kutta step's stability
+# function R(z) = N(z) / D(z) applied on the Krylov basis in one stage (coefficients constant first).
+ROOT_METHODS = ("exp", "trapezoid", "radau2", "radau3")
+ROOT_PADE = {
+    "trapezoid": ((1.0, 1 / 2), (1.0, -1 / 2)),
+    "radau2": ((1.0, 1 / 3), (1.0, -2 / 3, 1 / 6)),
+    "radau3": ((1.0, 2 / 5, 1 / 20), (1.0, -3 / 5, 3 / 20, -1 / 60)),
+}
 
-    def __init__(self, alpha: List[float], beta: List[float]):
+
+def check_root_method(method: str):
+    if method not in ROOT_METHODS:
+        raise ValueError(f"root_method must be one of {ROOT_METHODS}, got {method!r}")
+
+
+def root_stability(method: str, z: complex) -> complex:
+    """R(z) of ``method`` (exp(z) for "exp")."""
+    check_root_method(method)
+    z = complex(z)
+    if method == "exp":
+        return complex(np.exp(z))
+    num, den = ROOT_PADE[method]
+    return np.polyval(num[::-1], z) / np.polyval(den[::-1], z)
+
+
+def rational_apply(method: str, M: np.ndarray, rhs: np.ndarray) -> np.ndarray:
+    """D(M)^{-1} N(M) rhs for a small dense M (polynomials by Horner)."""
+    num, den = ROOT_PADE[method]
+    eye = np.eye(M.shape[0], dtype=complex)
+
+    def poly(c):
+        out = c[-1] * eye
+        for k in range(len(c) - 2, -1, -1):
+            out = out @ M + c[k] * eye
+        return out
+
+    return np.linalg.solve(poly(den), poly(num) @ rhs)
+
+
+def rational_shift(vals, tau: complex, shift: bool) -> complex:
+    """The Ritz value maximising Re(tau lambda); none without ``shift`` or in real time."""
+    if not shift or len(vals) == 0 or complex(tau).real == 0:
+        return 0j
+    v = np.asarray(vals, dtype=complex)
+    return complex(v[int(np.argmax(np.real(tau * v)))])
+
+
+class Tridiagonal:
+    """Eigendecomposition of the Lanczos tridiagonal, evaluated for any trial step: the exponential
+    (``method`` "exp") or a rational root's R(tau (T - sigma)) e1 (no scale returned)."""
+
+    def __init__(self, alpha: List[float], beta: List[float], method: str = "exp",
+                 shift: bool = True):
+        self.method, self.shift = method, shift
         a, b = np.asarray(alpha, float), np.asarray(beta, float)
         if not (np.isfinite(a).all() and np.isfinite(b).all()):
             raise ValueError("Lanczos tridiagonal has non-finite entries: the input vector or "
@@ -124,6 +180,11 @@ class Tridiagonal:
                 self.vals, self.vecs = np.linalg.eigh(Tm)
 
     def coeffs(self, tau: complex):
+        if self.method != "exp":
+            sigma = rational_shift(self.vals, tau, self.shift)
+            w = np.array([root_stability(self.method, tau * (v - sigma)) for v in self.vals]) \
+                * self.vecs[0, :]
+            return self.vecs.astype(complex) @ w, 0j
         ex = np.real(tau * self.vals)
         i = int(np.argmax(ex))
         sigma = float(self.vals[i]) if ex[i] > SHIFT_TRIGGER else 0.0
@@ -136,7 +197,8 @@ class Hessenberg:
     root_methods.jl:136-141): exp(tau (Hm - sigma I)) e1 with the spectral shift sigma taken from
     the real part of the eigenvalue that grows fastest under tau."""
 
-    def __init__(self, Hm: np.ndarray):
+    def __init__(self, Hm: np.ndarray, method: str = "exp", shift: bool = True):
+        self.method, self.shift = method, shift
         if not np.isfinite(Hm).all():
             raise ValueError("Arnoldi Hessenberg matrix has non-finite entries: the input vector "
                              "or the operator is not finite")
@@ -146,6 +208,12 @@ class Hessenberg:
     def coeffs(self, tau: complex):
         if self.Hm.size == 0:
             return np.zeros(0, dtype=complex), 0j
+        if self.method != "exp":
+            sigma = rational_shift(self.vals, tau, self.shift)
+            n = self.Hm.shape[0]
+            e1 = np.zeros(n, dtype=complex)
+            e1[0] = 1.0
+            return rational_apply(self.method, tau * (self.Hm - sigma * np.eye(n)), e1), 0j
         ex = np.real(tau * self.vals)
         i = int(np.argmax(ex))
         sigma = float(self.vals[i].real) if ex[i] > SHIFT_TRIGGER else 0.0
@@ -189,8 +257,14 @@ class ExpvReport:
 
 def lanczos_expv(apply, tau: complex, v0, *, tol: float = 1e-15, maxiter: int = ROOT_MAXITER,
                  reorth: bool = True, conv_tol: float = 0.0, substeps: int = 0,
-                 on_substep: Optional[Callable[[int], None]] = None):
-    """exp(tau A) v0 for a Hermitian device action ``apply`` (``lanczos_expv``)."""
+                 on_substep: Optional[Callable[[int], None]] = None, root_method: str = "exp",
+                 rational_shift: bool = True):
+    """exp(tau A) v0 for a Hermitian device action ``apply`` (``lanczos_expv``); a rational
+    ``root_method`` applies that implicit method's stability function in one stage."""
+    check_root_method(root_method)
+    if root_method != "exp":
+        substeps = 1
+    rk = (root_method, rational_shift)
     beta0 = T.norm(v0)
     if beta0 == 0:
         return v0, ExpvReport(0, 0, 0.0)
@@ -216,7 +290,8 @@ def lanczos_expv(apply, tau: complex, v0, *, tol: float = 1e-15, maxiter: int = 
             b = T.norm(r)
             if b < tol:
                 break
-            if conv_tol > 0 and tail_sup(Tridiagonal(alpha, beta), tau_rem, b, conv_tol) < conv_tol:
+            if conv_tol > 0 and tail_sup(Tridiagonal(alpha, beta, *rk), tau_rem, b,
+                                         conv_tol) < conv_tol:
                 break
             beta.append(b)
             basis.append(T.scale(1.0 / b, r))
@@ -230,7 +305,7 @@ def lanczos_expv(apply, tau: complex, v0, *, tol: float = 1e-15, maxiter: int = 
                 for _p in range(2):
                     cs = T.inner_many(basis, r)
                     T.axpy_many_([-c for c in cs], basis, r)
-        tri = Tridiagonal(alpha, beta)
+        tri = Tridiagonal(alpha, beta, *rk)
         betak = T.norm(r)
         if substeps == 1:
             tau_s = tau_rem
@@ -260,11 +335,17 @@ def lanczos_expv(apply, tau: complex, v0, *, tol: float = 1e-15, maxiter: int = 
 
 def arnoldi_expv(apply, tau: complex, v0, *, tol: float = 1e-13, maxiter: int = ROOT_MAXITER,
                  conv_tol: float = 0.0, substeps: int = 0,
-                 on_substep: Optional[Callable[[int], None]] = None):
+                 on_substep: Optional[Callable[[int], None]] = None, root_method: str = "exp",
+                 rational_shift: bool = True):
     """exp(tau A) v0 for a non-Hermitian device action ``apply`` (``arnoldi_expv``,
     src/BUG/Core/LocalSolvers/krylov.jl:343-442): upper-Hessenberg Arnoldi with classical
     Gram-Schmidt applied twice per vector (each pass one batched inner-product launch against the
-    whole basis), the Lanczos path's tail estimate, substep search and scale bookkeeping."""
+    whole basis), the Lanczos path's tail estimate, substep search and scale bookkeeping; a
+    rational ``root_method`` takes the whole step in one stage (krylov.jl:365-370)."""
+    check_root_method(root_method)
+    if root_method != "exp":
+        substeps = 1
+    rk = (root_method, rational_shift)
     beta0 = T.norm(v0)
     if beta0 == 0:
         return v0, ExpvReport(0, 0, 0.0)
@@ -293,14 +374,14 @@ def arnoldi_expv(apply, tau: complex, v0, *, tol: float = 1e-13, maxiter: int = 
             betak = T.norm(r)
             if betak < tol:
                 break
-            if conv_tol > 0 and tail_sup(Hessenberg(H[:dim, :dim].copy()), tau_rem, betak,
+            if conv_tol > 0 and tail_sup(Hessenberg(H[:dim, :dim].copy(), *rk), tau_rem, betak,
                                          conv_tol) < conv_tol:
                 break
             H[k, k - 1] = betak
             if k == maxiter:
                 break
             basis.append(T.scale(1.0 / betak, r))
-        hess = Hessenberg(H[:dim, :dim].copy())
+        hess = Hessenberg(H[:dim, :dim].copy(), *rk)
         if substeps == 1:
             tau_s = tau_rem
         elif substeps > 1:

@@ -180,6 +180,9 @@ class StepOptions:
     closure_probe: int = 0              # order 1: measure frontiers 0..m-1 (bug_step.jl:33-38)
     root_report: bool = False           # the root's account (bug_step_stages.jl:242-398)
     outside_residual: bool = False      # with root_report: measure and re-enrich (I - P) H psi_1
+    root_method: str = "exp"            # exp | trapezoid | radau2 | radau3 (bug_step.jl:76-81)
+    root_shift: bool = True             # rational roots: shift to the extreme Ritz value (:82-86)
+    root_maxiter: int = 30              # root Krylov basis cap (_ROOT_MAXITER; bug_step.jl:87)
 
 
 @dataclass
@@ -1053,7 +1056,8 @@ def _cap_at(wcap, b):
 
 
 def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None,
-                 probe_hook=None, rng: Optional[la.ProbeRNG] = None) -> StepInfo:
+                 probe_hook=None, rng: Optional[la.ProbeRNG] = None,
+                 frames_report: Optional[dict] = None) -> StepInfo:
     """One order-p CBE-BUG step in place (bug_step.jl:17-510); order > 1 closes the root
     frontiers by one enriching sweep each (oracle/closure.py), or by the measured loop under
     ``closure_checks``.  ``rng`` is a probe stream the caller keeps across steps (the reference's
@@ -1078,6 +1082,9 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
         raise ValueError("frames_only needs root_report off and truncate = False")
     if opts.closure_probe < 0:
         raise ValueError("closure_probe must be a non-negative count")
+    ok.check_root_method(opts.root_method)
+    if opts.root_maxiter < 1:
+        raise ValueError("root_maxiter must be positive")
     d = len(psi.sites[0].shifts)
     tau = complex(tau)
     es = float(opts.eshift)
@@ -1234,12 +1241,13 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
         def apply_root(C):
             napply[0] += 1
             return ob.apply_zero_site(Lr, Rr, C, shift=es)
+        rk = dict(root_method=opts.root_method, rational_shift=opts.root_shift)
         if opts.hermitian:
-            return ok.lanczos_expv(apply_root, tau, start, tol=1e-15, maxiter=ok.ROOT_MAXITER,
+            return ok.lanczos_expv(apply_root, tau, start, tol=1e-15, maxiter=opts.root_maxiter,
                                    reorth=opts.reorth, conv_tol=opts.tol_expv, substeps=0,
-                                   **vec_ops)
-        return ok.arnoldi_expv(apply_root, tau, start, tol=1e-15, maxiter=ok.ROOT_MAXITER,
-                               conv_tol=opts.tol_expv, substeps=0, **vec_ops)
+                                   **rk, **vec_ops)
+        return ok.arnoldi_expv(apply_root, tau, start, tol=1e-15, maxiter=opts.root_maxiter,
+                               conv_tol=opts.tol_expv, substeps=0, **rk, **vec_ops)
 
     if opts.frames_only:
         result = root_start
@@ -1251,6 +1259,9 @@ def cbe_bug_step(psi: MPS, mpo, tau: complex, opts: Optional[StepOptions] = None
         root_start, result, rep, report = report_root(
             psi, mpo, fr, Lr, Rr, root_start, result, rep, solve_with, tau, alloc, opts, wcap,
             nrm, rng, recurrence, ledger, expanded)
+    if frames_report is not None:       # bug_step.jl:455-467
+        frames_report.update(left=list(left_frames), right=list(right_frames), root_link=rl,
+                             lenv=lroot, renv=rroot, eshift=es)
     if not opts.frames_only:
         ledger.expv_err = rep.err
         ledger.expv_iters = rep.iterations

@@ -6,6 +6,8 @@
 * ``tail_sup``, ``pick_tau``             -- src/BUG/Core/LocalSolvers/root_methods.jl:86-130
 * ``lanczos_expv``                       -- krylov.jl:128-263 (``:exp`` root, ``reorth``, adaptive
                                             substeps), on any vector type through callbacks
+* rational roots ``root_stability``, ``rational_apply``, ``rational_coeffs``, ``hess_rational``
+                                         -- root_methods.jl:5-71,143-157 (trapezoid, Radau IIA)
 
 Pinned by tests/test_oracle_krylov.py: expv against ``scipy.linalg.expm`` on random Hermitian
 matrices, weights against a dense exponential, the 2x2 cancellation case of
@@ -27,6 +29,12 @@ RESIDUAL_MAX_PANELS = 4096     # :36
 TAIL_NODES = 8                 # _TAIL_NODES, root_methods.jl:83
 EXPV_SHIFT_TRIGGER = 30.0      # krylov.jl:26
 EXPV_MAX_NORM = 1e150
+
+
+def _exp(x: float) -> float:
+    """exp that saturates to inf instead of raising (a weight past the double range is infinite,
+    as the IEEE exponential gives it)."""
+    return math.inf if x > 709.0 else math.exp(x)
 
 
 def residual_integral(defect_norm: Callable, M: np.ndarray, tau: complex,
@@ -53,8 +61,8 @@ def residual_integral(defect_norm: Callable, M: np.ndarray, tau: complex,
         return 0.0
     lw = math.log(abs(tau)) + peak + math.log(np.exp(logs - peak).sum())
     if unitary or not state_seed:
-        return math.exp(lw)
-    return math.exp(lw - sigma - math.log(max(np.linalg.norm(col), np.finfo(float).tiny)))
+        return _exp(lw)
+    return _exp(lw - sigma - math.log(max(np.linalg.norm(col), np.finfo(float).tiny)))
 
 
 def kry_weight(Hm: np.ndarray, k: int, tau: complex, state_seed: bool = True) -> float:
@@ -140,6 +148,74 @@ def pick_tau(evalcoeff, tau_rem: complex, betak: float, conv_tol: float) -> comp
     return max(lo, 1e-6) * tau_rem
 
 
+# ---- the rational roots (root_methods.jl:5-71,143-157) ----------------------------------------------
+# One implicit Runge-Kutta step's stability function R(z) = N(z) / D(z), applied on the Krylov
+# basis in one stage; coefficients constant term first.
+ROOT_METHODS = ("exp", "trapezoid", "radau2", "radau3")
+ROOT_PADE = {
+    "trapezoid": ([1.0, 1 / 2], [1.0, -1 / 2]),
+    "radau2": ([1.0, 1 / 3], [1.0, -2 / 3, 1 / 6]),
+    "radau3": ([1.0, 2 / 5, 1 / 20], [1.0, -3 / 5, 3 / 20, -1 / 60]),
+}
+
+
+def check_root_method(method: str):
+    if method not in ROOT_METHODS:
+        raise ValueError(f"root_method must be one of {ROOT_METHODS}, got {method!r}")
+
+
+def root_stability(method: str, z: complex) -> complex:
+    """R(z) of ``method`` (exp(z) for "exp")."""
+    check_root_method(method)
+    if method == "exp":
+        return complex(np.exp(complex(z)))
+    num, den = ROOT_PADE[method]
+    z = complex(z)
+    return sum(c * z ** k for k, c in enumerate(num)) / sum(c * z ** k for k, c in enumerate(den))
+
+
+def rational_apply(method: str, M: np.ndarray, rhs: np.ndarray) -> np.ndarray:
+    """D(M)^{-1} N(M) rhs for a small dense M (the polynomials by Horner)."""
+    num, den = ROOT_PADE[method]
+    n = M.shape[0]
+
+    def poly(c):
+        out = c[-1] * np.eye(n, dtype=complex)
+        for k in range(len(c) - 2, -1, -1):
+            out = out @ M + c[k] * np.eye(n)
+        return out
+
+    return np.linalg.solve(poly(den), poly(num) @ rhs)
+
+
+def rational_shift(vals, tau: complex, shift: bool = True) -> complex:
+    """The Ritz value maximising Re(tau lambda); none without ``shift``, for no values, or when tau
+    is purely imaginary (real time)."""
+    if not shift or len(vals) == 0 or complex(tau).real == 0:
+        return 0j
+    v = np.asarray(vals, dtype=complex)
+    return complex(v[int(np.argmax(np.real(tau * v)))])
+
+
+def rational_coeffs(vals, vecs, tau: complex, method: str, shift: bool = True):
+    """R(tau (T - sigma)) e_1 from the tridiagonal's eigendecomposition; no scale returned."""
+    sigma = rational_shift(vals, tau, shift)
+    w = np.array([root_stability(method, tau * (complex(v) - sigma)) for v in vals]) \
+        * vecs[0, :].astype(complex)
+    return vecs.astype(complex) @ w, 0j
+
+
+def hess_rational(Hm: np.ndarray, tau: complex, method: str, shift: bool = True):
+    """R(tau (Hm - sigma)) e_1 on the Arnoldi Hessenberg; no scale returned."""
+    if Hm.size == 0:
+        return np.zeros(0, dtype=complex), 0j
+    sigma = rational_shift(np.linalg.eigvals(Hm), tau, shift)
+    n = Hm.shape[0]
+    e1 = np.zeros(n, dtype=complex)
+    e1[0] = 1.0
+    return rational_apply(method, tau * (Hm - sigma * np.eye(n)), e1), 0j
+
+
 class ExpvReport:
     __slots__ = ("iterations", "substeps", "err")
 
@@ -149,12 +225,22 @@ class ExpvReport:
 
 def lanczos_expv(apply, tau: complex, vector, *, inner, norm, axpy, scale,
                  tol: float = 1e-13, maxiter: int = 60, reorth: bool = False,
-                 conv_tol: float = 0.0, substeps: int = 1):
+                 conv_tol: float = 0.0, substeps: int = 1, root_method: str = "exp",
+                 rational_shift: bool = True):
     """exp(tau A) vector for Hermitian ``apply``; vector algebra through the callbacks.
 
     ``inner(a, b)`` = <a|b>, ``norm(a)``, ``axpy(alpha, x, y)`` = y + alpha x (new object),
-    ``scale(alpha, x)`` = alpha x.  Returns (result, ExpvReport).
+    ``scale(alpha, x)`` = alpha x.  ``root_method`` other than "exp" applies that implicit
+    method's stability function in one stage (krylov.jl:153-160).  Returns (result, ExpvReport).
     """
+    check_root_method(root_method)
+    if root_method != "exp":
+        substeps = 1
+
+    def tri_coeffs(vals, vecs, t):
+        if root_method == "exp":
+            return coeffs_from_eigen(vals, vecs, t)
+        return rational_coeffs(vals, vecs, t, root_method, rational_shift)
     beta0 = norm(vector)
     if beta0 == 0:
         return vector, ExpvReport(0, 0, 0.0)
@@ -183,7 +269,7 @@ def lanczos_expv(apply, tau: complex, vector, *, inner, norm, axpy, scale,
                 break
             if conv_tol > 0:
                 vals, vecs = tridiag_eigen(alpha, betas)
-                tt = tail_sup(lambda t: coeffs_from_eigen(vals, vecs, t), tau_rem, b, above=conv_tol)
+                tt = tail_sup(lambda t: tri_coeffs(vals, vecs, t), tau_rem, b, above=conv_tol)
                 if tt < conv_tol:
                     break
             betas.append(b)
@@ -202,7 +288,7 @@ def lanczos_expv(apply, tau: complex, vector, *, inner, norm, axpy, scale,
         vals, vecs = tridiag_eigen(alpha, betas)
 
         def evalcoeff(t, vals=vals, vecs=vecs):
-            return coeffs_from_eigen(vals, vecs, complex(t))
+            return tri_coeffs(vals, vecs, complex(t))
 
         betak = norm(res)
         if substeps == 1:
@@ -242,11 +328,21 @@ def hess_coeffs(Hm: np.ndarray, tau: complex):
 
 
 def arnoldi_expv(apply, tau: complex, vector, *, inner, norm, axpy, scale,
-                 tol: float = 1e-13, maxiter: int = 60, conv_tol: float = 0.0, substeps: int = 1):
+                 tol: float = 1e-13, maxiter: int = 60, conv_tol: float = 0.0, substeps: int = 1,
+                 root_method: str = "exp", rational_shift: bool = True):
     """exp(tau A) vector for a non-Hermitian ``apply`` (arnoldi_expv, krylov.jl:343-442): the
     upper-Hessenberg Arnoldi factorisation with every vector orthogonalised twice (CGS2,
     krylov.jl:321-332), the same tail estimate, substep search and scale bookkeeping as
-    ``lanczos_expv``.  Returns (result, ExpvReport)."""
+    ``lanczos_expv``; a rational ``root_method`` takes the whole step in one stage
+    (krylov.jl:365-370).  Returns (result, ExpvReport)."""
+    check_root_method(root_method)
+    if root_method != "exp":
+        substeps = 1
+
+    def hcoeffs(Hm, t):
+        if root_method == "exp":
+            return hess_coeffs(Hm, t)
+        return hess_rational(Hm, t, root_method, rational_shift)
     beta0 = norm(vector)
     if beta0 == 0:
         return vector, ExpvReport(0, 0, 0.0)
@@ -280,7 +376,7 @@ def arnoldi_expv(apply, tau: complex, vector, *, inner, norm, axpy, scale,
                 break
             if conv_tol > 0:
                 Hk = H[:dim, :dim].copy()
-                tt = tail_sup(lambda t: hess_coeffs(Hk, t), tau_rem, rn, above=conv_tol)
+                tt = tail_sup(lambda t: hcoeffs(Hk, t), tau_rem, rn, above=conv_tol)
                 if tt < conv_tol:
                     break
             H[k, k - 1] = rn
@@ -290,7 +386,7 @@ def arnoldi_expv(apply, tau: complex, vector, *, inner, norm, axpy, scale,
         Hm = H[:dim, :dim].copy()
 
         def evalcoeff(t, Hm=Hm):
-            return hess_coeffs(Hm, complex(t))
+            return hcoeffs(Hm, complex(t))
 
         if substeps == 1:
             tau_s = tau_rem
