@@ -89,6 +89,8 @@ def parse():
                    help="Heston instances (2^10 x 2^10 grid) stepped together through "
                         "bug_step_batch (0: skip)")
     p.add_argument("--heston-steps", type=int, default=2, help="batched Heston steps")
+    p.add_argument("--heston-root", default="exp", choices=["exp", "trapezoid", "radau2", "radau3"],
+                   help="root integrator of --workload heston_batch (a rational one runs unshifted)")
     p.add_argument("--heston-workers", type=int, default=16,
                    help="worker threads of the batched step's per-instance stages")
     p.add_argument("--autotune", choices=["table", "tune", "off"], default="table",
@@ -601,6 +603,8 @@ def heston_phase(args):
             for _ in range(args.heston)]
     kw = dict(order=2, maxdim=128, hermitian=False, work_ceiling_rule="maxdim",
               precision="fp64")
+    if args.heston_root != "exp":
+        kw.update(root_method=args.heston_root, root_shift=False, root_maxiter=200)
     states = [C.CState(Q.qtt_mps(init)) for _ in prms]
     mpos = [C.CMPO(Q.heston_mpo(g, Q.HestonParams(*p))) for p in prms]
     torch.cuda.synchronize()
@@ -610,16 +614,21 @@ def heston_phase(args):
                         "hermitian=false, FP64, one bug_step_batch call per step "
                         f"({args.heston_workers} worker threads)"),
            "steps": 0, "seconds": [], "budget_status": [], "phase_seconds": []}
+    active, dropped = list(range(len(states))), []
+    out["dropped"] = dropped
     for _ in range(args.heston_steps):
         if out["seconds"] and left() < 1.5 * out["seconds"][-1] + 30:
             OUT.setdefault("cut", []).append(f"heston batch: {out['steps']} of "
                                              f"{args.heston_steps} batched steps in budget")
             break
         t0 = time.perf_counter()
-        infos = C.bug_step_batch(states, mpos, 0.02, workers=args.heston_workers, **kw)
+        infos = batched_step(C, states, mpos, active, dropped, kw, args.heston_workers)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        if not infos:
+            break
         out["steps"] += 1
+        out.setdefault("instances_per_step", []).append(len(infos))
         out["seconds"].append(round(dt, 2))
         out["budget_status"].append(sorted({i["status"] for i in infos}))
         out["phase_seconds"].append({"root_lockstep": round(infos[0]["seconds"]["root"], 2),
@@ -628,7 +637,8 @@ def heston_phase(args):
              f"{infos[0]['seconds']['root']:.2f}s, expv iters max "
              f"{max(i['expv_iters'] for i in infos)}")
     if out["seconds"]:
-        out["instance_steps_per_s"] = round(args.heston / float(np.mean(out["seconds"])), 4)
+        out["instance_steps_per_s"] = round(float(np.sum(out["instances_per_step"]))
+                                            / float(np.sum(out["seconds"])), 4)
         out["max_bond_after"] = max(max(s.bond_dims()) for s in states)
     # one instance alone through bug_step, for the sequential rate
     if left() > 40:
@@ -643,6 +653,30 @@ def heston_phase(args):
                 out["instance_steps_per_s"] * out["single_instance_step_s"], 2)
     note(f"heston batch: {out}")
     return out
+
+
+def batched_step(C, states, mpos, active, dropped, kw, workers):
+    """One ``bug_step_batch`` over the active instances.  An instance whose step fails (a stiff
+    parameter set whose state underflows to zero and then turns non-finite, as the oracle's own
+    step does on it) is dropped from the batch and the batched step retried: a failed call changes
+    no state.  Returns the reports of the instances stepped."""
+    import re
+
+    from bondupdategalerkin_amd import _lib
+
+    while active:
+        try:
+            return C.bug_step_batch([states[i] for i in active], [mpos[i] for i in active], 0.02,
+                                    workers=workers, **kw)
+        except _lib.BugError as e:
+            m = re.search(r"instance (\d+):", str(e))
+            if m is None:
+                raise
+            bad = active[int(m.group(1))]
+            dropped.append({"instance": bad, "error": re.sub(r"^.*instance \d+: ", "", str(e))})
+            active.remove(bad)
+            note(f"dropped instance {bad}: {e}")
+    return []
 
 
 def heston_batch_run(args):
@@ -672,22 +706,38 @@ def heston_batch_run(args):
                 "config": {"workload": (f"{args.heston} Heston QTT instances, 2^10 x 2^10 grid, "
                                         "order 2, maxdim 128, tau 0.02, hermitian=false, one "
                                         "bug_step_batch call per step"),
-                           "instances": args.heston, "workers": args.heston_workers}})
-    statuses, secs = [], []
+                           "instances": args.heston, "workers": args.heston_workers,
+                           "root": args.heston_root}})
+    statuses, secs, counts, unmet = [], [], [], {}
+    active, dropped = list(range(len(states))), []
     for k in range(args.warmup + args.steps):
+        if left() < 30:
+            OUT.setdefault("cut", []).append(f"heston batch: {len(secs)} of {args.steps} timed steps")
+            break
         t0 = time.perf_counter()
-        infos = C.bug_step_batch(states, mpos, 0.02, workers=args.heston_workers, **kw)
+        infos = batched_step(C, states, mpos, active, dropped, kw, args.heston_workers)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        if not infos:
+            break
         if k >= args.warmup:
             secs.append(dt)
+            counts.append(len(infos))
             statuses.append(sum(i["status"] == "met" for i in infos))
-        note(f"batched step {k + 1}: {dt:.2f}s root {infos[0]['seconds']['root']:.2f}s "
-             f"met {sum(i['status'] == 'met' for i in infos)}/{len(infos)} max bond "
-             f"{max(i['r_final'] for i in infos)}")
-    OUT["ms_per_step"] = round(1e3 * float(np.mean(secs)), 1)
-    OUT["value"] = round(args.heston / float(np.mean(secs)), 4)
+            unmet = {}
+            for i in infos:
+                for name in i["unmet_names"]:
+                    unmet[name] = unmet.get(name, 0) + 1
+        note(f"batched step {k + 1}: {dt:.2f}s ({len(infos)} instances) root "
+             f"{infos[0]['seconds']['root']:.2f}s met {statuses[-1] if statuses else 0}/"
+             f"{len(infos)} max bond {max(i['r_final'] for i in infos)}")
+    OUT["steps"] = len(secs)
+    OUT["ms_per_step"] = round(1e3 * float(np.mean(secs)), 1) if secs else None
+    OUT["value"] = round(float(np.sum(counts)) / float(np.sum(secs)), 4) if secs else None
     OUT["met_per_step"] = statuses
+    OUT["instances_per_step"] = counts
+    OUT["unmet_last_step"] = unmet
+    OUT["dropped"] = dropped
 
 
 def c3_quench_run(args):

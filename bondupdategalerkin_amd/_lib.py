@@ -108,7 +108,8 @@ class BugStepOpts(ctypes.Structure):
                 ("fixed_width", ctypes.c_int32), ("tail_factor", ctypes.c_double),
                 ("krylov_mode", ctypes.c_int32), ("mode", ctypes.c_int32),
                 ("exact", ctypes.c_int32), ("fold_omega", ctypes.c_int32),
-                ("krylov_rank_tol", ctypes.c_double)]
+                ("krylov_rank_tol", ctypes.c_double), ("root_method", ctypes.c_int32),
+                ("root_shift", ctypes.c_int32), ("root_maxiter", ctypes.c_int32)]
 
 
 class BugBondLog(ctypes.Structure):

@@ -1413,17 +1413,24 @@ __global__ __launch_bounds__(kBjThreads) void bj_round_kernel(const BjMat* __res
 // The rotation test and angles are those of the staged kernel (|g_ab|^2 > eps^2 n g_aa g_bb);
 // the Gram matrix is recomputed from the columns every round, so its updates within the round
 // never accumulate past one round, and convergence is judged on fresh inner products.
-constexpr int kGbjCols = 32;   // columns of a block pair (2w)
+constexpr int kGbjCols = 32;   // widest block pair (2w)
 constexpr int kGbjRows = 64;   // rows per chunk
-constexpr int kGbjThreads = 256;
+// LDS of one instance: the gram kernel stages CC columns of a chunk, the rotate kernel holds G and
+// Z (CC x (CC + 1) each; its row staging for the update reuses G's storage)
+constexpr size_t gbj_gram_lds(int cc) { return sizeof(double2) * (size_t)cc * (kGbjRows + 1); }
+constexpr size_t gbj_rot_lds(int cc) { return 2 * sizeof(double2) * (size_t)cc * (cc + 1); }
 
-__global__ __launch_bounds__(kGbjThreads) void gbj_gram_kernel(const BjMat* __restrict__ mats,
-                                                               const int2* __restrict__ tasks,
-                                                               const double2* __restrict__ X,
-                                                               double2* __restrict__ gbuf, int round,
-                                                               const int* __restrict__ flags_prev,
-                                                               int maxch) {
-    __shared__ double2 xs[kGbjCols][kGbjRows + 1];
+// CC columns per block pair on NT = (CC / 2)^2 threads
+template <int CC, int NT>
+__global__ __launch_bounds__(NT) void gbj_gram_kernel(const BjMat* __restrict__ mats,
+                                                      const int2* __restrict__ tasks,
+                                                      const double2* __restrict__ X,
+                                                      double2* __restrict__ gbuf, int round,
+                                                      const int* __restrict__ flags_prev,
+                                                      int maxch) {
+    static_assert((CC / 2) * (CC / 2) == NT, "one 2 x 2 block of G per thread");
+    extern __shared__ __align__(16) unsigned char gbj_gram_lds_raw[];
+    double2 (*xs)[kGbjRows + 1] = reinterpret_cast<double2 (*)[kGbjRows + 1]>(gbj_gram_lds_raw);
     const int2 tk = tasks[blockIdx.x];
     const BjMat mt = mats[tk.x];
     if (round >= mt.N - 1 || flags_prev[tk.x] == 0) return;
@@ -1439,7 +1446,7 @@ __global__ __launch_bounds__(kGbjThreads) void gbj_gram_kernel(const BjMat* __re
     const int rows = min(kGbjRows, n - r0);
     const double2* x = X + mt.x_off;
     const int tid = threadIdx.x;
-    for (int e = tid; e < kGbjCols * kGbjRows; e += kGbjThreads) {
+    for (int e = tid; e < CC * kGbjRows; e += NT) {
         const int j = e / kGbjRows, i = e % kGbjRows;
         double2 val = double2{0.0, 0.0};
         if (j < c && i < rows) {
@@ -1450,7 +1457,7 @@ __global__ __launch_bounds__(kGbjThreads) void gbj_gram_kernel(const BjMat* __re
     }
     __syncthreads();
     // thread (bi, bj) owns the 2 x 2 block G[2bi:2bi+2, 2bj:2bj+2]
-    const int bi = tid >> 4, bj = tid & 15;
+    const int bi = tid / (CC / 2), bj = tid % (CC / 2);
     double2 g00{0.0, 0.0}, g01{0.0, 0.0}, g10{0.0, 0.0}, g11{0.0, 0.0};
     for (int i = 0; i < rows; ++i) {
         const double2 a0 = xs[2 * bi][i], a1 = xs[2 * bi + 1][i];
@@ -1461,25 +1468,26 @@ __global__ __launch_bounds__(kGbjThreads) void gbj_gram_kernel(const BjMat* __re
         g10.x += a1.x * b0.x + a1.y * b0.y;  g10.y += a1.x * b0.y - a1.y * b0.x;
         g11.x += a1.x * b1.x + a1.y * b1.y;  g11.y += a1.x * b1.y - a1.y * b1.x;
     }
-    double2* g = gbuf + ((int64_t)blockIdx.x * maxch + blockIdx.y) * (kGbjCols * kGbjCols);
-    g[(2 * bi) * kGbjCols + 2 * bj] = g00;
-    g[(2 * bi) * kGbjCols + 2 * bj + 1] = g01;
-    g[(2 * bi + 1) * kGbjCols + 2 * bj] = g10;
-    g[(2 * bi + 1) * kGbjCols + 2 * bj + 1] = g11;
+    double2* g = gbuf + ((int64_t)blockIdx.x * maxch + blockIdx.y) * (CC * CC);
+    g[(2 * bi) * CC + 2 * bj] = g00;
+    g[(2 * bi) * CC + 2 * bj + 1] = g01;
+    g[(2 * bi + 1) * CC + 2 * bj] = g10;
+    g[(2 * bi + 1) * CC + 2 * bj + 1] = g11;
 }
 
-__global__ __launch_bounds__(kGbjThreads) void gbj_rotate_kernel(const BjMat* __restrict__ mats,
-                                                                 const int2* __restrict__ tasks,
-                                                                 double2* __restrict__ X,
-                                                                 double2* __restrict__ V,
-                                                                 const double2* __restrict__ gbuf,
-                                                                 int round,
-                                                                 const int* __restrict__ flags_prev,
-                                                                 int* __restrict__ flags_cur,
-                                                                 double eps, int inner, int maxch) {
-    __shared__ double2 G[kGbjCols][kGbjCols + 1];
-    __shared__ double2 Z[kGbjCols][kGbjCols + 1];
-    __shared__ double2 xs[kGbjRows / 2][kGbjCols + 1];
+template <int CC, int NT>
+__global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict__ mats,
+                                                        const int2* __restrict__ tasks,
+                                                        double2* __restrict__ X,
+                                                        double2* __restrict__ V,
+                                                        const double2* __restrict__ gbuf,
+                                                        int round,
+                                                        const int* __restrict__ flags_prev,
+                                                        int* __restrict__ flags_cur,
+                                                        double eps, int inner, int maxch) {
+    extern __shared__ __align__(16) unsigned char gbj_rot_lds_raw[];
+    double2 (*G)[CC + 1] = reinterpret_cast<double2 (*)[CC + 1]>(gbj_rot_lds_raw);
+    double2 (*Z)[CC + 1] = G + CC;
     __shared__ int s_rot;
     const int2 tk = tasks[blockIdx.x];
     const BjMat mt = mats[tk.x];
@@ -1496,12 +1504,12 @@ __global__ __launch_bounds__(kGbjThreads) void gbj_rotate_kernel(const BjMat* __
     const int rows = min(kGbjRows, n - r0);
     const int nch = (n + kGbjRows - 1) / kGbjRows;
     const int tid = threadIdx.x;
-    const double2* gp = gbuf + (int64_t)blockIdx.x * maxch * (kGbjCols * kGbjCols);
-    for (int e = tid; e < c * c; e += kGbjThreads) {
+    const double2* gp = gbuf + (int64_t)blockIdx.x * maxch * (CC * CC);
+    for (int e = tid; e < c * c; e += NT) {
         const int a = e / c, b = e % c;
         double sr = 0.0, si = 0.0;
         for (int k = 0; k < nch; ++k) {
-            const double2 v = gp[(int64_t)k * (kGbjCols * kGbjCols) + a * kGbjCols + b];
+            const double2 v = gp[(int64_t)k * (CC * CC) + a * CC + b];
             sr += v.x;
             si += v.y;
         }
@@ -1515,7 +1523,7 @@ __global__ __launch_bounds__(kGbjThreads) void gbj_rotate_kernel(const BjMat* __
     // Every pair the inner sweep visits, tested on G as summed: if none passes the rotation
     // test, the first inner round rotates nothing, G stays as it is, and so do all later ones;
     // the sweep is skipped (the result is the one the sweep would give).
-    for (int e = tid; e < c * c; e += kGbjThreads) {
+    for (int e = tid; e < c * c; e += NT) {
         const int a = e / c, b = e % c;
         if (a >= b) continue;
         if (cross_only && !(a < cp && b >= cp)) continue;
@@ -1533,15 +1541,15 @@ __global__ __launch_bounds__(kGbjThreads) void gbj_rotate_kernel(const BjMat* __
     const int C = c + (c & 1);
     const int np = cross_only ? small : C / 2;
     const int nrounds = needed ? (cross_only ? big : C - 1) : 0;
-    // Thread t works on pair g = t / kGbjCols, entry k = t % kGbjCols of the pair's two columns
-    // (then of its two rows), kGbjThreads / kGbjCols pairs at a time.  Every thread of a pair
-    // computes the pair's rotation itself from G as the round found it: the pair's kGbjCols
-    // threads sit in one wavefront and read G[a][a], G[b][b], G[a][b] before any of them writes,
+    // Thread t works on pair g = t / CC, entry k = t % CC of the pair's two columns (then of its
+    // two rows), NT / CC pairs at a time.  Every thread of a pair computes the pair's rotation
+    // itself from G as the round found it: the pair's CC <= 64 threads sit in one wavefront and read G[a][a], G[b][b], G[a][b] before any of them writes,
     // and no other pair writes columns a or b, so a round needs only the barrier between its
     // column and row updates and the one after them.
-    constexpr int kPairsAtOnce = kGbjThreads / kGbjCols;
-    constexpr int kSlots = (kGbjCols / 2 + kPairsAtOnce - 1) / kPairsAtOnce;
-    const int gl = tid / kGbjCols, k = tid % kGbjCols;
+    static_assert(CC <= 64, "a pair's threads must share one wavefront");
+    constexpr int kPairsAtOnce = NT / CC;
+    constexpr int kSlots = (CC / 2 + kPairsAtOnce - 1) / kPairsAtOnce;
+    const int gl = tid / CC, k = tid % CC;
     bool any = false;
     for (int pass = 0; pass < inner; ++pass) {
         for (int r = 0; r < nrounds; ++r) {
@@ -1620,18 +1628,20 @@ __global__ __launch_bounds__(kGbjThreads) void gbj_rotate_kernel(const BjMat* __
     __syncthreads();
     if (!s_rot) return;                       // uniform: read after the last barrier
     auto gcol = [&](int j) { return j < cp ? bp * w + j : bq * w + (j - cp); };
-    // M[rows, cols] <- M[rows, cols] Z for M = X, then V, half a chunk of rows at a time; each
-    // thread forms a 2 x 2 tile of the result (two rows, two columns), so every LDS value it
-    // loads feeds two products
-    constexpr int H = kGbjRows / 2;
-    static_assert((H / 2) * (kGbjCols / 2) == kGbjThreads, "one 2 x 2 tile per thread");
+    // M[rows, cols] <- M[rows, cols] Z for M = X, then V, H rows of the chunk at a time, staged in
+    // G's storage (G is not read past the barrier above); each thread forms a 2 x 2 tile of the
+    // result (two rows, two columns), so every LDS value it loads feeds two products
+    constexpr int H = 4 * NT / CC;
+    static_assert((H / 2) * (CC / 2) == NT && H <= CC && kGbjRows % H == 0, "one 2 x 2 tile per thread");
+    constexpr int kPasses = kGbjRows / H;
+    double2 (*xs)[CC + 1] = G;
     const int ti = (tid % (H / 2)) * 2, tj = (tid / (H / 2)) * 2;   // tile origin
-    for (int pass = 0; pass < 4; ++pass) {
-        double2* m = (pass < 2 ? X + mt.x_off : V + mt.v_off);
-        const int h0 = (pass & 1) * H;
+    for (int pass = 0; pass < 2 * kPasses; ++pass) {
+        double2* m = (pass < kPasses ? X + mt.x_off : V + mt.v_off);
+        const int h0 = (pass % kPasses) * H;
         const int hr = min(H, rows - h0);
         if (hr <= 0) continue;                // uniform
-        for (int e = tid; e < c * H; e += kGbjThreads) {
+        for (int e = tid; e < c * H; e += NT) {
             const int j = e / H, i = e % H;
             if (i < hr) xs[i][j] = m[(int64_t)gcol(j) * n + r0 + h0 + i];
         }
@@ -2438,17 +2448,24 @@ void gram_block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& 
     HIP_CHECK(hipGetLastError());
     int used = 0;
     if (!tasks.empty()) {
+        // block pairs of up to 32 columns on 256 threads (64 columns on 1024 threads halves the
+        // rounds but was measured slower: 8.3 against 5.8 ms on the centre cut of the L = 128,
+        // D = 1024 state, the inner rotations on sixteen waves cost more than the rounds saved)
+        constexpr int cc = 32;
         double2* gbuf = static_cast<double2*>(scratch(
-            ctx, st, sizeof(double2) * (size_t)kGbjCols * kGbjCols * maxch * tasks.size(), 6));
+            ctx, st, sizeof(double2) * (size_t)cc * cc * maxch * tasks.size(), 6));
         int* hp = reinterpret_cast<int*>(pinned(ctx, sizeof(int) * nmat));
         const dim3 grid((unsigned)tasks.size(), (unsigned)maxch);
+        const int inner = std::max(1, ctx->gbj_inner);
+        const double eps = 2.220446049250313e-16;
         for (int s = 0; s < max_sweeps; ++s) {
             for (int r = 0; r < rounds; ++r) {
-                hipLaunchKernelGGL(gbj_gram_kernel, grid, dim3(kGbjThreads), 0, st, dm, dt, X, gbuf,
-                                   r, flags + (size_t)s * nmat, maxch);
-                hipLaunchKernelGGL(gbj_rotate_kernel, grid, dim3(kGbjThreads), 0, st, dm, dt, X, V,
-                                   gbuf, r, flags + (size_t)s * nmat, flags + (size_t)(s + 1) * nmat,
-                                   2.220446049250313e-16, std::max(1, ctx->gbj_inner), maxch);
+                const int* fp = flags + (size_t)s * nmat;
+                int* fc = flags + (size_t)(s + 1) * nmat;
+                hipLaunchKernelGGL((gbj_gram_kernel<cc, 256>), grid, dim3(256), gbj_gram_lds(cc), st,
+                                   dm, dt, X, gbuf, r, fp, maxch);
+                hipLaunchKernelGGL((gbj_rotate_kernel<cc, 256>), grid, dim3(256), gbj_rot_lds(cc), st,
+                                   dm, dt, X, V, gbuf, r, fp, fc, eps, inner, maxch);
             }
             HIP_CHECK(hipGetLastError());
             ++used;

@@ -1169,6 +1169,9 @@ void validate(const bug_step_opts& o, int L, int& rl) {
     if (o.exact) throw StepError(BUG_EINVAL, "exact = true does not run through bug_step");
     if (o.fold_omega == 0) throw StepError(BUG_EINVAL, "unfolded leaks do not run through bug_step");
     if (o.krylov_rank_tol != 0.0) throw StepError(BUG_EINVAL, "ranked Krylov selection does not run through bug_step");
+    if (o.root_method < BUG_ROOT_EXP || o.root_method > BUG_ROOT_RADAU3)
+        throw StepError(BUG_EINVAL, "root_method must be exp, trapezoid, radau2 or radau3");
+    if (o.root_maxiter < 1) throw StepError(BUG_EINVAL, "root_maxiter must be positive");
     if (o.work_ceiling_rule == BUG_CEIL_BOND_STRICT && !dflt && !(f > 1.0))
         throw StepError(BUG_EINVAL, "strict bond factor must exceed 1");
     if (o.work_ceiling_rule == BUG_CEIL_FRAME_ROOT) {
@@ -1436,9 +1439,10 @@ void run_step(const Exec& ex, MPS& psi, const std::vector<MpoSite>& mpo, cplx ta
         return apply_zero_site(ex, rp.Lr, rp.Rr, C, ex.prec);
     };
     ExpvReport rep;
+    const RootRule rule{o.root_method, o.root_shift != 0};
     Tensor result = rp.hermitian
-        ? lanczos_expv(ex, apply_root, tau, rp.start, 1e-15, kRootMaxIter, o.reorth != 0, rp.te, rep)
-        : arnoldi_expv(ex, apply_root, tau, rp.start, 1e-15, kRootMaxIter, rp.te, rep);
+        ? lanczos_expv(ex, apply_root, tau, rp.start, 1e-15, o.root_maxiter, o.reorth != 0, rp.te, rep, rule)
+        : arnoldi_expv(ex, apply_root, tau, rp.start, 1e-15, o.root_maxiter, rp.te, rep, rule);
     step_root_report(rp, rep, napply, now() - ts, out);
     step_finish(ex, psi, rp, result, o, out);
 }
@@ -1631,7 +1635,8 @@ void run_step_batch(bug_ctx* ctx, hipStream_t st, std::vector<MPS>& work,
     std::vector<ExpvReport> reps;
     const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     expv_lockstep(ex, sg, apply, taus, o.hermitian != 0, static_cast<const double2*>(v0.p), 1e-15,
-                  kRootMaxIter, o.reorth != 0, rps[0].te, static_cast<double2*>(res.p), reps, hw);
+                  o.root_maxiter, o.reorth != 0, rps[0].te, static_cast<double2*>(res.p), reps, hw,
+                  RootRule{o.root_method, o.root_shift != 0});
     std::vector<Tensor> results(k);
     for (int i = 0; i < k; ++i) {
         results[i] = t_empty(ex, rps[i].start.L());
@@ -1782,6 +1787,9 @@ void bug_step_opts_default(bug_step_opts* o) {
     o->exact = 0;
     o->fold_omega = -1;
     o->krylov_rank_tol = 0.0;
+    o->root_method = BUG_ROOT_EXP;
+    o->root_shift = 1;
+    o->root_maxiter = kRootMaxIter;
 }
 
 int bug_mpo_create(int32_t L, const bug_mpo_site* sites, bug_mpo** out) {

@@ -25,10 +25,24 @@ double residual_integral(int index, const std::vector<cplx>& M, int n, cplx tau,
 // weight of the (k+1)-th Krylov vector in exp(tau H) v1, H row-major with leading dimension ld
 double direction_weight(const std::vector<cplx>& H, int ld, int k, cplx tau, bool state_seed);
 
+// The root integrator (root_methods.jl:5-71): the exponential, or the stability function
+// R = N / D of one implicit Runge-Kutta step applied in one stage; `shift` moves a rational root's
+// spectrum to the extreme Ritz value (and drops the scale) when tau has a real part.
+struct RootRule {
+    int method = BUG_ROOT_EXP;
+    bool shift = true;
+    bool rational() const { return method != BUG_ROOT_EXP; }
+};
+// R(z) of a rational method
+cplx root_stability(int method, cplx z);
+// R(M) rhs = D(M)^{-1} N(M) rhs for a small dense M (row-major n x n)
+std::vector<cplx> rational_apply(int method, const std::vector<cplx>& M, int n, const std::vector<cplx>& rhs);
+
 struct Tridiagonal {
     int n = 0;
     std::vector<double> vals, vecs;       // vecs column-major
-    Tridiagonal(const std::vector<double>& alpha, const std::vector<double>& beta);
+    RootRule rule;
+    Tridiagonal(const std::vector<double>& alpha, const std::vector<double>& beta, RootRule r = RootRule());
     std::vector<cplx> coeffs(cplx tau, cplx& logscale) const;
 };
 
@@ -36,7 +50,8 @@ struct Hessenberg {
     std::vector<cplx> Hm;                 // row-major n x n
     int n = 0;
     std::vector<cplx> vals;
-    Hessenberg(const std::vector<cplx>& H, int n);
+    RootRule rule;
+    Hessenberg(const std::vector<cplx>& H, int n, RootRule r = RootRule());
     std::vector<cplx> coeffs(cplx tau, cplx& logscale) const;
 };
 
@@ -63,13 +78,15 @@ using PackedApplyFn = std::function<void(const double2* x, double2* y)>;
 void expv_lockstep(const Exec& ex, const PackedSegs& sg, const PackedApplyFn& apply,
                    const std::vector<cplx>& tau, bool hermitian, const double2* v0, double tol,
                    int maxiter, bool reorth, double conv_tol, double2* out,
-                   std::vector<ExpvReport>& reps, int host_threads);
+                   std::vector<ExpvReport>& reps, int host_threads, RootRule rule = RootRule());
 
+// a rational `rule` takes the whole step in one stage (krylov.jl:153-160,365-370)
 Tensor lanczos_expv(const Exec& ex, const std::function<Tensor(const Tensor&)>& apply, cplx tau,
                     const Tensor& v0, double tol, int maxiter, bool reorth, double conv_tol,
-                    ExpvReport& rep);
+                    ExpvReport& rep, RootRule rule = RootRule());
 Tensor arnoldi_expv(const Exec& ex, const std::function<Tensor(const Tensor&)>& apply, cplx tau,
-                    const Tensor& v0, double tol, int maxiter, double conv_tol, ExpvReport& rep);
+                    const Tensor& v0, double tol, int maxiter, double conv_tol, ExpvReport& rep,
+                    RootRule rule = RootRule());
 
 }  // namespace host
 }  // namespace bug

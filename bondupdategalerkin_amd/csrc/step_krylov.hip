@@ -150,7 +150,101 @@ double direction_weight(const std::vector<cplx>& H, int ld, int k, cplx tau, boo
 
 // ---- small propagators ----------------------------------------------------------------------------
 
-Tridiagonal::Tridiagonal(const std::vector<double>& alpha, const std::vector<double>& beta) {
+// ---- the rational roots (root_methods.jl:5-71,143-157) --------------------------------------------
+
+namespace {
+// numerator / denominator coefficients of each method's stability function, constant term first
+struct Pade {
+    std::vector<double> num, den;
+};
+const Pade& pade_of(int method) {
+    static const Pade trap{{1.0, 1.0 / 2}, {1.0, -1.0 / 2}};
+    static const Pade rad2{{1.0, 1.0 / 3}, {1.0, -2.0 / 3, 1.0 / 6}};
+    static const Pade rad3{{1.0, 2.0 / 5, 1.0 / 20}, {1.0, -3.0 / 5, 3.0 / 20, -1.0 / 60}};
+    switch (method) {
+        case BUG_ROOT_TRAPEZOID: return trap;
+        case BUG_ROOT_RADAU2: return rad2;
+        case BUG_ROOT_RADAU3: return rad3;
+        default: throw StepError(BUG_EINVAL, "not a rational root method");
+    }
+}
+
+cplx poly_at(const std::vector<double>& c, cplx z) {
+    cplx out = c.back();
+    for (int k = (int)c.size() - 2; k >= 0; --k) out = out * z + c[k];
+    return out;
+}
+
+// the shift of a rational root: the Ritz value maximising Re(tau lambda), none without `shift`
+// or in real time (tau purely imaginary)
+template <typename V>
+cplx rational_sigma(const std::vector<V>& vals, cplx tau, bool shift) {
+    if (!shift || vals.empty() || tau.real() == 0.0) return 0.0;
+    size_t best = 0;
+    for (size_t i = 1; i < vals.size(); ++i)
+        if ((tau * cplx(vals[i])).real() > (tau * cplx(vals[best])).real()) best = i;
+    return cplx(vals[best]);
+}
+}  // namespace
+
+cplx root_stability(int method, cplx z) {
+    if (method == BUG_ROOT_EXP) return std::exp(z);
+    const Pade& p = pade_of(method);
+    return poly_at(p.num, z) / poly_at(p.den, z);
+}
+
+std::vector<cplx> rational_apply(int method, const std::vector<cplx>& M, int n, const std::vector<cplx>& rhs) {
+    const Pade& p = pade_of(method);
+    // P(M) by Horner, row-major
+    auto poly = [&](const std::vector<double>& c) {
+        std::vector<cplx> out((size_t)n * n, 0.0);
+        for (int i = 0; i < n; ++i) out[(size_t)i * n + i] = c.back();
+        for (int k = (int)c.size() - 2; k >= 0; --k) {
+            std::vector<cplx> next((size_t)n * n, 0.0);
+            for (int i = 0; i < n; ++i)
+                for (int l = 0; l < n; ++l) {
+                    const cplx a = out[(size_t)i * n + l];
+                    if (a == 0.0) continue;
+                    for (int j = 0; j < n; ++j) next[(size_t)i * n + j] += a * M[(size_t)l * n + j];
+                }
+            for (int i = 0; i < n; ++i) next[(size_t)i * n + i] += c[k];
+            out.swap(next);
+        }
+        return out;
+    };
+    const std::vector<cplx> N = poly(p.num), D = poly(p.den);
+    std::vector<cplx> b(n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) b[i] += N[(size_t)i * n + j] * rhs[j];
+    // D x = b by LU with partial pivoting
+    std::vector<cplx> A = D;
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::abs(A[(size_t)r * n + c]) > std::abs(A[(size_t)piv * n + c])) piv = r;
+        if (piv != c) {
+            for (int j = 0; j < n; ++j) std::swap(A[(size_t)c * n + j], A[(size_t)piv * n + j]);
+            std::swap(b[c], b[piv]);
+        }
+        const cplx d = A[(size_t)c * n + c];
+        if (d == 0.0) throw StepError(BUG_ENONFINITE, "singular denominator of the rational root");
+        for (int r = c + 1; r < n; ++r) {
+            const cplx f = A[(size_t)r * n + c] / d;
+            if (f == 0.0) continue;
+            for (int j = c; j < n; ++j) A[(size_t)r * n + j] -= f * A[(size_t)c * n + j];
+            b[r] -= f * b[c];
+        }
+    }
+    for (int c = n - 1; c >= 0; --c) {
+        cplx sum = b[c];
+        for (int k = c + 1; k < n; ++k) sum -= A[(size_t)c * n + k] * b[k];
+        b[c] = sum / A[(size_t)c * n + c];
+    }
+    return b;
+}
+
+Tridiagonal::Tridiagonal(const std::vector<double>& alpha, const std::vector<double>& beta, RootRule r)
+    : rule(r) {
     for (double v : alpha)
         if (!std::isfinite(v))
             throw StepError(BUG_ENONFINITE, "Lanczos tridiagonal has non-finite entries: the input "
@@ -169,6 +263,17 @@ Tridiagonal::Tridiagonal(const std::vector<double>& alpha, const std::vector<dou
 }
 
 std::vector<cplx> Tridiagonal::coeffs(cplx tau, cplx& logscale) const {
+    if (rule.rational()) {
+        // R(tau (T - sigma)) e1 from the eigendecomposition; no scale returned
+        const cplx sigma = rational_sigma(vals, tau, rule.shift);
+        std::vector<cplx> w(n), out(n, 0.0);
+        for (int k = 0; k < n; ++k)
+            w[k] = root_stability(rule.method, tau * (vals[k] - sigma)) * vecs[(size_t)k * n + 0];
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < n; ++k) out[i] += vecs[(size_t)k * n + i] * w[k];
+        logscale = 0.0;
+        return out;
+    }
     int imax = 0;
     double exmax = -std::numeric_limits<double>::infinity();
     for (int i = 0; i < n; ++i) {
@@ -191,7 +296,7 @@ std::vector<cplx> Tridiagonal::coeffs(cplx tau, cplx& logscale) const {
     return out;
 }
 
-Hessenberg::Hessenberg(const std::vector<cplx>& H, int n_) : Hm(H), n(n_) {
+Hessenberg::Hessenberg(const std::vector<cplx>& H, int n_, RootRule r) : Hm(H), n(n_), rule(r) {
     for (const cplx& v : Hm)
         if (!std::isfinite(v.real()) || !std::isfinite(v.imag()))
             throw StepError(BUG_ENONFINITE, "Arnoldi Hessenberg matrix has non-finite entries: the "
@@ -202,6 +307,21 @@ Hessenberg::Hessenberg(const std::vector<cplx>& H, int n_) : Hm(H), n(n_) {
 std::vector<cplx> Hessenberg::coeffs(cplx tau, cplx& logscale) const {
     logscale = 0.0;
     if (n == 0) return {};
+    if (rule.rational()) {
+        // R(tau (Hm - sigma)) e1 by one solve with the denominator; no scale returned
+        const cplx sigma = rational_sigma(vals, tau, rule.shift);
+        std::vector<cplx> M((size_t)n * n);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j)
+                M[(size_t)i * n + j] = tau * (Hm[(size_t)i * n + j] - (i == j ? sigma : 0.0));
+        std::vector<cplx> e1(n, 0.0);
+        e1[0] = 1.0;
+        std::vector<cplx> out = rational_apply(rule.method, M, n, e1);
+        for (const cplx& v : out)
+            if (!std::isfinite(v.real()) || !std::isfinite(v.imag()))
+                throw StepError(BUG_ENONFINITE, "the rational root overflowed");
+        return out;
+    }
     int imax = 0;
     double exmax = -std::numeric_limits<double>::infinity();
     for (int i = 0; i < n; ++i) {
@@ -270,7 +390,7 @@ static Tensor finish_scale(const Exec& ex, Tensor cur, cplx logscale) {
 
 Tensor lanczos_expv(const Exec& ex, const std::function<Tensor(const Tensor&)>& apply, cplx tau,
                     const Tensor& v0, double tol, int maxiter, bool reorth, double conv_tol,
-                    ExpvReport& rep) {
+                    ExpvReport& rep, RootRule rule) {
     rep = ExpvReport();
     const double beta0 = norm(ex, v0);
     if (beta0 == 0.0) return v0;
@@ -290,7 +410,7 @@ Tensor lanczos_expv(const Exec& ex, const std::function<Tensor(const Tensor&)>& 
         for (int it = 2; it <= maxiter; ++it) {
             const double b = norm(ex, r);
             if (b < tol) break;
-            if (conv_tol > 0 && tail_sup(Tridiagonal(alpha, beta), tau_rem, b, conv_tol) < conv_tol) break;
+            if (conv_tol > 0 && tail_sup(Tridiagonal(alpha, beta, rule), tau_rem, b, conv_tol) < conv_tol) break;
             beta.push_back(b);
             basis.push_back(scale(ex, 1.0 / b, r));
             r = apply(basis.back());
@@ -308,9 +428,10 @@ Tensor lanczos_expv(const Exec& ex, const std::function<Tensor(const Tensor&)>& 
                     axpy_many_(ex, cs, bp, r);
                 }
         }
-        const Tridiagonal tri(alpha, beta);
+        const Tridiagonal tri(alpha, beta, rule);
         const double betak = norm(ex, r);
-        const cplx tau_s = pick_tau(tri, tau_rem, betak, conv_tol);
+        // an implicit root takes the whole step at once
+        const cplx tau_s = rule.rational() ? tau_rem : pick_tau(tri, tau_rem, betak, conv_tol);
         cplx ls;
         const std::vector<cplx> coeff = tri.coeffs(tau_s, ls);
         rep.err += tail_sup(tri, tau_s, betak, std::numeric_limits<double>::infinity()) * cn / beta0;
@@ -333,7 +454,8 @@ Tensor lanczos_expv(const Exec& ex, const std::function<Tensor(const Tensor&)>& 
 }
 
 Tensor arnoldi_expv(const Exec& ex, const std::function<Tensor(const Tensor&)>& apply, cplx tau,
-                    const Tensor& v0, double tol, int maxiter, double conv_tol, ExpvReport& rep) {
+                    const Tensor& v0, double tol, int maxiter, double conv_tol, ExpvReport& rep,
+                    RootRule rule) {
     rep = ExpvReport();
     const double beta0 = norm(ex, v0);
     if (beta0 == 0.0) return v0;
@@ -344,7 +466,7 @@ Tensor arnoldi_expv(const Exec& ex, const std::function<Tensor(const Tensor&)>& 
         std::vector<cplx> S((size_t)dim * dim);
         for (int i = 0; i < dim; ++i)
             for (int j = 0; j < dim; ++j) S[(size_t)i * dim + j] = H[(size_t)i * ld + j];
-        return Hessenberg(S, dim);
+        return Hessenberg(S, dim, rule);
     };
     while (true) {
         rep.substeps += 1;
@@ -374,7 +496,7 @@ Tensor arnoldi_expv(const Exec& ex, const std::function<Tensor(const Tensor&)>& 
             basis.push_back(scale(ex, 1.0 / betak, r));
         }
         const Hessenberg hess = lead(H, dim);
-        const cplx tau_s = pick_tau(hess, tau_rem, betak, conv_tol);
+        const cplx tau_s = rule.rational() ? tau_rem : pick_tau(hess, tau_rem, betak, conv_tol);
         cplx ls;
         const std::vector<cplx> coeff = hess.coeffs(tau_s, ls);
         rep.err += tail_sup(hess, tau_s, betak, std::numeric_limits<double>::infinity()) * cn / beta0;
@@ -538,7 +660,7 @@ struct Packed {
 void expv_lockstep(const Exec& ex, const PackedSegs& sg, const PackedApplyFn& apply,
                    const std::vector<cplx>& tau, bool hermitian, const double2* v0, double tol,
                    int maxiter, bool reorth, double conv_tol, double2* out,
-                   std::vector<ExpvReport>& reps, int host_threads) {
+                   std::vector<ExpvReport>& reps, int host_threads, RootRule rule) {
     const int K = sg.K();
     if ((int)tau.size() != K) throw StepError(BUG_EINVAL, "one tau per instance");
     reps.assign(K, ExpvReport());
@@ -603,7 +725,7 @@ void expv_lockstep(const Exec& ex, const PackedSegs& sg, const PackedApplyFn& ap
                 pool.run(K, [&](int i) {
                     if (!pass[i]) return;
                     if (b[i] < tol ||
-                        (conv_tol > 0 && tail_sup(Tridiagonal(alpha[i], beta[i]), tau_rem[i], b[i], conv_tol) < conv_tol)) {
+                        (conv_tol > 0 && tail_sup(Tridiagonal(alpha[i], beta[i], rule), tau_rem[i], b[i], conv_tol) < conv_tol)) {
                         pass[i] = 0;
                         betak[i] = b[i];
                     }
@@ -669,7 +791,7 @@ void expv_lockstep(const Exec& ex, const PackedSegs& sg, const PackedApplyFn& ap
                         std::vector<cplx> S((size_t)k * k);
                         for (int a = 0; a < k; ++a)
                             for (int c = 0; c < k; ++c) S[(size_t)a * k + c] = H[i][(size_t)a * ld + c];
-                        stop = tail_sup(Hessenberg(S, k), tau_rem[i], b[i], conv_tol) < conv_tol;
+                        stop = tail_sup(Hessenberg(S, k, rule), tau_rem[i], b[i], conv_tol) < conv_tol;
                     }
                     if (!stop) {
                         H[i][(size_t)k * ld + (k - 1)] = b[i];
@@ -692,8 +814,8 @@ void expv_lockstep(const Exec& ex, const PackedSegs& sg, const PackedApplyFn& ap
             if (!act[i]) return;
             std::vector<cplx> c;
             if (hermitian) {
-                const Tridiagonal tri(alpha[i], beta[i]);
-                tau_s[i] = pick_tau(tri, tau_rem[i], betak[i], conv_tol);
+                const Tridiagonal tri(alpha[i], beta[i], rule);
+                tau_s[i] = rule.rational() ? tau_rem[i] : pick_tau(tri, tau_rem[i], betak[i], conv_tol);
                 c = tri.coeffs(tau_s[i], ls[i]);
                 err[i] = tail_sup(tri, tau_s[i], betak[i], std::numeric_limits<double>::infinity());
             } else {
@@ -701,8 +823,8 @@ void expv_lockstep(const Exec& ex, const PackedSegs& sg, const PackedApplyFn& ap
                 std::vector<cplx> S((size_t)d * d);
                 for (int a = 0; a < d; ++a)
                     for (int b = 0; b < d; ++b) S[(size_t)a * d + b] = H[i][(size_t)a * ld + b];
-                const Hessenberg hess(S, d);
-                tau_s[i] = pick_tau(hess, tau_rem[i], betak[i], conv_tol);
+                const Hessenberg hess(S, d, rule);
+                tau_s[i] = rule.rational() ? tau_rem[i] : pick_tau(hess, tau_rem[i], betak[i], conv_tol);
                 c = hess.coeffs(tau_s[i], ls[i]);
                 err[i] = tail_sup(hess, tau_s[i], betak[i], std::numeric_limits<double>::infinity());
             }

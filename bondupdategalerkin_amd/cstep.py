@@ -25,6 +25,7 @@ from .mpo import MPO
 from .mps import SymMPS
 
 CEIL_RULES = {"bond": 0, "maxdim": 1, "bond_strict": 2, "frame_root": 3}
+ROOT_METHODS = {"exp": 0, "trapezoid": 1, "radau2": 2, "radau3": 3}
 KRYLOV_MODES = {"auto": 0, "lanczos": 1, "arnoldi": 2}
 UNMET_BITS = [(1, "sample"), (2, "krylov"), (4, "closure"), (8, "projection"), (16, "expv"),
               (32, "trunc")]
@@ -165,7 +166,8 @@ def step_options(*, order: int = 1, maxdim: Optional[int] = None, hermitian: boo
                  root: Optional[int] = None, reorth: bool = True, truncate: bool = True,
                  precision: str = "fp64", parallel: bool = False, rng: Optional[int] = None,
                  expansion=None, exact: bool = False, fold_omega: Optional[bool] = None,
-                 krylov_rank_tol: float = 0.0) -> _lib.BugStepOpts:
+                 krylov_rank_tol: float = 0.0, root_method: str = "exp", root_shift: bool = True,
+                 root_maxiter: Optional[int] = None) -> _lib.BugStepOpts:
     """The options POD from the reference's keywords (the Python step's names)."""
     if tol_root is not None and tol_expv is not None:
         raise ValueError("tol_root and tol_expv name the same tolerance; pass one")
@@ -206,6 +208,12 @@ def step_options(*, order: int = 1, maxdim: Optional[int] = None, hermitian: boo
     o.exact = 1 if exact else 0
     o.fold_omega = -1 if fold_omega is None else (1 if fold_omega else 0)
     o.krylov_rank_tol = krylov_rank_tol
+    if root_method not in ROOT_METHODS:
+        raise ValueError(f"root_method must be one of {tuple(ROOT_METHODS)}")
+    o.root_method = ROOT_METHODS[root_method]
+    o.root_shift = 1 if root_shift else 0
+    if root_maxiter is not None:
+        o.root_maxiter = root_maxiter
     return o
 
 

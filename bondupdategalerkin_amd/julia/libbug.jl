@@ -32,6 +32,8 @@ const BUG_UNMET_NAMES = ((1, :sample), (2, :krylov), (4, :closure), (8, :project
 const BUG_CEIL_RULES = Dict(:bond => Int32(0), :maxdim => Int32(1), :bond_strict => Int32(2),
                             :frame_root => Int32(3))
 const BUG_KRYLOV_MODES = Dict(:auto => Int32(0), :lanczos => Int32(1), :arnoldi => Int32(2))
+const BUG_ROOT_METHODS = Dict(:exp => Int32(0), :trapezoid => Int32(1), :radau2 => Int32(2),
+                              :radau3 => Int32(3))
 
 # ── C structs (field for field as in include/bug.h; Julia lays isbits fields out as C does) ──
 
@@ -88,6 +90,9 @@ mutable struct BugStepOpts
     exact::Int32
     fold_omega::Int32
     krylov_rank_tol::Float64
+    root_method::Int32
+    root_shift::Int32
+    root_maxiter::Int32
     BugStepOpts() = new()
 end
 
@@ -392,7 +397,8 @@ function step_options(; order::Int = 1, maxdim::Union{Nothing,Int} = nothing,
                       parallel::Bool = false, rng::Union{Nothing,Integer} = nothing,
                       fixed_width::Union{Nothing,Int} = nothing,
                       tail_factor::Union{Nothing,Real} = nothing,
-                      krylov_mode::Symbol = :auto)
+                      krylov_mode::Symbol = :auto, root_method::Symbol = :exp,
+                      root_shift::Bool = true, root_maxiter::Union{Nothing,Int} = nothing)
     tol_expv !== nothing && tol_root !== nothing &&
         throw(ArgumentError("tol_root and tol_expv name the same tolerance; pass one"))
     haskey(BUG_CEIL_RULES, work_ceiling_rule) ||
@@ -420,6 +426,11 @@ function step_options(; order::Int = 1, maxdim::Union{Nothing,Int} = nothing,
     fixed_width === nothing || (o.fixed_width = fixed_width)
     tail_factor === nothing || (o.tail_factor = tail_factor)
     o.krylov_mode = BUG_KRYLOV_MODES[krylov_mode]
+    haskey(BUG_ROOT_METHODS, root_method) ||
+        throw(ArgumentError("root_method must be one of $(keys(BUG_ROOT_METHODS))"))
+    o.root_method = BUG_ROOT_METHODS[root_method]
+    o.root_shift = root_shift
+    root_maxiter === nothing || (o.root_maxiter = root_maxiter)
     return o
 end
 

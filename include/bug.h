@@ -398,6 +398,12 @@ typedef struct bug_mpo bug_mpo;
 #define BUG_KRYLOV_LANCZOS 1
 #define BUG_KRYLOV_ARNOLDI 2
 
+/* bug_step_opts.root_method: the root integrator (ROOT_METHODS, root_methods.jl:5-20) */
+#define BUG_ROOT_EXP 0          /* the Krylov exponential with substeps (the reference's default) */
+#define BUG_ROOT_TRAPEZOID 1    /* the implicit trapezoid's stability function, one stage */
+#define BUG_ROOT_RADAU2 2       /* Radau IIA, two stages */
+#define BUG_ROOT_RADAU3 3       /* Radau IIA, three stages */
+
 /* bits of bug_step_info.unmet: which tests of the step's budget failed (_unmet!, basis_budget.jl) */
 #define BUG_UNMET_SAMPLE 1
 #define BUG_UNMET_KRYLOV 2
@@ -430,6 +436,10 @@ typedef struct bug_step_opts {
     int32_t exact;                /* must be 0 here */
     int32_t fold_omega;           /* -1 default (folded leaks for order > 1), 1 folded; 0 refused */
     double krylov_rank_tol;       /* must be 0 here */
+    int32_t root_method;          /* BUG_ROOT_* (bug_step.jl:76-81) */
+    int32_t root_shift;           /* rational roots: 1 shift to the extreme Ritz value and drop the
+                                     scale; 0 apply R(tau A_G) as it is (a linear PDE flow) */
+    int32_t root_maxiter;         /* the root's Krylov basis cap (30) */
 } bug_step_opts;
 
 /* One half-sweep bond visit of the step (_log_bond!, basis_budget.jl:399-436). */

@@ -169,6 +169,48 @@ def test_c_step_equals_python_device_step(gpu, precision, tol):
                             [fam_from_device(t) for t in psi_p.tensors]) <= tol, n
 
 
+@pytest.mark.parametrize("method,shift,kind", [("radau2", False, "heston"),
+                                                ("trapezoid", False, "heston"),
+                                                ("radau3", True, "chain")])
+def test_c_step_rational_root_equals_python_device_step(gpu, method, shift, kind):
+    """The rational roots (root_methods.jl:5-71,143-157) through the C entry point against the
+    package's Python step: an unshifted Radau / trapezoid root on a Heston grid (the linear-flow
+    step of qtt_bug_step!) and a shifted Radau IIA(3) root on an XXZ chain at real tau (the
+    imaginary-time use, where the shift applies), equal to 1e-10 with the same Krylov size."""
+    import math
+
+    from bondupdategalerkin_amd import bug_step as B
+    from bondupdategalerkin_amd import cstep as C
+    from bondupdategalerkin_amd import mps
+    from bondupdategalerkin_amd import qtt as Q
+    from bondupdategalerkin_amd.mpo import xxz_mpo
+
+    if kind == "heston":
+        g = Q.qtt_grid(4, 4, x=(-1.0, 1.0), y=(0.0, 0.5), periodic_x=False, periodic_y=False)
+        init = Q.qtt_separable(g, lambda x: math.exp(0.5 * x), lambda v: math.exp(-v))
+        make = lambda: Q.qtt_mps(init)                                   # noqa: E731
+        mpo = Q.heston_mpo(g, Q.HestonParams(0.03, 0.01, 2.0, 0.09, 0.4, -0.7))
+        tau, herm = 0.02, False
+    else:
+        make = lambda: mps.neel_state(12)                                # noqa: E731
+        mpo = xxz_mpo(12, delta=0.5)
+        tau, herm = -0.05, True
+    kw = dict(order=2, maxdim=32, work_ceiling_rule="maxdim", hermitian=herm, root_method=method,
+              root_shift=shift, root_maxiter=200 if kind == "heston" else 30, tol_root=1e-11)
+    psi_p = make()
+    st, mp = C.CState(make()), C.CMPO(mpo)
+    for n in range(3):
+        print(f"case {method} shift={shift} {kind} step {n}", flush=True)
+        info_p = B.cbe_bug_step(psi_p, mpo, tau, **kw)
+        info_c = C.bug_step(st, mp, tau, **kw)
+        psi_c = st.to_mps()
+        assert psi_c.bond_dims() == psi_p.bond_dims(), n
+        assert info_c["expv_iters"] == info_p.ledger.expv_iters, n
+        assert info_c["status"] == B.budget_status(info_p.ledger), n
+        assert mps_distance([fam_from_device(t) for t in psi_c.tensors],
+                            [fam_from_device(t) for t in psi_p.tensors]) <= 1e-10, n
+
+
 def test_c_step_refuses_options_before_touching_the_state(gpu):
     from bondupdategalerkin_amd import _lib
     from bondupdategalerkin_amd import bug_step as B

@@ -39,19 +39,23 @@ def _chain_dense(cs):
     return om.mps_dense([fam_from_device(t) for t in cs.to_mps().tensors])
 
 
-@pytest.mark.parametrize("bits,workers,order", [(4, 3, 2), (5, 2, 2), (4, 5, 1)])
-def test_heston_batch_equals_single_steps(gpu, bits, workers, order):
-    """Non-Hermitian roots (Arnoldi with substeps): five Heston parameter sets, two steps."""
+@pytest.mark.parametrize("bits,workers,order,root", [(4, 3, 2, "exp"), (5, 2, 2, "exp"),
+                                                     (4, 5, 1, "exp"), (4, 3, 2, "radau2")])
+def test_heston_batch_equals_single_steps(gpu, bits, workers, order, root):
+    """Non-Hermitian roots (Arnoldi with substeps, or the unshifted Radau IIA(2) root of the
+    linear-flow step): five Heston parameter sets, two steps."""
     from bondupdategalerkin_amd import cstep as C
 
     kw = dict(order=order, maxdim=32, hermitian=False, work_ceiling_rule="maxdim", tol_basis=1e-8,
               tol_trunc=1e-12)
+    if root != "exp":
+        kw.update(root_method=root, root_shift=False, root_maxiter=200)
     built = [_heston(bits, p) for p in _PARAMS]
     single = [(C.CState(psi), C.CMPO(mpo)) for psi, mpo in built]
     batch = [(C.CState(psi), C.CMPO(mpo)) for psi, mpo in built]
     taus = [0.02, 0.015, 0.02, 0.025, 0.01]
     for step in range(2):
-        print(f"case bits={bits} workers={workers} order={order} step={step}", flush=True)
+        print(f"case bits={bits} workers={workers} order={order} root={root} step={step}", flush=True)
         ref = [C.bug_step(s, m, t, **kw) for (s, m), t in zip(single, taus)]
         got = C.bug_step_batch([s for s, _ in batch], [m for _, m in batch], taus,
                                workers=workers, **kw)
