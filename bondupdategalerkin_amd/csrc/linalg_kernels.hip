@@ -1410,11 +1410,61 @@ __global__ __launch_bounds__(kBjThreads) void bj_round_kernel(const BjMat* __res
 //                        on G itself (the one-sided rotations of the columns, applied to their
 //                        Gram matrix: G <- J^H G J, Z <- Z J), and applies the accumulated Z to its
 //                        row chunk of X and of V.
-// The rotation test and angles are those of the staged kernel (|g_ab|^2 > eps^2 n g_aa g_bb);
+// The rotation test and the angle formula are those of the staged kernel (|g_ab|^2 > eps^2 n g_aa g_bb);
 // the Gram matrix is recomputed from the columns every round, so its updates within the round
 // never accumulate past one round, and convergence is judged on fresh inner products.
 constexpr int kGbjCols = 32;   // widest block pair (2w)
-constexpr int kGbjRows = 64;   // rows per chunk
+constexpr int kGbjRotThreads = 512;   // rotate kernel: every pair of an inner round at once
+
+// 1 / sqrt(x) and 1 / x for x in [1e-280, 1e280]: the hardware estimates (about 23 correct bits)
+// refined by two Newton steps, a few ulps in all; the IEEE sequences' range scaling and special
+// cases are not needed in that range
+__device__ __forceinline__ double rsq_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    double r = fma(-h * y, y, 0.5);
+    y = fma(y, r, y);
+    r = fma(-h * y, y, 0.5);
+    return fma(y, r, y);
+}
+__device__ __forceinline__ double rcp_nr(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+// The Jacobi rotation of the Hermitian 2 x 2 [[al, g], [conj g, be]], ag2 = |g|^2 > 0:
+// zeta = (be - al) / (2 |g|), t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), cs = 1 / sqrt(1 + t^2),
+// sn = cs t, phase e = g / |g|.  The latency of this chain sets the pace of the inner sweeps, so
+// it runs on the refined estimates; |g|^2 outside their range takes the IEEE operations.
+__device__ __forceinline__ void jacobi_rotation(double al, double be, double gr, double gi, double ag2,
+                                                double& cs, double& sn, double& er, double& ei) {
+    double inv, t;
+    if (ag2 >= 1e-280 && ag2 <= 1e280) {
+        inv = rsq_nr(ag2);
+        const double zeta = 0.5 * (be - al) * inv;
+        const double az = fabs(zeta);
+        if (az < 1e100) {
+            const double z1 = fma(zeta, zeta, 1.0);
+            t = rcp_nr(az + z1 * rsq_nr(z1));
+        } else {
+            t = 0.5 / az;             // sqrt(1 + zeta^2) = |zeta| in working precision
+        }
+        if (zeta < 0.0) t = -t;
+        cs = rsq_nr(fma(t, t, 1.0));
+    } else {
+        inv = 1.0 / sqrt(ag2);
+        const double zeta = 0.5 * (be - al) * inv;
+        t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        cs = 1.0 / sqrt(1.0 + t * t);
+    }
+    sn = cs * t;
+    er = gr * inv;
+    ei = gi * inv;
+}
+constexpr int kGbjRows = 32;   // rows per chunk (64 measured slower: twice the update work per workgroup)
 // LDS of one instance: the gram kernel stages CC columns of a chunk, the rotate kernel holds G and
 // Z (CC x (CC + 1) each; its row staging for the update reuses G's storage)
 constexpr size_t gbj_gram_lds(int cc) { return sizeof(double2) * (size_t)cc * (kGbjRows + 1); }
@@ -1504,6 +1554,7 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
     const int rows = min(kGbjRows, n - r0);
     const int nch = (n + kGbjRows - 1) / kGbjRows;
     const int tid = threadIdx.x;
+    auto gcol = [&](int j) { return j < cp ? bp * w + j : bq * w + (j - cp); };
     const double2* gp = gbuf + (int64_t)blockIdx.x * maxch * (CC * CC);
     for (int e = tid; e < c * c; e += NT) {
         const int a = e / c, b = e % c;
@@ -1515,6 +1566,30 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
         }
         G[a][b] = double2{sr, si};
         Z[a][b] = double2{a == b ? 1.0 : 0.0, 0.0};
+    }
+    // The update at the end rewrites this chunk's rows of the pair's columns of X and V, H rows a
+    // pass; no other workgroup writes them in this round, so they are loaded here, after the
+    // partial Grams (whose loads the sum above has waited for), and their latency runs under the
+    // inner sweep, which touches only LDS, instead of after it.
+    // (the first NZ threads form the products; all of them stage and meet the barriers)
+    constexpr int NZ = NT < 256 ? NT : 256;
+    constexpr int H = 4 * NZ / CC;
+    static_assert((H / 2) * (CC / 2) == NZ && H <= CC && kGbjRows % H == 0, "one 2 x 2 tile per thread");
+    constexpr int kPasses = kGbjRows / H;
+    constexpr int kPre = (CC * H + NT - 1) / NT;
+    double2 pre[2 * kPasses][kPre];
+#pragma unroll
+    for (int pass = 0; pass < 2 * kPasses; ++pass) {
+        const double2* m = pass < kPasses ? X + mt.x_off : V + mt.v_off;
+        const int h0 = (pass % kPasses) * H;
+        const int hr = min(H, rows - h0);
+#pragma unroll
+        for (int t = 0; t < kPre; ++t) {
+            const int e = tid + t * NT;
+            const int j = e / H, i = e % H;
+            pre[pass][t] = double2{0.0, 0.0};
+            if (e < c * H && i < hr) pre[pass][t] = m[(int64_t)gcol(j) * n + r0 + h0 + i];
+        }
     }
     if (tid == 0) s_rot = 0;
     __syncthreads();
@@ -1575,14 +1650,7 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
                 const double gr = G[a][b].x, gi = G[a][b].y;
                 const double ag2 = gr * gr + gi * gi;
                 if (!(ag2 > tol2 * (al * be) && ag2 != 0.0)) continue;
-                const double ag = sqrt(ag2);
-                const double inv = 1.0 / ag;
-                const double zeta = 0.5 * (be - al) * inv;
-                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                cs[sl] = 1.0 / sqrt(1.0 + t * t);
-                sn[sl] = cs[sl] * t;
-                er[sl] = gr * inv;
-                ei[sl] = gi * inv;
+                jacobi_rotation(al, be, gr, gi, ag2, cs[sl], sn[sl], er[sl], ei[sl]);
                 ra[sl] = a;
                 rb[sl] = b;
             }
@@ -1627,26 +1695,25 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
     if (any) s_rot = 1;
     __syncthreads();
     if (!s_rot) return;                       // uniform: read after the last barrier
-    auto gcol = [&](int j) { return j < cp ? bp * w + j : bq * w + (j - cp); };
     // M[rows, cols] <- M[rows, cols] Z for M = X, then V, H rows of the chunk at a time, staged in
     // G's storage (G is not read past the barrier above); each thread forms a 2 x 2 tile of the
     // result (two rows, two columns), so every LDS value it loads feeds two products
-    constexpr int H = 4 * NT / CC;
-    static_assert((H / 2) * (CC / 2) == NT && H <= CC && kGbjRows % H == 0, "one 2 x 2 tile per thread");
-    constexpr int kPasses = kGbjRows / H;
     double2 (*xs)[CC + 1] = G;
-    const int ti = (tid % (H / 2)) * 2, tj = (tid / (H / 2)) * 2;   // tile origin
+    const int ti = (tid % (H / 2)) * 2, tj = (tid / (H / 2)) * 2;   // tile origin (tid < NZ)
+#pragma unroll
     for (int pass = 0; pass < 2 * kPasses; ++pass) {
         double2* m = (pass < kPasses ? X + mt.x_off : V + mt.v_off);
         const int h0 = (pass % kPasses) * H;
         const int hr = min(H, rows - h0);
         if (hr <= 0) continue;                // uniform
-        for (int e = tid; e < c * H; e += NT) {
+#pragma unroll
+        for (int t = 0; t < kPre; ++t) {
+            const int e = tid + t * NT;
             const int j = e / H, i = e % H;
-            if (i < hr) xs[i][j] = m[(int64_t)gcol(j) * n + r0 + h0 + i];
+            if (e < c * H && i < hr) xs[i][j] = pre[pass][t];
         }
         __syncthreads();
-        if (tj < c) {
+        if (tid < NZ && tj < c) {
             double2 o00{0.0, 0.0}, o01{0.0, 0.0}, o10{0.0, 0.0}, o11{0.0, 0.0};
             const bool j1 = tj + 1 < c;
             for (int kk = 0; kk < c; ++kk) {
@@ -2464,7 +2531,8 @@ void gram_block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& 
                 int* fc = flags + (size_t)(s + 1) * nmat;
                 hipLaunchKernelGGL((gbj_gram_kernel<cc, 256>), grid, dim3(256), gbj_gram_lds(cc), st,
                                    dm, dt, X, gbuf, r, fp, maxch);
-                hipLaunchKernelGGL((gbj_rotate_kernel<cc, 256>), grid, dim3(256), gbj_rot_lds(cc), st,
+                hipLaunchKernelGGL((gbj_rotate_kernel<cc, kGbjRotThreads>), grid, dim3(kGbjRotThreads),
+                                   gbj_rot_lds(cc), st,
                                    dm, dt, X, V, gbuf, r, fp, fc, eps, inner, maxch);
             }
             HIP_CHECK(hipGetLastError());
