@@ -1465,10 +1465,10 @@ __device__ __forceinline__ void jacobi_rotation(double al, double be, double gr,
     ei = gi * inv;
 }
 constexpr int kGbjRows = 32;   // rows per chunk (64 measured slower: twice the update work per workgroup)
-// LDS of one instance: the gram kernel stages CC columns of a chunk, the rotate kernel holds G and
-// Z (CC x (CC + 1) each; its row staging for the update reuses G's storage)
+// LDS of one instance: the gram kernel stages CC columns of a chunk, the rotate kernel holds G,
+// Z and the staged V rows (CC x (CC + 1) each; the X rows are staged in G's storage)
 constexpr size_t gbj_gram_lds(int cc) { return sizeof(double2) * (size_t)cc * (kGbjRows + 1); }
-constexpr size_t gbj_rot_lds(int cc) { return 2 * sizeof(double2) * (size_t)cc * (cc + 1); }
+constexpr size_t gbj_rot_lds(int cc) { return 3 * sizeof(double2) * (size_t)cc * (cc + 1); }
 
 // CC columns per block pair on NT = (CC / 2)^2 threads
 template <int CC, int NT>
@@ -1556,39 +1556,36 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
     const int tid = threadIdx.x;
     auto gcol = [&](int j) { return j < cp ? bp * w + j : bq * w + (j - cp); };
     const double2* gp = gbuf + (int64_t)blockIdx.x * maxch * (CC * CC);
-    for (int e = tid; e < c * c; e += NT) {
-        const int a = e / c, b = e % c;
+    // G and Z over the whole CC x CC: zero outside the pair's c columns, where the update's
+    // products below read Z
+    for (int e = tid; e < CC * CC; e += NT) {
+        const int a = e / CC, b = e % CC;
         double sr = 0.0, si = 0.0;
-        for (int k = 0; k < nch; ++k) {
-            const double2 v = gp[(int64_t)k * (CC * CC) + a * CC + b];
-            sr += v.x;
-            si += v.y;
-        }
+        if (a < c && b < c)
+            for (int k = 0; k < nch; ++k) {
+                const double2 v = gp[(int64_t)k * (CC * CC) + a * CC + b];
+                sr += v.x;
+                si += v.y;
+            }
         G[a][b] = double2{sr, si};
-        Z[a][b] = double2{a == b ? 1.0 : 0.0, 0.0};
+        Z[a][b] = double2{a == b && a < c ? 1.0 : 0.0, 0.0};
     }
-    // The update at the end rewrites this chunk's rows of the pair's columns of X and V, H rows a
-    // pass; no other workgroup writes them in this round, so they are loaded here, after the
-    // partial Grams (whose loads the sum above has waited for), and their latency runs under the
-    // inner sweep, which touches only LDS, instead of after it.
-    // (the first NZ threads form the products; all of them stage and meet the barriers)
-    constexpr int NZ = NT < 256 ? NT : 256;
-    constexpr int H = 4 * NZ / CC;
-    static_assert((H / 2) * (CC / 2) == NZ && H <= CC && kGbjRows % H == 0, "one 2 x 2 tile per thread");
-    constexpr int kPasses = kGbjRows / H;
-    constexpr int kPre = (CC * H + NT - 1) / NT;
-    double2 pre[2 * kPasses][kPre];
+    // The update at the end rewrites this chunk's rows of the pair's columns of X and V; no other
+    // workgroup writes them in this round, so they are loaded here, after the partial Grams (whose
+    // loads the sum above has waited for), and their latency runs under the inner sweep, which
+    // touches only LDS, instead of after it.  Zero outside the c columns and the chunk's rows.
+    static_assert(CC == 32 && kGbjRows == 32 && NT == 512, "the update's tiling: 8 waves, 16 x 16 tiles");
+    constexpr int kPre = CC * kGbjRows / NT;
+    double2 pre[2][kPre];
 #pragma unroll
-    for (int pass = 0; pass < 2 * kPasses; ++pass) {
-        const double2* m = pass < kPasses ? X + mt.x_off : V + mt.v_off;
-        const int h0 = (pass % kPasses) * H;
-        const int hr = min(H, rows - h0);
+    for (int mat = 0; mat < 2; ++mat) {
+        const double2* m = mat == 0 ? X + mt.x_off : V + mt.v_off;
 #pragma unroll
         for (int t = 0; t < kPre; ++t) {
             const int e = tid + t * NT;
-            const int j = e / H, i = e % H;
-            pre[pass][t] = double2{0.0, 0.0};
-            if (e < c * H && i < hr) pre[pass][t] = m[(int64_t)gcol(j) * n + r0 + h0 + i];
+            const int j = e / kGbjRows, i = e % kGbjRows;
+            pre[mat][t] = double2{0.0, 0.0};
+            if (j < c && i < rows) pre[mat][t] = m[(int64_t)gcol(j) * n + r0 + i];
         }
     }
     if (tid == 0) s_rot = 0;
@@ -1695,45 +1692,46 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
     if (any) s_rot = 1;
     __syncthreads();
     if (!s_rot) return;                       // uniform: read after the last barrier
-    // M[rows, cols] <- M[rows, cols] Z for M = X, then V, H rows of the chunk at a time, staged in
-    // G's storage (G is not read past the barrier above); each thread forms a 2 x 2 tile of the
-    // result (two rows, two columns), so every LDS value it loads feeds two products
-    double2 (*xs)[CC + 1] = G;
-    const int ti = (tid % (H / 2)) * 2, tj = (tid / (H / 2)) * 2;   // tile origin (tid < NZ)
+    // M_chunk <- M_chunk Z for M = X and V (32 x 32 complex each) on FP64 MFMA 16x16x4: the X rows
+    // staged in G's storage (G is not read past the barrier above), the V rows in their own;
+    // wave w forms the 16 x 16 output tile (w / 2 % 2, w % 2) of X (w < 4) or of V.  Fed as
+    // a = Z^T and b = M^T, the product is (M Z)^T: lane l holds output row l % 16, so 16 lanes
+    // store 16 consecutive complex values of one column.
+    double2 (*xsx)[CC + 1] = G;
+    double2 (*xsv)[CC + 1] = G + 2 * CC;
 #pragma unroll
-    for (int pass = 0; pass < 2 * kPasses; ++pass) {
-        double2* m = (pass < kPasses ? X + mt.x_off : V + mt.v_off);
-        const int h0 = (pass % kPasses) * H;
-        const int hr = min(H, rows - h0);
-        if (hr <= 0) continue;                // uniform
+    for (int t = 0; t < kPre; ++t) {
+        const int e = tid + t * NT;
+        const int j = e / kGbjRows, i = e % kGbjRows;
+        xsx[i][j] = pre[0][t];
+        xsv[i][j] = pre[1][t];
+    }
+    __syncthreads();
+    {
+        typedef double dv4 __attribute__((ext_vector_type(4)));
+        const int wave = tid >> 6, lane = tid & 63;
+        const int mat = wave >> 2, tr = ((wave >> 1) & 1) * 16, tc = (wave & 1) * 16;
+        const int li = lane & 15, lk = lane >> 4;
+        double2 (*xs)[CC + 1] = mat == 0 ? xsx : xsv;
+        dv4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int t = 0; t < kPre; ++t) {
-            const int e = tid + t * NT;
-            const int j = e / H, i = e % H;
-            if (e < c * H && i < hr) xs[i][j] = pre[pass][t];
+        for (int k0 = 0; k0 < CC; k0 += 4) {
+            const double2 z = Z[k0 + lk][tc + li];
+            const double2 x = xs[tr + li][k0 + lk];
+            // (M Z)^T = Z^T M^T: re = Zr^T Mr^T - Zi^T Mi^T, im = Zi^T Mr^T + Zr^T Mi^T
+            accr = __builtin_amdgcn_mfma_f64_16x16x4f64(z.x, x.x, accr, 0, 0, 0);
+            accr = __builtin_amdgcn_mfma_f64_16x16x4f64(-z.y, x.y, accr, 0, 0, 0);
+            acci = __builtin_amdgcn_mfma_f64_16x16x4f64(z.y, x.x, acci, 0, 0, 0);
+            acci = __builtin_amdgcn_mfma_f64_16x16x4f64(z.x, x.y, acci, 0, 0, 0);
         }
-        __syncthreads();
-        if (tid < NZ && tj < c) {
-            double2 o00{0.0, 0.0}, o01{0.0, 0.0}, o10{0.0, 0.0}, o11{0.0, 0.0};
-            const bool j1 = tj + 1 < c;
-            for (int kk = 0; kk < c; ++kk) {
-                const double2 p0 = xs[ti][kk], p1 = xs[ti + 1][kk];
-                const double2 z0 = Z[kk][tj], z1 = j1 ? Z[kk][tj + 1] : double2{0.0, 0.0};
-                o00.x += p0.x * z0.x - p0.y * z0.y;  o00.y += p0.x * z0.y + p0.y * z0.x;
-                o01.x += p0.x * z1.x - p0.y * z1.y;  o01.y += p0.x * z1.y + p0.y * z1.x;
-                o10.x += p1.x * z0.x - p1.y * z0.y;  o10.y += p1.x * z0.y + p1.y * z0.x;
-                o11.x += p1.x * z1.x - p1.y * z1.y;  o11.y += p1.x * z1.y + p1.y * z1.x;
-            }
-            double2* c0 = m + (int64_t)gcol(tj) * n + r0 + h0;
-            if (ti < hr) c0[ti] = o00;
-            if (ti + 1 < hr) c0[ti + 1] = o10;
-            if (j1) {
-                double2* c1 = m + (int64_t)gcol(tj + 1) * n + r0 + h0;
-                if (ti < hr) c1[ti] = o01;
-                if (ti + 1 < hr) c1[ti + 1] = o11;
-            }
+        // register r of lane l: (M Z)[tr + l % 16][tc + l / 16 + 4 r]
+        double2* m = mat == 0 ? X + mt.x_off : V + mt.v_off;
+        const int i = tr + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int j = tc + lk + 4 * r;
+            if (i < rows && j < c) m[(int64_t)gcol(j) * n + r0 + i] = double2{accr[r], acci[r]};
         }
-        __syncthreads();
     }
     if (tid == 0) flags_cur[tk.x] = 1;
 }
