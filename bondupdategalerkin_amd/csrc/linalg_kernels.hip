@@ -616,23 +616,26 @@ __global__ __launch_bounds__(kQrThreads) void qrp_kernel(const MatJob* __restric
 // Panels of PW columns (16, or 8 for blocks of more than 512 rows so that a lane's RPT rows stay in
 // registers).  The panel kernel (one workgroup per matrix, one wave per panel column, the
 // column's rows held in registers, RPT per lane) factors the panel with the same
-// reflectors as qr_kernel, then builds the compact-WY factor T (H_p0 ... H_p0+jb-1 = I - V T V^H)
-// and Y = V T.  The trailing columns are updated with two grouped GEMMs, W = Y^H A_trail and
-// A_trail -= V W; Q = H_0 ... H_{k-1} [I; 0] is formed backwards per panel with W = V^H Q_sub,
-// Q_sub -= Y W.  qr_finish_kernel extracts R and applies the positive-diagonal gauge.
+// reflectors as qr_kernel and builds the compact-WY factor T (H_p0 ... H_p0+jb-1 = I - V T V^H).
+// The trailing columns are updated slab by slab, W = T^H (V^H A_trail) and A_trail -= V W;
+// Q = H_0 ... H_{k-1} [I; 0] is formed backwards per panel with W = T (V^H Q_sub),
+// Q_sub -= V W.  qr_finish_kernel extracts R and applies the positive-diagonal gauge.
 
 constexpr int kQrPanel = 16;   // widest panel
 
 struct QrPanelJob {
-    int64_t a_off, v_off, y_off;   // A (lda); V and Y: m x k column-major, leading dimension m
+    int64_t a_off, v_off, pad0;    // A (lda); V: m x k column-major, leading dimension m
     int32_t m, n, lda, pad;
 };
+
+// T of one panel of one matrix: kQrPanel x kQrPanel row-major, the jb x jb upper triangle used
+constexpr int kQrTSlot = kQrPanel * kQrPanel;
 
 template <int RPT, int PW>
 __global__ __launch_bounds__(64 * PW) void qr_panel_kernel(const QrPanelJob* __restrict__ jobs, int p0,
                                                            double2* __restrict__ A,
                                                            double2* __restrict__ Vb,
-                                                           double2* __restrict__ Yb) {
+                                                           double2* __restrict__ Tp) {
     __shared__ double2 vbuf[2][64 * RPT];
     __shared__ double2 s_tau[PW];
     __shared__ double2 Gs[PW][PW];
@@ -743,90 +746,66 @@ __global__ __launch_bounds__(64 * PW) void qr_panel_kernel(const QrPanelJob* __r
         }
     }
     __syncthreads();
-    // T: T[c][c] = tau_c, T[0:c, c] = -tau_c T[0:c, 0:c] G[0:c, c]
-    for (int c = 0; c < jb; ++c) {
-        const int a = threadIdx.x;
-        if (a < c) {
+    // T = U^-1 with U = diag(1 / tau) + the strict upper triangle of G (G[a][c] = v_a^H v_c; the
+    // recurrence T[0:c, c] = -tau_c T[0:c, 0:c] G[0:c, c] says T U = I).  Column c on thread c by
+    // back-substitution of U T[:, c] = e_c: T[c][c] = tau_c, T[i][c] = -tau_i sum_{a=i+1..c}
+    // G[i][a] T[a][c]; the columns are independent, so no barrier separates them.
+    if ((int)threadIdx.x < jb) {
+        const int c = threadIdx.x;
+        Ts[c][c] = s_tau[c];
+        for (int i = c - 1; i >= 0; --i) {
             double sr = 0.0, si = 0.0;
-            for (int b = a; b < c; ++b) {
-                const double2 tv = Ts[a][b], g = Gs[b][c];
-                sr += tv.x * g.x - tv.y * g.y;
-                si += tv.x * g.y + tv.y * g.x;
+            for (int a = i + 1; a <= c; ++a) {
+                const double2 g = Gs[i][a], t = Ts[a][c];
+                sr += g.x * t.x - g.y * t.y;
+                si += g.x * t.y + g.y * t.x;
             }
-            const double2 tc = s_tau[c];
-            Ts[a][c] = double2{-(tc.x * sr - tc.y * si), -(tc.x * si + tc.y * sr)};
-        } else if (a == c) {
-            Ts[c][c] = s_tau[c];
-        } else if (a < PW) {
-            Ts[a][c] = double2{0.0, 0.0};
+            const double2 ti = s_tau[i];
+            Ts[i][c] = double2{-(ti.x * sr - ti.y * si), -(ti.x * si + ti.y * sr)};
         }
-        __syncthreads();
-    }
-    // Y = V T, column c = sum_{a <= c} v_a T[a][c]: broadcast v_a for a = jb-1 .. 0; wave a turns
-    // its own v into y_a = v_a T[a][a] right after broadcasting it.
-    for (int a = jb - 1; a >= 0; --a) {
-        double2* buf = vbuf[a & 1];
-        if (wave == a) {
-            const double2 ta = Ts[a][a];
-#pragma unroll
-            for (int i = 0; i < RPT; ++i) {
-                buf[lane + 64 * i] = x[i];
-                const double2 v = x[i];
-                x[i] = double2{v.x * ta.x - v.y * ta.y, v.x * ta.y + v.y * ta.x};
-            }
-        }
-        __syncthreads();
-        if (wave > a && own) {
-            const double2 ta = Ts[a][wave];
-#pragma unroll
-            for (int i = 0; i < RPT; ++i) {
-                const double2 v = buf[lane + 64 * i];
-                x[i].x += v.x * ta.x - v.y * ta.y;
-                x[i].y += v.x * ta.y + v.y * ta.x;
-            }
-        }
-    }
-    if (own) {
-        double2* ycol = Yb + job.y_off + p0 + (int64_t)(p0 + wave) * m;
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-            const int r = lane + 64 * i;
-            if (r < mp) ycol[r] = x[i];
-        }
+        double2* tout = Tp + (int64_t)blockIdx.x * kQrTSlot;
+        for (int i = 0; i < jb; ++i) tout[i * kQrPanel + c] = i <= c ? Ts[i][c] : double2{0.0, 0.0};
     }
 }
 
-// ---- fused block-reflector application: C <- C - P (R^H C) --------------------------------
+// ---- fused block-reflector application: C <- C - V op(T) (V^H C) ----------------------------
 //
-// One workgroup per 16-column slab of C (mp rows).  W = R^H C_slab (jb x 16, jb <= 16) is
+// One workgroup per 16-column slab of C (mp rows).  W0 = V^H C_slab (jb x 16, jb <= 16) is
 // accumulated by v_mfma_f64_16x16x4f64 with the row range split over the four waves and the
-// partials summed through LDS; then C_slab -= P W, again by MFMA, 16 rows at a time.  The trailing
-// update of the blocked QR is (P, R) = (V, Y) on A; the backward formation of Q is (P, R) = (Y, V).
+// partials summed through LDS; W = op(T) W0 (16 x 16 on every lane's own entries); then
+// C_slab -= V W, again by MFMA, 16 rows at a time.  The trailing update of the blocked QR applies
+// (I - V T V^H)^H (op = ^H, `trans` 1); the backward formation of Q applies I - V T V^H (op = I).
 // MFMA f64 16x16x4 operands: lane l of "a" holds X[l % 16][l / 16], lane l of "b" holds
 // Y[l / 16][l % 16]; register r of lane l of D = X Y holds D[l / 16 + 4 r][l % 16].
 
 typedef double dvec4 __attribute__((ext_vector_type(4)));
 
 struct ReflTask {
-    int64_t c_off, p_off, r_off;   // C slab origin (ldc); P and R origins (leading dimension ldp)
-    int32_t mp, ncols, jb, ldc, ldp, pad;
+    int64_t c_off, p_off, t_off;   // C slab origin (ldc); V origin (leading dimension ldp); T slot
+    int32_t mp, ncols, jb, ldc, ldp, trans;
 };
 
 constexpr int kReflThreads = 256;
 
 __global__ __launch_bounds__(kReflThreads) void apply_reflector_kernel(
-    const ReflTask* __restrict__ tasks, double2* __restrict__ C, const double2* __restrict__ P,
-    const double2* __restrict__ R) {
+    const ReflTask* __restrict__ tasks, double2* __restrict__ C, const double2* __restrict__ Vb,
+    const double2* __restrict__ Tb) {
     __shared__ double2 Wp[4][16][16];
+    __shared__ double2 Ts[16][17];
     const ReflTask tk = tasks[blockIdx.x];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lk = lane >> 4;
     const int mp = tk.mp, nc = tk.ncols, jb = tk.jb;
     double2* c = C + tk.c_off;
-    const double2* p = P + tk.p_off;
-    const double2* r = R + tk.r_off;
+    const double2* p = Vb + tk.p_off;
+    const double2* r = p;
     const int64_t ldc = tk.ldc, ldp = tk.ldp;
-    // W = R^H C: X = R^H (a: conj(R[k][lr])), Y = C (b: C[k][lr])
+    static_assert(kReflThreads == 256 && kQrPanel == 16, "one T entry per thread");
+    {
+        const int a = threadIdx.x >> 4, b = threadIdx.x & 15;
+        Ts[a][b] = (a < jb && b < jb) ? Tb[tk.t_off + a * kQrPanel + b] : double2{0.0, 0.0};
+    }
+    // W0 = V^H C: X = V^H (a: conj(V[k][lr])), Y = C (b: C[k][lr])
     dvec4 wr = {0.0, 0.0, 0.0, 0.0}, wi = {0.0, 0.0, 0.0, 0.0};
     // four k-steps per iteration, loads issued before the products so their latencies overlap
     for (int k0 = 4 * wave; k0 < mp; k0 += 64) {
@@ -854,20 +833,36 @@ __global__ __launch_bounds__(kReflThreads) void apply_reflector_kernel(
 #pragma unroll
     for (int q = 0; q < 4; ++q) Wp[wave][lk + 4 * q][lr] = double2{wr[q], wi[q]};
     __syncthreads();
-    // every wave needs W as the b operand of P W: lane l, chunk kk: W[kk + lk][lr]
-    double2 wb[4];
-#pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-        const int i = 4 * kk + lk;
+    // W0 = the waves' partials summed in a fixed order, then W = op(T) W0 (op(T)[i][a] =
+    // conj(T[a][i]) for the trailing update, T[i][a] for the Q formation), one entry per thread
+    __shared__ double2 W0s[16][17];
+    const int ea = threadIdx.x >> 4, eb = threadIdx.x & 15;
+    {
         double2 s{0.0, 0.0};
 #pragma unroll
         for (int w = 0; w < 4; ++w) {
-            s.x += Wp[w][i][lr].x;
-            s.y += Wp[w][i][lr].y;
+            s.x += Wp[w][ea][eb].x;
+            s.y += Wp[w][ea][eb].y;
         }
-        wb[kk] = s;
+        W0s[ea][eb] = s;
     }
-    // C -= P W, 16 rows per tile: X = P (a: P[row0 + lr][kk + lk]), Y = W
+    __syncthreads();
+    {
+        double sr = 0.0, si = 0.0;
+        for (int a = 0; a < 16; ++a) {
+            const double2 t = tk.trans ? double2{Ts[a][ea].x, -Ts[a][ea].y} : Ts[ea][a];
+            const double2 v = W0s[a][eb];
+            sr += t.x * v.x - t.y * v.y;
+            si += t.x * v.y + t.y * v.x;
+        }
+        Wp[0][ea][eb] = double2{sr, si};
+    }
+    __syncthreads();
+    // every wave needs W as the b operand of V W: lane l, chunk kk: W[4 kk + lk][lr]
+    double2 wb[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) wb[kk] = Wp[0][4 * kk + lk][lr];
+    // C -= V W, 16 rows per tile: X = V (a: V[row0 + lr][kk + lk]), Y = W
     for (int row0 = 16 * wave; row0 < mp; row0 += 64) {
         dvec4 dr = {0.0, 0.0, 0.0, 0.0}, di = {0.0, 0.0, 0.0, 0.0};
         double2 xv[4], cv[4];
@@ -2066,14 +2061,16 @@ void blocked_qr(bug_ctx* ctx, hipStream_t st, const std::vector<MatJob>& jobs, d
     for (int i = 0; i < nmat; ++i) {
         const MatJob& j = jobs[i];
         const int k = std::min(j.m, j.n);
-        pj[i] = QrPanelJob{j.a_off, vt, vt, j.m, j.n, j.lda, 0};
+        pj[i] = QrPanelJob{j.a_off, vt, 0, j.m, j.n, j.lda, 0};
         vt += (int64_t)j.m * k;
         kmax = std::max(kmax, k);
     }
-    double2* ws = static_cast<double2*>(scratch(ctx, st, sizeof(double2) * std::max<int64_t>(2 * vt, 1), 5));
-    double2* Vb = ws;
-    double2* Yb = ws + vt;
     const int npan = (kmax + pw - 1) / pw;
+    // V (m x k per matrix) and the panels' T factors (kQrTSlot per panel and matrix)
+    const int64_t tsz = (int64_t)npan * nmat * kQrTSlot;
+    double2* ws = static_cast<double2*>(scratch(ctx, st, sizeof(double2) * std::max<int64_t>(vt + tsz, 1), 5));
+    double2* Vb = ws;
+    double2* Tb = ws + vt;
     // slab tasks: trailing updates of panel p at [tstart[p], tstart[p+1]), Q formation of panel p
     // at [qstart[p], qstart[p+1])
     std::vector<ReflTask> tasks;
@@ -2094,9 +2091,11 @@ void blocked_qr(bug_ctx* ctx, hipStream_t st, const std::vector<MatJob>& jobs, d
                 const int nc = pass == 0 ? j.n - p0 - jb : k - p0;
                 const int64_t base = pass == 0 ? j.a_off + p0 : j.q_off + p0;
                 const int64_t ld = pass == 0 ? j.lda : j.ldq;
+                const int64_t to = ((int64_t)p * nmat + i) * kQrTSlot;
                 for (int s0 = 0; s0 < nc; s0 += 16)
-                    tasks.push_back(ReflTask{base + (int64_t)(c0 + s0) * ld, vo, vo, mp,
-                                             std::min(16, nc - s0), jb, (int32_t)ld, j.m, 0});
+                    tasks.push_back(ReflTask{base + (int64_t)(c0 + s0) * ld, vo, to, mp,
+                                             std::min(16, nc - s0), jb, (int32_t)ld, j.m,
+                                             pass == 0 ? 1 : 0});
                 if (nc > 0) flops += 16.0 * mp * nc * (double)jb;
             }
         }
@@ -2120,23 +2119,24 @@ void blocked_qr(bug_ctx* ctx, hipStream_t st, const std::vector<MatJob>& jobs, d
         int mp = 0;
         for (const MatJob& j : jobs)
             if (std::min(j.m, j.n) > p0) mp = std::max(mp, j.m - p0);
+        double2* Tp = Tb + (int64_t)p * nmat * kQrTSlot;
         if (pw == 8)
-            hipLaunchKernelGGL((qr_panel_kernel<16, 8>), dim3(nmat), dim3(512), 0, st, dp, p0, A, Vb, Yb);
+            hipLaunchKernelGGL((qr_panel_kernel<16, 8>), dim3(nmat), dim3(512), 0, st, dp, p0, A, Vb, Tp);
         else if (mp <= 256)
-            hipLaunchKernelGGL((qr_panel_kernel<4, 16>), dim3(nmat), dim3(1024), 0, st, dp, p0, A, Vb, Yb);
+            hipLaunchKernelGGL((qr_panel_kernel<4, 16>), dim3(nmat), dim3(1024), 0, st, dp, p0, A, Vb, Tp);
         else
-            hipLaunchKernelGGL((qr_panel_kernel<8, 16>), dim3(nmat), dim3(1024), 0, st, dp, p0, A, Vb, Yb);
+            hipLaunchKernelGGL((qr_panel_kernel<8, 16>), dim3(nmat), dim3(1024), 0, st, dp, p0, A, Vb, Tp);
         HIP_CHECK(hipGetLastError());
         const size_t nt = tstart[p + 1] - tstart[p];
         if (nt > 0)
             hipLaunchKernelGGL(apply_reflector_kernel, dim3((unsigned)nt), dim3(kReflThreads), 0, st,
-                               dt + tstart[p], A, Vb, Yb);
+                               dt + tstart[p], A, Vb, Tb);
     }
     for (int p = npan - 1; p >= 0; --p) {
         const size_t nq = qstart[p + 1] - qstart[p];
         if (nq > 0)
             hipLaunchKernelGGL(apply_reflector_kernel, dim3((unsigned)nq), dim3(kReflThreads), 0, st,
-                               dt + qstart[p], Q, Yb, Vb);
+                               dt + qstart[p], Q, Vb, Tb);
     }
     hipLaunchKernelGGL(qr_finish_kernel, dim3(nmat, std::min(32, std::max(1, (kmax + 15) / 16))),
                        dim3(1024), 0, st, dj, A, Q, R);
