@@ -785,12 +785,13 @@ struct ReflTask {
     int32_t mp, ncols, jb, ldc, ldp, trans;
 };
 
-constexpr int kReflThreads = 256;
+constexpr int kReflThreads = 512;
+constexpr int kReflWaves = kReflThreads / 64;   // a multiple of 4: groups of four waves share 64 rows
 
 __global__ __launch_bounds__(kReflThreads) void apply_reflector_kernel(
     const ReflTask* __restrict__ tasks, double2* __restrict__ C, const double2* __restrict__ Vb,
     const double2* __restrict__ Tb) {
-    __shared__ double2 Wp[4][16][16];
+    __shared__ double2 Wp[kReflWaves][16][16];
     __shared__ double2 Ts[16][17];
     const ReflTask tk = tasks[blockIdx.x];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -800,15 +801,15 @@ __global__ __launch_bounds__(kReflThreads) void apply_reflector_kernel(
     const double2* p = Vb + tk.p_off;
     const double2* r = p;
     const int64_t ldc = tk.ldc, ldp = tk.ldp;
-    static_assert(kReflThreads == 256 && kQrPanel == 16, "one T entry per thread");
-    {
-        const int a = threadIdx.x >> 4, b = threadIdx.x & 15;
-        Ts[a][b] = (a < jb && b < jb) ? Tb[tk.t_off + a * kQrPanel + b] : double2{0.0, 0.0};
-    }
+    static_assert(kReflThreads >= 256 && kReflWaves % 4 == 0 && kQrPanel == 16, "one T entry per thread");
+    const bool entry = threadIdx.x < 256;          // the threads that own one 16 x 16 entry
+    const int ea = (threadIdx.x >> 4) & 15, eb = threadIdx.x & 15;
+    if (entry) Ts[ea][eb] = (ea < jb && eb < jb) ? Tb[tk.t_off + ea * kQrPanel + eb] : double2{0.0, 0.0};
     // W0 = V^H C: X = V^H (a: conj(V[k][lr])), Y = C (b: C[k][lr])
     dvec4 wr = {0.0, 0.0, 0.0, 0.0}, wi = {0.0, 0.0, 0.0, 0.0};
-    // four k-steps per iteration, loads issued before the products so their latencies overlap
-    for (int k0 = 4 * wave; k0 < mp; k0 += 64) {
+    // four k-steps per iteration, loads issued before the products so their latencies overlap;
+    // each group of four waves covers 64 consecutive rows
+    for (int k0 = 64 * (wave >> 2) + 4 * (wave & 3); k0 < mp; k0 += 16 * kReflWaves) {
         double2 xv[4], yv[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -836,18 +837,17 @@ __global__ __launch_bounds__(kReflThreads) void apply_reflector_kernel(
     // W0 = the waves' partials summed in a fixed order, then W = op(T) W0 (op(T)[i][a] =
     // conj(T[a][i]) for the trailing update, T[i][a] for the Q formation), one entry per thread
     __shared__ double2 W0s[16][17];
-    const int ea = threadIdx.x >> 4, eb = threadIdx.x & 15;
-    {
+    if (entry) {
         double2 s{0.0, 0.0};
 #pragma unroll
-        for (int w = 0; w < 4; ++w) {
+        for (int w = 0; w < kReflWaves; ++w) {
             s.x += Wp[w][ea][eb].x;
             s.y += Wp[w][ea][eb].y;
         }
         W0s[ea][eb] = s;
     }
     __syncthreads();
-    {
+    if (entry) {
         double sr = 0.0, si = 0.0;
         for (int a = 0; a < 16; ++a) {
             const double2 t = tk.trans ? double2{Ts[a][ea].x, -Ts[a][ea].y} : Ts[ea][a];
@@ -863,7 +863,7 @@ __global__ __launch_bounds__(kReflThreads) void apply_reflector_kernel(
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk) wb[kk] = Wp[0][4 * kk + lk][lr];
     // C -= V W, 16 rows per tile: X = V (a: V[row0 + lr][kk + lk]), Y = W
-    for (int row0 = 16 * wave; row0 < mp; row0 += 64) {
+    for (int row0 = 16 * wave; row0 < mp; row0 += 16 * kReflWaves) {
         dvec4 dr = {0.0, 0.0, 0.0, 0.0}, di = {0.0, 0.0, 0.0, 0.0};
         double2 xv[4], cv[4];
 #pragma unroll
@@ -2519,10 +2519,14 @@ void gram_block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& 
         constexpr int cc = 32;
         double2* gbuf = static_cast<double2*>(scratch(
             ctx, st, sizeof(double2) * (size_t)cc * cc * maxch * tasks.size(), 6));
-        int* hp = reinterpret_cast<int*>(pinned(ctx, sizeof(int) * nmat));
+        int* hp = reinterpret_cast<int*>(pinned(ctx, sizeof(int) * (size_t)nmat * nrows));
+        for (hipEvent_t& e : ctx->sync_ev)
+            if (!e) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         const dim3 grid((unsigned)tasks.size(), (unsigned)maxch);
         const int inner = std::max(1, ctx->gbj_inner);
         const double eps = 2.220446049250313e-16;
+        // The convergence read of sweep s overlaps sweep s + 1, launched before it (as in
+        // block_jacobi_t): a sweep after one that rotated nothing returns at once in every launch.
         for (int s = 0; s < max_sweeps; ++s) {
             for (int r = 0; r < rounds; ++r) {
                 const int* fp = flags + (size_t)s * nmat;
@@ -2535,13 +2539,17 @@ void gram_block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& 
             }
             HIP_CHECK(hipGetLastError());
             ++used;
-            HIP_CHECK(hipMemcpyAsync(hp, flags + (size_t)(s + 1) * nmat, sizeof(int) * nmat,
-                                     hipMemcpyDeviceToHost, st));
-            HIP_CHECK(hipStreamSynchronize(st));
+            HIP_CHECK(hipMemcpyAsync(hp + (size_t)s * nmat, flags + (size_t)(s + 1) * nmat,
+                                     sizeof(int) * nmat, hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipEventRecord(ctx->sync_ev[s & 1], st));
+            if (s == 0) continue;
+            HIP_CHECK(hipEventSynchronize(ctx->sync_ev[(s - 1) & 1]));
             bool any = false;
-            for (int i = 0; i < nmat; ++i) any |= hp[i] != 0;
+            for (int i = 0; i < nmat; ++i) any |= hp[(size_t)(s - 1) * nmat + i] != 0;
             if (!any) break;
         }
+        // the last sweep's read (or the speculative sweep) must land before the flags are reused
+        HIP_CHECK(hipEventSynchronize(ctx->sync_ev[(used - 1) & 1]));
     }
     if (sweeps_dev) {
         std::vector<int> all((size_t)nmat * nrows), host(nmat);
