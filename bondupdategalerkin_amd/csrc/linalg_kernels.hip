@@ -1491,14 +1491,23 @@ __global__ __launch_bounds__(NT) void gbj_gram_kernel(const BjMat* __restrict__ 
     const int rows = min(kGbjRows, n - r0);
     const double2* x = X + mt.x_off;
     const int tid = threadIdx.x;
-    for (int e = tid; e < CC * kGbjRows; e += NT) {
+    // every load in flight before the first store to LDS
+    constexpr int kS = (CC * kGbjRows + NT - 1) / NT;
+    double2 val[kS];
+#pragma unroll
+    for (int t = 0; t < kS; ++t) {
+        const int e = tid + t * NT;
         const int j = e / kGbjRows, i = e % kGbjRows;
-        double2 val = double2{0.0, 0.0};
-        if (j < c && i < rows) {
+        val[t] = double2{0.0, 0.0};
+        if (e < CC * kGbjRows && j < c && i < rows) {
             const int col = j < cp ? bp * w + j : bq * w + (j - cp);
-            val = x[(int64_t)col * n + r0 + i];
+            val[t] = x[(int64_t)col * n + r0 + i];
         }
-        xs[j][i] = val;
+    }
+#pragma unroll
+    for (int t = 0; t < kS; ++t) {
+        const int e = tid + t * NT;
+        if (e < CC * kGbjRows) xs[e / kGbjRows][e % kGbjRows] = val[t];
     }
     __syncthreads();
     // thread (bi, bj) owns the 2 x 2 block G[2bi:2bi+2, 2bj:2bj+2]
@@ -1552,18 +1561,48 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
     auto gcol = [&](int j) { return j < cp ? bp * w + j : bq * w + (j - cp); };
     const double2* gp = gbuf + (int64_t)blockIdx.x * maxch * (CC * CC);
     // G and Z over the whole CC x CC: zero outside the pair's c columns, where the update's
-    // products below read Z
-    for (int e = tid; e < CC * CC; e += NT) {
-        const int a = e / CC, b = e % CC;
-        double sr = 0.0, si = 0.0;
-        if (a < c && b < c)
-            for (int k = 0; k < nch; ++k) {
-                const double2 v = gp[(int64_t)k * (CC * CC) + a * CC + b];
-                sr += v.x;
-                si += v.y;
-            }
-        G[a][b] = double2{sr, si};
-        Z[a][b] = double2{a == b && a < c ? 1.0 : 0.0, 0.0};
+    // products below read Z.  The chunks' partials are summed in chunk order, eight chunks' loads
+    // in flight at a time (a loop of single loads would wait out one memory latency per chunk).
+    {
+        constexpr int kE = (CC * CC + NT - 1) / NT;
+        constexpr int kBatch = 8;
+        double sr[kE], si[kE];
+        int ea[kE], eb[kE];
+        bool live[kE];
+#pragma unroll
+        for (int u = 0; u < kE; ++u) {
+            const int e = tid + u * NT;
+            ea[u] = e / CC;
+            eb[u] = e % CC;
+            live[u] = e < CC * CC && ea[u] < c && eb[u] < c;
+            sr[u] = 0.0;
+            si[u] = 0.0;
+        }
+        for (int k0 = 0; k0 < nch; k0 += kBatch) {
+            double2 v[kBatch][kE];
+#pragma unroll
+            for (int kk = 0; kk < kBatch; ++kk)
+#pragma unroll
+                for (int u = 0; u < kE; ++u) {
+                    v[kk][u] = double2{0.0, 0.0};
+                    if (live[u] && k0 + kk < nch)
+                        v[kk][u] = gp[(int64_t)(k0 + kk) * (CC * CC) + ea[u] * CC + eb[u]];
+                }
+#pragma unroll
+            for (int kk = 0; kk < kBatch; ++kk)
+#pragma unroll
+                for (int u = 0; u < kE; ++u) {
+                    sr[u] += v[kk][u].x;
+                    si[u] += v[kk][u].y;
+                }
+        }
+#pragma unroll
+        for (int u = 0; u < kE; ++u) {
+            if (tid + u * NT >= CC * CC) continue;
+            const int a = ea[u], b = eb[u];
+            G[a][b] = double2{sr[u], si[u]};
+            Z[a][b] = double2{a == b && a < c ? 1.0 : 0.0, 0.0};
+        }
     }
     // The update at the end rewrites this chunk's rows of the pair's columns of X and V; no other
     // workgroup writes them in this round, so they are loaded here, after the partial Grams (whose
