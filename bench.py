@@ -234,9 +234,10 @@ def main():
     peak = PEAK[ckind]
     # HBM bytes per apply, from the two PMC passes (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors
     # from a plain copy of known size) recorded in profiles/heff_apply_L128_D1024_mixed.md; only
-    # for the shape and form they were measured on
-    traffic = (126611 * 1024 if (ckind == "fp32" and L == 128 and D == 1024 and args.delta == 0.5
-                                 and args.autotune == "table" and args.tail == 0) else None)
+    # for the shape and form they were measured on (the default state; 126611 KiB on the flat one)
+    traffic = None
+    if ckind == "fp32" and L == 128 and D == 1024 and args.delta == 0.5 and args.autotune == "table":
+        traffic = {1e-9: 115269 * 1024, 0.0: 126611 * 1024}.get(args.tail)
     OUT["roofline"] = {
         "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
         "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
