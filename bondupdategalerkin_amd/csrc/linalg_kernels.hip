@@ -1510,23 +1510,27 @@ __global__ __launch_bounds__(NT) void gbj_gram_kernel(const BjMat* __restrict__ 
         if (e < CC * kGbjRows) xs[e / kGbjRows][e % kGbjRows] = val[t];
     }
     __syncthreads();
-    // thread (bi, bj) owns the 2 x 2 block G[2bi:2bi+2, 2bj:2bj+2]
-    const int bi = tid / (CC / 2), bj = tid % (CC / 2);
-    double2 g00{0.0, 0.0}, g01{0.0, 0.0}, g10{0.0, 0.0}, g11{0.0, 0.0};
-    for (int i = 0; i < rows; ++i) {
-        const double2 a0 = xs[2 * bi][i], a1 = xs[2 * bi + 1][i];
-        const double2 b0 = xs[2 * bj][i], b1 = xs[2 * bj + 1][i];
-        // conj(a) b
-        g00.x += a0.x * b0.x + a0.y * b0.y;  g00.y += a0.x * b0.y - a0.y * b0.x;
-        g01.x += a0.x * b1.x + a0.y * b1.y;  g01.y += a0.x * b1.y - a0.y * b1.x;
-        g10.x += a1.x * b0.x + a1.y * b0.y;  g10.y += a1.x * b0.y - a1.y * b0.x;
-        g11.x += a1.x * b1.x + a1.y * b1.y;  g11.y += a1.x * b1.y - a1.y * b1.x;
+    // G = X_k^H X_k on FP64 MFMA 16x16x4, one 16 x 16 tile of G per wave, k running over the
+    // chunk's rows (zero past `rows` and outside the pair's columns, as staged):
+    // re = Xr^T Xr + Xi^T Xi, im = Xr^T Xi - Xi^T Xr.  Operand a lane l holds
+    // X^H[ta + l % 16][k] = conj(X[k][ta + l % 16]), operand b holds X[k][tb + l % 16] with
+    // k = k0 + l / 16; register r of lane l holds G[ta + l / 16 + 4 r][tb + l % 16].
+    static_assert(CC == 32 && NT == 256 && kGbjRows % 4 == 0, "four waves, one 16 x 16 tile each");
+    typedef double dv4 __attribute__((ext_vector_type(4)));
+    const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+    const int ta = (wave >> 1) * 16, tb = (wave & 1) * 16;
+    dv4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k0 = 0; k0 < kGbjRows; k0 += 4) {
+        const double2 a = xs[ta + li][k0 + lk], b = xs[tb + li][k0 + lk];
+        accr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, accr, 0, 0, 0);
+        accr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, accr, 0, 0, 0);
+        acci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, acci, 0, 0, 0);
+        acci = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.x, acci, 0, 0, 0);
     }
     double2* g = gbuf + ((int64_t)blockIdx.x * maxch + blockIdx.y) * (CC * CC);
-    g[(2 * bi) * CC + 2 * bj] = g00;
-    g[(2 * bi) * CC + 2 * bj + 1] = g01;
-    g[(2 * bi + 1) * CC + 2 * bj] = g10;
-    g[(2 * bi + 1) * CC + 2 * bj + 1] = g11;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) g[(ta + lk + 4 * r) * CC + tb + li] = double2{accr[r], acci[r]};
 }
 
 template <int CC, int NT>
