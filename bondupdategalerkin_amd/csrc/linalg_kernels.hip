@@ -1626,7 +1626,11 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
             if (j < c && i < rows) pre[mat][t] = m[(int64_t)gcol(j) * n + r0 + i];
         }
     }
-    if (tid == 0) s_rot = 0;
+    __shared__ int s_need;
+    if (tid == 0) {
+        s_rot = 0;
+        s_need = 0;
+    }
     __syncthreads();
     const double tol2 = eps * eps * (double)n;
     const bool cross_only = round > 0 && cp > 0 && cq > 0;
@@ -1639,13 +1643,10 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
         if (cross_only && !(a < cp && b >= cp)) continue;
         const double gr = G[a][b].x, gi = G[a][b].y;
         const double ag2 = gr * gr + gi * gi;
-        if (ag2 > tol2 * (G[a][a].x * G[b][b].x) && ag2 != 0.0) s_rot = 1;
+        if (ag2 > tol2 * (G[a][a].x * G[b][b].x) && ag2 != 0.0) s_need = 1;
     }
     __syncthreads();
-    const bool needed = s_rot != 0;
-    __syncthreads();
-    if (tid == 0) s_rot = 0;
-    __syncthreads();
+    const bool needed = s_need != 0;
     const int big = cp >= cq ? cp : cq, small = cp >= cq ? cq : cp;
     const int big0 = cp >= cq ? 0 : cp, small0 = cp >= cq ? cp : 0;
     const int C = c + (c & 1);
