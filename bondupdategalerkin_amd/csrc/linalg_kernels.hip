@@ -1408,7 +1408,8 @@ __global__ __launch_bounds__(kBjThreads) void bj_round_kernel(const BjMat* __res
 // The rotation test and the angle formula are those of the staged kernel (|g_ab|^2 > eps^2 n g_aa g_bb);
 // the Gram matrix is recomputed from the columns every round, so its updates within the round
 // never accumulate past one round, and convergence is judged on fresh inner products.
-constexpr int kGbjCols = 32;   // widest block pair (2w)
+constexpr int kGbjCols = 32;   // widest block pair (2w) of the default kernels
+constexpr int kGbjColsWide = 64;   // of the wide ones (1024 threads)
 constexpr int kGbjRotThreads = 512;   // rotate kernel: every pair of an inner round at once
 
 // 1 / sqrt(x) and 1 / x for x in [1e-280, 1e280]: the hardware estimates (about 23 correct bits)
@@ -1460,10 +1461,13 @@ __device__ __forceinline__ void jacobi_rotation(double al, double be, double gr,
     ei = gi * inv;
 }
 constexpr int kGbjRows = 32;   // rows per chunk (64 measured slower: twice the update work per workgroup)
-// LDS of one instance: the gram kernel stages CC columns of a chunk, the rotate kernel holds G,
-// Z and the staged V rows (CC x (CC + 1) each; the X rows are staged in G's storage)
+// LDS of one instance: the gram kernel stages CC columns of a chunk, the rotate kernel holds G
+// and Z (CC x (CC + 1) each) and stages the X and V rows in G's storage (and, for CC < 64, in a
+// third such array)
 constexpr size_t gbj_gram_lds(int cc) { return sizeof(double2) * (size_t)cc * (kGbjRows + 1); }
-constexpr size_t gbj_rot_lds(int cc) { return 3 * sizeof(double2) * (size_t)cc * (cc + 1); }
+constexpr size_t gbj_rot_lds(int cc) {
+    return (cc >= 2 * kGbjRows ? 2 : 3) * sizeof(double2) * (size_t)cc * (cc + 1);
+}
 
 // CC columns per block pair on NT = (CC / 2)^2 threads
 template <int CC, int NT>
@@ -1515,10 +1519,11 @@ __global__ __launch_bounds__(NT) void gbj_gram_kernel(const BjMat* __restrict__ 
     // re = Xr^T Xr + Xi^T Xi, im = Xr^T Xi - Xi^T Xr.  Operand a lane l holds
     // X^H[ta + l % 16][k] = conj(X[k][ta + l % 16]), operand b holds X[k][tb + l % 16] with
     // k = k0 + l / 16; register r of lane l holds G[ta + l / 16 + 4 r][tb + l % 16].
-    static_assert(CC == 32 && NT == 256 && kGbjRows % 4 == 0, "four waves, one 16 x 16 tile each");
+    static_assert(CC % 16 == 0 && NT == 64 * (CC / 16) * (CC / 16) && kGbjRows % 4 == 0,
+                  "one 16 x 16 tile of G per wave");
     typedef double dv4 __attribute__((ext_vector_type(4)));
     const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
-    const int ta = (wave >> 1) * 16, tb = (wave & 1) * 16;
+    const int ta = (wave / (CC / 16)) * 16, tb = (wave % (CC / 16)) * 16;
     dv4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int k0 = 0; k0 < kGbjRows; k0 += 4) {
@@ -1542,7 +1547,9 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
                                                         int round,
                                                         const int* __restrict__ flags_prev,
                                                         int* __restrict__ flags_cur,
-                                                        double eps, int inner, int maxch) {
+                                                        double eps, int inner, int maxch,
+                                                        const double* __restrict__ fro2,
+                                                        double floor_rel2) {
     extern __shared__ __align__(16) unsigned char gbj_rot_lds_raw[];
     double2 (*G)[CC + 1] = reinterpret_cast<double2 (*)[CC + 1]>(gbj_rot_lds_raw);
     double2 (*Z)[CC + 1] = G + CC;
@@ -1565,11 +1572,12 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
     auto gcol = [&](int j) { return j < cp ? bp * w + j : bq * w + (j - cp); };
     const double2* gp = gbuf + (int64_t)blockIdx.x * maxch * (CC * CC);
     // G and Z over the whole CC x CC: zero outside the pair's c columns, where the update's
-    // products below read Z.  The chunks' partials are summed in chunk order, eight chunks' loads
-    // in flight at a time (a loop of single loads would wait out one memory latency per chunk).
+    // products below read Z.  The chunks' partials are summed in chunk order, several chunks'
+    // loads in flight at a time (a loop of single loads would wait out one memory latency per
+    // chunk).
     {
         constexpr int kE = (CC * CC + NT - 1) / NT;
-        constexpr int kBatch = 8;
+        constexpr int kBatch = kE <= 2 ? 8 : 16 / kE;       // loads in flight per thread: 16
         double sr[kE], si[kE];
         int ea[kE], eb[kE];
         bool live[kE];
@@ -1612,7 +1620,7 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
     // workgroup writes them in this round, so they are loaded here, after the partial Grams (whose
     // loads the sum above has waited for), and their latency runs under the inner sweep, which
     // touches only LDS, instead of after it.  Zero outside the c columns and the chunk's rows.
-    static_assert(CC == 32 && kGbjRows == 32 && NT == 512, "the update's tiling: 8 waves, 16 x 16 tiles");
+    static_assert((CC * kGbjRows) % NT == 0, "whole chunk rows per thread");
     constexpr int kPre = CC * kGbjRows / NT;
     double2 pre[2][kPre];
 #pragma unroll
@@ -1633,6 +1641,13 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
     }
     __syncthreads();
     const double tol2 = eps * eps * (double)n;
+    // A pair whose two columns together hold less than floor_rel2 of the matrix's squared
+    // Frobenius norm is left alone, as in the staged kernel: rotations among such columns only
+    // reshape directions whose singular values lie below floor_rel2^(1/2) ||X||_F, which no
+    // selection keeps, and a column can only grow by rotating against a larger partner, which
+    // is never skipped.  (Columns of rounding noise would otherwise be orthogonalized to full
+    // relative accuracy among themselves, sweep after sweep.)
+    const double floor2 = floor_rel2 * fro2[tk.x];
     const bool cross_only = round > 0 && cp > 0 && cq > 0;
     // Every pair the inner sweep visits, tested on G as summed: if none passes the rotation
     // test, the first inner round rotates nothing, G stays as it is, and so do all later ones;
@@ -1643,7 +1658,8 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
         if (cross_only && !(a < cp && b >= cp)) continue;
         const double gr = G[a][b].x, gi = G[a][b].y;
         const double ag2 = gr * gr + gi * gi;
-        if (ag2 > tol2 * (G[a][a].x * G[b][b].x) && ag2 != 0.0) s_need = 1;
+        const double al = G[a][a].x, be = G[b][b].x;
+        if (al + be >= floor2 && ag2 > tol2 * (al * be) && ag2 != 0.0) s_need = 1;
     }
     __syncthreads();
     const bool needed = s_need != 0;
@@ -1652,15 +1668,18 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
     const int C = c + (c & 1);
     const int np = cross_only ? small : C / 2;
     const int nrounds = needed ? (cross_only ? big : C - 1) : 0;
-    // Thread t works on pair g = t / CC, entry k = t % CC of the pair's two columns (then of its
-    // two rows), NT / CC pairs at a time.  Every thread of a pair computes the pair's rotation
-    // itself from G as the round found it: the pair's CC <= 64 threads sit in one wavefront and read G[a][a], G[b][b], G[a][b] before any of them writes,
+    // Thread t works on pair g = t / 32, entries k = t % 32 + 32 e (e < CC / 32) of the pair's two
+    // columns (then of its two rows): every pair of an inner round at once.  Every thread of a
+    // pair computes the pair's rotation itself from G as the round found it: the pair's 32
+    // threads sit in one wavefront and read G[a][a], G[b][b], G[a][b] before any of them writes,
     // and no other pair writes columns a or b, so a round needs only the barrier between its
     // column and row updates and the one after them.
-    static_assert(CC <= 64, "a pair's threads must share one wavefront");
-    constexpr int kPairsAtOnce = NT / CC;
-    constexpr int kSlots = (CC / 2 + kPairsAtOnce - 1) / kPairsAtOnce;
-    const int gl = tid / CC, k = tid % CC;
+    constexpr int kTpp = 32;
+    constexpr int kPairsAtOnce = NT / kTpp;
+    static_assert(kPairsAtOnce == CC / 2 && CC % kTpp == 0, "one pair per 32 threads");
+    constexpr int kSlots = 1;
+    constexpr int kEnt = CC / kTpp;
+    const int gl = tid / kTpp, kb = tid % kTpp;
     bool any = false;
     for (int pass = 0; pass < inner; ++pass) {
         for (int r = 0; r < nrounds; ++r) {
@@ -1685,7 +1704,7 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
                 const double al = G[a][a].x, be = G[b][b].x;
                 const double gr = G[a][b].x, gi = G[a][b].y;
                 const double ag2 = gr * gr + gi * gi;
-                if (!(ag2 > tol2 * (al * be) && ag2 != 0.0)) continue;
+                if (!(al + be >= floor2 && ag2 > tol2 * (al * be) && ag2 != 0.0)) continue;
                 jacobi_rotation(al, be, gr, gi, ag2, cs[sl], sn[sl], er[sl], ei[sl]);
                 ra[sl] = a;
                 rb[sl] = b;
@@ -1694,8 +1713,10 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
             // per pair), and the compiler keeps them in order
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 #pragma unroll
-            for (int sl = 0; sl < kSlots; ++sl) {
-                const int a = ra[sl];
+            for (int sl = 0; sl < kSlots; ++sl)
+#pragma unroll
+            for (int en = 0; en < kEnt; ++en) {
+                const int a = ra[sl], k = kb + en * kTpp;
                 if (a < 0 || k >= c) continue;
                 const int b = rb[sl];
                 any = true;
@@ -1715,8 +1736,10 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
             __syncthreads();
             // rows: M[a, :] <- c M[a, :] - s e M[b, :];  M[b, :] <- s conj(e) M[a, :] + c M[b, :]
 #pragma unroll
-            for (int sl = 0; sl < kSlots; ++sl) {
-                const int a = ra[sl];
+            for (int sl = 0; sl < kSlots; ++sl)
+#pragma unroll
+            for (int en = 0; en < kEnt; ++en) {
+                const int a = ra[sl], k = kb + en * kTpp;
                 if (a < 0 || k >= c) continue;
                 const int b = rb[sl];
                 const double2 p = G[a][k], q = G[b][k];
@@ -1731,13 +1754,14 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
     if (any) s_rot = 1;
     __syncthreads();
     if (!s_rot) return;                       // uniform: read after the last barrier
-    // M_chunk <- M_chunk Z for M = X and V (32 x 32 complex each) on FP64 MFMA 16x16x4: the X rows
-    // staged in G's storage (G is not read past the barrier above), the V rows in their own;
-    // wave w forms the 16 x 16 output tile (w / 2 % 2, w % 2) of X (w < 4) or of V.  Fed as
+    // M_chunk <- M_chunk Z for M = X and V (kGbjRows x CC complex each) on FP64 MFMA 16x16x4: the
+    // rows staged in G's storage (G is not read past the barrier above; for CC < 64 the V rows in
+    // a third array); wave w forms one 16 x 16 output tile of X (the first half of the waves) or
+    // of V.  Fed as
     // a = Z^T and b = M^T, the product is (M Z)^T: lane l holds output row l % 16, so 16 lanes
     // store 16 consecutive complex values of one column.
     double2 (*xsx)[CC + 1] = G;
-    double2 (*xsv)[CC + 1] = G + 2 * CC;
+    double2 (*xsv)[CC + 1] = CC >= 2 * kGbjRows ? G + kGbjRows : G + 2 * CC;
 #pragma unroll
     for (int t = 0; t < kPre; ++t) {
         const int e = tid + t * NT;
@@ -1749,7 +1773,10 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
     {
         typedef double dv4 __attribute__((ext_vector_type(4)));
         const int wave = tid >> 6, lane = tid & 63;
-        const int mat = wave >> 2, tr = ((wave >> 1) & 1) * 16, tc = (wave & 1) * 16;
+        constexpr int kTpm = (kGbjRows / 16) * (CC / 16);       // output tiles per matrix
+        static_assert(NT / 64 == 2 * kTpm, "one 16 x 16 output tile per wave");
+        const int mat = wave / kTpm, tile = wave % kTpm;
+        const int tr = (tile / (CC / 16)) * 16, tc = (tile % (CC / 16)) * 16;
         const int li = lane & 15, lk = lane >> 4;
         double2 (*xs)[CC + 1] = mat == 0 ? xsx : xsv;
         dv4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
@@ -2527,7 +2554,10 @@ void mixed_block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>&
 void gram_block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& list, double2* X,
                        double2* V, int max_sweeps, int* sweeps_dev) {
     const int nmat = (int)list.size();
-    const int c = std::max(2, std::min(kGbjCols, ctx->gbj_cols) & ~1);
+    // block pairs of up to 32 columns on 512-thread rotate workgroups, or of up to 64 on 1024
+    // threads when the context asks for wider pairs (bug_set_gram_jacobi_params)
+    const bool wide = ctx->gbj_cols > kGbjCols;
+    const int c = std::max(2, std::min(wide ? kGbjColsWide : kGbjCols, ctx->gbj_cols) & ~1);
     const int w = c / 2;
     std::vector<BjMat> mats(nmat);
     std::vector<int2> tasks;
@@ -2546,7 +2576,10 @@ void gram_block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& 
         }
     }
     const int nrows = max_sweeps + 1;
-    int* flags = static_cast<int*>(scratch(ctx, st, sizeof(int) * (size_t)nmat * nrows, 3));
+    const size_t flag_bytes = (sizeof(int) * (size_t)nmat * nrows + 255) & ~size_t(255);
+    char* fbase = static_cast<char*>(scratch(ctx, st, flag_bytes + sizeof(double) * (size_t)nmat, 3));
+    int* flags = reinterpret_cast<int*>(fbase);
+    double* fro2 = reinterpret_cast<double*>(fbase + flag_bytes);
     std::vector<unsigned char> blob(sizeof(BjMat) * nmat + sizeof(int2) * tasks.size());
     std::memcpy(blob.data(), mats.data(), sizeof(BjMat) * nmat);
     if (!tasks.empty()) std::memcpy(blob.data() + sizeof(BjMat) * nmat, tasks.data(), sizeof(int2) * tasks.size());
@@ -2555,14 +2588,18 @@ void gram_block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& 
     const int2* dt = reinterpret_cast<const int2*>(dblob + sizeof(BjMat) * nmat);
     hipLaunchKernelGGL(bj_setup_kernel<double2>, dim3(nmat, 16), dim3(256), 0, st, dm, nmat, V, flags, nrows);
     HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(bj_norm_kernel<double>, dim3(nmat), dim3(256), 0, st, dm, X, fro2);
+    HIP_CHECK(hipGetLastError());
+    const double floor_rel2 = ctx->svd_floor * ctx->svd_floor;
     int used = 0;
     if (!tasks.empty()) {
-        // block pairs of up to 32 columns on 256 threads (64 columns on 1024 threads halves the
-        // rounds but was measured slower: 8.3 against 5.8 ms on the centre cut of the L = 128,
-        // D = 1024 state, the inner rotations on sixteen waves cost more than the rounds saved)
-        constexpr int cc = 32;
+        const int cc = wide ? kGbjColsWide : kGbjCols;
         double2* gbuf = static_cast<double2*>(scratch(
             ctx, st, sizeof(double2) * (size_t)cc * cc * maxch * tasks.size(), 6));
+        if (wide)
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(gbj_rotate_kernel<kGbjColsWide, 1024>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)gbj_rot_lds(kGbjColsWide)));
         int* hp = reinterpret_cast<int*>(pinned(ctx, sizeof(int) * (size_t)nmat * nrows));
         for (hipEvent_t& e : ctx->sync_ev)
             if (!e) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -2575,11 +2612,20 @@ void gram_block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& 
             for (int r = 0; r < rounds; ++r) {
                 const int* fp = flags + (size_t)s * nmat;
                 int* fc = flags + (size_t)(s + 1) * nmat;
-                hipLaunchKernelGGL((gbj_gram_kernel<cc, 256>), grid, dim3(256), gbj_gram_lds(cc), st,
-                                   dm, dt, X, gbuf, r, fp, maxch);
-                hipLaunchKernelGGL((gbj_rotate_kernel<cc, kGbjRotThreads>), grid, dim3(kGbjRotThreads),
-                                   gbj_rot_lds(cc), st,
-                                   dm, dt, X, V, gbuf, r, fp, fc, eps, inner, maxch);
+                if (wide) {
+                    constexpr int wc = kGbjColsWide;
+                    hipLaunchKernelGGL((gbj_gram_kernel<wc, 1024>), grid, dim3(1024), gbj_gram_lds(wc), st,
+                                       dm, dt, X, gbuf, r, fp, maxch);
+                    hipLaunchKernelGGL((gbj_rotate_kernel<wc, 1024>), grid, dim3(1024), gbj_rot_lds(wc), st,
+                                       dm, dt, X, V, gbuf, r, fp, fc, eps, inner, maxch, fro2, floor_rel2);
+                } else {
+                    constexpr int nc = kGbjCols;
+                    hipLaunchKernelGGL((gbj_gram_kernel<nc, 256>), grid, dim3(256), gbj_gram_lds(nc), st,
+                                       dm, dt, X, gbuf, r, fp, maxch);
+                    hipLaunchKernelGGL((gbj_rotate_kernel<nc, kGbjRotThreads>), grid, dim3(kGbjRotThreads),
+                                       gbj_rot_lds(nc), st,
+                                       dm, dt, X, V, gbuf, r, fp, fc, eps, inner, maxch, fro2, floor_rel2);
+                }
             }
             HIP_CHECK(hipGetLastError());
             ++used;
@@ -2909,8 +2955,8 @@ int bug_set_jacobi_params(bug_ctx* ctx, int32_t cols, int32_t inner_sweeps) {
 
 int bug_set_gram_jacobi_params(bug_ctx* ctx, int32_t cols, int32_t inner_sweeps, int32_t min_n) {
     return guarded(ctx, [&] {
-        if (cols < 2 || cols > kGbjCols || inner_sweeps < 1 || inner_sweeps > 8 || min_n < 0)
-            throw Error(BUG_EINVAL, "gram jacobi params: cols in 2..32, inner sweeps in 1..8, min_n >= 0");
+        if (cols < 2 || cols > kGbjColsWide || inner_sweeps < 1 || inner_sweeps > 8 || min_n < 0)
+            throw Error(BUG_EINVAL, "gram jacobi params: cols in 2..64, inner sweeps in 1..8, min_n >= 0");
         ctx->gbj_cols = cols & ~1;
         ctx->gbj_inner = inner_sweeps;
         ctx->gbj_min_n = min_n;

@@ -264,3 +264,37 @@ def test_axpy_many_equals_chained_axpys(gpu):
     assert diff <= 1e-14 * chained.data.abs().max().item()
     with pytest.raises(ValueError):
         T.axpy_many_(alphas[:3], xs[:2], y0)
+
+
+@pytest.mark.parametrize("cols", [32, 64])
+def test_gram_block_jacobi_pair_widths(gpu, cols):
+    """The Gram block Jacobi with block pairs of 32 columns (the default, 512-thread rotate
+    workgroups) and of 64 (1024 threads, selectable through bug_set_gram_jacobi_params): blocks
+    wide enough to take it, with graded spectra (1 .. 1e-12), give singular values accurate to
+    1e-13 sigma_max, reconstruct, and return orthonormal factors."""
+    from bondupdategalerkin_amd import _lib
+    from bondupdategalerkin_amd import tensor_ops as T
+
+    ctx = _lib.context()
+    ctx.check(ctx._lib.bug_set_gram_jacobi_params(ctx.handle, cols, 1, 128))
+    try:
+        rng = np.random.default_rng(3)
+        dims = {-2: (300, 190), 0: (610, 455), 2: (530, 470), 4: (40, 33)}
+        C = ob.Fam({q: d[0] for q, d in dims.items()}, {q: d[1] for q, d in dims.items()}, [0])
+        sig = {}
+        for q, (m, n) in dims.items():
+            Qa, _ = np.linalg.qr(rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n)))
+            Qb, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+            sig[q] = 10.0 ** (-12.0 * np.arange(n) / (n - 1))
+            C.blocks[(0, q)] = (Qa * sig[q][None, :]) @ Qb.conj().T
+        r = T.svd_core(device_from_fam(C))
+        Uf, Vf = fam_from_device(r.U), fam_from_device(r.Vh)
+        for q, (m, n) in dims.items():
+            s, M = r.S[q], C.blocks[(0, q)]
+            U, Vh = Uf.blocks[(0, q)], Vf.blocks[(0, q)]
+            assert np.abs(s - sig[q]).max() < 1e-13 * sig[q][0], q
+            assert np.abs(U * s[None, :] @ Vh - M).max() < 1e-12, q
+            assert np.abs(U.conj().T @ U - np.eye(n)).max() < 1e-11, q
+            assert np.abs(Vh @ Vh.conj().T - np.eye(n)).max() < 1e-11, q
+    finally:
+        ctx.check(ctx._lib.bug_set_gram_jacobi_params(ctx.handle, 32, 1, 128))
