@@ -1547,9 +1547,7 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
                                                         int round,
                                                         const int* __restrict__ flags_prev,
                                                         int* __restrict__ flags_cur,
-                                                        double eps, int inner, int maxch,
-                                                        const double* __restrict__ fro2,
-                                                        double floor_rel2) {
+                                                        double eps, int inner, int maxch) {
     extern __shared__ __align__(16) unsigned char gbj_rot_lds_raw[];
     double2 (*G)[CC + 1] = reinterpret_cast<double2 (*)[CC + 1]>(gbj_rot_lds_raw);
     double2 (*Z)[CC + 1] = G + CC;
@@ -1641,13 +1639,6 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
     }
     __syncthreads();
     const double tol2 = eps * eps * (double)n;
-    // A pair whose two columns together hold less than floor_rel2 of the matrix's squared
-    // Frobenius norm is left alone, as in the staged kernel: rotations among such columns only
-    // reshape directions whose singular values lie below floor_rel2^(1/2) ||X||_F, which no
-    // selection keeps, and a column can only grow by rotating against a larger partner, which
-    // is never skipped.  (Columns of rounding noise would otherwise be orthogonalized to full
-    // relative accuracy among themselves, sweep after sweep.)
-    const double floor2 = floor_rel2 * fro2[tk.x];
     const bool cross_only = round > 0 && cp > 0 && cq > 0;
     // Every pair the inner sweep visits, tested on G as summed: if none passes the rotation
     // test, the first inner round rotates nothing, G stays as it is, and so do all later ones;
@@ -1658,8 +1649,7 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
         if (cross_only && !(a < cp && b >= cp)) continue;
         const double gr = G[a][b].x, gi = G[a][b].y;
         const double ag2 = gr * gr + gi * gi;
-        const double al = G[a][a].x, be = G[b][b].x;
-        if (al + be >= floor2 && ag2 > tol2 * (al * be) && ag2 != 0.0) s_need = 1;
+        if (ag2 > tol2 * (G[a][a].x * G[b][b].x) && ag2 != 0.0) s_need = 1;
     }
     __syncthreads();
     const bool needed = s_need != 0;
@@ -1704,7 +1694,7 @@ __global__ __launch_bounds__(NT) void gbj_rotate_kernel(const BjMat* __restrict_
                 const double al = G[a][a].x, be = G[b][b].x;
                 const double gr = G[a][b].x, gi = G[a][b].y;
                 const double ag2 = gr * gr + gi * gi;
-                if (!(al + be >= floor2 && ag2 > tol2 * (al * be) && ag2 != 0.0)) continue;
+                if (!(ag2 > tol2 * (al * be) && ag2 != 0.0)) continue;
                 jacobi_rotation(al, be, gr, gi, ag2, cs[sl], sn[sl], er[sl], ei[sl]);
                 ra[sl] = a;
                 rb[sl] = b;
@@ -2576,10 +2566,7 @@ void gram_block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& 
         }
     }
     const int nrows = max_sweeps + 1;
-    const size_t flag_bytes = (sizeof(int) * (size_t)nmat * nrows + 255) & ~size_t(255);
-    char* fbase = static_cast<char*>(scratch(ctx, st, flag_bytes + sizeof(double) * (size_t)nmat, 3));
-    int* flags = reinterpret_cast<int*>(fbase);
-    double* fro2 = reinterpret_cast<double*>(fbase + flag_bytes);
+    int* flags = static_cast<int*>(scratch(ctx, st, sizeof(int) * (size_t)nmat * nrows, 3));
     std::vector<unsigned char> blob(sizeof(BjMat) * nmat + sizeof(int2) * tasks.size());
     std::memcpy(blob.data(), mats.data(), sizeof(BjMat) * nmat);
     if (!tasks.empty()) std::memcpy(blob.data() + sizeof(BjMat) * nmat, tasks.data(), sizeof(int2) * tasks.size());
@@ -2588,9 +2575,6 @@ void gram_block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& 
     const int2* dt = reinterpret_cast<const int2*>(dblob + sizeof(BjMat) * nmat);
     hipLaunchKernelGGL(bj_setup_kernel<double2>, dim3(nmat, 16), dim3(256), 0, st, dm, nmat, V, flags, nrows);
     HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(bj_norm_kernel<double>, dim3(nmat), dim3(256), 0, st, dm, X, fro2);
-    HIP_CHECK(hipGetLastError());
-    const double floor_rel2 = ctx->svd_floor * ctx->svd_floor;
     int used = 0;
     if (!tasks.empty()) {
         const int cc = wide ? kGbjColsWide : kGbjCols;
@@ -2617,14 +2601,14 @@ void gram_block_jacobi(bug_ctx* ctx, hipStream_t st, const std::vector<SvdJob>& 
                     hipLaunchKernelGGL((gbj_gram_kernel<wc, 1024>), grid, dim3(1024), gbj_gram_lds(wc), st,
                                        dm, dt, X, gbuf, r, fp, maxch);
                     hipLaunchKernelGGL((gbj_rotate_kernel<wc, 1024>), grid, dim3(1024), gbj_rot_lds(wc), st,
-                                       dm, dt, X, V, gbuf, r, fp, fc, eps, inner, maxch, fro2, floor_rel2);
+                                       dm, dt, X, V, gbuf, r, fp, fc, eps, inner, maxch);
                 } else {
                     constexpr int nc = kGbjCols;
                     hipLaunchKernelGGL((gbj_gram_kernel<nc, 256>), grid, dim3(256), gbj_gram_lds(nc), st,
                                        dm, dt, X, gbuf, r, fp, maxch);
                     hipLaunchKernelGGL((gbj_rotate_kernel<nc, kGbjRotThreads>), grid, dim3(kGbjRotThreads),
                                        gbj_rot_lds(nc), st,
-                                       dm, dt, X, V, gbuf, r, fp, fc, eps, inner, maxch, fro2, floor_rel2);
+                                       dm, dt, X, V, gbuf, r, fp, fc, eps, inner, maxch);
                 }
             }
             HIP_CHECK(hipGetLastError());
