@@ -310,9 +310,10 @@ int bug_set_qr_variant(bug_ctx* ctx, int32_t variant);
  * columns per task (1..8). */
 int bug_set_jacobi_params(bug_ctx* ctx, int32_t cols, int32_t inner_sweeps);
 
-/* Tuning of the Gram block Jacobi: columns per block pair (2..32), rotation sweeps over the pair's
- * Gram matrix per round (1..8), and the smallest matrix dimension of a batch from which the
- * automatic SVD choice uses it. */
+/* Tuning of the Gram block Jacobi: columns per block pair (2..64; up to 32 on 512-thread rotate
+ * workgroups, the default 32, above 32 on 1024-thread ones), rotation sweeps over the pair's Gram
+ * matrix per round (1..8), and the smallest matrix dimension of a batch from which the automatic
+ * SVD choice uses it. */
 int bug_set_gram_jacobi_params(bug_ctx* ctx, int32_t cols, int32_t inner_sweeps, int32_t min_n);
 /* FP32-preconditioned block Jacobi (svd_variant 5; no counterpart in the reference, whose gesdd
  * runs in FP64 throughout): at most fp32_sweeps FP32 sweeps choose the starting basis of the
